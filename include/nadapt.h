@@ -1,0 +1,176 @@
+/*
+ * nadapt: MI355X (gfx950) batched step-doubling rigid-body step behind a C ABI.
+ *
+ * Every entry point takes plain device pointers and sizes; `stream` is a hipStream_t
+ * passed as void* (NULL = default stream).  Status codes: 0 ok, 1 incomplete frame
+ * (max_substeps reached), 2 inner convex solve failed, negative = error (message in
+ * na_last_error_string()).  Nothing here allocates on the caller's behalf except
+ * na_solver_create, which owns the controller vectors and scratch.
+ *
+ * Reference interfaces replaced (mardigiorgio/newton-adaptive, Python/Warp):
+ *   na_solver_create   SolverSAPAdaptive.__init__        newton/_src/solvers/sap/solver_sap_adaptive.py:446-601
+ *                      SolverMuJoCoAdaptive.__init__     newton/_src/solvers/mujoco/solver_mujoco_adaptive.py:443-580
+ *   na_step            SolverSAPAdaptive.step/integrate  solver_sap_adaptive.py:910-975
+ *                      SolverMuJoCoAdaptive.step         solver_mujoco_adaptive.py:774-849
+ *   na_step_plain      the same boundary call as one batched launch per iteration plus the
+ *                      4-byte flag read of _run_substep_loop (solver_sap_adaptive.py:877-907)
+ *   na_reset           reset / _reset_worlds             solver_sap_adaptive.py:982-1001, solver_mujoco_adaptive.py:234,912
+ *   na_collide         CollisionPipeline.collide         newton/_src/sim/collide.py:920-1250
+ *   na_collide_debug   compute_shape_aabbs + broad phase collide.py:150-275, geometry/broad_phase_sap.py:510-677
+ *   na_substep         SolverSAPAdaptive.substep         solver_sap_adaptive.py:714-729
+ *   na_attempt         _substep_body (one step-doubling attempt, no commit) solver_sap_adaptive.py:732-849
+ *   na_ctrl_*          controller kernels                solver_mujoco_adaptive.py:63-378, solver_sap_adaptive.py:237-345
+ */
+#ifndef NADAPT_H
+#define NADAPT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model arrays (device pointers, SoA).  Layout: bodies of world w are
+ * [world_body_start[w], world_body_start[w+1]); world-local shapes of world w are
+ * [world_shape_start[w], world_shape_start[w+1]); world -1 shapes are listed in
+ * global_shapes.  Mirrors newton Model fields (newton/_src/sim/model.py:36). */
+typedef struct na_model_desc {
+  int32_t world_count, body_count, shape_count, global_count, filter_count;
+  const int32_t* world_body_start;  /* [W+1] */
+  const int32_t* world_shape_start; /* [W+1] */
+  const int32_t* global_shapes;     /* [G] */
+  const float* body_com;            /* [B,3] */
+  const float* body_mass;           /* [B] */
+  const float* body_inv_mass;       /* [B] */
+  const float* body_inertia;        /* [B,3,3] body frame */
+  const float* body_inv_inertia;    /* [B,3,3] */
+  const int32_t* shape_type;        /* GeoType codes */
+  const int32_t* shape_body;
+  const int32_t* shape_collision_group;
+  const int32_t* shape_world;
+  const float* shape_transform;     /* [S,7] p, q(xyzw) in body frame */
+  const float* shape_scale;         /* [S,3] */
+  const float* shape_margin;
+  const float* shape_gap;
+  const float* shape_material_ke;
+  const float* shape_material_mu;
+  const float* shape_collision_radius;
+  const int32_t* filter_pairs;      /* [F,2] sorted canonical (min,max) */
+  float gravity[3];
+} na_model_desc;
+
+/* Solver settings (constructor kwargs of the reference solvers). */
+typedef struct na_config {
+  int32_t mode;            /* 0 "fixed", 1 "adaptive" */
+  int32_t controller;      /* 0 SAP (_adapt_dt), 1 MuJoCo (_calc_adjusted_step) */
+  double tol, dt_inner_init, dt_inner_min, dt_inner_max, divergence_threshold;
+  int32_t max_substeps, max_iterations, max_rigid_contact, broad_phase; /* broad_phase: 0 sap, 1 nxn */
+  float optimality_rel_tol, optimality_abs_tol, contact_tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
+  int32_t max_cg;
+} na_config;
+
+/* Device pointers to the per-world controller vectors (owned by the solver). */
+typedef struct na_telemetry {
+  double* dt;
+  double* dt_half;
+  double* ideal_dt;
+  double* sim_time;
+  double* next_time;
+  double* last_raw_error;
+  double* last_error;          /* accepted error */
+  uint8_t* accepted;
+  uint8_t* commit;
+  uint8_t* diverged;
+  int32_t* substeps;           /* accepted steps in the last frame */
+  int32_t* iterations;         /* attempts in the last frame */
+  int32_t* solve_fail;
+  int32_t* overflow;           /* contact/pair capacity overflows */
+  unsigned long long* cum_accepted; /* [1] */
+  long long* work;             /* [W,6]: attempts, newton iters, cg iters, contacts, pairs, solves */
+  double* body_q64;            /* [B,7] FP64 pose accumulator */
+} na_telemetry;
+
+/* Contacts SoA output (newton/_src/sim/contacts.py:137-345). */
+typedef struct na_contacts_desc {
+  int32_t capacity;
+  int32_t* count;              /* [1] total generated (may exceed capacity) */
+  int32_t* shape0;
+  int32_t* shape1;
+  float* point0;               /* [C,3] body frame */
+  float* point1;
+  float* offset0;
+  float* offset1;
+  float* normal;               /* [C,3] world, A -> B */
+  float* margin0;
+  float* margin1;
+  int32_t* world_count_out;    /* [W] contacts per world (before capacity clamp) */
+  int32_t* scratch;            /* [W*per_world*16] staging words, see na_collide_scratch_words */
+} na_contacts_desc;
+
+typedef struct na_model na_model;
+typedef struct na_solver na_solver;
+
+const char* na_last_error_string(void);
+int na_device_info(char* name, int name_len, int* cu_count, int* lds_per_block);
+
+int na_model_create(const na_model_desc* desc, na_model** out);
+int na_model_destroy(na_model* m);
+
+/* `buffers`: caller-owned controller vectors (all pointers set, sized for the model), or
+ * NULL to let the solver allocate them. */
+int na_solver_create(const na_model* m, const na_config* cfg, const na_telemetry* buffers, na_solver** out);
+int na_solver_destroy(na_solver* s);
+int na_solver_telemetry(const na_solver* s, na_telemetry* out);
+/* Schedule: persistent grid of `grid_waves` wavefronts (<=0: automatic) pulling worlds
+ * in `order` (device [W] permutation, or NULL for natural order). */
+int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order);
+
+/* Boundary call: advance every world by dt_outer in place (served form: one persistent
+ * launch, on-device ragged loop). */
+int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer, void* stream);
+/* Same boundary call in the lock-step form: one launch over all worlds per iteration and a
+ * 4-byte flag read back per iteration.  Returns the status of the host loop. */
+int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
+                  int32_t* iterations_out, void* stream);
+/* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
+int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);
+
+/* Contact generation into the Contacts arrays, world-major deterministic order. */
+int64_t na_collide_scratch_words(const na_model* m, int32_t per_world);
+int na_collide(const na_model* m, const float* body_q, const na_contacts_desc* out, int32_t per_world,
+               int32_t broad_phase, void* stream);
+/* AABBs [S,3] lo/hi and per-world candidate pairs [W, max_pairs, 2] (global shape ids,
+ * canonical) with counts [W]. */
+int na_collide_debug(const na_model* m, const float* body_q, float* aabb_lo, float* aabb_hi, int32_t* pairs,
+                     int32_t* pair_count, int32_t max_pairs, int32_t broad_phase, void* stream);
+
+/* One inner step per world (contacts at q, convex solve at h[w], FP64 integration).
+ * stats [W,4]: newton iterations, factorizations, converged, contacts. */
+int na_substep(const na_solver* s, const double* q64, const float* qd, const float* f, const double* h,
+               double* q64_out, float* qd_out, int32_t* stats, void* stream);
+/* One step-doubling attempt per world from (q64, qd) with dt[w]: full and doubled results,
+ * FP64 error norm, solve status and contact count; no commit, no controller update. */
+int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
+               double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
+               int32_t* n_contacts, void* stream);
+
+/* Controller kernels with the reference float32 arithmetic (one thread per world). */
+int na_ctrl_calc_adjusted_step_f32(int32_t n, const float* err, const float* dt, float* ideal_dt, uint8_t* accepted,
+                                   uint8_t* commit, uint8_t* diverged, float tol, float dt_min, float div_thr,
+                                   void* stream);
+int na_ctrl_adapt_dt_f32(int32_t n, const float* err, float* sim_time, const float* next_time, float* dt,
+                         float* dt_half, float* ideal_dt, uint8_t* diverged, uint8_t* accept, float* accepted_error,
+                         int32_t* substeps, int32_t* cum_accepted, int32_t mode, float tol, float dt_min,
+                         float dt_max, float div_thr, void* stream);
+int na_ctrl_rebase_time_f32(int32_t n, float* sim_time, float* next_time, void* stream);
+int na_ctrl_reset_worlds_f32(int32_t n, const uint8_t* mask, float dt_init, float* ideal_dt, float* dt,
+                             float* dt_half, float* sim_time, float* next_time, uint8_t* diverged,
+                             uint8_t* accepted, void* stream);
+int na_ctrl_clamp_dt_to_boundary_f32(int32_t n, float* dt, float* dt_half, const float* sim_time,
+                                     const float* next_time, void* stream);
+int na_ctrl_inf_norm_error_f32(int32_t n, int32_t coords, const float* q_full, const float* q_double,
+                               const float* scale, float* err, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+"""``Contacts`` buffer and ``CollisionPipeline`` with the reference's surface.
+
+``Contacts`` keeps the ``rigid_contact_*`` SoA names, shapes and dtypes of
+``newton/_src/sim/contacts.py:137-345``; ``CollisionPipeline`` keeps
+``CollisionPipeline(model, broad_phase=..., rigid_contact_max=...)``, ``.contacts()`` and
+``.collide(state, contacts)`` of ``newton/_src/sim/collide.py:485-511,875,920``.  Contact
+generation runs in ``libnadapt.so`` (one wavefront per world); the output order is
+deterministic: worlds in order, then candidate pairs in sweep order, then manifold slot.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .native import model_handle
+
+BROAD_PHASES = {"sap": 0, "nxn": 1}
+
+
+class Contacts:
+    """Rigid contact SoA buffer (``contacts.py:91``)."""
+
+    def __init__(self, rigid_contact_max: int, device):
+        n = int(rigid_contact_max)
+        d = torch.device(device)
+        self.rigid_contact_max = n
+        self.device = d
+        self.rigid_contact_count = torch.zeros(1, dtype=torch.int32, device=d)
+        self.rigid_contact_shape0 = torch.full((n,), -1, dtype=torch.int32, device=d)
+        self.rigid_contact_shape1 = torch.full((n,), -1, dtype=torch.int32, device=d)
+        self.rigid_contact_point0 = torch.zeros((n, 3), dtype=torch.float32, device=d)
+        self.rigid_contact_point1 = torch.zeros((n, 3), dtype=torch.float32, device=d)
+        self.rigid_contact_offset0 = torch.zeros((n, 3), dtype=torch.float32, device=d)
+        self.rigid_contact_offset1 = torch.zeros((n, 3), dtype=torch.float32, device=d)
+        self.rigid_contact_normal = torch.zeros((n, 3), dtype=torch.float32, device=d)
+        self.rigid_contact_margin0 = torch.zeros(n, dtype=torch.float32, device=d)
+        self.rigid_contact_margin1 = torch.zeros(n, dtype=torch.float32, device=d)
+        self.rigid_contact_tids = torch.full((n,), -1, dtype=torch.int32, device=d)
+        self.rigid_contact_world_count = None
+
+    def clear(self) -> None:
+        self.rigid_contact_count.zero_()
+        self.rigid_contact_shape0.fill_(-1)
+        self.rigid_contact_shape1.fill_(-1)
+
+
+class CollisionPipeline:
+    """AABB -> broad phase -> primitive narrow phase -> ``Contacts`` (``collide.py:464``)."""
+
+    def __init__(self, model, *, broad_phase: str = "sap", rigid_contact_max: int | None = None,
+                 rigid_contact_max_per_world: int = 128, **_kw):
+        if broad_phase not in BROAD_PHASES:
+            raise ValueError(f"broad_phase must be one of {tuple(BROAD_PHASES)}, got {broad_phase!r}")
+        if not 1 <= int(rigid_contact_max_per_world) <= 128:
+            raise ValueError("rigid_contact_max_per_world must be in [1, 128]")
+        self.model = model
+        self.broad_phase = broad_phase
+        self.per_world = int(rigid_contact_max_per_world)
+        self.rigid_contact_max = int(rigid_contact_max) if rigid_contact_max is not None \
+            else self.per_world * model.world_count
+        self._handle = model_handle(model)
+        L = _lib.lib()
+        words = int(L.na_collide_scratch_words(self._handle.handle, self.per_world))
+        self._scratch = torch.zeros(words, dtype=torch.int32, device=model.device)
+        self._world_counts = torch.zeros(model.world_count, dtype=torch.int32, device=model.device)
+
+    def contacts(self) -> Contacts:
+        return Contacts(self.rigid_contact_max, self.model.device)
+
+    def collide(self, state, contacts: Contacts, stream=None) -> Contacts:
+        if contacts.rigid_contact_max < 1:
+            raise ValueError("contacts buffer has no capacity")
+        d = _lib.NaContactsDesc()
+        d.capacity = contacts.rigid_contact_max
+        d.count = contacts.rigid_contact_count.data_ptr()
+        d.shape0 = contacts.rigid_contact_shape0.data_ptr()
+        d.shape1 = contacts.rigid_contact_shape1.data_ptr()
+        d.point0 = contacts.rigid_contact_point0.data_ptr()
+        d.point1 = contacts.rigid_contact_point1.data_ptr()
+        d.offset0 = contacts.rigid_contact_offset0.data_ptr()
+        d.offset1 = contacts.rigid_contact_offset1.data_ptr()
+        d.normal = contacts.rigid_contact_normal.data_ptr()
+        d.margin0 = contacts.rigid_contact_margin0.data_ptr()
+        d.margin1 = contacts.rigid_contact_margin1.data_ptr()
+        d.world_count_out = self._world_counts.data_ptr()
+        d.scratch = self._scratch.data_ptr()
+        body_q = state.body_q.contiguous()
+        _lib.check(_lib.lib().na_collide(self._handle.handle, body_q.data_ptr(), C.byref(d), self.per_world,
+                                         BROAD_PHASES[self.broad_phase], _lib.stream_handle(stream)),
+                   "na_collide")
+        contacts.rigid_contact_world_count = self._world_counts
+        return contacts
+
+    def debug_pairs(self, state, max_pairs: int = 1024, stream=None):
+        """AABBs and candidate pairs (for pair-set parity checks): (lo [S,3], hi [S,3],
+        pairs [W, max_pairs, 2], counts [W])."""
+        m = self.model
+        lo = torch.zeros((m.shape_count, 3), dtype=torch.float32, device=m.device)
+        hi = torch.zeros_like(lo)
+        pairs = torch.full((m.world_count, max_pairs, 2), -1, dtype=torch.int32, device=m.device)
+        cnt = torch.zeros(m.world_count, dtype=torch.int32, device=m.device)
+        _lib.check(_lib.lib().na_collide_debug(self._handle.handle, state.body_q.contiguous().data_ptr(),
+                                               lo.data_ptr(), hi.data_ptr(), pairs.data_ptr(), cnt.data_ptr(),
+                                               max_pairs, BROAD_PHASES[self.broad_phase],
+                                               _lib.stream_handle(stream)), "na_collide_debug")
+        return lo, hi, pairs, cnt

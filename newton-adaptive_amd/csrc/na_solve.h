@@ -1,0 +1,669 @@
+// Convex compliant-contact (SAP) solve of one world inside one wavefront.
+//
+// The problem is the one the reference solves through the external sap_warp solver
+// (call sites solver_sap_adaptive.py:489-501,714-729): with A = diag(m I, I_world) per free
+// body, free-motion velocity v* and contact Jacobian J,
+//     l(v) = 1/2 |v - v*|_A^2 + sum_i 1/2 |P_F(y_i)|_R^2,  y_i = -R^-1 (J_i v - v_hat_i),
+// minimised by Newton's method.  One lane owns one body (registers) and lanes stride over
+// contacts (LDS); J^T gamma and J^T G J are scattered through LDS float atomics.
+// The Newton direction comes from block-Jacobi preconditioned conjugate gradients on
+// H = A + J^T G J (matrix free), the step length from an exact line search seeded by the
+// cubic Hermite minimiser (N&W eq. 3.59, test_adaptive_cubic_linesearch.py:36-60).
+// Optimality follows the SAP criterion |D g| <= eps_a + eps_r max(|D p|, |D J^T gamma|).
+#pragma once
+#include "na_collide.h"
+
+struct NaConfig {
+  int mode;        // 0 fixed, 1 adaptive
+  int controller;  // 0 SAP (_adapt_dt), 1 MuJoCo (_calc_adjusted_step)
+  double tol, dt_init, dt_min, dt_max, divergence_threshold;
+  int max_substeps, max_iterations, max_contacts, broad_phase;  // broad_phase: 0 sap, 1 nxn
+  float opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
+  int max_cg;
+};
+
+struct BodyLDS {
+  float q[7][NA_MAX_BODIES];     // f32 pose used for contact generation (SoA)
+  float x[3][NA_MAX_BODIES];     // COM position at the step start
+  float iwi[6][NA_MAX_BODIES];   // world inverse inertia (xx, yy, zz, xy, xz, yz)
+  float im[NA_MAX_BODIES];       // inverse mass
+  float v[6][NA_MAX_BODIES];     // f32 exchange vector (PCG directions)
+  float acc[6][NA_MAX_BODIES];   // f32 scatter target (H p)
+  double vd[6][NA_MAX_BODIES];   // f64 exchange vector (Newton iterate)
+  double accd[6][NA_MAX_BODIES]; // f64 scatter target (J^T gamma)
+  float pre[21][NA_MAX_BODIES];  // block-Jacobi accumulation (6x6 lower, row-major packed)
+};
+
+struct SolveLDS {
+  float t1[3][NA_MAX_CONTACTS];
+  float ra[3][NA_MAX_CONTACTS];  // contact point - COM of body A
+  float rb[3][NA_MAX_CONTACTS];
+  float Rn[NA_MAX_CONTACTS], Rt[NA_MAX_CONTACTS], vhat[NA_MAX_CONTACTS];
+  double vc0[3][NA_MAX_CONTACTS];
+  double dvc[3][NA_MAX_CONTACTS];
+  float G[6][NA_MAX_CONTACTS];  // symmetric 3x3: 00 11 22 01 02 12
+};
+
+// The segment scratch is only live during contact generation and the per-contact solver
+// data only during a solve, so they share storage.
+struct WorldLDS {
+  union {
+    SegLDS seg;
+    SolveLDS sol;
+  };
+  ContactLDS con;
+  BodyLDS body;
+};
+
+// Per-lane body registers for one solve.
+struct BodyReg {
+  float m, im;
+  float I[6];    // world inertia sym (xx, yy, zz, xy, xz, yz)
+  double vs[6];  // v*
+  float D[6];    // diag(A)^-1/2
+};
+
+template <typename T>
+__device__ __forceinline__ void sym_mul(const float* S, const T* x, T* y) {
+  y[0] = S[0] * x[0] + S[3] * x[1] + S[4] * x[2];
+  y[1] = S[3] * x[0] + S[1] * x[1] + S[5] * x[2];
+  y[2] = S[4] * x[0] + S[5] * x[1] + S[2] * x[2];
+}
+template <typename T>
+__device__ __forceinline__ void applyA(const BodyReg& b, const T* x, T* y) {
+  y[0] = b.m * x[0];
+  y[1] = b.m * x[1];
+  y[2] = b.m * x[2];
+  sym_mul<T>(b.I, x + 3, y + 3);
+}
+
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 dd(f3 a) { return d3{a.x, a.y, a.z}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b) {
+  return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// frame rows: t1, t2 = n x t1, n;  J_b row k = sign * (F_k, r x F_k)
+struct Frame {
+  f3 t1, t2, n;
+};
+__device__ __forceinline__ Frame load_frame(const ContactLDS& C, const SolveLDS& X, int c) {
+  Frame F;
+  F.n = mk(C.n[0][c], C.n[1][c], C.n[2][c]);
+  F.t1 = mk(X.t1[0][c], X.t1[1][c], X.t1[2][c]);
+  F.t2 = cross3(F.n, F.t1);
+  return F;
+}
+// vc = J v (f32) for contact c from B.v
+__device__ __forceinline__ f3 contact_vel(const BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F,
+                                          int c) {
+  f3 rel = mk(0.f, 0.f, 0.f);
+  int ba = C.ba[c], bb = C.bb[c];
+  if (bb >= 0) {
+    f3 v = mk(B.v[0][bb], B.v[1][bb], B.v[2][bb]);
+    f3 w = mk(B.v[3][bb], B.v[4][bb], B.v[5][bb]);
+    rel = rel + v + cross3(w, mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]));
+  }
+  if (ba >= 0) {
+    f3 v = mk(B.v[0][ba], B.v[1][ba], B.v[2][ba]);
+    f3 w = mk(B.v[3][ba], B.v[4][ba], B.v[5][ba]);
+    rel = rel - (v + cross3(w, mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])));
+  }
+  return mk(dot3(F.t1, rel), dot3(F.t2, rel), dot3(F.n, rel));
+}
+// vc = J v (f64) for contact c from B.vd
+__device__ __forceinline__ d3 contact_vel_d(const BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F,
+                                            int c) {
+  d3 rel = d3{0.0, 0.0, 0.0};
+  int ba = C.ba[c], bb = C.bb[c];
+  if (bb >= 0) {
+    d3 w = d3{B.vd[3][bb], B.vd[4][bb], B.vd[5][bb]};
+    d3 x = dcross(w, dd(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c])));
+    rel.x += B.vd[0][bb] + x.x; rel.y += B.vd[1][bb] + x.y; rel.z += B.vd[2][bb] + x.z;
+  }
+  if (ba >= 0) {
+    d3 w = d3{B.vd[3][ba], B.vd[4][ba], B.vd[5][ba]};
+    d3 x = dcross(w, dd(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])));
+    rel.x -= B.vd[0][ba] + x.x; rel.y -= B.vd[1][ba] + x.y; rel.z -= B.vd[2][ba] + x.z;
+  }
+  return d3{ddot(dd(F.t1), rel), ddot(dd(F.t2), rel), ddot(dd(F.n), rel)};
+}
+// acc += J^T lambda (f32 LDS atomics)
+__device__ __forceinline__ void scatter_JT(BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F, int c,
+                                           f3 lam) {
+  f3 fw = F.t1 * lam.x + F.t2 * lam.y + F.n * lam.z;  // world impulse on B
+  int ba = C.ba[c], bb = C.bb[c];
+  if (bb >= 0) {
+    f3 tq = cross3(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]), fw);
+    atomicAdd(&B.acc[0][bb], fw.x); atomicAdd(&B.acc[1][bb], fw.y); atomicAdd(&B.acc[2][bb], fw.z);
+    atomicAdd(&B.acc[3][bb], tq.x); atomicAdd(&B.acc[4][bb], tq.y); atomicAdd(&B.acc[5][bb], tq.z);
+  }
+  if (ba >= 0) {
+    f3 tq = cross3(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c]), fw);
+    atomicAdd(&B.acc[0][ba], -fw.x); atomicAdd(&B.acc[1][ba], -fw.y); atomicAdd(&B.acc[2][ba], -fw.z);
+    atomicAdd(&B.acc[3][ba], -tq.x); atomicAdd(&B.acc[4][ba], -tq.y); atomicAdd(&B.acc[5][ba], -tq.z);
+  }
+}
+// accd += J^T gamma (f64 LDS atomics)
+__device__ __forceinline__ void scatter_JT_d(BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F, int c,
+                                             d3 lam) {
+  d3 t1 = dd(F.t1), t2 = dd(F.t2), n = dd(F.n);
+  d3 fw = d3{t1.x * lam.x + t2.x * lam.y + n.x * lam.z, t1.y * lam.x + t2.y * lam.y + n.y * lam.z,
+             t1.z * lam.x + t2.z * lam.y + n.z * lam.z};
+  int ba = C.ba[c], bb = C.bb[c];
+  if (bb >= 0) {
+    d3 tq = dcross(dd(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c])), fw);
+    atomicAdd(&B.accd[0][bb], fw.x); atomicAdd(&B.accd[1][bb], fw.y); atomicAdd(&B.accd[2][bb], fw.z);
+    atomicAdd(&B.accd[3][bb], tq.x); atomicAdd(&B.accd[4][bb], tq.y); atomicAdd(&B.accd[5][bb], tq.z);
+  }
+  if (ba >= 0) {
+    d3 tq = dcross(dd(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])), fw);
+    atomicAdd(&B.accd[0][ba], -fw.x); atomicAdd(&B.accd[1][ba], -fw.y); atomicAdd(&B.accd[2][ba], -fw.z);
+    atomicAdd(&B.accd[3][ba], -tq.x); atomicAdd(&B.accd[4][ba], -tq.y); atomicAdd(&B.accd[5][ba], -tq.z);
+  }
+}
+
+// SAP projection onto the friction cone in the R norm (FP64); returns cost 1/2 gamma^T R gamma,
+// writes gamma and (when G != nullptr) the symmetric G = dP/dy R^-1 (00 11 22 01 02 12) in f32.
+__device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, d3 y, d3& g, float* G) {
+  double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
+  double muh = mu * Rt / Rn;
+  if (yr <= mu * yn) {
+    g = y;
+    if (G) { G[0] = (float)(1.0 / Rt); G[1] = G[0]; G[2] = (float)(1.0 / Rn); G[3] = 0.f; G[4] = 0.f; G[5] = 0.f; }
+  } else if (yn <= -muh * yr) {
+    g = d3{0.0, 0.0, 0.0};
+    if (G) G[0] = G[1] = G[2] = G[3] = G[4] = G[5] = 0.f;
+  } else {
+    double t0 = y.x / yr, t1 = y.y / yr;
+    double den = 1.0 + mu * muh;
+    double gn = (yn + muh * yr) / den;
+    g = d3{mu * gn * t0, mu * gn * t1, gn};
+    if (G) {
+      double a = mu * muh / den, b = mu * gn / yr;
+      G[0] = (float)((a * t0 * t0 + b * (1.0 - t0 * t0)) / Rt);
+      G[1] = (float)((a * t1 * t1 + b * (1.0 - t1 * t1)) / Rt);
+      G[3] = (float)((a - b) * t0 * t1 / Rt);
+      G[2] = (float)(1.0 / (den * Rn));
+      G[4] = (float)(mu / den * t0 / Rn);
+      G[5] = (float)(mu / den * t1 / Rn);
+    }
+  }
+  return 0.5 * (Rt * (g.x * g.x + g.y * g.y) + Rn * g.z * g.z);
+}
+// second derivative of the contact cost along u: u^T G u with G of the FP64 projection
+__device__ __forceinline__ double sap_curv(double mu, double Rt, double Rn, d3 y, d3 g, d3 u) {
+  double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
+  double muh = mu * Rt / Rn;
+  if (yr <= mu * yn) return (u.x * u.x + u.y * u.y) / Rt + u.z * u.z / Rn;
+  if (yn <= -muh * yr) return 0.0;
+  double t0 = y.x / yr, t1 = y.y / yr;
+  double den = 1.0 + mu * muh;
+  double gn = g.z;
+  double a = mu * muh / den, b = mu * gn / yr;
+  double ut = u.x * t0 + u.y * t1;
+  double uu = u.x * u.x + u.y * u.y;
+  // u_t^T (a t t^T + b (I - t t^T)) u_t / Rt + 2 u_n (mu/den) t.u_t / Rn + u_n^2 / (den Rn)
+  return (a * ut * ut + b * (uu - ut * ut)) / Rt + 2.0 * u.z * (mu / den) * ut / Rn + u.z * u.z / (den * Rn);
+}
+__device__ __forceinline__ f3 sym3_mul(const float* G, f3 u) {
+  return mk(G[0] * u.x + G[3] * u.y + G[4] * u.z, G[3] * u.x + G[1] * u.y + G[5] * u.z,
+            G[4] * u.x + G[5] * u.y + G[2] * u.z);
+}
+
+// scatter J_b^T G J_b (6x6 lower packed) into the block-Jacobi accumulator of body b
+__device__ __forceinline__ void scatter_block(BodyLDS& B, int b, const Frame& F, f3 r, const float* G) {
+  float J[3][6];
+  f3 Fk[3] = {F.t1, F.t2, F.n};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f3 a = cross3(r, Fk[k]);
+    J[k][0] = Fk[k].x; J[k][1] = Fk[k].y; J[k][2] = Fk[k].z;
+    J[k][3] = a.x; J[k][4] = a.y; J[k][5] = a.z;
+  }
+  float GJ[3][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    f3 col = sym3_mul(G, mk(J[0][i], J[1][i], J[2][i]));
+    GJ[0][i] = col.x; GJ[1][i] = col.y; GJ[2][i] = col.z;
+  }
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+#pragma unroll
+    for (int j = 0; j <= i; ++j) {
+      float v = J[0][i] * GJ[0][j] + J[1][i] * GJ[1][j] + J[2][i] * GJ[2][j];
+      atomicAdd(&B.pre[idx][b], v);
+      ++idx;
+    }
+  }
+}
+
+// 6x6 Cholesky of the packed lower triangle, in place; returns false if not SPD
+#define LI(i, j) L[(i) * ((i) + 1) / 2 + (j)]
+__device__ __forceinline__ bool chol6(float* L) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    float s = LI(j, j);
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= LI(j, k) * LI(j, k);
+    if (!(s > 0.f)) return false;
+    float d = sqrtf(s);
+    LI(j, j) = d;
+    float id = 1.f / d;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      float t = LI(i, j);
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= LI(i, k) * LI(j, k);
+      LI(i, j) = t * id;
+    }
+  }
+  return true;
+}
+__device__ __forceinline__ void chol6_solve(const float* L, const float* b, float* x) {
+  float y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    float s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= LI(i, k) * y[k];
+    y[i] = s / LI(i, i);
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    float s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) s -= LI(k, i) * x[k];
+    x[i] = s / LI(i, i);
+  }
+}
+#undef LI
+
+// Cubic Hermite minimiser on [0, h] (N&W eq. 3.59); returns a <= 0 on the fallback conditions.
+__device__ __forceinline__ double cubic_min_d(double c0, double d0, double c1, double d1, double h) {
+  double theta = d0 + d1 - 3.0 * (c1 - c0) / h;
+  double disc = theta * theta - d0 * d1;
+  if (!isfinite(disc) || disc < 0.0) return -1.0;
+  double gamma = sqrt(disc);
+  double denom = d1 - d0 + 2.0 * gamma;
+  if (denom == 0.0) return -1.0;
+  double a = h - h * (d1 + gamma - theta) / denom;
+  if (!isfinite(a) || a <= 0.0 || a > h) return -1.0;
+  return a;
+}
+
+struct SolveStats {
+  int iterations, factorizations, cg_iterations;
+  bool converged;
+};
+
+// Set up the per-contact data of a step h: frame, lever arms, regularisation, v_hat.
+// Requires B.x, B.iwi, B.im filled for the step's start pose.
+__device__ void contact_setup(int nc, float h, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X) {
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    f3 n = mk(C.n[0][c], C.n[1][c], C.n[2][c]);
+    f3 e = fabsf(n.x) < fabsf(n.y) ? (fabsf(n.x) < fabsf(n.z) ? mk(1.f, 0.f, 0.f) : mk(0.f, 0.f, 1.f))
+                                   : (fabsf(n.y) < fabsf(n.z) ? mk(0.f, 1.f, 0.f) : mk(0.f, 0.f, 1.f));
+    f3 t1 = nrm3(cross3(n, e));
+    X.t1[0][c] = t1.x; X.t1[1][c] = t1.y; X.t1[2][c] = t1.z;
+    f3 p = mk(C.p[0][c], C.p[1][c], C.p[2][c]);
+    float wsum = 0.f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      int b = s == 0 ? C.ba[c] : C.bb[c];
+      f3 r = mk(0.f, 0.f, 0.f);
+      if (b >= 0) {
+        r = p - mk(B.x[0][b], B.x[1][b], B.x[2][b]);
+        float Ii[6] = {B.iwi[0][b], B.iwi[1][b], B.iwi[2][b], B.iwi[3][b], B.iwi[4][b], B.iwi[5][b]};
+        float Ir[3];
+        float rr[3] = {r.x, r.y, r.z};
+        sym_mul<float>(Ii, rr, Ir);
+        wsum += 3.f * B.im[b] + dot3(r, r) * (Ii[0] + Ii[1] + Ii[2]) - (r.x * Ir[0] + r.y * Ir[1] + r.z * Ir[2]);
+      }
+      if (s == 0) { X.ra[0][c] = r.x; X.ra[1][c] = r.y; X.ra[2][c] = r.z; }
+      else { X.rb[0][c] = r.x; X.rb[1][c] = r.y; X.rb[2][c] = r.z; }
+    }
+    float wd = wsum * (1.f / 3.f);
+    float rn_soft = 1.f / (h * C.k[c] * (h + cfg.tau_d));
+    float rn_rigid = cfg.beta * cfg.beta / (4.f * 9.8696044010893586f) * wd;
+    X.Rn[c] = fmaxf(rn_rigid, rn_soft);
+    X.Rt[c] = cfg.sigma * wd;
+    X.vhat[c] = -C.phi0[c] / (h + cfg.tau_d);
+  }
+  wave_sync();
+}
+
+// FP64 evaluation at the iterate in B.vd: gamma, G (f32), J^T gamma into B.accd (zeroed here),
+// contact velocities into X.vc0.  Returns the summed contact cost (wave total).
+__device__ double eval_contacts(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X, int nb) {
+  for (int i = lane_id(); i < nb; i += NA_WAVE) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) B.accd[k][i] = 0.0;
+  }
+  wave_sync();
+  double cost = 0.0;
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    Frame F = load_frame(C, X, c);
+    d3 vc = contact_vel_d(B, C, X, F, c);
+    X.vc0[0][c] = vc.x; X.vc0[1][c] = vc.y; X.vc0[2][c] = vc.z;
+    double Rt = X.Rt[c], Rn = X.Rn[c];
+    d3 y = d3{-vc.x / Rt, -vc.y / Rt, -(vc.z - (double)X.vhat[c]) / Rn};
+    d3 g;
+    float G[6];
+    cost += sap_project(C.mu[c], Rt, Rn, y, g, G);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) X.G[k][c] = G[k];
+    scatter_JT_d(B, C, X, F, c, g);
+  }
+  wave_sync();
+  return wave_sum_d(cost);
+}
+
+// B.acc = J^T G J p for p in B.v (zeroes B.acc first), f32
+__device__ void apply_JGJ(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X, int nb) {
+  for (int i = lane_id(); i < nb; i += NA_WAVE) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) B.acc[k][i] = 0.f;
+  }
+  wave_sync();
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    Frame F = load_frame(C, X, c);
+    f3 u = contact_vel(B, C, X, F, c);
+    float G[6] = {X.G[0][c], X.G[1][c], X.G[2][c], X.G[3][c], X.G[4][c], X.G[5][c]};
+    f3 w = sym3_mul(G, u);
+    scatter_JT(B, C, X, F, c, w);
+  }
+  wave_sync();
+}
+
+// FP64 line-search evaluation of the contact part at alpha (wave totals of l, l', l'').
+__device__ __forceinline__ void ls_contacts(int nc, const ContactLDS& C, const SolveLDS& X, double alpha, double& l,
+                                            double& dl, double& d2l) {
+  double a = 0.0, b = 0.0, c2 = 0.0;
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    d3 dv = d3{X.dvc[0][c], X.dvc[1][c], X.dvc[2][c]};
+    d3 vc = d3{X.vc0[0][c] + alpha * dv.x, X.vc0[1][c] + alpha * dv.y, X.vc0[2][c] + alpha * dv.z};
+    double Rt = X.Rt[c], Rn = X.Rn[c];
+    d3 y = d3{-vc.x / Rt, -vc.y / Rt, -(vc.z - (double)X.vhat[c]) / Rn};
+    d3 g;
+    a += sap_project(C.mu[c], Rt, Rn, y, g, nullptr);
+    b -= ddot(g, dv);
+    c2 += sap_curv(C.mu[c], Rt, Rn, y, g, dv);
+  }
+  l = wave_sum_d(a);
+  dl = wave_sum_d(b);
+  d2l = wave_sum_d(c2);
+}
+
+// Newton solve with FP64 residuals and FP32 PCG directions.  Lane b < nb owns body b:
+// `br` its constants, `v` its velocity (in: guess, out: solution).  Returns true when
+// the SAP optimality criterion holds.
+__device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X,
+                          const BodyReg& br, double* v, SolveStats& st) {
+  const int l = lane_id();
+  const bool own = l < nb;
+  st.iterations = 0;
+  st.factorizations = 0;
+  st.cg_iterations = 0;
+  st.converged = false;
+  for (int it = 0; it <= cfg.max_iterations; ++it) {
+    // gradient g = A (v - v*) - J^T gamma, FP64
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) B.vd[k][l] = v[k];
+    }
+    wave_sync();
+    double ccost = eval_contacts(nc, C, B, X, nb);
+    double g[6], d[6], Ad[6];
+    double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
+    if (own) {
+      double p[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d[k] = v[k] - br.vs[k];
+      applyA<double>(br, d, Ad);
+      applyA<double>(br, v, p);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        double jc = B.accd[k][l];
+        g[k] = Ad[k] - jc;
+        mom += 0.5 * d[k] * Ad[k];
+        double s = br.D[k];
+        gn += (s * g[k]) * (s * g[k]);
+        pn += (s * p[k]) * (s * p[k]);
+        jn += (s * jc) * (s * jc);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) { g[k] = 0.0; d[k] = 0.0; Ad[k] = 0.0; }
+    }
+    gn = sqrt(wave_sum_d(gn));
+    pn = sqrt(wave_sum_d(pn));
+    jn = sqrt(wave_sum_d(jn));
+    const double mom0 = wave_sum_d(mom);
+    const double l0 = mom0 + ccost;
+    if (!(gn == gn)) return false;
+    if (gn <= (double)cfg.opt_abs_tol + (double)cfg.opt_rel_tol * fmax(pn, jn)) {
+      st.converged = true;
+      return true;
+    }
+    if (it == cfg.max_iterations) break;
+
+    // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32)
+    for (int i = l; i < nb; i += NA_WAVE) {
+#pragma unroll
+      for (int k = 0; k < 21; ++k) B.pre[k][i] = 0.f;
+    }
+    wave_sync();
+    for (int c = l; c < nc; c += NA_WAVE) {
+      Frame F = load_frame(C, X, c);
+      float G[6] = {X.G[0][c], X.G[1][c], X.G[2][c], X.G[3][c], X.G[4][c], X.G[5][c]};
+      int ba = C.ba[c], bb = C.bb[c];
+      if (ba >= 0) scatter_block(B, ba, F, mk(X.ra[0][c], X.ra[1][c], X.ra[2][c]), G);
+      if (bb >= 0) scatter_block(B, bb, F, mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]), G);
+    }
+    wave_sync();
+    float L[21];
+    bool spd = true;
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 21; ++k) L[k] = B.pre[k][l];
+      L[0] += br.m; L[2] += br.m; L[5] += br.m;
+      // packed indices of rows/cols 3..5: (3,3)=9 (4,3)=13 (4,4)=14 (5,3)=18 (5,4)=19 (5,5)=20
+      L[9] += br.I[0]; L[14] += br.I[1]; L[20] += br.I[2];
+      L[13] += br.I[3]; L[18] += br.I[4]; L[19] += br.I[5];
+      spd = chol6(L);
+    }
+    if (__any(!spd)) return false;
+    st.factorizations++;
+
+    // PCG on H x = -g in f32, right-hand side scaled to O(1)
+    const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
+    float x[6], r[6], z[6], pp[6];
+    float rz = 0.f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { x[k] = 0.f; r[k] = (float)(-g[k] * gscale); z[k] = 0.f; }
+    if (own) chol6_solve(L, r, z);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { pp[k] = z[k]; rz += r[k] * z[k]; }
+    rz = wave_sum(rz);
+    const float tol2 = cfg.cg_rel_tol * cfg.cg_rel_tol * rz;
+    for (int k = 0; k < cfg.max_cg && rz > tol2 && rz > 0.f; ++k) {
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) B.v[q][l] = pp[q];
+      }
+      wave_sync();
+      apply_JGJ(nc, C, B, X, nb);
+      float Hp[6];
+      float pHp = 0.f;
+      if (own) {
+        applyA<float>(br, pp, Hp);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { Hp[q] += B.acc[q][l]; pHp += pp[q] * Hp[q]; }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Hp[q] = 0.f;
+      }
+      pHp = wave_sum(pHp);
+      if (!(pHp > 0.f)) break;
+      float alpha = rz / pHp;
+      float rzn = 0.f;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { x[q] += alpha * pp[q]; r[q] -= alpha * Hp[q]; }
+      if (own) chol6_solve(L, r, z);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) rzn += r[q] * z[q];
+      rzn = wave_sum(rzn);
+      float beta = rzn / rz;
+      rz = rzn;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) pp[q] = z[q] + beta * pp[q];
+      st.cg_iterations++;
+    }
+    double dx[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dx[q] = own ? (double)x[q] / gscale : 0.0;
+
+    // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a)
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) B.vd[q][l] = dx[q];
+    }
+    wave_sync();
+    for (int c = l; c < nc; c += NA_WAVE) {
+      Frame F = load_frame(C, X, c);
+      d3 u = contact_vel_d(B, C, X, F, c);
+      X.dvc[0][c] = u.x; X.dvc[1][c] = u.y; X.dvc[2][c] = u.z;
+    }
+    wave_sync();
+    double aa = 0.0, b0 = 0.0;
+    if (own) {
+      double Ax[6];
+      applyA<double>(br, dx, Ax);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { aa += dx[q] * Ax[q]; b0 += dx[q] * Ad[q]; }
+    }
+    aa = wave_sum_d(aa);
+    b0 = wave_sum_d(b0);
+    double lc, dlc, d2c;
+    ls_contacts(nc, C, X, 0.0, lc, dlc, d2c);
+    const double dl0 = b0 + dlc;
+    const double d2l0 = aa + d2c;
+    if (!(dl0 < 0.0)) { st.iterations = it + 1; break; }
+    ls_contacts(nc, C, X, 1.0, lc, dlc, d2c);
+    const double l1 = mom0 + b0 + 0.5 * aa + lc;
+    const double dl1 = b0 + aa + dlc;
+    double alpha;
+    if (dl1 <= 0.0) {
+      alpha = 1.0;
+    } else {
+      double lo = 0.0, hi = 1.0;
+      alpha = cubic_min_d(l0, dl0, l1, dl1, 1.0);
+      if (!(alpha > 0.0)) alpha = fmin(fmax(-dl0 / d2l0, 0.0), 1.0);
+      if (!(alpha > lo && alpha < hi)) alpha = 0.5;
+      for (int k = 0; k < 60; ++k) {
+        ls_contacts(nc, C, X, alpha, lc, dlc, d2c);
+        double dl = b0 + alpha * aa + dlc;
+        double ddl = aa + d2c;
+        if (fabs(dl) <= (double)cfg.ls_rel_tol * fabs(dl0)) break;
+        if (dl < 0.0) lo = alpha;
+        else hi = alpha;
+        double an = ddl > 0.0 ? alpha - dl / ddl : -1.0;
+        if (!(an > lo && an < hi)) an = 0.5 * (lo + hi);
+        if (hi - lo <= 1e-14 * hi) { alpha = an; break; }
+        alpha = an;
+      }
+    }
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) v[q] += alpha * dx[q];
+    }
+    st.iterations = it + 1;
+  }
+  return st.converged;
+}
+
+// ---------------------------------------------------------------- body setup and integration
+// Per-lane constants of body (lane < nb) at pose q (f64), velocity v0, wrench f, step h.
+__device__ void body_setup(const NaModel& M, int gb, const double* q, const float* v0, const float* f, double h,
+                           BodyReg& br, BodyLDS& B, int l) {
+  m3 R = qmat((float)q[3], (float)q[4], (float)q[5], (float)q[6]);
+  const float* Ib = M.body_inertia + 9 * gb;
+  const float* Ibi = M.body_inv_inertia + 9 * gb;
+  float Iw[9], Iwi[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float a = 0.f, ai = 0.f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          a += R.m[3 * i + k] * Ib[3 * k + m] * R.m[3 * j + m];
+          ai += R.m[3 * i + k] * Ibi[3 * k + m] * R.m[3 * j + m];
+        }
+      }
+      Iw[3 * i + j] = a;
+      Iwi[3 * i + j] = ai;
+    }
+  }
+  br.m = M.body_mass[gb];
+  br.im = M.body_inv_mass[gb];
+  br.I[0] = Iw[0]; br.I[1] = Iw[4]; br.I[2] = Iw[8]; br.I[3] = Iw[1]; br.I[4] = Iw[2]; br.I[5] = Iw[5];
+  float Ii[6] = {Iwi[0], Iwi[4], Iwi[8], Iwi[1], Iwi[2], Iwi[5]};
+  f3 com = mk(M.body_com[3 * gb], M.body_com[3 * gb + 1], M.body_com[3 * gb + 2]);
+  f3 xc = mk((float)q[0], (float)q[1], (float)q[2]) + mv(R, com);
+  B.x[0][l] = xc.x; B.x[1][l] = xc.y; B.x[2][l] = xc.z;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) B.iwi[k][l] = Ii[k];
+  B.im[l] = br.im;
+  double gmask = br.im > 0.f ? 1.0 : 0.0;
+  double hd = h;
+  br.vs[0] = (double)v0[0] + hd * ((double)f[0] * br.im + (double)M.gx * gmask);
+  br.vs[1] = (double)v0[1] + hd * ((double)f[1] * br.im + (double)M.gy * gmask);
+  br.vs[2] = (double)v0[2] + hd * ((double)f[2] * br.im + (double)M.gz * gmask);
+  double w0[3] = {v0[3], v0[4], v0[5]}, Iw0[3];
+  sym_mul<double>(br.I, w0, Iw0);
+  d3 tau = d3{(double)f[3] - (w0[1] * Iw0[2] - w0[2] * Iw0[1]), (double)f[4] - (w0[2] * Iw0[0] - w0[0] * Iw0[2]),
+              (double)f[5] - (w0[0] * Iw0[1] - w0[1] * Iw0[0])};
+  double t3[3] = {tau.x, tau.y, tau.z}, dw[3];
+  sym_mul<double>(Ii, t3, dw);
+  br.vs[3] = w0[0] + hd * dw[0];
+  br.vs[4] = w0[1] + hd * dw[1];
+  br.vs[5] = w0[2] + hd * dw[2];
+  br.D[0] = br.D[1] = br.D[2] = rsqrtf(br.m);
+  br.D[3] = rsqrtf(br.I[0]);
+  br.D[4] = rsqrtf(br.I[1]);
+  br.D[5] = rsqrtf(br.I[2]);
+}
+
+// Semi-implicit update in double (solver.py:50-94, no damping): x_com += h v,
+// q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.
+__device__ void integrate_body(const NaModel& M, int gb, const double* q, const double* v, double h, double* qo) {
+  double qx = q[3], qy = q[4], qz = q[5], qw = q[6];
+  double cx = M.body_com[3 * gb], cy = M.body_com[3 * gb + 1], cz = M.body_com[3 * gb + 2];
+  double xc[3];
+  xc[0] = q[0] + (1 - 2 * (qy * qy + qz * qz)) * cx + 2 * (qx * qy - qz * qw) * cy + 2 * (qx * qz + qy * qw) * cz;
+  xc[1] = q[1] + 2 * (qx * qy + qz * qw) * cx + (1 - 2 * (qx * qx + qz * qz)) * cy + 2 * (qy * qz - qx * qw) * cz;
+  xc[2] = q[2] + 2 * (qx * qz - qy * qw) * cx + 2 * (qy * qz + qx * qw) * cy + (1 - 2 * (qx * qx + qy * qy)) * cz;
+  xc[0] += h * v[0];
+  xc[1] += h * v[1];
+  xc[2] += h * v[2];
+  double wx = v[3], wy = v[4], wz = v[5];
+  double dx = wy * qz - wz * qy + wx * qw;
+  double dy = wz * qx - wx * qz + wy * qw;
+  double dz = wx * qy - wy * qx + wz * qw;
+  double dw = -(wx * qx + wy * qy + wz * qz);
+  double nx = qx + 0.5 * h * dx, ny = qy + 0.5 * h * dy, nz = qz + 0.5 * h * dz, nw = qw + 0.5 * h * dw;
+  double il = 1.0 / sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+  nx *= il; ny *= il; nz *= il; nw *= il;
+  qo[0] = xc[0] - ((1 - 2 * (ny * ny + nz * nz)) * cx + 2 * (nx * ny - nz * nw) * cy + 2 * (nx * nz + ny * nw) * cz);
+  qo[1] = xc[1] - (2 * (nx * ny + nz * nw) * cx + (1 - 2 * (nx * nx + nz * nz)) * cy + 2 * (ny * nz - nx * nw) * cz);
+  qo[2] = xc[2] - (2 * (nx * nz - ny * nw) * cx + 2 * (ny * nz + nx * nw) * cy + (1 - 2 * (nx * nx + ny * ny)) * cz);
+  qo[3] = nx; qo[4] = ny; qo[5] = nz; qo[6] = nw;
+}

@@ -1,0 +1,1156 @@
+// nadapt: gfx950 kernels and the C ABI of the batched step-doubling rigid-body step.
+//
+// One wavefront owns one world.  The served boundary call (na_step) is a single persistent
+// launch: waves pull worlds from an atomic queue and march each world to its boundary
+// entirely on the device (collide at q_t, full solve, half solve, collide at q_mid, half
+// solve, FP64 error norm, FP64 controller, masked commit), keeping the world's pose (FP64),
+// velocity and controller scalars in registers between attempts.  The lock-step form
+// (na_step_plain) runs the same attempt as one launch per iteration over all worlds with a
+// 4-byte flag read back per iteration, like the reference's host loop
+// (solver_sap_adaptive.py:877-907).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+
+#include "../../include/nadapt.h"
+#include "na_solve.h"
+
+// ---------------------------------------------------------------- errors
+static thread_local std::string g_err;
+static int fail(const char* msg) {
+  g_err = msg;
+  return -1;
+}
+#define HIPCHK(x)                                                               \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      g_err = std::string(#x) + ": " + hipGetErrorString(e_);                   \
+      return -1;                                                                \
+    }                                                                           \
+  } while (0)
+
+extern "C" const char* na_last_error_string(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- objects
+struct na_model {
+  NaModel M;
+  int max_nb, max_seg, max_nl;
+};
+
+struct Ctrl {
+  double *dt, *dt_half, *ideal_dt, *sim_time, *next_time, *last_error, *acc_error;
+  uint8_t *accepted, *commit, *diverged;
+  int32_t *substeps, *iterations, *solve_fail, *overflow;
+  unsigned long long* cum;
+  long long* work;  // [W, NA_WORK]
+};
+#define NA_WORK 6
+
+struct na_solver {
+  const na_model* model;
+  NaConfig cfg;
+  Ctrl K;
+  double* q64;
+  int32_t* queue;  // [1]
+  int32_t* flag;   // [1]
+  int32_t* flag_host;
+  const int32_t* order;
+  int grid_waves;
+  void* pool;
+};
+
+// ---------------------------------------------------------------- controller (FP64 on device)
+#define DRAKE_SAFETY 0.9
+#define DRAKE_MIN_SHRINK 0.1
+#define DRAKE_MAX_GROW 5.0
+#define DRAKE_HYST_HIGH 1.2
+#define DRAKE_HYST_LOW 0.9
+
+template <typename T>
+__device__ __forceinline__ T drake_new_step(T step, T e, T tol) {
+  T ee = e > (T)1.0e-30 ? e : (T)1.0e-30;
+  T ns = (T)DRAKE_SAFETY * step * sqrt(tol / ee);
+  if (ns > (T)DRAKE_HYST_LOW * step && ns < (T)DRAKE_HYST_HIGH * step) ns = step;
+  T lo = (T)DRAKE_MIN_SHRINK * step, hi = (T)DRAKE_MAX_GROW * step;
+  return ns < lo ? lo : (ns > hi ? hi : ns);
+}
+template <typename T>
+__device__ __forceinline__ bool is_div_err(T e, T thr) {
+  return isnan(e) || isinf(e) || e >= thr;
+}
+
+// _adapt_dt (solver_sap_adaptive.py:237-345); returns accept.  `acc_inc` counts accepted steps.
+template <typename T>
+__device__ __forceinline__ bool adapt_dt(T e, T& sim_time, T next_time, T& dt, T& dt_half, T& ideal_dt,
+                                         uint8_t& diverged, T& acc_err, int& substeps, int& acc_inc, int mode,
+                                         T tol, T dt_min, T dt_max, T thr) {
+  T step = dt;
+  if (sim_time >= next_time || step <= (T)0) return false;
+  bool div = is_div_err(e, thr);
+  if (mode == 0) {
+    if (div) {
+      sim_time = next_time;
+      diverged = 1;
+      return false;
+    }
+    sim_time = sim_time + step;
+    acc_err = e;
+    substeps += 1;
+    acc_inc += 1;
+    return true;
+  }
+  if (step <= dt_min * (T)1.001) {
+    if (div) {
+      sim_time = next_time;
+      diverged = 1;
+      ideal_dt = dt_min;
+      return false;
+    }
+    sim_time = sim_time + step;
+    acc_err = e;
+    substeps += 1;
+    acc_inc += 1;
+    ideal_dt = drake_new_step(step, e, tol);
+    return true;
+  }
+  if (div) {
+    T ns = (T)DRAKE_MIN_SHRINK * step;
+    ideal_dt = ns;
+    T d = ns < dt_min ? dt_min : (ns > dt_max ? dt_max : ns);
+    dt = d;
+    dt_half = d * (T)0.5;
+    return false;
+  }
+  T ns = drake_new_step(step, e, tol);
+  bool acc = (e <= tol) || (ns >= step);
+  T d = ns < dt_min ? dt_min : (ns > dt_max ? dt_max : ns);
+  if (acc) {
+    sim_time = sim_time + step;
+    acc_err = e;
+    substeps += 1;
+    acc_inc += 1;
+  }
+  ideal_dt = ns;
+  dt = d;
+  dt_half = d * (T)0.5;
+  return acc;
+}
+
+// _calc_adjusted_step (solver_mujoco_adaptive.py:120-198)
+template <typename T>
+__device__ __forceinline__ void calc_adjusted_step(T e, T step, T& ideal_dt, uint8_t& accepted, uint8_t& commit,
+                                                   uint8_t& diverged, T tol, T dt_min, T thr) {
+  bool div = is_div_err(e, thr);
+  if (step <= (T)0) {
+    accepted = 1;
+    commit = 1;
+    return;
+  }
+  if (step <= dt_min * (T)1.001) {
+    if (div) {
+      accepted = 1;
+      commit = 0;
+      diverged = 1;
+      ideal_dt = dt_min;
+      return;
+    }
+    if (e > tol) {
+      accepted = 1;
+      commit = 1;
+      ideal_dt = dt_min;
+      return;
+    }
+  }
+  if (div) {
+    accepted = 0;
+    commit = 0;
+    ideal_dt = (T)DRAKE_MIN_SHRINK * step;
+    return;
+  }
+  T ee = e > (T)1.0e-30 ? e : (T)1.0e-30;
+  T ns = (T)DRAKE_SAFETY * step * sqrt(tol / ee);
+  if (ns > (T)DRAKE_HYST_LOW * step && ns < (T)DRAKE_HYST_HIGH * step) ns = step;
+  T lo = (T)DRAKE_MIN_SHRINK * step, hi = (T)DRAKE_MAX_GROW * step;
+  ns = ns < lo ? lo : (ns > hi ? hi : ns);
+  bool acc = (e <= tol) || (ns >= step);
+  accepted = acc;
+  commit = acc;
+  ideal_dt = ns;
+}
+
+// ---------------------------------------------------------------- reference-semantics f32 kernels
+__global__ void k_calc_adjusted_step_f32(int n, const float* err, const float* dt, float* ideal, uint8_t* acc,
+                                         uint8_t* com, uint8_t* div, float tol, float dt_min, float thr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float idl = ideal[i];
+  uint8_t a = acc[i], c = com[i], d = div[i];
+  calc_adjusted_step<float>(err[i], dt[i], idl, a, c, d, tol, dt_min, thr);
+  ideal[i] = idl;
+  acc[i] = a;
+  com[i] = c;
+  div[i] = d;
+}
+__global__ void k_adapt_dt_f32(int n, const float* err, float* sim, const float* nxt, float* dt, float* dth,
+                               float* ideal, uint8_t* div, uint8_t* acc, float* acc_err, int* sub, int* cum, int mode,
+                               float tol, float dt_min, float dt_max, float thr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = sim[i], d = dt[i], dh = dth[i], idl = ideal[i], ae = acc_err[i];
+  uint8_t dv = div[i];
+  int sb = sub[i], inc = 0;
+  bool a = adapt_dt<float>(err[i], s, nxt[i], d, dh, idl, dv, ae, sb, inc, mode, tol, dt_min, dt_max, thr);
+  sim[i] = s; dt[i] = d; dth[i] = dh; ideal[i] = idl; acc_err[i] = ae; div[i] = dv; sub[i] = sb; acc[i] = a;
+  if (inc) atomicAdd(cum, inc);
+}
+__global__ void k_rebase_time_f32(int n, float* sim, float* nxt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float b = nxt[i];
+  sim[i] = sim[i] - b;
+  nxt[i] = nxt[i] - b;
+}
+__global__ void k_reset_worlds_f32(int n, const uint8_t* mask, float dt_init, float* ideal, float* dt, float* dth,
+                                   float* sim, float* nxt, uint8_t* div, uint8_t* acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !mask[i]) return;
+  ideal[i] = dt_init; dt[i] = dt_init; dth[i] = dt_init * 0.5f;
+  sim[i] = 0.f; nxt[i] = 0.f; div[i] = 0; acc[i] = 0;
+}
+__global__ void k_clamp_dt_f32(int n, float* dt, float* dth, const float* sim, const float* nxt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float rem = nxt[i] - sim[i];
+  if (rem <= 0.f) { dt[i] = 0.f; dth[i] = 0.f; }
+  else if (dt[i] > rem) { dt[i] = rem; dth[i] = rem * 0.5f; }
+}
+__global__ void k_inf_norm_f32(int n, int coords, const float* qf, const float* qd, const float* scale, float* err) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n) return;
+  float m = 0.f;
+  bool nan = false;
+  for (int i = 0; i < coords; ++i) {
+    float s = scale[w * coords + i] * fabsf(qd[w * coords + i] - qf[w * coords + i]);
+    if (s != s) nan = true;
+    m = fmaxf(m, s);
+  }
+  if (nan || isinf(m)) m = 1.0e10f;
+  err[w] = m;
+}
+
+// ---------------------------------------------------------------- world-level device pipeline
+struct WorldCtx {
+  int w, b0, nb, nl, nseg;
+};
+__device__ __forceinline__ WorldCtx world_ctx(const NaModel& M, int w) {
+  WorldCtx c;
+  c.w = w;
+  c.b0 = M.world_body_start[w];
+  c.nb = M.world_body_start[w + 1] - c.b0;
+  c.nl = M.world_shape_start[w + 1] - M.world_shape_start[w];
+  c.nseg = c.nl + M.G;
+  return c;
+}
+
+// Contacts of the world at this lane's body pose q (f64).  Returns the contact count kept.
+__device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, const double* q,
+                             int& overflow, long long* wk) {
+  const int l = lane_id();
+  if (l < wc.nb) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) S.body.q[k][l] = (float)q[k];
+  }
+  wave_sync();
+  stage_aabb(M, wc.w, wc.nseg, wc.nl, wc.b0, &S.body.q[0][0], NA_MAX_BODIES, S.seg);
+  int np = cfg.broad_phase == 1 ? stage_broad_nxn(M, wc.nseg, S.seg) : stage_broad_sap(M, wc.nseg, S.seg);
+  int cap = min(cfg.max_contacts, NA_MAX_CONTACTS);
+  int nc = stage_narrow(M, wc.b0, np, cap, S.seg, S.con);
+  if (np > NA_MAX_PAIRS || nc > cap) overflow = 1;
+  wk[3] += min(nc, cap);
+  wk[4] += np;
+  return min(nc, cap);
+}
+
+// One convex step of size h from (q, v0) with the contacts in LDS; integrates into qo/vo.
+__device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, int nc,
+                            const double* q, const float* v0, const float* f, double h, const float* guess, double* qo,
+                            float* vo, long long* wk) {
+  const int l = lane_id();
+  const bool own = l < wc.nb;
+  BodyReg br;
+  if (own) body_setup(M, wc.b0 + l, q, v0, f, h, br, S.body, l);
+  wave_sync();
+  contact_setup(nc, (float)h, cfg, S.con, S.body, S.sol);
+  double v[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = guess[k];
+  SolveStats st;
+  bool ok = sap_solve(wc.nb, nc, cfg, S.con, S.body, S.sol, br, v, st);
+  if (own) integrate_body(M, wc.b0 + l, q, v, h, qo);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vo[k] = (float)v[k];
+  wk[1] += st.iterations;
+  wk[2] += st.cg_iterations;
+  wk[5] += 1;
+  return ok;
+}
+
+// Step-doubling attempt (solver_sap_adaptive.py:732-849): returns the FP64 error norm
+// (1e10 sentinel for NaN/inf), `ok` false when a convex solve did not converge.
+__device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S,
+                                const double* q, const float* v, const float* f, double dt, double* qf, float* vf,
+                                double* qd, float* vd, bool& ok, int& overflow, int& nc_out, long long* wk) {
+  const int l = lane_id();
+  const bool own = l < wc.nb;
+  int nc = world_collide(M, cfg, wc, S, q, overflow, wk);
+  nc_out = nc;
+  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, v, qf, vf, wk);
+  double err = 0.0;
+  if (cfg.mode == 1) {
+    float g1[6];
+    double qm[7];
+    float vm[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g1[k] = 0.5f * (v[k] + vf[k]);
+    ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
+    int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
+    ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, vf, qd, vd, wk) && ok;
+    double e = 0.0;
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        double d = fabs(qd[k] - qf[k]);
+        e = (d > e || d != d) ? d : e;
+      }
+    }
+    err = wave_max_d(e);
+  } else {
+    double e = 0.0;
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        qd[k] = qf[k];
+        if (qf[k] != qf[k]) e = qf[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) vd[k] = vf[k];
+    }
+    err = wave_max_d(e);
+  }
+  if (isnan(err) || isinf(err)) err = 1.0e10;
+  wk[0] += 1;
+  return err;
+}
+
+// Per-world controller scalars held in registers during a frame.
+struct WCtrl {
+  double dt, dt_half, ideal_dt, sim_time, next_time, last_error, acc_error;
+  uint8_t accepted, commit, diverged;
+  int substeps, iterations, solve_fail, overflow;
+};
+__device__ __forceinline__ WCtrl load_ctrl(const Ctrl& K, int w) {
+  WCtrl c;
+  c.dt = K.dt[w]; c.dt_half = K.dt_half[w]; c.ideal_dt = K.ideal_dt[w]; c.sim_time = K.sim_time[w];
+  c.next_time = K.next_time[w]; c.last_error = K.last_error[w]; c.acc_error = K.acc_error[w];
+  c.accepted = K.accepted[w]; c.commit = K.commit[w]; c.diverged = K.diverged[w];
+  c.substeps = K.substeps[w]; c.iterations = K.iterations[w]; c.solve_fail = K.solve_fail[w];
+  c.overflow = K.overflow[w];
+  return c;
+}
+__device__ __forceinline__ void store_ctrl(const Ctrl& K, int w, const WCtrl& c) {
+  K.dt[w] = c.dt; K.dt_half[w] = c.dt_half; K.ideal_dt[w] = c.ideal_dt; K.sim_time[w] = c.sim_time;
+  K.next_time[w] = c.next_time; K.last_error[w] = c.last_error; K.acc_error[w] = c.acc_error;
+  K.accepted[w] = c.accepted; K.commit[w] = c.commit; K.diverged[w] = c.diverged;
+  K.substeps[w] = c.substeps; K.iterations[w] = c.iterations; K.solve_fail[w] = c.solve_fail;
+  K.overflow[w] = c.overflow;
+}
+
+// _open_frame + _seed_dt (SAP) or _apply_dt_cap + _rebase_time + _boundary_advance (MuJoCo)
+__device__ __forceinline__ void open_frame(const NaConfig& cfg, WCtrl& c, double dt_outer, double eff_max) {
+  if (cfg.controller == 0) {
+    double base = c.next_time;
+    c.sim_time = c.sim_time - base;
+    c.next_time = dt_outer;
+    double d = cfg.mode == 0 ? cfg.dt_init : c.ideal_dt;
+    d = d < cfg.dt_min ? cfg.dt_min : (d > eff_max ? eff_max : d);
+    c.dt = d;
+    c.dt_half = 0.5 * d;
+    c.substeps = 0;
+    c.diverged = 0;
+  } else {
+    double d = c.ideal_dt;
+    d = d < cfg.dt_min ? cfg.dt_min : (d > eff_max ? eff_max : d);
+    c.dt = d;
+    c.dt_half = 0.5 * d;
+    double base = c.next_time;
+    c.sim_time = c.sim_time - base;
+    c.next_time = c.next_time - base;
+    c.next_time = c.next_time + dt_outer;
+    c.diverged = 0;
+    c.substeps = 0;
+  }
+  c.iterations = 0;
+  c.solve_fail = 0;
+  c.overflow = 0;
+}
+
+// One iteration of the boundary loop for one world (clamp, attempt, controller, commit).
+// q/v hold this lane's body state and are updated in place.  Returns accepted-step count (0/1).
+__device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, WCtrl& c,
+                               double* q, float* v, const float* f, double eff_max, long long* wk) {
+  c.iterations += 1;
+  double rem = c.next_time - c.sim_time;
+  if (rem <= 0.0) { c.dt = 0.0; c.dt_half = 0.0; }
+  else if (c.dt > rem) { c.dt = rem; c.dt_half = 0.5 * rem; }
+  double step = c.dt;
+  double qf[7], qd[7];
+  float vf[6], vd[6];
+  bool ok = true;
+  int ovf = 0, nc = 0;
+  double err = 0.0;
+  if (step > 0.0 && wc.nb > 0) {
+    err = world_attempt(M, cfg, wc, S, q, v, f, step, qf, vf, qd, vd, ok, ovf, nc, wk);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { qf[k] = q[k]; qd[k] = q[k]; }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { vf[k] = v[k]; vd[k] = v[k]; }
+  }
+  if (ovf) c.overflow = 1;
+  if (!ok) {
+    err = cfg.divergence_threshold;
+    c.solve_fail = 1;
+  }
+  c.last_error = err;
+  bool commit;
+  int inc = 0;
+  if (cfg.controller == 0) {
+    commit = adapt_dt<double>(err, c.sim_time, c.next_time, c.dt, c.dt_half, c.ideal_dt, c.diverged, c.acc_error,
+                              c.substeps, inc, cfg.mode, cfg.tol, cfg.dt_min, eff_max, cfg.divergence_threshold);
+    c.accepted = commit;
+  } else {
+    calc_adjusted_step<double>(err, c.dt, c.ideal_dt, c.accepted, c.commit, c.diverged, cfg.tol, cfg.dt_min,
+                               cfg.divergence_threshold);
+    commit = c.commit;
+    if (c.accepted) {
+      c.sim_time += c.dt;
+      c.acc_error = err;
+      if (step > 0.0) { c.substeps += 1; inc = 1; }
+    }
+    if (c.diverged) c.sim_time = c.next_time;
+    double d = c.ideal_dt;
+    d = d < cfg.dt_min ? cfg.dt_min : (d > eff_max ? eff_max : d);
+    c.dt = d;
+    c.dt_half = 0.5 * d;
+  }
+  if (commit) {
+    const double* sq = cfg.mode == 0 ? qf : qd;
+    const float* sv = cfg.mode == 0 ? vf : vd;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = sq[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = sv[k];
+  }
+  return inc;
+}
+
+// Load this lane's body (FP64 shadow kept when it still rounds to the caller's f32 pose).
+__device__ __forceinline__ void load_body(const WorldCtx& wc, const double* q64, const float* body_q,
+                                          const float* body_qd, const float* body_f, double* q, float* v, float* f) {
+  const int l = lane_id();
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+    bool same = true;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      q[k] = q64[7 * gb + k];
+      if (body_q) same = same && ((float)q[k] == body_q[7 * gb + k]);
+    }
+    if (!same) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) q[k] = (double)body_q[7 * gb + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      v[k] = body_qd[6 * gb + k];
+      f[k] = body_f ? body_f[6 * gb + k] : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = 0.0;
+    q[6] = 1.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { v[k] = 0.f; f[k] = 0.f; }
+  }
+}
+__device__ __forceinline__ void store_body(const WorldCtx& wc, double* q64, float* body_q, float* body_qd,
+                                           const double* q, const float* v) {
+  const int l = lane_id();
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      q64[7 * gb + k] = q[k];
+      body_q[7 * gb + k] = (float)q[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) body_qd[6 * gb + k] = v[k];
+  }
+}
+__device__ __forceinline__ void flush_work(const Ctrl& K, int w, const long long* wk) {
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < NA_WORK; ++k) K.work[(long long)w * NA_WORK + k] += wk[k];
+  }
+}
+
+// ---------------------------------------------------------------- served boundary call
+__global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                              float* body_qd, const float* body_f, double dt_outer, double eff_max,
+                                              int32_t* queue, const int32_t* order) {
+  __shared__ WorldLDS S;
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = atomicAdd(queue, 1);
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= M.W) break;
+    int w = order ? order[slot] : slot;
+    WorldCtx wc = world_ctx(M, w);
+    double q[7];
+    float v[6], f[6];
+    load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
+    WCtrl c = load_ctrl(K, w);
+    open_frame(cfg, c, dt_outer, eff_max);
+    long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+    int acc = 0;
+    for (int it = 0; it < cfg.max_substeps; ++it) {
+      if (!(c.sim_time < c.next_time)) break;
+      acc += world_iteration(M, cfg, wc, S, c, q, v, f, eff_max, wk);
+      if (c.solve_fail) break;
+    }
+    store_body(wc, q64, body_q, body_qd, q, v);
+    if (lane_id() == 0) {
+      store_ctrl(K, w, c);
+      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    }
+    flush_work(K, w, wk);
+  }
+}
+
+// ---------------------------------------------------------------- lock-step boundary call
+__global__ void k_plain_open(NaModel M, NaConfig cfg, Ctrl K, double dt_outer, double eff_max) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= M.W) return;
+  WCtrl c = load_ctrl(K, w);
+  open_frame(cfg, c, dt_outer, eff_max);
+  store_ctrl(K, w, c);
+}
+__global__ void k_sync_shadow(int B, double* q64, const float* body_q) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  bool same = true;
+  for (int k = 0; k < 7; ++k) same = same && ((float)q64[7 * b + k] == body_q[7 * b + k]);
+  if (!same)
+    for (int k = 0; k < 7; ++k) q64[7 * b + k] = (double)body_q[7 * b + k];
+}
+__global__ void k_round_shadow(int B, const double* q64, float* body_q) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 7 * B) return;
+  body_q[i] = (float)q64[i];
+}
+__global__ void k_zero_i32(int32_t* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_qd,
+                                                   const float* body_f, double eff_max, int32_t* flag) {
+  __shared__ WorldLDS S;
+  int w = blockIdx.x;
+  WCtrl c = load_ctrl(K, w);
+  if (!(c.sim_time < c.next_time)) return;
+  WorldCtx wc = world_ctx(M, w);
+  const int l = lane_id();
+  double q[7];
+  float v[6], f[6];
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = q64[7 * gb + k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      v[k] = body_qd[6 * gb + k];
+      f[k] = body_f ? body_f[6 * gb + k] : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = 0.0;
+    q[6] = 1.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { v[k] = 0.f; f[k] = 0.f; }
+  }
+  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  int acc = world_iteration(M, cfg, wc, S, c, q, v, f, eff_max, wk);
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q64[7 * gb + k] = q[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) body_qd[6 * gb + k] = v[k];
+  }
+  if (l == 0) {
+    store_ctrl(K, w, c);
+    if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    if (c.solve_fail) atomicOr(flag, 2);
+    else if (c.sim_time < c.next_time) atomicOr(flag, 1);
+  }
+  flush_work(K, w, wk);
+}
+
+// ---------------------------------------------------------------- reset
+__global__ void k_reset(int W, const uint8_t* mask, double dt_init, Ctrl K) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W || (mask && !mask[i])) return;
+  K.ideal_dt[i] = dt_init;
+  K.dt[i] = dt_init;
+  K.dt_half[i] = dt_init * 0.5;
+  K.sim_time[i] = 0.0;
+  K.next_time[i] = 0.0;
+  K.diverged[i] = 0;
+  K.accepted[i] = 0;
+}
+
+// ---------------------------------------------------------------- contacts API
+#define NA_STAGE_WORDS 20
+__global__ __launch_bounds__(64) void k_collide_api(NaModel M, NaConfig cfg, const float* body_q, int per_world,
+                                                    int32_t* staging, int32_t* world_counts) {
+  __shared__ WorldLDS S;
+  int w = blockIdx.x;
+  WorldCtx wc = world_ctx(M, w);
+  const int l = lane_id();
+  double q[7];
+  if (l < wc.nb) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = body_q[7 * (wc.b0 + l) + k];
+  }
+  int ovf = 0;
+  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  NaConfig c2 = cfg;
+  c2.max_contacts = min(per_world, NA_MAX_CONTACTS);
+  int nc = world_collide(M, c2, wc, S, q, ovf, wk);
+  if (l == 0) world_counts[w] = nc;
+  // write_contact (collide.py:62-147): body-frame points and offsets
+  for (int c = l; c < nc; c += NA_WAVE) {
+    int ga = S.seg.gidx[S.con.sa[c]], gb = S.seg.gidx[S.con.sb[c]];
+    f3 n = mk(S.con.n[0][c], S.con.n[1][c], S.con.n[2][c]);
+    f3 ctr = mk(S.con.p[0][c], S.con.p[1][c], S.con.p[2][c]);
+    float d = S.con.dist[c], ra = S.con.ra[c], rb = S.con.rb[c];
+    float ma = M.shape_margin[ga], mb = M.shape_margin[gb];
+    f3 aw = ctr - n * (0.5f * d + ra);
+    f3 bw = ctr + n * (0.5f * d + rb);
+    f3 p0 = aw, p1 = bw, o0 = n * (ra + ma), o1 = n * (-(rb + mb));
+    int ba = S.con.ba[c], bb = S.con.bb[c];
+    if (ba >= 0) {
+      m3 R = qmat(S.body.q[3][ba], S.body.q[4][ba], S.body.q[5][ba], S.body.q[6][ba]);
+      p0 = mtv(R, aw - mk(S.body.q[0][ba], S.body.q[1][ba], S.body.q[2][ba]));
+      o0 = mtv(R, o0);
+    }
+    if (bb >= 0) {
+      m3 R = qmat(S.body.q[3][bb], S.body.q[4][bb], S.body.q[5][bb], S.body.q[6][bb]);
+      p1 = mtv(R, bw - mk(S.body.q[0][bb], S.body.q[1][bb], S.body.q[2][bb]));
+      o1 = mtv(R, o1);
+    }
+    int32_t* o = staging + ((long long)w * per_world + c) * NA_STAGE_WORDS;
+    o[0] = ga; o[1] = gb;
+    float vals[17] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, o0.x, o0.y, o0.z, o1.x, o1.y, o1.z,
+                      n.x, n.y, n.z, ra + ma, rb + mb};
+#pragma unroll
+    for (int k = 0; k < 17; ++k) o[2 + k] = __float_as_int(vals[k]);
+  }
+}
+// exclusive scan of per-world kept counts (single workgroup), total into count[0]
+__global__ void k_scan_counts(int W, int per_world, const int32_t* counts, int32_t* offsets, int32_t* total) {
+  __shared__ int32_t buf[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < W; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < W ? min(counts[i], per_world) : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < W) offsets[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) total[0] = carry;
+}
+__global__ void k_compact_contacts(int W, int per_world, const int32_t* staging, const int32_t* counts,
+                                   const int32_t* offsets, na_contacts_desc out) {
+  int w = blockIdx.x;
+  int nc = min(counts[w], per_world);
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+    int dst = offsets[w] + c;
+    if (dst >= out.capacity) continue;
+    const int32_t* s = staging + ((long long)w * per_world + c) * NA_STAGE_WORDS;
+    out.shape0[dst] = s[0];
+    out.shape1[dst] = s[1];
+    const float* f = (const float*)(s + 2);
+    for (int k = 0; k < 3; ++k) {
+      out.point0[3 * dst + k] = f[k];
+      out.point1[3 * dst + k] = f[3 + k];
+      out.offset0[3 * dst + k] = f[6 + k];
+      out.offset1[3 * dst + k] = f[9 + k];
+      out.normal[3 * dst + k] = f[12 + k];
+    }
+    out.margin0[dst] = f[15];
+    out.margin1[dst] = f[16];
+  }
+}
+
+__global__ __launch_bounds__(64) void k_collide_debug(NaModel M, int broad_phase, const float* body_q, float* lo,
+                                                      float* hi, int32_t* pairs, int32_t* pair_count, int max_pairs) {
+  __shared__ WorldLDS S;
+  int w = blockIdx.x;
+  WorldCtx wc = world_ctx(M, w);
+  const int l = lane_id();
+  if (l < wc.nb) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) S.body.q[k][l] = body_q[7 * (wc.b0 + l) + k];
+  }
+  wave_sync();
+  stage_aabb(M, w, wc.nseg, wc.nl, wc.b0, &S.body.q[0][0], NA_MAX_BODIES, S.seg);
+  for (int i = l; i < wc.nseg; i += NA_WAVE) {
+    if (i >= wc.nl && w != 0) continue;  // global shapes are written once
+    int s = S.seg.gidx[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[3 * s + k] = S.seg.lo[k][i];
+      hi[3 * s + k] = S.seg.hi[k][i];
+    }
+  }
+  int np = broad_phase == 1 ? stage_broad_nxn(M, wc.nseg, S.seg) : stage_broad_sap(M, wc.nseg, S.seg);
+  if (l == 0) pair_count[w] = np;
+  int m = min(np, min(max_pairs, NA_MAX_PAIRS));
+  for (int k = l; k < m; k += NA_WAVE) {
+    uint32_t pr = S.seg.pairs[k];
+    int a = S.seg.gidx[pr >> 8], b = S.seg.gidx[pr & 0xff];
+    pairs[((long long)w * max_pairs + k) * 2] = min(a, b);
+    pairs[((long long)w * max_pairs + k) * 2 + 1] = max(a, b);
+  }
+}
+
+// ---------------------------------------------------------------- parity entry kernels
+__global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const double* q64, const float* qd,
+                                                const float* f, const double* h, double* q_out, float* qd_out,
+                                                int32_t* stats) {
+  __shared__ WorldLDS S;
+  int w = blockIdx.x;
+  WorldCtx wc = world_ctx(M, w);
+  const int l = lane_id();
+  double q[7], qo[7];
+  float v[6], ff[6], vo[6];
+  load_body(wc, q64, nullptr, qd, f, q, v, ff);
+  if (l < wc.nb) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = q64[7 * (wc.b0 + l) + k];
+  }
+  if (wc.nb == 0) return;
+  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  int ovf = 0;
+  int nc = world_collide(M, cfg, wc, S, q, ovf, wk);
+  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk);
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q_out[7 * gb + k] = qo[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) qd_out[6 * gb + k] = vo[k];
+  }
+  if (l == 0) {
+    stats[4 * w + 0] = (int)wk[1];
+    stats[4 * w + 1] = (int)wk[1];
+    stats[4 * w + 2] = ok ? 1 : 0;
+    stats[4 * w + 3] = nc;
+  }
+}
+__global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const double* q64, const float* qd,
+                                                const float* f, const double* dt, double* q_full, float* qd_full,
+                                                double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
+                                                int32_t* n_contacts) {
+  __shared__ WorldLDS S;
+  int w = blockIdx.x;
+  WorldCtx wc = world_ctx(M, w);
+  const int l = lane_id();
+  double q[7], qf[7], qd2[7];
+  float v[6], ff[6], vf[6], vd[6];
+  load_body(wc, q64, nullptr, qd, f, q, v, ff);
+  if (l < wc.nb) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) q[k] = q64[7 * (wc.b0 + l) + k];
+  }
+  if (wc.nb == 0) {
+    if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
+    return;
+  }
+  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  bool ok = true;
+  int ovf = 0, nc = 0;
+  double e = world_attempt(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);
+  if (l < wc.nb) {
+    int gb = wc.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { q_full[7 * gb + k] = qf[k]; q_dbl[7 * gb + k] = qd2[k]; }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { qd_full[6 * gb + k] = vf[k]; qd_dbl[6 * gb + k] = vd[k]; }
+  }
+  if (l == 0) {
+    err[w] = e;
+    solve_ok[w] = ok ? 1 : 0;
+    n_contacts[w] = nc;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static inline hipStream_t S_(void* s) { return (hipStream_t)s; }
+static inline int blocks(long long n, int t) { return (int)((n + t - 1) / t); }
+
+extern "C" int na_device_info(char* name, int name_len, int* cu_count, int* lds_per_block) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    strncpy(name, p.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (lds_per_block) *lds_per_block = (int)sizeof(WorldLDS);
+  return 0;
+}
+
+extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
+  if (!d || !out) return fail("na_model_create: null argument");
+  if (d->world_count <= 0) return fail("na_model_create: world_count must be > 0");
+  na_model* m = new na_model();
+  NaModel& M = m->M;
+  M.W = d->world_count; M.B = d->body_count; M.S = d->shape_count; M.G = d->global_count; M.F = d->filter_count;
+  M.world_body_start = d->world_body_start; M.world_shape_start = d->world_shape_start;
+  M.global_shapes = d->global_shapes;
+  M.body_com = d->body_com; M.body_mass = d->body_mass; M.body_inv_mass = d->body_inv_mass;
+  M.body_inertia = d->body_inertia; M.body_inv_inertia = d->body_inv_inertia;
+  M.shape_type = d->shape_type; M.shape_body = d->shape_body; M.shape_group = d->shape_collision_group;
+  M.shape_world = d->shape_world; M.shape_xform = d->shape_transform; M.shape_scale = d->shape_scale;
+  M.shape_margin = d->shape_margin; M.shape_gap = d->shape_gap; M.shape_ke = d->shape_material_ke;
+  M.shape_mu = d->shape_material_mu; M.shape_radius = d->shape_collision_radius; M.filter_pairs = d->filter_pairs;
+  M.gx = d->gravity[0]; M.gy = d->gravity[1]; M.gz = d->gravity[2];
+  // validate per-world sizes against the wave-per-world limits
+  std::string err;
+  int* wbs = new int[M.W + 1];
+  int* wss = new int[M.W + 1];
+  hipError_t e1 = hipMemcpy(wbs, d->world_body_start, sizeof(int) * (M.W + 1), hipMemcpyDeviceToHost);
+  hipError_t e2 = hipMemcpy(wss, d->world_shape_start, sizeof(int) * (M.W + 1), hipMemcpyDeviceToHost);
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    delete[] wbs; delete[] wss; delete m;
+    return fail("na_model_create: could not read world_body_start / world_shape_start (device pointers?)");
+  }
+  m->max_nb = 0; m->max_nl = 0;
+  for (int w = 0; w < M.W; ++w) {
+    m->max_nb = wbs[w + 1] - wbs[w] > m->max_nb ? wbs[w + 1] - wbs[w] : m->max_nb;
+    m->max_nl = wss[w + 1] - wss[w] > m->max_nl ? wss[w + 1] - wss[w] : m->max_nl;
+  }
+  delete[] wbs; delete[] wss;
+  m->max_seg = m->max_nl + M.G;
+  if (m->max_nb > NA_MAX_BODIES) { delete m; return fail("na_model_create: a world has more than 64 bodies"); }
+  if (m->max_seg > NA_MAX_SEG) {
+    delete m;
+    return fail("na_model_create: a world segment (world shapes + global shapes) exceeds 128 shapes");
+  }
+  *out = m;
+  return 0;
+}
+extern "C" int na_model_destroy(na_model* m) {
+  delete m;
+  return 0;
+}
+
+static NaConfig to_cfg(const na_config* c) {
+  NaConfig k;
+  k.mode = c->mode; k.controller = c->controller; k.tol = c->tol; k.dt_init = c->dt_inner_init;
+  k.dt_min = c->dt_inner_min; k.dt_max = c->dt_inner_max; k.divergence_threshold = c->divergence_threshold;
+  k.max_substeps = c->max_substeps; k.max_iterations = c->max_iterations;
+  k.max_contacts = c->max_rigid_contact < NA_MAX_CONTACTS ? c->max_rigid_contact : NA_MAX_CONTACTS;
+  k.broad_phase = c->broad_phase; k.opt_rel_tol = c->optimality_rel_tol; k.opt_abs_tol = c->optimality_abs_tol;
+  k.tau_d = c->contact_tau_d; k.sigma = c->sigma; k.beta = c->beta; k.ls_rel_tol = c->ls_rel_tol;
+  k.cg_rel_tol = c->cg_rel_tol; k.max_cg = c->max_cg;
+  return k;
+}
+
+extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_telemetry* buffers,
+                                na_solver** out) {
+  if (!m || !c || !out) return fail("na_solver_create: null argument");
+  if (c->mode != 0 && c->mode != 1) return fail("na_solver_create: mode must be 0 (fixed) or 1 (adaptive)");
+  if (c->controller != 0 && c->controller != 1) return fail("na_solver_create: controller must be 0 or 1");
+  if (!(c->tol > 0)) return fail("na_solver_create: tol must be > 0");
+  if (!(c->dt_inner_init > 0)) return fail("na_solver_create: dt_inner_init must be > 0");
+  if (!(c->dt_inner_min > 0)) return fail("na_solver_create: dt_inner_min must be > 0");
+  if (!(c->dt_inner_max > 0)) return fail("na_solver_create: dt_inner_max must be > 0");
+  if (c->max_substeps < 1) return fail("na_solver_create: max_substeps must be >= 1");
+  if (c->max_rigid_contact < 1 || c->max_rigid_contact > NA_MAX_CONTACTS)
+    return fail("na_solver_create: max_rigid_contact must be in [1, 128] per world");
+  na_solver* s = new na_solver();
+  s->model = m;
+  s->cfg = to_cfg(c);
+  s->pool = nullptr;
+  int W = m->M.W, B = m->M.B;
+  char* pool = nullptr;
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    char* p = pool + off;
+    off += (n + 255) & ~(size_t)255;
+    return p;
+  };
+  size_t need = 0;
+  size_t sizes_own[] = {8u * W, 8u * W, 8u * W, 8u * W, 8u * W, 8u * W, 8u * W, (size_t)W, (size_t)W, (size_t)W,
+                        4u * W, 4u * W, 4u * W, 4u * W, 8u, sizeof(long long) * NA_WORK * W,
+                        sizeof(double) * 7 * (size_t)(B > 0 ? B : 1), 4u, 4u};
+  size_t sizes_ext[] = {4u, 4u};
+  if (buffers) for (size_t z : sizes_ext) need += (z + 255) & ~(size_t)255;
+  else for (size_t z : sizes_own) need += (z + 255) & ~(size_t)255;
+  if (hipMalloc(&pool, need) != hipSuccess) { delete s; return fail("na_solver_create: hipMalloc failed"); }
+  if (hipMemset(pool, 0, need) != hipSuccess) { hipFree(pool); delete s; return fail("na_solver_create: memset"); }
+  s->pool = pool;
+  Ctrl& K = s->K;
+  if (buffers) {
+    const na_telemetry* b = buffers;
+    if (!b->dt || !b->dt_half || !b->ideal_dt || !b->sim_time || !b->next_time || !b->last_raw_error ||
+        !b->last_error || !b->accepted || !b->commit || !b->diverged || !b->substeps || !b->iterations ||
+        !b->solve_fail || !b->overflow || !b->cum_accepted || !b->work || !b->body_q64) {
+      hipFree(pool);
+      delete s;
+      return fail("na_solver_create: every buffer pointer must be set when buffers are supplied");
+    }
+    K.dt = b->dt; K.dt_half = b->dt_half; K.ideal_dt = b->ideal_dt; K.sim_time = b->sim_time;
+    K.next_time = b->next_time; K.last_error = b->last_raw_error; K.acc_error = b->last_error;
+    K.accepted = b->accepted; K.commit = b->commit; K.diverged = b->diverged; K.substeps = b->substeps;
+    K.iterations = b->iterations; K.solve_fail = b->solve_fail; K.overflow = b->overflow;
+    K.cum = b->cum_accepted; K.work = b->work; s->q64 = b->body_q64;
+  } else {
+    K.dt = (double*)take(8u * W); K.dt_half = (double*)take(8u * W); K.ideal_dt = (double*)take(8u * W);
+    K.sim_time = (double*)take(8u * W); K.next_time = (double*)take(8u * W); K.last_error = (double*)take(8u * W);
+    K.acc_error = (double*)take(8u * W);
+    K.accepted = (uint8_t*)take(W); K.commit = (uint8_t*)take(W); K.diverged = (uint8_t*)take(W);
+    K.substeps = (int32_t*)take(4u * W); K.iterations = (int32_t*)take(4u * W);
+    K.solve_fail = (int32_t*)take(4u * W); K.overflow = (int32_t*)take(4u * W);
+    K.cum = (unsigned long long*)take(8);
+    K.work = (long long*)take(sizeof(long long) * NA_WORK * W);
+    s->q64 = (double*)take(sizeof(double) * 7 * (size_t)(B > 0 ? B : 1));
+  }
+  s->queue = (int32_t*)take(4);
+  s->flag = (int32_t*)take(4);
+  if (hipHostMalloc(&s->flag_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    hipFree(pool);
+    delete s;
+    return fail("na_solver_create: hipHostMalloc failed");
+  }
+  s->order = nullptr;
+  s->grid_waves = 0;
+  // controller defaults (reset of all worlds)
+  k_reset<<<blocks(W, 256), 256>>>(W, nullptr, s->cfg.dt_init, K);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  *out = s;
+  return 0;
+}
+extern "C" int na_solver_destroy(na_solver* s) {
+  if (!s) return 0;
+  if (s->pool) (void)hipFree(s->pool);
+  if (s->flag_host) (void)hipHostFree(s->flag_host);
+  delete s;
+  return 0;
+}
+extern "C" int na_solver_telemetry(const na_solver* s, na_telemetry* t) {
+  if (!s || !t) return fail("na_solver_telemetry: null argument");
+  const Ctrl& K = s->K;
+  t->dt = K.dt; t->dt_half = K.dt_half; t->ideal_dt = K.ideal_dt; t->sim_time = K.sim_time;
+  t->next_time = K.next_time; t->last_raw_error = K.last_error; t->last_error = K.acc_error;
+  t->accepted = K.accepted; t->commit = K.commit; t->diverged = K.diverged; t->substeps = K.substeps;
+  t->iterations = K.iterations; t->solve_fail = K.solve_fail; t->overflow = K.overflow;
+  t->cum_accepted = K.cum; t->work = K.work; t->body_q64 = s->q64;
+  return 0;
+}
+extern "C" int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order) {
+  if (!s) return fail("na_solver_set_schedule: null solver");
+  s->grid_waves = grid_waves;
+  s->order = order;
+  return 0;
+}
+
+static int default_grid(const na_solver* s) {
+  int W = s->model->M.W;
+  if (s->grid_waves > 0) return s->grid_waves < W ? s->grid_waves : W;
+  int dev = 0, cus = 256, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_frame, 64, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  int g = cus * per_cu;
+  return g < W ? g : W;
+}
+
+extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
+                       void* stream) {
+  if (!s || !body_q || !body_qd) return fail("na_step: null argument");
+  if (!(dt_outer > 0)) return fail("na_step: dt (dt_outer) must be > 0");
+  const NaModel& M = s->model->M;
+  double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
+  k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
+  int grid = default_grid(s);
+  k_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
+                                       s->order);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
+                             int32_t* iterations_out, void* stream) {
+  if (!s || !body_q || !body_qd) return fail("na_step_plain: null argument");
+  if (!(dt_outer > 0)) return fail("na_step_plain: dt (dt_outer) must be > 0");
+  const NaModel& M = s->model->M;
+  hipStream_t st = S_(stream);
+  double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
+  k_sync_shadow<<<blocks(M.B, 256), 256, 0, st>>>(M.B, s->q64, body_q);
+  k_plain_open<<<blocks(M.W, 256), 256, 0, st>>>(M, s->cfg, s->K, dt_outer, eff);
+  int status = 0, it = 0;
+  for (; it < s->cfg.max_substeps; ++it) {
+    k_zero_i32<<<1, 64, 0, st>>>(s->flag, 1);
+    k_plain_iter<<<M.W, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_qd, body_f, eff, s->flag);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int f = *s->flag_host;
+    if (f >= 2) { status = 2; ++it; break; }
+    if (f == 0) { ++it; break; }
+  }
+  if (status == 0 && it >= s->cfg.max_substeps && *s->flag_host != 0) status = 1;
+  k_round_shadow<<<blocks(7LL * M.B, 256), 256, 0, st>>>(M.B, s->q64, body_q);
+  HIPCHK(hipGetLastError());
+  if (iterations_out) *iterations_out = it;
+  return status;
+}
+
+extern "C" int na_reset(na_solver* s, const uint8_t* mask, void* stream) {
+  if (!s) return fail("na_reset: null solver");
+  int W = s->model->M.W;
+  k_reset<<<blocks(W, 256), 256, 0, S_(stream)>>>(W, mask, s->cfg.dt_init, s->K);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int64_t na_collide_scratch_words(const na_model* m, int32_t per_world) {
+  if (!m) return -1;
+  return (int64_t)m->M.W * per_world * NA_STAGE_WORDS + 2LL * m->M.W;
+}
+
+extern "C" int na_collide(const na_model* m, const float* body_q, const na_contacts_desc* out, int32_t per_world,
+                          int32_t broad_phase, void* stream) {
+  if (!m || !body_q || !out) return fail("na_collide: null argument");
+  if (per_world < 1 || per_world > NA_MAX_CONTACTS) return fail("na_collide: per-world capacity must be in [1, 128]");
+  const NaModel& M = m->M;
+  hipStream_t st = S_(stream);
+  NaConfig cfg{};
+  cfg.max_contacts = per_world;
+  cfg.broad_phase = broad_phase;
+  int32_t* staging = out->scratch;
+  int32_t* offsets = out->scratch + (long long)M.W * per_world * NA_STAGE_WORDS;
+  k_collide_api<<<M.W, 64, 0, st>>>(M, cfg, body_q, per_world, staging, out->world_count_out);
+  k_scan_counts<<<1, 1024, 0, st>>>(M.W, per_world, out->world_count_out, offsets, out->count);
+  k_compact_contacts<<<M.W, 64, 0, st>>>(M.W, per_world, staging, out->world_count_out, offsets, *out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int na_collide_debug(const na_model* m, const float* body_q, float* lo, float* hi, int32_t* pairs,
+                                int32_t* pair_count, int32_t max_pairs, int32_t broad_phase, void* stream) {
+  if (!m || !body_q || !lo || !hi || !pairs || !pair_count) return fail("na_collide_debug: null argument");
+  const NaModel& M = m->M;
+  k_collide_debug<<<M.W, 64, 0, S_(stream)>>>(M, broad_phase, body_q, lo, hi, pairs, pair_count, max_pairs);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int na_substep(const na_solver* s, const double* q64, const float* qd, const float* f, const double* h,
+                          double* q_out, float* qd_out, int32_t* stats, void* stream) {
+  if (!s || !q64 || !qd || !h || !q_out || !qd_out || !stats) return fail("na_substep: null argument");
+  const NaModel& M = s->model->M;
+  k_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, h, q_out, qd_out, stats);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
+                          double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err,
+                          int32_t* solve_ok, int32_t* n_contacts, void* stream) {
+  if (!s || !q64 || !qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok || !n_contacts)
+    return fail("na_attempt: null argument");
+  const NaModel& M = s->model->M;
+  k_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, dt, q_full, qd_full, q_dbl, qd_dbl, err, solve_ok,
+                                        n_contacts);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- controller entry points
+extern "C" int na_ctrl_calc_adjusted_step_f32(int32_t n, const float* err, const float* dt, float* ideal,
+                                              uint8_t* acc, uint8_t* com, uint8_t* div, float tol, float dt_min,
+                                              float thr, void* stream) {
+  if (n <= 0) return 0;
+  k_calc_adjusted_step_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, err, dt, ideal, acc, com, div, tol, dt_min, thr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_ctrl_adapt_dt_f32(int32_t n, const float* err, float* sim, const float* nxt, float* dt, float* dth,
+                                    float* ideal, uint8_t* div, uint8_t* acc, float* acc_err, int32_t* sub,
+                                    int32_t* cum, int32_t mode, float tol, float dt_min, float dt_max, float thr,
+                                    void* stream) {
+  if (n <= 0) return 0;
+  k_adapt_dt_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, err, sim, nxt, dt, dth, ideal, div, acc, acc_err, sub, cum,
+                                                         mode, tol, dt_min, dt_max, thr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_ctrl_rebase_time_f32(int32_t n, float* sim, float* nxt, void* stream) {
+  if (n <= 0) return 0;
+  k_rebase_time_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, sim, nxt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_ctrl_reset_worlds_f32(int32_t n, const uint8_t* mask, float dt_init, float* ideal, float* dt,
+                                        float* dth, float* sim, float* nxt, uint8_t* div, uint8_t* acc, void* stream) {
+  if (n <= 0) return 0;
+  k_reset_worlds_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, mask, dt_init, ideal, dt, dth, sim, nxt, div, acc);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_ctrl_clamp_dt_to_boundary_f32(int32_t n, float* dt, float* dth, const float* sim, const float* nxt,
+                                                void* stream) {
+  if (n <= 0) return 0;
+  k_clamp_dt_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, dt, dth, sim, nxt);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_ctrl_inf_norm_error_f32(int32_t n, int32_t coords, const float* qf, const float* qd,
+                                          const float* scale, float* err, void* stream) {
+  if (n <= 0) return 0;
+  k_inf_norm_f32<<<blocks(n, 256), 256, 0, S_(stream)>>>(n, coords, qf, qd, scale, err);
+  HIPCHK(hipGetLastError());
+  return 0;
+}

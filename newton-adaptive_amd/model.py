@@ -1,0 +1,180 @@
+"""Device-resident containers with Newton's names, shapes and units.
+
+``Model`` / ``State`` / ``Control`` keep the attribute names, shapes and dtypes of
+the reference's ``newton/_src/sim/model.py:36``, ``state.py:47`` and ``control.py:31``
+so host code written against Newton reads the same arrays.  The arrays are torch
+tensors on the HIP device; their ``data_ptr()`` values are what cross the C ABI.
+
+Layout rules the HIP kernels rely on (checked in ``ModelBuilder.finalize``):
+
+* the bodies of world ``w`` are contiguous, ``[world_body_start[w], world_body_start[w+1])``;
+* the world-local shapes of world ``w`` are contiguous, ``[world_shape_start[w], world_shape_start[w+1])``;
+* shapes of world ``-1`` (ground plane, static walls) are listed in ``global_shapes``;
+* every body carries one FREE joint, so ``joint_q`` (7 per body: world position of the
+  body origin, quaternion ``xyzw``) and ``joint_qd`` (6 per body: COM linear velocity,
+  angular velocity, both world frame) hold exactly the numbers of ``body_q`` / ``body_qd``
+  (``state.py:110-117,149-153``).  ``State.joint_q`` / ``joint_qd`` are therefore views of
+  the same device memory as ``body_q`` / ``body_qd``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class GeoType:
+    """Shape type codes (``newton/_src/geometry/types.py:76-106``)."""
+
+    NONE = 0
+    PLANE = 1
+    HFIELD = 2
+    SPHERE = 3
+    CAPSULE = 4
+    ELLIPSOID = 5
+    CYLINDER = 6
+    BOX = 7
+    MESH = 8
+    CONE = 9
+    CONVEX_MESH = 10
+
+
+SUPPORTED_SHAPES = (GeoType.PLANE, GeoType.SPHERE, GeoType.CAPSULE, GeoType.BOX)
+
+
+def _t(arr, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(arr), dtype=dtype).to(device)
+
+
+class State:
+    """Per-world mutable state (``newton/_src/sim/state.py:47``).
+
+    ``body_q``  [B, 7] float32: origin position + quaternion (x, y, z, w).
+    ``body_qd`` [B, 6] float32: COM linear velocity (world), angular velocity (world).
+    ``body_f``  [B, 6] float32: external wrench at the COM (world).
+    ``joint_q`` [7B] / ``joint_qd`` [6B]: FREE-joint coordinates, views of the same memory.
+    """
+
+    def __init__(self, body_q: torch.Tensor, body_qd: torch.Tensor, body_f: torch.Tensor):
+        self.body_q = body_q
+        self.body_qd = body_qd
+        self.body_f = body_f
+        self.joint_q = body_q.view(-1)
+        self.joint_qd = body_qd.view(-1)
+
+    @property
+    def body_count(self) -> int:
+        return int(self.body_q.shape[0])
+
+    def assign(self, other: "State") -> None:
+        """Copy ``other`` into this state (``state.py:169``)."""
+        self.body_q.copy_(other.body_q)
+        self.body_qd.copy_(other.body_qd)
+        self.body_f.copy_(other.body_f)
+
+    def clear_forces(self) -> None:
+        self.body_f.zero_()
+
+
+class Control:
+    """Control inputs (``newton/_src/sim/control.py:31``); free bodies take no joint targets."""
+
+    def __init__(self, model: "Model"):
+        self.joint_f = torch.zeros(model.joint_dof_count, dtype=torch.float32, device=model.device)
+
+    def clear(self) -> None:
+        self.joint_f.zero_()
+
+
+class Model:
+    """Static scene description (``newton/_src/sim/model.py:36``).
+
+    Built by :meth:`ModelBuilder.finalize`; all arrays live on ``device``.
+    """
+
+    def __init__(self, host: dict, device):
+        self.device = torch.device(device)
+        self._host = host
+        d = self.device
+        self.world_count = int(host["world_count"])
+        self.body_count = int(host["body_count"])
+        self.shape_count = int(host["shape_count"])
+        self.joint_count = self.body_count
+        self.joint_coord_count = 7 * self.body_count
+        self.joint_dof_count = 6 * self.body_count
+        self.up_axis = 2
+        self.gravity = _t(host["gravity"].reshape(1, 3), torch.float32, d)
+
+        self.body_q = _t(host["body_q"], torch.float32, d)
+        self.body_qd = _t(host["body_qd"], torch.float32, d)
+        self.body_com = _t(host["body_com"], torch.float32, d)
+        self.body_mass = _t(host["body_mass"], torch.float32, d)
+        self.body_inv_mass = _t(host["body_inv_mass"], torch.float32, d)
+        self.body_inertia = _t(host["body_inertia"], torch.float32, d)
+        self.body_inv_inertia = _t(host["body_inv_inertia"], torch.float32, d)
+        self.body_world = _t(host["body_world"], torch.int32, d)
+        self.joint_q = self.body_q.view(-1)
+        self.joint_qd = self.body_qd.view(-1)
+
+        self.shape_type = _t(host["shape_type"], torch.int32, d)
+        self.shape_body = _t(host["shape_body"], torch.int32, d)
+        self.shape_transform = _t(host["shape_transform"], torch.float32, d)
+        self.shape_scale = _t(host["shape_scale"], torch.float32, d)
+        self.shape_margin = _t(host["shape_margin"], torch.float32, d)
+        self.shape_gap = _t(host["shape_gap"], torch.float32, d)
+        self.shape_collision_group = _t(host["shape_collision_group"], torch.int32, d)
+        self.shape_world = _t(host["shape_world"], torch.int32, d)
+        self.shape_collision_radius = _t(host["shape_collision_radius"], torch.float32, d)
+        self.shape_material_ke = _t(host["shape_material_ke"], torch.float32, d)
+        self.shape_material_kd = _t(host["shape_material_kd"], torch.float32, d)
+        self.shape_material_mu = _t(host["shape_material_mu"], torch.float32, d)
+        fp = host["shape_collision_filter_pairs"].reshape(-1, 2)
+        self.shape_collision_filter_pairs = _t(fp if len(fp) else np.zeros((0, 2)), torch.int32, d)
+
+        self.world_body_start = _t(host["world_body_start"], torch.int32, d)
+        self.world_shape_start = _t(host["world_shape_start"], torch.int32, d)
+        self.global_shapes = _t(host["global_shapes"], torch.int32, d)
+
+    # --- host mirrors (numpy) used by the oracle-side tests and the autotuner ---
+    def host(self, name: str) -> np.ndarray:
+        return self._host[name]
+
+    def bodies_per_world(self) -> np.ndarray:
+        s = self._host["world_body_start"]
+        return s[1:] - s[:-1]
+
+    def shapes_per_world(self) -> np.ndarray:
+        s = self._host["world_shape_start"]
+        return s[1:] - s[:-1] + len(self._host["global_shapes"])
+
+    def state(self) -> State:
+        """Fresh state initialised from the model's reference pose (``model.py:1572``)."""
+        return State(self.body_q.clone(), self.body_qd.clone(),
+                     torch.zeros((self.body_count, 6), dtype=torch.float32, device=self.device))
+
+    def control(self) -> Control:
+        return Control(self)
+
+    def contacts(self, rigid_contact_max: int | None = None):
+        from .contacts import Contacts
+
+        cap = rigid_contact_max if rigid_contact_max is not None else 128 * self.world_count
+        return Contacts(cap, self.device)
+
+    def collide(self, state: State, contacts=None):
+        """One-shot contact generation through a default pipeline (``model.py:1744``)."""
+        from .contacts import CollisionPipeline
+
+        if not hasattr(self, "_default_pipeline"):
+            self._default_pipeline = CollisionPipeline(self)
+        pipe = self._default_pipeline
+        if contacts is None:
+            contacts = pipe.contacts()
+        pipe.collide(state, contacts)
+        return contacts
+
+
+def eval_fk(model: Model, joint_q: torch.Tensor, joint_qd: torch.Tensor, state: State) -> None:
+    """Forward kinematics for FREE joints: body pose/twist are the joint coordinates."""
+    state.body_q.view(-1).copy_(joint_q.view(-1))
+    state.body_qd.view(-1).copy_(joint_qd.view(-1))

@@ -1,0 +1,76 @@
+"""Native handles: a :class:`Model` bound to ``na_model`` in ``libnadapt.so``."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+
+
+class ModelHandle:
+    """Owns the ``na_model`` of a :class:`Model` (the device arrays stay owned by the Model)."""
+
+    def __init__(self, model):
+        if model.device.type != "cuda":
+            raise RuntimeError("the nadapt library runs on a HIP device; build the model with device='cuda:0'")
+        L = _lib.lib()
+        d = _lib.NaModelDesc()
+        d.world_count = model.world_count
+        d.body_count = model.body_count
+        d.shape_count = model.shape_count
+        d.global_count = int(model.global_shapes.numel())
+        d.filter_count = int(model.shape_collision_filter_pairs.shape[0])
+        self._keep = []
+
+        def p(t):
+            t = t.contiguous()
+            self._keep.append(t)
+            return t.data_ptr() if t.numel() else None
+
+        # empty arrays still need a valid pointer for kernels that never read them
+        dummy_i = torch.zeros(2, dtype=torch.int32, device=model.device)
+        self._keep.append(dummy_i)
+        d.world_body_start = p(model.world_body_start)
+        d.world_shape_start = p(model.world_shape_start)
+        d.global_shapes = p(model.global_shapes) or dummy_i.data_ptr()
+        d.body_com = p(model.body_com)
+        d.body_mass = p(model.body_mass)
+        d.body_inv_mass = p(model.body_inv_mass)
+        d.body_inertia = p(model.body_inertia)
+        d.body_inv_inertia = p(model.body_inv_inertia)
+        d.shape_type = p(model.shape_type)
+        d.shape_body = p(model.shape_body)
+        d.shape_collision_group = p(model.shape_collision_group)
+        d.shape_world = p(model.shape_world)
+        d.shape_transform = p(model.shape_transform)
+        d.shape_scale = p(model.shape_scale)
+        d.shape_margin = p(model.shape_margin)
+        d.shape_gap = p(model.shape_gap)
+        d.shape_material_ke = p(model.shape_material_ke)
+        d.shape_material_mu = p(model.shape_material_mu)
+        d.shape_collision_radius = p(model.shape_collision_radius)
+        d.filter_pairs = p(model.shape_collision_filter_pairs) or dummy_i.data_ptr()
+        g = model.gravity.view(-1).cpu().tolist()
+        d.gravity = (C.c_float * 3)(*g)
+        self.desc = d
+        h = C.c_void_p()
+        _lib.check(L.na_model_create(C.byref(d), C.byref(h)), "na_model_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None) is not None and self.handle.value:
+                _lib.lib().na_model_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def model_handle(model) -> ModelHandle:
+    h = getattr(model, "_na_handle", None)
+    if h is None:
+        h = ModelHandle(model)
+        model._na_handle = h
+    return h

@@ -1,0 +1,143 @@
+"""Seeded synthetic scenes for the step-doubling rigid-body workloads.
+
+World ``w`` draws from ``numpy.random.default_rng(seed + w)`` and orientations are
+Shoemake-uniform quaternions, as in the reference scene script
+``scripts/scenes/contact_objects.py:136-185``.  The bin walls follow
+``contact_objects.py:81-100`` (static boxes in world ``-1``).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .builder import ModelBuilder
+from .model import GeoType
+
+
+def random_unit_quaternion(rng) -> tuple:
+    """Shoemake uniform rotation, ``(x, y, z, w)`` (``contact_objects.py:136``)."""
+    u1, u2, u3 = rng.random(), rng.random(), rng.random()
+    s1 = math.sqrt(1.0 - u1)
+    s2 = math.sqrt(u1)
+    a1 = 2.0 * math.pi * u2
+    a2 = 2.0 * math.pi * u3
+    return (s1 * math.sin(a1), s1 * math.cos(a1), s2 * math.sin(a2), s2 * math.cos(a2))
+
+
+def _add_bin(builder: ModelBuilder, half_inner: float, cfg, wall_half_height: float = 0.5,
+             thickness: float = 0.025) -> None:
+    for px, py, hx, hy in [
+        (-(half_inner + thickness), 0.0, thickness, half_inner + thickness),
+        (half_inner + thickness, 0.0, thickness, half_inner + thickness),
+        (0.0, -(half_inner + thickness), half_inner + thickness, thickness),
+        (0.0, half_inner + thickness, half_inner + thickness, thickness),
+    ]:
+        builder.add_shape_box(-1, xform=(px, py, wall_half_height, 0.0, 0.0, 0.0, 1.0), hx=hx, hy=hy,
+                              hz=wall_half_height, cfg=cfg)
+
+
+def box_drop(world_count: int = 64, seed: int = 0, device="cuda:0", ke: float = 1.0e4, mu: float = 0.5):
+    """One 0.1 m, 1 kg box per world dropped from z ~ U[0.3, 0.8] with a random orientation
+    onto a ground plane (friction 0.5)."""
+    builder = ModelBuilder()
+    cfg = ModelBuilder.ShapeConfig(density=1000.0, ke=ke, mu=mu)
+    for w in range(world_count):
+        rng = np.random.default_rng(seed + w)
+        builder.begin_world()
+        z = rng.uniform(0.3, 0.8)
+        q = random_unit_quaternion(rng)
+        b = builder.add_body(xform=(0.0, 0.0, z, *q))
+        builder.add_shape_box(b, hx=0.05, hy=0.05, hz=0.05, cfg=cfg)
+        builder.end_world()
+    builder.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=ke, mu=mu))
+    return builder.finalize(device)
+
+
+def box_stack(world_count: int = 1024, seed: int = 0, device="cuda:0", ke: float = 1.0e6, layers: int = 4):
+    """Stack of ``layers`` boxes per world: edge ~ U[0.05, 0.15] m, density 1000,
+    stiffness ``ke``, one friction coefficient ~ U[0.3, 1.0] per world.  Boxes start
+    1 mm apart, centred with a small seeded offset and yaw."""
+    builder = ModelBuilder()
+    for w in range(world_count):
+        rng = np.random.default_rng(seed + w)
+        mu = rng.uniform(0.3, 1.0)
+        cfg = ModelBuilder.ShapeConfig(density=1000.0, ke=ke, mu=mu)
+        builder.begin_world()
+        z = 0.0
+        for _ in range(layers):
+            edge = rng.uniform(0.05, 0.15)
+            h = 0.5 * edge
+            z += h + 1.0e-3
+            dx, dy = rng.uniform(-0.1, 0.1, size=2) * h
+            yaw = rng.uniform(-0.3, 0.3)
+            q = (0.0, 0.0, math.sin(0.5 * yaw), math.cos(0.5 * yaw))
+            b = builder.add_body(xform=(dx, dy, z, *q))
+            builder.add_shape_box(b, hx=h, hy=h, hz=h, cfg=cfg)
+            z += h
+        builder.end_world()
+    builder.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=ke, mu=0.8))
+    return builder.finalize(device)
+
+
+SIZE_MEDIAN = 0.08
+SIZE_SIGMA = 0.4
+
+
+def _add_random_primitive(builder: ModelBuilder, rng, cfg, placed: list):
+    """Add one random primitive; returns nothing.  Sizes are lognormal(ln 0.08, 0.4) clipped
+    to +-2 sigma so that a drop position can always be found inside the bin."""
+    kind = int(rng.integers(0, 3))
+    s = float(np.clip(rng.lognormal(math.log(SIZE_MEDIAN), SIZE_SIGMA),
+                      SIZE_MEDIAN * math.exp(-2 * SIZE_SIGMA), SIZE_MEDIAN * math.exp(2 * SIZE_SIGMA)))
+    q = random_unit_quaternion(rng)
+    if kind == 0:
+        scale, bound = (0.5 * s,), 0.5 * s
+    elif kind == 1:
+        scale = tuple(0.5 * s * rng.uniform(0.6, 1.0, size=3))
+        bound = float(np.linalg.norm(scale))
+    else:
+        scale, bound = (0.3 * s, 0.5 * s), 0.8 * s
+    x, y = rng.uniform(-0.45 + bound, 0.45 - bound, size=2) if bound < 0.45 else (0.0, 0.0)
+    # drop height: clear every bounding sphere already placed above this spot
+    z = bound + 0.02
+    for (px, py, pz, pr) in placed:
+        if (px - x) ** 2 + (py - y) ** 2 < (pr + bound) ** 2:
+            z = max(z, pz + pr + bound + 0.005)
+    placed.append((x, y, z, bound))
+    b = builder.add_body(xform=(x, y, z, *q))
+    if kind == 0:
+        builder.add_shape_sphere(b, radius=scale[0], cfg=cfg)
+    elif kind == 1:
+        builder.add_shape_box(b, hx=scale[0], hy=scale[1], hz=scale[2], cfg=cfg)
+    else:
+        builder.add_shape_capsule(b, radius=scale[0], half_height=scale[1], cfg=cfg)
+
+
+def uneven_pile(world_count: int = 4096, seed: int = 0, device="cuda:0", max_prims: int = 32,
+                min_prims: int = 1, ke: float = 1.0e4, counts=None):
+    """Uneven worlds: N ~ U{min_prims..max_prims} primitives from {sphere, box, capsule},
+    size ~ lognormal(ln 0.08 m, 0.4), dropped as a random pile into a 1 m^2 walled bin.
+    Drop heights are chosen so that no two bounding spheres start overlapping."""
+    builder = ModelBuilder()
+    for w in range(world_count):
+        rng = np.random.default_rng(seed + w)
+        n = int(counts[w]) if counts is not None else int(rng.integers(min_prims, max_prims + 1))
+        mu = rng.uniform(0.3, 1.0)
+        cfg = ModelBuilder.ShapeConfig(density=1000.0, ke=ke, mu=mu)
+        builder.begin_world()
+        placed: list = []
+        for _ in range(n):
+            _add_random_primitive(builder, rng, cfg, placed)
+        builder.end_world()
+    ground = ModelBuilder.ShapeConfig(ke=ke, mu=0.8)
+    builder.add_ground_plane(cfg=ground)
+    _add_bin(builder, 0.5, ModelBuilder.ShapeConfig(ke=ke, mu=0.5))
+    return builder.finalize(device)
+
+
+def shape_kinds(model) -> np.ndarray:
+    """Per-world histogram of body shape types (diagnostic)."""
+    st = model.host("shape_type")
+    return np.bincount(st, minlength=GeoType.BOX + 1)

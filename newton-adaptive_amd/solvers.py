@@ -1,0 +1,337 @@
+"""Per-world step-doubling solvers with the reference's Python surface.
+
+``SolverSAPAdaptive`` mirrors ``newton/_src/solvers/sap/solver_sap_adaptive.py:437-1001``
+and ``SolverMuJoCoAdaptive`` mirrors ``newton/_src/solvers/mujoco/solver_mujoco_adaptive.py:418-1042``
+(constructor kwargs, ``step`` / ``step_dt`` / ``reset``, telemetry properties, ``ValueError``
+on invalid settings, ``RuntimeError`` when the inner convex solve does not converge).  Both
+run the same gfx950 path in ``libnadapt.so``: contact generation, the convex compliant
+contact solve and step doubling per world inside one wavefront; they differ only in the
+controller (``_adapt_dt`` with fixed/adaptive modes vs ``_calc_adjusted_step`` with the
+separate commit mask and floor-divergence latch).
+
+The controller runs in FP64 on the device; ``dt`` / ``sim_time`` / errors are float64
+tensors.  Body poses accumulate in an FP64 shadow (``body_q64``) that is kept across
+frames while the caller's float32 ``body_q`` still equals its rounding.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from . import _lib
+from .contacts import BROAD_PHASES, CollisionPipeline
+from .native import model_handle
+
+_MODE_CODES = {"fixed": 0, "adaptive": 1}
+_LINE_SEARCHES = ("exact_root", "armijo_decay")
+_WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves")
+
+
+class _SolverBase:
+    _controller = 0
+
+    def __init__(self, model, *, mode="adaptive", tol=1e-3, dt_inner_init=0.01, dt_inner_min=1e-6,
+                 dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=30,
+                 contact_preset_variant="drake", line_search_variant="exact_root", contact_tau_d=0.01,
+                 broad_phase="sap", optimality_rel_tol=None, schedule="served", **_kw):
+        if mode not in _MODE_CODES:
+            raise ValueError(f"mode must be one of {tuple(_MODE_CODES)}, got {mode!r}.")
+        if float(tol) <= 0.0:
+            raise ValueError(f"tol must be > 0, got {tol!r}.")
+        if float(dt_inner_init) <= 0.0:
+            raise ValueError(f"dt_inner_init must be > 0, got {dt_inner_init!r}.")
+        if float(dt_inner_min) <= 0.0:
+            raise ValueError(f"dt_inner_min must be > 0, got {dt_inner_min!r}.")
+        if dt_inner_max is not None and float(dt_inner_max) <= 0.0:
+            raise ValueError(f"dt_inner_max must be > 0 when provided, got {dt_inner_max!r}.")
+        if int(max_substeps) < 1:
+            raise ValueError(f"max_substeps must be >= 1, got {max_substeps!r}.")
+        if not 1 <= int(max_rigid_contact) <= 128:
+            raise ValueError(f"max_rigid_contact must be in [1, 128] per world, got {max_rigid_contact!r}.")
+        if contact_preset_variant != "drake":
+            raise ValueError(f"contact_preset_variant must be 'drake', got {contact_preset_variant!r}.")
+        if line_search_variant not in _LINE_SEARCHES:
+            raise ValueError(f"line_search_variant must be one of {_LINE_SEARCHES}, got {line_search_variant!r}.")
+        if broad_phase not in BROAD_PHASES:
+            raise ValueError(f"broad_phase must be one of {tuple(BROAD_PHASES)}, got {broad_phase!r}.")
+        if schedule not in ("served", "plain"):
+            raise ValueError(f"schedule must be 'served' or 'plain', got {schedule!r}.")
+        self.model = model
+        self._mode = mode
+        self._tol = float(tol)
+        self._dt_inner_init = float(dt_inner_init)
+        self._dt_min = float(dt_inner_min)
+        self._dt_max = float(dt_inner_max) if dt_inner_max is not None else math.inf
+        self._max_substeps = int(max_substeps)
+        self._max_rigid_contact = int(max_rigid_contact)
+        self._schedule = schedule
+        self._world_count = int(model.world_count)
+        W, B, dev = self._world_count, model.body_count, model.device
+
+        cfg = _lib.NaConfig()
+        cfg.mode = _MODE_CODES[mode]
+        cfg.controller = self._controller
+        cfg.tol = self._tol
+        cfg.dt_inner_init = self._dt_inner_init
+        cfg.dt_inner_min = self._dt_min
+        cfg.dt_inner_max = self._dt_max if math.isfinite(self._dt_max) else 1.0e30
+        cfg.divergence_threshold = 1.0e9
+        cfg.max_substeps = self._max_substeps
+        cfg.max_iterations = int(max_iterations)
+        cfg.max_rigid_contact = self._max_rigid_contact
+        cfg.broad_phase = BROAD_PHASES[broad_phase]
+        # CENIC kappa*tol (solver_sap_adaptive.py:485-501) floored at what FP32 residuals resolve
+        self.optimality_rel_tol = float(optimality_rel_tol) if optimality_rel_tol is not None \
+            else max(1.0e-3 * self._tol, 1.0e-6)
+        cfg.optimality_rel_tol = self.optimality_rel_tol
+        cfg.optimality_abs_tol = 1.0e-12
+        cfg.contact_tau_d = float(contact_tau_d)
+        cfg.sigma = 1.0e-3
+        cfg.beta = 1.0
+        cfg.ls_rel_tol = 1.0e-4
+        cfg.cg_rel_tol = 1.0e-4
+        cfg.max_cg = 200
+        self._cfg = cfg
+
+        f64 = dict(dtype=torch.float64, device=dev)
+        self._dt = torch.full((W,), self._dt_inner_init, **f64)
+        self._dt_half = torch.full((W,), 0.5 * self._dt_inner_init, **f64)
+        self._ideal_dt = torch.full((W,), self._dt_inner_init, **f64)
+        self._sim_time = torch.zeros(W, **f64)
+        self._next_time = torch.zeros(W, **f64)
+        self._last_error = torch.zeros(W, **f64)
+        self._accepted_error = torch.zeros(W, **f64)
+        self._accepted = torch.zeros(W, dtype=torch.bool, device=dev)
+        self._commit = torch.zeros(W, dtype=torch.bool, device=dev)
+        self._diverged = torch.zeros(W, dtype=torch.bool, device=dev)
+        self._substeps_frame = torch.zeros(W, dtype=torch.int32, device=dev)
+        self._iterations = torch.zeros(W, dtype=torch.int32, device=dev)
+        self._solve_fail = torch.zeros(W, dtype=torch.int32, device=dev)
+        self._overflow = torch.zeros(W, dtype=torch.int32, device=dev)
+        self._cum_accepted = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._work = torch.zeros((W, len(_WORK_FIELDS)), dtype=torch.int64, device=dev)
+        self._q64 = torch.zeros((max(B, 1), 7), **f64)
+        self._full_world_mask = torch.ones(W, dtype=torch.bool, device=dev)
+
+        t = _lib.NaTelemetry()
+        t.dt, t.dt_half, t.ideal_dt = self._dt.data_ptr(), self._dt_half.data_ptr(), self._ideal_dt.data_ptr()
+        t.sim_time, t.next_time = self._sim_time.data_ptr(), self._next_time.data_ptr()
+        t.last_raw_error, t.last_error = self._last_error.data_ptr(), self._accepted_error.data_ptr()
+        t.accepted, t.commit, t.diverged = (self._accepted.data_ptr(), self._commit.data_ptr(),
+                                            self._diverged.data_ptr())
+        t.substeps, t.iterations = self._substeps_frame.data_ptr(), self._iterations.data_ptr()
+        t.solve_fail, t.overflow = self._solve_fail.data_ptr(), self._overflow.data_ptr()
+        t.cum_accepted, t.work, t.body_q64 = (self._cum_accepted.data_ptr(), self._work.data_ptr(),
+                                              self._q64.data_ptr())
+        self._handle = model_handle(model)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().na_solver_create(self._handle.handle, C.byref(cfg), C.byref(t), C.byref(h)),
+                   "na_solver_create")
+        self._h = h
+        self._order = None
+        self._pipeline = None
+        self._contacts = None
+        self.last_plain_iterations = 0
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                _lib.lib().na_solver_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ schedule
+    def set_schedule(self, grid_waves: int = 0, order: torch.Tensor | None = None) -> None:
+        """Persistent grid size (0: one wave per resident slot) and world visiting order."""
+        if order is not None:
+            order = order.to(device=self.model.device, dtype=torch.int32).contiguous()
+            if order.numel() != self._world_count:
+                raise ValueError("order must be a permutation of the worlds")
+        self._order = order
+        _lib.check(_lib.lib().na_solver_set_schedule(self._h, int(grid_waves),
+                                                     order.data_ptr() if order is not None else None),
+                   "na_solver_set_schedule")
+
+    # ------------------------------------------------------------------ boundary call
+    def integrate(self, state, control, dt_outer: float, stream=None):
+        s = _lib.stream_handle(stream)
+        bf = state.body_f if getattr(state, "body_f", None) is not None else None
+        if self._schedule == "served":
+            st = _lib.check(_lib.lib().na_step(self._h, state.body_q.data_ptr(), state.body_qd.data_ptr(),
+                                               bf.data_ptr() if bf is not None else None, float(dt_outer), s),
+                            "na_step")
+            # the served launch ends with per-world solve status on the device
+            if int(self._solve_fail.max().item()) != 0:
+                self._raise_solve_failure()
+        else:
+            it = C.c_int32(0)
+            st = _lib.check(_lib.lib().na_step_plain(self._h, state.body_q.data_ptr(), state.body_qd.data_ptr(),
+                                                     bf.data_ptr() if bf is not None else None, float(dt_outer),
+                                                     C.byref(it), s), "na_step_plain")
+            self.last_plain_iterations = it.value
+            if st == 2:
+                self._raise_solve_failure()
+        return state
+
+    def _raise_solve_failure(self):
+        raise RuntimeError(
+            f"{type(self).__name__} inner convex solve failed to converge to "
+            f"optimality_rel_tol={self.optimality_rel_tol:.3e}.")
+
+    def step(self, state_in, state_out, control, contacts=None, dt=None, apply_forces=None):
+        """Advance every world by ``dt`` (= ``dt_outer``) in place; returns ``(state_in, state_out)``
+        (``solver_sap_adaptive.py:962-975``)."""
+        if dt is None:
+            raise ValueError(f"{type(self).__name__}.step requires dt (the outer boundary period).")
+        if apply_forces is not None:
+            apply_forces(state_in)
+        self.integrate(state_in, control, float(dt))
+        return state_in, state_out
+
+    def step_dt(self, dt_outer, state_0, state_1, control, apply_forces=None):
+        """Legacy ``(dt, s0, s1, control)`` order (``solver_sap_adaptive.py:977-979``)."""
+        return self.step(state_0, state_1, control, None, dt_outer, apply_forces=apply_forces)
+
+    def reset(self, state=None, world_mask: torch.Tensor | None = None, flags=0) -> None:
+        """Restore per-world controller state for masked worlds (``_reset_worlds``)."""
+        mask = None
+        if world_mask is not None:
+            mask = world_mask.to(device=self.model.device, dtype=torch.uint8).contiguous()
+            if mask.numel() != self._world_count:
+                raise ValueError("world_mask must have one entry per world")
+        _lib.check(_lib.lib().na_reset(self._h, mask.data_ptr() if mask is not None else None,
+                                       _lib.stream_handle()), "na_reset")
+
+    # ------------------------------------------------------------------ telemetry
+    @property
+    def diverged(self):
+        return self._diverged
+
+    @property
+    def dt(self):
+        return self._dt
+
+    @property
+    def sim_time(self):
+        return self._sim_time
+
+    @property
+    def last_error(self):
+        return self._accepted_error
+
+    @property
+    def last_raw_error(self):
+        return self._last_error
+
+    @property
+    def accepted(self):
+        return self._accepted
+
+    @property
+    def substeps(self):
+        return self._substeps_frame
+
+    @property
+    def iteration_count(self):
+        return self._iterations.max().view(1)
+
+    @property
+    def mode(self) -> str:
+        return self._mode
+
+    @property
+    def tiling(self) -> str:
+        return "adaptive"
+
+    @property
+    def body_q64(self):
+        return self._q64
+
+    @property
+    def contacts(self):
+        """Contacts at the working state (generated on request; the step keeps its own in LDS)."""
+        return self._contacts
+
+    def collide(self, state):
+        if self._pipeline is None:
+            self._pipeline = CollisionPipeline(self.model, rigid_contact_max_per_world=self._max_rigid_contact)
+            self._contacts = self._pipeline.contacts()
+        return self._pipeline.collide(state, self._contacts)
+
+    def work_counters(self) -> dict:
+        w = self._work.sum(dim=0).tolist()
+        return dict(zip(_WORK_FIELDS, w))
+
+    def reset_work_counters(self) -> None:
+        self._work.zero_()
+
+    def cumulative_substeps(self) -> int:
+        """Accepted inner steps times the 3 evaluations of one step-doubling attempt."""
+        return int(self._cum_accepted.item()) * 3
+
+    def accepted_steps(self) -> int:
+        return int(self._cum_accepted.item())
+
+    def reset_compute_counter(self) -> None:
+        self._cum_accepted.zero_()
+
+    def get_max_contact_count(self) -> int:
+        return self._max_rigid_contact * self._world_count
+
+    def get_status_summary(self) -> dict:
+        """Six-scalar summary (``solver_mujoco_adaptive.py:1021-1042``)."""
+        return {
+            "sim_time_min": float(self._sim_time.min()),
+            "sim_time_max": float(self._sim_time.max()),
+            "error_max": float(self._accepted_error.max()),
+            "accept_count": int(self._accepted.sum()),
+            "dt_min": float(self._dt.min()),
+            "dt_max": float(self._dt.max()),
+        }
+
+    def update_contacts(self, contacts, state=None) -> None:
+        return None
+
+    def notify_model_changed(self, flags: int) -> None:
+        return None
+
+
+class SolverSAPAdaptive(_SolverBase):
+    """Per-world adaptive (step-doubling) convex-contact integrator (``solver_sap_adaptive.py:437``)."""
+
+    _controller = 0
+
+
+class SolverMuJoCoAdaptive(_SolverBase):
+    """Adaptive solver with the MuJoCo-variant controller (``solver_mujoco_adaptive.py:418``):
+    ``accepted`` advances time, ``commit`` writes the doubled state, a floor-diverged world holds
+    its last good state and is latched in ``diverged``."""
+
+    _controller = 1
+
+    def __init__(self, model, *, tol=1e-3, dt_inner_init=0.01, dt_inner_min=1e-6, dt_inner_max=None,
+                 dt_mode="per_world", tiling="ragged", max_substeps=256, **kwargs):
+        if dt_mode != "per_world":
+            raise ValueError(f"dt_mode must be 'per_world' (the only supported mode), got {dt_mode!r}")
+        if tiling != "ragged":
+            raise ValueError(f"tiling must be 'ragged' (the only supported mode), got {tiling!r}")
+        kwargs.pop("mode", None)
+        super().__init__(model, mode="adaptive", tol=tol, dt_inner_init=dt_inner_init, dt_inner_min=dt_inner_min,
+                         dt_inner_max=dt_inner_max, max_substeps=max_substeps, **kwargs)
+        self._tiling = tiling
+
+    @property
+    def tiling(self) -> str:
+        return "ragged"
+
+    @property
+    def commit(self):
+        return self._commit
+
+    @property
+    def cumulative_iterations(self):
+        return self._iterations.sum().view(1)

@@ -1,0 +1,1420 @@
+/*
+ * CPU oracle for the batched step-doubling rigid-body step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product path never links or calls it.
+ *
+ * It restates, in plain C over REAL (double for the oracle build, float for the
+ * CPU-baseline port), the reference algorithm of mardigiorgio/newton-adaptive:
+ *
+ *   controller   newton/_src/solvers/mujoco/solver_mujoco_adaptive.py:63-415
+ *                newton/_src/solvers/sap/solver_sap_adaptive.py:89-434,732-849,877-960
+ *   AABBs        newton/_src/sim/collide.py:150-275
+ *   broad phase  newton/_src/geometry/broad_phase_common.py:18-180 (brute-force n^2 set,
+ *                the semantics of newton/tests/test_broad_phase.py:90-148)
+ *   narrow phase newton/_src/geometry/narrow_phase.py:154-604,
+ *                newton/_src/geometry/collision_primitive.py:49-1215
+ *   contacts     newton/_src/sim/collide.py:62-147, geometry/contact_data.py:59-117
+ *   integrator   newton/_src/solvers/solver.py:50-94
+ *   inner solve  the convex SAP problem the reference solves through the absent
+ *                external sap_warp (call sites solver_sap_adaptive.py:489-501,714-729),
+ *                restated from the published SAP formulation (Castro et al. 2022) and
+ *                CENIC (Kurtz & Castro, arXiv:2511.08771): primal Newton on
+ *                l(v) = 1/2|v-v*|_A^2 + sum_i 1/2 |P_F(y_i)|_R^2, dense Cholesky,
+ *                exact line search seeded by the cubic Hermite minimiser (N&W 3.59).
+ *
+ * Box-box and capsule-box, which the reference routes to GJK/MPR + polygon
+ * clipping (narrow_phase.py:629-779), are restated here by this project's own
+ * SAT + face clipping and segment/box signed-distance minimisation (the same
+ * algorithm the HIP path implements).
+ *
+ * Pinning (see tests/): controller KATs from test_adaptive_floor_nan_guard.py,
+ * test_adaptive_time_rebase.py, test_adaptive_reset_worlds.py; cubic KATs from
+ * test_adaptive_cubic_linesearch.py; primitive closed-form tables from
+ * test_collision_primitives.py; brute-force pair sets; solver optimality and
+ * invariants (free fall closed form, momentum, energy bound, rest).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef OR_FLOAT
+typedef float real;
+#define RS(x) sqrtf(x)
+#define RA(x) fabsf(x)
+#else
+typedef double real;
+#define RS(x) sqrt(x)
+#define RA(x) fabs(x)
+#endif
+
+#define OR_MAXVAL 1.0e10
+#define OR_PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------ vectors */
+typedef struct { real x, y, z; } v3;
+typedef struct { real x, y, z, w; } qt;
+
+static inline v3 V(real x, real y, real z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(v3 a, real s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline real dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline real len(v3 a) { return RS(dot(a, a)); }
+static inline real comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline void setc(v3* a, int i, real s) { if (i == 0) a->x = s; else if (i == 1) a->y = s; else a->z = s; }
+static inline v3 nrm(v3 a) { real l = len(a); return l > 0 ? scl(a, (real)1 / l) : a; }
+static inline real clampr(real x, real lo, real hi) { return x < lo ? lo : (x > hi ? hi : x); }
+static inline real maxr(real a, real b) { return a > b ? a : b; }
+static inline real minr(real a, real b) { return a < b ? a : b; }
+
+/* rotation matrix (row-major) of quaternion (x, y, z, w) */
+typedef struct { real m[9]; } m33;
+static m33 qmat(qt q) {
+  m33 r;
+  real x = q.x, y = q.y, z = q.z, w = q.w;
+  r.m[0] = 1 - 2 * (y * y + z * z); r.m[1] = 2 * (x * y - z * w);     r.m[2] = 2 * (x * z + y * w);
+  r.m[3] = 2 * (x * y + z * w);     r.m[4] = 1 - 2 * (x * x + z * z); r.m[5] = 2 * (y * z - x * w);
+  r.m[6] = 2 * (x * z - y * w);     r.m[7] = 2 * (y * z + x * w);     r.m[8] = 1 - 2 * (x * x + y * y);
+  return r;
+}
+static inline v3 mv(const m33* a, v3 v) {
+  return V(a->m[0] * v.x + a->m[1] * v.y + a->m[2] * v.z, a->m[3] * v.x + a->m[4] * v.y + a->m[5] * v.z,
+           a->m[6] * v.x + a->m[7] * v.y + a->m[8] * v.z);
+}
+static inline v3 mtv(const m33* a, v3 v) {
+  return V(a->m[0] * v.x + a->m[3] * v.y + a->m[6] * v.z, a->m[1] * v.x + a->m[4] * v.y + a->m[7] * v.z,
+           a->m[2] * v.x + a->m[5] * v.y + a->m[8] * v.z);
+}
+static inline v3 col(const m33* a, int j) { return V(a->m[j], a->m[3 + j], a->m[6 + j]); }
+static qt qmul(qt a, qt b) {
+  qt r;
+  r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+  r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+  r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+  r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+  return r;
+}
+static inline v3 qrot(qt q, v3 v) { m33 r = qmat(q); return mv(&r, v); }
+
+/* ------------------------------------------------------------------ model */
+typedef struct or_model {
+  int world_count, body_count, shape_count, global_count;
+  const int32_t *world_body_start, *world_shape_start, *global_shapes;
+  const double *body_com, *body_mass, *body_inv_mass, *body_inertia, *body_inv_inertia;
+  const int32_t *shape_type, *shape_body, *shape_group, *shape_world;
+  const double *shape_xform, *shape_scale, *shape_margin, *shape_gap, *shape_ke, *shape_mu, *shape_radius;
+  int filter_count;
+  const int32_t* filter_pairs; /* sorted canonical (min, max) */
+  double gravity[3];
+} or_model;
+
+typedef struct or_config {
+  int mode;            /* 0 fixed, 1 adaptive (solver_sap_adaptive.py:64-65) */
+  int controller;      /* 0 SAP _adapt_dt, 1 MuJoCo _calc_adjusted_step */
+  double tol, dt_init, dt_min, dt_max, divergence_threshold;
+  int max_substeps, max_iterations, max_contacts;
+  double opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol;
+} or_config;
+
+enum { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
+
+/* ====================================================================
+ * Controller, float32 semantics exactly as the reference kernels.
+ * ==================================================================== */
+#define DRAKE_SAFETY 0.9
+#define DRAKE_MIN_SHRINK 0.1
+#define DRAKE_MAX_GROW 5.0
+#define DRAKE_HYST_HIGH 1.2
+#define DRAKE_HYST_LOW 0.9
+
+/* _calc_adjusted_step, solver_mujoco_adaptive.py:120-198 */
+#define DEFINE_CALC_ADJUSTED(T, SFX, SQ)                                                                   \
+  void or_calc_adjusted_step_##SFX(int n, const T* err, const T* dt, T* ideal_dt, uint8_t* accepted,      \
+                                   uint8_t* commit, uint8_t* diverged, T tol, T dt_min, T div_thr) {       \
+    for (int w = 0; w < n; ++w) {                                                                          \
+      T e = err[w], step = dt[w];                                                                          \
+      int is_div = isnan(e) || isinf(e) || e >= div_thr;                                                  \
+      if (step <= (T)0) { accepted[w] = 1; commit[w] = 1; continue; }                                     \
+      if (step <= dt_min * (T)1.001) {                                                                     \
+        if (is_div) { accepted[w] = 1; commit[w] = 0; diverged[w] = 1; ideal_dt[w] = dt_min; continue; }  \
+        if (e > tol) { accepted[w] = 1; commit[w] = 1; ideal_dt[w] = dt_min; continue; }                  \
+      }                                                                                                    \
+      if (is_div) { accepted[w] = 0; commit[w] = 0; ideal_dt[w] = (T)DRAKE_MIN_SHRINK * step; continue; } \
+      T ee = e > (T)1.0e-30 ? e : (T)1.0e-30;                                                              \
+      T ns = (T)DRAKE_SAFETY * step * SQ(tol / ee);                                                        \
+      if (ns > (T)DRAKE_HYST_LOW * step && ns < (T)DRAKE_HYST_HIGH * step) ns = step;                     \
+      T lo = (T)DRAKE_MIN_SHRINK * step, hi = (T)DRAKE_MAX_GROW * step;                                    \
+      ns = ns < lo ? lo : (ns > hi ? hi : ns);                                                             \
+      int acc = (e <= tol) || (ns >= step);                                                                \
+      accepted[w] = (uint8_t)acc; commit[w] = (uint8_t)acc; ideal_dt[w] = ns;                              \
+    }                                                                                                      \
+  }
+DEFINE_CALC_ADJUSTED(float, f32, sqrtf)
+DEFINE_CALC_ADJUSTED(double, f64, sqrt)
+
+/* _adapt_dt, solver_sap_adaptive.py:237-345 */
+#define DEFINE_ADAPT_DT(T, SFX, SQ)                                                                         \
+  void or_adapt_dt_##SFX(int n, const T* err, T* sim_time, const T* next_time, T* dt, T* dt_half,          \
+                         T* ideal_dt, uint8_t* diverged, uint8_t* accept, T* accepted_error,              \
+                         int32_t* substeps, int32_t* cum_accepted, int mode, T tol, T dt_min, T dt_max,    \
+                         T div_thr) {                                                                      \
+    for (int w = 0; w < n; ++w) {                                                                          \
+      T step = dt[w];                                                                                      \
+      if (sim_time[w] >= next_time[w] || step <= (T)0) { accept[w] = 0; continue; }                       \
+      T e = err[w];                                                                                        \
+      int is_div = isnan(e) || isinf(e) || e >= div_thr;                                                  \
+      if (mode == 0) {                                                                                     \
+        if (is_div) { accept[w] = 0; sim_time[w] = next_time[w]; diverged[w] = 1; continue; }            \
+        accept[w] = 1; sim_time[w] = sim_time[w] + step; accepted_error[w] = e; substeps[w] += 1;          \
+        cum_accepted[0] += 1; continue;                                                                    \
+      }                                                                                                    \
+      T ee = e > (T)1.0e-30 ? e : (T)1.0e-30;                                                              \
+      if (step <= dt_min * (T)1.001) {                                                                     \
+        if (is_div) { accept[w] = 0; sim_time[w] = next_time[w]; diverged[w] = 1; ideal_dt[w] = dt_min;   \
+                      continue; }                                                                          \
+        accept[w] = 1; sim_time[w] = sim_time[w] + step; accepted_error[w] = e; substeps[w] += 1;          \
+        cum_accepted[0] += 1;                                                                              \
+        T ns = (T)DRAKE_SAFETY * step * SQ(tol / ee);                                                      \
+        if (ns > (T)DRAKE_HYST_LOW * step && ns < (T)DRAKE_HYST_HIGH * step) ns = step;                   \
+        T lo = (T)DRAKE_MIN_SHRINK * step, hi = (T)DRAKE_MAX_GROW * step;                                  \
+        ideal_dt[w] = ns < lo ? lo : (ns > hi ? hi : ns);                                                  \
+        continue;                                                                                          \
+      }                                                                                                    \
+      if (is_div) {                                                                                        \
+        accept[w] = 0; T ns = (T)DRAKE_MIN_SHRINK * step; ideal_dt[w] = ns;                                \
+        T d = ns < dt_min ? dt_min : (ns > dt_max ? dt_max : ns); dt[w] = d; dt_half[w] = d * (T)0.5;     \
+        continue;                                                                                          \
+      }                                                                                                    \
+      T ns = (T)DRAKE_SAFETY * step * SQ(tol / ee);                                                        \
+      if (ns > (T)DRAKE_HYST_LOW * step && ns < (T)DRAKE_HYST_HIGH * step) ns = step;                     \
+      T lo = (T)DRAKE_MIN_SHRINK * step, hi = (T)DRAKE_MAX_GROW * step;                                    \
+      ns = ns < lo ? lo : (ns > hi ? hi : ns);                                                             \
+      int acc = (e <= tol) || (ns >= step);                                                                \
+      T d = ns < dt_min ? dt_min : (ns > dt_max ? dt_max : ns);                                            \
+      if (acc) { accept[w] = 1; sim_time[w] = sim_time[w] + step; accepted_error[w] = e; substeps[w] += 1; \
+                 cum_accepted[0] += 1; }                                                                   \
+      else accept[w] = 0;                                                                                  \
+      ideal_dt[w] = ns; dt[w] = d; dt_half[w] = d * (T)0.5;                                                \
+    }                                                                                                      \
+  }
+DEFINE_ADAPT_DT(float, f32, sqrtf)
+DEFINE_ADAPT_DT(double, f64, sqrt)
+
+/* _rebase_time, solver_mujoco_adaptive.py:340-358 */
+void or_rebase_time_f32(int n, float* sim_time, float* next_time) {
+  for (int i = 0; i < n; ++i) { float b = next_time[i]; sim_time[i] = sim_time[i] - b; next_time[i] = next_time[i] - b; }
+}
+/* _reset_worlds, solver_mujoco_adaptive.py:234-261 */
+void or_reset_worlds_f32(int n, const uint8_t* mask, float dt_init, float* ideal_dt, float* dt, float* dt_half,
+                         float* sim_time, float* next_time, uint8_t* diverged, uint8_t* accepted) {
+  for (int i = 0; i < n; ++i) {
+    if (!mask[i]) continue;
+    ideal_dt[i] = dt_init; dt[i] = dt_init; dt_half[i] = dt_init * 0.5f;
+    sim_time[i] = 0.0f; next_time[i] = 0.0f; diverged[i] = 0; accepted[i] = 0;
+  }
+}
+/* _clamp_dt_to_boundary, solver_mujoco_adaptive.py:361-378 */
+void or_clamp_dt_to_boundary_f32(int n, float* dt, float* dt_half, const float* sim_time, const float* next_time) {
+  for (int i = 0; i < n; ++i) {
+    float rem = next_time[i] - sim_time[i];
+    if (rem <= 0.0f) { dt[i] = 0.0f; dt_half[i] = 0.0f; }
+    else if (dt[i] > rem) { dt[i] = rem; dt_half[i] = rem * 0.5f; }
+  }
+}
+/* _inf_norm_state_error_kernel, solver_mujoco_adaptive.py:78-108 */
+void or_inf_norm_error_f32(int n, int coords, const float* q_full, const float* q_double, const float* scale,
+                           float* err) {
+  for (int w = 0; w < n; ++w) {
+    float m = 0.0f;
+    for (int i = 0; i < coords; ++i) {
+      float d = fabsf(q_double[w * coords + i] - q_full[w * coords + i]);
+      float s = scale[w * coords + i] * d;
+      m = m > s ? m : s; /* wp.max(max_err, x): NaN handling below */
+      if (isnan(s)) m = s;
+    }
+    if (isnan(m) || isinf(m)) m = 1.0e10f;
+    err[w] = m;
+  }
+}
+
+/* cubic Hermite minimiser on [0, h] (N&W eq. 3.59), test_adaptive_cubic_linesearch.py:36-60.
+ * Returns 1 and *alpha on success, 0 on the fallback conditions. */
+int or_cubic_min(double c0, double d0, double c1, double d1, double h, double* alpha) {
+  double theta = d0 + d1 - 3.0 * (c1 - c0) / h;
+  double disc = theta * theta - d0 * d1;
+  if (!isfinite(disc) || disc < 0.0) return 0;
+  double gamma = sqrt(disc);
+  double denom = d1 - d0 + 2.0 * gamma;
+  if (denom == 0.0) return 0;
+  double a = h - h * (d1 + gamma - theta) / denom;
+  if (!isfinite(a) || a <= 0.0 || a > h) return 0;
+  *alpha = a;
+  return 1;
+}
+
+/* ====================================================================
+ * Narrow-phase primitives (collision_primitive.py).  Conventions:
+ * normal from A into B, position = midpoint between surfaces,
+ * distance negative = penetration.
+ * ==================================================================== */
+typedef struct { int n; real dist[4]; v3 pos[4]; v3 normal[4]; } manifold;
+
+static void mf_init(manifold* m) { m->n = 0; for (int i = 0; i < 4; ++i) m->dist[i] = (real)OR_MAXVAL; }
+
+/* closest_segment_point, collision_primitive.py:49-54 */
+static v3 closest_segment_point(v3 a, v3 b, v3 pt) {
+  v3 ab = sub(b, a);
+  real t = dot(sub(pt, a), ab) / (dot(ab, ab) + (real)1e-6);
+  return add(a, scl(ab, clampr(t, 0, 1)));
+}
+
+/* collide_plane_sphere, :101-107 */
+static void collide_plane_sphere(v3 n, v3 ppos, v3 spos, real r, real* dist, v3* pos) {
+  real d = dot(sub(spos, ppos), n) - r;
+  *dist = d;
+  *pos = sub(spos, scl(n, r + (real)0.5 * d));
+}
+
+/* collide_sphere_sphere, :111-140 */
+static void collide_sphere_sphere(v3 p1, real r1, v3 p2, real r2, real* dist, v3* pos, v3* n) {
+  v3 dir = sub(p2, p1);
+  real d = len(dir);
+  v3 nn = d == 0 ? V(1, 0, 0) : scl(dir, (real)1 / d);
+  d = d - (r1 + r2);
+  *dist = d;
+  *pos = add(p1, scl(nn, r1 + (real)0.5 * d));
+  *n = nn;
+}
+
+/* collide_sphere_capsule, :144-176 */
+static void collide_sphere_capsule(v3 sp, real sr, v3 cp, v3 axis, real cr, real hl, real* dist, v3* pos, v3* n) {
+  v3 seg = scl(axis, hl);
+  v3 pt = closest_segment_point(sub(cp, seg), add(cp, seg), sp);
+  collide_sphere_sphere(sp, sr, pt, cr, dist, pos, n);
+}
+
+/* collide_capsule_capsule, :181-274 */
+static void collide_capsule_capsule(v3 p1, v3 a1, real r1, real h1, v3 p2, v3 a2, real r2, real h2, manifold* m) {
+  v3 axis1 = scl(a1, h1), axis2 = scl(a2, h2), dif = sub(p1, p2);
+  real ma = dot(axis1, axis1), mb = -dot(axis1, axis2), mc = dot(axis2, axis2);
+  real u = -dot(axis1, dif), v = dot(axis2, dif);
+  real det = ma * mc - mb * mb;
+  v3 nrm0;
+  if (RA(det) >= (real)1e-15) {
+    real inv = (real)1 / det;
+    real x1 = (mc * u - mb * v) * inv, x2 = (ma * v - mb * u) * inv;
+    if (x1 > 1) { x1 = 1; x2 = (v - mb) / mc; }
+    else if (x1 < -1) { x1 = -1; x2 = (v + mb) / mc; }
+    if (x2 > 1) { x2 = 1; x1 = clampr((u - mb) / ma, -1, 1); }
+    else if (x2 < -1) { x2 = -1; x1 = clampr((u + mb) / ma, -1, 1); }
+    v3 v1 = add(p1, scl(axis1, x1)), v2 = add(p2, scl(axis2, x2));
+    collide_sphere_sphere(v1, r1, v2, r2, &m->dist[0], &m->pos[0], &nrm0);
+    m->normal[0] = nrm0;
+    m->normal[1] = nrm0;
+  } else {
+    v3 v1 = add(p1, axis1);
+    real x2 = clampr((v - mb) / mc, -1, 1);
+    v3 v2 = add(p2, scl(axis2, x2));
+    collide_sphere_sphere(v1, r1, v2, r2, &m->dist[0], &m->pos[0], &nrm0);
+    v3 n1;
+    v1 = sub(p1, axis1);
+    x2 = clampr((v + mb) / mc, -1, 1);
+    v2 = add(p2, scl(axis2, x2));
+    collide_sphere_sphere(v1, r1, v2, r2, &m->dist[1], &m->pos[1], &n1);
+    m->normal[0] = nrm0;
+    m->normal[1] = nrm0;
+  }
+}
+
+/* collide_plane_capsule, :279-331 (contact frame unused by the writer) */
+static void collide_plane_capsule(v3 n, v3 ppos, v3 cpos, v3 axis, real r, real hl, manifold* m) {
+  v3 seg = scl(axis, hl);
+  collide_plane_sphere(n, ppos, add(cpos, seg), r, &m->dist[0], &m->pos[0]);
+  collide_plane_sphere(n, ppos, sub(cpos, seg), r, &m->dist[1], &m->pos[1]);
+  m->normal[0] = n; m->normal[1] = n;
+}
+
+/* collide_plane_box, :366-439: keep up to 4 deepest corners within `margin` */
+static void collide_plane_box(v3 n, v3 ppos, v3 bpos, const m33* rot, v3 size, real margin, manifold* m) {
+  real center_dist = dot(sub(bpos, ppos), n);
+  int nc = 0, worst = 0;
+  for (int i = 0; i < 8; ++i) {
+    v3 c = V((i & 1) ? size.x : -size.x, (i & 2) ? size.y : -size.y, (i & 4) ? size.z : -size.z);
+    c = mv(rot, c);
+    real cd = center_dist + dot(n, c);
+    if (cd > margin) continue;
+    v3 cpos = sub(add(c, bpos), scl(n, (real)0.5 * cd));
+    if (nc < 4) {
+      m->dist[nc] = cd; m->pos[nc] = cpos;
+      if (nc == 0 || cd > m->dist[worst]) worst = nc;
+      nc++;
+    } else if (cd < m->dist[worst]) {
+      m->dist[worst] = cd; m->pos[worst] = cpos;
+      worst = 0;
+      for (int k = 1; k < 4; ++k) if (m->dist[k] > m->dist[worst]) worst = k;
+    }
+  }
+  for (int k = 0; k < 4; ++k) m->normal[k] = n;
+}
+
+/* collide_sphere_box, :1162-1214 */
+static void collide_sphere_box(v3 sp, real sr, v3 bpos, const m33* rot, v3 size, real* dist, v3* pos, v3* n) {
+  v3 center = mtv(rot, sub(sp, bpos));
+  v3 cl = V(maxr(-size.x, minr(size.x, center.x)), maxr(-size.y, minr(size.y, center.y)),
+            maxr(-size.z, minr(size.z, center.z)));
+  v3 dv = sub(cl, center);
+  real d = len(dv);
+  v3 cdir = d > 0 ? scl(dv, (real)1 / d) : dv;
+  v3 p, nl;
+  if (d <= (real)1e-6) {
+    real closest = 2 * (size.x + size.y + size.z);
+    int k = 0;
+    for (int i = 0; i < 6; ++i) {
+      real fd = RA(((i % 2) ? (real)1 : (real)-1) * comp(size, i / 2) - comp(center, i / 2));
+      if (closest > fd) { closest = fd; k = i; }
+    }
+    v3 nearest = V(0, 0, 0);
+    setc(&nearest, k / 2, (k % 2) ? (real)-1 : (real)1);
+    p = add(center, scl(nearest, (sr - closest) / 2));
+    nl = nearest;
+    *dist = -closest - sr;
+  } else {
+    v3 deepest = add(center, scl(cdir, sr));
+    p = scl(add(cl, deepest), (real)0.5);
+    nl = cdir;
+    *dist = d - sr;
+  }
+  *n = mv(rot, nl);
+  *pos = add(bpos, mv(rot, p));
+}
+
+/* ---- capsule-box (this project's restatement; reference uses GJK/MPR) ----
+ * Signed distance of point p (box frame) to the box, and its outward gradient. */
+static real box_sdf(v3 p, v3 h, v3* grad) {
+  v3 q = V(RA(p.x) - h.x, RA(p.y) - h.y, RA(p.z) - h.z);
+  v3 qo = V(maxr(q.x, 0), maxr(q.y, 0), maxr(q.z, 0));
+  real out = len(qo);
+  real in = minr(maxr(q.x, maxr(q.y, q.z)), 0);
+  if (out > 0) {
+    v3 g = V(p.x < 0 ? -qo.x : qo.x, p.y < 0 ? -qo.y : qo.y, p.z < 0 ? -qo.z : qo.z);
+    *grad = scl(g, (real)1 / out);
+  } else {
+    int k = (q.x >= q.y && q.x >= q.z) ? 0 : (q.y >= q.z ? 1 : 2);
+    v3 g = V(0, 0, 0);
+    setc(&g, k, comp(p, k) < 0 ? (real)-1 : (real)1);
+    *grad = g;
+  }
+  return out + in;
+}
+
+#define CAPBOX_ITERS 40
+#define CAPBOX_PARALLEL 0.2
+
+static void collide_capsule_box(v3 cpos, v3 axis, real r, real hl, v3 bpos, const m33* rot, v3 size, manifold* m) {
+  v3 p0 = mtv(rot, sub(sub(cpos, scl(axis, hl)), bpos));
+  v3 p1 = mtv(rot, sub(add(cpos, scl(axis, hl)), bpos));
+  v3 d = sub(p1, p0);
+  /* golden-section minimisation of the convex sdf(p0 + t d) over t in [0, 1] */
+  const real gr = (real)0.6180339887498949;
+  real a = 0, b = 1, c1 = b - gr * (b - a), c2 = a + gr * (b - a);
+  v3 g;
+  real f1 = box_sdf(add(p0, scl(d, c1)), size, &g), f2 = box_sdf(add(p0, scl(d, c2)), size, &g);
+  for (int it = 0; it < CAPBOX_ITERS; ++it) {
+    if (f1 <= f2) { b = c2; c2 = c1; f2 = f1; c1 = b - gr * (b - a); f1 = box_sdf(add(p0, scl(d, c1)), size, &g); }
+    else { a = c1; c1 = c2; f1 = f2; c2 = a + gr * (b - a); f2 = box_sdf(add(p0, scl(d, c2)), size, &g); }
+  }
+  real ts = (real)0.5 * (a + b);
+  v3 ps = add(p0, scl(d, ts));
+  v3 gs;
+  real fs = box_sdf(ps, size, &gs);
+  real dl = len(d);
+  int two = dl > 0 && RA(dot(d, gs)) < (real)CAPBOX_PARALLEL * dl;
+  real ts_list[2];
+  int cnt;
+  if (two) { ts_list[0] = 0; ts_list[1] = 1; cnt = 2; }
+  else { ts_list[0] = ts; cnt = 1; }
+  for (int k = 0; k < cnt; ++k) {
+    v3 pk = add(p0, scl(d, ts_list[k]));
+    v3 gk;
+    real fk = two ? box_sdf(pk, size, &gk) : fs;
+    if (!two) gk = gs;
+    /* normal from capsule (A) to box (B) is -grad; midpoint between surfaces */
+    v3 nl = scl(gk, -1);
+    real dist = fk - r;
+    v3 mid = add(pk, scl(nl, r + (real)0.5 * dist));
+    m->dist[k] = dist;
+    m->pos[k] = add(bpos, mv(rot, mid));
+    m->normal[k] = mv(rot, nl);
+  }
+}
+
+/* ---- box-box (this project's restatement; reference uses GJK/MPR + clipping) ---- */
+#define BB_EDGE_BIAS 1.0e-4
+static int clip_poly(const v3* in, int n, v3 pn, real off, v3* out) {
+  /* keep the half-space dot(pn, x) <= off (Sutherland-Hodgman) */
+  int m = 0;
+  for (int i = 0; i < n; ++i) {
+    v3 a = in[i], b = in[(i + 1) % n];
+    real da = dot(pn, a) - off, db = dot(pn, b) - off;
+    if (da <= 0) out[m++] = a;
+    if ((da < 0 && db > 0) || (da > 0 && db < 0)) {
+      real t = da / (da - db);
+      out[m++] = add(a, scl(sub(b, a), t));
+    }
+  }
+  return m;
+}
+
+static void collide_box_box(v3 pa, const m33* ra, v3 ha, v3 pb, const m33* rb, v3 hb, real gap, manifold* m) {
+  v3 dv = sub(pb, pa);
+  v3 A[3] = {col(ra, 0), col(ra, 1), col(ra, 2)}, B[3] = {col(rb, 0), col(rb, 1), col(rb, 2)};
+  real hA[3] = {ha.x, ha.y, ha.z}, hB[3] = {hb.x, hb.y, hb.z};
+  real best = -(real)OR_MAXVAL;
+  int best_axis = -1;
+  v3 best_n = V(0, 0, 1);
+  real scale = maxr(maxr(ha.x, maxr(ha.y, ha.z)), maxr(hb.x, maxr(hb.y, hb.z)));
+  for (int k = 0; k < 15; ++k) {
+    v3 L;
+    if (k < 3) L = A[k];
+    else if (k < 6) L = B[k - 3];
+    else {
+      L = cross(A[(k - 6) / 3], B[(k - 6) % 3]);
+      real l = len(L);
+      if (l < (real)1e-6) continue;
+      L = scl(L, (real)1 / l);
+    }
+    real rA = hA[0] * RA(dot(A[0], L)) + hA[1] * RA(dot(A[1], L)) + hA[2] * RA(dot(A[2], L));
+    real rB = hB[0] * RA(dot(B[0], L)) + hB[1] * RA(dot(B[1], L)) + hB[2] * RA(dot(B[2], L));
+    real dl = dot(dv, L);
+    real s = RA(dl) - rA - rB;
+    real bias = k < 6 ? 0 : (real)BB_EDGE_BIAS * scale;
+    if (s > best + bias) { best = s; best_axis = k; best_n = dl < 0 ? scl(L, -1) : L; }
+  }
+  if (best > gap) return; /* separated beyond the contact gap on some axis */
+  v3 n = best_n; /* from A to B */
+  if (best_axis < 6) {
+    int ref_is_a = best_axis < 3;
+    const v3* R = ref_is_a ? A : B;
+    const v3* I = ref_is_a ? B : A;
+    const real* hR = ref_is_a ? hA : hB;
+    const real* hI = ref_is_a ? hB : hA;
+    v3 pR = ref_is_a ? pa : pb, pI = ref_is_a ? pb : pa;
+    v3 nref = ref_is_a ? n : scl(n, -1); /* reference face normal, toward the incident box */
+    int ax = ref_is_a ? best_axis : best_axis - 3;
+    v3 fc = add(pR, scl(nref, hR[ax])); /* reference face centre */
+    /* incident face: most anti-parallel to nref */
+    int ik = 0; real mind = (real)OR_MAXVAL; real sgn = 1;
+    for (int k = 0; k < 3; ++k) {
+      real dd = dot(I[k], nref);
+      if (dd < mind) { mind = dd; ik = k; sgn = 1; }
+      if (-dd < mind) { mind = -dd; ik = k; sgn = -1; }
+    }
+    v3 ic = add(pI, scl(I[ik], sgn * hI[ik]));
+    int u = (ik + 1) % 3, w = (ik + 2) % 3;
+    v3 poly[8], tmp[8];
+    poly[0] = add(ic, add(scl(I[u], hI[u]), scl(I[w], hI[w])));
+    poly[1] = add(ic, add(scl(I[u], -hI[u]), scl(I[w], hI[w])));
+    poly[2] = add(ic, add(scl(I[u], -hI[u]), scl(I[w], -hI[w])));
+    poly[3] = add(ic, add(scl(I[u], hI[u]), scl(I[w], -hI[w])));
+    int np = 4;
+    int ru = (ax + 1) % 3, rw = (ax + 2) % 3;
+    /* clip against the 4 side planes of the reference face */
+    np = clip_poly(poly, np, R[ru], dot(R[ru], pR) + hR[ru], tmp);
+    np = clip_poly(tmp, np, scl(R[ru], -1), -dot(R[ru], pR) + hR[ru], poly);
+    np = clip_poly(poly, np, R[rw], dot(R[rw], pR) + hR[rw], tmp);
+    np = clip_poly(tmp, np, scl(R[rw], -1), -dot(R[rw], pR) + hR[rw], poly);
+    real dd[8]; v3 pp[8]; int kept = 0;
+    for (int i = 0; i < np; ++i) {
+      real di = dot(sub(poly[i], fc), nref);
+      if (di > gap) continue;
+      dd[kept] = di;
+      pp[kept] = sub(poly[i], scl(nref, (real)0.5 * di)); /* midpoint to the reference face */
+      kept++;
+    }
+    int sel[4]; int ns = 0;
+    if (kept <= 4) { for (int i = 0; i < kept; ++i) sel[ns++] = i; }
+    else {
+      int i0 = 0;
+      for (int i = 1; i < kept; ++i) if (dd[i] < dd[i0]) i0 = i;
+      int i1 = -1; real far = -1;
+      for (int i = 0; i < kept; ++i) { real q = dot(sub(pp[i], pp[i0]), sub(pp[i], pp[i0])); if (i != i0 && q > far) { far = q; i1 = i; } }
+      int i2 = -1, i3 = -1; real amax = 0, amin = 0;
+      v3 e01 = sub(pp[i1], pp[i0]);
+      for (int i = 0; i < kept; ++i) {
+        if (i == i0 || i == i1) continue;
+        real ar = dot(cross(e01, sub(pp[i], pp[i0])), nref);
+        if (ar > amax) { amax = ar; i2 = i; }
+        if (ar < amin) { amin = ar; i3 = i; }
+      }
+      sel[ns++] = i0; sel[ns++] = i1;
+      if (i2 >= 0) sel[ns++] = i2;
+      if (i3 >= 0) sel[ns++] = i3;
+    }
+    for (int i = 0; i < ns; ++i) {
+      m->dist[i] = dd[sel[i]];
+      m->pos[i] = pp[sel[i]];
+      m->normal[i] = n;
+    }
+  } else {
+    int ia = (best_axis - 6) / 3, ib = (best_axis - 6) % 3;
+    /* supporting edges: A's edge toward +n, B's edge toward -n */
+    v3 ca = pa, cb = pb;
+    for (int k = 0; k < 3; ++k) {
+      if (k != ia) ca = add(ca, scl(A[k], (dot(A[k], n) > 0 ? 1 : -1) * hA[k]));
+      if (k != ib) cb = add(cb, scl(B[k], (dot(B[k], n) > 0 ? -1 : 1) * hB[k]));
+    }
+    /* closest points of the two edge lines, clamped to the edges */
+    v3 da = A[ia], db = B[ib], r = sub(ca, cb);
+    real b = dot(da, db), c = dot(da, r), f = dot(db, r);
+    real den = 1 - b * b;
+    real s = den > (real)1e-12 ? (b * f - c) / den : 0;
+    s = clampr(s, -hA[ia], hA[ia]);
+    real t = clampr(b * s + f, -hB[ib], hB[ib]);
+    s = clampr(b * t - c, -hA[ia], hA[ia]);
+    v3 qa = add(ca, scl(da, s)), qb = add(cb, scl(db, t));
+    real di = dot(sub(qb, qa), n);
+    m->dist[0] = di;
+    m->pos[0] = scl(add(qa, qb), (real)0.5);
+    m->normal[0] = n;
+  }
+}
+
+/* ====================================================================
+ * Shape poses, AABBs, broad phase, narrow phase
+ * ==================================================================== */
+typedef struct { v3 p; qt q; } xf;
+
+static xf shape_world_xf(const or_model* M, const double* body_q, int b0, int s) {
+  const double* sx = M->shape_xform + 7 * s;
+  xf local = {V((real)sx[0], (real)sx[1], (real)sx[2]), {(real)sx[3], (real)sx[4], (real)sx[5], (real)sx[6]}};
+  int b = M->shape_body[s];
+  if (b < 0) return local;
+  const double* bq = body_q + 7 * (b - b0);
+  xf body = {V((real)bq[0], (real)bq[1], (real)bq[2]), {(real)bq[3], (real)bq[4], (real)bq[5], (real)bq[6]}};
+  xf out;
+  out.p = add(body.p, qrot(body.q, local.p));
+  out.q = qmul(body.q, local.q);
+  return out;
+}
+
+/* compute_shape_aabbs, collide.py:150-275 (closed forms of the support-map AABB) */
+/* body_q holds the bodies starting at global index b0 */
+void or_shape_aabb(const or_model* M, const double* body_q, int b0, int s, double* lo, double* hi) {
+  xf X = shape_world_xf(M, body_q, b0, s);
+  const double* sc = M->shape_scale + 3 * s;
+  real eg = (real)(M->shape_margin[s] + M->shape_gap[s]);
+  m33 R = qmat(X.q);
+  v3 h;
+  int t = M->shape_type[s];
+  if (t == GEO_PLANE && sc[0] == 0.0 && sc[1] == 0.0) {
+    real r = (real)M->shape_radius[s];
+    h = V(r, r, r);
+  } else if (t == GEO_SPHERE) {
+    h = V((real)sc[0], (real)sc[0], (real)sc[0]);
+  } else if (t == GEO_CAPSULE) {
+    v3 ax = col(&R, 2);
+    real hl = (real)sc[1], r = (real)sc[0];
+    h = V(RA(ax.x) * hl + r, RA(ax.y) * hl + r, RA(ax.z) * hl + r);
+  } else {
+    v3 e = (t == GEO_PLANE) ? V((real)(0.5 * sc[0]), (real)(0.5 * sc[1]), 0) : V((real)sc[0], (real)sc[1], (real)sc[2]);
+    h = V(RA(R.m[0]) * e.x + RA(R.m[1]) * e.y + RA(R.m[2]) * e.z, RA(R.m[3]) * e.x + RA(R.m[4]) * e.y + RA(R.m[5]) * e.z,
+          RA(R.m[6]) * e.x + RA(R.m[7]) * e.y + RA(R.m[8]) * e.z);
+  }
+  lo[0] = X.p.x - h.x - eg; lo[1] = X.p.y - h.y - eg; lo[2] = X.p.z - h.z - eg;
+  hi[0] = X.p.x + h.x + eg; hi[1] = X.p.y + h.y + eg; hi[2] = X.p.z + h.z + eg;
+}
+
+/* test_group_pair / test_world_and_group_pair, broad_phase_common.py:132-180 */
+static int group_pair(int a, int b) {
+  if (a == 0 || b == 0) return 0;
+  if (a > 0) return a == b || b < 0;
+  return a != b;
+}
+static int world_group_pair(int wa, int wb, int ga, int gb) {
+  if (wa != -1 && wb != -1 && wa != wb) return 0;
+  return group_pair(ga, gb);
+}
+static int excluded(const or_model* M, int a, int b) {
+  int lo = 0, hi = M->filter_count - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int fa = M->filter_pairs[2 * mid], fb = M->filter_pairs[2 * mid + 1];
+    if (fa == a && fb == b) return 1;
+    if (a < fa || (a == fa && b < fb)) hi = mid - 1; else lo = mid + 1;
+  }
+  return 0;
+}
+
+/* Brute-force candidate pairs of one world segment (its shapes + the global shapes),
+ * over the given AABBs (the n^2 oracle of test_broad_phase.py:90-148 with the pipeline's
+ * zero extra cutoff, collide.py:1003-1016).  lo/hi are indexed by segment position
+ * (world-local shapes first, then the global shapes).  Pairs are canonical (min, max) shape
+ * indices, sorted lexicographically. */
+static int cmp_pair(const void* a, const void* b) {
+  const int32_t* x = (const int32_t*)a; const int32_t* y = (const int32_t*)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+  return 0;
+}
+static int segment_shape(const or_model* M, int w, int i) {
+  int s0 = M->world_shape_start[w], nl = M->world_shape_start[w + 1] - s0;
+  return i < nl ? s0 + i : M->global_shapes[i - nl];
+}
+static int segment_size(const or_model* M, int w) {
+  return M->world_shape_start[w + 1] - M->world_shape_start[w] + M->global_count;
+}
+int or_world_pairs(const or_model* M, int w, const double* lo, const double* hi, int32_t* pairs, int cap) {
+  int n = segment_size(M, w);
+  int cnt = 0;
+  for (int i = 0; i < n; ++i) {
+    int a = segment_shape(M, w, i);
+    for (int j = i + 1; j < n; ++j) {
+      int b = segment_shape(M, w, j);
+      int sa = a < b ? a : b, sb = a < b ? b : a;
+      int wa = M->shape_world[sa], wb = M->shape_world[sb];
+      if (wa == -1 && wb == -1) continue; /* global-global pairs belong to the dedicated -1 segment */
+      if (!world_group_pair(wa, wb, M->shape_group[sa], M->shape_group[sb])) continue;
+      if (M->filter_count > 0 && excluded(M, sa, sb)) continue;
+      const double *la = lo + 3 * i, *ua = hi + 3 * i, *lb = lo + 3 * j, *ub = hi + 3 * j;
+      if (la[0] <= ub[0] && ua[0] >= lb[0] && la[1] <= ub[1] && ua[1] >= lb[1] && la[2] <= ub[2] && ua[2] >= lb[2]) {
+        if (cnt < cap) { pairs[2 * cnt] = sa; pairs[2 * cnt + 1] = sb; }
+        cnt++;
+      }
+    }
+  }
+  qsort(pairs, cnt < cap ? cnt : cap, 2 * sizeof(int32_t), cmp_pair);
+  return cnt;
+}
+
+/* Test entry: pairs of world w from AABBs given per global shape index ([S,3] each). */
+int or_world_pairs_global(const or_model* M, int w, const double* lo_g, const double* hi_g, int32_t* pairs, int cap) {
+  int n = segment_size(M, w);
+  double* lo = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
+  double* hi = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    int s = segment_shape(M, w, i);
+    memcpy(lo + 3 * i, lo_g + 3 * s, 3 * sizeof(double));
+    memcpy(hi + 3 * i, hi_g + 3 * s, 3 * sizeof(double));
+  }
+  int cnt = or_world_pairs(M, w, lo, hi, pairs, cap);
+  free(lo); free(hi);
+  return cnt;
+}
+
+/* solver-side contact record (world frame at the collision state) */
+typedef struct {
+  int shape_a, shape_b, body_a, body_b, sub;
+  real n[3], c[3];
+  real dist, phi0, r_a, r_b, m_a, m_b, k, mu;
+} or_contact;
+
+/* Narrow phase of one canonical pair, narrow_phase.py:154-604 routing.
+ * Returns the number of contacts written (gap-checked, contact_data.py:86-117). */
+static int narrow_pair(const or_model* M, const double* body_q, int b0, int sa, int sb, or_contact* out) {
+  int ta = M->shape_type[sa], tb = M->shape_type[sb];
+  if (ta > tb) { int t = sa; sa = sb; sb = t; t = ta; ta = tb; tb = t; }
+  xf Xa = shape_world_xf(M, body_q, b0, sa), Xb = shape_world_xf(M, body_q, b0, sb);
+  const double *da = M->shape_scale + 3 * sa, *db = M->shape_scale + 3 * sb;
+  v3 sca = V((real)da[0], (real)da[1], (real)da[2]), scb = V((real)db[0], (real)db[1], (real)db[2]);
+  real gap_sum = (real)(M->shape_gap[sa] + M->shape_gap[sb]);
+  real ma = (real)M->shape_margin[sa], mb = (real)M->shape_margin[sb];
+  real ra = (ta == GEO_SPHERE || ta == GEO_CAPSULE) ? sca.x : 0;
+  real rb = (tb == GEO_SPHERE || tb == GEO_CAPSULE) ? scb.x : 0;
+  manifold m;
+  mf_init(&m);
+  m33 Ra = qmat(Xa.q), Rb = qmat(Xb.q);
+  v3 za = col(&Ra, 2), zb = col(&Rb, 2);
+  if (ta == GEO_PLANE && tb == GEO_SPHERE) {
+    collide_plane_sphere(za, Xa.p, Xb.p, scb.x, &m.dist[0], &m.pos[0]);
+    m.normal[0] = za;
+  } else if (ta == GEO_PLANE && tb == GEO_BOX) {
+    collide_plane_box(za, Xa.p, Xb.p, &Rb, scb, gap_sum, &m);
+  } else if (ta == GEO_SPHERE && tb == GEO_SPHERE) {
+    collide_sphere_sphere(Xa.p, sca.x, Xb.p, scb.x, &m.dist[0], &m.pos[0], &m.normal[0]);
+  } else if (ta == GEO_PLANE && tb == GEO_CAPSULE) {
+    collide_plane_capsule(za, Xa.p, Xb.p, zb, scb.x, scb.y, &m);
+  } else if (ta == GEO_SPHERE && tb == GEO_CAPSULE) {
+    collide_sphere_capsule(Xa.p, sca.x, Xb.p, zb, scb.x, scb.y, &m.dist[0], &m.pos[0], &m.normal[0]);
+  } else if (ta == GEO_CAPSULE && tb == GEO_CAPSULE) {
+    collide_capsule_capsule(Xa.p, za, sca.x, sca.y, Xb.p, zb, scb.x, scb.y, &m);
+  } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
+    collide_sphere_box(Xa.p, sca.x, Xb.p, &Rb, scb, &m.dist[0], &m.pos[0], &m.normal[0]);
+  } else if (ta == GEO_CAPSULE && tb == GEO_BOX) {
+    collide_capsule_box(Xa.p, za, sca.x, sca.y, Xb.p, &Rb, scb, &m);
+  } else if (ta == GEO_BOX && tb == GEO_BOX) {
+    collide_box_box(Xa.p, &Ra, sca, Xb.p, &Rb, scb, gap_sum + ma + mb, &m);
+  } else {
+    return 0; /* plane-plane and unsupported types produce nothing */
+  }
+  int cnt = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (!(m.dist[k] < (real)OR_MAXVAL)) continue;
+    v3 n = nrm(m.normal[k]);
+    /* contact_passes_gap_check: distance between the writer's core points minus radii+margins */
+    v3 cpos = m.pos[k];
+    v3 pa = sub(cpos, scl(n, (real)0.5 * m.dist[k] + ra));
+    v3 pb = add(cpos, scl(n, (real)0.5 * m.dist[k] + rb));
+    real d = dot(sub(pb, pa), n) - (ra + rb + ma + mb);
+    if (!(d <= gap_sum)) continue;
+    or_contact* c = out + cnt;
+    c->shape_a = sa; c->shape_b = sb; c->sub = k;
+    c->body_a = M->shape_body[sa]; c->body_b = M->shape_body[sb];
+    c->n[0] = n.x; c->n[1] = n.y; c->n[2] = n.z;
+    c->c[0] = cpos.x; c->c[1] = cpos.y; c->c[2] = cpos.z;
+    c->dist = m.dist[k]; c->phi0 = d; c->r_a = ra; c->r_b = rb; c->m_a = ma; c->m_b = mb;
+    c->k = (real)(0.5 * (M->shape_ke[sa] + M->shape_ke[sb]));
+    c->mu = (real)(M->shape_mu[sa] > M->shape_mu[sb] ? M->shape_mu[sa] : M->shape_mu[sb]);
+    cnt++;
+  }
+  return cnt;
+}
+
+/* Single-pair entry for the primitive KATs: shapes given directly (types, poses, scales).
+ * out: [n*8] = dist, pos(3), normal(3), phi0 */
+int or_collide_shapes(int ta, const double* xa, const double* sa_, int tb, const double* xb, const double* sb_,
+                      double gap_sum, double* out) {
+  xf Xa = {V((real)xa[0], (real)xa[1], (real)xa[2]), {(real)xa[3], (real)xa[4], (real)xa[5], (real)xa[6]}};
+  xf Xb = {V((real)xb[0], (real)xb[1], (real)xb[2]), {(real)xb[3], (real)xb[4], (real)xb[5], (real)xb[6]}};
+  v3 sca = V((real)sa_[0], (real)sa_[1], (real)sa_[2]), scb = V((real)sb_[0], (real)sb_[1], (real)sb_[2]);
+  manifold m;
+  mf_init(&m);
+  m33 Ra = qmat(Xa.q), Rb = qmat(Xb.q);
+  v3 za = col(&Ra, 2), zb = col(&Rb, 2);
+  if (ta == GEO_PLANE && tb == GEO_SPHERE) { collide_plane_sphere(za, Xa.p, Xb.p, scb.x, &m.dist[0], &m.pos[0]); m.normal[0] = za; }
+  else if (ta == GEO_PLANE && tb == GEO_BOX) collide_plane_box(za, Xa.p, Xb.p, &Rb, scb, (real)gap_sum, &m);
+  else if (ta == GEO_SPHERE && tb == GEO_SPHERE) collide_sphere_sphere(Xa.p, sca.x, Xb.p, scb.x, &m.dist[0], &m.pos[0], &m.normal[0]);
+  else if (ta == GEO_PLANE && tb == GEO_CAPSULE) collide_plane_capsule(za, Xa.p, Xb.p, zb, scb.x, scb.y, &m);
+  else if (ta == GEO_SPHERE && tb == GEO_CAPSULE) collide_sphere_capsule(Xa.p, sca.x, Xb.p, zb, scb.x, scb.y, &m.dist[0], &m.pos[0], &m.normal[0]);
+  else if (ta == GEO_CAPSULE && tb == GEO_CAPSULE) collide_capsule_capsule(Xa.p, za, sca.x, sca.y, Xb.p, zb, scb.x, scb.y, &m);
+  else if (ta == GEO_SPHERE && tb == GEO_BOX) collide_sphere_box(Xa.p, sca.x, Xb.p, &Rb, scb, &m.dist[0], &m.pos[0], &m.normal[0]);
+  else if (ta == GEO_CAPSULE && tb == GEO_BOX) collide_capsule_box(Xa.p, za, sca.x, sca.y, Xb.p, &Rb, scb, &m);
+  else if (ta == GEO_BOX && tb == GEO_BOX) collide_box_box(Xa.p, &Ra, sca, Xb.p, &Rb, scb, (real)gap_sum, &m);
+  int cnt = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (!(m.dist[k] < (real)OR_MAXVAL)) continue;
+    double* o = out + 8 * cnt;
+    o[0] = m.dist[k]; o[1] = m.pos[k].x; o[2] = m.pos[k].y; o[3] = m.pos[k].z;
+    o[4] = m.normal[k].x; o[5] = m.normal[k].y; o[6] = m.normal[k].z; o[7] = 0;
+    cnt++;
+  }
+  return cnt;
+}
+
+/* Contact generation of one world at pose body_q (that world's bodies only):
+ * AABBs -> brute-force pairs -> narrow phase.
+ * Contacts are ordered by (shape_a, shape_b) of the canonical pair, then slot. */
+int or_collide_world(const or_model* M, int w, const double* body_q, or_contact* out, int cap, int* n_pairs) {
+  int n = segment_size(M, w);
+  double* lo = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
+  double* hi = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
+  int b0 = M->world_body_start[w];
+  for (int i = 0; i < n; ++i) or_shape_aabb(M, body_q, b0, segment_shape(M, w, i), lo + 3 * i, hi + 3 * i);
+  int pcap = n * (n - 1) / 2 + 1;
+  int32_t* pairs = (int32_t*)malloc(sizeof(int32_t) * 2 * pcap);
+  int np = or_world_pairs(M, w, lo, hi, pairs, pcap);
+  if (n_pairs) *n_pairs = np;
+  int cnt = 0;
+  or_contact tmp[4];
+  for (int p = 0; p < np; ++p) {
+    int k = narrow_pair(M, body_q, b0, pairs[2 * p], pairs[2 * p + 1], tmp);
+    for (int i = 0; i < k; ++i) {
+      if (cnt < cap) out[cnt] = tmp[i];
+      cnt++;
+    }
+  }
+  free(pairs); free(lo); free(hi);
+  return cnt;
+}
+
+/* Flat export of one world's contacts for tests: per contact 14 doubles
+ * (shape_a, shape_b, sub, n[3], c[3], dist, phi0, k, mu, r_a) */
+int or_collide_world_flat(const or_model* M, int w, const double* body_q, double* out, int cap, int* n_pairs) {
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
+  int cnt = or_collide_world(M, w, body_q + 7 * M->world_body_start[w], cs, cap, n_pairs);
+  int m = cnt < cap ? cnt : cap;
+  for (int i = 0; i < m; ++i) {
+    double* o = out + 14 * i;
+    o[0] = cs[i].shape_a; o[1] = cs[i].shape_b; o[2] = cs[i].sub;
+    o[3] = cs[i].n[0]; o[4] = cs[i].n[1]; o[5] = cs[i].n[2];
+    o[6] = cs[i].c[0]; o[7] = cs[i].c[1]; o[8] = cs[i].c[2];
+    o[9] = cs[i].dist; o[10] = cs[i].phi0; o[11] = cs[i].k; o[12] = cs[i].mu; o[13] = cs[i].r_a;
+  }
+  free(cs);
+  return cnt;
+}
+
+/* ====================================================================
+ * Convex SAP solve of one world (dense, exact Newton).
+ * ==================================================================== */
+typedef struct {
+  int nb, nc, N;
+  real* m;     /* [nb] */
+  real* Iw;    /* [nb*9] world inertia */
+  real* Iwi;   /* [nb*9] its inverse */
+  real* xc;    /* [nb*3] COM position */
+  real* vstar; /* [N] */
+  real* Ad;    /* [N] diag(A) */
+  /* contacts */
+  int* cb;     /* [nc*2] local body indices (-1 static) */
+  real* J;     /* [nc*2*18] 3x6 blocks per body (rows t1, t2, n) */
+  real* Rt; real* Rn; real* vhat; real* mu;
+  /* scratch */
+  real* H; real* g; real* dv; real* v; real* vc; real* dvc; real* gam; real* G; real* tmp;
+} sap_ctx;
+
+/* P_F projection in the R norm and its Jacobian-derived Hessian G = dP/dy R^-1 */
+static real project(real mu, real Rt, real Rn, const real* y, real* gam, real* G) {
+  real yr = RS(y[0] * y[0] + y[1] * y[1]), yn = y[2];
+  real muh = mu * Rt / Rn;
+  for (int i = 0; i < 9; ++i) G[i] = 0;
+  if (yr <= mu * yn) {
+    gam[0] = y[0]; gam[1] = y[1]; gam[2] = y[2];
+    G[0] = 1 / Rt; G[4] = 1 / Rt; G[8] = 1 / Rn;
+  } else if (yn <= -muh * yr) {
+    gam[0] = gam[1] = gam[2] = 0;
+  } else {
+    real t0 = y[0] / yr, t1 = y[1] / yr;
+    real den = 1 + mu * muh;
+    real gn = (yn + muh * yr) / den;
+    gam[0] = mu * gn * t0; gam[1] = mu * gn * t1; gam[2] = gn;
+    real a = mu * muh / den, b = mu * gn / yr;
+    real P[9];
+    P[0] = a * t0 * t0 + b * (1 - t0 * t0); P[1] = a * t0 * t1 - b * t0 * t1; P[2] = mu / den * t0;
+    P[3] = a * t1 * t0 - b * t1 * t0;       P[4] = a * t1 * t1 + b * (1 - t1 * t1); P[5] = mu / den * t1;
+    P[6] = muh / den * t0;                  P[7] = muh / den * t1;               P[8] = 1 / den;
+    for (int r = 0; r < 3; ++r) { G[3 * r + 0] = P[3 * r + 0] / Rt; G[3 * r + 1] = P[3 * r + 1] / Rt; G[3 * r + 2] = P[3 * r + 2] / Rn; }
+  }
+  return (real)0.5 * (Rt * (gam[0] * gam[0] + gam[1] * gam[1]) + Rn * gam[2] * gam[2]);
+}
+
+static void contact_velocity(const sap_ctx* S, int c, const real* v, real* vc) {
+  vc[0] = vc[1] = vc[2] = 0;
+  for (int s = 0; s < 2; ++s) {
+    int b = S->cb[2 * c + s];
+    if (b < 0) continue;
+    const real* Jb = S->J + (2 * c + s) * 18;
+    const real* vb = v + 6 * b;
+    for (int r = 0; r < 3; ++r) {
+      real acc = 0;
+      for (int k = 0; k < 6; ++k) acc += Jb[6 * r + k] * vb[k];
+      vc[r] += acc;
+    }
+  }
+}
+
+/* momentum A x for block-diagonal A = diag(m I, Iw) */
+static void apply_A(const sap_ctx* S, const real* x, real* y) {
+  for (int b = 0; b < S->nb; ++b) {
+    const real* xb = x + 6 * b; real* yb = y + 6 * b; const real* I = S->Iw + 9 * b;
+    yb[0] = S->m[b] * xb[0]; yb[1] = S->m[b] * xb[1]; yb[2] = S->m[b] * xb[2];
+    for (int r = 0; r < 3; ++r) yb[3 + r] = I[3 * r] * xb[3] + I[3 * r + 1] * xb[4] + I[3 * r + 2] * xb[5];
+  }
+}
+
+/* cost, gradient (g = A(v-v*) - J^T gamma) and, if G != NULL, per-contact Hessian blocks */
+static real eval_cost(sap_ctx* S, const real* v, real* g, real* gam, real* G, real* pnorm, real* jnorm) {
+  int N = S->N;
+  real* d = S->tmp;
+  for (int i = 0; i < N; ++i) d[i] = v[i] - S->vstar[i];
+  apply_A(S, d, g);
+  real cost = 0;
+  for (int i = 0; i < N; ++i) cost += (real)0.5 * d[i] * g[i];
+  real* jc = S->tmp + N;
+  for (int i = 0; i < N; ++i) jc[i] = 0;
+  for (int c = 0; c < S->nc; ++c) {
+    real vc[3], y[3], gl[3], Gl[9];
+    contact_velocity(S, c, v, vc);
+    y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
+    cost += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    if (gam) memcpy(gam + 3 * c, gl, sizeof(gl));
+    if (G) memcpy(G + 9 * c, Gl, sizeof(Gl));
+    for (int s = 0; s < 2; ++s) {
+      int b = S->cb[2 * c + s];
+      if (b < 0) continue;
+      const real* Jb = S->J + (2 * c + s) * 18;
+      for (int k = 0; k < 6; ++k) jc[6 * b + k] += Jb[k] * gl[0] + Jb[6 + k] * gl[1] + Jb[12 + k] * gl[2];
+    }
+  }
+  for (int i = 0; i < N; ++i) g[i] -= jc[i];
+  if (pnorm) {
+    real* p = S->tmp + 2 * N;
+    apply_A(S, v, p);
+    real pn = 0, jn = 0;
+    for (int i = 0; i < N; ++i) { real s = 1 / RS(S->Ad[i]); pn += (s * p[i]) * (s * p[i]); jn += (s * jc[i]) * (s * jc[i]); }
+    *pnorm = RS(pn); *jnorm = RS(jn);
+  }
+  return cost;
+}
+
+/* l(alpha) along dv: value, first and second derivative */
+static void ls_eval(sap_ctx* S, const real* v, const real* dv, real alpha, real* l, real* dl, real* d2l) {
+  int N = S->N;
+  real* d = S->tmp;
+  real* Adv = S->tmp + N;
+  for (int i = 0; i < N; ++i) d[i] = v[i] + alpha * dv[i] - S->vstar[i];
+  apply_A(S, dv, Adv);
+  real mom = 0, dm = 0, ddm = 0;
+  real* Ad = S->tmp + 2 * N;
+  apply_A(S, d, Ad);
+  for (int i = 0; i < N; ++i) { mom += (real)0.5 * d[i] * Ad[i]; dm += dv[i] * Ad[i]; ddm += dv[i] * Adv[i]; }
+  real* va = S->tmp + 3 * N;
+  for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
+  for (int c = 0; c < S->nc; ++c) {
+    real vc[3], dvc[3], y[3], gl[3], Gl[9];
+    contact_velocity(S, c, va, vc);
+    contact_velocity(S, c, dv, dvc);
+    y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
+    mom += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    dm -= gl[0] * dvc[0] + gl[1] * dvc[1] + gl[2] * dvc[2];
+    for (int r = 0; r < 3; ++r) ddm += dvc[r] * (Gl[3 * r] * dvc[0] + Gl[3 * r + 1] * dvc[1] + Gl[3 * r + 2] * dvc[2]);
+  }
+  *l = mom; *dl = dm; *d2l = ddm;
+}
+
+static int cholesky(real* A, int N) {
+  for (int j = 0; j < N; ++j) {
+    real s = A[j * N + j];
+    for (int k = 0; k < j; ++k) s -= A[j * N + k] * A[j * N + k];
+    if (!(s > 0)) return 0;
+    real d = RS(s);
+    A[j * N + j] = d;
+    for (int i = j + 1; i < N; ++i) {
+      real t = A[i * N + j];
+      for (int k = 0; k < j; ++k) t -= A[i * N + k] * A[j * N + k];
+      A[i * N + j] = t / d;
+    }
+  }
+  return 1;
+}
+static void chol_solve(const real* L, int N, real* x) {
+  for (int i = 0; i < N; ++i) { real s = x[i]; for (int k = 0; k < i; ++k) s -= L[i * N + k] * x[k]; x[i] = s / L[i * N + i]; }
+  for (int i = N - 1; i >= 0; --i) { real s = x[i]; for (int k = i + 1; k < N; ++k) s -= L[k * N + i] * x[k]; x[i] = s / L[i * N + i]; }
+}
+
+typedef struct { int iterations, factorizations, converged; real residual; } solve_stats;
+
+static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) {
+  int N = S->N;
+  real rel = (real)C->opt_rel_tol, abs_tol = (real)C->opt_abs_tol;
+  st->iterations = 0; st->factorizations = 0; st->converged = 0;
+  for (int it = 0; it <= C->max_iterations; ++it) {
+    real pn, jn;
+    eval_cost(S, v, S->g, S->gam, S->G, &pn, &jn);
+    real gn = 0;
+    for (int i = 0; i < N; ++i) { real s = S->g[i] / RS(S->Ad[i]); gn += s * s; }
+    gn = RS(gn);
+    st->residual = gn;
+    if (!(gn == gn)) return 0;
+    if (gn <= abs_tol + rel * maxr(pn, jn)) { st->converged = 1; return 1; }
+    if (it == C->max_iterations) break;
+    /* H = A + sum J^T G J (dense) */
+    memset(S->H, 0, sizeof(real) * N * N);
+    for (int b = 0; b < S->nb; ++b) {
+      int o = 6 * b;
+      S->H[o * N + o] = S->m[b]; S->H[(o + 1) * N + o + 1] = S->m[b]; S->H[(o + 2) * N + o + 2] = S->m[b];
+      for (int r = 0; r < 3; ++r) for (int q = 0; q < 3; ++q) S->H[(o + 3 + r) * N + o + 3 + q] = S->Iw[9 * b + 3 * r + q];
+    }
+    for (int c = 0; c < S->nc; ++c) {
+      const real* Gc = S->G + 9 * c;
+      for (int s1 = 0; s1 < 2; ++s1) {
+        int b1 = S->cb[2 * c + s1];
+        if (b1 < 0) continue;
+        const real* J1 = S->J + (2 * c + s1) * 18;
+        real GJ[18]; /* G * J1 (3x6) */
+        for (int r = 0; r < 3; ++r) for (int k = 0; k < 6; ++k) GJ[6 * r + k] = Gc[3 * r] * J1[k] + Gc[3 * r + 1] * J1[6 + k] + Gc[3 * r + 2] * J1[12 + k];
+        for (int s2 = 0; s2 < 2; ++s2) {
+          int b2 = S->cb[2 * c + s2];
+          if (b2 < 0) continue;
+          const real* J2 = S->J + (2 * c + s2) * 18;
+          for (int i = 0; i < 6; ++i) for (int k = 0; k < 6; ++k)
+            S->H[(6 * b2 + i) * N + 6 * b1 + k] += J2[i] * GJ[k] + J2[6 + i] * GJ[6 + k] + J2[12 + i] * GJ[12 + k];
+        }
+      }
+    }
+    if (!cholesky(S->H, N)) return 0;
+    st->factorizations++;
+    for (int i = 0; i < N; ++i) S->dv[i] = -S->g[i];
+    chol_solve(S->H, N, S->dv);
+    /* exact line search on [0, 1]: root of l'(alpha), cubic-seeded safeguarded Newton */
+    real l0, d0, dd0, l1, d1, dd1;
+    ls_eval(S, v, S->dv, 0, &l0, &d0, &dd0);
+    if (!(d0 < 0)) { st->iterations = it + 1; break; }
+    ls_eval(S, v, S->dv, 1, &l1, &d1, &dd1);
+    real alpha;
+    if (d1 <= 0) alpha = 1;
+    else {
+      real lo = 0, hi = 1;
+      double ac;
+      alpha = or_cubic_min(l0, d0, l1, d1, 1.0, &ac) ? (real)ac : clampr(-d0 / dd0, 0, 1);
+      if (!(alpha > lo && alpha < hi)) alpha = (real)0.5;
+      for (int k = 0; k < 100; ++k) {
+        real l, d, dd;
+        ls_eval(S, v, S->dv, alpha, &l, &d, &dd);
+        if (RA(d) <= (real)C->ls_rel_tol * RA(d0)) break;
+        if (d < 0) lo = alpha; else hi = alpha;
+        real an = dd > 0 ? alpha - d / dd : -1;
+        if (!(an > lo && an < hi)) an = (real)0.5 * (lo + hi);
+        if (hi - lo <= (real)1e-14 * hi) { alpha = an; break; }
+        alpha = an;
+      }
+    }
+    for (int i = 0; i < N; ++i) v[i] += alpha * S->dv[i];
+    st->iterations = it + 1;
+  }
+  return st->converged;
+}
+
+/* ---- per-world state views and integration ---- */
+typedef struct {
+  const or_model* M;
+  int w, b0, nb;
+} world_view;
+
+/* Build the SAP problem of one world for step h from pose q, velocity qd, wrench f. */
+static void sap_setup(sap_ctx* S, const world_view* W, const or_config* C, const double* q, const double* qd,
+                      const double* f, real h, const or_contact* cs, int nc) {
+  const or_model* M = W->M;
+  int nb = W->nb;
+  S->nb = nb; S->nc = nc; S->N = 6 * nb;
+  for (int b = 0; b < nb; ++b) {
+    int gb = W->b0 + b;
+    const double* bq = q + 7 * b;
+    qt r = {(real)bq[3], (real)bq[4], (real)bq[5], (real)bq[6]};
+    m33 R = qmat(r);
+    const double* Ib = M->body_inertia + 9 * gb;
+    const double* Ibi = M->body_inv_inertia + 9 * gb;
+    real m = (real)M->body_mass[gb];
+    S->m[b] = m;
+    /* Iw = R Ib R^T and its inverse R Ib^-1 R^T */
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+      real a = 0, ai = 0;
+      for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) {
+        a += R.m[3 * i + k] * (real)Ib[3 * k + l] * R.m[3 * j + l];
+        ai += R.m[3 * i + k] * (real)Ibi[3 * k + l] * R.m[3 * j + l];
+      }
+      S->Iw[9 * b + 3 * i + j] = a; S->Iwi[9 * b + 3 * i + j] = ai;
+    }
+    v3 com = V((real)M->body_com[3 * gb], (real)M->body_com[3 * gb + 1], (real)M->body_com[3 * gb + 2]);
+    v3 xc = add(V((real)bq[0], (real)bq[1], (real)bq[2]), mv(&R, com));
+    S->xc[3 * b] = xc.x; S->xc[3 * b + 1] = xc.y; S->xc[3 * b + 2] = xc.z;
+    /* free motion: v* = v0 + h (g + f/m),  w* = w0 + h Iw^-1 (tau - w0 x Iw w0) */
+    const double* vb = qd + 6 * b;
+    const double* fb = f + 6 * b;
+    real im = (real)M->body_inv_mass[gb];
+    for (int k = 0; k < 3; ++k) S->vstar[6 * b + k] = (real)vb[k] + h * ((real)fb[k] * im + (real)M->gravity[k] * (im > 0 ? 1 : 0));
+    v3 w0 = V((real)vb[3], (real)vb[4], (real)vb[5]);
+    v3 Iw0 = V(S->Iw[9 * b] * w0.x + S->Iw[9 * b + 1] * w0.y + S->Iw[9 * b + 2] * w0.z,
+               S->Iw[9 * b + 3] * w0.x + S->Iw[9 * b + 4] * w0.y + S->Iw[9 * b + 5] * w0.z,
+               S->Iw[9 * b + 6] * w0.x + S->Iw[9 * b + 7] * w0.y + S->Iw[9 * b + 8] * w0.z);
+    v3 tau = sub(V((real)fb[3], (real)fb[4], (real)fb[5]), cross(w0, Iw0));
+    const real* Ii = S->Iwi + 9 * b;
+    S->vstar[6 * b + 3] = w0.x + h * (Ii[0] * tau.x + Ii[1] * tau.y + Ii[2] * tau.z);
+    S->vstar[6 * b + 4] = w0.y + h * (Ii[3] * tau.x + Ii[4] * tau.y + Ii[5] * tau.z);
+    S->vstar[6 * b + 5] = w0.z + h * (Ii[6] * tau.x + Ii[7] * tau.y + Ii[8] * tau.z);
+    S->Ad[6 * b] = S->Ad[6 * b + 1] = S->Ad[6 * b + 2] = m;
+    S->Ad[6 * b + 3] = S->Iw[9 * b]; S->Ad[6 * b + 4] = S->Iw[9 * b + 4]; S->Ad[6 * b + 5] = S->Iw[9 * b + 8];
+  }
+  real tau_d = (real)C->tau_d;
+  for (int c = 0; c < nc; ++c) {
+    const or_contact* k = cs + c;
+    v3 n = V(k->n[0], k->n[1], k->n[2]);
+    /* tangent basis: t1 orthogonal to n using the axis least aligned with n */
+    v3 e = RA(n.x) < RA(n.y) ? (RA(n.x) < RA(n.z) ? V(1, 0, 0) : V(0, 0, 1)) : (RA(n.y) < RA(n.z) ? V(0, 1, 0) : V(0, 0, 1));
+    v3 t1 = nrm(cross(n, e));
+    v3 t2 = cross(n, t1);
+    v3 F[3] = {t1, t2, n};
+    v3 p = V(k->c[0], k->c[1], k->c[2]);
+    real wsum = 0;
+    for (int s = 0; s < 2; ++s) {
+      int gb = s == 0 ? k->body_a : k->body_b;
+      int b = gb < 0 ? -1 : gb - W->b0;
+      S->cb[2 * c + s] = b;
+      real* Jb = S->J + (2 * c + s) * 18;
+      if (b < 0) { memset(Jb, 0, sizeof(real) * 18); continue; }
+      real sg = s == 0 ? -1 : 1;
+      v3 r = sub(p, V(S->xc[3 * b], S->xc[3 * b + 1], S->xc[3 * b + 2]));
+      for (int row = 0; row < 3; ++row) {
+        v3 ang = cross(r, F[row]);
+        Jb[6 * row + 0] = sg * F[row].x; Jb[6 * row + 1] = sg * F[row].y; Jb[6 * row + 2] = sg * F[row].z;
+        Jb[6 * row + 3] = sg * ang.x; Jb[6 * row + 4] = sg * ang.y; Jb[6 * row + 5] = sg * ang.z;
+      }
+      /* trace(W_bb) = 3/m + |r|^2 tr(Iw^-1) - r^T Iw^-1 r */
+      const real* Ii = S->Iwi + 9 * b;
+      real tr = Ii[0] + Ii[4] + Ii[8];
+      v3 Ir = V(Ii[0] * r.x + Ii[1] * r.y + Ii[2] * r.z, Ii[3] * r.x + Ii[4] * r.y + Ii[5] * r.z,
+                Ii[6] * r.x + Ii[7] * r.y + Ii[8] * r.z);
+      wsum += 3 / S->m[b] + dot(r, r) * tr - dot(r, Ir);
+    }
+    real wdiag = wsum / 3;
+    real Rn_soft = 1 / (h * k->k * (h + tau_d));
+    real Rn_rigid = (real)(C->beta * C->beta / (4.0 * OR_PI * OR_PI)) * wdiag;
+    S->Rn[c] = maxr(Rn_rigid, Rn_soft);
+    S->Rt[c] = (real)C->sigma * wdiag;
+    S->vhat[c] = -k->phi0 / (h + tau_d);
+    S->mu[c] = k->mu;
+  }
+}
+
+static void sap_alloc(sap_ctx* S, int nb, int nc) {
+  int N = 6 * nb;
+  int ncc = nc > 0 ? nc : 1;
+  S->m = (real*)calloc(nb, sizeof(real)); S->Iw = (real*)calloc(9 * nb, sizeof(real));
+  S->Iwi = (real*)calloc(9 * nb, sizeof(real)); S->xc = (real*)calloc(3 * nb, sizeof(real));
+  S->vstar = (real*)calloc(N, sizeof(real)); S->Ad = (real*)calloc(N, sizeof(real));
+  S->cb = (int*)calloc(2 * ncc, sizeof(int)); S->J = (real*)calloc(36 * ncc, sizeof(real));
+  S->Rt = (real*)calloc(ncc, sizeof(real)); S->Rn = (real*)calloc(ncc, sizeof(real));
+  S->vhat = (real*)calloc(ncc, sizeof(real)); S->mu = (real*)calloc(ncc, sizeof(real));
+  S->H = (real*)calloc((size_t)N * N, sizeof(real)); S->g = (real*)calloc(N, sizeof(real));
+  S->dv = (real*)calloc(N, sizeof(real)); S->v = (real*)calloc(N, sizeof(real));
+  S->gam = (real*)calloc(3 * ncc, sizeof(real)); S->G = (real*)calloc(9 * ncc, sizeof(real));
+  S->tmp = (real*)calloc(4 * N, sizeof(real));
+  S->vc = NULL; S->dvc = NULL;
+}
+static void sap_free(sap_ctx* S) {
+  free(S->m); free(S->Iw); free(S->Iwi); free(S->xc); free(S->vstar); free(S->Ad); free(S->cb); free(S->J);
+  free(S->Rt); free(S->Rn); free(S->vhat); free(S->mu); free(S->H); free(S->g); free(S->dv); free(S->v);
+  free(S->gam); free(S->G); free(S->tmp);
+}
+
+/* Semi-implicit update (solver.py:50-94 without damping): x_com += h v,
+ * q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.  Always in double. */
+static void integrate_world(const world_view* W, const double* q, const real* v, double h, double* q_out, double* qd_out) {
+  const or_model* M = W->M;
+  for (int b = 0; b < W->nb; ++b) {
+    int gb = W->b0 + b;
+    const double* bq = q + 7 * b;
+    double qx = bq[3], qy = bq[4], qz = bq[5], qw = bq[6];
+    double cx = M->body_com[3 * gb], cy = M->body_com[3 * gb + 1], cz = M->body_com[3 * gb + 2];
+    /* COM position */
+    double R0[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw),
+                    2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw),
+                    2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)};
+    double xc[3];
+    for (int i = 0; i < 3; ++i) xc[i] = bq[i] + R0[3 * i] * cx + R0[3 * i + 1] * cy + R0[3 * i + 2] * cz;
+    const real* vb = v + 6 * b;
+    for (int i = 0; i < 3; ++i) xc[i] += h * (double)vb[i];
+    double wx = vb[3], wy = vb[4], wz = vb[5];
+    /* dq = (w, 0) * q */
+    double dx = wy * qz - wz * qy + wx * qw;
+    double dy = wz * qx - wx * qz + wy * qw;
+    double dz = wx * qy - wy * qx + wz * qw;
+    double dw = -(wx * qx + wy * qy + wz * qz);
+    double nx = qx + 0.5 * h * dx, ny = qy + 0.5 * h * dy, nz = qz + 0.5 * h * dz, nw = qw + 0.5 * h * dw;
+    double l = sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+    nx /= l; ny /= l; nz /= l; nw /= l;
+    double R1[9] = {1 - 2 * (ny * ny + nz * nz), 2 * (nx * ny - nz * nw), 2 * (nx * nz + ny * nw),
+                    2 * (nx * ny + nz * nw), 1 - 2 * (nx * nx + nz * nz), 2 * (ny * nz - nx * nw),
+                    2 * (nx * nz - ny * nw), 2 * (ny * nz + nx * nw), 1 - 2 * (nx * nx + ny * ny)};
+    double* o = q_out + 7 * b;
+    for (int i = 0; i < 3; ++i) o[i] = xc[i] - (R1[3 * i] * cx + R1[3 * i + 1] * cy + R1[3 * i + 2] * cz);
+    o[3] = nx; o[4] = ny; o[5] = nz; o[6] = nw;
+    for (int i = 0; i < 6; ++i) qd_out[6 * b + i] = vb[i];
+  }
+}
+
+/* One inner step (= CENIC ComputeNextContinuousState, solver_sap_adaptive.py:714-729):
+ * given contacts computed at the step's start pose, solve and integrate.
+ * Returns 1 when the convex solve converged. */
+static int inner_step(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
+                      double h, const or_contact* cs, int nc, const real* guess, double* q_out, double* qd_out,
+                      real* v_out, solve_stats* st) {
+  sap_ctx S;
+  sap_alloc(&S, W->nb, nc);
+  sap_setup(&S, W, C, q, qd, f, (real)h, cs, nc);
+  real* v = S.v;
+  for (int i = 0; i < S.N; ++i) v[i] = guess ? guess[i] : (real)qd[i];
+  int ok = sap_newton(&S, C, v, st);
+  integrate_world(W, q, v, h, q_out, qd_out);
+  if (v_out) memcpy(v_out, v, sizeof(real) * S.N);
+  sap_free(&S);
+  return ok;
+}
+
+/* Exported single-step entry: contacts generated at q, one solve + integrate of world w.
+ * q/qd/f/q_out/qd_out hold that world's bodies only. */
+int or_substep_world(const or_model* M, const or_config* C, int w, const double* q, const double* qd,
+                     const double* f, double h, double* q_out, double* qd_out, int* stats) {
+  world_view W = {M, w, M->world_body_start[w], M->world_body_start[w + 1] - M->world_body_start[w]};
+  int cap = C->max_contacts;
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
+  int np;
+  int nc = or_collide_world(M, w, q, cs, cap, &np);
+  if (nc > cap) nc = cap;
+  solve_stats st;
+  int ok = inner_step(&W, C, q, qd, f, h, cs, nc, NULL, q_out, qd_out, NULL, &st);
+  if (stats) { stats[0] = st.iterations; stats[1] = st.factorizations; stats[2] = st.converged; stats[3] = nc; }
+  free(cs);
+  return ok;
+}
+
+/* ====================================================================
+ * Step-doubling attempt and frame loop (solver_sap_adaptive.py:732-960,
+ * solver_mujoco_adaptive.py:692-849), FP64 controller.
+ * ==================================================================== */
+typedef struct {
+  double *dt, *dt_half, *ideal_dt, *sim_time, *next_time, *last_error, *accepted_error;
+  uint8_t *accepted, *commit, *diverged;
+  int32_t *substeps, *iterations, *solve_fail;
+  int64_t* cum_accepted;
+} or_ctrl;
+
+/* Attempt of one world from (q, qd): writes full/double results, returns error (sentinel on NaN). */
+static double attempt(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
+                      double dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, int* solve_ok,
+                      int* n_contacts) {
+  int nb = W->nb, N = 6 * nb;
+  int cap = C->max_contacts;
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
+  real* vfull = (real*)malloc(sizeof(real) * (N > 0 ? N : 1));
+  real* guess = (real*)malloc(sizeof(real) * (N > 0 ? N : 1));
+  double* q_mid = (double*)malloc(sizeof(double) * 7 * (nb > 0 ? nb : 1));
+  double* qd_mid = (double*)malloc(sizeof(double) * 6 * (nb > 0 ? nb : 1));
+  solve_stats st;
+  int ok = 1, np;
+  int nc = or_collide_world(W->M, W->w, q, cs, cap, &np);
+  if (nc > cap) nc = cap;
+  if (n_contacts) *n_contacts = nc;
+  ok &= inner_step(W, C, q, qd, f, dt, cs, nc, NULL, q_full, qd_full, vfull, &st);
+  double err = 0.0;
+  if (C->mode == 1) {
+    for (int i = 0; i < N; ++i) guess[i] = (real)0.5 * ((real)qd[i] + vfull[i]);
+    ok &= inner_step(W, C, q, qd, f, 0.5 * dt, cs, nc, guess, q_mid, qd_mid, NULL, &st);
+    nc = or_collide_world(W->M, W->w, q_mid, cs, cap, &np);
+    if (nc > cap) nc = cap;
+    ok &= inner_step(W, C, q_mid, qd_mid, f, 0.5 * dt, cs, nc, vfull, q_dbl, qd_dbl, NULL, &st);
+    for (int i = 0; i < 7 * nb; ++i) {
+      double d = fabs(q_dbl[i] - q_full[i]);
+      if (isnan(d)) { err = NAN; break; }
+      if (d > err) err = d;
+    }
+  } else {
+    memcpy(q_dbl, q_full, sizeof(double) * 7 * nb);
+    memcpy(qd_dbl, qd_full, sizeof(double) * 6 * nb);
+    for (int i = 0; i < 7 * nb; ++i) if (isnan(q_full[i])) err = NAN;
+  }
+  if (isnan(err) || isinf(err)) err = 1.0e10;
+  *solve_ok = ok;
+  free(cs); free(vfull); free(guess); free(q_mid); free(qd_mid);
+  return err;
+}
+
+/* Exported single attempt for parity tests: out_err[w], and full/double states of every world. */
+void or_attempt_all(const or_model* M, const or_config* C, const double* q, const double* qd, const double* f,
+                    const double* dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
+                    int32_t* solve_ok, int32_t* n_contacts) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int w = 0; w < M->world_count; ++w) {
+    int b0 = M->world_body_start[w];
+    world_view W = {M, w, b0, M->world_body_start[w + 1] - b0};
+    int ok, nc;
+    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, dt[w], q_full + 7 * b0, qd_full + 6 * b0,
+                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc);
+    solve_ok[w] = ok;
+    n_contacts[w] = nc;
+  }
+}
+
+/* Whole boundary call for every world: open frame, march attempts until the boundary.
+ * Per-world marching is exactly the lock-step loop of the reference (done worlds are no-ops).
+ * q/qd are updated in place.  Returns 0 ok, 2 if any inner solve failed (RuntimeError in
+ * the reference, solver_sap_adaptive.py:900-905). */
+int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, const double* f, or_ctrl* K,
+                  double dt_outer) {
+  int status = 0;
+  double eff_dt_max = C->dt_max < dt_outer ? C->dt_max : dt_outer;
+  int64_t cum = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : status) reduction(+ : cum)
+  for (int w = 0; w < M->world_count; ++w) {
+    int b0 = M->world_body_start[w], nb = M->world_body_start[w + 1] - b0;
+    world_view W = {M, w, b0, nb};
+    double *qw = q + 7 * b0, *qdw = qd + 6 * b0;
+    const double* fw = f + 6 * b0;
+    double* qf = (double*)malloc(sizeof(double) * 7 * (nb ? nb : 1));
+    double* qdf = (double*)malloc(sizeof(double) * 6 * (nb ? nb : 1));
+    double* qdb = (double*)malloc(sizeof(double) * 7 * (nb ? nb : 1));
+    double* qddb = (double*)malloc(sizeof(double) * 6 * (nb ? nb : 1));
+    K->iterations[w] = 0;
+    if (C->controller == 0) {
+      /* _open_frame + _seed_dt (solver_sap_adaptive.py:89-132) */
+      double base = K->next_time[w];
+      K->sim_time[w] = K->sim_time[w] - base;
+      K->next_time[w] = dt_outer;
+      double d = C->mode == 0 ? C->dt_init : K->ideal_dt[w];
+      d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+      K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+      K->substeps[w] = 0; K->diverged[w] = 0;
+    } else {
+      /* _apply_dt_cap, _rebase_time, _boundary_advance (solver_mujoco_adaptive.py:805-833) */
+      double d = K->ideal_dt[w];
+      d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+      K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+      double base = K->next_time[w];
+      K->sim_time[w] -= base; K->next_time[w] -= base;
+      K->next_time[w] += dt_outer;
+      K->diverged[w] = 0;
+    }
+    for (int it = 0; it < C->max_substeps; ++it) {
+      if (!(K->sim_time[w] < K->next_time[w])) break;
+      K->iterations[w]++;
+      /* _clamp_dt_to_boundary */
+      double rem = K->next_time[w] - K->sim_time[w];
+      if (rem <= 0) { K->dt[w] = 0; K->dt_half[w] = 0; }
+      else if (K->dt[w] > rem) { K->dt[w] = rem; K->dt_half[w] = 0.5 * rem; }
+      double step = K->dt[w];
+      int ok = 1, nc;
+      double err;
+      if (step > 0) err = attempt(&W, C, qw, qdw, fw, step, qf, qdf, qdb, qddb, &ok, &nc);
+      else { err = 0; memcpy(qdb, qw, sizeof(double) * 7 * nb); memcpy(qddb, qdw, sizeof(double) * 6 * nb);
+             memcpy(qf, qw, sizeof(double) * 7 * nb); memcpy(qdf, qdw, sizeof(double) * 6 * nb); }
+      if (!ok) { err = C->divergence_threshold; K->solve_fail[w] = 1; status = 2; }
+      K->last_error[w] = err;
+      int commit;
+      if (C->controller == 0) {
+        int32_t cum1 = 0;
+        or_adapt_dt_f64(1, &K->last_error[w], &K->sim_time[w], &K->next_time[w], &K->dt[w], &K->dt_half[w],
+                        &K->ideal_dt[w], &K->diverged[w], &K->accepted[w], &K->accepted_error[w], &K->substeps[w],
+                        &cum1, C->mode, C->tol, C->dt_min, eff_dt_max, C->divergence_threshold);
+        cum += cum1;
+        commit = K->accepted[w];
+      } else {
+        or_calc_adjusted_step_f64(1, &K->last_error[w], &K->dt[w], &K->ideal_dt[w], &K->accepted[w], &K->commit[w],
+                                  &K->diverged[w], C->tol, C->dt_min, C->divergence_threshold);
+        commit = K->commit[w];
+        if (K->accepted[w]) {
+          K->sim_time[w] += K->dt[w];
+          K->accepted_error[w] = err;
+          if (step > 0) { K->substeps[w] += 1; cum += 1; }
+        }
+        if (K->diverged[w]) K->sim_time[w] = K->next_time[w];
+        double d = K->ideal_dt[w];
+        d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+        K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+      }
+      if (commit) {
+        const double* sq = C->mode == 0 ? qf : qdb;
+        const double* sqd = C->mode == 0 ? qdf : qddb;
+        memcpy(qw, sq, sizeof(double) * 7 * nb);
+        memcpy(qdw, sqd, sizeof(double) * 6 * nb);
+      }
+      if (!ok) break;
+    }
+    free(qf); free(qdf); free(qdb); free(qddb);
+  }
+  K->cum_accepted[0] += cum;
+  return status;
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU oracle (``oracle/oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``, always as the checker, never as the thing measured
+or shipped.  The FP64 build (``liboracle64.so``) is the oracle; the FP32 build
+(``liboracle32.so``) of the same source is the CPU-baseline port.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+f32p = C.POINTER(C.c_float)
+u8p = C.POINTER(C.c_uint8)
+i64p = C.POINTER(C.c_int64)
+
+
+class OrModel(C.Structure):
+    _fields_ = [
+        ("world_count", C.c_int), ("body_count", C.c_int), ("shape_count", C.c_int), ("global_count", C.c_int),
+        ("world_body_start", i32p), ("world_shape_start", i32p), ("global_shapes", i32p),
+        ("body_com", f64p), ("body_mass", f64p), ("body_inv_mass", f64p), ("body_inertia", f64p),
+        ("body_inv_inertia", f64p),
+        ("shape_type", i32p), ("shape_body", i32p), ("shape_group", i32p), ("shape_world", i32p),
+        ("shape_xform", f64p), ("shape_scale", f64p), ("shape_margin", f64p), ("shape_gap", f64p),
+        ("shape_ke", f64p), ("shape_mu", f64p), ("shape_radius", f64p),
+        ("filter_count", C.c_int), ("filter_pairs", i32p),
+        ("gravity", C.c_double * 3),
+    ]
+
+
+class OrConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("controller", C.c_int),
+        ("tol", C.c_double), ("dt_init", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("divergence_threshold", C.c_double),
+        ("max_substeps", C.c_int), ("max_iterations", C.c_int), ("max_contacts", C.c_int),
+        ("opt_rel_tol", C.c_double), ("opt_abs_tol", C.c_double), ("tau_d", C.c_double), ("sigma", C.c_double),
+        ("beta", C.c_double), ("ls_rel_tol", C.c_double),
+    ]
+
+
+class OrCtrl(C.Structure):
+    _fields_ = [
+        ("dt", f64p), ("dt_half", f64p), ("ideal_dt", f64p), ("sim_time", f64p), ("next_time", f64p),
+        ("last_error", f64p), ("accepted_error", f64p),
+        ("accepted", u8p), ("commit", u8p), ("diverged", u8p),
+        ("substeps", i32p), ("iterations", i32p), ("solve_fail", i32p), ("cum_accepted", i64p),
+    ]
+
+
+def build() -> None:
+    """Compile both oracle builds with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+_LIBS: dict = {}
+
+
+def lib(precision: str = "f64"):
+    name = "liboracle64.so" if precision == "f64" else "liboracle32.so"
+    if name not in _LIBS:
+        path = os.path.join(_BUILD, name)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_collide_world_flat.restype = C.c_int
+        L.or_world_pairs_global.restype = C.c_int
+        L.or_collide_shapes.restype = C.c_int
+        L.or_substep_world.restype = C.c_int
+        L.or_step_frame.restype = C.c_int
+        L.or_cubic_min.restype = C.c_int
+        L.or_cubic_min.argtypes = [C.c_double] * 5 + [f64p]
+        L.or_num_threads.restype = C.c_int
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+class OracleModel:
+    """Host copy of a model's arrays in float64 (values rounded through float32, i.e. the
+    numbers the device sees) plus the ctypes descriptor pointing at them."""
+
+    def __init__(self, model):
+        h = model._host
+        f = lambda k: np.ascontiguousarray(np.asarray(h[k], dtype=np.float32).astype(np.float64))
+        i = lambda k: np.ascontiguousarray(np.asarray(h[k], dtype=np.int32))
+        self.keep = {
+            "world_body_start": i("world_body_start"), "world_shape_start": i("world_shape_start"),
+            "global_shapes": i("global_shapes") if len(h["global_shapes"]) else np.zeros(1, np.int32),
+            "body_com": f("body_com"), "body_mass": f("body_mass"), "body_inv_mass": f("body_inv_mass"),
+            "body_inertia": f("body_inertia"), "body_inv_inertia": f("body_inv_inertia"),
+            "shape_type": i("shape_type"), "shape_body": i("shape_body"), "shape_group": i("shape_collision_group"),
+            "shape_world": i("shape_world"), "shape_xform": f("shape_transform"), "shape_scale": f("shape_scale"),
+            "shape_margin": f("shape_margin"), "shape_gap": f("shape_gap"), "shape_ke": f("shape_material_ke"),
+            "shape_mu": f("shape_material_mu"), "shape_radius": f("shape_collision_radius"),
+            "filter_pairs": i("shape_collision_filter_pairs").reshape(-1) if len(h["shape_collision_filter_pairs"])
+            else np.zeros(2, np.int32),
+        }
+        k = self.keep
+        m = OrModel()
+        m.world_count = int(h["world_count"])
+        m.body_count = int(h["body_count"])
+        m.shape_count = int(h["shape_count"])
+        m.global_count = int(len(h["global_shapes"]))
+        for name in ("world_body_start", "world_shape_start", "global_shapes", "shape_type", "shape_body",
+                     "shape_group", "shape_world", "filter_pairs"):
+            setattr(m, name, _ptr(k[name], i32p))
+        for name in ("body_com", "body_mass", "body_inv_mass", "body_inertia", "body_inv_inertia", "shape_xform",
+                     "shape_scale", "shape_margin", "shape_gap", "shape_ke", "shape_mu", "shape_radius"):
+            setattr(m, name, _ptr(k[name], f64p))
+        m.filter_count = int(len(h["shape_collision_filter_pairs"]))
+        g = np.asarray(h["gravity"], dtype=np.float32).astype(np.float64)
+        m.gravity = (C.c_double * 3)(*g)
+        self.desc = m
+        self.world_count = m.world_count
+        self.body_count = m.body_count
+        self.shape_count = m.shape_count
+        self.world_body_start = k["world_body_start"]
+
+
+def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
+                max_substeps=16, max_iterations=30, max_contacts=128, opt_rel_tol=None, opt_abs_tol=1e-14,
+                tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10) -> OrConfig:
+    c = OrConfig()
+    c.mode = 1 if mode == "adaptive" else 0
+    c.controller = 0 if controller == "sap" else 1
+    c.tol, c.dt_init, c.dt_min, c.dt_max = tol, dt_init, dt_min, dt_max
+    c.divergence_threshold = 1.0e9
+    c.max_substeps, c.max_iterations, c.max_contacts = max_substeps, max_iterations, max_contacts
+    c.opt_rel_tol = opt_rel_tol if opt_rel_tol is not None else 1e-10
+    c.opt_abs_tol, c.tau_d, c.sigma, c.beta, c.ls_rel_tol = opt_abs_tol, tau_d, sigma, beta, ls_rel_tol
+    return c
+
+
+def collide_world(om: OracleModel, w: int, body_q: np.ndarray, cap: int = 128, precision="f64"):
+    """Contacts of world ``w`` at global pose ``body_q`` [B, 7]: (count, n_pairs, rows[count, 14])."""
+    q = np.ascontiguousarray(body_q, dtype=np.float64)
+    out = np.zeros((max(cap, 1), 14), dtype=np.float64)
+    npairs = C.c_int(0)
+    n = lib(precision).or_collide_world_flat(C.byref(om.desc), w, _ptr(q, f64p), _ptr(out, f64p), cap,
+                                             C.byref(npairs))
+    return n, npairs.value, out[:min(n, cap)]
+
+
+def world_pairs(om: OracleModel, w: int, lo: np.ndarray, hi: np.ndarray, cap: int = 1 << 16) -> np.ndarray:
+    lo = np.ascontiguousarray(lo, dtype=np.float64)
+    hi = np.ascontiguousarray(hi, dtype=np.float64)
+    out = np.zeros((cap, 2), dtype=np.int32)
+    n = lib().or_world_pairs_global(C.byref(om.desc), w, _ptr(lo, f64p), _ptr(hi, f64p), _ptr(out, i32p), cap)
+    return out[:min(n, cap)].copy()
+
+
+def shape_aabbs(om: OracleModel, body_q: np.ndarray):
+    L = lib()
+    q = np.ascontiguousarray(body_q, dtype=np.float64)
+    S = om.shape_count
+    lo = np.zeros((S, 3)); hi = np.zeros((S, 3))
+    for s in range(S):
+        L.or_shape_aabb(C.byref(om.desc), _ptr(q, f64p), 0, s, _ptr(lo[s], f64p), _ptr(hi[s], f64p))
+    return lo, hi
+
+
+def collide_shapes(ta, xa, sa, tb, xb, sb, gap_sum=0.0, precision="f64"):
+    out = np.zeros((4, 8))
+    xa = np.ascontiguousarray(xa, dtype=np.float64); xb = np.ascontiguousarray(xb, dtype=np.float64)
+    sa = np.ascontiguousarray(sa, dtype=np.float64); sb = np.ascontiguousarray(sb, dtype=np.float64)
+    n = lib(precision).or_collide_shapes(int(ta), _ptr(xa, f64p), _ptr(sa, f64p), int(tb), _ptr(xb, f64p),
+                                         _ptr(sb, f64p), C.c_double(gap_sum), _ptr(out, f64p))
+    return out[:n]
+
+
+def cubic_min(c0, d0, c1, d1, h):
+    a = C.c_double(0.0)
+    ok = lib().or_cubic_min(c0, d0, c1, d1, h, C.byref(a))
+    return a.value if ok else None
+
+
+def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, precision="f64"):
+    b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
+    q = np.ascontiguousarray(q[b0:b1], dtype=np.float64)
+    qd = np.ascontiguousarray(qd[b0:b1], dtype=np.float64)
+    f = np.ascontiguousarray(f[b0:b1], dtype=np.float64)
+    qo = np.zeros_like(q); qdo = np.zeros_like(qd)
+    stats = np.zeros(4, dtype=np.int32)
+    ok = lib(precision).or_substep_world(C.byref(om.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(qd, f64p),
+                                         _ptr(f, f64p), C.c_double(h), _ptr(qo, f64p), _ptr(qdo, f64p),
+                                         _ptr(stats, i32p))
+    return qo, qdo, stats, ok
+
+
+def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt, precision="f64"):
+    W, B = om.world_count, om.body_count
+    q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
+    qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
+    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32)
+    lib(precision).or_attempt_all(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
+                                  _ptr(dt, f64p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
+                                  _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p))
+    return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok, n_contacts=nc)
+
+
+class OracleSolver:
+    """Frame-level driver of the oracle with the reference solver's controller state."""
+
+    def __init__(self, om: OracleModel, cfg: OrConfig, precision="f64"):
+        self.om, self.cfg, self.precision = om, cfg, precision
+        W = om.world_count
+        self.a = {
+            "dt": np.full(W, cfg.dt_init), "dt_half": np.full(W, 0.5 * cfg.dt_init),
+            "ideal_dt": np.full(W, cfg.dt_init), "sim_time": np.zeros(W), "next_time": np.zeros(W),
+            "last_error": np.zeros(W), "accepted_error": np.zeros(W),
+            "accepted": np.zeros(W, np.uint8), "commit": np.zeros(W, np.uint8), "diverged": np.zeros(W, np.uint8),
+            "substeps": np.zeros(W, np.int32), "iterations": np.zeros(W, np.int32),
+            "solve_fail": np.zeros(W, np.int32), "cum_accepted": np.zeros(1, np.int64),
+        }
+        k = OrCtrl()
+        for name, arr in self.a.items():
+            t = {np.float64: f64p, np.uint8: u8p, np.int32: i32p, np.int64: i64p}[arr.dtype.type]
+            setattr(k, name, _ptr(arr, t))
+        self.ctrl = k
+
+    def step(self, q: np.ndarray, qd: np.ndarray, f: np.ndarray, dt_outer: float) -> int:
+        assert q.dtype == np.float64 and q.flags.c_contiguous and qd.dtype == np.float64 and qd.flags.c_contiguous
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        return lib(self.precision).or_step_frame(C.byref(self.om.desc), C.byref(self.cfg), _ptr(q, f64p),
+                                                 _ptr(qd, f64p), _ptr(f, f64p), C.byref(self.ctrl),
+                                                 C.c_double(dt_outer))
+
+
+def controller_lib():
+    """Handle for the float32 reference-semantics controller functions."""
+    L = lib()
+    return L
