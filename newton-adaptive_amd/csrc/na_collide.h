@@ -212,20 +212,24 @@ __device__ void capsule_box(f3 cp, f3 ax, float r, float hl, f3 bp, const m3& R,
   f3 gs;
   float fs = box_sdf(p0 + d * ts, h, gs);
   float dl = len3(d);
-  bool two = dl > 0.f && fabsf(dot3(d, gs)) < NA_CAPBOX_PARALLEL * dl;
-  int cnt = two ? 2 : 1;
-  for (int k = 0; k < cnt; ++k) {
-    float t = two ? (float)k : ts;
+  // the segment minimum always; both endpoints too when the axis lies along the face
+  bool par = dl > 0.f && fabsf(dot3(d, gs)) < NA_CAPBOX_PARALLEL * dl;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float t = k == 0 ? ts : (float)(k - 1);
+    if (k > 0 && (!par || fabsf(t - ts) <= 0.05f)) continue;
     f3 pk = p0 + d * t;
     f3 gk = gs;
     float fk = fs;
-    if (two) fk = box_sdf(pk, h, gk);
+    if (k > 0) fk = box_sdf(pk, h, gk);
     f3 nl = -gk;
     float dist = fk - r;
     f3 mid = pk + nl * (r + 0.5f * dist);
-    m.dist[k] = dist;
-    m.pos[k] = bp + mv(R, mid);
-    m.nrm[k] = mv(R, nl);
+    if (cnt == 0) { m.dist[0] = dist; m.pos[0] = bp + mv(R, mid); m.nrm[0] = mv(R, nl); }
+    else if (cnt == 1) { m.dist[1] = dist; m.pos[1] = bp + mv(R, mid); m.nrm[1] = mv(R, nl); }
+    else { m.dist[2] = dist; m.pos[2] = bp + mv(R, mid); m.nrm[2] = mv(R, nl); }
+    cnt++;
   }
 }
 

@@ -431,27 +431,26 @@ static void collide_capsule_box(v3 cpos, v3 axis, real r, real hl, v3 bpos, cons
     else { a = c1; c1 = c2; f1 = f2; c2 = a + gr * (b - a); f2 = box_sdf(add(p0, scl(d, c2)), size, &g); }
   }
   real ts = (real)0.5 * (a + b);
-  v3 ps = add(p0, scl(d, ts));
   v3 gs;
-  real fs = box_sdf(ps, size, &gs);
+  real fs = box_sdf(add(p0, scl(d, ts)), size, &gs);
   real dl = len(d);
-  int two = dl > 0 && RA(dot(d, gs)) < (real)CAPBOX_PARALLEL * dl;
-  real ts_list[2];
-  int cnt;
-  if (two) { ts_list[0] = 0; ts_list[1] = 1; cnt = 2; }
-  else { ts_list[0] = ts; cnt = 1; }
-  for (int k = 0; k < cnt; ++k) {
-    v3 pk = add(p0, scl(d, ts_list[k]));
-    v3 gk;
-    real fk = two ? box_sdf(pk, size, &gk) : fs;
-    if (!two) gk = gs;
+  /* the segment minimum always; both endpoints too when the axis lies along the face */
+  int par = dl > 0 && RA(dot(d, gs)) < (real)CAPBOX_PARALLEL * dl;
+  real tl[3] = {ts, 0, 1};
+  int cnt = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (k > 0 && (!par || RA(tl[k] - ts) <= (real)0.05)) continue;
+    v3 pk = add(p0, scl(d, tl[k]));
+    v3 gk = gs;
+    real fk = k == 0 ? fs : box_sdf(pk, size, &gk);
     /* normal from capsule (A) to box (B) is -grad; midpoint between surfaces */
     v3 nl = scl(gk, -1);
     real dist = fk - r;
     v3 mid = add(pk, scl(nl, r + (real)0.5 * dist));
-    m->dist[k] = dist;
-    m->pos[k] = add(bpos, mv(rot, mid));
-    m->normal[k] = mv(rot, nl);
+    m->dist[cnt] = dist;
+    m->pos[cnt] = add(bpos, mv(rot, mid));
+    m->normal[cnt] = mv(rot, nl);
+    cnt++;
   }
 }
 
@@ -1240,7 +1239,7 @@ static int inner_step(const world_view* W, const or_config* C, const double* q, 
 /* Exported single-step entry: contacts generated at q, one solve + integrate of world w.
  * q/qd/f/q_out/qd_out hold that world's bodies only. */
 int or_substep_world(const or_model* M, const or_config* C, int w, const double* q, const double* qd,
-                     const double* f, double h, double* q_out, double* qd_out, int* stats) {
+                     const double* f, double h, const double* guess, double* q_out, double* qd_out, int* stats) {
   world_view W = {M, w, M->world_body_start[w], M->world_body_start[w + 1] - M->world_body_start[w]};
   int cap = C->max_contacts;
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
@@ -1248,7 +1247,13 @@ int or_substep_world(const or_model* M, const or_config* C, int w, const double*
   int nc = or_collide_world(M, w, q, cs, cap, &np);
   if (nc > cap) nc = cap;
   solve_stats st;
-  int ok = inner_step(&W, C, q, qd, f, h, cs, nc, NULL, q_out, qd_out, NULL, &st);
+  real* g = NULL;
+  if (guess) {
+    g = (real*)malloc(sizeof(real) * 6 * (W.nb ? W.nb : 1));
+    for (int i = 0; i < 6 * W.nb; ++i) g[i] = (real)guess[i];
+  }
+  int ok = inner_step(&W, C, q, qd, f, h, cs, nc, g, q_out, qd_out, NULL, &st);
+  free(g);
   if (stats) { stats[0] = st.iterations; stats[1] = st.factorizations; stats[2] = st.converged; stats[3] = nc; }
   free(cs);
   return ok;

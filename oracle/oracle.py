@@ -1,9 +1,9 @@
 """ctypes binding of the CPU oracle (``oracle/oracle.c``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py``, always as the checker, never as the thing measured
-or shipped.  The FP64 build (``liboracle64.so``) is the oracle; the FP32 build
-(``liboracle32.so``) of the same source is the CPU-baseline port.
+``cpu_baseline`` leg of ``bench.py``, always as the checker, never as the product path.
+``liboracle64.so`` is the FP64 restatement; timed on the host it is also the CPU baseline
+("port").
 """
 
 from __future__ import annotations
@@ -70,8 +70,8 @@ def _ptr(a, t):
 _LIBS: dict = {}
 
 
-def lib(precision: str = "f64"):
-    name = "liboracle64.so" if precision == "f64" else "liboracle32.so"
+def lib():
+    name = "liboracle64.so"
     if name not in _LIBS:
         path = os.path.join(_BUILD, name)
         if not os.path.exists(path):
@@ -145,12 +145,12 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     return c
 
 
-def collide_world(om: OracleModel, w: int, body_q: np.ndarray, cap: int = 128, precision="f64"):
+def collide_world(om: OracleModel, w: int, body_q: np.ndarray, cap: int = 128):
     """Contacts of world ``w`` at global pose ``body_q`` [B, 7]: (count, n_pairs, rows[count, 14])."""
     q = np.ascontiguousarray(body_q, dtype=np.float64)
     out = np.zeros((max(cap, 1), 14), dtype=np.float64)
     npairs = C.c_int(0)
-    n = lib(precision).or_collide_world_flat(C.byref(om.desc), w, _ptr(q, f64p), _ptr(out, f64p), cap,
+    n = lib().or_collide_world_flat(C.byref(om.desc), w, _ptr(q, f64p), _ptr(out, f64p), cap,
                                              C.byref(npairs))
     return n, npairs.value, out[:min(n, cap)]
 
@@ -173,11 +173,11 @@ def shape_aabbs(om: OracleModel, body_q: np.ndarray):
     return lo, hi
 
 
-def collide_shapes(ta, xa, sa, tb, xb, sb, gap_sum=0.0, precision="f64"):
+def collide_shapes(ta, xa, sa, tb, xb, sb, gap_sum=0.0):
     out = np.zeros((4, 8))
     xa = np.ascontiguousarray(xa, dtype=np.float64); xb = np.ascontiguousarray(xb, dtype=np.float64)
     sa = np.ascontiguousarray(sa, dtype=np.float64); sb = np.ascontiguousarray(sb, dtype=np.float64)
-    n = lib(precision).or_collide_shapes(int(ta), _ptr(xa, f64p), _ptr(sa, f64p), int(tb), _ptr(xb, f64p),
+    n = lib().or_collide_shapes(int(ta), _ptr(xa, f64p), _ptr(sa, f64p), int(tb), _ptr(xb, f64p),
                                          _ptr(sb, f64p), C.c_double(gap_sum), _ptr(out, f64p))
     return out[:n]
 
@@ -188,26 +188,29 @@ def cubic_min(c0, d0, c1, d1, h):
     return a.value if ok else None
 
 
-def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, precision="f64"):
+def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, guess=None):
+    """One inner step of world ``w`` (contacts at q): returns (q_out, qd_out, stats, ok) with
+    stats = (newton iterations, factorizations, converged, contacts)."""
     b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
     q = np.ascontiguousarray(q[b0:b1], dtype=np.float64)
     qd = np.ascontiguousarray(qd[b0:b1], dtype=np.float64)
     f = np.ascontiguousarray(f[b0:b1], dtype=np.float64)
+    g = None if guess is None else np.ascontiguousarray(np.asarray(guess, dtype=np.float64).reshape(-1))
     qo = np.zeros_like(q); qdo = np.zeros_like(qd)
     stats = np.zeros(4, dtype=np.int32)
-    ok = lib(precision).or_substep_world(C.byref(om.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(qd, f64p),
-                                         _ptr(f, f64p), C.c_double(h), _ptr(qo, f64p), _ptr(qdo, f64p),
-                                         _ptr(stats, i32p))
+    ok = lib().or_substep_world(C.byref(om.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
+                                C.c_double(h), _ptr(g, f64p) if g is not None else None, _ptr(qo, f64p),
+                                _ptr(qdo, f64p), _ptr(stats, i32p))
     return qo, qdo, stats, ok
 
 
-def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt, precision="f64"):
+def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt):
     W, B = om.world_count, om.body_count
     q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
     qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
     err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32)
-    lib(precision).or_attempt_all(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
+    lib().or_attempt_all(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
                                   _ptr(dt, f64p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
                                   _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p))
     return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok, n_contacts=nc)
@@ -216,8 +219,8 @@ def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt, precision="f64"):
 class OracleSolver:
     """Frame-level driver of the oracle with the reference solver's controller state."""
 
-    def __init__(self, om: OracleModel, cfg: OrConfig, precision="f64"):
-        self.om, self.cfg, self.precision = om, cfg, precision
+    def __init__(self, om: OracleModel, cfg: OrConfig):
+        self.om, self.cfg = om, cfg
         W = om.world_count
         self.a = {
             "dt": np.full(W, cfg.dt_init), "dt_half": np.full(W, 0.5 * cfg.dt_init),
@@ -236,12 +239,8 @@ class OracleSolver:
     def step(self, q: np.ndarray, qd: np.ndarray, f: np.ndarray, dt_outer: float) -> int:
         assert q.dtype == np.float64 and q.flags.c_contiguous and qd.dtype == np.float64 and qd.flags.c_contiguous
         f = np.ascontiguousarray(f, dtype=np.float64)
-        return lib(self.precision).or_step_frame(C.byref(self.om.desc), C.byref(self.cfg), _ptr(q, f64p),
+        return lib().or_step_frame(C.byref(self.om.desc), C.byref(self.cfg), _ptr(q, f64p),
                                                  _ptr(qd, f64p), _ptr(f, f64p), C.byref(self.ctrl),
                                                  C.c_double(dt_outer))
 
 
-def controller_lib():
-    """Handle for the float32 reference-semantics controller functions."""
-    L = lib()
-    return L

@@ -1,0 +1,171 @@
+"""GPU step-doubling path against the FP64 oracle (tolerances are stated per check).
+
+* one inner step per world (``na_substep``): pose within 1e-5 (relative to max(|q|, 1)),
+  velocity within 1e-5 (relative to max(|v|, 1) m/s);
+* one step-doubling attempt (``na_attempt``): accepted-step error norms within 1e-6 absolute,
+  doubled pose within 1e-5;
+* boundary calls: every world lands on dt_outer; repeated runs are bit-identical; the lock-step
+  form (``na_step_plain``) and the persistent served form give bit-identical states (same
+  per-world arithmetic); fixed mode does constant work; tighter tolerance does more work.
+"""
+
+import importlib
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+GE = importlib.import_module("__graft_entry__")
+
+
+def _settle(na, model, frames, **kw):
+    state = model.state()
+    solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=128, **kw)
+    for _ in range(frames):
+        solver.step(state, state, model.control(), None, 0.01)
+    torch.cuda.synchronize()
+    return state, solver
+
+
+def _substep(nalib, solver, q64, qd, f, h):
+    W, B = solver.model.world_count, solver.model.body_count
+    qo = torch.zeros((B, 7), dtype=torch.float64, device=DEV)
+    vo = torch.zeros((B, 6), dtype=torch.float32, device=DEV)
+    st = torch.zeros((W, 4), dtype=torch.int32, device=DEV)
+    hh = torch.full((W,), h, dtype=torch.float64, device=DEV)
+    nalib.check(nalib.lib().na_substep(solver._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(),
+                                       qo.data_ptr(), vo.data_ptr(), st.data_ptr(), nalib.stream_handle()), "substep")
+    torch.cuda.synchronize()
+    return qo.cpu().numpy(), vo.cpu().numpy(), st.cpu().numpy()
+
+
+SCENES = [("drop", 25), ("drop", 40), ("stack", 3), ("stack", 30), ("pile", 20), ("pile", 60)]
+
+
+def _model(na, name):
+    if name == "drop":
+        return na.scenes.box_drop(64, seed=4, device=DEV)
+    if name == "stack":
+        return na.scenes.box_stack(64, seed=5, device=DEV)
+    return na.scenes.uneven_pile(64, seed=6, device=DEV)
+
+
+@pytest.mark.parametrize("name,frames", SCENES)
+def test_substep_matches_oracle(na, O, nalib, name, frames):
+    model = _model(na, name)
+    state, solver = _settle(na, model, frames)
+    q64 = solver.body_q64.clone()
+    qd = state.body_qd.clone()
+    f = torch.zeros_like(qd)
+    qo, vo, st = _substep(nalib, solver, q64, qd, f, 2e-3)
+    assert st[:, 2].min() == 1, "GPU solve did not converge"
+    om = O.OracleModel(model)
+    cfg = O.make_config(tol=1e-3)
+    qn, qdn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64)
+    for w in range(model.world_count):
+        b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
+        rq, rv, rst, ok = O.substep_world(om, cfg, w, qn, qdn, np.zeros_like(qdn), 2e-3)
+        assert ok and rst[3] == st[w, 3], f"world {w}: contacts {st[w, 3]} vs {rst[3]}"
+        dq = np.abs(qo[b0:b1] - rq).max() / max(np.abs(rq).max(), 1.0)
+        dv = np.abs(vo[b0:b1] - rv).max() / max(np.abs(rv).max(), 1.0)
+        assert dq <= 1e-5 and dv <= 1e-5, f"world {w}: dq={dq:.2e} dv={dv:.2e}"
+
+
+@pytest.mark.parametrize("name,frames", SCENES)
+def test_attempt_error_norm_matches_oracle(na, O, name, frames):
+    model = _model(na, name)
+    state, solver = _settle(na, model, frames)
+    q64 = solver.body_q64.clone()
+    qd = state.body_qd.clone()
+    f = torch.zeros_like(qd)
+    for dt in (1e-3, 4e-3):
+        dtt = torch.full((model.world_count,), dt, dtype=torch.float64, device=DEV)
+        out = GE._attempt(solver, q64, qd, f, dtt)
+        ref = O.attempt_all(O.OracleModel(model), O.make_config(tol=1e-3), q64.cpu().numpy(),
+                            qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy())
+        assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
+        same_nc = out["n_contacts"] == ref["n_contacts"]
+        assert same_nc.mean() > 0.95
+        de = np.abs(out["err"] - ref["err"])[same_nc]
+        assert de.max() <= 1e-6, f"max |d err| = {de.max():.2e}"
+        dq = np.abs(out["q_double"] - ref["q_double"]).max(axis=1)
+        assert dq.max() <= 1e-5
+
+
+def test_boundary_determinism_and_plain_equals_served(na):
+    runs = {}
+    for schedule in ("served", "plain", "served"):
+        model = na.scenes.uneven_pile(128, seed=9, device=DEV, max_prims=12)
+        state = model.state()
+        solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256,
+                                      schedule=schedule)
+        for _ in range(6):
+            solver.step(state, state, model.control(), None, 1.0 / 60.0)
+            torch.cuda.synchronize()
+            st = solver.sim_time.cpu().numpy()
+            assert np.allclose(st, 1.0 / 60.0, rtol=1e-12, atol=1e-15)
+        q = state.body_q.cpu().numpy()
+        assert np.all(np.isfinite(q))
+        # FP64 shadow rounds to the caller's float32 pose
+        assert np.array_equal(solver.body_q64.float().cpu().numpy(), q)
+        key = schedule if schedule not in runs else schedule + "2"
+        runs[key] = (q, state.body_qd.cpu().numpy(), solver.accepted_steps())
+    assert np.array_equal(runs["served"][0], runs["served2"][0])
+    assert np.array_equal(runs["served"][0], runs["plain"][0])
+    assert np.array_equal(runs["served"][1], runs["plain"][1])
+    assert runs["served"][2] == runs["plain"][2]
+
+
+def test_fixed_mode_constant_work_and_tol_ordering(na):
+    model = na.scenes.box_drop(32, seed=1, device=DEV)
+    state = model.state()
+    fixed = na.SolverSAPAdaptive(model, mode="fixed", dt_inner_init=0.0025, max_substeps=64)
+    work = []
+    for _ in range(10):
+        fixed.step(state, state, model.control(), None, 0.01)
+        work.append(int(fixed.substeps.sum().item()))
+    assert len(set(work)) == 1 and work[0] == 32 * 4
+    acc = {}
+    for tol in (1e-2, 1e-5):
+        m = na.scenes.box_drop(32, seed=1, device=DEV)
+        s = m.state()
+        sol = na.SolverSAPAdaptive(m, tol=tol, dt_inner_init=0.01, max_substeps=256)
+        for _ in range(40):
+            sol.step(s, s, m.control(), None, 0.01)
+        acc[tol] = sol.accepted_steps()
+    assert acc[1e-5] >= acc[1e-2]
+
+
+def test_mujoco_variant_controller_runs(na):
+    model = na.scenes.box_drop(16, seed=2, device=DEV)
+    state = model.state()
+    solver = na.SolverMuJoCoAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01)
+    for _ in range(60):
+        solver.step(state, state, model.control(), None, 0.01)
+    torch.cuda.synchronize()
+    q = state.body_q.cpu().numpy()
+    assert np.all(np.isfinite(q)) and np.all(q[:, 2] > 0.03)
+    assert np.allclose(solver.sim_time.cpu().numpy(), 0.01, rtol=1e-12)
+    assert not bool(solver.diverged.any())
+
+
+def test_reset_masked_and_invalid_args(na):
+    model = na.scenes.box_drop(8, seed=2, device=DEV)
+    state = model.state()
+    solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3)
+    solver.step(state, state, model.control(), None, 0.01)
+    mask = torch.tensor([1, 0] * 4, dtype=torch.bool, device=DEV)
+    solver.reset(state, world_mask=mask)
+    torch.cuda.synchronize()
+    st = solver.sim_time.cpu().numpy()
+    assert np.all(st[::2] == 0.0) and np.all(st[1::2] > 0.0)
+    assert np.all(solver.dt.cpu().numpy()[::2] == 1e-3)
+    with pytest.raises(ValueError):
+        solver.step(state, state, model.control(), None, None)
+    with pytest.raises(ValueError):
+        na.SolverSAPAdaptive(model, mode="bogus")
+    with pytest.raises(ValueError):
+        na.SolverSAPAdaptive(model, tol=0.0)
