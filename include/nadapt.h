@@ -85,7 +85,9 @@ typedef struct na_telemetry {
   int32_t* solve_fail;
   int32_t* overflow;           /* contact/pair capacity overflows */
   unsigned long long* cum_accepted; /* [1] */
-  long long* work;             /* [W,6]: attempts, newton iters, cg iters, contacts, pairs, solves */
+  long long* work;             /* [W,12]: attempts, newton iters, cg iters, contacts, pairs, solves,
+                                  then cycles of collision, setup, residual, preconditioner, PCG,
+                                  line search (zero unless built with -DNA_PHASE_TIMERS) */
   double* body_q64;            /* [B,7] FP64 pose accumulator */
 } na_telemetry;
 
@@ -144,7 +146,9 @@ int na_collide_debug(const na_model* m, const float* body_q, float* aabb_lo, flo
                      int32_t* pair_count, int32_t max_pairs, int32_t broad_phase, void* stream);
 
 /* One inner step per world (contacts at q, convex solve at h[w], FP64 integration).
- * stats [W,4]: newton iterations, factorizations, converged, contacts. */
+ * stats [W,6] int32: Newton iterations, CG iterations, converged, contacts, exit reason
+ * (0 converged, 1 iteration cap, 2 no descent, 3 preconditioner not SPD, 4 NaN), final
+ * relative residual (float bits). */
 int na_substep(const na_solver* s, const double* q64, const float* qd, const float* f, const double* h,
                double* q64_out, float* qd_out, int32_t* stats, void* stream);
 /* One step-doubling attempt per world from (q64, qd) with dt[w]: full and doubled results,

@@ -68,14 +68,16 @@ struct ContactLDS {
 };
 
 // ---------------------------------------------------------------- primitives
+// up to 8 contacts per pair: box-box keeps its whole clipped face polygon
+#define NA_MF 8
 struct Manifold {
-  float dist[4];
-  f3 pos[4];
-  f3 nrm[4];
+  float dist[NA_MF];
+  f3 pos[NA_MF];
+  f3 nrm[NA_MF];
 };
 __device__ __forceinline__ void mf_clear(Manifold& m) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) m.dist[i] = NA_MAXVAL;
+  for (int i = 0; i < NA_MF; ++i) m.dist[i] = NA_MAXVAL;
 }
 
 __device__ __forceinline__ f3 closest_segment_point(f3 a, f3 b, f3 pt) {
@@ -318,37 +320,11 @@ __device__ void box_box(f3 pa, const m3& Ra, f3 ha, f3 pb, const m3& Rb, f3 hb, 
       pp[kept] = poly[i] - nref * (0.5f * di);
       kept++;
     }
-    int sel[4];
-    int ns = 0;
-    if (kept <= 4) {
-      for (int i = 0; i < kept; ++i) sel[ns++] = i;
-    } else {
-      int i0 = 0;
-      for (int i = 1; i < kept; ++i)
-        if (dd[i] < dd[i0]) i0 = i;
-      int i1 = -1;
-      float far = -1.f;
-      for (int i = 0; i < kept; ++i) {
-        float q = dot3(pp[i] - pp[i0], pp[i] - pp[i0]);
-        if (i != i0 && q > far) { far = q; i1 = i; }
-      }
-      int i2 = -1, i3 = -1;
-      float amax = 0.f, amin = 0.f;
-      f3 e01 = pp[i1] - pp[i0];
-      for (int i = 0; i < kept; ++i) {
-        if (i == i0 || i == i1) continue;
-        float ar = dot3(cross3(e01, pp[i] - pp[i0]), nref);
-        if (ar > amax) { amax = ar; i2 = i; }
-        if (ar < amin) { amin = ar; i3 = i; }
-      }
-      sel[ns++] = i0;
-      sel[ns++] = i1;
-      if (i2 >= 0) sel[ns++] = i2;
-      if (i3 >= 0) sel[ns++] = i3;
-    }
-    for (int i = 0; i < ns; ++i) {
-      m.dist[i] = dd[sel[i]];
-      m.pos[i] = pp[sel[i]];
+    // the whole clipped polygon is the manifold (continuous in the poses; a reduction to 4
+    // points would switch between near-equal-depth candidates)
+    for (int i = 0; i < kept; ++i) {
+      m.dist[i] = dd[i];
+      m.pos[i] = pp[i];
       m.nrm[i] = n;
     }
   } else {
@@ -595,7 +571,7 @@ struct PairOut {
   int mask;    // bit k set when manifold slot k passed the gap check
   int sa, sb;  // segment positions after type ordering
   Manifold m;
-  float phi0[4];
+  float phi0[NA_MF];
   float ra, rb;
 };
 __device__ void narrow_pair(const NaModel& M, const SegLDS& S, int i, int j, PairOut& o) {
@@ -652,7 +628,7 @@ __device__ void narrow_pair(const NaModel& M, const SegLDS& S, int i, int j, Pai
     return;
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < NA_MF; ++k) {
     if (!(m.dist[k] < NA_MAXVAL)) continue;
     f3 n = nrm3(m.nrm[k]);
     f3 qa = m.pos[k] - n * (0.5f * m.dist[k] + ra);
@@ -689,7 +665,7 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
       float mu = fmaxf(M.shape_mu[ga], M.shape_mu[gb]);
       int slot = nc + excl;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NA_MF; ++q) {
         if (!((o.mask >> q) & 1)) continue;
         if (slot < cap) {
           C.ba[slot] = (int8_t)(ba < 0 ? -1 : ba - b0);

@@ -296,10 +296,23 @@ __device__ __forceinline__ double cubic_min_d(double c0, double d0, double c1, d
   return a;
 }
 
+// exit reasons of the Newton loop
+enum { NA_EXIT_CONVERGED = 0, NA_EXIT_MAX_ITER = 1, NA_EXIT_NOT_DESCENT = 2, NA_EXIT_CHOL = 3, NA_EXIT_NAN = 4 };
 struct SolveStats {
-  int iterations, factorizations, cg_iterations;
+  int iterations, factorizations, cg_iterations, reason;
   bool converged;
+  float residual;  // |D g| / max(|D p|, |D J^T gamma|) at exit
+  long long t_eval, t_pre, t_cg, t_ls;  // shader-clock cycles per phase (NA_PHASE_TIMERS builds)
 };
+
+// Phase cycle counters: compiled in only for diagnostic builds (-DNA_PHASE_TIMERS).
+#ifdef NA_PHASE_TIMERS
+#define NA_TIC(v) long long v = clock64()
+#define NA_TOC(dst, v) (dst) += clock64() - (v)
+#else
+#define NA_TIC(v)
+#define NA_TOC(dst, v)
+#endif
 
 // Set up the per-contact data of a step h: frame, lever arms, regularisation, v_hat.
 // Requires B.x, B.iwi, B.im filled for the step's start pose.
@@ -410,8 +423,12 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
   st.factorizations = 0;
   st.cg_iterations = 0;
   st.converged = false;
+  st.reason = NA_EXIT_MAX_ITER;
+  st.residual = 0.f;
+  st.t_eval = st.t_pre = st.t_cg = st.t_ls = 0;
   for (int it = 0; it <= cfg.max_iterations; ++it) {
     // gradient g = A (v - v*) - J^T gamma, FP64
+    NA_TIC(t0);
     if (own) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) B.vd[k][l] = v[k];
@@ -445,14 +462,21 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     jn = sqrt(wave_sum_d(jn));
     const double mom0 = wave_sum_d(mom);
     const double l0 = mom0 + ccost;
-    if (!(gn == gn)) return false;
+    NA_TOC(st.t_eval, t0);
+    st.residual = (float)(gn / fmax(fmax(pn, jn), 1e-300));
+    if (!(gn == gn)) {
+      st.reason = NA_EXIT_NAN;
+      return false;
+    }
     if (gn <= (double)cfg.opt_abs_tol + (double)cfg.opt_rel_tol * fmax(pn, jn)) {
       st.converged = true;
+      st.reason = NA_EXIT_CONVERGED;
       return true;
     }
     if (it == cfg.max_iterations) break;
 
     // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32)
+    NA_TIC(t1);
     for (int i = l; i < nb; i += NA_WAVE) {
 #pragma unroll
       for (int k = 0; k < 21; ++k) B.pre[k][i] = 0.f;
@@ -477,10 +501,15 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       L[13] += br.I[3]; L[18] += br.I[4]; L[19] += br.I[5];
       spd = chol6(L);
     }
-    if (__any(!spd)) return false;
+    if (__any(!spd)) {
+      st.reason = NA_EXIT_CHOL;
+      return false;
+    }
     st.factorizations++;
+    NA_TOC(st.t_pre, t1);
 
     // PCG on H x = -g in f32, right-hand side scaled to O(1)
+    NA_TIC(t2);
     const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
     float x[6], r[6], z[6], pp[6];
     float rz = 0.f;
@@ -527,6 +556,8 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     double dx[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) dx[q] = own ? (double)x[q] / gscale : 0.0;
+    NA_TOC(st.t_cg, t2);
+    NA_TIC(t3);
 
     // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a)
     if (own) {
@@ -553,7 +584,11 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     ls_contacts(nc, C, X, 0.0, lc, dlc, d2c);
     const double dl0 = b0 + dlc;
     const double d2l0 = aa + d2c;
-    if (!(dl0 < 0.0)) { st.iterations = it + 1; break; }
+    if (!(dl0 < 0.0)) {
+      st.iterations = it + 1;
+      st.reason = NA_EXIT_NOT_DESCENT;
+      break;
+    }
     ls_contacts(nc, C, X, 1.0, lc, dlc, d2c);
     const double l1 = mom0 + b0 + 0.5 * aa + lc;
     const double dl1 = b0 + aa + dlc;
@@ -582,6 +617,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
 #pragma unroll
       for (int q = 0; q < 6; ++q) v[q] += alpha * dx[q];
     }
+    NA_TOC(st.t_ls, t3);
     st.iterations = it + 1;
   }
   return st.converged;

@@ -46,7 +46,10 @@ struct Ctrl {
   unsigned long long* cum;
   long long* work;  // [W, NA_WORK]
 };
-#define NA_WORK 6
+// per-world work counters: attempts, Newton iterations, CG iterations, contacts, pairs, solves,
+// then shader-clock cycles of collision, solve setup, residual evaluation, preconditioner, PCG,
+// line search (the cycle counters stay zero unless built with -DNA_PHASE_TIMERS)
+#define NA_WORK 12
 
 struct na_solver {
   const na_model* model;
@@ -258,6 +261,7 @@ __device__ __forceinline__ WorldCtx world_ctx(const NaModel& M, int w) {
 __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, const double* q,
                              int& overflow, long long* wk) {
   const int l = lane_id();
+  NA_TIC(t0);
   if (l < wc.nb) {
 #pragma unroll
     for (int k = 0; k < 7; ++k) S.body.q[k][l] = (float)q[k];
@@ -270,19 +274,22 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
   if (np > NA_MAX_PAIRS || nc > cap) overflow = 1;
   wk[3] += min(nc, cap);
   wk[4] += np;
+  NA_TOC(wk[6], t0);
   return min(nc, cap);
 }
 
 // One convex step of size h from (q, v0) with the contacts in LDS; integrates into qo/vo.
 __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, int nc,
                             const double* q, const float* v0, const float* f, double h, const float* guess, double* qo,
-                            float* vo, long long* wk) {
+                            float* vo, long long* wk, SolveStats* out_st = nullptr) {
   const int l = lane_id();
   const bool own = l < wc.nb;
+  NA_TIC(t0);
   BodyReg br;
   if (own) body_setup(M, wc.b0 + l, q, v0, f, h, br, S.body, l);
   wave_sync();
   contact_setup(nc, (float)h, cfg, S.con, S.body, S.sol);
+  NA_TOC(wk[7], t0);
   double v[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) v[k] = guess[k];
@@ -294,6 +301,11 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
   wk[1] += st.iterations;
   wk[2] += st.cg_iterations;
   wk[5] += 1;
+  wk[8] += st.t_eval;
+  wk[9] += st.t_pre;
+  wk[10] += st.t_cg;
+  wk[11] += st.t_ls;
+  if (out_st) *out_st = st;
   return ok;
 }
 
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, d
     load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
     WCtrl c = load_ctrl(K, w);
     open_frame(cfg, c, dt_outer, eff_max);
-    long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+    long long wk[NA_WORK] = {};
     int acc = 0;
     for (int it = 0; it < cfg.max_substeps; ++it) {
       if (!(c.sim_time < c.next_time)) break;
@@ -590,7 +602,7 @@ __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl
 #pragma unroll
     for (int k = 0; k < 6; ++k) { v[k] = 0.f; f[k] = 0.f; }
   }
-  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  long long wk[NA_WORK] = {};
   int acc = world_iteration(M, cfg, wc, S, c, q, v, f, eff_max, wk);
   if (l < wc.nb) {
     int gb = wc.b0 + l;
@@ -635,7 +647,7 @@ __global__ __launch_bounds__(64) void k_collide_api(NaModel M, NaConfig cfg, con
     for (int k = 0; k < 7; ++k) q[k] = body_q[7 * (wc.b0 + l) + k];
   }
   int ovf = 0;
-  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  long long wk[NA_WORK] = {};
   NaConfig c2 = cfg;
   c2.max_contacts = min(per_world, NA_MAX_CONTACTS);
   int nc = world_collide(M, c2, wc, S, q, ovf, wk);
@@ -764,10 +776,11 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
     for (int k = 0; k < 7; ++k) q[k] = q64[7 * (wc.b0 + l) + k];
   }
   if (wc.nb == 0) return;
-  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  long long wk[NA_WORK] = {};
   int ovf = 0;
   int nc = world_collide(M, cfg, wc, S, q, ovf, wk);
-  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk);
+  SolveStats st;
+  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st);
   if (l < wc.nb) {
     int gb = wc.b0 + l;
 #pragma unroll
@@ -776,10 +789,12 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
     for (int k = 0; k < 6; ++k) qd_out[6 * gb + k] = vo[k];
   }
   if (l == 0) {
-    stats[4 * w + 0] = (int)wk[1];
-    stats[4 * w + 1] = (int)wk[1];
-    stats[4 * w + 2] = ok ? 1 : 0;
-    stats[4 * w + 3] = nc;
+    stats[6 * w + 0] = st.iterations;
+    stats[6 * w + 1] = st.cg_iterations;
+    stats[6 * w + 2] = ok ? 1 : 0;
+    stats[6 * w + 3] = nc;
+    stats[6 * w + 4] = st.reason;
+    stats[6 * w + 5] = __float_as_int(st.residual);
   }
 }
 __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const double* q64, const float* qd,
@@ -801,7 +816,7 @@ __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const d
     if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
     return;
   }
-  long long wk[NA_WORK] = {0, 0, 0, 0, 0, 0};
+  long long wk[NA_WORK] = {};
   bool ok = true;
   int ovf = 0, nc = 0;
   double e = world_attempt(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);

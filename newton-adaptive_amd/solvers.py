@@ -27,14 +27,16 @@ from .native import model_handle
 
 _MODE_CODES = {"fixed": 0, "adaptive": 1}
 _LINE_SEARCHES = ("exact_root", "armijo_decay")
-_WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves")
+_WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves",
+                "cycles_collide", "cycles_setup", "cycles_residual", "cycles_preconditioner", "cycles_pcg",
+                "cycles_line_search")
 
 
 class _SolverBase:
     _controller = 0
 
     def __init__(self, model, *, mode="adaptive", tol=1e-3, dt_inner_init=0.01, dt_inner_min=1e-6,
-                 dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=30,
+                 dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=100,
                  contact_preset_variant="drake", line_search_variant="exact_root", contact_tau_d=0.01,
                  broad_phase="sap", optimality_rel_tol=None, schedule="served", **_kw):
         if mode not in _MODE_CODES:
@@ -83,9 +85,9 @@ class _SolverBase:
         cfg.max_iterations = int(max_iterations)
         cfg.max_rigid_contact = self._max_rigid_contact
         cfg.broad_phase = BROAD_PHASES[broad_phase]
-        # CENIC kappa*tol (solver_sap_adaptive.py:485-501) floored at what FP32 residuals resolve
+        # CENIC kappa*tol with kappa = 1e-3, floored at 1e-8 (solver_sap_adaptive.py:485-501)
         self.optimality_rel_tol = float(optimality_rel_tol) if optimality_rel_tol is not None \
-            else max(1.0e-3 * self._tol, 1.0e-6)
+            else max(1.0e-3 * self._tol, 1.0e-8)
         cfg.optimality_rel_tol = self.optimality_rel_tol
         cfg.optimality_abs_tol = 1.0e-12
         cfg.contact_tau_d = float(contact_tau_d)

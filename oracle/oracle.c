@@ -233,13 +233,14 @@ void or_inf_norm_error_f32(int n, int coords, const float* q_full, const float* 
                            float* err) {
   for (int w = 0; w < n; ++w) {
     float m = 0.0f;
+    int nan_seen = 0;
     for (int i = 0; i < coords; ++i) {
       float d = fabsf(q_double[w * coords + i] - q_full[w * coords + i]);
       float s = scale[w * coords + i] * d;
-      m = m > s ? m : s; /* wp.max(max_err, x): NaN handling below */
-      if (isnan(s)) m = s;
+      if (isnan(s)) nan_seen = 1; /* a non-finite coordinate poisons the world's error */
+      m = m > s ? m : s;
     }
-    if (isnan(m) || isinf(m)) m = 1.0e10f;
+    if (nan_seen || isnan(m) || isinf(m)) m = 1.0e10f;
     err[w] = m;
   }
 }
@@ -264,9 +265,11 @@ int or_cubic_min(double c0, double d0, double c1, double d1, double h, double* a
  * normal from A into B, position = midpoint between surfaces,
  * distance negative = penetration.
  * ==================================================================== */
-typedef struct { int n; real dist[4]; v3 pos[4]; v3 normal[4]; } manifold;
+/* up to 8 contacts: box-box keeps its whole clipped face polygon */
+#define OR_MAXMF 8
+typedef struct { int n; real dist[OR_MAXMF]; v3 pos[OR_MAXMF]; v3 normal[OR_MAXMF]; } manifold;
 
-static void mf_init(manifold* m) { m->n = 0; for (int i = 0; i < 4; ++i) m->dist[i] = (real)OR_MAXVAL; }
+static void mf_init(manifold* m) { m->n = 0; for (int i = 0; i < OR_MAXMF; ++i) m->dist[i] = (real)OR_MAXVAL; }
 
 /* closest_segment_point, collision_primitive.py:49-54 */
 static v3 closest_segment_point(v3 a, v3 b, v3 pt) {
@@ -537,28 +540,11 @@ static void collide_box_box(v3 pa, const m33* ra, v3 ha, v3 pb, const m33* rb, v
       pp[kept] = sub(poly[i], scl(nref, (real)0.5 * di)); /* midpoint to the reference face */
       kept++;
     }
-    int sel[4]; int ns = 0;
-    if (kept <= 4) { for (int i = 0; i < kept; ++i) sel[ns++] = i; }
-    else {
-      int i0 = 0;
-      for (int i = 1; i < kept; ++i) if (dd[i] < dd[i0]) i0 = i;
-      int i1 = -1; real far = -1;
-      for (int i = 0; i < kept; ++i) { real q = dot(sub(pp[i], pp[i0]), sub(pp[i], pp[i0])); if (i != i0 && q > far) { far = q; i1 = i; } }
-      int i2 = -1, i3 = -1; real amax = 0, amin = 0;
-      v3 e01 = sub(pp[i1], pp[i0]);
-      for (int i = 0; i < kept; ++i) {
-        if (i == i0 || i == i1) continue;
-        real ar = dot(cross(e01, sub(pp[i], pp[i0])), nref);
-        if (ar > amax) { amax = ar; i2 = i; }
-        if (ar < amin) { amin = ar; i3 = i; }
-      }
-      sel[ns++] = i0; sel[ns++] = i1;
-      if (i2 >= 0) sel[ns++] = i2;
-      if (i3 >= 0) sel[ns++] = i3;
-    }
-    for (int i = 0; i < ns; ++i) {
-      m->dist[i] = dd[sel[i]];
-      m->pos[i] = pp[sel[i]];
+    /* the whole clipped polygon is the manifold: it moves continuously with the poses, where a
+     * reduction to 4 points would switch between near-equal-depth candidates */
+    for (int i = 0; i < kept; ++i) {
+      m->dist[i] = dd[i];
+      m->pos[i] = pp[i];
       m->normal[i] = n;
     }
   } else {
@@ -753,7 +739,7 @@ static int narrow_pair(const or_model* M, const double* body_q, int b0, int sa, 
     return 0; /* plane-plane and unsupported types produce nothing */
   }
   int cnt = 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < OR_MAXMF; ++k) {
     if (!(m.dist[k] < (real)OR_MAXVAL)) continue;
     v3 n = nrm(m.normal[k]);
     /* contact_passes_gap_check: distance between the writer's core points minus radii+margins */
@@ -776,7 +762,7 @@ static int narrow_pair(const or_model* M, const double* body_q, int b0, int sa, 
 }
 
 /* Single-pair entry for the primitive KATs: shapes given directly (types, poses, scales).
- * out: [n*8] = dist, pos(3), normal(3), phi0 */
+ * out: [n*8] (n <= 8) = dist, pos(3), normal(3), 0 */
 int or_collide_shapes(int ta, const double* xa, const double* sa_, int tb, const double* xb, const double* sb_,
                       double gap_sum, double* out) {
   xf Xa = {V((real)xa[0], (real)xa[1], (real)xa[2]), {(real)xa[3], (real)xa[4], (real)xa[5], (real)xa[6]}};
@@ -796,7 +782,7 @@ int or_collide_shapes(int ta, const double* xa, const double* sa_, int tb, const
   else if (ta == GEO_CAPSULE && tb == GEO_BOX) collide_capsule_box(Xa.p, za, sca.x, sca.y, Xb.p, &Rb, scb, &m);
   else if (ta == GEO_BOX && tb == GEO_BOX) collide_box_box(Xa.p, &Ra, sca, Xb.p, &Rb, scb, (real)gap_sum, &m);
   int cnt = 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < OR_MAXMF; ++k) {
     if (!(m.dist[k] < (real)OR_MAXVAL)) continue;
     double* o = out + 8 * cnt;
     o[0] = m.dist[k]; o[1] = m.pos[k].x; o[2] = m.pos[k].y; o[3] = m.pos[k].z;
@@ -820,7 +806,7 @@ int or_collide_world(const or_model* M, int w, const double* body_q, or_contact*
   int np = or_world_pairs(M, w, lo, hi, pairs, pcap);
   if (n_pairs) *n_pairs = np;
   int cnt = 0;
-  or_contact tmp[4];
+  or_contact tmp[OR_MAXMF];
   for (int p = 0; p < np; ++p) {
     int k = narrow_pair(M, body_q, b0, pairs[2 * p], pairs[2 * p + 1], tmp);
     for (int i = 0; i < k; ++i) {

@@ -174,7 +174,7 @@ def shape_aabbs(om: OracleModel, body_q: np.ndarray):
 
 
 def collide_shapes(ta, xa, sa, tb, xb, sb, gap_sum=0.0):
-    out = np.zeros((4, 8))
+    out = np.zeros((8, 8))
     xa = np.ascontiguousarray(xa, dtype=np.float64); xb = np.ascontiguousarray(xb, dtype=np.float64)
     sa = np.ascontiguousarray(sa, dtype=np.float64); sb = np.ascontiguousarray(sb, dtype=np.float64)
     n = lib().or_collide_shapes(int(ta), _ptr(xa, f64p), _ptr(sa, f64p), int(tb), _ptr(xb, f64p),

@@ -78,7 +78,7 @@ def test_adapt_dt_bit_exact(nalib, O, mode):
                             _p(o["cum"], C.c_int32), mode, C.c_float(1e-3), C.c_float(1e-6), C.c_float(0.01),
                             C.c_float(1e9))
     g = {k: _t(v.copy()) for k, v in arrs.items()}
-    ge, gn = _t(err), _t(nxt)
+    ge, gn = _t(err), _t(nxt)  # named: inputs must outlive the asynchronous launch
     nalib.check(nalib.lib().na_ctrl_adapt_dt_f32(n, ge.data_ptr(), g["sim"].data_ptr(), gn.data_ptr(),
                                                  g["dt"].data_ptr(), g["dth"].data_ptr(), g["ideal"].data_ptr(),
                                                  g["div"].data_ptr(), g["acc"].data_ptr(), g["ae"].data_ptr(),
@@ -97,6 +97,7 @@ def test_rebase_reset_clamp_infnorm_bit_exact(nalib, O):
     nxt = np.array([1.0e6, 5.0, 1048576.0, 0.0, 10.0], np.float32)
     gs, gn = _t(sim), _t(nxt)
     nalib.check(lib.na_ctrl_rebase_time_f32(len(sim), gs.data_ptr(), gn.data_ptr(), None), "rebase")
+    torch.cuda.synchronize()
     os_, on = sim.copy(), nxt.copy()
     O.lib().or_rebase_time_f32(len(sim), _p(os_, C.c_float), _p(on, C.c_float))
     torch.cuda.synchronize()
@@ -138,7 +139,8 @@ def test_rebase_reset_clamp_infnorm_bit_exact(nalib, O):
     qd[9, 0] = np.inf
     sc = np.ones((64, 7), np.float32)
     ge = torch.zeros(64, dtype=torch.float32, device=DEV)
-    nalib.check(lib.na_ctrl_inf_norm_error_f32(64, 7, _t(qf).data_ptr(), _t(qd).data_ptr(), _t(sc).data_ptr(),
+    gqf, gqd, gsc = _t(qf), _t(qd), _t(sc)  # keep the inputs alive until the kernel has run
+    nalib.check(lib.na_ctrl_inf_norm_error_f32(64, 7, gqf.data_ptr(), gqd.data_ptr(), gsc.data_ptr(),
                                                ge.data_ptr(), None), "infnorm")
     oe = np.zeros(64, np.float32)
     O.lib().or_inf_norm_error_f32(64, 7, _p(qf, C.c_float), _p(qd, C.c_float), _p(sc, C.c_float), _p(oe, C.c_float))
