@@ -193,45 +193,84 @@ __device__ __forceinline__ float box_sdf(f3 p, f3 h, f3& g) {
   }
   return out + in;
 }
-#define NA_CAPBOX_ITERS 40
+#define NA_CAPBOX_ITERS 32
 #define NA_CAPBOX_PARALLEL 0.2f
 __device__ void capsule_box(f3 cp, f3 ax, float r, float hl, f3 bp, const m3& R, f3 h, Manifold& m) {
   f3 p0 = mtv(R, (cp - ax * hl) - bp);
   f3 p1 = mtv(R, (cp + ax * hl) - bp);
   f3 d = p1 - p0;
-  const float gr = 0.6180339887498949f;
-  float a = 0.f, b = 1.f, c1 = b - gr * (b - a), c2 = a + gr * (b - a);
+  // minimiser of the convex sdf(p0 + t d) over [0, 1]: bisection on the sign of the directional
+  // derivative grad(sdf).d (accurate to rounding, where comparing values is only ~sqrt(eps))
   f3 g;
-  float f1 = box_sdf(p0 + d * c1, h, g), f2 = box_sdf(p0 + d * c2, h, g);
-  for (int it = 0; it < NA_CAPBOX_ITERS; ++it) {
-    if (f1 <= f2) {
-      b = c2; c2 = c1; f2 = f1; c1 = b - gr * (b - a); f1 = box_sdf(p0 + d * c1, h, g);
+  float a = 0.f, b = 1.f;
+  box_sdf(p0, h, g);
+  if (dot3(g, d) >= 0.f) {
+    b = 0.f;
+  } else {
+    box_sdf(p0 + d, h, g);
+    if (dot3(g, d) <= 0.f) {
+      a = 1.f;
     } else {
-      a = c1; c1 = c2; f1 = f2; c2 = a + gr * (b - a); f2 = box_sdf(p0 + d * c2, h, g);
+      for (int it = 0; it < NA_CAPBOX_ITERS; ++it) {
+        float mid = 0.5f * (a + b);
+        box_sdf(p0 + d * mid, h, g);
+        if (dot3(g, d) < 0.f) a = mid;
+        else b = mid;
+      }
     }
   }
   float ts = 0.5f * (a + b);
   f3 gs;
   float fs = box_sdf(p0 + d * ts, h, gs);
   float dl = len3(d);
-  // the segment minimum always; both endpoints too when the axis lies along the face
+  // Along a face the distance is linear in t over the part of the segment above the face, so
+  // its minimum is at an end of that part (not unique when parallel).  Contacts: the ends of
+  // the segment clipped to the face slab, plus the golden-section minimum when it lies outside
+  // the clipped part (where it is unique).
   bool par = dl > 0.f && fabsf(dot3(d, gs)) < NA_CAPBOX_PARALLEL * dl;
-  int cnt = 0;
+  float tl[3] = {ts, ts, ts};
+  int nt = 1;
+  if (par) {
+    int fk = (fabsf(gs.x) >= fabsf(gs.y) && fabsf(gs.x) >= fabsf(gs.z)) ? 0 : (fabsf(gs.y) >= fabsf(gs.z) ? 1 : 2);
+    float lo = 0.f, hi = 1.f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j == fk) continue;
+      float pj = comp3(p0, j), dj = comp3(d, j), hj = comp3(h, j);
+      if (fabsf(dj) < 1e-12f) {
+        if (fabsf(pj) > hj) { lo = 1.f; hi = 0.f; }
+      } else {
+        float t1 = (-hj - pj) / dj, t2 = (hj - pj) / dj;
+        lo = fmaxf(lo, fminf(t1, t2));
+        hi = fminf(hi, fmaxf(t1, t2));
+      }
+    }
+    if (lo <= hi) {
+      tl[0] = lo;
+      nt = 1;
+      if (hi - lo > 1e-6f) { tl[1] = hi; nt = 2; }
+      f3 g_;
+      float fe = fminf(box_sdf(p0 + d * lo, h, g_), box_sdf(p0 + d * hi, h, g_));
+      // the minimum is strictly deeper than the clipped ends (off the face, or inside the box)
+      if (ts < lo - 1e-4f || ts > hi + 1e-4f || fs < fe - 1e-6f) {
+        if (nt == 1) tl[1] = ts;
+        else tl[2] = ts;
+        nt++;
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    float t = k == 0 ? ts : (float)(k - 1);
-    if (k > 0 && (!par || fabsf(t - ts) <= 0.05f)) continue;
-    f3 pk = p0 + d * t;
-    f3 gk = gs;
-    float fk = fs;
-    if (k > 0) fk = box_sdf(pk, h, gk);
+    if (k >= nt) continue;
+    f3 pk = p0 + d * tl[k];
+    f3 gk;
+    float fk = box_sdf(pk, h, gk);
     f3 nl = -gk;
     float dist = fk - r;
     f3 mid = pk + nl * (r + 0.5f * dist);
-    if (cnt == 0) { m.dist[0] = dist; m.pos[0] = bp + mv(R, mid); m.nrm[0] = mv(R, nl); }
-    else if (cnt == 1) { m.dist[1] = dist; m.pos[1] = bp + mv(R, mid); m.nrm[1] = mv(R, nl); }
-    else { m.dist[2] = dist; m.pos[2] = bp + mv(R, mid); m.nrm[2] = mv(R, nl); }
-    cnt++;
+    m.dist[k] = dist;
+    m.pos[k] = bp + mv(R, mid);
+    m.nrm[k] = mv(R, nl);
   }
 }
 

@@ -417,43 +417,78 @@ static real box_sdf(v3 p, v3 h, v3* grad) {
   return out + in;
 }
 
-#define CAPBOX_ITERS 40
+#define CAPBOX_ITERS 48
 #define CAPBOX_PARALLEL 0.2
 
 static void collide_capsule_box(v3 cpos, v3 axis, real r, real hl, v3 bpos, const m33* rot, v3 size, manifold* m) {
   v3 p0 = mtv(rot, sub(sub(cpos, scl(axis, hl)), bpos));
   v3 p1 = mtv(rot, sub(add(cpos, scl(axis, hl)), bpos));
   v3 d = sub(p1, p0);
-  /* golden-section minimisation of the convex sdf(p0 + t d) over t in [0, 1] */
-  const real gr = (real)0.6180339887498949;
-  real a = 0, b = 1, c1 = b - gr * (b - a), c2 = a + gr * (b - a);
+  /* minimiser of the convex sdf(p0 + t d) over [0, 1]: bisection on the sign of the directional
+   * derivative grad(sdf).d (accurate to rounding, where comparing values is only ~sqrt(eps)) */
   v3 g;
-  real f1 = box_sdf(add(p0, scl(d, c1)), size, &g), f2 = box_sdf(add(p0, scl(d, c2)), size, &g);
-  for (int it = 0; it < CAPBOX_ITERS; ++it) {
-    if (f1 <= f2) { b = c2; c2 = c1; f2 = f1; c1 = b - gr * (b - a); f1 = box_sdf(add(p0, scl(d, c1)), size, &g); }
-    else { a = c1; c1 = c2; f1 = f2; c2 = a + gr * (b - a); f2 = box_sdf(add(p0, scl(d, c2)), size, &g); }
+  real a = 0, b = 1;
+  box_sdf(p0, size, &g);
+  if (dot(g, d) >= 0) b = 0;
+  else {
+    box_sdf(add(p0, d), size, &g);
+    if (dot(g, d) <= 0) a = 1;
+    else
+      for (int it = 0; it < CAPBOX_ITERS; ++it) {
+        real mid = (real)0.5 * (a + b);
+        box_sdf(add(p0, scl(d, mid)), size, &g);
+        if (dot(g, d) < 0) a = mid; else b = mid;
+      }
   }
   real ts = (real)0.5 * (a + b);
   v3 gs;
   real fs = box_sdf(add(p0, scl(d, ts)), size, &gs);
   real dl = len(d);
-  /* the segment minimum always; both endpoints too when the axis lies along the face */
+  /* Along a face the distance is linear in t over the part of the segment that lies above the
+   * face, so its minimum is attained at an end of that part (and is not unique when the segment
+   * is parallel).  Contacts: the two ends of the segment clipped to the face slab, plus the
+   * golden-section minimum when it lies outside the clipped part (then it is unique). */
   int par = dl > 0 && RA(dot(d, gs)) < (real)CAPBOX_PARALLEL * dl;
-  real tl[3] = {ts, 0, 1};
-  int cnt = 0;
-  for (int k = 0; k < 3; ++k) {
-    if (k > 0 && (!par || RA(tl[k] - ts) <= (real)0.05)) continue;
+  real tl[3];
+  int nt = 0;
+  if (par) {
+    int fk = (RA(gs.x) >= RA(gs.y) && RA(gs.x) >= RA(gs.z)) ? 0 : (RA(gs.y) >= RA(gs.z) ? 1 : 2);
+    real lo = 0, hi = 1;
+    for (int j = 0; j < 3; ++j) {
+      if (j == fk) continue;
+      real pj = comp(p0, j), dj = comp(d, j), hj = comp(size, j);
+      if (RA(dj) < (real)1e-12) {
+        if (RA(pj) > hj) { lo = 1; hi = 0; }
+      } else {
+        real t1 = (-hj - pj) / dj, t2 = (hj - pj) / dj;
+        lo = maxr(lo, minr(t1, t2));
+        hi = minr(hi, maxr(t1, t2));
+      }
+    }
+    if (lo <= hi) {
+      tl[nt++] = lo;
+      if (hi - lo > (real)1e-6) tl[nt++] = hi;
+      v3 g_;
+      real fe = minr(box_sdf(add(p0, scl(d, lo)), size, &g_), box_sdf(add(p0, scl(d, hi)), size, &g_));
+      /* the minimum is strictly deeper than the clipped ends (off the face, or inside the box) */
+      if (ts < lo - (real)1e-4 || ts > hi + (real)1e-4 || fs < fe - (real)1e-6) tl[nt++] = ts;
+    } else {
+      tl[nt++] = ts;
+    }
+  } else {
+    tl[nt++] = ts;
+  }
+  for (int k = 0; k < nt; ++k) {
     v3 pk = add(p0, scl(d, tl[k]));
-    v3 gk = gs;
-    real fk = k == 0 ? fs : box_sdf(pk, size, &gk);
+    v3 gk;
+    real fk = box_sdf(pk, size, &gk);
     /* normal from capsule (A) to box (B) is -grad; midpoint between surfaces */
     v3 nl = scl(gk, -1);
     real dist = fk - r;
     v3 mid = add(pk, scl(nl, r + (real)0.5 * dist));
-    m->dist[cnt] = dist;
-    m->pos[cnt] = add(bpos, mv(rot, mid));
-    m->normal[cnt] = mv(rot, nl);
-    cnt++;
+    m->dist[k] = dist;
+    m->pos[k] = add(bpos, mv(rot, mid));
+    m->normal[k] = mv(rot, nl);
   }
 }
 
