@@ -66,6 +66,7 @@ typedef struct na_config {
   int32_t max_substeps, max_iterations, max_rigid_contact, broad_phase; /* broad_phase: 0 sap, 1 nxn */
   float optimality_rel_tol, optimality_abs_tol, contact_tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
   int32_t max_cg;
+  float cg_forcing;        /* > 0: inexact-Newton PCG forcing bound (eta_k <= cg_forcing); 0: fixed cg_rel_tol */
 } na_config;
 
 /* Device pointers to the per-world controller vectors (owned by the solver). */

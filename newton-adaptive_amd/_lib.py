@@ -46,7 +46,7 @@ class NaConfig(C.Structure):
         ("max_substeps", i32), ("max_iterations", i32), ("max_rigid_contact", i32), ("broad_phase", i32),
         ("optimality_rel_tol", C.c_float), ("optimality_abs_tol", C.c_float), ("contact_tau_d", C.c_float),
         ("sigma", C.c_float), ("beta", C.c_float), ("ls_rel_tol", C.c_float), ("cg_rel_tol", C.c_float),
-        ("max_cg", i32),
+        ("max_cg", i32), ("cg_forcing", C.c_float),
     ]
 
 

@@ -71,39 +71,85 @@ __device__ __forceinline__ f3 qrot(qf q, f3 v) { return mv(qmat(q.x, q.y, q.z, q
 // ---------------------------------------------------------------- wave collectives
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NA_WAVE);
-  return v;
+// Wave-wide reductions without LDS: DPP steps inside each 16-lane row (quad xor 1, quad
+// xor 2, half-row mirror, row mirror) leave the row total in every lane of the row, then
+// the four row totals are read from lanes 0/16/32/48 and combined in a fixed order.  Every
+// lane returns the same bits.  Callers keep all 64 lanes active.
+#define NA_DPP_XOR1 0xB1        // quad_perm [1,0,3,2]
+#define NA_DPP_XOR2 0x4E        // quad_perm [2,3,0,1]
+#define NA_DPP_HALF_MIRROR 0x141
+#define NA_DPP_MIRROR 0x140
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NA_WAVE);
-  return v;
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, NA_WAVE));
-  return v;
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = dpp_i<CTRL>((int)(b & 0xffffffffll));
+  int hi = dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ double wave_max_d(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double u = __shfl_xor(v, o, NA_WAVE);
-    v = (u > v || u != u) ? u : v;  // NaN propagates
-  }
-  return v;
+__device__ __forceinline__ float lane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
-__device__ __forceinline__ int wave_sum_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NA_WAVE);
-  return v;
+__device__ __forceinline__ double lane_d(double v, int l) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, NA_WAVE));
-  return v;
+
+struct OpAdd {
+  template <typename T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }
+};
+struct OpMin {
+  template <typename T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
+struct OpMaxNan {  // max that propagates NaN from either side
+  template <typename T>
+  __device__ __forceinline__ T operator()(T a, T b) const { return (b > a || b != b) ? b : a; }
+};
+
+template <typename Op>
+__device__ __forceinline__ float wave_reduce_f(float v, Op op) {
+  v = op(v, dpp_f<NA_DPP_XOR1>(v));
+  v = op(v, dpp_f<NA_DPP_XOR2>(v));
+  v = op(v, dpp_f<NA_DPP_HALF_MIRROR>(v));
+  v = op(v, dpp_f<NA_DPP_MIRROR>(v));
+  return op(op(lane_f(v, 0), lane_f(v, 16)), op(lane_f(v, 32), lane_f(v, 48)));
 }
+template <typename Op>
+__device__ __forceinline__ double wave_reduce_d(double v, Op op) {
+  v = op(v, dpp_d<NA_DPP_XOR1>(v));
+  v = op(v, dpp_d<NA_DPP_XOR2>(v));
+  v = op(v, dpp_d<NA_DPP_HALF_MIRROR>(v));
+  v = op(v, dpp_d<NA_DPP_MIRROR>(v));
+  return op(op(lane_d(v, 0), lane_d(v, 16)), op(lane_d(v, 32), lane_d(v, 48)));
+}
+template <typename Op>
+__device__ __forceinline__ int wave_reduce_i(int v, Op op) {
+  v = op(v, dpp_i<NA_DPP_XOR1>(v));
+  v = op(v, dpp_i<NA_DPP_XOR2>(v));
+  v = op(v, dpp_i<NA_DPP_HALF_MIRROR>(v));
+  v = op(v, dpp_i<NA_DPP_MIRROR>(v));
+  return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+            op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+__device__ __forceinline__ float wave_sum(float v) { return wave_reduce_f(v, OpAdd()); }
+__device__ __forceinline__ double wave_sum_d(double v) { return wave_reduce_d(v, OpAdd()); }
+__device__ __forceinline__ float wave_max(float v) { return wave_reduce_f(v, OpMaxNan()); }
+__device__ __forceinline__ double wave_max_d(double v) { return wave_reduce_d(v, OpMaxNan()); }
+__device__ __forceinline__ int wave_sum_i(int v) { return wave_reduce_i(v, OpAdd()); }
+__device__ __forceinline__ int wave_min_i(int v) { return wave_reduce_i(v, OpMin()); }
 // inclusive prefix sum over the 64 lanes (Hillis-Steele with shuffles)
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int l = lane_id();

@@ -20,18 +20,23 @@ struct NaConfig {
   int max_substeps, max_iterations, max_contacts, broad_phase;  // broad_phase: 0 sap, 1 nxn
   float opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
   int max_cg;
+  float cg_forcing;  // > 0: upper bound of the inexact-Newton PCG forcing term; 0: fixed cg_rel_tol
 };
 
+// Body slot NA_DUMMY stands for the static side of a contact: its velocity entries stay zero
+// for the whole solve, so contact loops gather from it without branching.
+#define NA_DUMMY NA_MAX_BODIES
+#define NA_BSLOT (NA_MAX_BODIES + 1)
 struct BodyLDS {
   float q[7][NA_MAX_BODIES];     // f32 pose used for contact generation (SoA)
   float x[3][NA_MAX_BODIES];     // COM position at the step start
   float iwi[6][NA_MAX_BODIES];   // world inverse inertia (xx, yy, zz, xy, xz, yz)
   float im[NA_MAX_BODIES];       // inverse mass
-  float v[6][NA_MAX_BODIES];     // f32 exchange vector (PCG directions)
-  float acc[6][NA_MAX_BODIES];   // f32 scatter target (H p)
-  double vd[6][NA_MAX_BODIES];   // f64 exchange vector (Newton iterate)
-  double accd[6][NA_MAX_BODIES]; // f64 scatter target (J^T gamma)
-  float pre[21][NA_MAX_BODIES];  // block-Jacobi accumulation (6x6 lower, row-major packed)
+  float v[6][NA_BSLOT];          // f32 exchange vector (PCG directions)
+  float acc[6][NA_BSLOT];        // f32 scatter target (H p)
+  double vd[6][NA_BSLOT];        // f64 exchange vector (Newton iterate, line-search direction)
+  double accd[6][NA_BSLOT];      // f64 scatter target (J^T gamma)
+  float pre[21][NA_BSLOT];       // block-Jacobi accumulation (6x6 lower, row-major packed)
 };
 
 struct SolveLDS {
@@ -39,6 +44,7 @@ struct SolveLDS {
   float ra[3][NA_MAX_CONTACTS];  // contact point - COM of body A
   float rb[3][NA_MAX_CONTACTS];
   float Rn[NA_MAX_CONTACTS], Rt[NA_MAX_CONTACTS], vhat[NA_MAX_CONTACTS];
+  double iRn[NA_MAX_CONTACTS], iRt[NA_MAX_CONTACTS];  // FP64 reciprocals of Rn, Rt
   double vc0[3][NA_MAX_CONTACTS];
   double dvc[3][NA_MAX_CONTACTS];
   float G[6][NA_MAX_CONTACTS];  // symmetric 3x3: 00 11 22 01 02 12
@@ -86,136 +92,133 @@ __device__ __forceinline__ d3 dcross(d3 a, d3 b) {
   return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-// frame rows: t1, t2 = n x t1, n;  J_b row k = sign * (F_k, r x F_k)
-struct Frame {
-  f3 t1, t2, n;
+// One contact as the solve sees it: frame rows t1, t2 = n x t1, n; lever arms; body slots.
+// J_b row k = sign * (F_k, r x F_k), sign + for body B, - for body A.  An absent side (static
+// shape, or a lane past the contact count) has slot NA_DUMMY and flag false.
+struct CRow {
+  f3 t1, t2, n, ra, rb;
+  int sa, sb;
+  bool ha, hb;
 };
-__device__ __forceinline__ Frame load_frame(const ContactLDS& C, const SolveLDS& X, int c) {
-  Frame F;
-  F.n = mk(C.n[0][c], C.n[1][c], C.n[2][c]);
-  F.t1 = mk(X.t1[0][c], X.t1[1][c], X.t1[2][c]);
-  F.t2 = cross3(F.n, F.t1);
-  return F;
-}
-// vc = J v (f32) for contact c from B.v
-__device__ __forceinline__ f3 contact_vel(const BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F,
-                                          int c) {
-  f3 rel = mk(0.f, 0.f, 0.f);
+__device__ __forceinline__ CRow load_row(const ContactLDS& C, const SolveLDS& X, int c, bool valid) {
+  CRow r;
+  r.n = mk(C.n[0][c], C.n[1][c], C.n[2][c]);
+  r.t1 = mk(X.t1[0][c], X.t1[1][c], X.t1[2][c]);
+  r.t2 = cross3(r.n, r.t1);
+  r.ra = mk(X.ra[0][c], X.ra[1][c], X.ra[2][c]);
+  r.rb = mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]);
   int ba = C.ba[c], bb = C.bb[c];
-  if (bb >= 0) {
-    f3 v = mk(B.v[0][bb], B.v[1][bb], B.v[2][bb]);
-    f3 w = mk(B.v[3][bb], B.v[4][bb], B.v[5][bb]);
-    rel = rel + v + cross3(w, mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]));
-  }
-  if (ba >= 0) {
-    f3 v = mk(B.v[0][ba], B.v[1][ba], B.v[2][ba]);
-    f3 w = mk(B.v[3][ba], B.v[4][ba], B.v[5][ba]);
-    rel = rel - (v + cross3(w, mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])));
-  }
-  return mk(dot3(F.t1, rel), dot3(F.t2, rel), dot3(F.n, rel));
+  r.ha = valid && ba >= 0;
+  r.hb = valid && bb >= 0;
+  r.sa = r.ha ? ba : NA_DUMMY;
+  r.sb = r.hb ? bb : NA_DUMMY;
+  return r;
 }
-// vc = J v (f64) for contact c from B.vd
-__device__ __forceinline__ d3 contact_vel_d(const BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F,
-                                            int c) {
-  d3 rel = d3{0.0, 0.0, 0.0};
-  int ba = C.ba[c], bb = C.bb[c];
-  if (bb >= 0) {
-    d3 w = d3{B.vd[3][bb], B.vd[4][bb], B.vd[5][bb]};
-    d3 x = dcross(w, dd(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c])));
-    rel.x += B.vd[0][bb] + x.x; rel.y += B.vd[1][bb] + x.y; rel.z += B.vd[2][bb] + x.z;
-  }
-  if (ba >= 0) {
-    d3 w = d3{B.vd[3][ba], B.vd[4][ba], B.vd[5][ba]};
-    d3 x = dcross(w, dd(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])));
-    rel.x -= B.vd[0][ba] + x.x; rel.y -= B.vd[1][ba] + x.y; rel.z -= B.vd[2][ba] + x.z;
-  }
-  return d3{ddot(dd(F.t1), rel), ddot(dd(F.t2), rel), ddot(dd(F.n), rel)};
+// relative contact velocity J v in the contact frame (f32, from B.v)
+__device__ __forceinline__ f3 row_vel(const BodyLDS& B, const CRow& r) {
+  f3 va = mk(B.v[0][r.sa], B.v[1][r.sa], B.v[2][r.sa]), wa = mk(B.v[3][r.sa], B.v[4][r.sa], B.v[5][r.sa]);
+  f3 vb = mk(B.v[0][r.sb], B.v[1][r.sb], B.v[2][r.sb]), wb = mk(B.v[3][r.sb], B.v[4][r.sb], B.v[5][r.sb]);
+  f3 rel = (vb + cross3(wb, r.rb)) - (va + cross3(wa, r.ra));
+  return mk(dot3(r.t1, rel), dot3(r.t2, rel), dot3(r.n, rel));
 }
-// acc += J^T lambda (f32 LDS atomics)
-__device__ __forceinline__ void scatter_JT(BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F, int c,
-                                           f3 lam) {
-  f3 fw = F.t1 * lam.x + F.t2 * lam.y + F.n * lam.z;  // world impulse on B
-  int ba = C.ba[c], bb = C.bb[c];
-  if (bb >= 0) {
-    f3 tq = cross3(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]), fw);
-    atomicAdd(&B.acc[0][bb], fw.x); atomicAdd(&B.acc[1][bb], fw.y); atomicAdd(&B.acc[2][bb], fw.z);
-    atomicAdd(&B.acc[3][bb], tq.x); atomicAdd(&B.acc[4][bb], tq.y); atomicAdd(&B.acc[5][bb], tq.z);
+// relative contact velocity J v in the contact frame (f64, from B.vd)
+__device__ __forceinline__ d3 row_vel_d(const BodyLDS& B, const CRow& r) {
+  d3 va = d3{B.vd[0][r.sa], B.vd[1][r.sa], B.vd[2][r.sa]}, wa = d3{B.vd[3][r.sa], B.vd[4][r.sa], B.vd[5][r.sa]};
+  d3 vb = d3{B.vd[0][r.sb], B.vd[1][r.sb], B.vd[2][r.sb]}, wb = d3{B.vd[3][r.sb], B.vd[4][r.sb], B.vd[5][r.sb]};
+  d3 xa = dcross(wa, dd(r.ra)), xb = dcross(wb, dd(r.rb));
+  d3 rel = d3{vb.x + xb.x - va.x - xa.x, vb.y + xb.y - va.y - xa.y, vb.z + xb.z - va.z - xa.z};
+  return d3{ddot(dd(r.t1), rel), ddot(dd(r.t2), rel), ddot(dd(r.n), rel)};
+}
+// acc += J^T lambda (f32 LDS atomics; only real bodies are written)
+__device__ __forceinline__ void row_scatter(BodyLDS& B, const CRow& r, f3 lam) {
+  f3 fw = r.t1 * lam.x + r.t2 * lam.y + r.n * lam.z;  // world impulse on B
+  f3 tb = cross3(r.rb, fw), ta = cross3(r.ra, fw);
+  if (r.hb) {
+    atomicAdd(&B.acc[0][r.sb], fw.x); atomicAdd(&B.acc[1][r.sb], fw.y); atomicAdd(&B.acc[2][r.sb], fw.z);
+    atomicAdd(&B.acc[3][r.sb], tb.x); atomicAdd(&B.acc[4][r.sb], tb.y); atomicAdd(&B.acc[5][r.sb], tb.z);
   }
-  if (ba >= 0) {
-    f3 tq = cross3(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c]), fw);
-    atomicAdd(&B.acc[0][ba], -fw.x); atomicAdd(&B.acc[1][ba], -fw.y); atomicAdd(&B.acc[2][ba], -fw.z);
-    atomicAdd(&B.acc[3][ba], -tq.x); atomicAdd(&B.acc[4][ba], -tq.y); atomicAdd(&B.acc[5][ba], -tq.z);
+  if (r.ha) {
+    atomicAdd(&B.acc[0][r.sa], -fw.x); atomicAdd(&B.acc[1][r.sa], -fw.y); atomicAdd(&B.acc[2][r.sa], -fw.z);
+    atomicAdd(&B.acc[3][r.sa], -ta.x); atomicAdd(&B.acc[4][r.sa], -ta.y); atomicAdd(&B.acc[5][r.sa], -ta.z);
   }
 }
-// accd += J^T gamma (f64 LDS atomics)
-__device__ __forceinline__ void scatter_JT_d(BodyLDS& B, const ContactLDS& C, const SolveLDS& X, const Frame& F, int c,
-                                             d3 lam) {
-  d3 t1 = dd(F.t1), t2 = dd(F.t2), n = dd(F.n);
+// accd += J^T gamma (f64 LDS atomics; only real bodies are written)
+__device__ __forceinline__ void row_scatter_d(BodyLDS& B, const CRow& r, d3 lam) {
+  d3 t1 = dd(r.t1), t2 = dd(r.t2), n = dd(r.n);
   d3 fw = d3{t1.x * lam.x + t2.x * lam.y + n.x * lam.z, t1.y * lam.x + t2.y * lam.y + n.y * lam.z,
              t1.z * lam.x + t2.z * lam.y + n.z * lam.z};
-  int ba = C.ba[c], bb = C.bb[c];
-  if (bb >= 0) {
-    d3 tq = dcross(dd(mk(X.rb[0][c], X.rb[1][c], X.rb[2][c])), fw);
-    atomicAdd(&B.accd[0][bb], fw.x); atomicAdd(&B.accd[1][bb], fw.y); atomicAdd(&B.accd[2][bb], fw.z);
-    atomicAdd(&B.accd[3][bb], tq.x); atomicAdd(&B.accd[4][bb], tq.y); atomicAdd(&B.accd[5][bb], tq.z);
+  d3 tb = dcross(dd(r.rb), fw), ta = dcross(dd(r.ra), fw);
+  if (r.hb) {
+    atomicAdd(&B.accd[0][r.sb], fw.x); atomicAdd(&B.accd[1][r.sb], fw.y); atomicAdd(&B.accd[2][r.sb], fw.z);
+    atomicAdd(&B.accd[3][r.sb], tb.x); atomicAdd(&B.accd[4][r.sb], tb.y); atomicAdd(&B.accd[5][r.sb], tb.z);
   }
-  if (ba >= 0) {
-    d3 tq = dcross(dd(mk(X.ra[0][c], X.ra[1][c], X.ra[2][c])), fw);
-    atomicAdd(&B.accd[0][ba], -fw.x); atomicAdd(&B.accd[1][ba], -fw.y); atomicAdd(&B.accd[2][ba], -fw.z);
-    atomicAdd(&B.accd[3][ba], -tq.x); atomicAdd(&B.accd[4][ba], -tq.y); atomicAdd(&B.accd[5][ba], -tq.z);
+  if (r.ha) {
+    atomicAdd(&B.accd[0][r.sa], -fw.x); atomicAdd(&B.accd[1][r.sa], -fw.y); atomicAdd(&B.accd[2][r.sa], -fw.z);
+    atomicAdd(&B.accd[3][r.sa], -ta.x); atomicAdd(&B.accd[4][r.sa], -ta.y); atomicAdd(&B.accd[5][r.sa], -ta.z);
   }
 }
 
 // SAP projection onto the friction cone in the R norm (FP64); returns cost 1/2 gamma^T R gamma,
-// writes gamma and (when G != nullptr) the symmetric G = dP/dy R^-1 (00 11 22 01 02 12) in f32.
-__device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, d3 y, d3& g, float* G) {
+// writes gamma and the symmetric G = dP/dy R^-1 (00 11 22 01 02 12) in f32.  iRt = 1/Rt,
+// iRn = 1/Rn.  Regions: stiction |y_t| <= mu y_n, separation y_n <= -mu_hat |y_t|, sliding.
+__device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, double iRt, double iRn, d3 y, d3& g,
+                                              float* G) {
   double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
-  double muh = mu * Rt / Rn;
+  double muh = mu * Rt * iRn;
   if (yr <= mu * yn) {
     g = y;
-    if (G) { G[0] = (float)(1.0 / Rt); G[1] = G[0]; G[2] = (float)(1.0 / Rn); G[3] = 0.f; G[4] = 0.f; G[5] = 0.f; }
+    G[0] = (float)iRt; G[1] = G[0]; G[2] = (float)iRn; G[3] = 0.f; G[4] = 0.f; G[5] = 0.f;
   } else if (yn <= -muh * yr) {
     g = d3{0.0, 0.0, 0.0};
-    if (G) G[0] = G[1] = G[2] = G[3] = G[4] = G[5] = 0.f;
+    G[0] = G[1] = G[2] = G[3] = G[4] = G[5] = 0.f;
   } else {
-    double t0 = y.x / yr, t1 = y.y / yr;
-    double den = 1.0 + mu * muh;
-    double gn = (yn + muh * yr) / den;
+    double iyr = 1.0 / yr;
+    double t0 = y.x * iyr, t1 = y.y * iyr;
+    double iden = 1.0 / (1.0 + mu * muh);
+    double gn = (yn + muh * yr) * iden;
     g = d3{mu * gn * t0, mu * gn * t1, gn};
-    if (G) {
-      double a = mu * muh / den, b = mu * gn / yr;
-      G[0] = (float)((a * t0 * t0 + b * (1.0 - t0 * t0)) / Rt);
-      G[1] = (float)((a * t1 * t1 + b * (1.0 - t1 * t1)) / Rt);
-      G[3] = (float)((a - b) * t0 * t1 / Rt);
-      G[2] = (float)(1.0 / (den * Rn));
-      G[4] = (float)(mu / den * t0 / Rn);
-      G[5] = (float)(mu / den * t1 / Rn);
-    }
+    double a = mu * muh * iden, b = mu * gn * iyr;
+    G[0] = (float)((a * t0 * t0 + b * (1.0 - t0 * t0)) * iRt);
+    G[1] = (float)((a * t1 * t1 + b * (1.0 - t1 * t1)) * iRt);
+    G[3] = (float)((a - b) * t0 * t1 * iRt);
+    G[2] = (float)(iden * iRn);
+    G[4] = (float)(mu * iden * t0 * iRn);
+    G[5] = (float)(mu * iden * t1 * iRn);
   }
   return 0.5 * (Rt * (g.x * g.x + g.y * g.y) + Rn * g.z * g.z);
 }
-// second derivative of the contact cost along u: u^T G u with G of the FP64 projection
-__device__ __forceinline__ double sap_curv(double mu, double Rt, double Rn, d3 y, d3 g, d3 u) {
+// Contact cost along a line: at y (velocity vc), with direction u = J dv in contact velocity,
+// returns cost, adds -gamma.u to `d1` and u^T G u to `d2` (G of the FP64 projection).
+__device__ __forceinline__ double sap_line(double mu, double Rt, double Rn, double iRt, double iRn, d3 y, d3 u,
+                                           double& d1, double& d2) {
   double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
-  double muh = mu * Rt / Rn;
-  if (yr <= mu * yn) return (u.x * u.x + u.y * u.y) / Rt + u.z * u.z / Rn;
+  double muh = mu * Rt * iRn;
+  if (yr <= mu * yn) {
+    d1 -= y.x * u.x + y.y * u.y + y.z * u.z;
+    d2 += (u.x * u.x + u.y * u.y) * iRt + u.z * u.z * iRn;
+    return 0.5 * (Rt * (y.x * y.x + y.y * y.y) + Rn * y.z * y.z);
+  }
   if (yn <= -muh * yr) return 0.0;
-  double t0 = y.x / yr, t1 = y.y / yr;
-  double den = 1.0 + mu * muh;
-  double gn = g.z;
-  double a = mu * muh / den, b = mu * gn / yr;
+  double iyr = 1.0 / yr;
+  double t0 = y.x * iyr, t1 = y.y * iyr;
+  double iden = 1.0 / (1.0 + mu * muh);
+  double gn = (yn + muh * yr) * iden;
+  double gt0 = mu * gn * t0, gt1 = mu * gn * t1;
+  d1 -= gt0 * u.x + gt1 * u.y + gn * u.z;
+  double a = mu * muh * iden, b = mu * gn * iyr;
   double ut = u.x * t0 + u.y * t1;
   double uu = u.x * u.x + u.y * u.y;
   // u_t^T (a t t^T + b (I - t t^T)) u_t / Rt + 2 u_n (mu/den) t.u_t / Rn + u_n^2 / (den Rn)
-  return (a * ut * ut + b * (uu - ut * ut)) / Rt + 2.0 * u.z * (mu / den) * ut / Rn + u.z * u.z / (den * Rn);
+  d2 += (a * ut * ut + b * (uu - ut * ut)) * iRt + 2.0 * u.z * mu * iden * ut * iRn + u.z * u.z * iden * iRn;
+  return 0.5 * (Rt * (gt0 * gt0 + gt1 * gt1) + Rn * gn * gn);
 }
 __device__ __forceinline__ f3 sym3_mul(const float* G, f3 u) {
   return mk(G[0] * u.x + G[3] * u.y + G[4] * u.z, G[3] * u.x + G[1] * u.y + G[5] * u.z,
             G[4] * u.x + G[5] * u.y + G[2] * u.z);
 }
 
-// scatter J_b^T G J_b (6x6 lower packed) into the block-Jacobi accumulator of body b
-__device__ __forceinline__ void scatter_block(BodyLDS& B, int b, const Frame& F, f3 r, const float* G) {
+// scatter J_b^T G J_b (6x6 lower packed) into the block-Jacobi accumulator of body slot b
+__device__ __forceinline__ void scatter_block(BodyLDS& B, int b, const CRow& F, f3 r, const float* G) {
   float J[3][6];
   f3 Fk[3] = {F.t1, F.t2, F.n};
 #pragma unroll
@@ -242,7 +245,8 @@ __device__ __forceinline__ void scatter_block(BodyLDS& B, int b, const Frame& F,
   }
 }
 
-// 6x6 Cholesky of the packed lower triangle, in place; returns false if not SPD
+// 6x6 Cholesky of the packed lower triangle, in place; the diagonal slots hold the
+// reciprocals 1/L_jj so that the solves only multiply.  Returns false if not SPD.
 #define LI(i, j) L[(i) * ((i) + 1) / 2 + (j)]
 __device__ __forceinline__ bool chol6(float* L) {
 #pragma unroll
@@ -251,9 +255,8 @@ __device__ __forceinline__ bool chol6(float* L) {
 #pragma unroll
     for (int k = 0; k < j; ++k) s -= LI(j, k) * LI(j, k);
     if (!(s > 0.f)) return false;
-    float d = sqrtf(s);
-    LI(j, j) = d;
-    float id = 1.f / d;
+    float id = rsqrtf(s);
+    LI(j, j) = id;
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       float t = LI(i, j);
@@ -264,6 +267,7 @@ __device__ __forceinline__ bool chol6(float* L) {
   }
   return true;
 }
+// x = (L L^T)^-1 b with the factor of chol6
 __device__ __forceinline__ void chol6_solve(const float* L, const float* b, float* x) {
   float y[6];
 #pragma unroll
@@ -271,14 +275,14 @@ __device__ __forceinline__ void chol6_solve(const float* L, const float* b, floa
     float s = b[i];
 #pragma unroll
     for (int k = 0; k < i; ++k) s -= LI(i, k) * y[k];
-    y[i] = s / LI(i, i);
+    y[i] = s * LI(i, i);
   }
 #pragma unroll
   for (int i = 5; i >= 0; --i) {
     float s = y[i];
 #pragma unroll
     for (int k = i + 1; k < 6; ++k) s -= LI(k, i) * x[k];
-    x[i] = s / LI(i, i);
+    x[i] = s * LI(i, i);
   }
 }
 #undef LI
@@ -343,78 +347,163 @@ __device__ void contact_setup(int nc, float h, const NaConfig& cfg, const Contac
     float wd = wsum * (1.f / 3.f);
     float rn_soft = 1.f / (h * C.k[c] * (h + cfg.tau_d));
     float rn_rigid = cfg.beta * cfg.beta / (4.f * 9.8696044010893586f) * wd;
-    X.Rn[c] = fmaxf(rn_rigid, rn_soft);
-    X.Rt[c] = cfg.sigma * wd;
+    float Rn = fmaxf(rn_rigid, rn_soft), Rt = cfg.sigma * wd;
+    X.Rn[c] = Rn;
+    X.Rt[c] = Rt;
+    X.iRn[c] = 1.0 / (double)Rn;
+    X.iRt[c] = 1.0 / (double)Rt;
     X.vhat[c] = -C.phi0[c] / (h + cfg.tau_d);
   }
   wave_sync();
 }
 
-// FP64 evaluation at the iterate in B.vd: gamma, G (f32), J^T gamma into B.accd (zeroed here),
-// contact velocities into X.vc0.  Returns the summed contact cost (wave total).
-__device__ double eval_contacts(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X, int nb) {
-  for (int i = lane_id(); i < nb; i += NA_WAVE) {
+// Contact passes.  Lane l handles contacts l and l + 64 (at most NA_MAX_CONTACTS = 128); the
+// NG = 1 or 2 groups are loaded together before any arithmetic, so one wait covers all
+// gathers.  Later passes over X.vc0 / X.dvc keep this assignment and need no barrier.
+static_assert(NA_MAX_CONTACTS <= 2 * NA_WAVE, "contact passes assume at most two contacts per lane");
+
+// FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
+// lanes): gamma, G (f32), J^T gamma into B.accd, contact velocities into X.vc0.  Returns this
+// lane's part of the contact cost.
+template <int NG>
+__device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X) {
+  const int l = lane_id();
+  CRow r[NG];
+  d3 vc[NG];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) B.accd[k][i] = 0.0;
-  }
-  wave_sync();
+  for (int g = 0; g < NG; ++g) r[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
+#pragma unroll
+  for (int g = 0; g < NG; ++g) vc[g] = row_vel_d(B, r[g]);
   double cost = 0.0;
-  for (int c = lane_id(); c < nc; c += NA_WAVE) {
-    Frame F = load_frame(C, X, c);
-    d3 vc = contact_vel_d(B, C, X, F, c);
-    X.vc0[0][c] = vc.x; X.vc0[1][c] = vc.y; X.vc0[2][c] = vc.z;
-    double Rt = X.Rt[c], Rn = X.Rn[c];
-    d3 y = d3{-vc.x / Rt, -vc.y / Rt, -(vc.z - (double)X.vhat[c]) / Rn};
-    d3 g;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int c = l + g * NA_WAVE;
+    const bool valid = c < nc;
+    X.vc0[0][c] = vc[g].x; X.vc0[1][c] = vc[g].y; X.vc0[2][c] = vc[g].z;
+    double iRt = X.iRt[c], iRn = X.iRn[c];
+    d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)X.vhat[c]) * iRn};
+    d3 gam;
     float G[6];
-    cost += sap_project(C.mu[c], Rt, Rn, y, g, G);
+    double cc = sap_project(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, gam, G);
+    cost += valid ? cc : 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) X.G[k][c] = G[k];
-    scatter_JT_d(B, C, X, F, c, g);
+    row_scatter_d(B, r[g], gam);
   }
   wave_sync();
-  return wave_sum_d(cost);
+  return cost;
+}
+__device__ __forceinline__ double eval_contacts(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X) {
+  return nc > NA_WAVE ? eval_contacts_g<2>(nc, C, B, X) : eval_contacts_g<1>(nc, C, B, X);
 }
 
-// B.acc = J^T G J p for p in B.v (zeroes B.acc first), f32
-__device__ void apply_JGJ(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X, int nb) {
-  for (int i = lane_id(); i < nb; i += NA_WAVE) {
+// B.acc += J^T G J p for p in B.v (B.acc zeroed by the caller, both visible), f32
+template <int NG>
+__device__ __forceinline__ void apply_JGJ_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
+  const int l = lane_id();
+  CRow r[NG];
+  float G[NG][6];
+  f3 u[NG];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) B.acc[k][i] = 0.f;
+  for (int g = 0; g < NG; ++g) {
+    const int c = l + g * NA_WAVE;
+    r[g] = load_row(C, X, c, c < nc);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) G[g][k] = X.G[k][c];
   }
+#pragma unroll
+  for (int g = 0; g < NG; ++g) u[g] = row_vel(B, r[g]);
+#pragma unroll
+  for (int g = 0; g < NG; ++g) row_scatter(B, r[g], sym3_mul(G[g], u[g]));
   wave_sync();
-  for (int c = lane_id(); c < nc; c += NA_WAVE) {
-    Frame F = load_frame(C, X, c);
-    f3 u = contact_vel(B, C, X, F, c);
-    float G[6] = {X.G[0][c], X.G[1][c], X.G[2][c], X.G[3][c], X.G[4][c], X.G[5][c]};
-    f3 w = sym3_mul(G, u);
-    scatter_JT(B, C, X, F, c, w);
+}
+__device__ __forceinline__ void apply_JGJ(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
+  if (nc > NA_WAVE) apply_JGJ_g<2>(nc, C, B, X);
+  else apply_JGJ_g<1>(nc, C, B, X);
+}
+
+// Block-Jacobi accumulation: B.pre[b] += J_b^T G J_b for both sides of every contact.
+template <int NG>
+__device__ __forceinline__ void precond_scatter_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
+  const int l = lane_id();
+  CRow r[NG];
+  float G[NG][6];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int c = l + g * NA_WAVE;
+    r[g] = load_row(C, X, c, c < nc);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) G[g][k] = X.G[k][c];
+  }
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    if (r[g].ha) scatter_block(B, r[g].sa, r[g], r[g].ra, G[g]);
+    if (r[g].hb) scatter_block(B, r[g].sb, r[g], r[g].rb, G[g]);
   }
   wave_sync();
 }
 
-// FP64 line-search evaluation of the contact part at alpha (wave totals of l, l', l'').
-__device__ __forceinline__ void ls_contacts(int nc, const ContactLDS& C, const SolveLDS& X, double alpha, double& l,
-                                            double& dl, double& d2l) {
-  double a = 0.0, b = 0.0, c2 = 0.0;
-  for (int c = lane_id(); c < nc; c += NA_WAVE) {
-    d3 dv = d3{X.dvc[0][c], X.dvc[1][c], X.dvc[2][c]};
-    d3 vc = d3{X.vc0[0][c] + alpha * dv.x, X.vc0[1][c] + alpha * dv.y, X.vc0[2][c] + alpha * dv.z};
-    double Rt = X.Rt[c], Rn = X.Rn[c];
-    d3 y = d3{-vc.x / Rt, -vc.y / Rt, -(vc.z - (double)X.vhat[c]) / Rn};
-    d3 g;
-    a += sap_project(C.mu[c], Rt, Rn, y, g, nullptr);
-    b -= ddot(g, dv);
-    c2 += sap_curv(C.mu[c], Rt, Rn, y, g, dv);
+// Line-search setup: contact-space direction X.dvc = J dx (dx in B.vd) and this lane's contact
+// terms at alpha = 0 (slope d1c0, curvature d2c0) and alpha = 1 (cost lc1, slope d1c1).
+template <int NG>
+__device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const BodyLDS& B, SolveLDS& X, double& d1c0,
+                                           double& d2c0, double& lc1, double& d1c1) {
+  const int l = lane_id();
+  CRow r[NG];
+  d3 u[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) r[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
+#pragma unroll
+  for (int g = 0; g < NG; ++g) u[g] = row_vel_d(B, r[g]);
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int c = l + g * NA_WAVE;
+    X.dvc[0][c] = u[g].x; X.dvc[1][c] = u[g].y; X.dvc[2][c] = u[g].z;
+    double iRt = X.iRt[c], iRn = X.iRn[c], Rt = X.Rt[c], Rn = X.Rn[c], mu = C.mu[c];
+    double vh = (double)X.vhat[c];
+    d3 w0 = d3{X.vc0[0][c], X.vc0[1][c], X.vc0[2][c]};
+    d3 y0 = d3{-w0.x * iRt, -w0.y * iRt, -(w0.z - vh) * iRn};
+    d3 y1 = d3{-(w0.x + u[g].x) * iRt, -(w0.y + u[g].y) * iRt, -(w0.z + u[g].z - vh) * iRn};
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+    sap_line(mu, Rt, Rn, iRt, iRn, y0, u[g], a1, a2);
+    double cb = sap_line(mu, Rt, Rn, iRt, iRn, y1, u[g], b1, b2);
+    if (c < nc) {
+      d1c0 += a1;
+      d2c0 += a2;
+      lc1 += cb;
+      d1c1 += b1;
+    }
   }
-  l = wave_sum_d(a);
-  dl = wave_sum_d(b);
-  d2l = wave_sum_d(c2);
+}
+
+// Contact part of the line function at alpha for this lane's contacts (FP64): adds the cost,
+// its first and second derivatives.
+template <int NG>
+__device__ __forceinline__ void ls_contacts_g(int nc, const ContactLDS& C, const SolveLDS& X, double alpha, double& l,
+                                              double& dl, double& d2l) {
+  const int ln = lane_id();
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int c = ln + g * NA_WAVE;
+    d3 dv = d3{X.dvc[0][c], X.dvc[1][c], X.dvc[2][c]};
+    double iRt = X.iRt[c], iRn = X.iRn[c];
+    d3 y = d3{-(X.vc0[0][c] + alpha * dv.x) * iRt, -(X.vc0[1][c] + alpha * dv.y) * iRt,
+              -(X.vc0[2][c] + alpha * dv.z - (double)X.vhat[c]) * iRn};
+    double a = 0.0, b = 0.0;
+    double cc = sap_line(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, dv, a, b);
+    if (c < nc) {
+      l += cc;
+      dl += a;
+      d2l += b;
+    }
+  }
 }
 
 // Newton solve with FP64 residuals and FP32 PCG directions.  Lane b < nb owns body b:
 // `br` its constants, `v` its velocity (in: guess, out: solution).  Returns true when
-// the SAP optimality criterion holds.
+// the SAP optimality criterion holds.  With cfg.cg_forcing > 0 the PCG tolerance of Newton
+// iteration k is eta_k = min(cg_forcing, sqrt(|Dg_k| / |Dg_0|)), never tighter than needed
+// to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
 __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X,
                           const BodyReg& br, double* v, SolveStats& st) {
   const int l = lane_id();
@@ -426,15 +515,28 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
   st.reason = NA_EXIT_MAX_ITER;
   st.residual = 0.f;
   st.t_eval = st.t_pre = st.t_cg = st.t_ls = 0;
-  for (int it = 0; it <= cfg.max_iterations; ++it) {
-    // gradient g = A (v - v*) - J^T gamma, FP64
+  // settings in registers for the loops below
+  const int max_iterations = cfg.max_iterations, max_cg = cfg.max_cg;
+  const double opt_abs = cfg.opt_abs_tol, opt_rel = cfg.opt_rel_tol, ls_rel = cfg.ls_rel_tol;
+  const double cg_rel = cfg.cg_rel_tol, forcing = cfg.cg_forcing;
+  const bool two = nc > NA_WAVE;
+  if (l == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { B.v[k][NA_DUMMY] = 0.f; B.vd[k][NA_DUMMY] = 0.0; }
+  }
+  double gn_first = 0.0;
+  for (int it = 0; it <= max_iterations; ++it) {
+    // gradient g = A (v - v*) - J^T gamma, FP64.  The owner lane also clears its
+    // accumulators for this iteration (J^T gamma, preconditioner blocks).
     NA_TIC(t0);
     if (own) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) B.vd[k][l] = v[k];
+      for (int k = 0; k < 6; ++k) { B.vd[k][l] = v[k]; B.accd[k][l] = 0.0; }
+#pragma unroll
+      for (int k = 0; k < 21; ++k) B.pre[k][l] = 0.f;
     }
     wave_sync();
-    double ccost = eval_contacts(nc, C, B, X, nb);
+    double ccost = eval_contacts(nc, C, B, X);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -461,35 +563,26 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     pn = sqrt(wave_sum_d(pn));
     jn = sqrt(wave_sum_d(jn));
     const double mom0 = wave_sum_d(mom);
-    const double l0 = mom0 + ccost;
+    const double l0 = mom0 + wave_sum_d(ccost);
     NA_TOC(st.t_eval, t0);
     st.residual = (float)(gn / fmax(fmax(pn, jn), 1e-300));
     if (!(gn == gn)) {
       st.reason = NA_EXIT_NAN;
       return false;
     }
-    if (gn <= (double)cfg.opt_abs_tol + (double)cfg.opt_rel_tol * fmax(pn, jn)) {
+    const double target = opt_abs + opt_rel * fmax(pn, jn);
+    if (gn <= target) {
       st.converged = true;
       st.reason = NA_EXIT_CONVERGED;
       return true;
     }
-    if (it == cfg.max_iterations) break;
+    if (it == max_iterations) break;
+    if (it == 0) gn_first = gn;
 
     // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32)
     NA_TIC(t1);
-    for (int i = l; i < nb; i += NA_WAVE) {
-#pragma unroll
-      for (int k = 0; k < 21; ++k) B.pre[k][i] = 0.f;
-    }
-    wave_sync();
-    for (int c = l; c < nc; c += NA_WAVE) {
-      Frame F = load_frame(C, X, c);
-      float G[6] = {X.G[0][c], X.G[1][c], X.G[2][c], X.G[3][c], X.G[4][c], X.G[5][c]};
-      int ba = C.ba[c], bb = C.bb[c];
-      if (ba >= 0) scatter_block(B, ba, F, mk(X.ra[0][c], X.ra[1][c], X.ra[2][c]), G);
-      if (bb >= 0) scatter_block(B, bb, F, mk(X.rb[0][c], X.rb[1][c], X.rb[2][c]), G);
-    }
-    wave_sync();
+    if (two) precond_scatter_g<2>(nc, C, B, X);
+    else precond_scatter_g<1>(nc, C, B, X);
     float L[21];
     bool spd = true;
     if (own) {
@@ -510,6 +603,12 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
 
     // PCG on H x = -g in f32, right-hand side scaled to O(1)
     NA_TIC(t2);
+    double eta = cg_rel;
+    if (forcing > 0.0) {
+      double e = fmin(sqrt(gn / gn_first), forcing);
+      e = fmax(e, fmax(eta, 0.1 * target / gn));
+      eta = fmin(e, fmax(forcing, eta));
+    }
     const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
     float x[6], r[6], z[6], pp[6];
     float rz = 0.f;
@@ -519,14 +618,14 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
 #pragma unroll
     for (int k = 0; k < 6; ++k) { pp[k] = z[k]; rz += r[k] * z[k]; }
     rz = wave_sum(rz);
-    const float tol2 = cfg.cg_rel_tol * cfg.cg_rel_tol * rz;
-    for (int k = 0; k < cfg.max_cg && rz > tol2 && rz > 0.f; ++k) {
+    const float tol2 = (float)(eta * eta) * rz;
+    for (int k = 0; k < max_cg && rz > tol2 && rz > 0.f; ++k) {
       if (own) {
 #pragma unroll
-        for (int q = 0; q < 6; ++q) B.v[q][l] = pp[q];
+        for (int q = 0; q < 6; ++q) { B.v[q][l] = pp[q]; B.acc[q][l] = 0.f; }
       }
       wave_sync();
-      apply_JGJ(nc, C, B, X, nb);
+      apply_JGJ(nc, C, B, X);
       float Hp[6];
       float pHp = 0.f;
       if (own) {
@@ -559,18 +658,16 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     NA_TOC(st.t_cg, t2);
     NA_TIC(t3);
 
-    // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a)
+    // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a).
+    // One pass computes the contact-space direction and the contact terms at a = 0 and 1.
     if (own) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) B.vd[q][l] = dx[q];
     }
     wave_sync();
-    for (int c = l; c < nc; c += NA_WAVE) {
-      Frame F = load_frame(C, X, c);
-      d3 u = contact_vel_d(B, C, X, F, c);
-      X.dvc[0][c] = u.x; X.dvc[1][c] = u.y; X.dvc[2][c] = u.z;
-    }
-    wave_sync();
+    double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
+    if (two) ls_setup_g<2>(nc, C, B, X, d1c0, d2c0, lc1, d1c1);
+    else ls_setup_g<1>(nc, C, B, X, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
@@ -580,18 +677,15 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     }
     aa = wave_sum_d(aa);
     b0 = wave_sum_d(b0);
-    double lc, dlc, d2c;
-    ls_contacts(nc, C, X, 0.0, lc, dlc, d2c);
-    const double dl0 = b0 + dlc;
-    const double d2l0 = aa + d2c;
+    const double dl0 = b0 + wave_sum_d(d1c0);
+    const double d2l0 = aa + wave_sum_d(d2c0);
+    const double l1 = mom0 + b0 + 0.5 * aa + wave_sum_d(lc1);
+    const double dl1 = b0 + aa + wave_sum_d(d1c1);
     if (!(dl0 < 0.0)) {
       st.iterations = it + 1;
       st.reason = NA_EXIT_NOT_DESCENT;
       break;
     }
-    ls_contacts(nc, C, X, 1.0, lc, dlc, d2c);
-    const double l1 = mom0 + b0 + 0.5 * aa + lc;
-    const double dl1 = b0 + aa + dlc;
     double alpha;
     if (dl1 <= 0.0) {
       alpha = 1.0;
@@ -601,10 +695,12 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       if (!(alpha > 0.0)) alpha = fmin(fmax(-dl0 / d2l0, 0.0), 1.0);
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
       for (int k = 0; k < 60; ++k) {
-        ls_contacts(nc, C, X, alpha, lc, dlc, d2c);
-        double dl = b0 + alpha * aa + dlc;
-        double ddl = aa + d2c;
-        if (fabs(dl) <= (double)cfg.ls_rel_tol * fabs(dl0)) break;
+        double lc = 0.0, dlc = 0.0, d2c = 0.0;
+        if (two) ls_contacts_g<2>(nc, C, X, alpha, lc, dlc, d2c);
+        else ls_contacts_g<1>(nc, C, X, alpha, lc, dlc, d2c);
+        double dl = b0 + alpha * aa + wave_sum_d(dlc);
+        double ddl = aa + wave_sum_d(d2c);
+        if (fabs(dl) <= ls_rel * fabs(dl0)) break;
         if (dl < 0.0) lo = alpha;
         else hi = alpha;
         double an = ddl > 0.0 ? alpha - dl / ddl : -1.0;

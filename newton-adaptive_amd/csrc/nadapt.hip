@@ -905,7 +905,7 @@ static NaConfig to_cfg(const na_config* c) {
   k.max_contacts = c->max_rigid_contact < NA_MAX_CONTACTS ? c->max_rigid_contact : NA_MAX_CONTACTS;
   k.broad_phase = c->broad_phase; k.opt_rel_tol = c->optimality_rel_tol; k.opt_abs_tol = c->optimality_abs_tol;
   k.tau_d = c->contact_tau_d; k.sigma = c->sigma; k.beta = c->beta; k.ls_rel_tol = c->ls_rel_tol;
-  k.cg_rel_tol = c->cg_rel_tol; k.max_cg = c->max_cg;
+  k.cg_rel_tol = c->cg_rel_tol; k.max_cg = c->max_cg; k.cg_forcing = c->cg_forcing;
   return k;
 }
 
@@ -941,7 +941,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   if (buffers) for (size_t z : sizes_ext) need += (z + 255) & ~(size_t)255;
   else for (size_t z : sizes_own) need += (z + 255) & ~(size_t)255;
   if (hipMalloc(&pool, need) != hipSuccess) { delete s; return fail("na_solver_create: hipMalloc failed"); }
-  if (hipMemset(pool, 0, need) != hipSuccess) { hipFree(pool); delete s; return fail("na_solver_create: memset"); }
+  if (hipMemset(pool, 0, need) != hipSuccess) { (void)hipFree(pool); delete s; return fail("na_solver_create: memset"); }
   s->pool = pool;
   Ctrl& K = s->K;
   if (buffers) {
@@ -949,7 +949,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     if (!b->dt || !b->dt_half || !b->ideal_dt || !b->sim_time || !b->next_time || !b->last_raw_error ||
         !b->last_error || !b->accepted || !b->commit || !b->diverged || !b->substeps || !b->iterations ||
         !b->solve_fail || !b->overflow || !b->cum_accepted || !b->work || !b->body_q64) {
-      hipFree(pool);
+      (void)hipFree(pool);
       delete s;
       return fail("na_solver_create: every buffer pointer must be set when buffers are supplied");
     }
@@ -972,7 +972,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   s->queue = (int32_t*)take(4);
   s->flag = (int32_t*)take(4);
   if (hipHostMalloc(&s->flag_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
-    hipFree(pool);
+    (void)hipFree(pool);
     delete s;
     return fail("na_solver_create: hipHostMalloc failed");
   }
