@@ -56,6 +56,11 @@ typedef struct na_model_desc {
   const float* shape_collision_radius;
   const int32_t* filter_pairs;      /* [F,2] sorted canonical (min,max) */
   float gravity[3];
+  /* Optional [W] arrays: one past each world's last body / shape.  NULL means the start arrays
+     are packed prefix sums ([W+1], end[w] = start[w+1]); separate end arrays let a world pool
+     hold worlds in arbitrary arena ranges with gaps (serving). */
+  const int32_t* world_body_end;
+  const int32_t* world_shape_end;
 } na_model_desc;
 
 /* Solver settings (constructor kwargs of the reference solvers). */
@@ -134,6 +139,19 @@ int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, do
  * 4-byte flag read back per iteration.  Returns the status of the host loop. */
 int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
                   int32_t* iterations_out, void* stream);
+/* Continuous-batching rollout launch over the worlds listed in `active` (no counterpart in the
+   reference: the serving form of the boundary call).  Each listed world w runs boundary frames of
+   dt_outer until frames_done[w] reaches horizon_frames[w] or accepted[w] reaches
+   horizon_accepted[w] (status[w] = 1), its convex solve fails (status 2), or budget_ms of device
+   time has passed since the launch began (status stays 0; the world resumes exactly in a later
+   call).  A rollout of F frames leaves the same state as F calls of na_step.  world_tol (may be
+   NULL) gives a per-world error tolerance.  Asynchronous on `stream`; budget_ms <= 0 means no
+   budget. */
+int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
+             const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
+             const double* world_tol, float* body_q, float* body_qd, const float* body_f, double dt_outer,
+             double budget_ms, void* stream);
+
 /* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
 int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);
 

@@ -10,6 +10,7 @@ from .builder import ModelBuilder
 from .contacts import CollisionPipeline, Contacts
 from .model import Control, GeoType, Model, State, eval_fk
 from .solvers import SolverMuJoCoAdaptive, SolverSAPAdaptive
+from .serving import RolloutRequest, ServingEngine, requests_from_model, world_arrays
 from . import scenes
 
 

@@ -19,6 +19,8 @@ struct NaModel {
   int W, B, S, G, F;
   const int32_t* world_body_start;
   const int32_t* world_shape_start;
+  const int32_t* world_body_end;   // one past the world's last body (world_body_start + 1 for a packed model)
+  const int32_t* world_shape_end;  // one past the world's last shape
   const int32_t* global_shapes;
   const float* body_com;          // [B,3]
   const float* body_mass;         // [B]
@@ -463,6 +465,8 @@ __device__ __forceinline__ bool pair_passes(const NaModel& M, const SegLDS& S, i
   if (wa == -1 && wb == -1) return false;
   if (wa != -1 && wb != -1 && wa != wb) return false;
   if (!group_pair(M.shape_group[sa], M.shape_group[sb])) return false;
+  int ba = M.shape_body[sa], bb = M.shape_body[sb];
+  if (ba >= 0 && ba == bb) return false;  // shapes of one body never collide
   if (M.F > 0 && pair_excluded(M, sa, sb)) return false;
   return S.lo[0][i] <= S.hi[0][j] && S.hi[0][i] >= S.lo[0][j] && S.lo[1][i] <= S.hi[1][j] &&
          S.hi[1][i] >= S.lo[1][j] && S.lo[2][i] <= S.hi[2][j] && S.hi[2][i] >= S.lo[2][j];

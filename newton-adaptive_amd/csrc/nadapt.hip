@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <vector>
 
 #include "../../include/nadapt.h"
 #include "na_solve.h"
@@ -251,8 +252,8 @@ __device__ __forceinline__ WorldCtx world_ctx(const NaModel& M, int w) {
   WorldCtx c;
   c.w = w;
   c.b0 = M.world_body_start[w];
-  c.nb = M.world_body_start[w + 1] - c.b0;
-  c.nl = M.world_shape_start[w + 1] - M.world_shape_start[w];
+  c.nb = M.world_body_end[w] - c.b0;
+  c.nl = M.world_shape_end[w] - M.world_shape_start[w];
   c.nseg = c.nl + M.G;
   return c;
 }
@@ -551,6 +552,81 @@ __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, d
   }
 }
 
+// ---------------------------------------------------------------- rollout serving
+// Per-world rollout bookkeeping of the serving loop.  A world runs boundary frames of dt_outer
+// (open_frame, iterations until the boundary or max_substeps) until it has completed
+// horizon_frames frames or taken horizon_accepted accepted steps (the latter may end it
+// inside a frame).  The arithmetic of every iteration is the boundary call's, so a rollout of
+// F frames leaves the same state as F boundary calls.
+struct ServeSlots {
+  const int32_t* horizon_frames;    // [W]
+  const int32_t* horizon_accepted;  // [W]
+  int32_t* frames_done;             // [W]
+  int32_t* accepted;                // [W] accepted steps taken so far
+  int32_t* status;                  // [W] 0 running, 1 horizon reached, 2 convex solve failed
+  const double* tol;                // [W] per-world error tolerance, or null for cfg.tol
+};
+
+// Persistent serving launch over the worlds listed in `active`.  Waves take list entries from an
+// atomic counter and advance each world until its horizon, a solve failure or the time budget
+// (`budget` ticks of the constant-rate wall clock, measured from the wave's start); a world
+// stopped by the budget keeps its exact state (FP64 pose shadow, controller, frame position) and
+// resumes in a later launch.  Every wave exits: the loop ends when the list is exhausted or the
+// budget has passed, and each world loop runs at most max_substeps iterations per frame.
+__global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                              float* body_qd, const float* body_f, double dt_outer, double eff_max,
+                                              int32_t* queue, const int32_t* active, int n_active, ServeSlots SV,
+                                              long long budget) {
+  __shared__ WorldLDS S;
+  const long long t_end = wall_clock64() + budget;
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= n_active) break;
+    const int w = active[slot];
+    if (SV.status[w] != 0) continue;
+    WorldCtx wc = world_ctx(M, w);
+    NaConfig wcfg = cfg;
+    if (SV.tol) {
+      wcfg.tol = SV.tol[w];
+      wcfg.opt_rel_tol = (float)fmax(1.0e-3 * wcfg.tol, 1.0e-8);
+    }
+    double q[7];
+    float v[6], f[6];
+    load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
+    WCtrl c = load_ctrl(K, w);
+    long long wk[NA_WORK] = {};
+    int frames = SV.frames_done[w], accepted = SV.accepted[w], status = 0, acc = 0;
+    const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
+    // a world without any finite horizon has nothing to run (keeps every wave's loop finite)
+    if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
+    while (status == 0) {
+      if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
+        // at a frame boundary (or the frame's iteration cap): finish, or open the next frame
+        if (c.iterations > 0 || c.sim_time > 0.0) ++frames;
+        if (frames >= hf) { status = 1; break; }
+        open_frame(wcfg, c, dt_outer, eff_max);
+      }
+      int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, eff_max, wk);
+      acc += inc;
+      accepted += inc;
+      if (c.solve_fail) { status = 2; break; }
+      if (accepted >= ha) { status = 1; break; }
+      if (wall_clock64() >= t_end) break;
+    }
+    store_body(wc, q64, body_q, body_qd, q, v);
+    if (lane_id() == 0) {
+      store_ctrl(K, w, c);
+      SV.frames_done[w] = frames;
+      SV.accepted[w] = accepted;
+      SV.status[w] = status;
+      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    }
+    flush_work(K, w, wk);
+  }
+}
+
 // ---------------------------------------------------------------- lock-step boundary call
 __global__ void k_plain_open(NaModel M, NaConfig cfg, Ctrl K, double dt_outer, double eff_max) {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -631,6 +707,7 @@ __global__ void k_reset(int W, const uint8_t* mask, double dt_init, Ctrl K) {
   K.next_time[i] = 0.0;
   K.diverged[i] = 0;
   K.accepted[i] = 0;
+  K.iterations[i] = 0;
 }
 
 // ---------------------------------------------------------------- contacts API
@@ -859,6 +936,8 @@ extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
   NaModel& M = m->M;
   M.W = d->world_count; M.B = d->body_count; M.S = d->shape_count; M.G = d->global_count; M.F = d->filter_count;
   M.world_body_start = d->world_body_start; M.world_shape_start = d->world_shape_start;
+  M.world_body_end = d->world_body_end ? d->world_body_end : d->world_body_start + 1;
+  M.world_shape_end = d->world_shape_end ? d->world_shape_end : d->world_shape_start + 1;
   M.global_shapes = d->global_shapes;
   M.body_com = d->body_com; M.body_mass = d->body_mass; M.body_inv_mass = d->body_inv_mass;
   M.body_inertia = d->body_inertia; M.body_inv_inertia = d->body_inv_inertia;
@@ -869,20 +948,22 @@ extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
   M.gx = d->gravity[0]; M.gy = d->gravity[1]; M.gz = d->gravity[2];
   // validate per-world sizes against the wave-per-world limits
   std::string err;
-  int* wbs = new int[M.W + 1];
-  int* wss = new int[M.W + 1];
-  hipError_t e1 = hipMemcpy(wbs, d->world_body_start, sizeof(int) * (M.W + 1), hipMemcpyDeviceToHost);
-  hipError_t e2 = hipMemcpy(wss, d->world_shape_start, sizeof(int) * (M.W + 1), hipMemcpyDeviceToHost);
-  if (e1 != hipSuccess || e2 != hipSuccess) {
-    delete[] wbs; delete[] wss; delete m;
-    return fail("na_model_create: could not read world_body_start / world_shape_start (device pointers?)");
+  // the end arrays are either separate [W] arrays or the start arrays shifted by one
+  std::vector<int> wbs(M.W + 1), wss(M.W + 1), wbe(M.W + 1), wse(M.W + 1);
+  hipError_t e1 = hipMemcpy(wbs.data(), d->world_body_start, sizeof(int) * M.W, hipMemcpyDeviceToHost);
+  hipError_t e2 = hipMemcpy(wss.data(), d->world_shape_start, sizeof(int) * M.W, hipMemcpyDeviceToHost);
+  hipError_t e3 = hipMemcpy(wbe.data(), M.world_body_end, sizeof(int) * M.W, hipMemcpyDeviceToHost);
+  hipError_t e4 = hipMemcpy(wse.data(), M.world_shape_end, sizeof(int) * M.W, hipMemcpyDeviceToHost);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    delete m;
+    return fail("na_model_create: could not read the world body/shape ranges (device pointers?)");
   }
   m->max_nb = 0; m->max_nl = 0;
   for (int w = 0; w < M.W; ++w) {
-    m->max_nb = wbs[w + 1] - wbs[w] > m->max_nb ? wbs[w + 1] - wbs[w] : m->max_nb;
-    m->max_nl = wss[w + 1] - wss[w] > m->max_nl ? wss[w + 1] - wss[w] : m->max_nl;
+    if (wbe[w] < wbs[w] || wse[w] < wss[w]) { delete m; return fail("na_model_create: a world range ends before it starts"); }
+    m->max_nb = wbe[w] - wbs[w] > m->max_nb ? wbe[w] - wbs[w] : m->max_nb;
+    m->max_nl = wse[w] - wss[w] > m->max_nl ? wse[w] - wss[w] : m->max_nl;
   }
-  delete[] wbs; delete[] wss;
   m->max_seg = m->max_nl + M.G;
   if (m->max_nb > NA_MAX_BODIES) { delete m; return fail("na_model_create: a world has more than 64 bodies"); }
   if (m->max_seg > NA_MAX_SEG) {
@@ -1031,6 +1112,33 @@ extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float*
   int grid = default_grid(s);
   k_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
                                        s->order);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
+                        const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
+                        const double* world_tol, float* body_q, float* body_qd, const float* body_f,
+                        double dt_outer, double budget_ms, void* stream) {
+  if (!s || !body_q || !body_qd) return fail("na_serve: null argument");
+  if (n_active < 0 || n_active > s->model->M.W) return fail("na_serve: n_active must be in [0, world_count]");
+  if (n_active > 0 && (!active || !horizon_frames || !horizon_accepted || !frames_done || !accepted || !status))
+    return fail("na_serve: null slot array");
+  if (!(dt_outer > 0)) return fail("na_serve: dt_outer must be > 0");
+  if (n_active == 0) return 0;
+  const NaModel& M = s->model->M;
+  hipStream_t st = S_(stream);
+  int dev = 0, rate_khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+  long long budget = budget_ms > 0 ? (long long)(budget_ms * (double)rate_khz) : (1LL << 62);
+  double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
+  ServeSlots SV{horizon_frames, horizon_accepted, frames_done, accepted, status, world_tol};
+  k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
+  int grid = default_grid(s);
+  grid = grid < n_active ? grid : n_active;
+  k_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue, active,
+                               n_active, SV, budget);
   HIPCHK(hipGetLastError());
   return 0;
 }

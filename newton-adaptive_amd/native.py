@@ -54,6 +54,10 @@ class ModelHandle:
         d.filter_pairs = p(model.shape_collision_filter_pairs) or dummy_i.data_ptr()
         g = model.gravity.view(-1).cpu().tolist()
         d.gravity = (C.c_float * 3)(*g)
+        # a world pool keeps explicit [W] end arrays; a packed model leaves them NULL
+        wbe, wse = getattr(model, "world_body_end", None), getattr(model, "world_shape_end", None)
+        d.world_body_end = p(wbe) if wbe is not None else None
+        d.world_shape_end = p(wse) if wse is not None else None
         self.desc = d
         h = C.c_void_p()
         _lib.check(L.na_model_create(C.byref(d), C.byref(h)), "na_model_create")

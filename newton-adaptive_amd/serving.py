@@ -1,0 +1,487 @@
+"""Continuous-batching rollout serving of the step-doubling rigid-body step.
+
+A :class:`ServingEngine` keeps a device-resident pool of worlds.  Each pool slot owns an arena
+range of bodies and of shapes; the global shapes (ground, bin walls) are shared by every slot.
+The engine runs a host loop of *rounds*:
+
+1. admit pending requests, oldest first, while a slot, arena space and the HBM budget allow;
+   their arrays are copied into the pool behind the running launch on the same stream;
+2. launch ``na_serve`` over every resident world that has not finished: one persistent launch in
+   which waves take worlds from an atomic list and advance each one through its boundary frames
+   at its own adaptive dt until its horizon or the round's device-time budget;
+3. retire finished requests (final state copied out, latency recorded) and free their slots.
+
+Requests that arrive while others are on the GPU join the next round's launch, so their work
+overlaps on the device with the older rollouts.  A world interrupted by the budget resumes with
+its exact state, and its arithmetic never depends on the slot, arena position or the other
+worlds, so a request's result is bit-identical alone and in any mix.
+
+There is no counterpart of this loop in the reference; it wraps the same per-world boundary
+loop as ``SolverSAPAdaptive.step`` (``solver_sap_adaptive.py:877-975``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .model import _t
+from .solvers import SolverMuJoCoAdaptive, SolverSAPAdaptive
+
+_I32_MAX = 2**31 - 1
+
+
+@dataclass
+class RolloutRequest:
+    """One rollout: a single world (host arrays), its horizon and tolerance.
+
+    The world runs boundary frames of the engine's ``dt_outer`` until it has completed
+    ``horizon_frames`` frames or taken ``horizon_accepted`` accepted steps, whichever comes first
+    (``None`` = no limit of that kind; at least one must be set).
+    """
+
+    bodies: dict
+    shapes: dict
+    horizon_frames: int | None = None
+    horizon_accepted: int | None = None
+    tol: float | None = None
+    arrival: float = 0.0
+    rid: int = -1
+    # filled in by the engine
+    admitted_at: float = math.nan
+    finished_at: float = math.nan
+    status: int = 0
+    accepted: int = 0
+    frames: int = 0
+    body_q: np.ndarray | None = None
+    body_qd: np.ndarray | None = None
+    slot: int = -1
+    b0: int = -1
+    s0: int = -1
+
+    @property
+    def body_count(self) -> int:
+        return int(self.bodies["body_q"].shape[0])
+
+    @property
+    def shape_count(self) -> int:
+        return int(self.shapes["shape_type"].shape[0])
+
+    @property
+    def latency(self) -> float:
+        return self.finished_at - self.arrival
+
+
+_BODY_FIELDS = ("body_q", "body_qd", "body_com", "body_mass", "body_inv_mass", "body_inertia", "body_inv_inertia")
+_SHAPE_FIELDS = ("shape_type", "shape_body", "shape_collision_group", "shape_transform", "shape_scale",
+                 "shape_margin", "shape_gap", "shape_material_ke", "shape_material_mu", "shape_collision_radius")
+
+
+def world_arrays(model, w: int, state=None) -> tuple[dict, dict]:
+    """Host arrays of world ``w`` of a finalized :class:`Model` (shape bodies as local indices).
+    With ``state``, the world starts from that state's body_q / body_qd instead of the model's."""
+    wb, ws = model.host("world_body_start"), model.host("world_shape_start")
+    b0, b1, s0, s1 = int(wb[w]), int(wb[w + 1]), int(ws[w]), int(ws[w + 1])
+    bodies = {k: np.array(model.host(k)[b0:b1]) for k in _BODY_FIELDS}
+    if state is not None:
+        bodies["body_q"] = state["body_q"][b0:b1].copy()
+        bodies["body_qd"] = state["body_qd"][b0:b1].copy()
+    shapes = {k: np.array(model.host(k)[s0:s1]) for k in _SHAPE_FIELDS}
+    sb = shapes["shape_body"].astype(np.int64)
+    shapes["shape_body"] = np.where(sb >= 0, sb - b0, -1).astype(np.int32)
+    return bodies, shapes
+
+
+def requests_from_model(model, *, horizon_frames=None, horizon_accepted=None, tol=None, arrivals=None,
+                        worlds=None, state=None) -> list:
+    """One :class:`RolloutRequest` per world of ``model`` (scalars or per-world sequences).
+    ``state``: start the worlds from this :class:`State` (float32 pose and velocity)."""
+    worlds = range(model.world_count) if worlds is None else worlds
+    host_state = None
+    if state is not None:
+        host_state = {"body_q": state.body_q.detach().cpu().numpy(), "body_qd": state.body_qd.detach().cpu().numpy()}
+    out = []
+    for i, w in enumerate(worlds):
+        b, s = world_arrays(model, w, host_state)
+
+        def pick(x):
+            return x[i] if isinstance(x, (list, tuple, np.ndarray)) else x
+
+        out.append(RolloutRequest(b, s, horizon_frames=pick(horizon_frames), horizon_accepted=pick(horizon_accepted),
+                                  tol=pick(tol), arrival=float(pick(arrivals)) if arrivals is not None else 0.0,
+                                  rid=i))
+    return out
+
+
+class _Arena:
+    """First-fit allocator of contiguous index ranges in ``[0, capacity)``."""
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)]  # sorted (start, length)
+
+    def alloc(self, n: int):
+        if n == 0:
+            return 0
+        for i, (s, l) in enumerate(self.free):
+            if l >= n:
+                if l == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (s + n, l - n)
+                return s
+        return None
+
+    def release(self, start: int, n: int) -> None:
+        if n == 0:
+            return
+        self.free.append((start, n))
+        self.free.sort()
+        merged = []
+        for s, l in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == s:
+                merged[-1] = (merged[-1][0], merged[-1][1] + l)
+            else:
+                merged.append((s, l))
+        self.free = merged
+
+    @property
+    def free_total(self) -> int:
+        return sum(l for _, l in self.free)
+
+
+class WorldPool:
+    """Device arrays of a world pool, laid out like a :class:`Model` so the solver binds to it.
+
+    Bodies ``[0, body_capacity)`` and shapes ``[0, shape_capacity)`` are arena space; the global
+    shapes of ``globals_model`` follow at ``[shape_capacity, shape_capacity + G)`` so that, as in a
+    packed model, every global shape has a larger index than every world shape.  Slot ``w`` holds
+    its bodies in ``[world_body_start[w], world_body_end[w])`` and its shapes likewise.
+    """
+
+    def __init__(self, globals_model, max_worlds: int, body_capacity: int, shape_capacity: int, device):
+        self.device = torch.device(device)
+        d = self.device
+        g = globals_model.host("global_shapes").astype(np.int64)
+        G = len(g)
+        self.world_count = int(max_worlds)
+        self.body_count = int(body_capacity)
+        self.shape_count = int(shape_capacity) + G
+        self.shape_capacity = int(shape_capacity)
+        self.joint_count = self.body_count
+        self.joint_coord_count = 7 * self.body_count
+        self.joint_dof_count = 6 * self.body_count
+        self.gravity = globals_model.gravity.clone().to(d)
+        f32, i32 = dict(dtype=torch.float32, device=d), dict(dtype=torch.int32, device=d)
+        Bc, Sc = self.body_count, self.shape_count
+        self.body_q = torch.zeros((Bc, 7), **f32)
+        self.body_q[:, 6] = 1.0
+        self.body_qd = torch.zeros((Bc, 6), **f32)
+        self.body_f = torch.zeros((Bc, 6), **f32)
+        self.body_com = torch.zeros((Bc, 3), **f32)
+        self.body_mass = torch.zeros(Bc, **f32)
+        self.body_inv_mass = torch.zeros(Bc, **f32)
+        self.body_inertia = torch.zeros((Bc, 3, 3), **f32)
+        self.body_inv_inertia = torch.zeros((Bc, 3, 3), **f32)
+        self.shape_type = torch.zeros(Sc, **i32)
+        self.shape_body = torch.full((Sc,), -1, **i32)
+        self.shape_collision_group = torch.zeros(Sc, **i32)
+        self.shape_world = torch.full((Sc,), -1, **i32)
+        self.shape_transform = torch.zeros((Sc, 7), **f32)
+        self.shape_scale = torch.zeros((Sc, 3), **f32)
+        self.shape_margin = torch.zeros(Sc, **f32)
+        self.shape_gap = torch.zeros(Sc, **f32)
+        self.shape_material_ke = torch.zeros(Sc, **f32)
+        self.shape_material_mu = torch.zeros(Sc, **f32)
+        self.shape_collision_radius = torch.zeros(Sc, **f32)
+        self.shape_collision_filter_pairs = torch.zeros((0, 2), **i32)
+        W = self.world_count
+        self.world_body_start = torch.zeros(W, **i32)
+        self.world_body_end = torch.zeros(W, **i32)
+        self.world_shape_start = torch.zeros(W, **i32)
+        self.world_shape_end = torch.zeros(W, **i32)
+        self.global_shapes = torch.arange(self.shape_capacity, Sc, **i32)
+        # global shapes: copied with the same float32 rounding a finalized Model applies
+        gs = slice(self.shape_capacity, Sc)
+        self.shape_type[gs] = _t(globals_model.host("shape_type")[g], torch.int32, d)
+        self.shape_collision_group[gs] = _t(globals_model.host("shape_collision_group")[g], torch.int32, d)
+        for name in ("shape_transform", "shape_scale", "shape_margin", "shape_gap", "shape_material_ke",
+                     "shape_material_mu", "shape_collision_radius"):
+            getattr(self, name)[gs] = _t(globals_model.host(name)[g], torch.float32, d)
+
+    def state(self):
+        from .model import State
+
+        return State(self.body_q, self.body_qd, self.body_f)
+
+
+def _bytes_per_body() -> int:
+    # pool arrays (q 28, qd 24, f 24, com 12, mass 8, inertia 72) + FP64 pose shadow (56)
+    return 28 + 24 + 24 + 12 + 8 + 72 + 56
+
+
+def _bytes_per_shape() -> int:
+    return 4 * 4 + 28 + 12 + 6 * 4
+
+
+def _bytes_per_slot() -> int:
+    # controller vectors (7 f64, 3 u8, 4 i32), work counters (12 i64), serving vectors (5 i32, 1 f64),
+    # world ranges (4 i32)
+    return 7 * 8 + 3 + 4 * 4 + 12 * 8 + 5 * 4 + 8 + 4 * 4
+
+
+def check_request(r: RolloutRequest, global_count: int, budget_bytes: int) -> None:
+    """Raise ``ValueError`` for a request the pool can never hold."""
+    if r.horizon_frames is None and r.horizon_accepted is None:
+        raise ValueError("a request needs horizon_frames or horizon_accepted")
+    for h in (r.horizon_frames, r.horizon_accepted):
+        if h is not None and not 1 <= int(h) <= _I32_MAX - 1:
+            raise ValueError(f"horizons must be in [1, 2^31 - 2], got {h!r}")
+    if r.tol is not None and not float(r.tol) > 0.0:
+        raise ValueError(f"tol must be > 0, got {r.tol!r}")
+    if r.body_count < 1 or r.body_count > 64 or r.shape_count + global_count > 128:
+        raise ValueError("a world holds 1..64 bodies and at most 128 segment shapes (world + global)")
+    if ServingEngine.request_bytes(r) > budget_bytes:
+        raise ValueError("request larger than the HBM budget")
+
+
+@dataclass
+class ServingStats:
+    completed: int = 0
+    failed: int = 0
+    accepted_steps: int = 0
+    wall_s: float = 0.0
+    rounds: int = 0
+    latencies: list = field(default_factory=list)
+
+    def summary(self) -> dict:
+        lat = np.asarray(self.latencies, dtype=np.float64)
+        pct = (lambda p: float(np.percentile(lat, p))) if len(lat) else (lambda p: math.nan)
+        return {"completed": self.completed, "failed": self.failed, "accepted_steps": self.accepted_steps,
+                "wall_s": self.wall_s, "rounds": self.rounds,
+                "accepted_steps_per_s": self.accepted_steps / self.wall_s if self.wall_s > 0 else math.nan,
+                "latency_p50_s": pct(50), "latency_p99_s": pct(99),
+                "latency_max_s": float(lat.max()) if len(lat) else math.nan}
+
+
+class ServingEngine:
+    """Persistent request loop over a device-resident world pool (see the module docstring).
+
+    ``globals_model``: a finalized model whose world ``-1`` shapes are the shared environment.
+    ``hbm_budget_bytes``: admission limit on the resident bytes of admitted requests (bodies,
+    shapes and per-slot state); by default the whole pool.  ``budget_ms``: device time per
+    launch (<= 0: a launch runs every resident world to its horizon).  Remaining keyword
+    arguments go to the solver (``tol`` is the default per-request tolerance).
+    """
+
+    def __init__(self, globals_model, *, max_worlds: int = 4096, body_capacity: int | None = None,
+                 shape_capacity: int | None = None, hbm_budget_bytes: int | None = None, dt_outer: float = 0.01,
+                 budget_ms: float = 20.0, controller: str = "sap", device="cuda:0", **solver_kw):
+        if max_worlds < 1:
+            raise ValueError("max_worlds must be >= 1")
+        if not dt_outer > 0:
+            raise ValueError("dt_outer must be > 0")
+        body_capacity = int(body_capacity if body_capacity is not None else 32 * max_worlds)
+        shape_capacity = int(shape_capacity if shape_capacity is not None else 32 * max_worlds)
+        self.pool = WorldPool(globals_model, max_worlds, body_capacity, shape_capacity, device)
+        self.G = int(self.pool.global_shapes.numel())
+        self.dt_outer = float(dt_outer)
+        self.budget_ms = float(budget_ms)
+        cls = {"sap": SolverSAPAdaptive, "mujoco": SolverMuJoCoAdaptive}[controller]
+        solver_kw.setdefault("max_substeps", 256)
+        solver_kw.setdefault("world_order", "index")
+        self.solver = cls(self.pool, **solver_kw)
+        self.default_tol = self.solver._tol
+        d = self.pool.device
+        W = self.pool.world_count
+        i32 = dict(dtype=torch.int32, device=d)
+        self.horizon_frames = torch.zeros(W, **i32)
+        self.horizon_accepted = torch.zeros(W, **i32)
+        self.frames_done = torch.zeros(W, **i32)
+        self.accepted = torch.zeros(W, **i32)
+        self.status = torch.zeros(W, **i32)
+        self.world_tol = torch.full((W,), self.default_tol, dtype=torch.float64, device=d)
+        self.capacity_bytes = (body_capacity * _bytes_per_body() + shape_capacity * _bytes_per_shape()
+                               + W * _bytes_per_slot())
+        self.hbm_budget_bytes = int(hbm_budget_bytes) if hbm_budget_bytes is not None else self.capacity_bytes
+        self.resident_bytes = 0
+        self._bodies = _Arena(body_capacity)
+        self._shapes = _Arena(shape_capacity)
+        self._free_slots = list(range(W - 1, -1, -1))
+        self.pending: list[RolloutRequest] = []
+        self.resident: dict[int, RolloutRequest] = {}
+        self.finished: list[RolloutRequest] = []
+        self.stats = ServingStats()
+        self._stream = torch.cuda.current_stream(d)
+
+    # ------------------------------------------------------------------ admission
+    @staticmethod
+    def request_bytes(req: RolloutRequest) -> int:
+        return req.body_count * _bytes_per_body() + req.shape_count * _bytes_per_shape() + _bytes_per_slot()
+
+    def submit(self, reqs) -> None:
+        """Queue requests (kept in arrival order)."""
+        reqs = [reqs] if isinstance(reqs, RolloutRequest) else list(reqs)
+        for r in reqs:
+            check_request(r, self.G, self.hbm_budget_bytes)
+        self.pending.extend(reqs)
+        self.pending.sort(key=lambda r: r.arrival)
+
+    def admit(self, now: float) -> list:
+        """Admit arrived requests (oldest first) while a slot, arena space and the budget allow."""
+        batch = []
+        while self.pending and self.pending[0].arrival <= now and self._free_slots:
+            r = self.pending[0]
+            need = self.request_bytes(r)
+            if self.resident_bytes + need > self.hbm_budget_bytes:
+                break
+            b0 = self._bodies.alloc(r.body_count)
+            if b0 is None:
+                break
+            s0 = self._shapes.alloc(r.shape_count)
+            if s0 is None:
+                self._bodies.release(b0, r.body_count)
+                break
+            self.pending.pop(0)
+            r.slot, r.b0, r.s0 = self._free_slots.pop(), b0, s0
+            r.admitted_at = now
+            self.resident_bytes += need
+            self.resident[r.slot] = r
+            batch.append(r)
+        if batch:
+            self._upload(batch)
+        return batch
+
+    def _upload(self, batch) -> None:
+        P, d = self.pool, self.pool.device
+        bidx = np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in batch]).astype(np.int64)
+        sidx = np.concatenate([np.arange(r.s0, r.s0 + r.shape_count) for r in batch]).astype(np.int64)
+        bi = torch.as_tensor(bidx, device=d)
+        si = torch.as_tensor(sidx, device=d)
+        for k in _BODY_FIELDS:
+            vals = np.concatenate([r.bodies[k] for r in batch])
+            dst = getattr(P, k)
+            dst.index_copy_(0, bi, _t(vals, torch.float32, d).reshape((-1,) + tuple(dst.shape[1:])))
+        P.body_f.index_fill_(0, bi, 0.0)
+        # FP64 pose shadow = the float32 pose, as a first boundary call would seed it
+        self.solver._q64.index_copy_(0, bi, P.body_q.index_select(0, bi).double())
+        for k in _SHAPE_FIELDS:
+            vals = np.concatenate([r.shapes[k] for r in batch])
+            if k == "shape_body":
+                vals = np.concatenate([np.where(r.shapes[k] >= 0, r.shapes[k] + r.b0, -1) for r in batch])
+            dt = torch.int32 if k in ("shape_type", "shape_body", "shape_collision_group") else torch.float32
+            dst = getattr(P, k)
+            dst.index_copy_(0, si, _t(vals, dt, d).reshape((-1,) + tuple(dst.shape[1:])))
+        P.shape_world.index_copy_(0, si, torch.as_tensor(
+            np.concatenate([np.full(r.shape_count, r.slot) for r in batch]), dtype=torch.int32, device=d))
+        slots = torch.as_tensor([r.slot for r in batch], dtype=torch.int64, device=d)
+
+        def put(t, vals, dtype):
+            t.index_copy_(0, slots, torch.as_tensor(vals, dtype=dtype, device=d))
+
+        put(P.world_body_start, [r.b0 for r in batch], torch.int32)
+        put(P.world_body_end, [r.b0 + r.body_count for r in batch], torch.int32)
+        put(P.world_shape_start, [r.s0 for r in batch], torch.int32)
+        put(P.world_shape_end, [r.s0 + r.shape_count for r in batch], torch.int32)
+        put(self.horizon_frames, [r.horizon_frames if r.horizon_frames is not None else _I32_MAX for r in batch],
+            torch.int32)
+        put(self.horizon_accepted,
+            [r.horizon_accepted if r.horizon_accepted is not None else _I32_MAX for r in batch], torch.int32)
+        put(self.world_tol, [r.tol if r.tol is not None else self.default_tol for r in batch], torch.float64)
+        zeros = [0] * len(batch)
+        put(self.frames_done, zeros, torch.int32)
+        put(self.accepted, zeros, torch.int32)
+        put(self.status, zeros, torch.int32)
+        mask = torch.zeros(P.world_count, dtype=torch.uint8, device=d)
+        mask.index_fill_(0, slots, 1)
+        self._reset_mask = mask  # kept alive until the asynchronous reset has run
+        _lib.check(_lib.lib().na_reset(self.solver._h, mask.data_ptr(), _lib.stream_handle()), "na_reset")
+
+    # ------------------------------------------------------------------ rounds
+    def launch(self) -> int:
+        """Enqueue one persistent launch over the resident, unfinished worlds (oldest first)."""
+        live = sorted((r for r in self.resident.values() if r.status == 0), key=lambda r: (r.arrival, r.rid))
+        if not live:
+            return 0
+        self._active = torch.as_tensor([r.slot for r in live], dtype=torch.int32, device=self.pool.device)
+        P = self.pool
+        L = _lib.lib()
+        _lib.check(L.na_serve(self.solver._h, self._active.data_ptr(), len(live), self.horizon_frames.data_ptr(),
+                              self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
+                              self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
+                              P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
+                              self.budget_ms, _lib.stream_handle()), "na_serve")
+        self.stats.rounds += 1
+        return len(live)
+
+    def retire(self, now: float) -> list:
+        """Read per-world status after the launch; copy out and free every finished request."""
+        if not self.resident:
+            return []
+        status = self.status.cpu().numpy()
+        slots = [s for s, r in self.resident.items() if status[s] != 0]
+        if not slots:
+            return []
+        acc = self.accepted.cpu().numpy()
+        frames = self.frames_done.cpu().numpy()
+        done = [self.resident.pop(s) for s in slots]
+        bidx = torch.as_tensor(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in done]),
+                               dtype=torch.int64, device=self.pool.device)
+        q = self.pool.body_q.index_select(0, bidx).cpu().numpy()
+        qd = self.pool.body_qd.index_select(0, bidx).cpu().numpy()
+        o = 0
+        for r in done:
+            n = r.body_count
+            r.body_q, r.body_qd = q[o:o + n], qd[o:o + n]
+            o += n
+            r.status, r.accepted, r.frames = int(status[r.slot]), int(acc[r.slot]), int(frames[r.slot])
+            r.finished_at = now
+            self.resident_bytes -= self.request_bytes(r)
+            self._bodies.release(r.b0, r.body_count)
+            self._shapes.release(r.s0, r.shape_count)
+            self._free_slots.append(r.slot)
+            self.stats.accepted_steps += r.accepted
+            if r.status == 1:
+                self.stats.completed += 1
+                self.stats.latencies.append(r.latency)
+            else:
+                self.stats.failed += 1
+            self.finished.append(r)
+        return done
+
+    def run(self, requests=None, *, realtime: bool = True, max_wall_s: float | None = None) -> dict:
+        """Serve ``requests`` (plus anything already queued) to completion.
+
+        With ``realtime`` the requests' ``arrival`` times are seconds after the start of this call
+        and requests are admitted only once they have arrived; otherwise all are admitted at
+        once.  Returns :meth:`ServingStats.summary`.
+        """
+        if requests is not None:
+            self.submit(requests)
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0 if realtime else math.inf
+
+        self.admit(now())
+        while self.pending or self.resident:
+            if not self.resident:
+                wait = self.pending[0].arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                self.admit(now())
+                continue
+            self.launch()
+            self.admit(now())  # copies go behind the running launch on the same stream
+            self._stream.synchronize()
+            self.retire(time.perf_counter() - t0)
+            if max_wall_s is not None and time.perf_counter() - t0 > max_wall_s:
+                break
+        self.stats.wall_s = time.perf_counter() - t0
+        return self.stats.summary()
