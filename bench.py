@@ -1,17 +1,22 @@
-"""Accepted world-steps/s of the served adaptive step on one MI355X.
+"""Accepted world-steps/s of the served adaptive step on one MI355X, against the lock-step form.
 
-Workload (default): 4096 uneven worlds, each N ~ U{1..32} primitives from {sphere, box,
-capsule} (size lognormal(ln 0.08 m, 0.4)) dropped into a 1 m^2 walled bin, step-doubling
-error control at tol 1e-3, dt_outer = 10 ms per boundary call.  A "step" is one boundary call
-over all worlds.  `value` is accepted world-steps/s of the served form (one persistent launch
-per boundary call, worlds marched to their boundary on the device); the lock-step form (one
-launch per iteration over all worlds plus a 4-byte flag read, the reference's host loop) is
-timed on the same frames and reported beside it.
+Workload (default): 4096 uneven worlds, each N ~ U{1..32} primitives from {sphere, box, capsule}
+(size lognormal(ln 0.08 m, 0.4)) dropped into a 1 m^2 walled bin, step-doubling error control at
+tol 1e-3, dt_outer = 10 ms.  A "step" is one 10 ms boundary frame of every world.  `--warmup`
+untimed boundary calls let the piles land; the timed region then runs exactly `--steps` frames
+of every world from that settled state in three forms that leave bit-identical states:
 
-The JSON line also carries the HBM roofline of the dominant kernel (algorithmic bytes from
-the per-world work counters over the launch time measured with HIP events on the launch
-stream) and a CPU baseline: the FP64 CPU restatement timed on a bounded sample of the same
-workload on the host cores.
+* served (`value`): the rollout form of the serving engine -- one persistent launch (`na_serve`)
+  in which waves take worlds from an atomic list and march each one through all its frames at
+  its own adaptive dt, with no barrier between worlds or frames;
+* plain: the reference's host loop -- per frame, one launch over all worlds per step-doubling
+  iteration and a 4-byte flag read back per iteration, worlds in lock step;
+* frames: one persistent launch per frame (`na_step`), worlds synchronised at every boundary.
+
+The JSON line also carries the roofline of the dominant kernel (algorithmic bytes of the work
+recorded by the per-world device counters over the launch time measured with HIP events on the
+launch stream), a serving run with Poisson arrivals (p50/p99 rollout latency) and a CPU
+baseline: the FP64 CPU restatement timed on a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -19,6 +24,7 @@ from __future__ import annotations
 import argparse
 import importlib
 import json
+import math
 import os
 import sys
 import time
@@ -27,21 +33,22 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+KW = dict(dt_inner_init=1e-3, max_substeps=256)
 
 
-def algorithmic_bytes(model, work, adaptive=True):
-    """Algorithmic HBM bytes of the step-doubling attempts recorded in `work` [W, 6]
-    (attempts, Newton iterations, CG iterations, contacts, pairs, solves): per world-attempt
-    3 inner steps (body state 104 B/body r+w, mass properties 92 B/body, per Newton iteration
-    92 B/contact + 48 B/body), 2 collision passes (176 B/shape, 104 B/pair, 80 B/contact),
-    the error norm (12 B/coord + 8), 64 B of controller scalars and the masked commit
-    (3 x (4 nq + 4 nv + 52 nb))."""
+def algorithmic_bytes(nb, ns, work, adaptive=True):
+    """Algorithmic HBM bytes of the step-doubling attempts recorded in `work` [W, >=6]
+    (attempts, Newton iterations, CG iterations, contacts, pairs, solves) for worlds of `nb`
+    bodies and `ns` segment shapes: per world-attempt 3 inner steps (body state 104 B/body r+w,
+    mass properties 92 B/body, per Newton iteration 92 B/contact + 48 B/body), 2 collision
+    passes (176 B/shape, 104 B/pair, 80 B/contact), the error norm (12 B/coord + 8), 64 B of
+    controller scalars and the masked commit (3 x (4 nq + 4 nv + 52 nb))."""
     import numpy as np
 
     w = work.cpu().numpy().astype(np.float64)
-    att, newton, _cg, contacts, pairs, solves = (w[:, i] for i in range(6))  # noqa: F841
-    nb = model.bodies_per_world().astype(np.float64)
-    ns = model.shapes_per_world().astype(np.float64)
+    att, newton, contacts, pairs = w[:, 0], w[:, 1], w[:, 3], w[:, 4]
+    nb = np.asarray(nb, dtype=np.float64)
+    ns = np.asarray(ns, dtype=np.float64)
     nq, nv = 7 * nb, 6 * nb
     collisions = np.maximum(2 * att if adaptive else att, 1)
     cbar = contacts / collisions
@@ -51,8 +58,9 @@ def algorithmic_bytes(model, work, adaptive=True):
     return float(b.sum())
 
 
-def cpu_baseline(model_fn, tol, dt_outer, sample_worlds, frames, threads):
-    """Time the FP64 CPU restatement (OpenMP over worlds) on the first `sample_worlds` worlds."""
+def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, sample_worlds, frames, threads):
+    """Time the FP64 CPU restatement (OpenMP over worlds) on the first `sample_worlds` worlds of
+    the workload, starting from the same settled state as the timed GPU region."""
     import numpy as np
 
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
@@ -60,14 +68,13 @@ def cpu_baseline(model_fn, tol, dt_outer, sample_worlds, frames, threads):
 
     m = model_fn(sample_worlds)
     om = O.OracleModel(m)
-    cfg = O.make_config(tol=tol, dt_init=1e-3, dt_max=dt_outer, max_substeps=256, max_iterations=100,
-                        opt_rel_tol=max(1e-3 * tol, 1e-8))
+    cfg = O.make_config(tol=tol, dt_init=KW["dt_inner_init"], dt_max=dt_outer, max_substeps=KW["max_substeps"],
+                        max_iterations=100, opt_rel_tol=max(1e-3 * tol, 1e-8))
     s = O.OracleSolver(om, cfg)
-    q = m.host("body_q").astype(np.float32).astype(np.float64).copy()
-    qd = np.zeros((m.body_count, 6))
-    f = np.zeros((m.body_count, 6))
-    s.step(q, qd, f, dt_outer)  # warm-up frame
-    s.a["cum_accepted"][:] = 0
+    B = m.body_count
+    q = np.ascontiguousarray(q0[:B].astype(np.float64))
+    qd = np.ascontiguousarray(qd0[:B].astype(np.float64))
+    f = np.zeros((B, 6))
     t0 = time.perf_counter()
     for _ in range(frames):
         s.step(q, qd, f, dt_outer)
@@ -75,23 +82,30 @@ def cpu_baseline(model_fn, tol, dt_outer, sample_worlds, frames, threads):
     acc = int(s.a["cum_accepted"][0])
     return {"value": acc / el, "unit": "accepted world-steps/s", "cores": int(O.lib().or_num_threads()),
             "kind": "port",
-            "sample": f"FP64 CPU restatement, first {sample_worlds} worlds of the same workload, {frames} frames "
-                      f"of {dt_outer * 1e3:.0f} ms after 1 warm-up frame ({acc} accepted steps in {el:.1f} s)"}
+            "sample": f"FP64 CPU restatement (OpenMP over worlds), first {sample_worlds} worlds of the workload, "
+                      f"{frames} frames of {dt_outer * 1e3:.0f} ms from the same settled state "
+                      f"({acc} accepted steps in {el:.1f} s)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--dt-outer", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-frames", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1200)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--cpu-worlds", type=int, default=64)
     ap.add_argument("--cpu-frames", type=int, default=10)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="finish with a 1 GiB read + 1 GiB write elementwise kernel (scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; worlds shard across GPUs by contiguous ranges "
@@ -103,42 +117,63 @@ def main():
 
     na = importlib.import_module("newton-adaptive_amd")
     dev = "cuda:0"
+    kw = dict(KW, tol=args.tol, dt_inner_max=args.dt_outer)
 
-    def make(n):
-        return na.scenes.uneven_pile(n, seed=args.seed, device=dev)
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
 
-    def run(schedule, frames, warmup):
-        model = make(args.worlds)
-        state = model.state()
-        solver = na.SolverSAPAdaptive(model, tol=args.tol, dt_inner_init=1e-3, dt_inner_max=args.dt_outer,
-                                      max_substeps=256, schedule=schedule)
-        ctrl = model.control()
-        for _ in range(warmup):
-            solver.step(state, state, ctrl, None, args.dt_outer)
-        torch.cuda.synchronize()
-        solver.reset_compute_counter()
-        solver.reset_work_counters()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        e0.record()
-        for _ in range(frames):
-            solver.step(state, state, ctrl, None, args.dt_outer)
-        e1.record()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        gpu_ms = e0.elapsed_time(e1)
-        acc = solver.accepted_steps()
-        ok = bool(torch.isfinite(state.body_q).all())
-        return dict(model=model, solver=solver, wall=wall, gpu_ms=gpu_ms, accepted=acc, finite=ok,
-                    work=solver._work.clone())
+    model = na.scenes.uneven_pile(args.worlds, seed=args.seed, device=dev)
+    W = model.world_count
+    log(f"[bench] {W} worlds, {model.body_count} bodies; {args.warmup} warm-up frames")
+    st = model.state()
+    warm = na.SolverSAPAdaptive(model, **kw)
+    for _ in range(args.warmup):
+        warm.step(st, st, model.control(), None, args.dt_outer)
+    torch.cuda.synchronize()
+    q0, qd0 = st.body_q.clone(), st.body_qd.clone()
+    del warm
 
-    served = run("served", args.steps, args.warmup)
-    value = served["accepted"] / served["wall"]
-    wk = served["work"].sum(dim=0).tolist()
-    bytes_total = algorithmic_bytes(served["model"], served["work"])
-    launch_s = served["gpu_ms"] / 1e3 / args.steps
-    achieved = bytes_total / args.steps / launch_s / 1e9
+    # ---------------------------------------------------------------- served rollout (timed)
+    eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
+                           shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
+                           device=dev, **kw)
+    reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
+    eng.submit(reqs)
+    eng.admit(math.inf)
+    eng.solver.reset_work_counters()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    eng.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    eng.retire(wall)
+    assert not eng.resident, "every rollout finishes in the one launch without a time budget"
+    launch_ms = e0.elapsed_time(e1)
+    accepted = sum(r.accepted for r in reqs)
+    value = accepted / wall
+    slots = np.array([r.slot for r in reqs])
+    nb = np.zeros(W)
+    ns = np.zeros(W)
+    nb[slots] = [r.body_count for r in reqs]
+    ns[slots] = [r.shape_count + eng.G for r in reqs]
+    work = eng.solver._work.clone()
+    wk = work.sum(dim=0).tolist()
+    bytes_launch = algorithmic_bytes(nb, ns, work)
+    achieved = bytes_launch / (launch_ms / 1e3) / 1e9
+    wb = model.host("world_body_start")
+    q_srv = np.zeros((model.body_count, 7), np.float32)
+    qd_srv = np.zeros((model.body_count, 6), np.float32)
+    for r in reqs:
+        q_srv[wb[r.rid]:wb[r.rid + 1]] = r.body_q
+        qd_srv[wb[r.rid]:wb[r.rid + 1]] = r.body_qd
+    finite = bool(np.isfinite(q_srv).all())
+    log(f"[bench] served: {accepted} accepted steps in {wall:.3f} s ({value:.0f}/s), launch {launch_ms:.1f} ms")
+    del eng
+
     out = {
         "metric": "accepted world-steps/s at tol 1e-3 (served adaptive vs plain); p99 rollout latency",
         "value": value,
@@ -146,39 +181,85 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": served["wall"] / args.steps * 1e3,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (fp64 residual/controller/pose)",
         "data": "synthetic (seeded scenes)",
-        "config": {"workload": f"uneven_pile_{args.worlds}: {args.worlds} worlds x U{{1..32}} sphere/box/capsule "
-                               f"in a 1 m^2 bin, tol {args.tol:g}, dt_outer {args.dt_outer * 1e3:g} ms",
-                   "worlds": args.worlds, "tol": args.tol, "dt_outer": args.dt_outer, "seed": args.seed},
-        "served": {"accepted": served["accepted"], "wall_s": served["wall"], "finite": served["finite"],
-                   "attempts": int(wk[0]), "newton_iterations": int(wk[1]), "cg_iterations": int(wk[2]),
-                   "contacts": int(wk[3]), "pairs": int(wk[4])},
+        "config": {"workload": f"uneven_pile_{W}: {W} worlds x U{{1..32}} sphere/box/capsule in a 1 m^2 bin, "
+                               f"tol {args.tol:g}, {args.steps} frames of {args.dt_outer * 1e3:g} ms after "
+                               f"{args.warmup} warm-up frames",
+                   "worlds": W, "tol": args.tol, "dt_outer": args.dt_outer, "seed": args.seed},
+        "served": {"form": "persistent rollout launch (na_serve), no barrier between worlds or frames",
+                   "accepted": accepted, "wall_s": wall, "finite": finite, "attempts": int(wk[0]),
+                   "newton_iterations": int(wk[1]), "cg_iterations": int(wk[2]), "contacts": int(wk[3]),
+                   "pairs": int(wk[4])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_frame",
-                     "algorithmic_bytes_per_launch": bytes_total / args.steps, "launch_ms": launch_s * 1e3},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_serve",
+                     "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": launch_ms},
     }
+
+    # ---------------------------------------------------------------- lock-step forms
+    def frames_form(schedule):
+        s2 = na.State(q0.clone(), qd0.clone(), torch.zeros_like(qd0))
+        sol = na.SolverSAPAdaptive(model, schedule=schedule, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            sol.step(s2, s2, model.control(), None, args.dt_outer)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        acc = sol.accepted_steps()
+        same = bool(np.array_equal(s2.body_q.cpu().numpy(), q_srv) and np.array_equal(s2.body_qd.cpu().numpy(), qd_srv))
+        log(f"[bench] {schedule}: {acc} accepted in {el:.3f} s ({acc / el:.0f}/s), identical to served: {same}")
+        return {"value": acc / el, "accepted": acc, "wall_s": el, "identical_to_served": same}
+
     if not args.no_plain:
-        plain = run("plain", args.steps, args.warmup)
-        pv = plain["accepted"] / plain["wall"]
-        out["plain"] = {"value": pv, "accepted": plain["accepted"], "wall_s": plain["wall"],
-                        "iterations_last_frame": plain["solver"].last_plain_iterations}
-        out["served_over_plain"] = value / pv if pv > 0 else None
+        out["plain"] = frames_form("plain")
+        out["plain"]["form"] = "reference host loop: one launch per iteration over all worlds + 4-byte flag readback"
+        out["served_over_plain"] = value / out["plain"]["value"]
+    if not args.no_frames:
+        out["frames"] = frames_form("served")
+        out["frames"]["form"] = "one persistent launch per boundary frame (na_step)"
+
+    # ---------------------------------------------------------------- serving (Poisson arrivals)
+    if not args.no_serving:
+        smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=args.serve_rate,
+                                                   device=dev)
+        seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
+                                budget_ms=20.0, device=dev, **kw)
+        log(f"[bench] serving {len(sreqs)} requests at {args.serve_rate:g}/s")
+        summ = seng.run(sreqs, realtime=True, max_wall_s=120.0)
+        out["serving"] = dict(summ, requests=len(sreqs), arrival_rate_per_s=args.serve_rate,
+                              workload="1..64 primitives per world (uneven pile distribution), horizon U[200, 2000] "
+                                       "accepted steps, tol in {1e-2, 1e-3, 1e-4}, Poisson arrivals, "
+                                       "20 ms device-time rounds")
+        out["p99_rollout_latency_s"] = summ["latency_p99_s"]
+        del seng
+
+    # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(make_cpu_factory(na, args.seed), args.tol, args.dt_outer,
+
+        def cpu_model(n):
+            return na.scenes.uneven_pile(n, seed=args.seed, device="cpu")
+
+        out["cpu_baseline"] = cpu_baseline(cpu_model, q0.cpu().numpy(), qd0.cpu().numpy(), args.tol, args.dt_outer,
                                            args.cpu_worlds, args.cpu_frames, threads)
+    traffic_file = os.path.join(REPO, "profiles", "k_serve_uneven_pile_4096_traffic.json")
+    if os.path.exists(traffic_file) and W == 4096 and args.steps == 20 and args.warmup == 20:
+        with open(traffic_file) as fh:
+            tr = json.load(fh)
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = os.path.relpath(traffic_file, REPO)
     print(json.dumps(out))
-
-
-def make_cpu_factory(na, seed):
-    def f(n):
-        return na.scenes.uneven_pile(n, seed=seed, device="cpu")
-    return f
+    if args.copy_probe:
+        x = torch.ones(256 * 1024 * 1024, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

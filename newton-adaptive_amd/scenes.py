@@ -143,6 +143,25 @@ def uneven_pile(world_count: int = 4096, seed: int = 0, device="cuda:0", max_pri
     return builder.finalize(device)
 
 
+def serving_requests(count: int, seed: int = 0, rate: float | None = None, max_prims: int = 64,
+                     horizons=(200, 2000), tols=(1.0e-2, 1.0e-3, 1.0e-4), device="cpu"):
+    """Rollout requests of the serving workload: request ``i`` is world ``i`` of
+    ``uneven_pile(count, seed, max_prims=max_prims)`` (1..max_prims primitives), with a horizon
+    of U{horizons} accepted steps, a tolerance drawn from ``tols`` and Poisson arrivals at
+    ``rate`` per second (all at t = 0 when ``rate`` is None).  Returns ``(model, requests)``;
+    the model's global shapes (ground and bin) are the shared environment."""
+    from .serving import requests_from_model
+
+    model = uneven_pile(count, seed=seed, device=device, max_prims=max_prims)
+    rng = np.random.default_rng(seed + 1_000_003)
+    H = rng.integers(int(horizons[0]), int(horizons[1]) + 1, size=count)
+    T = rng.choice(np.asarray(tols, dtype=np.float64), size=count)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate else np.zeros(count)
+    reqs = requests_from_model(model, horizon_accepted=[int(h) for h in H], tol=[float(t) for t in T],
+                               arrivals=[float(a) for a in arrivals])
+    return model, reqs
+
+
 def shape_kinds(model) -> np.ndarray:
     """Per-world histogram of body shape types (diagnostic)."""
     st = model.host("shape_type")
