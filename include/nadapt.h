@@ -145,12 +145,14 @@ int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body
    horizon_accepted[w] (status[w] = 1), its convex solve fails (status 2), or budget_ms of device
    time has passed since the launch began (status stays 0; the world resumes exactly in a later
    call).  A rollout of F frames leaves the same state as F calls of na_step.  world_tol (may be
-   NULL) gives a per-world error tolerance.  Asynchronous on `stream`; budget_ms <= 0 means no
-   budget. */
+   NULL) gives a per-world error tolerance.  max_iterations > 0 caps the step-doubling iterations
+   each world takes in this call (1 = the lock-step form of the reference's host loop);
+   unfinished (device int32, may be NULL) is set to 1 when any listed world is still running.
+   Asynchronous on `stream`; budget_ms <= 0 means no budget. */
 int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
              const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
              const double* world_tol, float* body_q, float* body_qd, const float* body_f, double dt_outer,
-             double budget_ms, void* stream);
+             double budget_ms, int32_t max_iterations, int32_t* unfinished, void* stream);
 
 /* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
 int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);

@@ -77,7 +77,7 @@ SIGNATURES = {
     "na_step": (C.c_int, [vp, vp, vp, vp, C.c_double, vp]),
     "na_step_plain": (C.c_int, [vp, vp, vp, vp, C.c_double, C.POINTER(i32), vp]),
     "na_reset": (C.c_int, [vp, vp, vp]),
-    "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, vp]),
+    "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),

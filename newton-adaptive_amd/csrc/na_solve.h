@@ -34,9 +34,17 @@ struct BodyLDS {
   float im[NA_MAX_BODIES];       // inverse mass
   float v[6][NA_BSLOT];          // f32 exchange vector (PCG directions)
   float acc[6][NA_BSLOT];        // f32 scatter target (H p)
-  double vd[6][NA_BSLOT];        // f64 exchange vector (Newton iterate, line-search direction)
-  double accd[6][NA_BSLOT];      // f64 scatter target (J^T gamma)
-  float pre[21][NA_BSLOT];       // block-Jacobi accumulation (6x6 lower, row-major packed)
+  // The FP64 vectors are dead from the gradient (which consumes accd) until the line search
+  // (which rewrites vd), exactly the span in which the preconditioner blocks are accumulated
+  // and factorised, so the two share storage; this keeps a world's LDS under 40 KB (four
+  // worlds per CU).
+  union {
+    struct {
+      double vd[6][NA_BSLOT];    // f64 exchange vector (Newton iterate, line-search direction)
+      double accd[6][NA_BSLOT];  // f64 scatter target (J^T gamma)
+    };
+    float pre[21][NA_BSLOT];     // block-Jacobi accumulation (6x6 lower, row-major packed)
+  };
 };
 
 struct SolveLDS {
@@ -522,7 +530,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
   const bool two = nc > NA_WAVE;
   if (l == 0) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) { B.v[k][NA_DUMMY] = 0.f; B.vd[k][NA_DUMMY] = 0.0; }
+    for (int k = 0; k < 6; ++k) B.v[k][NA_DUMMY] = 0.f;
   }
   double gn_first = 0.0;
   for (int it = 0; it <= max_iterations; ++it) {
@@ -532,8 +540,10 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     if (own) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) { B.vd[k][l] = v[k]; B.accd[k][l] = 0.0; }
+    }
+    if (l == 0) {  // the static side's zero velocity (the slot is overwritten by the blocks)
 #pragma unroll
-      for (int k = 0; k < 21; ++k) B.pre[k][l] = 0.f;
+      for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
     double ccost = eval_contacts(nc, C, B, X);
@@ -579,8 +589,15 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     if (it == max_iterations) break;
     if (it == 0) gn_first = gn;
 
-    // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32)
+    // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32); its accumulator shares
+    // storage with vd / accd, so every lane must have read its J^T gamma first
     NA_TIC(t1);
+    wave_sync();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 21; ++k) B.pre[k][l] = 0.f;
+    }
+    wave_sync();
     if (two) precond_scatter_g<2>(nc, C, B, X);
     else precond_scatter_g<1>(nc, C, B, X);
     float L[21];
@@ -660,9 +677,14 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
 
     // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a).
     // One pass computes the contact-space direction and the contact terms at a = 0 and 1.
+    wave_sync();  // vd shares storage with the preconditioner blocks read above
     if (own) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) B.vd[q][l] = dx[q];
+    }
+    if (l == 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) B.vd[q][NA_DUMMY] = 0.0;
     }
     wave_sync();
     double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;

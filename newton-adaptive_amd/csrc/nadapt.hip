@@ -568,15 +568,18 @@ struct ServeSlots {
 };
 
 // Persistent serving launch over the worlds listed in `active`.  Waves take list entries from an
-// atomic counter and advance each world until its horizon, a solve failure or the time budget
-// (`budget` ticks of the constant-rate wall clock, measured from the wave's start); a world
-// stopped by the budget keeps its exact state (FP64 pose shadow, controller, frame position) and
-// resumes in a later launch.  Every wave exits: the loop ends when the list is exhausted or the
-// budget has passed, and each world loop runs at most max_substeps iterations per frame.
+// atomic counter and advance each world until its horizon, a solve failure, `max_iters`
+// iterations (<= 0: no limit) or the time budget (`budget` ticks of the constant-rate wall
+// clock, measured from the wave's start); a world stopped early keeps its exact state (FP64 pose
+// shadow, controller, frame position) and resumes in a later launch.  With max_iters = 1 every
+// launch advances each listed world by one step-doubling iteration: the lock-step form.
+// `unfinished` (optional) gets a bit set when any world is still running after its turn.  Every
+// wave exits: the loop ends when the list is exhausted or the budget has passed, and each world
+// loop runs at most max_substeps iterations per frame.
 __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
                                               float* body_qd, const float* body_f, double dt_outer, double eff_max,
                                               int32_t* queue, const int32_t* active, int n_active, ServeSlots SV,
-                                              long long budget) {
+                                              long long budget, int max_iters, int32_t* unfinished) {
   __shared__ WorldLDS S;
   const long long t_end = wall_clock64() + budget;
   for (;;) {
@@ -601,6 +604,7 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
     const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
     // a world without any finite horizon has nothing to run (keeps every wave's loop finite)
     if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
+    int iters = 0;
     while (status == 0) {
       if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
         // at a frame boundary (or the frame's iteration cap): finish, or open the next frame
@@ -613,6 +617,11 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
       accepted += inc;
       if (c.solve_fail) { status = 2; break; }
       if (accepted >= ha) { status = 1; break; }
+      if (++iters == max_iters) {
+        // lock-step form: a world whose frame just ended and whose last frame it was is done now
+        if (!(c.sim_time < c.next_time) && frames + 1 >= hf) { ++frames; status = 1; }
+        break;
+      }
       if (wall_clock64() >= t_end) break;
     }
     store_body(wc, q64, body_q, body_qd, q, v);
@@ -622,6 +631,7 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
       SV.accepted[w] = accepted;
       SV.status[w] = status;
       if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+      if (status == 0 && unfinished) atomicOr(unfinished, 1);
     }
     flush_work(K, w, wk);
   }
@@ -1119,7 +1129,8 @@ extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float*
 extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
                         const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
                         const double* world_tol, float* body_q, float* body_qd, const float* body_f,
-                        double dt_outer, double budget_ms, void* stream) {
+                        double dt_outer, double budget_ms, int32_t max_iterations, int32_t* unfinished,
+                        void* stream) {
   if (!s || !body_q || !body_qd) return fail("na_serve: null argument");
   if (n_active < 0 || n_active > s->model->M.W) return fail("na_serve: n_active must be in [0, world_count]");
   if (n_active > 0 && (!active || !horizon_frames || !horizon_accepted || !frames_done || !accepted || !status))
@@ -1137,8 +1148,9 @@ extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, c
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
   int grid = default_grid(s);
   grid = grid < n_active ? grid : n_active;
+  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
   k_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue, active,
-                               n_active, SV, budget);
+                               n_active, SV, budget, max_iterations, unfinished);
   HIPCHK(hipGetLastError());
   return 0;
 }

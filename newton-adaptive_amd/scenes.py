@@ -144,21 +144,23 @@ def uneven_pile(world_count: int = 4096, seed: int = 0, device="cuda:0", max_pri
 
 
 def serving_requests(count: int, seed: int = 0, rate: float | None = None, max_prims: int = 64,
-                     horizons=(200, 2000), tols=(1.0e-2, 1.0e-3, 1.0e-4), device="cpu"):
-    """Rollout requests of the serving workload: request ``i`` is world ``i`` of
-    ``uneven_pile(count, seed, max_prims=max_prims)`` (1..max_prims primitives), with a horizon
-    of U{horizons} accepted steps, a tolerance drawn from ``tols`` and Poisson arrivals at
-    ``rate`` per second (all at t = 0 when ``rate`` is None).  Returns ``(model, requests)``;
-    the model's global shapes (ground and bin) are the shared environment."""
+                     horizons=(200, 2000), tols=(1.0e-2, 1.0e-3, 1.0e-4), device="cpu", distinct: int | None = None):
+    """Rollout requests of the serving workload: request ``i`` starts from world ``i % distinct``
+    of ``uneven_pile(distinct, seed, max_prims=max_prims)`` (1..max_prims primitives), with its
+    own horizon of U{horizons} accepted steps, a tolerance drawn from ``tols`` and Poisson
+    arrivals at ``rate`` per second (all at t = 0 when ``rate`` is None).  ``distinct`` defaults
+    to ``count``.  Returns ``(model, requests)``; the model's global shapes (ground and bin) are
+    the shared environment."""
     from .serving import requests_from_model
 
-    model = uneven_pile(count, seed=seed, device=device, max_prims=max_prims)
+    distinct = count if distinct is None else min(int(distinct), count)
+    model = uneven_pile(distinct, seed=seed, device=device, max_prims=max_prims)
     rng = np.random.default_rng(seed + 1_000_003)
     H = rng.integers(int(horizons[0]), int(horizons[1]) + 1, size=count)
     T = rng.choice(np.asarray(tols, dtype=np.float64), size=count)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate else np.zeros(count)
     reqs = requests_from_model(model, horizon_accepted=[int(h) for h in H], tol=[float(t) for t in T],
-                               arrivals=[float(a) for a in arrivals])
+                               arrivals=[float(a) for a in arrivals], worlds=[i % distinct for i in range(count)])
     return model, reqs
 
 

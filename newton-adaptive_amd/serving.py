@@ -278,11 +278,23 @@ class ServingEngine:
     shapes and per-slot state); by default the whole pool.  ``budget_ms``: device time per
     launch (<= 0: a launch runs every resident world to its horizon).  Remaining keyword
     arguments go to the solver (``tol`` is the default per-request tolerance).
+
+    ``schedule="continuous"`` is the served form described in the module docstring.
+    ``schedule="static"`` is the fixed-batch baseline with the reference's host loop: when the
+    pool is empty it admits every arrived request that fits, then advances the whole batch in
+    lock step, one launch per step-doubling iteration followed by a 4-byte "any world
+    unfinished" readback, until every world of the batch has reached its horizon; results are
+    returned when the batch ends.  Both schedules give every request the same arithmetic, so
+    their results are bit-identical.
     """
 
     def __init__(self, globals_model, *, max_worlds: int = 4096, body_capacity: int | None = None,
                  shape_capacity: int | None = None, hbm_budget_bytes: int | None = None, dt_outer: float = 0.01,
-                 budget_ms: float = 20.0, controller: str = "sap", device="cuda:0", **solver_kw):
+                 budget_ms: float = 20.0, controller: str = "sap", schedule: str = "continuous", device="cuda:0",
+                 **solver_kw):
+        if schedule not in ("continuous", "static"):
+            raise ValueError(f"schedule must be 'continuous' or 'static', got {schedule!r}")
+        self.schedule = schedule
         if max_worlds < 1:
             raise ValueError("max_worlds must be >= 1")
         if not dt_outer > 0:
@@ -319,6 +331,8 @@ class ServingEngine:
         self.finished: list[RolloutRequest] = []
         self.stats = ServingStats()
         self._stream = torch.cuda.current_stream(d)
+        self._unfinished = torch.zeros(1, dtype=torch.int32, device=d)
+        self._unfinished_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
 
     # ------------------------------------------------------------------ admission
     @staticmethod
@@ -404,8 +418,9 @@ class ServingEngine:
         _lib.check(_lib.lib().na_reset(self.solver._h, mask.data_ptr(), _lib.stream_handle()), "na_reset")
 
     # ------------------------------------------------------------------ rounds
-    def launch(self) -> int:
-        """Enqueue one persistent launch over the resident, unfinished worlds (oldest first)."""
+    def launch(self, max_iterations: int = 0, unfinished: torch.Tensor | None = None) -> int:
+        """Enqueue one persistent launch over the resident, unfinished worlds (oldest first).
+        ``max_iterations`` > 0 caps the iterations per world (1: lock step)."""
         live = sorted((r for r in self.resident.values() if r.status == 0), key=lambda r: (r.arrival, r.rid))
         if not live:
             return 0
@@ -416,7 +431,9 @@ class ServingEngine:
                               self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
                               self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
                               P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
-                              self.budget_ms, _lib.stream_handle()), "na_serve")
+                              self.budget_ms if max_iterations <= 0 else 0.0, int(max_iterations),
+                              unfinished.data_ptr() if unfinished is not None else None,
+                              _lib.stream_handle()), "na_serve")
         self.stats.rounds += 1
         return len(live)
 
@@ -469,6 +486,9 @@ class ServingEngine:
         def now():
             return time.perf_counter() - t0 if realtime else math.inf
 
+        if self.schedule == "static":
+            return self._run_static(t0, now, max_wall_s)
+
         self.admit(now())
         while self.pending or self.resident:
             if not self.resident:
@@ -485,3 +505,27 @@ class ServingEngine:
                 break
         self.stats.wall_s = time.perf_counter() - t0
         return self.stats.summary()
+
+    def _run_static(self, t0, now, max_wall_s) -> dict:
+        iterations = 0
+        while self.pending or self.resident:
+            if not self.admit(now()):
+                wait = self.pending[0].arrival - (time.perf_counter() - t0)
+                time.sleep(min(max(wait, 0.0), 0.05))
+                continue
+            while True:  # one fixed batch, lock step
+                self.launch(max_iterations=1, unfinished=self._unfinished)
+                self._unfinished_host.copy_(self._unfinished, non_blocking=True)
+                self._stream.synchronize()
+                iterations += 1
+                if int(self._unfinished_host[0]) == 0:
+                    break
+                if max_wall_s is not None and time.perf_counter() - t0 > max_wall_s:
+                    break
+            self.retire(time.perf_counter() - t0)
+            if max_wall_s is not None and time.perf_counter() - t0 > max_wall_s:
+                break
+        self.stats.wall_s = time.perf_counter() - t0
+        out = self.stats.summary()
+        out["lockstep_iterations"] = iterations
+        return out

@@ -39,7 +39,7 @@ class _SolverBase:
                  dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=100,
                  contact_preset_variant="drake", line_search_variant="exact_root", contact_tau_d=0.01,
                  broad_phase="sap", optimality_rel_tol=None, schedule="served", cg_rel_tol=1.0e-4,
-                 cg_forcing=0.1, world_order="cost", **_kw):
+                 cg_forcing=0.1, world_order="cost", ls_rel_tol=1.0e-4, **_kw):
         if mode not in _MODE_CODES:
             raise ValueError(f"mode must be one of {tuple(_MODE_CODES)}, got {mode!r}.")
         if float(tol) <= 0.0:
@@ -64,6 +64,8 @@ class _SolverBase:
             raise ValueError(f"cg_rel_tol must be in (0, 1), got {cg_rel_tol!r}.")
         if not 0.0 <= float(cg_forcing) < 1.0:
             raise ValueError(f"cg_forcing must be in [0, 1), got {cg_forcing!r}.")
+        if not 0.0 < float(ls_rel_tol) < 1.0:
+            raise ValueError(f"ls_rel_tol must be in (0, 1), got {ls_rel_tol!r}.")
         if world_order not in ("cost", "index"):
             raise ValueError(f"world_order must be 'cost' or 'index', got {world_order!r}.")
         if schedule not in ("served", "plain"):
@@ -101,7 +103,7 @@ class _SolverBase:
         cfg.contact_tau_d = float(contact_tau_d)
         cfg.sigma = 1.0e-3
         cfg.beta = 1.0
-        cfg.ls_rel_tol = 1.0e-4
+        cfg.ls_rel_tol = float(ls_rel_tol)
         cfg.cg_rel_tol = float(cg_rel_tol)
         cfg.max_cg = 200
         cfg.cg_forcing = float(cg_forcing)

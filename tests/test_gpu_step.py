@@ -115,6 +115,32 @@ def test_attempt_error_norm_matches_oracle(na, O, name, frames):
         assert dq.max() <= 1e-5
 
 
+@pytest.mark.parametrize("frames", [3, 30])
+def test_stiff_stack_attempt_at_tol_1e4_matches_oracle(na, O, frames):
+    """Stiff contact stack (4 boxes, ke 1e6, mu ~ U[0.3, 1]) at tol 1e-4: error norms within 1e-6
+    and the doubled pose within 1e-5 of the FP64 oracle."""
+    model = na.scenes.box_stack(64, seed=5, device=DEV)
+    state = model.state()
+    solver = na.SolverSAPAdaptive(model, tol=1e-4, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
+    for _ in range(frames):
+        solver.step(state, state, model.control(), None, 0.01)
+    torch.cuda.synchronize()
+    q64 = solver.body_q64.clone()
+    qd = state.body_qd.clone()
+    f = torch.zeros_like(qd)
+    for dt in (2.5e-4, 1e-3):
+        dtt = torch.full((model.world_count,), dt, dtype=torch.float64, device=DEV)
+        out = GE._attempt(solver, q64, qd, f, dtt)
+        ref = O.attempt_all(O.OracleModel(model), O.make_config(tol=1e-4), q64.cpu().numpy(),
+                            qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy())
+        assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
+        same_nc = out["n_contacts"] == ref["n_contacts"]
+        assert same_nc.mean() > 0.95
+        de = np.abs(out["err"] - ref["err"])[same_nc]
+        assert de.max() <= 1e-6, f"max |d err| = {de.max():.2e}"
+        assert np.abs(out["q_double"] - ref["q_double"]).max() <= 1e-5
+
+
 def test_boundary_determinism_and_plain_equals_served(na):
     runs = {}
     for schedule in ("served", "plain", "served"):

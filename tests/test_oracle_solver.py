@@ -154,3 +154,29 @@ def test_tighter_tol_uses_more_steps_and_fixed_mode_is_constant(na, O):
     assert tight >= loose
     _, per = run(1e-3, mode="fixed")
     assert len(set(per)) == 1 and per[0] == 4 * 4
+
+
+def test_box_drop_1000_accepted_steps_comes_to_rest(na, O):
+    """The CPU-runnable small case: 64 one-box worlds (0.1 m, 1 kg, z0 ~ U[0.3, 0.8], random
+    orientation, mu 0.5) at tol 1e-3 from dt0 = 1e-3 until every world has taken 1000 accepted
+    steps.  Every box ends at rest on a face: COM height = half extent (within the contact
+    compliance), no velocity, one box axis vertical."""
+    m = na.scenes.box_drop(64, seed=0, device="cpu")
+    om = O.OracleModel(m)
+    s = O.OracleSolver(om, O.make_config(tol=1e-3, dt_init=1e-3, dt_max=0.01, max_substeps=256,
+                                         opt_rel_tol=1e-6))
+    q, qd, f = _state(m)
+    per_world = np.zeros(64, dtype=np.int64)
+    frames = 0
+    while per_world.min() < 1000:
+        assert s.step(q, qd, f, 0.01) == 0
+        per_world += s.a["substeps"]
+        frames += 1
+        assert frames < 5000
+    com_z = q[:, 2]  # the body origin is the box centre
+    assert np.all(np.abs(com_z - 0.05) < 2e-3), com_z
+    assert np.abs(qd).max() < 1e-3
+    x, y, z, w = q[:, 3], q[:, 4], q[:, 5], q[:, 6]
+    # third row of the rotation matrix: world z of the box axes
+    rz = np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=1)
+    assert np.all(np.abs(rz).max(axis=1) > 1 - 1e-4)

@@ -135,3 +135,29 @@ def test_admission_respects_hbm_budget():
     smallest = min(serving.ServingEngine.request_bytes(r) for r in reqs)
     assert 3 <= peak <= 3 * per // smallest and peak < 24
     assert eng.stats.completed == 24 and all(r.frames == 2 for r in reqs)
+
+
+@pytest.mark.gpu
+def test_static_lockstep_batches_equal_continuous():
+    """The fixed-batch baseline (one launch per iteration, 4-byte readback, batch held until its
+    longest rollout ends) gives every request the same arithmetic as continuous batching."""
+    model = na.scenes.uneven_pile(24, seed=31, device=DEV, max_prims=10)
+
+    def mk():
+        return na.requests_from_model(model, horizon_frames=[3 if i % 2 else None for i in range(24)],
+                                      horizon_accepted=[None if i % 2 else 25 + 4 * (i % 5) for i in range(24)],
+                                      tol=[[1e-2, 1e-3, 1e-4][i % 3] for i in range(24)])
+
+    a, b = mk(), mk()
+    sa = na.ServingEngine(model, max_worlds=8, body_capacity=160, shape_capacity=160, budget_ms=2.0, device=DEV,
+                          **KW).run(a, realtime=False)
+    sb = na.ServingEngine(model, max_worlds=8, body_capacity=160, shape_capacity=160, schedule="static", device=DEV,
+                          **KW).run(b, realtime=False)
+    print("continuous", sa, "static", sb, flush=True)
+    assert sa["completed"] == sb["completed"] == 24
+    assert sb["lockstep_iterations"] >= max(r.accepted for r in b)
+    for x, y in zip(a, b):
+        assert x.accepted == y.accepted and x.frames == y.frames
+        assert np.array_equal(x.body_q, y.body_q) and np.array_equal(x.body_qd, y.body_qd)
+    for r in b:
+        assert (r.frames == 3) if r.horizon_frames else (r.accepted == r.horizon_accepted)
