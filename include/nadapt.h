@@ -61,6 +61,13 @@ typedef struct na_model_desc {
      hold worlds in arbitrary arena ranges with gaps (serving). */
   const int32_t* world_body_end;
   const int32_t* world_shape_end;
+  /* Optional candidate pairs for the explicit broad phase (broad_phase 2; model.shape_contact_pairs
+     of the reference): canonical (min, max) global shape indices passing the world, group and
+     exclusion rules, grouped by world; world w's pairs are [world_pair_start[w],
+     world_pair_start[w+1]).  explicit_pair_count 0 = none. */
+  int32_t explicit_pair_count;
+  const int32_t* explicit_pairs;
+  const int32_t* world_pair_start;
 } na_model_desc;
 
 /* Solver settings (constructor kwargs of the reference solvers). */
@@ -68,7 +75,7 @@ typedef struct na_config {
   int32_t mode;            /* 0 "fixed", 1 "adaptive" */
   int32_t controller;      /* 0 SAP (_adapt_dt), 1 MuJoCo (_calc_adjusted_step) */
   double tol, dt_inner_init, dt_inner_min, dt_inner_max, divergence_threshold;
-  int32_t max_substeps, max_iterations, max_rigid_contact, broad_phase; /* broad_phase: 0 sap, 1 nxn */
+  int32_t max_substeps, max_iterations, max_rigid_contact, broad_phase; /* 0 sap, 1 nxn, 2 explicit */
   float optimality_rel_tol, optimality_abs_tol, contact_tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
   int32_t max_cg;
   float cg_forcing;        /* > 0: inexact-Newton PCG forcing bound (eta_k <= cg_forcing); 0: fixed cg_rel_tol */
@@ -153,6 +160,33 @@ int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_
              const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
              const double* world_tol, float* body_q, float* body_qd, const float* body_f, double dt_outer,
              double budget_ms, int32_t max_iterations, int32_t* unfinished, void* stream);
+
+/* MuJoCo coordinate adapters for the FREE joint every body carries (parent frame = world).
+   Newton joint_q [B,7] = origin position + quaternion xyzw, joint_qd [B,6] = COM linear velocity
+   + angular velocity (world); MuJoCo qpos [B,7] = origin position + quaternion wxyz, qvel [B,6] =
+   origin linear velocity (world) + angular velocity (body frame).  Replace
+   convert_warp_coords_to_mj_kernel (solvers/mujoco/kernels.py:705-789) and
+   convert_mj_coords_to_warp_kernel (kernels.py:578-666) for FREE joints.  Rows of one world are
+   contiguous, so equal-size worlds give MuJoCo's [nworld, nq] layout. */
+int na_coords_to_mujoco(const na_model* m, const float* joint_q, const float* joint_qd, float* qpos, float* qvel,
+                        void* stream);
+int na_coords_from_mujoco(const na_model* m, const float* qpos, const float* qvel, float* joint_q, float* joint_qd,
+                          void* stream);
+
+/* Frame-to-frame contact matching (replaces ContactMatcher.match, geometry/contact_match.py:566-,
+   called from collide.py:853-861 when contact_matching is on).  `cur` is the buffer na_collide just
+   filled (cur_offsets: its per-world start offsets, the W words after the staging area of the
+   na_collide scratch); prev_* describe the previous frame's buffer (shape pairs, world-space
+   midpoints from the previous call's cur_mid, normals, per-world offsets and unclamped counts).
+   Writes match_index for every current contact (>= 0 previous index, -1 new pair, -2 broken),
+   the current world midpoints into cur_mid [capacity,3] and prev_matched[k] = 1 for every
+   previous contact that was matched (prev_matched must be zero on entry). */
+int na_match_contacts(const na_model* m, const float* body_q, const na_contacts_desc* cur, int32_t per_world,
+                      const int32_t* cur_offsets, const int32_t* prev_shape0, const int32_t* prev_shape1,
+                      const float* prev_mid, const float* prev_normal, const int32_t* prev_offsets,
+                      const int32_t* prev_counts, int32_t prev_capacity, float pos_threshold,
+                      float normal_dot_threshold, int32_t* match_index, float* cur_mid, int32_t* prev_matched,
+                      void* stream);
 
 /* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
 int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);

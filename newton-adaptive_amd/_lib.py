@@ -36,6 +36,7 @@ class NaModelDesc(C.Structure):
         ("filter_pairs", vp),
         ("gravity", C.c_float * 3),
         ("world_body_end", vp), ("world_shape_end", vp),
+        ("explicit_pair_count", i32), ("explicit_pairs", vp), ("world_pair_start", vp),
     ]
 
 
@@ -77,6 +78,10 @@ SIGNATURES = {
     "na_step": (C.c_int, [vp, vp, vp, vp, C.c_double, vp]),
     "na_step_plain": (C.c_int, [vp, vp, vp, vp, C.c_double, C.POINTER(i32), vp]),
     "na_reset": (C.c_int, [vp, vp, vp]),
+    "na_coords_to_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "na_match_contacts": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, vp, vp,
+                                    vp, vp]),
+    "na_coords_from_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),

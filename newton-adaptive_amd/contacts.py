@@ -17,13 +17,19 @@ import torch
 from . import _lib
 from .native import model_handle
 
-BROAD_PHASES = {"sap": 0, "nxn": 1}
+BROAD_PHASES = {"sap": 0, "nxn": 1, "explicit": 2}
+MATCH_NOT_FOUND = -1  # contact_match.py:113
+MATCH_BROKEN = -2  # contact_match.py:116
 
 
 class Contacts:
-    """Rigid contact SoA buffer (``contacts.py:91``)."""
+    """Rigid contact SoA buffer (``contacts.py:91``).  With ``contact_matching`` it also holds
+    ``rigid_contact_match_index`` and, with ``contact_report``, the new / broken index lists
+    (``contacts.py:278-310``)."""
 
-    def __init__(self, rigid_contact_max: int, device):
+    def __init__(self, rigid_contact_max: int, device, contact_matching: bool = False, contact_report: bool = False):
+        if contact_report and not contact_matching:
+            raise ValueError("contact_report=True requires contact_matching=True")
         n = int(rigid_contact_max)
         d = torch.device(device)
         self.rigid_contact_max = n
@@ -40,18 +46,36 @@ class Contacts:
         self.rigid_contact_margin1 = torch.zeros(n, dtype=torch.float32, device=d)
         self.rigid_contact_tids = torch.full((n,), -1, dtype=torch.int32, device=d)
         self.rigid_contact_world_count = None
+        self.contact_matching = bool(contact_matching)
+        self.contact_report = bool(contact_report)
+        self.rigid_contact_match_index = torch.full((n,), -1, dtype=torch.int32, device=d) if contact_matching \
+            else None
+        if contact_report:
+            self.rigid_contact_new_indices = torch.zeros(n, dtype=torch.int32, device=d)
+            self.rigid_contact_new_count = torch.zeros(1, dtype=torch.int32, device=d)
+            self.rigid_contact_broken_indices = torch.zeros(n, dtype=torch.int32, device=d)
+            self.rigid_contact_broken_count = torch.zeros(1, dtype=torch.int32, device=d)
+        else:
+            self.rigid_contact_new_indices = self.rigid_contact_new_count = None
+            self.rigid_contact_broken_indices = self.rigid_contact_broken_count = None
 
     def clear(self) -> None:
         self.rigid_contact_count.zero_()
         self.rigid_contact_shape0.fill_(-1)
         self.rigid_contact_shape1.fill_(-1)
+        if self.rigid_contact_match_index is not None:
+            self.rigid_contact_match_index.fill_(-1)
 
 
 class CollisionPipeline:
     """AABB -> broad phase -> primitive narrow phase -> ``Contacts`` (``collide.py:464``)."""
 
     def __init__(self, model, *, broad_phase: str = "sap", rigid_contact_max: int | None = None,
-                 rigid_contact_max_per_world: int = 128, **_kw):
+                 rigid_contact_max_per_world: int = 128, contact_matching: bool = False, contact_report: bool = False,
+                 contact_matching_pos_threshold: float = 0.0005, contact_matching_normal_dot_threshold: float = 0.995,
+                 **_kw):
+        if contact_report and not contact_matching:
+            raise ValueError("contact_report=True requires contact_matching=True")
         if broad_phase not in BROAD_PHASES:
             raise ValueError(f"broad_phase must be one of {tuple(BROAD_PHASES)}, got {broad_phase!r}")
         if not 1 <= int(rigid_contact_max_per_world) <= 128:
@@ -61,14 +85,38 @@ class CollisionPipeline:
         self.per_world = int(rigid_contact_max_per_world)
         self.rigid_contact_max = int(rigid_contact_max) if rigid_contact_max is not None \
             else self.per_world * model.world_count
-        self._handle = model_handle(model)
+        self._handle = model_handle(model, with_pairs=broad_phase == "explicit")
         L = _lib.lib()
         words = int(L.na_collide_scratch_words(self._handle.handle, self.per_world))
         self._scratch = torch.zeros(words, dtype=torch.int32, device=model.device)
         self._world_counts = torch.zeros(model.world_count, dtype=torch.int32, device=model.device)
+        # frame-to-frame matching state (ContactMatcher, collide.py:853-861)
+        self.contact_matching = bool(contact_matching)
+        self.contact_report = bool(contact_report)
+        self.pos_threshold = float(contact_matching_pos_threshold)
+        self.normal_dot_threshold = float(contact_matching_normal_dot_threshold)
+        if self.contact_matching:
+            W, n, d = model.world_count, self.rigid_contact_max, model.device
+            i32 = dict(dtype=torch.int32, device=d)
+            self._prev_s0 = torch.full((n,), -1, **i32)
+            self._prev_s1 = torch.full((n,), -1, **i32)
+            self._prev_mid = torch.zeros((n, 3), dtype=torch.float32, device=d)
+            self._prev_n = torch.zeros((n, 3), dtype=torch.float32, device=d)
+            self._prev_off = torch.zeros(W, **i32)
+            self._prev_cnt = torch.zeros(W, **i32)
+            self._prev_total = torch.zeros(1, **i32)
+            self._prev_matched = torch.zeros(n, **i32)
+            self._cur_mid = torch.zeros((n, 3), dtype=torch.float32, device=d)
+
+    def reset_matching(self) -> None:
+        """Forget the previous frame (the next collide reports every contact as new)."""
+        if self.contact_matching:
+            self._prev_cnt.zero_()
+            self._prev_total.zero_()
 
     def contacts(self) -> Contacts:
-        return Contacts(self.rigid_contact_max, self.model.device)
+        return Contacts(self.rigid_contact_max, self.model.device, contact_matching=self.contact_matching,
+                        contact_report=self.contact_report)
 
     def collide(self, state, contacts: Contacts, stream=None) -> Contacts:
         if contacts.rigid_contact_max < 1:
@@ -92,7 +140,41 @@ class CollisionPipeline:
                                          BROAD_PHASES[self.broad_phase], _lib.stream_handle(stream)),
                    "na_collide")
         contacts.rigid_contact_world_count = self._world_counts
+        if self.contact_matching:
+            self._match(body_q, d, contacts, stream)
         return contacts
+
+    def _match(self, body_q, d, contacts, stream):
+        if contacts.rigid_contact_match_index is None:
+            raise ValueError("contacts were allocated without contact_matching")
+        W = self.model.world_count
+        n = self.rigid_contact_max
+        cur_off = self._scratch[W * self.per_world * 20: W * self.per_world * 20 + W]
+        contacts.rigid_contact_match_index.fill_(MATCH_NOT_FOUND)
+        self._prev_matched.zero_()
+        _lib.check(_lib.lib().na_match_contacts(
+            self._handle.handle, body_q.data_ptr(), C.byref(d), self.per_world, cur_off.data_ptr(),
+            self._prev_s0.data_ptr(), self._prev_s1.data_ptr(), self._prev_mid.data_ptr(), self._prev_n.data_ptr(),
+            self._prev_off.data_ptr(), self._prev_cnt.data_ptr(), n, self.pos_threshold, self.normal_dot_threshold,
+            contacts.rigid_contact_match_index.data_ptr(), self._cur_mid.data_ptr(), self._prev_matched.data_ptr(),
+            _lib.stream_handle(stream)), "na_match_contacts")
+        if self.contact_report:
+            total = int(min(int(contacts.rigid_contact_count.item()), n))
+            prev_total = int(self._prev_total.item())
+            new = torch.nonzero(contacts.rigid_contact_match_index[:total] < 0).flatten().to(torch.int32)
+            broken = torch.nonzero(self._prev_matched[:prev_total] == 0).flatten().to(torch.int32)
+            contacts.rigid_contact_new_indices[:len(new)] = new
+            contacts.rigid_contact_new_count.fill_(len(new))
+            contacts.rigid_contact_broken_indices[:len(broken)] = broken
+            contacts.rigid_contact_broken_count.fill_(len(broken))
+        # this frame becomes the previous one
+        self._prev_s0.copy_(contacts.rigid_contact_shape0)
+        self._prev_s1.copy_(contacts.rigid_contact_shape1)
+        self._prev_mid.copy_(self._cur_mid)
+        self._prev_n.copy_(contacts.rigid_contact_normal)
+        self._prev_off.copy_(cur_off)
+        self._prev_cnt.copy_(self._world_counts)
+        self._prev_total.copy_(torch.clamp(contacts.rigid_contact_count, max=n))
 
     def debug_pairs(self, state, max_pairs: int = 1024, stream=None):
         """AABBs and candidate pairs (for pair-set parity checks): (lo [S,3], hi [S,3],

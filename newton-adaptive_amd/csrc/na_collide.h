@@ -39,6 +39,9 @@ struct NaModel {
   const float* shape_mu;
   const float* shape_radius;
   const int32_t* filter_pairs;  // [F,2] sorted canonical
+  int P;                         // explicit candidate pairs (0: none given)
+  const int32_t* pair_list;      // [P,2] canonical (min, max) global shape indices, grouped by world
+  const int32_t* world_pair_start;  // [W+1] range of each world's pairs in pair_list
   float gx, gy, gz;
 };
 
@@ -505,6 +508,40 @@ __device__ int stage_broad_nxn(const NaModel& M, int n, SegLDS& S) {
   return np;
 }
 
+// Segment position of global shape index s in world w's segment (-1 if it is not in it).
+__device__ __forceinline__ int seg_position(const NaModel& M, int w, int nl, int s) {
+  int s0 = M.world_shape_start[w];
+  if (s >= s0 && s < s0 + nl) return s - s0;
+  for (int k = 0; k < M.G; ++k)
+    if (M.global_shapes[k] == s) return nl + k;
+  return -1;
+}
+
+// Stage 2c: explicit broad phase (broad_phase_nxn.py:394-470): the world's precomputed pairs
+// (already filtered by world, group and exclusion rules) tested for AABB overlap, in list order.
+__device__ int stage_broad_explicit(const NaModel& M, int w, int nl, SegLDS& S) {
+  const int p0 = M.world_pair_start[w], p1 = M.world_pair_start[w + 1];
+  int np = 0;
+  for (int base = p0; base < p1; base += NA_WAVE) {
+    int k = base + lane_id();
+    bool pred = false;
+    int i = 0, j = 0;
+    if (k < p1) {
+      i = seg_position(M, w, nl, M.pair_list[2 * k]);
+      j = seg_position(M, w, nl, M.pair_list[2 * k + 1]);
+      if (i > j) { int t = i; i = j; j = t; }
+      pred = i >= 0 && S.lo[0][i] <= S.hi[0][j] && S.hi[0][i] >= S.lo[0][j] && S.lo[1][i] <= S.hi[1][j] &&
+             S.hi[1][i] >= S.lo[1][j] && S.lo[2][i] <= S.hi[2][j] && S.hi[2][i] >= S.lo[2][j];
+    }
+    append_pair(pred, i, j, S, np);
+  }
+  wave_sync();
+  return np;
+}
+
+// Candidate pairs of world w's segment into S.pairs (0 sweep and prune, 1 all pairs, 2 explicit).
+__device__ __forceinline__ int stage_broad(const NaModel& M, int w, int nseg, int nl, int broad_phase, SegLDS& S);
+
 // Stage 2b: sweep-and-prune in LDS (broad_phase_sap.py:144-394): project on the fixed axis,
 // bitonic sort of the lower bounds, binary-search ranges, scan, sweep.
 __device__ int stage_broad_sap(const NaModel& M, int n, SegLDS& S) {
@@ -732,4 +769,9 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
   }
   wave_sync();
   return nc;
+}
+
+__device__ __forceinline__ int stage_broad(const NaModel& M, int w, int nseg, int nl, int broad_phase, SegLDS& S) {
+  if (broad_phase == 2) return stage_broad_explicit(M, w, nl, S);
+  return broad_phase == 1 ? stage_broad_nxn(M, nseg, S) : stage_broad_sap(M, nseg, S);
 }

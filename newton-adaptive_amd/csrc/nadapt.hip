@@ -244,6 +244,46 @@ __global__ void k_inf_norm_f32(int n, int coords, const float* qf, const float* 
   err[w] = m;
 }
 
+// ---------------------------------------------------------------- MuJoCo coordinate adapters
+// Every body carries one FREE joint whose parent frame is the world, so joint and body rows
+// coincide: Newton joint_q = (origin position, quaternion xyzw), joint_qd = (COM linear velocity,
+// angular velocity), both world frame; MuJoCo qpos = (origin position, quaternion wxyz), qvel =
+// (origin linear velocity in world, angular velocity in the body frame)
+// (kernels.py:705-789 convert_warp_coords_to_mj_kernel, :578-666 the inverse).
+__global__ void k_coords_to_mujoco(int B, const float* com, const float* jq, const float* jqd, float* qpos,
+                                   float* qvel) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float* q = jq + 7 * b;
+  const float* u = jqd + 6 * b;
+  m3 R = qmat(q[3], q[4], q[5], q[6]);
+  f3 c = mv(R, mk(com[3 * b], com[3 * b + 1], com[3 * b + 2]));
+  f3 w = mk(u[3], u[4], u[5]);
+  f3 vo = mk(u[0], u[1], u[2]) - cross3(w, c);
+  f3 wb = mtv(R, w);
+  float* p = qpos + 7 * b;
+  p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+  p[3] = q[6]; p[4] = q[3]; p[5] = q[4]; p[6] = q[5];
+  float* v = qvel + 6 * b;
+  v[0] = vo.x; v[1] = vo.y; v[2] = vo.z; v[3] = wb.x; v[4] = wb.y; v[5] = wb.z;
+}
+__global__ void k_coords_from_mujoco(int B, const float* com, const float* qpos, const float* qvel, float* jq,
+                                     float* jqd) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float* p = qpos + 7 * b;
+  const float* v = qvel + 6 * b;
+  float* q = jq + 7 * b;
+  q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
+  q[3] = p[4]; q[4] = p[5]; q[5] = p[6]; q[6] = p[3];
+  m3 R = qmat(q[3], q[4], q[5], q[6]);
+  f3 w = mv(R, mk(v[3], v[4], v[5]));
+  f3 c = mv(R, mk(com[3 * b], com[3 * b + 1], com[3 * b + 2]));
+  f3 vc = mk(v[0], v[1], v[2]) + cross3(w, c);
+  float* u = jqd + 6 * b;
+  u[0] = vc.x; u[1] = vc.y; u[2] = vc.z; u[3] = w.x; u[4] = w.y; u[5] = w.z;
+}
+
 // ---------------------------------------------------------------- world-level device pipeline
 struct WorldCtx {
   int w, b0, nb, nl, nseg;
@@ -269,7 +309,7 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
   }
   wave_sync();
   stage_aabb(M, wc.w, wc.nseg, wc.nl, wc.b0, &S.body.q[0][0], NA_MAX_BODIES, S.seg);
-  int np = cfg.broad_phase == 1 ? stage_broad_nxn(M, wc.nseg, S.seg) : stage_broad_sap(M, wc.nseg, S.seg);
+  int np = stage_broad(M, wc.w, wc.nseg, wc.nl, cfg.broad_phase, S.seg);
   int cap = min(cfg.max_contacts, NA_MAX_CONTACTS);
   int nc = stage_narrow(M, wc.b0, np, cap, S.seg, S.con);
   if (np > NA_MAX_PAIRS || nc > cap) overflow = 1;
@@ -815,6 +855,92 @@ __global__ void k_compact_contacts(int W, int per_world, const int32_t* staging,
   }
 }
 
+// Frame-to-frame contact matching (geometry/contact_match.py): one wave per world.  The
+// previous frame's contacts of the world are staged in LDS; every new contact looks for previous
+// contacts of the same shape pair and takes the closest one whose world midpoint
+// 1/2 (world(point0) + world(point1)) moved at most pos_thr and whose normal turned by at most
+// acos(dot_thr).  A previous contact goes to one new contact only: claims are packed
+// (distance^2 bits << 32 | new contact's index in its world) and resolved with a 64-bit LDS
+// atomic min, so the smallest distance wins and ties go to the lower index, independently of
+// scheduling.  match: >= 0 index of the previous contact, -1 no previous contact of the pair,
+// -2 pair present but no contact within the thresholds or the claim was lost.
+#define NA_MATCH_NOT_FOUND (-1)
+#define NA_MATCH_BROKEN (-2)
+__device__ __forceinline__ f3 contact_world_point(const NaModel& M, const float* body_q, int s, const float* p) {
+  int b = M.shape_body[s];
+  f3 x = mk(p[0], p[1], p[2]);
+  if (b < 0) return x;
+  const float* q = body_q + 7 * b;
+  return mk(q[0], q[1], q[2]) + mv(qmat(q[3], q[4], q[5], q[6]), x);
+}
+__global__ __launch_bounds__(64) void k_match_contacts(NaModel M, const float* body_q, na_contacts_desc cur,
+                                                       int per_world, const int32_t* cur_off,
+                                                       const int32_t* prev_s0, const int32_t* prev_s1,
+                                                       const float* prev_mid, const float* prev_n,
+                                                       const int32_t* prev_off, const int32_t* prev_cnt,
+                                                       int prev_cap, float pos_thr2, float dot_thr, int32_t* match,
+                                                       float* cur_mid, int32_t* prev_matched) {
+  __shared__ int ps0[NA_MAX_CONTACTS], ps1[NA_MAX_CONTACTS];
+  __shared__ float pm[3][NA_MAX_CONTACTS], pn[3][NA_MAX_CONTACTS];
+  __shared__ unsigned long long claim[NA_MAX_CONTACTS];
+  const int w = blockIdx.x, l = lane_id();
+  const int po = prev_off[w];
+  int pc = min(prev_cnt[w], per_world);
+  pc = max(0, min(pc, prev_cap - po));
+  const int co = cur_off[w];
+  int cc = min(cur.world_count_out[w], per_world);
+  cc = max(0, min(cc, cur.capacity - co));
+  for (int k = l; k < pc; k += NA_WAVE) {
+    ps0[k] = prev_s0[po + k];
+    ps1[k] = prev_s1[po + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pm[d][k] = prev_mid[3 * (po + k) + d];
+      pn[d][k] = prev_n[3 * (po + k) + d];
+    }
+    claim[k] = ~0ull;
+  }
+  __syncthreads();
+  int best[2] = {-1, -1}, res[2] = {NA_MATCH_NOT_FOUND, NA_MATCH_NOT_FOUND};
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int c = l + g * NA_WAVE;
+    if (c >= cc) continue;
+    const int i = co + c;
+    const int s0 = cur.shape0[i], s1 = cur.shape1[i];
+    f3 a = contact_world_point(M, body_q, s0, cur.point0 + 3 * i);
+    f3 b = contact_world_point(M, body_q, s1, cur.point1 + 3 * i);
+    f3 mid = (a + b) * 0.5f;
+    f3 n = mk(cur.normal[3 * i], cur.normal[3 * i + 1], cur.normal[3 * i + 2]);
+    cur_mid[3 * i] = mid.x; cur_mid[3 * i + 1] = mid.y; cur_mid[3 * i + 2] = mid.z;
+    bool pair = false;
+    float bd = 0.f;
+    for (int k = 0; k < pc; ++k) {
+      if (ps0[k] != s0 || ps1[k] != s1) continue;
+      pair = true;
+      f3 dm = mid - mk(pm[0][k], pm[1][k], pm[2][k]);
+      float d2 = dot3(dm, dm);
+      float dn = dot3(n, mk(pn[0][k], pn[1][k], pn[2][k]));
+      if (d2 <= pos_thr2 && dn >= dot_thr && (best[g] < 0 || d2 < bd)) { best[g] = k; bd = d2; }
+    }
+    res[g] = pair ? NA_MATCH_BROKEN : NA_MATCH_NOT_FOUND;
+    if (best[g] >= 0)
+      atomicMin(&claim[best[g]], ((unsigned long long)__float_as_uint(bd) << 32) | (unsigned long long)c);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const int c = l + g * NA_WAVE;
+    if (c >= cc) continue;
+    int r = res[g];
+    if (best[g] >= 0 && (int)(claim[best[g]] & 0xffffffffull) == c) {
+      r = po + best[g];
+      prev_matched[po + best[g]] = 1;
+    }
+    match[co + c] = r;
+  }
+}
+
 __global__ __launch_bounds__(64) void k_collide_debug(NaModel M, int broad_phase, const float* body_q, float* lo,
                                                       float* hi, int32_t* pairs, int32_t* pair_count, int max_pairs) {
   __shared__ WorldLDS S;
@@ -836,7 +962,7 @@ __global__ __launch_bounds__(64) void k_collide_debug(NaModel M, int broad_phase
       hi[3 * s + k] = S.seg.hi[k][i];
     }
   }
-  int np = broad_phase == 1 ? stage_broad_nxn(M, wc.nseg, S.seg) : stage_broad_sap(M, wc.nseg, S.seg);
+  int np = stage_broad(M, wc.w, wc.nseg, wc.nl, broad_phase, S.seg);
   if (l == 0) pair_count[w] = np;
   int m = min(np, min(max_pairs, NA_MAX_PAIRS));
   for (int k = l; k < m; k += NA_WAVE) {
@@ -956,6 +1082,11 @@ extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
   M.shape_margin = d->shape_margin; M.shape_gap = d->shape_gap; M.shape_ke = d->shape_material_ke;
   M.shape_mu = d->shape_material_mu; M.shape_radius = d->shape_collision_radius; M.filter_pairs = d->filter_pairs;
   M.gx = d->gravity[0]; M.gy = d->gravity[1]; M.gz = d->gravity[2];
+  M.P = d->explicit_pair_count; M.pair_list = d->explicit_pairs; M.world_pair_start = d->world_pair_start;
+  if (M.P < 0 || (M.P > 0 && (!M.pair_list || !M.world_pair_start))) {
+    delete m;
+    return fail("na_model_create: explicit pairs need explicit_pairs [P,2] and world_pair_start [W+1]");
+  }
   // validate per-world sizes against the wave-per-world limits
   std::string err;
   // the end arrays are either separate [W] arrays or the start arrays shifted by one
@@ -988,6 +1119,11 @@ extern "C" int na_model_destroy(na_model* m) {
   return 0;
 }
 
+// broad phase code valid for this model (explicit mode needs the model's pair list)
+static bool broad_phase_ok(const na_model* m, int bp) {
+  return bp == 0 || bp == 1 || (bp == 2 && m->M.P > 0);
+}
+
 static NaConfig to_cfg(const na_config* c) {
   NaConfig k;
   k.mode = c->mode; k.controller = c->controller; k.tol = c->tol; k.dt_init = c->dt_inner_init;
@@ -1012,6 +1148,8 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   if (c->max_substeps < 1) return fail("na_solver_create: max_substeps must be >= 1");
   if (c->max_rigid_contact < 1 || c->max_rigid_contact > NA_MAX_CONTACTS)
     return fail("na_solver_create: max_rigid_contact must be in [1, 128] per world");
+  if (!broad_phase_ok(m, c->broad_phase))
+    return fail("na_solver_create: broad_phase must be 0 (sap), 1 (nxn) or 2 (explicit, needs the model's pairs)");
   na_solver* s = new na_solver();
   s->model = m;
   s->cfg = to_cfg(c);
@@ -1182,6 +1320,25 @@ extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const 
   return status;
 }
 
+extern "C" int na_coords_to_mujoco(const na_model* m, const float* joint_q, const float* joint_qd, float* qpos,
+                                   float* qvel, void* stream) {
+  if (!m || !joint_q || !joint_qd || !qpos || !qvel) return fail("na_coords_to_mujoco: null argument");
+  int B = m->M.B;
+  if (B == 0) return 0;
+  k_coords_to_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, m->M.body_com, joint_q, joint_qd, qpos, qvel);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_coords_from_mujoco(const na_model* m, const float* qpos, const float* qvel, float* joint_q,
+                                     float* joint_qd, void* stream) {
+  if (!m || !joint_q || !joint_qd || !qpos || !qvel) return fail("na_coords_from_mujoco: null argument");
+  int B = m->M.B;
+  if (B == 0) return 0;
+  k_coords_from_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, m->M.body_com, qpos, qvel, joint_q, joint_qd);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int na_reset(na_solver* s, const uint8_t* mask, void* stream) {
   if (!s) return fail("na_reset: null solver");
   int W = s->model->M.W;
@@ -1199,6 +1356,7 @@ extern "C" int na_collide(const na_model* m, const float* body_q, const na_conta
                           int32_t broad_phase, void* stream) {
   if (!m || !body_q || !out) return fail("na_collide: null argument");
   if (per_world < 1 || per_world > NA_MAX_CONTACTS) return fail("na_collide: per-world capacity must be in [1, 128]");
+  if (!broad_phase_ok(m, broad_phase)) return fail("na_collide: invalid broad_phase for this model");
   const NaModel& M = m->M;
   hipStream_t st = S_(stream);
   NaConfig cfg{};
@@ -1213,9 +1371,30 @@ extern "C" int na_collide(const na_model* m, const float* body_q, const na_conta
   return 0;
 }
 
+extern "C" int na_match_contacts(const na_model* m, const float* body_q, const na_contacts_desc* cur,
+                                 int32_t per_world, const int32_t* cur_offsets, const int32_t* prev_shape0,
+                                 const int32_t* prev_shape1, const float* prev_mid, const float* prev_normal,
+                                 const int32_t* prev_offsets, const int32_t* prev_counts, int32_t prev_capacity,
+                                 float pos_threshold, float normal_dot_threshold, int32_t* match_index,
+                                 float* cur_mid, int32_t* prev_matched, void* stream) {
+  if (!m || !body_q || !cur || !cur_offsets || !prev_shape0 || !prev_shape1 || !prev_mid || !prev_normal ||
+      !prev_offsets || !prev_counts || !match_index || !cur_mid || !prev_matched)
+    return fail("na_match_contacts: null argument");
+  if (per_world < 1 || per_world > NA_MAX_CONTACTS) return fail("na_match_contacts: per-world capacity in [1, 128]");
+  if (!(pos_threshold >= 0.f) || !(normal_dot_threshold <= 1.f)) return fail("na_match_contacts: bad thresholds");
+  const NaModel& M = m->M;
+  k_match_contacts<<<M.W, 64, 0, S_(stream)>>>(M, body_q, *cur, per_world, cur_offsets, prev_shape0, prev_shape1,
+                                               prev_mid, prev_normal, prev_offsets, prev_counts, prev_capacity,
+                                               pos_threshold * pos_threshold, normal_dot_threshold, match_index,
+                                               cur_mid, prev_matched);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 extern "C" int na_collide_debug(const na_model* m, const float* body_q, float* lo, float* hi, int32_t* pairs,
                                 int32_t* pair_count, int32_t max_pairs, int32_t broad_phase, void* stream) {
   if (!m || !body_q || !lo || !hi || !pairs || !pair_count) return fail("na_collide_debug: null argument");
+  if (!broad_phase_ok(m, broad_phase)) return fail("na_collide_debug: invalid broad_phase for this model");
   const NaModel& M = m->M;
   k_collide_debug<<<M.W, 64, 0, S_(stream)>>>(M, broad_phase, body_q, lo, hi, pairs, pair_count, max_pairs);
   HIPCHK(hipGetLastError());

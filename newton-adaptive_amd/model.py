@@ -135,6 +135,37 @@ class Model:
         self.world_shape_start = _t(host["world_shape_start"], torch.int32, d)
         self.global_shapes = _t(host["global_shapes"], torch.int32, d)
 
+    # --- explicit broad-phase pair list (model.shape_contact_pairs of the reference) ---
+    def contact_pairs(self):
+        """Candidate shape pairs for the explicit broad phase: canonical ``(min, max)`` global
+        shape indices of every pair that the world, collision-group, same-body and filter rules
+        let collide (``broad_phase_common.py:18-180``), grouped by world, plus ``[W+1]`` world
+        ranges.  Global-global pairs are never included.  Computed on first use; returns host
+        arrays ``(pairs [P, 2], world_pair_start [W+1])``."""
+        if getattr(self, "_contact_pairs", None) is None:
+            h = self._host
+            ws, g = h["world_shape_start"], h["global_shapes"].astype(np.int64)
+            grp, body, sw = h["shape_collision_group"], h["shape_body"], h["shape_world"]
+            S = max(self.shape_count, 1)
+            fp = h["shape_collision_filter_pairs"].reshape(-1, 2).astype(np.int64)
+            fkey = fp[:, 0] * S + fp[:, 1]
+            out, starts = [], [0]
+            for w in range(self.world_count):
+                seg = np.concatenate([np.arange(ws[w], ws[w + 1], dtype=np.int64), g])
+                i, j = np.triu_indices(len(seg), 1)
+                a, b = np.minimum(seg[i], seg[j]), np.maximum(seg[i], seg[j])
+                ga, gb = grp[a], grp[b]
+                ok = ~((sw[a] == -1) & (sw[b] == -1))
+                ok &= (ga != 0) & (gb != 0) & np.where(ga > 0, (ga == gb) | (gb < 0), ga != gb)
+                ok &= ~((body[a] >= 0) & (body[a] == body[b]))
+                if len(fkey):
+                    ok &= ~np.isin(a * S + b, fkey)
+                out.append(np.stack([a[ok], b[ok]], axis=1))
+                starts.append(starts[-1] + int(ok.sum()))
+            pairs = np.concatenate(out).astype(np.int32) if out else np.zeros((0, 2), np.int32)
+            self._contact_pairs = (pairs.reshape(-1, 2), np.asarray(starts, dtype=np.int32))
+        return self._contact_pairs
+
     # --- host mirrors (numpy) used by the oracle-side tests and the autotuner ---
     def host(self, name: str) -> np.ndarray:
         return self._host[name]

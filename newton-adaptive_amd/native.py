@@ -12,7 +12,7 @@ from . import _lib
 class ModelHandle:
     """Owns the ``na_model`` of a :class:`Model` (the device arrays stay owned by the Model)."""
 
-    def __init__(self, model):
+    def __init__(self, model, with_pairs: bool = False):
         if model.device.type != "cuda":
             raise RuntimeError("the nadapt library runs on a HIP device; build the model with device='cuda:0'")
         L = _lib.lib()
@@ -58,6 +58,16 @@ class ModelHandle:
         wbe, wse = getattr(model, "world_body_end", None), getattr(model, "world_shape_end", None)
         d.world_body_end = p(wbe) if wbe is not None else None
         d.world_shape_end = p(wse) if wse is not None else None
+        # explicit broad-phase pairs, only when a caller asks for that broad phase
+        self.with_pairs = False
+        d.explicit_pair_count = 0
+        if with_pairs:
+            pairs, starts = model.contact_pairs()
+            if len(pairs):
+                d.explicit_pair_count = int(len(pairs))
+                d.explicit_pairs = p(torch.as_tensor(pairs, device=model.device))
+                d.world_pair_start = p(torch.as_tensor(starts, device=model.device))
+                self.with_pairs = True
         self.desc = d
         h = C.c_void_p()
         _lib.check(L.na_model_create(C.byref(d), C.byref(h)), "na_model_create")
@@ -72,9 +82,10 @@ class ModelHandle:
             pass
 
 
-def model_handle(model) -> ModelHandle:
+def model_handle(model, with_pairs: bool = False) -> ModelHandle:
+    """The model's cached handle; rebuilt once with the explicit pair list when asked for it."""
     h = getattr(model, "_na_handle", None)
-    if h is None:
-        h = ModelHandle(model)
+    if h is None or (with_pairs and not h.with_pairs):
+        h = ModelHandle(model, with_pairs=with_pairs or (h is not None and h.with_pairs))
         model._na_handle = h
     return h

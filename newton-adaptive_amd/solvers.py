@@ -139,7 +139,7 @@ class _SolverBase:
         t.solve_fail, t.overflow = self._solve_fail.data_ptr(), self._overflow.data_ptr()
         t.cum_accepted, t.work, t.body_q64 = (self._cum_accepted.data_ptr(), self._work.data_ptr(),
                                               self._q64.data_ptr())
-        self._handle = model_handle(model)
+        self._handle = model_handle(model, with_pairs=broad_phase == "explicit")
         h = C.c_void_p()
         _lib.check(_lib.lib().na_solver_create(self._handle.handle, C.byref(cfg), C.byref(t), C.byref(h)),
                    "na_solver_create")

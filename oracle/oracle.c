@@ -1438,6 +1438,45 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
   return status;
 }
 
+/* ---------------------------------------------------------------- MuJoCo coordinate adapters */
+/* FREE joint with the world as parent frame (kernels.py:705-789 Newton -> MuJoCo, :578-666 back):
+   qpos = (origin, quaternion wxyz); qvel = (origin velocity in world, angular velocity in the body
+   frame); Newton joint_q = (origin, quaternion xyzw), joint_qd = (COM velocity, angular velocity),
+   both world. */
+void or_coords_to_mujoco(int B, const double* com, const double* jq, const double* jqd, double* qpos,
+                         double* qvel) {
+  for (int b = 0; b < B; ++b) {
+    const double* q = jq + 7 * b;
+    const double* u = jqd + 6 * b;
+    qt r = {q[3], q[4], q[5], q[6]};
+    m33 R = qmat(r);
+    v3 c = mv(&R, V(com[3 * b], com[3 * b + 1], com[3 * b + 2]));
+    v3 w = V(u[3], u[4], u[5]);
+    v3 vo = sub(V(u[0], u[1], u[2]), cross(w, c));
+    v3 wb = mtv(&R, w);
+    double* p = qpos + 7 * b;
+    p[0] = q[0]; p[1] = q[1]; p[2] = q[2]; p[3] = q[6]; p[4] = q[3]; p[5] = q[4]; p[6] = q[5];
+    double* v = qvel + 6 * b;
+    v[0] = vo.x; v[1] = vo.y; v[2] = vo.z; v[3] = wb.x; v[4] = wb.y; v[5] = wb.z;
+  }
+}
+void or_coords_from_mujoco(int B, const double* com, const double* qpos, const double* qvel, double* jq,
+                           double* jqd) {
+  for (int b = 0; b < B; ++b) {
+    const double* p = qpos + 7 * b;
+    const double* v = qvel + 6 * b;
+    double* q = jq + 7 * b;
+    q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = p[4]; q[4] = p[5]; q[5] = p[6]; q[6] = p[3];
+    qt r = {q[3], q[4], q[5], q[6]};
+    m33 R = qmat(r);
+    v3 w = mv(&R, V(v[3], v[4], v[5]));
+    v3 c = mv(&R, V(com[3 * b], com[3 * b + 1], com[3 * b + 2]));
+    v3 vc = add(V(v[0], v[1], v[2]), cross(w, c));
+    double* u = jqd + 6 * b;
+    u[0] = vc.x; u[1] = vc.y; u[2] = vc.z; u[3] = w.x; u[4] = w.y; u[5] = w.z;
+  }
+}
+
 int or_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -188,6 +188,66 @@ def cubic_min(c0, d0, c1, d1, h):
     return a.value if ok else None
 
 
+def coords_to_mujoco(com, joint_q, joint_qd):
+    """FP64 FREE-joint conversion Newton -> MuJoCo: (qpos [B,7] wxyz, qvel [B,6])."""
+    com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
+    jq = np.ascontiguousarray(joint_q, dtype=np.float64).reshape(-1, 7)
+    jqd = np.ascontiguousarray(joint_qd, dtype=np.float64).reshape(-1, 6)
+    qpos = np.zeros_like(jq); qvel = np.zeros_like(jqd)
+    lib().or_coords_to_mujoco(len(com), _ptr(com, f64p), _ptr(jq, f64p), _ptr(jqd, f64p), _ptr(qpos, f64p),
+                              _ptr(qvel, f64p))
+    return qpos, qvel
+
+
+def coords_from_mujoco(com, qpos, qvel):
+    """FP64 FREE-joint conversion MuJoCo -> Newton: (joint_q [B,7] xyzw, joint_qd [B,6])."""
+    com = np.ascontiguousarray(com, dtype=np.float64).reshape(-1, 3)
+    qpos = np.ascontiguousarray(qpos, dtype=np.float64).reshape(-1, 7)
+    qvel = np.ascontiguousarray(qvel, dtype=np.float64).reshape(-1, 6)
+    jq = np.zeros_like(qpos); jqd = np.zeros_like(qvel)
+    lib().or_coords_from_mujoco(len(com), _ptr(com, f64p), _ptr(qpos, f64p), _ptr(qvel, f64p), _ptr(jq, f64p),
+                                _ptr(jqd, f64p))
+    return jq, jqd
+
+
+def match_contacts(prev: dict, cur: dict, pos_threshold: float, normal_dot_threshold: float) -> np.ndarray:
+    """Frame-to-frame matching restated from geometry/contact_match.py (module docstring and
+    ContactMatcher.match): per world, a new contact takes the closest previous contact of the same
+    (shape0, shape1) pair whose midpoint moved at most ``pos_threshold`` and whose normal dot is at
+    least ``normal_dot_threshold``; each previous contact goes to the closest claimant (ties: the
+    lower new index within the world), the others become -2; a pair without previous contacts is
+    -1.  ``prev``/``cur``: dicts of world ``off``/``cnt`` arrays and per-contact ``s0``, ``s1``,
+    ``mid`` [n,3], ``n`` [n,3].  Returns the match index per current contact."""
+    out = np.full(len(cur["s0"]), -1, dtype=np.int64)
+    for w in range(len(cur["off"])):
+        po, pc = int(prev["off"][w]), int(prev["cnt"][w])
+        co, cc = int(cur["off"][w]), int(cur["cnt"][w])
+        best, bestd = {}, {}
+        for c in range(cc):
+            i = co + c
+            pair, bk, bd = False, -1, 0.0
+            for k in range(pc):
+                j = po + k
+                if prev["s0"][j] != cur["s0"][i] or prev["s1"][j] != cur["s1"][i]:
+                    continue
+                pair = True
+                d2 = float(np.sum((cur["mid"][i] - prev["mid"][j]) ** 2))
+                dn = float(np.dot(cur["n"][i], prev["n"][j]))
+                if d2 <= pos_threshold ** 2 and dn >= normal_dot_threshold and (bk < 0 or d2 < bd):
+                    bk, bd = k, d2
+            out[i] = -2 if pair else -1
+            if bk >= 0:
+                best[c], bestd[c] = bk, bd
+        winner = {}
+        for c in sorted(best):
+            k = best[c]
+            if k not in winner or (bestd[c], c) < (bestd[winner[k]], winner[k]):
+                winner[k] = c
+        for k, c in winner.items():
+            out[co + c] = po + k
+    return out
+
+
 def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, guess=None):
     """One inner step of world ``w`` (contacts at q): returns (q_out, qd_out, stats, ok) with
     stats = (newton iterations, factorizations, converged, contacts)."""

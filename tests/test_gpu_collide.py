@@ -31,7 +31,7 @@ def scenes(na):
     return _scenes(na)
 
 
-@pytest.mark.parametrize("bp", ["sap", "nxn"])
+@pytest.mark.parametrize("bp", ["sap", "nxn", "explicit"])
 @pytest.mark.parametrize("name", ["drop", "stack", "pile", "pile64"])
 def test_pair_sets_bit_exact(na, O, scenes, name, bp):
     m = scenes[name]
@@ -124,3 +124,101 @@ def test_contacts_deterministic_and_capacity_clamped(na, scenes):
     total = int(a.rigid_contact_count.item())
     assert total > 500  # the count reports every contact; the arrays hold the first 500
     assert int((a.rigid_contact_shape0 >= 0).sum()) == 500
+
+
+def _records(c, n):
+    """Contacts as sortable records (shape pair, then the raw float words)."""
+    cols = [c.rigid_contact_shape0.cpu().numpy()[:n, None].astype(np.float64),
+            c.rigid_contact_shape1.cpu().numpy()[:n, None].astype(np.float64)]
+    for a in (c.rigid_contact_point0, c.rigid_contact_point1, c.rigid_contact_normal, c.rigid_contact_offset0):
+        cols.append(a.cpu().numpy()[:n].astype(np.float64))
+    r = np.concatenate(cols, axis=1)
+    return r[np.lexsort(r.T[::-1])]
+
+
+@pytest.mark.parametrize("name", ["stack", "pile"])
+def test_explicit_broad_phase_contacts_equal_all_pairs(na, scenes, name):
+    """The explicit pair list (world / group / same-body / filter rules, model.contact_pairs) gives
+    the same contacts as the all-pairs broad phase, bit for bit."""
+    m = scenes[name]
+    state = m.state()
+    out = {}
+    for bp in ("nxn", "explicit"):
+        pipe = na.CollisionPipeline(m, broad_phase=bp)
+        c = pipe.contacts()
+        pipe.collide(state, c)
+        torch.cuda.synchronize()
+        n = int(c.rigid_contact_count.item())
+        out[bp] = _records(c, n)
+    assert len(out["nxn"]) > 0 and np.array_equal(out["nxn"], out["explicit"])
+
+
+def _match_inputs(pipe, m, c, body_q):
+    """Per-world offsets / counts and world midpoints of a compacted Contacts buffer."""
+    cnt = np.minimum(c.rigid_contact_world_count.cpu().numpy(), pipe.per_world)
+    off = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    n = int(c.rigid_contact_count.item())
+    s0, s1 = c.rigid_contact_shape0.cpu().numpy()[:n], c.rigid_contact_shape1.cpu().numpy()[:n]
+    sb = m.host("shape_body")
+    q = body_q.astype(np.float64)
+
+    def world(s, p):
+        b = sb[s]
+        out = p.copy()
+        for i in range(len(s)):
+            if b[i] >= 0:
+                x, y, z, w = q[b[i], 3:]
+                R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                              [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                              [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+                out[i] = q[b[i], :3] + R @ p[i]
+        return out
+
+    p0 = c.rigid_contact_point0.cpu().numpy()[:n].astype(np.float64)
+    p1 = c.rigid_contact_point1.cpu().numpy()[:n].astype(np.float64)
+    mid = 0.5 * (world(s0, p0) + world(s1, p1))
+    return {"off": off, "cnt": cnt, "s0": s0, "s1": s1, "mid": mid,
+            "n": c.rigid_contact_normal.cpu().numpy()[:n].astype(np.float64)}
+
+
+def test_contact_matching_against_restatement(na, O):
+    m = na.scenes.uneven_pile(32, seed=8, device=DEV, max_prims=10)
+    state = m.state()
+    solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
+    for _ in range(40):
+        solver.step(state, state, m.control(), None, 0.01)
+    torch.cuda.synchronize()
+    pipe = na.CollisionPipeline(m, contact_matching=True, contact_report=True)
+    c = pipe.contacts()
+    pipe.collide(state, c)
+    torch.cuda.synchronize()
+    n0 = int(c.rigid_contact_count.item())
+    assert n0 > 0 and (c.rigid_contact_match_index.cpu().numpy()[:n0] == -1).all()
+    assert int(c.rigid_contact_new_count.item()) == n0
+    prev = _match_inputs(pipe, m, c, state.body_q.cpu().numpy())
+    # the same state again: every contact matches itself, nothing new, nothing broken
+    pipe.collide(state, c)
+    torch.cuda.synchronize()
+    assert np.array_equal(c.rigid_contact_match_index.cpu().numpy()[:n0], np.arange(n0))
+    assert int(c.rigid_contact_new_count.item()) == 0 and int(c.rigid_contact_broken_count.item()) == 0
+    # move one body of every other world by 1 mm along x (beyond the 0.5 mm threshold) and a
+    # second body by 0.1 mm (within it)
+    q = state.body_q.clone()
+    wb = m.host("world_body_start")
+    for w in range(0, m.world_count, 2):
+        if wb[w + 1] - wb[w] >= 2:
+            q[wb[w], 0] += 1e-3
+            q[wb[w] + 1, 1] += 1e-4
+    moved = na.State(q, state.body_qd.clone(), state.body_f.clone())
+    pipe.collide(moved, c)
+    torch.cuda.synchronize()
+    n1 = int(c.rigid_contact_count.item())
+    cur = _match_inputs(pipe, m, c, q.cpu().numpy())
+    ref = O.match_contacts(prev, cur, pipe.pos_threshold, pipe.normal_dot_threshold)
+    got = c.rigid_contact_match_index.cpu().numpy()[:n1]
+    assert (got == -2).sum() > 0 and (got >= 0).sum() > 0
+    assert np.array_equal(got, ref)
+    new = c.rigid_contact_new_indices.cpu().numpy()[:int(c.rigid_contact_new_count.item())]
+    assert np.array_equal(new, np.nonzero(got < 0)[0])
+    broken = c.rigid_contact_broken_indices.cpu().numpy()[:int(c.rigid_contact_broken_count.item())]
+    assert np.array_equal(broken, np.setdiff1d(np.arange(n0), got[got >= 0]))
