@@ -100,6 +100,8 @@ class ModelBuilder:
         self.shape_scale: list = []
         self.shape_cfg: list = []
         self.shape_world: list = []
+        # explicit joints (articulations); a model with none is all free bodies
+        self.joints: list = []
 
     # ------------------------------------------------------------------ counts
     @property
@@ -160,14 +162,47 @@ class ModelBuilder:
         """Add a rigid body with a FREE joint to the world (``builder.py:4105``)."""
         b = self.add_link(xform, mass, label)
         self.body_has_joint[b] = True
+        if self.joints:
+            self._add_joint(4, -1, b, IDENTITY, IDENTITY, (0.0, 0.0, 1.0), 0.0, 0.0, 0.0, 0.0, 0.0)
         return b
 
     def add_joint_free(self, child: int, parent: int = -1, **_kw) -> int:
-        """Attach ``child`` to the world with a FREE joint (``builder.py:4696``)."""
+        """Attach ``child`` to the world with a FREE joint (``builder.py:4696``).  In an
+        articulation this is the floating base; its coordinates start at the body's xform."""
         if parent != -1:
             raise ValueError("only world-parented FREE joints are supported on this path")
         self.body_has_joint[child] = True
+        if self.joints:
+            return self._add_joint(4, -1, child, IDENTITY, IDENTITY, (0.0, 0.0, 1.0), 0.0, 0.0, 0.0, 0.0, 0.0)
         return child
+
+    def add_joint_revolute(self, parent: int, child: int, parent_xform=IDENTITY, child_xform=IDENTITY,
+                           axis=(0.0, 0.0, 1.0), target_pos: float = 0.0, target_ke: float = 0.0,
+                           target_kd: float = 0.0, armature: float = 0.0, q0: float = 0.0, **_kw) -> int:
+        """Revolute joint (``builder.py:4387``): ``child`` turns about ``axis`` (joint frame) by the
+        joint angle; the joint frame is ``parent_xform`` in the parent body frame (the world for
+        ``parent = -1``) and ``child_xform`` in the child body frame.  A PD drive pulls the angle
+        to ``target_pos`` with stiffness ``target_ke`` and damping ``target_kd``; ``armature``
+        adds to the joint-space inertia.  ``q0`` is the initial angle."""
+        if self.joints == [] and any(self.body_has_joint):
+            # the free bodies built so far become explicit FREE joints
+            for b in range(self.body_count):
+                if self.body_has_joint[b]:
+                    self._add_joint(4, -1, b, IDENTITY, IDENTITY, (0.0, 0.0, 1.0), 0.0, 0.0, 0.0, 0.0, 0.0)
+        self.body_has_joint[child] = True
+        return self._add_joint(1, parent, child, parent_xform, child_xform, axis, target_pos, target_ke, target_kd,
+                               armature, q0)
+
+    def _add_joint(self, jtype, parent, child, xp, xc, axis, tpos, tke, tkd, arm, q0) -> int:
+        if parent >= child:
+            raise ValueError("a joint's parent body must be added before its child")
+        if any(j["child"] == child for j in self.joints):
+            raise ValueError(f"body {child} already has a joint")
+        self.joints.append(dict(type=int(jtype), parent=int(parent), child=int(child),
+                                X_p=tuple(float(v) for v in xp), X_c=tuple(float(v) for v in xc),
+                                axis=tuple(float(v) for v in axis), target_pos=float(tpos), target_ke=float(tke),
+                                target_kd=float(tkd), armature=float(arm), q0=float(q0)))
+        return len(self.joints) - 1
 
     # ------------------------------------------------------------------ shapes
     def _add_shape(self, body, gtype, xform, scale, cfg):
@@ -407,4 +442,50 @@ class ModelBuilder:
             "world_shape_start": world_shape_start,
             "global_shapes": global_shapes,
         }
+        if self.joints:
+            host.update(self._articulation(B, bw, world_count, world_body_start, host))
         return Model(host, device)
+
+    def _articulation(self, B, bw, W, world_body_start, host) -> dict:
+        """Joint arrays of an articulated model: one joint per body, in body order, parents first."""
+        by_child = {j["child"]: j for j in self.joints}
+        if sorted(by_child) != list(range(B)):
+            raise ValueError("in an articulated model every body needs exactly one joint (FREE or revolute)")
+        J = [by_child[b] for b in range(B)]
+        for b, j in enumerate(J):
+            if j["type"] == 4 and j["parent"] != -1:
+                raise ValueError("FREE joints must have the world as parent")
+            if j["parent"] >= 0 and bw[j["parent"]] != bw[b]:
+                raise ValueError("a joint connects bodies of one world")
+        jtype = np.asarray([j["type"] for j in J], dtype=np.int32)
+        nq = np.where(jtype == 4, 7, 1)
+        nd = np.where(jtype == 4, 6, 1)
+        q_start = np.concatenate([[0], np.cumsum(nq)]).astype(np.int32)
+        qd_start = np.concatenate([[0], np.cumsum(nd)]).astype(np.int32)
+        joint_q = np.zeros(int(q_start[-1]))
+        per_dof = {k: np.zeros(int(qd_start[-1])) for k in ("target_pos", "target_ke", "target_kd", "armature")}
+        for b, j in enumerate(J):
+            if j["type"] == 4:
+                joint_q[q_start[b]:q_start[b] + 7] = self.body_q[b]
+            else:
+                joint_q[q_start[b]] = j["q0"]
+                for k in per_dof:
+                    per_dof[k][qd_start[b]] = j[k]
+        art = {
+            "joint_type": jtype,
+            "joint_parent": np.asarray([j["parent"] for j in J], dtype=np.int32),
+            "joint_X_p": np.asarray([j["X_p"] for j in J], dtype=np.float64).reshape(B, 7),
+            "joint_X_c": np.asarray([j["X_c"] for j in J], dtype=np.float64).reshape(B, 7),
+            "joint_axis": np.asarray([j["axis"] for j in J], dtype=np.float64).reshape(B, 3),
+            "joint_q_start": q_start, "joint_qd_start": qd_start,
+            "world_joint_start": world_body_start.copy(),
+            "world_q_start": q_start[world_body_start], "world_qd_start": qd_start[world_body_start],
+            "joint_q": joint_q, "joint_qd": np.zeros(int(qd_start[-1])),
+            "joint_target_pos": per_dof["target_pos"], "joint_target_ke": per_dof["target_ke"],
+            "joint_target_kd": per_dof["target_kd"], "joint_armature": per_dof["armature"],
+        }
+        # initial body poses from the joint coordinates
+        from .articulation import fk_host
+        host_fk = dict(host, **art)
+        art["body_q"] = fk_host(host_fk, joint_q)
+        return art

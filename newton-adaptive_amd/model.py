@@ -135,6 +135,21 @@ class Model:
         self.world_shape_start = _t(host["world_shape_start"], torch.int32, d)
         self.global_shapes = _t(host["global_shapes"], torch.int32, d)
 
+        # articulated worlds: one joint per body (REVOLUTE / FREE root), generalized coordinates
+        self.is_articulated = "joint_type" in host
+        if self.is_articulated:
+            for name in ("joint_type", "joint_parent", "joint_q_start", "joint_qd_start", "world_joint_start",
+                         "world_q_start", "world_qd_start"):
+                setattr(self, name, _t(host[name], torch.int32, d))
+            for name in ("joint_X_p", "joint_X_c", "joint_axis", "joint_target_pos", "joint_target_ke",
+                         "joint_target_kd", "joint_armature"):
+                setattr(self, name, _t(host[name], torch.float32, d))
+            self.joint_q = _t(host["joint_q"], torch.float32, d)
+            self.joint_qd = _t(host["joint_qd"], torch.float32, d)
+            self.joint_count = self.body_count
+            self.joint_coord_count = int(host["joint_q_start"][-1])
+            self.joint_dof_count = int(host["joint_qd_start"][-1])
+
     # --- explicit broad-phase pair list (model.shape_contact_pairs of the reference) ---
     def contact_pairs(self):
         """Candidate shape pairs for the explicit broad phase: canonical ``(min, max)`` global
@@ -180,8 +195,12 @@ class Model:
 
     def state(self) -> State:
         """Fresh state initialised from the model's reference pose (``model.py:1572``)."""
-        return State(self.body_q.clone(), self.body_qd.clone(),
-                     torch.zeros((self.body_count, 6), dtype=torch.float32, device=self.device))
+        s = State(self.body_q.clone(), self.body_qd.clone(),
+                  torch.zeros((self.body_count, 6), dtype=torch.float32, device=self.device))
+        if self.is_articulated:
+            s.joint_q = self.joint_q.clone()
+            s.joint_qd = self.joint_qd.clone()
+        return s
 
     def control(self) -> Control:
         return Control(self)

@@ -168,3 +168,89 @@ def shape_kinds(model) -> np.ndarray:
     """Per-world histogram of body shape types (diagnostic)."""
     st = model.host("shape_type")
     return np.bincount(st, minlength=GeoType.BOX + 1)
+
+
+# ---------------------------------------------------------------- articulated worlds
+ROBOT_GROUP = -1  # one negative group: links of a robot never collide with each other
+
+
+def _terrain(builder, rng, cfg, count, half_extent, height=(0.02, 0.08)):
+    """``count`` static boxes of the current world (random size, yaw and height) on the ground."""
+    for _ in range(count):
+        hx, hy = rng.uniform(0.05, 0.2, size=2)
+        hz = rng.uniform(*height) * 0.5
+        x, y = rng.uniform(-half_extent, half_extent, size=2)
+        yaw = rng.uniform(-math.pi, math.pi)
+        q = (0.0, 0.0, math.sin(0.5 * yaw), math.cos(0.5 * yaw))
+        builder.add_shape_box(-1, xform=(x, y, hz, *q), hx=hx, hy=hy, hz=hz, cfg=cfg)
+
+
+def _add_arm(builder, rng, ke=1.0e4):
+    """7-dof revolute arm on a fixed base: capsule links of 0.16-0.26 m, joint axes alternating
+    z / y, PD drives to seeded random targets."""
+    link_cfg = ModelBuilder.ShapeConfig(density=1000.0, ke=ke, mu=0.6, gap=PILE_GAP, collision_group=ROBOT_GROUP)
+    axes = [(0, 0, 1), (0, 1, 0), (0, 0, 1), (0, 1, 0), (0, 0, 1), (0, 1, 0), (0, 0, 1)]
+    parent, offset = -1, (0.0, 0.0, 0.05, 0.0, 0.0, 0.0, 1.0)
+    for k in range(7):
+        half = float(rng.uniform(0.08, 0.13))
+        link = builder.add_link()
+        builder.add_shape_capsule(link, xform=(0.0, 0.0, half, 0.0, 0.0, 0.0, 1.0), radius=0.035, half_height=half,
+                                  cfg=link_cfg)
+        target = float(rng.uniform(-0.8, 0.8))
+        builder.add_joint_revolute(parent, link, parent_xform=offset, axis=axes[k], target_pos=target,
+                                   target_ke=60.0, target_kd=4.0, armature=0.01,
+                                   q0=float(rng.uniform(-0.3, 0.3)))
+        parent, offset = link, (0.0, 0.0, 2.0 * half, 0.0, 0.0, 0.0, 1.0)
+
+
+def _add_quadruped(builder, rng, ke=1.0e4):
+    """Floating-base quadruped: box trunk, four legs of hip (x axis), thigh and knee (y axis)
+    joints, capsule segments and sphere feet; PD drives to a standing pose with seeded offsets."""
+    link_cfg = ModelBuilder.ShapeConfig(density=500.0, ke=ke, mu=0.8, gap=PILE_GAP, collision_group=ROBOT_GROUP)
+    z0 = 0.42
+    trunk = builder.add_link(xform=(0.0, 0.0, z0, 0.0, 0.0, 0.0, 1.0))
+    builder.add_shape_box(trunk, hx=0.25, hy=0.1, hz=0.05, cfg=link_cfg)
+    builder.add_joint_free(trunk)
+    for sx in (1.0, -1.0):
+        for sy in (1.0, -1.0):
+            hip = builder.add_link()
+            builder.add_shape_sphere(hip, radius=0.035, cfg=link_cfg)
+            builder.add_joint_revolute(trunk, hip, parent_xform=(0.2 * sx, 0.1 * sy, 0.0, 0.0, 0.0, 0.0, 1.0),
+                                       axis=(1, 0, 0), target_pos=float(rng.uniform(-0.15, 0.15)), target_ke=40.0,
+                                       target_kd=1.0, armature=0.01)
+            thigh = builder.add_link()
+            builder.add_shape_capsule(thigh, xform=(0.0, 0.0, -0.1, 0.0, 0.0, 0.0, 1.0), radius=0.025,
+                                      half_height=0.08, cfg=link_cfg)
+            t0 = 0.6 + float(rng.uniform(-0.2, 0.2))
+            builder.add_joint_revolute(hip, thigh, parent_xform=(0.0, 0.05 * sy, 0.0, 0.0, 0.0, 0.0, 1.0),
+                                       axis=(0, 1, 0), target_pos=t0, target_ke=40.0, target_kd=1.0,
+                                       armature=0.01, q0=0.6)
+            shank = builder.add_link()
+            builder.add_shape_capsule(shank, xform=(0.0, 0.0, -0.1, 0.0, 0.0, 0.0, 1.0), radius=0.02,
+                                      half_height=0.08, cfg=link_cfg)
+            builder.add_shape_sphere(shank, xform=(0.0, 0.0, -0.2, 0.0, 0.0, 0.0, 1.0), radius=0.025, cfg=link_cfg)
+            k0 = -1.2 + float(rng.uniform(-0.2, 0.2))
+            builder.add_joint_revolute(thigh, shank, parent_xform=(0.0, 0.0, -0.2, 0.0, 0.0, 0.0, 1.0),
+                                       axis=(0, 1, 0), target_pos=k0, target_ke=40.0, target_kd=1.0,
+                                       armature=0.01, q0=-1.2)
+
+
+def articulated_mix(world_count: int = 8192, seed: int = 0, device="cuda:0", kinds=None, terrain_boxes: int = 6):
+    """Articulated worlds: world ``w`` is a 7-dof arm when ``kinds[w] == 0`` (default: even worlds)
+    and a 12-dof floating quadruped (18 dofs) otherwise, with seeded random joint targets, over
+    ``terrain_boxes`` random static boxes and the ground plane."""
+    builder = ModelBuilder()
+    terrain_cfg = ModelBuilder.ShapeConfig(ke=1.0e4, mu=0.8, gap=PILE_GAP)
+    for w in range(world_count):
+        rng = np.random.default_rng(seed + w)
+        kind = (w % 2) if kinds is None else int(kinds[w])
+        builder.begin_world()
+        if kind == 0:
+            _add_arm(builder, rng)
+            _terrain(builder, rng, terrain_cfg, terrain_boxes, 0.8)
+        else:
+            _add_quadruped(builder, rng)
+            _terrain(builder, rng, terrain_cfg, terrain_boxes, 0.6)
+        builder.end_world()
+    builder.add_ground_plane(cfg=ModelBuilder.ShapeConfig(ke=1.0e4, mu=0.8, gap=PILE_GAP))
+    return builder.finalize(device)

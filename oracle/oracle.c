@@ -888,6 +888,10 @@ typedef struct {
   real* Rt; real* Rn; real* vhat; real* mu;
   /* scratch */
   real* H; real* g; real* dv; real* v; real* vc; real* dvc; real* gam; real* G; real* tmp;
+  /* generalized-coordinate problems (articulated worlds): dense A [N,N] and dense contact rows
+     [nc, 3, N]; NULL for free bodies (block-diagonal A, per-body 3x6 blocks) */
+  real* Adense;
+  real* Jd;
 } sap_ctx;
 
 /* P_F projection in the R norm and its Jacobian-derived Hessian G = dP/dy R^-1 */
@@ -917,6 +921,15 @@ static real project(real mu, real Rt, real Rn, const real* y, real* gam, real* G
 
 static void contact_velocity(const sap_ctx* S, int c, const real* v, real* vc) {
   vc[0] = vc[1] = vc[2] = 0;
+  if (S->Jd) {
+    for (int r = 0; r < 3; ++r) {
+      const real* row = S->Jd + ((size_t)3 * c + r) * S->N;
+      real acc = 0;
+      for (int k = 0; k < S->N; ++k) acc += row[k] * v[k];
+      vc[r] = acc;
+    }
+    return;
+  }
   for (int s = 0; s < 2; ++s) {
     int b = S->cb[2 * c + s];
     if (b < 0) continue;
@@ -932,6 +945,14 @@ static void contact_velocity(const sap_ctx* S, int c, const real* v, real* vc) {
 
 /* momentum A x for block-diagonal A = diag(m I, Iw) */
 static void apply_A(const sap_ctx* S, const real* x, real* y) {
+  if (S->Adense) {
+    for (int i = 0; i < S->N; ++i) {
+      real acc = 0;
+      for (int k = 0; k < S->N; ++k) acc += S->Adense[(size_t)i * S->N + k] * x[k];
+      y[i] = acc;
+    }
+    return;
+  }
   for (int b = 0; b < S->nb; ++b) {
     const real* xb = x + 6 * b; real* yb = y + 6 * b; const real* I = S->Iw + 9 * b;
     yb[0] = S->m[b] * xb[0]; yb[1] = S->m[b] * xb[1]; yb[2] = S->m[b] * xb[2];
@@ -956,6 +977,13 @@ static real eval_cost(sap_ctx* S, const real* v, real* g, real* gam, real* G, re
     cost += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
     if (gam) memcpy(gam + 3 * c, gl, sizeof(gl));
     if (G) memcpy(G + 9 * c, Gl, sizeof(Gl));
+    if (S->Jd) {
+      for (int r = 0; r < 3; ++r) {
+        const real* row = S->Jd + ((size_t)3 * c + r) * N;
+        for (int k = 0; k < N; ++k) jc[k] += row[k] * gl[r];
+      }
+      continue;
+    }
     for (int s = 0; s < 2; ++s) {
       int b = S->cb[2 * c + s];
       if (b < 0) continue;
@@ -1037,12 +1065,24 @@ static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) 
     if (it == C->max_iterations) break;
     /* H = A + sum J^T G J (dense) */
     memset(S->H, 0, sizeof(real) * N * N);
-    for (int b = 0; b < S->nb; ++b) {
+    if (S->Adense) {
+      memcpy(S->H, S->Adense, sizeof(real) * N * N);
+      for (int c = 0; c < S->nc; ++c) {
+        const real* Gc = S->G + 9 * c;
+        const real* Jc = S->Jd + (size_t)3 * c * N;
+        for (int i = 0; i < N; ++i) {
+          real gj[3];
+          for (int r = 0; r < 3; ++r) gj[r] = Gc[3 * r] * Jc[i] + Gc[3 * r + 1] * Jc[N + i] + Gc[3 * r + 2] * Jc[2 * N + i];
+          for (int k = 0; k < N; ++k) S->H[(size_t)k * N + i] += Jc[k] * gj[0] + Jc[N + k] * gj[1] + Jc[2 * N + k] * gj[2];
+        }
+      }
+    }
+    for (int b = 0; b < S->nb && !S->Adense; ++b) {
       int o = 6 * b;
       S->H[o * N + o] = S->m[b]; S->H[(o + 1) * N + o + 1] = S->m[b]; S->H[(o + 2) * N + o + 2] = S->m[b];
       for (int r = 0; r < 3; ++r) for (int q = 0; q < 3; ++q) S->H[(o + 3 + r) * N + o + 3 + q] = S->Iw[9 * b + 3 * r + q];
     }
-    for (int c = 0; c < S->nc; ++c) {
+    for (int c = 0; c < S->nc && !S->Adense; ++c) {
       const real* Gc = S->G + 9 * c;
       for (int s1 = 0; s1 < 2; ++s1) {
         int b1 = S->cb[2 * c + s1];
@@ -1197,6 +1237,7 @@ static void sap_alloc(sap_ctx* S, int nb, int nc) {
   S->gam = (real*)calloc(3 * ncc, sizeof(real)); S->G = (real*)calloc(9 * ncc, sizeof(real));
   S->tmp = (real*)calloc(4 * N, sizeof(real));
   S->vc = NULL; S->dvc = NULL;
+  S->Adense = NULL; S->Jd = NULL;
 }
 static void sap_free(sap_ctx* S) {
   free(S->m); free(S->Iw); free(S->Iwi); free(S->xc); free(S->vstar); free(S->Ad); free(S->cb); free(S->J);
@@ -1436,6 +1477,501 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
   }
   K->cum_accepted[0] += cum;
   return status;
+}
+
+/* ====================================================================
+ * Articulated worlds: generalized-coordinate step doubling.
+ *
+ * Joints are REVOLUTE (1 dof) or FREE (root only, parent = world, identity joint frames), listed
+ * parent-before-child; joint j of world w moves body world_body_start[w] + (j - world_joint_start[w]).
+ * Coordinates: REVOLUTE angle / rate; FREE (origin position, quaternion xyzw) / (COM velocity,
+ * angular velocity), all world frame -- Newton's joint_q / joint_qd conventions (state.py:149-153).
+ * Spatial vectors are world-frame Pluecker coordinates at the world origin, (angular; linear).
+ * Dynamics: M(q) = sum_b J_b^T I_b J_b, bias c(q, v) from Newton-Euler (gravity included),
+ * implicit PD drives tau = ke (q_t - q - h v') - kd v' on revolute dofs, armature on the diagonal:
+ *   A = M + diag(armature + h (kd + h ke)),  A v* = (M + diag(armature)) v + h (ke (q_t - q) - c).
+ * The convex contact problem is the free-body one with dense A and contact rows J_c = F^T (P_b J_b
+ * - P_a J_a), P = [-[p]x, I] the point-velocity map; R_n, R_t use tr(J_c A^-1 J_c^T) / 3.
+ * ==================================================================== */
+#define OR_JT_REVOLUTE 1
+#define OR_JT_FREE 4
+#define OR_ART_W_MIN 1.0e-10 /* Delassus diagonal below which a contact cannot move [1/kg] */
+typedef struct or_art {
+  int joint_count;
+  const int32_t* world_joint_start; /* [W+1] */
+  const int32_t* joint_type;
+  const int32_t* joint_parent;      /* global body index, -1 world */
+  const double* joint_X_p;          /* [J,7] joint frame in the parent body (or world) frame */
+  const double* joint_X_c;          /* [J,7] joint frame in the child body frame */
+  const double* joint_axis;         /* [J,3] revolute axis in the joint frame */
+  const int32_t* joint_q_start;     /* [J+1] */
+  const int32_t* joint_qd_start;    /* [J+1] */
+  const double* target_pos;         /* [dofs] */
+  const double* target_ke;
+  const double* target_kd;
+  const double* armature;
+} or_art;
+
+typedef struct { double p[3]; double R[9]; } rigid;  /* world pose, R row-major */
+
+static void rigid_from7(const double* x, rigid* T) {
+  qt q = {x[3], x[4], x[5], x[6]};
+  m33 R = qmat(q);
+  for (int i = 0; i < 9; ++i) T->R[i] = R.m[i];
+  for (int i = 0; i < 3; ++i) T->p[i] = x[i];
+}
+static void rigid_mul(const rigid* a, const rigid* b, rigid* o) {
+  rigid t;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j)
+      t.R[3 * i + j] = a->R[3 * i] * b->R[j] + a->R[3 * i + 1] * b->R[3 + j] + a->R[3 * i + 2] * b->R[6 + j];
+    t.p[i] = a->p[i] + a->R[3 * i] * b->p[0] + a->R[3 * i + 1] * b->p[1] + a->R[3 * i + 2] * b->p[2];
+  }
+  *o = t;
+}
+static void rigid_inv(const rigid* a, rigid* o) {
+  rigid t;
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) t.R[3 * i + j] = a->R[3 * j + i];
+  for (int i = 0; i < 3; ++i) t.p[i] = -(t.R[3 * i] * a->p[0] + t.R[3 * i + 1] * a->p[1] + t.R[3 * i + 2] * a->p[2]);
+  *o = t;
+}
+static void rot_axis_angle(const double* a, double th, rigid* o) {
+  double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  double x = a[0] / n, y = a[1] / n, z = a[2] / n, s = sin(0.5 * th), c = cos(0.5 * th);
+  double x7[7] = {0, 0, 0, x * s, y * s, z * s, c};
+  rigid_from7(x7, o);
+}
+/* rotation matrix -> quaternion (x, y, z, w), w >= 0 branch-stable */
+static void rot_to_quat(const double* R, double* q) {
+  double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = sqrt(tr + 1.0) * 2;
+    q[3] = 0.25 * s; q[0] = (R[7] - R[5]) / s; q[1] = (R[2] - R[6]) / s; q[2] = (R[3] - R[1]) / s;
+  } else if (R[0] > R[4] && R[0] > R[8]) {
+    double s = sqrt(1.0 + R[0] - R[4] - R[8]) * 2;
+    q[3] = (R[7] - R[5]) / s; q[0] = 0.25 * s; q[1] = (R[1] + R[3]) / s; q[2] = (R[2] + R[6]) / s;
+  } else if (R[4] > R[8]) {
+    double s = sqrt(1.0 + R[4] - R[0] - R[8]) * 2;
+    q[3] = (R[2] - R[6]) / s; q[0] = (R[1] + R[3]) / s; q[1] = 0.25 * s; q[2] = (R[5] + R[7]) / s;
+  } else {
+    double s = sqrt(1.0 + R[8] - R[0] - R[4]) * 2;
+    q[3] = (R[3] - R[1]) / s; q[0] = (R[2] + R[6]) / s; q[1] = (R[5] + R[7]) / s; q[2] = 0.25 * s;
+  }
+}
+
+typedef struct {
+  int j0, nj, b0, nb, q0, nq, d0, nv;
+  rigid* X;       /* [nb] body poses */
+  double* S;      /* [nv, 6] motion subspace columns */
+  int* dof_body;  /* [nv] body moved by the dof's joint */
+  int* parent;    /* [nb] local parent body, -1 world */
+} art_kin;
+
+static void art_alloc(const or_model* M, const or_art* A, int w, art_kin* K) {
+  K->j0 = A->world_joint_start[w];
+  K->nj = A->world_joint_start[w + 1] - K->j0;
+  K->b0 = M->world_body_start[w];
+  K->nb = M->world_body_start[w + 1] - K->b0;
+  K->q0 = A->joint_q_start[K->j0];
+  K->nq = A->joint_q_start[K->j0 + K->nj] - K->q0;
+  K->d0 = A->joint_qd_start[K->j0];
+  K->nv = A->joint_qd_start[K->j0 + K->nj] - K->d0;
+  K->X = (rigid*)calloc(K->nb > 0 ? K->nb : 1, sizeof(rigid));
+  K->S = (double*)calloc(6 * (K->nv > 0 ? K->nv : 1), sizeof(double));
+  K->dof_body = (int*)calloc(K->nv > 0 ? K->nv : 1, sizeof(int));
+  K->parent = (int*)calloc(K->nb > 0 ? K->nb : 1, sizeof(int));
+}
+static void art_free(art_kin* K) { free(K->X); free(K->S); free(K->dof_body); free(K->parent); }
+
+/* forward kinematics and motion subspaces of world w at its coordinates q [nq] */
+static void art_fk(const or_model* M, const or_art* A, art_kin* K, const double* q) {
+  for (int jl = 0; jl < K->nj; ++jl) {
+    int j = K->j0 + jl, b = jl;
+    int pg = A->joint_parent[j];
+    int p = pg < 0 ? -1 : pg - K->b0;
+    K->parent[b] = p;
+    const double* qj = q + (A->joint_q_start[j] - K->q0);
+    int dj = A->joint_qd_start[j] - K->d0;
+    if (A->joint_type[j] == OR_JT_FREE) {
+      rigid_from7(qj, &K->X[b]);
+      const double* com = M->body_com + 3 * (K->b0 + b);
+      double c[3];
+      for (int i = 0; i < 3; ++i)
+        c[i] = K->X[b].p[i] + K->X[b].R[3 * i] * com[0] + K->X[b].R[3 * i + 1] * com[1] + K->X[b].R[3 * i + 2] * com[2];
+      /* dofs (v_com, w): spatial (w; v_O) with v_O = v_com - w x c = v_com + c x w */
+      for (int k = 0; k < 3; ++k) {
+        double* s = K->S + 6 * (dj + k);
+        for (int i = 0; i < 6; ++i) s[i] = 0;
+        s[3 + k] = 1;
+        double* t = K->S + 6 * (dj + 3 + k);
+        double e[3] = {0, 0, 0};
+        e[k] = 1;
+        for (int i = 0; i < 3; ++i) t[i] = e[i];
+        t[3] = c[1] * e[2] - c[2] * e[1]; t[4] = c[2] * e[0] - c[0] * e[2]; t[5] = c[0] * e[1] - c[1] * e[0];
+      }
+      for (int k = 0; k < 6; ++k) K->dof_body[dj + k] = b;
+    } else {
+      rigid Xp, Xj, Xc, Xci, R;
+      rigid_from7(A->joint_X_p + 7 * j, &Xp);
+      if (p >= 0) rigid_mul(&K->X[p], &Xp, &Xj);
+      else Xj = Xp;
+      rot_axis_angle(A->joint_axis + 3 * j, qj[0], &R);
+      rigid_from7(A->joint_X_c + 7 * j, &Xc);
+      rigid_inv(&Xc, &Xci);
+      rigid tmp;
+      rigid_mul(&Xj, &R, &tmp);
+      rigid_mul(&tmp, &Xci, &K->X[b]);
+      const double* ax = A->joint_axis + 3 * j;
+      double an = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+      double a[3], o[3];
+      for (int i = 0; i < 3; ++i) a[i] = (Xj.R[3 * i] * ax[0] + Xj.R[3 * i + 1] * ax[1] + Xj.R[3 * i + 2] * ax[2]) / an;
+      for (int i = 0; i < 3; ++i) o[i] = Xj.p[i];
+      double* s = K->S + 6 * dj;
+      s[0] = a[0]; s[1] = a[1]; s[2] = a[2];
+      s[3] = o[1] * a[2] - o[2] * a[1]; s[4] = o[2] * a[0] - o[0] * a[2]; s[5] = o[0] * a[1] - o[1] * a[0];
+      K->dof_body[dj] = b;
+    }
+  }
+}
+/* does body b's path from the root contain the joint of body a (a is b or an ancestor)? */
+static int art_on_path(const art_kin* K, int a, int b) {
+  for (int x = b; x >= 0; x = K->parent[x]) if (x == a) return 1;
+  return 0;
+}
+/* spatial inertia of body b at the world origin, (angular; linear) ordering, 6x6 row-major */
+static void art_inertia(const or_model* M, const art_kin* K, int b, double* I6, double* c_out) {
+  int gb = K->b0 + b;
+  const rigid* X = &K->X[b];
+  const double* com = M->body_com + 3 * gb;
+  const double* Ib = M->body_inertia + 9 * gb;
+  double m = M->body_mass[gb], c[3], Ic[9];
+  for (int i = 0; i < 3; ++i) c[i] = X->p[i] + X->R[3 * i] * com[0] + X->R[3 * i + 1] * com[1] + X->R[3 * i + 2] * com[2];
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+    double a = 0;
+    for (int k = 0; k < 3; ++k) for (int l = 0; l < 3; ++l) a += X->R[3 * i + k] * Ib[3 * k + l] * X->R[3 * j + l];
+    Ic[3 * i + j] = a;
+  }
+  double cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  double cx[9] = {0, -c[2], c[1], c[2], 0, -c[0], -c[1], c[0], 0};
+  for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+    I6[6 * i + j] = Ic[3 * i + j] + m * ((i == j ? cc : 0) - c[i] * c[j]);
+    I6[6 * i + 3 + j] = m * cx[3 * i + j];
+    I6[6 * (3 + i) + j] = -m * cx[3 * i + j];
+    I6[6 * (3 + i) + 3 + j] = i == j ? m : 0;
+  }
+  if (c_out) for (int i = 0; i < 3; ++i) c_out[i] = c[i];
+}
+static void mcross(const double* V, const double* m, double* o) { /* motion cross V x m */
+  const double *w = V, *v = V + 3, *a = m, *b = m + 3;
+  o[0] = w[1] * a[2] - w[2] * a[1]; o[1] = w[2] * a[0] - w[0] * a[2]; o[2] = w[0] * a[1] - w[1] * a[0];
+  o[3] = w[1] * b[2] - w[2] * b[1] + v[1] * a[2] - v[2] * a[1];
+  o[4] = w[2] * b[0] - w[0] * b[2] + v[2] * a[0] - v[0] * a[2];
+  o[5] = w[0] * b[1] - w[1] * b[0] + v[0] * a[1] - v[1] * a[0];
+}
+static void fcross(const double* V, const double* f, double* o) { /* force cross V x* f */
+  const double *w = V, *v = V + 3, *n = f, *fl = f + 3;
+  o[0] = w[1] * n[2] - w[2] * n[1] + v[1] * fl[2] - v[2] * fl[1];
+  o[1] = w[2] * n[0] - w[0] * n[2] + v[2] * fl[0] - v[0] * fl[2];
+  o[2] = w[0] * n[1] - w[1] * n[0] + v[0] * fl[1] - v[1] * fl[0];
+  o[3] = w[1] * fl[2] - w[2] * fl[1]; o[4] = w[2] * fl[0] - w[0] * fl[2]; o[5] = w[0] * fl[1] - w[1] * fl[0];
+}
+
+/* Mass matrix M [nv,nv] and bias c [nv] (velocity products and gravity) at (q, v). */
+static void art_dynamics(const or_model* M, const or_art* A, const art_kin* K, const double* v, double* Mm,
+                         double* c) {
+  int nv = K->nv, nb = K->nb;
+  memset(Mm, 0, sizeof(double) * nv * nv);
+  memset(c, 0, sizeof(double) * nv);
+  double* Vb = (double*)calloc(6 * (nb ? nb : 1), sizeof(double));
+  double* ab = (double*)calloc(6 * (nb ? nb : 1), sizeof(double));
+  /* body velocities V_b = sum_{dofs on path} S_k v_k */
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < nv; ++k)
+      if (art_on_path(K, K->dof_body[k], b))
+        for (int i = 0; i < 6; ++i) Vb[6 * b + i] += K->S[6 * k + i] * v[k];
+  /* velocity-product accelerations: revolute Sdot = V_child x S; FREE (v_com, w) dofs: the
+     angular columns carry c x e_k, whose rate gives (0; v_com x w) */
+  double* sdq = (double*)calloc(6 * (nv ? nv : 1), sizeof(double));
+  for (int jl = 0; jl < K->nj; ++jl) {
+    int j = K->j0 + jl, dj = A->joint_qd_start[j] - K->d0, b = jl;
+    if (A->joint_type[j] == OR_JT_FREE) {
+      const double *vc = v + dj, *w = v + dj + 3;
+      double* o = sdq + 6 * dj;
+      o[3] = vc[1] * w[2] - vc[2] * w[1]; o[4] = vc[2] * w[0] - vc[0] * w[2]; o[5] = vc[0] * w[1] - vc[1] * w[0];
+    } else {
+      double t[6];
+      mcross(Vb + 6 * b, K->S + 6 * dj, t);
+      for (int i = 0; i < 6; ++i) sdq[6 * dj + i] = t[i] * v[dj];
+    }
+  }
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < nv; ++k)
+      if (art_on_path(K, K->dof_body[k], b))
+        for (int i = 0; i < 6; ++i) ab[6 * b + i] += sdq[6 * k + i];
+  for (int b = 0; b < nb; ++b) {
+    double I6[36], cb[3], f[6], IV[6], t[6];
+    art_inertia(M, K, b, I6, cb);
+    double m = M->body_mass[K->b0 + b];
+    for (int i = 0; i < 6; ++i) {
+      f[i] = 0; IV[i] = 0;
+      for (int k = 0; k < 6; ++k) { f[i] += I6[6 * i + k] * ab[6 * b + k]; IV[i] += I6[6 * i + k] * Vb[6 * b + k]; }
+    }
+    fcross(Vb + 6 * b, IV, t);
+    const double* g = M->gravity;
+    double fg[6] = {m * (cb[1] * g[2] - cb[2] * g[1]), m * (cb[2] * g[0] - cb[0] * g[2]),
+                    m * (cb[0] * g[1] - cb[1] * g[0]), m * g[0], m * g[1], m * g[2]};
+    for (int i = 0; i < 6; ++i) f[i] += t[i] - fg[i];
+    /* J_b^T f and J_b^T I J_b */
+    for (int k = 0; k < nv; ++k) {
+      if (!art_on_path(K, K->dof_body[k], b)) continue;
+      const double* sk = K->S + 6 * k;
+      for (int i = 0; i < 6; ++i) c[k] += sk[i] * f[i];
+      double Isk[6];
+      for (int i = 0; i < 6; ++i) { Isk[i] = 0; for (int l = 0; l < 6; ++l) Isk[i] += I6[6 * i + l] * sk[l]; }
+      for (int l = 0; l < nv; ++l) {
+        if (!art_on_path(K, K->dof_body[l], b)) continue;
+        const double* sl = K->S + 6 * l;
+        double s = 0;
+        for (int i = 0; i < 6; ++i) s += sl[i] * Isk[i];
+        Mm[l * nv + k] += s;
+      }
+    }
+  }
+  free(Vb); free(ab); free(sdq);
+}
+
+/* body poses [nb,7] of world w at coordinates q (for collision) */
+static void art_body_q(const art_kin* K, double* bq) {
+  for (int b = 0; b < K->nb; ++b) {
+    for (int i = 0; i < 3; ++i) bq[7 * b + i] = K->X[b].p[i];
+    rot_to_quat(K->X[b].R, bq + 7 * b + 3);
+  }
+}
+
+/* Integrate coordinates: REVOLUTE angle += h rate; FREE: COM += h v_com, quaternion += 1/2 h (w,0) q
+   (normalised), origin = COM - R com (solver.py:50-94 without damping). */
+static void art_integrate(const or_model* M, const or_art* A, const art_kin* K, const double* q, const double* v,
+                          double h, double* qo) {
+  memcpy(qo, q, sizeof(double) * K->nq);
+  for (int jl = 0; jl < K->nj; ++jl) {
+    int j = K->j0 + jl, qj = A->joint_q_start[j] - K->q0, dj = A->joint_qd_start[j] - K->d0;
+    if (A->joint_type[j] == OR_JT_FREE) {
+      const double* bq = q + qj;
+      const double* com = M->body_com + 3 * (K->b0 + jl);
+      double qx = bq[3], qy = bq[4], qz = bq[5], qw = bq[6];
+      double R0[9] = {1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw),
+                      2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw),
+                      2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)};
+      double xc[3];
+      for (int i = 0; i < 3; ++i) xc[i] = bq[i] + R0[3 * i] * com[0] + R0[3 * i + 1] * com[1] + R0[3 * i + 2] * com[2] + h * v[dj + i];
+      double wx = v[dj + 3], wy = v[dj + 4], wz = v[dj + 5];
+      double dx = wy * qz - wz * qy + wx * qw, dy = wz * qx - wx * qz + wy * qw;
+      double dz = wx * qy - wy * qx + wz * qw, dw = -(wx * qx + wy * qy + wz * qz);
+      double nx = qx + 0.5 * h * dx, ny = qy + 0.5 * h * dy, nz = qz + 0.5 * h * dz, nw = qw + 0.5 * h * dw;
+      double l = sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+      nx /= l; ny /= l; nz /= l; nw /= l;
+      double R1[9] = {1 - 2 * (ny * ny + nz * nz), 2 * (nx * ny - nz * nw), 2 * (nx * nz + ny * nw),
+                      2 * (nx * ny + nz * nw), 1 - 2 * (nx * nx + nz * nz), 2 * (ny * nz - nx * nw),
+                      2 * (nx * nz - ny * nw), 2 * (ny * nz + nx * nw), 1 - 2 * (nx * nx + ny * ny)};
+      double* o = qo + qj;
+      for (int i = 0; i < 3; ++i) o[i] = xc[i] - (R1[3 * i] * com[0] + R1[3 * i + 1] * com[1] + R1[3 * i + 2] * com[2]);
+      o[3] = nx; o[4] = ny; o[5] = nz; o[6] = nw;
+    } else {
+      qo[qj] = q[qj] + h * v[dj];
+    }
+  }
+}
+
+/* One inner step of articulated world w (contacts cs at q): returns 1 when the solve converged. */
+static int art_inner_step(const or_model* M, const or_art* A, const or_config* C, int w, const double* q,
+                          const double* v0, double h, const or_contact* cs, int nc, const double* guess, double* q_out,
+                          double* v_out, solve_stats* st) {
+  art_kin K;
+  art_alloc(M, A, w, &K);
+  art_fk(M, A, &K, q);
+  int nv = K.nv;
+  double* Mm = (double*)calloc((size_t)nv * nv + 1, sizeof(double));
+  double* cb = (double*)calloc(nv + 1, sizeof(double));
+  art_dynamics(M, A, &K, v0, Mm, cb);
+  sap_ctx S;
+  sap_alloc(&S, 1, nc);
+  /* sap_alloc sized its vectors for 6 dofs; rebuild them for nv */
+  free(S.vstar); free(S.Ad); free(S.H); free(S.g); free(S.dv); free(S.v); free(S.tmp);
+  S.N = nv; S.nb = 0; S.nc = nc;
+  S.vstar = (real*)calloc(nv + 1, sizeof(real)); S.Ad = (real*)calloc(nv + 1, sizeof(real));
+  S.H = (real*)calloc((size_t)nv * nv + 1, sizeof(real)); S.g = (real*)calloc(nv + 1, sizeof(real));
+  S.dv = (real*)calloc(nv + 1, sizeof(real)); S.v = (real*)calloc(nv + 1, sizeof(real));
+  S.tmp = (real*)calloc(4 * nv + 4, sizeof(real));
+  S.Adense = (real*)calloc((size_t)nv * nv + 1, sizeof(real));
+  S.Jd = (real*)calloc((size_t)3 * (nc > 0 ? nc : 1) * nv, sizeof(real));
+  double* rhs = (double*)calloc(nv + 1, sizeof(double));
+  for (int i = 0; i < nv; ++i) {
+    double acc = 0;
+    for (int k = 0; k < nv; ++k) acc += Mm[i * nv + k] * v0[k];
+    rhs[i] = acc;
+  }
+  for (int i = 0; i < nv * nv; ++i) S.Adense[i] = (real)Mm[i];
+  for (int jl = 0; jl < K.nj; ++jl) {
+    int j = K.j0 + jl, dj = A->joint_qd_start[j] - K.d0, qj = A->joint_q_start[j] - K.q0;
+    int nd = A->joint_type[j] == OR_JT_FREE ? 6 : 1;
+    for (int k = 0; k < nd; ++k) {
+      int gd = K.d0 + dj + k, i = dj + k;
+      double arm = A->armature[gd], ke = A->target_ke[gd], kd = A->target_kd[gd];
+      S.Adense[(size_t)i * nv + i] += (real)(arm + h * (kd + h * ke));
+      rhs[i] += arm * v0[i];
+      if (A->joint_type[j] == OR_JT_REVOLUTE) rhs[i] += h * ke * (A->target_pos[gd] - q[qj]);
+    }
+  }
+  for (int i = 0; i < nv; ++i) rhs[i] -= h * cb[i];
+  /* v* = A^-1 rhs */
+  real* L = (real*)calloc((size_t)nv * nv + 1, sizeof(real));
+  memcpy(L, S.Adense, sizeof(real) * nv * nv);
+  int spd = cholesky(L, nv);
+  for (int i = 0; i < nv; ++i) S.vstar[i] = (real)rhs[i];
+  if (spd) chol_solve(L, nv, S.vstar);
+  for (int i = 0; i < nv; ++i) S.Ad[i] = S.Adense[(size_t)i * nv + i];
+  /* contact rows and regularisation */
+  real tau_d = (real)C->tau_d;
+  real* y = (real*)calloc(nv + 1, sizeof(real));
+  for (int c = 0; c < nc; ++c) {
+    const or_contact* k = cs + c;
+    v3 n = V(k->n[0], k->n[1], k->n[2]);
+    v3 e = RA(n.x) < RA(n.y) ? (RA(n.x) < RA(n.z) ? V(1, 0, 0) : V(0, 0, 1)) : (RA(n.y) < RA(n.z) ? V(0, 1, 0) : V(0, 0, 1));
+    v3 t1 = nrm(cross(n, e)), t2 = cross(n, t1);
+    v3 F[3] = {t1, t2, n};
+    double p[3] = {k->c[0], k->c[1], k->c[2]};
+    real* Jc = S.Jd + (size_t)3 * c * nv;
+    for (int s = 0; s < 2; ++s) {
+      int gb = s == 0 ? k->body_a : k->body_b;
+      if (gb < 0) continue;
+      int b = gb - K.b0;
+      double sg = s == 0 ? -1 : 1;
+      for (int d = 0; d < nv; ++d) {
+        if (!art_on_path(&K, K.dof_body[d], b)) continue;
+        const double* sd = K.S + 6 * d;
+        /* point velocity of column d: v_O + w x p */
+        double pv[3] = {sd[3] + sd[1] * p[2] - sd[2] * p[1], sd[4] + sd[2] * p[0] - sd[0] * p[2],
+                        sd[5] + sd[0] * p[1] - sd[1] * p[0]};
+        for (int r = 0; r < 3; ++r)
+          Jc[(size_t)r * nv + d] += (real)(sg * (F[r].x * pv[0] + F[r].y * pv[1] + F[r].z * pv[2]));
+      }
+    }
+    /* w = tr(J A^-1 J^T) / 3 via L^-1 J^T */
+    real wsum = 0;
+    for (int r = 0; r < 3; ++r) {
+      for (int i = 0; i < nv; ++i) {
+        real s2 = Jc[(size_t)r * nv + i];
+        for (int kk = 0; kk < i; ++kk) s2 -= L[(size_t)i * nv + kk] * y[kk];
+        y[i] = s2 / L[(size_t)i * nv + i];
+        wsum += y[i] * y[i];
+      }
+    }
+    real wdiag = wsum / 3;
+    if (wdiag < (real)OR_ART_W_MIN) {
+      /* the contact point cannot move (e.g. on a joint axis of a fixed base): no constraint */
+      for (size_t i = 0; i < (size_t)3 * nv; ++i) Jc[i] = 0;
+      S.Rn[c] = 1; S.Rt[c] = 1; S.vhat[c] = 0; S.mu[c] = k->mu;
+      continue;
+    }
+    real Rn_soft = 1 / ((real)h * k->k * ((real)h + tau_d));
+    real Rn_rigid = (real)(C->beta * C->beta / (4.0 * OR_PI * OR_PI)) * wdiag;
+    S.Rn[c] = maxr(Rn_rigid, Rn_soft);
+    S.Rt[c] = (real)C->sigma * wdiag;
+    S.vhat[c] = -k->phi0 / ((real)h + tau_d);
+    S.mu[c] = k->mu;
+  }
+  real* v = S.v;
+  for (int i = 0; i < nv; ++i) v[i] = guess ? (real)guess[i] : (real)v0[i];
+  int ok = spd && sap_newton(&S, C, v, st);
+  if (!spd) { st->iterations = 0; st->factorizations = 0; st->converged = 0; }
+  double* vd = (double*)calloc(nv + 1, sizeof(double));
+  for (int i = 0; i < nv; ++i) vd[i] = v[i];
+  art_integrate(M, A, &K, q, vd, h, q_out);
+  memcpy(v_out, vd, sizeof(double) * nv);
+  free(vd); free(y); free(L); free(rhs); free(Mm); free(cb); free(S.Adense); free(S.Jd);
+  sap_free(&S);
+  art_free(&K);
+  return ok;
+}
+
+static int art_collide(const or_model* M, const or_art* A, const or_config* C, int w, const double* q,
+                       or_contact* cs) {
+  art_kin K;
+  art_alloc(M, A, w, &K);
+  art_fk(M, A, &K, q);
+  double* bq = (double*)calloc(7 * (K.nb ? K.nb : 1), sizeof(double));
+  art_body_q(&K, bq);
+  int np;
+  int nc = or_collide_world(M, w, bq, cs, C->max_contacts, &np);
+  free(bq);
+  art_free(&K);
+  return nc < C->max_contacts ? nc : C->max_contacts;
+}
+
+/* Exported: body poses [nb,7] of world w at its coordinates q [nq] */
+void or_art_fk(const or_model* M, const or_art* A, int w, const double* q, double* body_q) {
+  art_kin K;
+  art_alloc(M, A, w, &K);
+  art_fk(M, A, &K, q);
+  art_body_q(&K, body_q);
+  art_free(&K);
+}
+/* Exported: mass matrix [nv,nv] and bias [nv] of world w (tests) */
+void or_art_dynamics(const or_model* M, const or_art* A, int w, const double* q, const double* v, double* Mm,
+                     double* c) {
+  art_kin K;
+  art_alloc(M, A, w, &K);
+  art_fk(M, A, &K, q);
+  art_dynamics(M, A, &K, v, Mm, c);
+  art_free(&K);
+}
+/* Exported single inner step of world w: stats = (newton iterations, factorizations, converged, contacts) */
+int or_art_substep(const or_model* M, const or_art* A, const or_config* C, int w, const double* q, const double* v,
+                   double h, double* q_out, double* v_out, int* stats) {
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (C->max_contacts > 0 ? C->max_contacts : 1));
+  int nc = art_collide(M, A, C, w, q, cs);
+  solve_stats st;
+  int ok = art_inner_step(M, A, C, w, q, v, h, cs, nc, NULL, q_out, v_out, &st);
+  if (stats) { stats[0] = st.iterations; stats[1] = st.factorizations; stats[2] = st.converged; stats[3] = nc; }
+  free(cs);
+  return ok;
+}
+/* Exported step-doubling attempt of every articulated world from (q, v) with per-world dt:
+   full / doubled coordinates and velocities, FP64 error norms (1e10 sentinel), solve status. */
+void or_art_attempt_all(const or_model* M, const or_art* A, const or_config* C, const double* q, const double* v,
+                        const double* dt, double* q_full, double* v_full, double* q_dbl, double* v_dbl, double* err,
+                        int32_t* solve_ok, int32_t* n_contacts) {
+  int W = M->world_count;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int w = 0; w < W; ++w) {
+    int j0 = A->world_joint_start[w], j1 = A->world_joint_start[w + 1];
+    int q0 = A->joint_q_start[j0], nq = A->joint_q_start[j1] - q0;
+    int d0 = A->joint_qd_start[j0], nv = A->joint_qd_start[j1] - d0;
+    or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (C->max_contacts > 0 ? C->max_contacts : 1));
+    double* qm = (double*)calloc(nq + 1, sizeof(double));
+    double* vm = (double*)calloc(nv + 1, sizeof(double));
+    double* g = (double*)calloc(nv + 1, sizeof(double));
+    solve_stats st;
+    int ok = 1;
+    double h = dt[w];
+    int nc = art_collide(M, A, C, w, q + q0, cs);
+    n_contacts[w] = nc;
+    ok &= art_inner_step(M, A, C, w, q + q0, v + d0, h, cs, nc, NULL, q_full + q0, v_full + d0, &st);
+    for (int i = 0; i < nv; ++i) g[i] = 0.5 * (v[d0 + i] + v_full[d0 + i]);
+    ok &= art_inner_step(M, A, C, w, q + q0, v + d0, 0.5 * h, cs, nc, g, qm, vm, &st);
+    nc = art_collide(M, A, C, w, qm, cs);
+    ok &= art_inner_step(M, A, C, w, qm, vm, 0.5 * h, cs, nc, v_full + d0, q_dbl + q0, v_dbl + d0, &st);
+    double e = 0;
+    for (int i = 0; i < nq; ++i) {
+      double d = fabs(q_dbl[q0 + i] - q_full[q0 + i]);
+      if (isnan(d)) { e = NAN; break; }
+      if (d > e) e = d;
+    }
+    if (isnan(e) || isinf(e)) e = 1.0e10;
+    err[w] = e;
+    solve_ok[w] = ok;
+    free(cs); free(qm); free(vm); free(g);
+  }
 }
 
 /* ---------------------------------------------------------------- MuJoCo coordinate adapters */

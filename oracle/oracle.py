@@ -131,6 +131,85 @@ class OracleModel:
         self.world_body_start = k["world_body_start"]
 
 
+class OrArt(C.Structure):
+    _fields_ = [
+        ("joint_count", C.c_int), ("world_joint_start", i32p), ("joint_type", i32p), ("joint_parent", i32p),
+        ("joint_X_p", f64p), ("joint_X_c", f64p), ("joint_axis", f64p), ("joint_q_start", i32p),
+        ("joint_qd_start", i32p), ("target_pos", f64p), ("target_ke", f64p), ("target_kd", f64p),
+        ("armature", f64p),
+    ]
+
+
+class OracleArt:
+    """Articulation arrays of an articulated model (float32-rounded, as the device sees them)."""
+
+    def __init__(self, model):
+        h = model._host
+        f = lambda k: np.ascontiguousarray(np.asarray(h[k], dtype=np.float32).astype(np.float64))
+        i = lambda k: np.ascontiguousarray(np.asarray(h[k], dtype=np.int32))
+        self.keep = {k: i(k) for k in ("world_joint_start", "joint_type", "joint_parent", "joint_q_start",
+                                       "joint_qd_start")}
+        self.keep.update({k: f(k) for k in ("joint_X_p", "joint_X_c", "joint_axis")})
+        self.keep.update({"target_pos": f("joint_target_pos"), "target_ke": f("joint_target_ke"),
+                          "target_kd": f("joint_target_kd"), "armature": f("joint_armature")})
+        a = OrArt()
+        a.joint_count = int(len(h["joint_type"]))
+        for k in ("world_joint_start", "joint_type", "joint_parent", "joint_q_start", "joint_qd_start"):
+            setattr(a, k, _ptr(self.keep[k], i32p))
+        for k in ("joint_X_p", "joint_X_c", "joint_axis", "target_pos", "target_ke", "target_kd", "armature"):
+            setattr(a, k, _ptr(self.keep[k], f64p))
+        self.desc = a
+        self.q_start = self.keep["joint_q_start"]
+        self.qd_start = self.keep["joint_qd_start"]
+        self.world_q_start = np.asarray(h["world_q_start"]).tolist() + [int(h["joint_q_start"][-1])]
+        self.world_qd_start = np.asarray(h["world_qd_start"]).tolist() + [int(h["joint_qd_start"][-1])]
+
+
+def art_fk(om, oa, w, q):
+    """Body poses [nb, 7] of articulated world ``w`` at its coordinates ``q`` [nq]."""
+    nb = int(om.world_body_start[w + 1] - om.world_body_start[w])
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros((nb, 7))
+    lib().or_art_fk(C.byref(om.desc), C.byref(oa.desc), w, _ptr(q, f64p), _ptr(out, f64p))
+    return out
+
+
+def art_dynamics(om, oa, w, q, v):
+    """Mass matrix [nv, nv] and bias force [nv] (velocity products and gravity) of world ``w``."""
+    nv = len(v)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    M = np.zeros((nv, nv)); c = np.zeros(nv)
+    lib().or_art_dynamics(C.byref(om.desc), C.byref(oa.desc), w, _ptr(q, f64p), _ptr(v, f64p), _ptr(M, f64p),
+                          _ptr(c, f64p))
+    return M, c
+
+
+def art_substep(om, oa, cfg, w, q, v, h):
+    """One inner step of articulated world ``w``: (q_out, v_out, stats, ok)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    qo, vo = np.zeros_like(q), np.zeros_like(v)
+    stats = np.zeros(4, dtype=np.int32)
+    ok = lib().or_art_substep(C.byref(om.desc), C.byref(oa.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(v, f64p),
+                              C.c_double(h), _ptr(qo, f64p), _ptr(vo, f64p), _ptr(stats, i32p))
+    return qo, vo, stats, ok
+
+
+def art_attempt_all(om, oa, cfg, q, v, dt):
+    """Step-doubling attempt of every articulated world (all coordinates of the model)."""
+    q = np.ascontiguousarray(q, dtype=np.float64); v = np.ascontiguousarray(v, dtype=np.float64)
+    dt = np.ascontiguousarray(dt, dtype=np.float64)
+    W = om.world_count
+    qf, qb = np.zeros_like(q), np.zeros_like(q)
+    vf, vb = np.zeros_like(v), np.zeros_like(v)
+    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32)
+    lib().or_art_attempt_all(C.byref(om.desc), C.byref(oa.desc), C.byref(cfg), _ptr(q, f64p), _ptr(v, f64p),
+                             _ptr(dt, f64p), _ptr(qf, f64p), _ptr(vf, f64p), _ptr(qb, f64p), _ptr(vb, f64p),
+                             _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p))
+    return dict(q_full=qf, v_full=vf, q_double=qb, v_double=vb, err=err, solve_ok=ok, n_contacts=nc)
+
+
 def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
                 max_substeps=16, max_iterations=30, max_contacts=128, opt_rel_tol=None, opt_abs_tol=1e-14,
                 tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10) -> OrConfig:
