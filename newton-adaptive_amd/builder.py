@@ -399,13 +399,28 @@ class ModelBuilder:
             if mass[b] > 0.0:
                 inv_inertia[b] = np.linalg.inv(inertia[b])
 
-        # filter pairs: shapes on the same body never collide with each other
+        # filter pairs: shapes on the same body never collide with each other, and the static
+        # shapes (body -1) count as one body (builder.py:6204-6208 with body_shapes = {-1: []}).
+        # Static shapes of two different worlds can never pass the world test, so their pairs are
+        # left out of the list: same-world and world-global static pairs are what remains.
         filt = []
-        for b in range(B):
-            ss = np.where(sb == b)[0]
+
+        def _all_pairs(ss):
             for i in range(len(ss)):
                 for j in range(i + 1, len(ss)):
-                    filt.append((int(ss[i]), int(ss[j])))
+                    filt.append((int(min(ss[i], ss[j])), int(max(ss[i], ss[j]))))
+
+        for b in range(B):
+            _all_pairs(np.where(sb == b)[0])
+        static = np.where(sb < 0)[0]
+        glob = static[sw[static] < 0]
+        _all_pairs(glob)
+        for w in np.unique(sw[static][sw[static] >= 0]):
+            local_static = static[sw[static] == w]
+            _all_pairs(local_static)
+            for s in local_static:
+                for g in glob:
+                    filt.append((int(min(s, g)), int(max(s, g))))
         filt = np.asarray(sorted(filt), dtype=np.int32).reshape(-1, 2)
 
         cfg = self.shape_cfg
