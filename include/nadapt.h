@@ -68,6 +68,22 @@ typedef struct na_model_desc {
   int32_t explicit_pair_count;
   const int32_t* explicit_pairs;
   const int32_t* world_pair_start;
+  /* Optional articulation (joint_type NULL: every body is a free body).  One joint per body:
+     joint j moves body j, parents come before children within one world, a world holds at most
+     16 bodies, 32 coordinates and 32 dofs.  Newton's joint_q / joint_qd layout
+     (state.py:149-153): REVOLUTE (type 1) one angle / rate; FREE (type 4, root only, parent -1)
+     origin position + quaternion xyzw / COM linear velocity + angular velocity, world frame. */
+  const int32_t* joint_type;       /* [B] */
+  const int32_t* joint_parent;     /* [B] global parent body, -1 world */
+  const int32_t* joint_q_start;    /* [B+1] */
+  const int32_t* joint_qd_start;   /* [B+1] */
+  const float* joint_X_p;          /* [B,7] joint frame in the parent body frame */
+  const float* joint_X_c;          /* [B,7] joint frame in the child body frame */
+  const float* joint_axis;         /* [B,3] revolute axis in the joint frame */
+  const float* joint_target_pos;   /* [D] PD drive targets, gains and armature per dof */
+  const float* joint_target_ke;
+  const float* joint_target_kd;
+  const float* joint_armature;
 } na_model_desc;
 
 /* Solver settings (constructor kwargs of the reference solvers). */
@@ -211,6 +227,26 @@ int na_substep(const na_solver* s, const double* q64, const float* qd, const flo
 int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
                double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
                int32_t* n_contacts, void* stream);
+
+/* Articulated models (model created with joint arrays).  The boundary call on joint coordinates:
+   every world advances by dt_outer with step doubling over its joint_q / joint_qd (error =
+   max-norm of the coordinate difference, _inf_norm_state_error_kernel, solver_sap_adaptive.py:
+   154-185; attempt sequence _substep_body, :732-849); replaces SolverSAPAdaptive.step (:962) for
+   models with joints.  joint_q / joint_qd are updated in place (an FP64 coordinate shadow is kept
+   while it still rounds to joint_q); body_q / body_qd (may be NULL) receive the bodies' poses and
+   COM / angular velocities at the new coordinates (eval_fk). */
+int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd, float* body_q, float* body_qd,
+                        double dt_outer, void* stream);
+/* One inner step per articulated world (contacts at q, convex solve at h[w], FP64 integration);
+   the parity form of substep (solver_sap_adaptive.py:714-729).  stats [W,4] int32: Newton
+   iterations, factorizations, converged, contacts. */
+int na_art_substep(const na_solver* s, const double* joint_q, const double* joint_qd, const double* h,
+                   double* q_out, double* qd_out, int32_t* stats, void* stream);
+/* One step-doubling attempt per articulated world from (joint_q, joint_qd) with dt[w]: full and
+   doubled coordinates / velocities, FP64 error norm, solve status, contacts at q. */
+int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
+                   double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok,
+                   int32_t* n_contacts, void* stream);
 
 /* Controller kernels with the reference float32 arithmetic (one thread per world). */
 int na_ctrl_calc_adjusted_step_f32(int32_t n, const float* err, const float* dt, float* ideal_dt, uint8_t* accepted,

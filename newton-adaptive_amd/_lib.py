@@ -37,6 +37,9 @@ class NaModelDesc(C.Structure):
         ("gravity", C.c_float * 3),
         ("world_body_end", vp), ("world_shape_end", vp),
         ("explicit_pair_count", i32), ("explicit_pairs", vp), ("world_pair_start", vp),
+        ("joint_type", vp), ("joint_parent", vp), ("joint_q_start", vp), ("joint_qd_start", vp),
+        ("joint_X_p", vp), ("joint_X_c", vp), ("joint_axis", vp), ("joint_target_pos", vp),
+        ("joint_target_ke", vp), ("joint_target_kd", vp), ("joint_armature", vp),
     ]
 
 
@@ -83,6 +86,9 @@ SIGNATURES = {
                                     vp, vp]),
     "na_coords_from_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
+    "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, vp]),
+    "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "na_art_attempt": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),

@@ -43,6 +43,18 @@ struct NaModel {
   const int32_t* pair_list;      // [P,2] canonical (min, max) global shape indices, grouped by world
   const int32_t* world_pair_start;  // [W+1] range of each world's pairs in pair_list
   float gx, gy, gz;
+  // articulated worlds (null for free bodies): joint j moves body j, parents before children
+  const int32_t* joint_type;      // [B] 1 revolute, 4 FREE (root only)
+  const int32_t* joint_parent;    // [B] global parent body, -1 world
+  const int32_t* joint_q_start;   // [B+1]
+  const int32_t* joint_qd_start;  // [B+1]
+  const float* joint_X_p;         // [B,7] joint frame in the parent frame
+  const float* joint_X_c;         // [B,7] joint frame in the child frame
+  const float* joint_axis;        // [B,3] revolute axis in the joint frame
+  const float* joint_target_pos;  // [D] per dof
+  const float* joint_target_ke;
+  const float* joint_target_kd;
+  const float* joint_armature;
 };
 
 // Per-wave LDS scratch of the segment (world shapes + global shapes).
@@ -58,19 +70,21 @@ struct SegLDS {
   uint32_t pairs[NA_MAX_PAIRS];  // (seg_a << 8) | seg_b
 };
 
-// Per-wave LDS contact set, consumed by the convex solve.
-struct ContactLDS {
-  int8_t ba[NA_MAX_CONTACTS], bb[NA_MAX_CONTACTS];  // local body index or -1 (static)
-  uint8_t sa[NA_MAX_CONTACTS], sb[NA_MAX_CONTACTS];  // segment positions (A, B after type ordering)
-  uint8_t sub[NA_MAX_CONTACTS];
-  float n[3][NA_MAX_CONTACTS];   // unit normal A -> B
-  float p[3][NA_MAX_CONTACTS];   // midpoint between the surfaces (world)
-  float dist[NA_MAX_CONTACTS];   // surface distance (no margins)
-  float phi0[NA_MAX_CONTACTS];   // dist - margin_a - margin_b
-  float k[NA_MAX_CONTACTS];
-  float mu[NA_MAX_CONTACTS];
-  float ra[NA_MAX_CONTACTS], rb[NA_MAX_CONTACTS];  // effective radii (writer core points)
+// Per-wave LDS contact set of capacity N, consumed by the convex solve.
+template <int N>
+struct ContactSet {
+  int8_t ba[N], bb[N];    // local body index or -1 (static)
+  uint8_t sa[N], sb[N];   // segment positions (A, B after type ordering)
+  uint8_t sub[N];
+  float n[3][N];          // unit normal A -> B
+  float p[3][N];          // midpoint between the surfaces (world)
+  float dist[N];          // surface distance (no margins)
+  float phi0[N];          // dist - margin_a - margin_b
+  float k[N];
+  float mu[N];
+  float ra[N], rb[N];     // effective radii (writer core points)
 };
+using ContactLDS = ContactSet<NA_MAX_CONTACTS>;
 
 // ---------------------------------------------------------------- primitives
 // up to 8 contacts per pair: box-box keeps its whole clipped face polygon
@@ -723,7 +737,9 @@ __device__ void narrow_pair(const NaModel& M, const SegLDS& S, int i, int j, Pai
 
 // Stage 3: narrow phase over the pair list, compacted into the LDS contact set in pair order.
 // Returns the number of contacts generated (may exceed `cap`; extra contacts are dropped).
-__device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const SegLDS& S, ContactLDS& C) {
+// `cap` must not exceed the capacity of the contact set.
+template <class CS>
+__device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const SegLDS& S, CS& C) {
   int nc = 0;
   np = min(np, NA_MAX_PAIRS);
   for (int base = 0; base < np; base += NA_WAVE) {

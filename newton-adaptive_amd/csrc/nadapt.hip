@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "../../include/nadapt.h"
-#include "na_solve.h"
+#include "na_art.h"
 
 // ---------------------------------------------------------------- errors
 static thread_local std::string g_err;
@@ -38,6 +38,7 @@ extern "C" const char* na_last_error_string(void) { return g_err.c_str(); }
 struct na_model {
   NaModel M;
   int max_nb, max_seg, max_nl;
+  bool articulated;
 };
 
 struct Ctrl {
@@ -449,6 +450,40 @@ __device__ __forceinline__ void open_frame(const NaConfig& cfg, WCtrl& c, double
   c.overflow = 0;
 }
 
+// Controller after an attempt of size `step` with error `err` (ok false: the convex solve
+// failed): records the error, advances time and dt, and returns whether to commit the attempt;
+// `inc` is set to 1 for an accepted step of positive size.
+__device__ __forceinline__ bool controller_update(const NaConfig& cfg, WCtrl& c, double err, bool ok, double step,
+                                                  double eff_max, int& inc) {
+  if (!ok) {
+    err = cfg.divergence_threshold;
+    c.solve_fail = 1;
+  }
+  c.last_error = err;
+  bool commit;
+  inc = 0;
+  if (cfg.controller == 0) {
+    commit = adapt_dt<double>(err, c.sim_time, c.next_time, c.dt, c.dt_half, c.ideal_dt, c.diverged, c.acc_error,
+                              c.substeps, inc, cfg.mode, cfg.tol, cfg.dt_min, eff_max, cfg.divergence_threshold);
+    c.accepted = commit;
+  } else {
+    calc_adjusted_step<double>(err, c.dt, c.ideal_dt, c.accepted, c.commit, c.diverged, cfg.tol, cfg.dt_min,
+                               cfg.divergence_threshold);
+    commit = c.commit;
+    if (c.accepted) {
+      c.sim_time += c.dt;
+      c.acc_error = err;
+      if (step > 0.0) { c.substeps += 1; inc = 1; }
+    }
+    if (c.diverged) c.sim_time = c.next_time;
+    double d = c.ideal_dt;
+    d = d < cfg.dt_min ? cfg.dt_min : (d > eff_max ? eff_max : d);
+    c.dt = d;
+    c.dt_half = 0.5 * d;
+  }
+  return commit;
+}
+
 // One iteration of the boundary loop for one world (clamp, attempt, controller, commit).
 // q/v hold this lane's body state and are updated in place.  Returns accepted-step count (0/1).
 __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, WCtrl& c,
@@ -472,32 +507,8 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
     for (int k = 0; k < 6; ++k) { vf[k] = v[k]; vd[k] = v[k]; }
   }
   if (ovf) c.overflow = 1;
-  if (!ok) {
-    err = cfg.divergence_threshold;
-    c.solve_fail = 1;
-  }
-  c.last_error = err;
-  bool commit;
   int inc = 0;
-  if (cfg.controller == 0) {
-    commit = adapt_dt<double>(err, c.sim_time, c.next_time, c.dt, c.dt_half, c.ideal_dt, c.diverged, c.acc_error,
-                              c.substeps, inc, cfg.mode, cfg.tol, cfg.dt_min, eff_max, cfg.divergence_threshold);
-    c.accepted = commit;
-  } else {
-    calc_adjusted_step<double>(err, c.dt, c.ideal_dt, c.accepted, c.commit, c.diverged, cfg.tol, cfg.dt_min,
-                               cfg.divergence_threshold);
-    commit = c.commit;
-    if (c.accepted) {
-      c.sim_time += c.dt;
-      c.acc_error = err;
-      if (step > 0.0) { c.substeps += 1; inc = 1; }
-    }
-    if (c.diverged) c.sim_time = c.next_time;
-    double d = c.ideal_dt;
-    d = d < cfg.dt_min ? cfg.dt_min : (d > eff_max ? eff_max : d);
-    c.dt = d;
-    c.dt_half = 0.5 * d;
-  }
+  const bool commit = controller_update(cfg, c, err, ok, step, eff_max, inc);
   if (commit) {
     const double* sq = cfg.mode == 0 ? qf : qd;
     const float* sv = cfg.mode == 0 ? vf : vd;
@@ -508,6 +519,7 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
   }
   return inc;
 }
+
 
 // Load this lane's body (FP64 shadow kept when it still rounds to the caller's f32 pose).
 __device__ __forceinline__ void load_body(const WorldCtx& wc, const double* q64, const float* body_q,
@@ -1047,6 +1059,164 @@ __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const d
   }
 }
 
+// ---------------------------------------------------------------- articulated worlds
+// One iteration of the boundary loop for an articulated world (clamp, attempt, controller,
+// commit); q / v are this lane's coordinate and dof velocity (FP64), updated in place.
+__device__ int art_iteration(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, WCtrl& c,
+                             double& q, double& v, double eff_max, long long* wk) {
+  c.iterations += 1;
+  double rem = c.next_time - c.sim_time;
+  if (rem <= 0.0) { c.dt = 0.0; c.dt_half = 0.0; }
+  else if (c.dt > rem) { c.dt = rem; c.dt_half = 0.5 * rem; }
+  const double step = c.dt;
+  double qf = q, vf = v, qd = q, vd = v;
+  bool ok = true;
+  int ovf = 0, nc = 0;
+  double err = 0.0;
+  if (step > 0.0 && aw.nb > 0) err = art_attempt(M, cfg, aw, A, q, v, step, qf, vf, qd, vd, ok, ovf, nc, wk);
+  if (ovf) c.overflow = 1;
+  int inc = 0;
+  if (controller_update(cfg, c, err, ok, step, eff_max, inc)) {
+    q = cfg.mode == 0 ? qf : qd;
+    v = cfg.mode == 0 ? vf : vd;
+  }
+  return inc;
+}
+
+// Body poses and COM / angular velocities of the world at coordinates q, velocities v (eval_fk).
+__device__ void art_store_bodies(const NaModel& M, const ArtWorld& aw, ArtLDS& A, double q, double v, float* body_q,
+                                 float* body_qd) {
+  const int l = lane_id();
+  art_kinematics(M, aw, A, q);
+  if (l < aw.nv) A.x[l] = v;
+  wave_sync();
+  art_body_velocity(aw, A);
+  if (l < aw.nb) {
+    const int gb = aw.b0 + l;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) body_q[7 * gb + k] = A.bqf[k][l];
+    if (body_qd) {
+      double w[3] = {A.V[l][0], A.V[l][1], A.V[l][2]}, c[3] = {A.cb[l][0], A.cb[l][1], A.cb[l][2]}, wc[3];
+      dcross3(w, c, wc);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        body_qd[6 * gb + k] = (float)(A.V[l][3 + k] + wc[k]);
+        body_qd[6 * gb + 3 + k] = (float)w[k];
+      }
+    }
+  }
+}
+
+// Served boundary call of articulated worlds: persistent waves over the world queue, each world
+// marched to its boundary on the device with FP64 coordinates (shadow q64 kept while it rounds to
+// the caller's joint_q) and velocities.
+__global__ __launch_bounds__(64) void k_art_frame(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* joint_q,
+                                                  float* joint_qd, float* body_q, float* body_qd, double dt_outer,
+                                                  double eff_max, int32_t* queue, const int32_t* order) {
+  __shared__ ArtLDS A;
+  const int l = lane_id();
+  for (;;) {
+    int slot = 0;
+    if (l == 0) slot = atomicAdd(queue, 1);
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= M.W) break;
+    const int w = order ? order[slot] : slot;
+    ArtWorld aw = art_world(M, w);
+    if (aw.nb == 0) continue;
+    art_load(M, aw, A);
+    double q = 0.0, v = 0.0;
+    int same = 1;
+    if (l < aw.nq) {
+      q = q64[aw.q0 + l];
+      same = (float)q == joint_q[aw.q0 + l] ? 1 : 0;
+    }
+    same = wave_min_i(same);
+    if (!same && l < aw.nq) q = (double)joint_q[aw.q0 + l];
+    if (l < aw.nv) v = (double)joint_qd[aw.d0 + l];
+    WCtrl c = load_ctrl(K, w);
+    open_frame(cfg, c, dt_outer, eff_max);
+    long long wk[NA_WORK] = {};
+    int acc = 0;
+    for (int it = 0; it < cfg.max_substeps; ++it) {
+      if (!(c.sim_time < c.next_time)) break;
+      acc += art_iteration(M, cfg, aw, A, c, q, v, eff_max, wk);
+      if (c.solve_fail) break;
+    }
+    if (l < aw.nq) {
+      q64[aw.q0 + l] = q;
+      joint_q[aw.q0 + l] = (float)q;
+    }
+    if (l < aw.nv) joint_qd[aw.d0 + l] = (float)v;
+    if (body_q) art_store_bodies(M, aw, A, q, v, body_q, body_qd);
+    if (l == 0) {
+      store_ctrl(K, w, c);
+      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    }
+    flush_work(K, w, wk);
+  }
+}
+
+// Parity entry kernels: one inner step / one step-doubling attempt per articulated world.
+__global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
+                                                    const double* h, double* q_out, double* qd_out, int32_t* stats) {
+  __shared__ ArtLDS A;
+  const int w = blockIdx.x;
+  const int l = lane_id();
+  ArtWorld aw = art_world(M, w);
+  if (aw.nb == 0) {
+    if (l < 4) stats[4 * w + l] = l == 2 ? 1 : 0;
+    return;
+  }
+  art_load(M, aw, A);
+  const double q = l < aw.nq ? q64[aw.q0 + l] : 0.0;
+  const double v = l < aw.nv ? qd64[aw.d0 + l] : 0.0;
+  long long wk[NA_WORK] = {};
+  int ovf = 0;
+  double mdiag;
+  art_kinematics(M, aw, A, q);
+  const double cb = art_dynamics(M, aw, A, v, mdiag);
+  const int nc = art_collide(M, cfg, aw, A, ovf, wk);
+  ArtStats st;
+  double vo, qo;
+  const bool ok = art_step(cfg, aw, A, nc, h[w], v, cb, mdiag, v, vo, qo, st);
+  if (l < aw.nq) q_out[aw.q0 + l] = qo;
+  if (l < aw.nv) qd_out[aw.d0 + l] = vo;
+  if (l == 0) {
+    stats[4 * w + 0] = st.iterations;
+    stats[4 * w + 1] = st.factorizations;
+    stats[4 * w + 2] = ok ? 1 : 0;
+    stats[4 * w + 3] = nc;
+  }
+}
+__global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
+                                                    const double* dt, double* q_full, double* qd_full, double* q_dbl,
+                                                    double* qd_dbl, double* err, int32_t* solve_ok,
+                                                    int32_t* n_contacts) {
+  __shared__ ArtLDS A;
+  const int w = blockIdx.x;
+  const int l = lane_id();
+  ArtWorld aw = art_world(M, w);
+  if (aw.nb == 0) {
+    if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
+    return;
+  }
+  art_load(M, aw, A);
+  const double q = l < aw.nq ? q64[aw.q0 + l] : 0.0;
+  const double v = l < aw.nv ? qd64[aw.d0 + l] : 0.0;
+  long long wk[NA_WORK] = {};
+  bool ok = true;
+  int ovf = 0, nc = 0;
+  double qf, vf, qd, vd;
+  const double e = art_attempt(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk);
+  if (l < aw.nq) { q_full[aw.q0 + l] = qf; q_dbl[aw.q0 + l] = qd; }
+  if (l < aw.nv) { qd_full[aw.d0 + l] = vf; qd_dbl[aw.d0 + l] = vd; }
+  if (l == 0) {
+    err[w] = e;
+    solve_ok[w] = ok ? 1 : 0;
+    n_contacts[w] = nc;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 static inline hipStream_t S_(void* s) { return (hipStream_t)s; }
 static inline int blocks(long long n, int t) { return (int)((n + t - 1) / t); }
@@ -1063,6 +1233,43 @@ extern "C" int na_device_info(char* name, int name_len, int* cu_count, int* lds_
   if (cu_count) *cu_count = p.multiProcessorCount;
   if (lds_per_block) *lds_per_block = (int)sizeof(WorldLDS);
   return 0;
+}
+
+// Articulation rules of the wave-per-world kernels: one joint per body in body order, revolute or
+// FREE (root only), parents first and inside the world, at most NA_ART_NB bodies, NA_ART_NQ
+// coordinates and NA_ART_ND dofs per world.  Returns an error message or "".
+static std::string validate_articulation(const NaModel& M, const std::vector<int>& wbs, const std::vector<int>& wbe) {
+  if (!M.joint_parent || !M.joint_q_start || !M.joint_qd_start || !M.joint_X_p || !M.joint_X_c || !M.joint_axis ||
+      !M.joint_target_pos || !M.joint_target_ke || !M.joint_target_kd || !M.joint_armature)
+    return "na_model_create: joint_type given without every other joint array";
+  const int B = M.B;
+  std::vector<int> jt(B > 0 ? B : 1), jp(B > 0 ? B : 1), qs(B + 1), ds(B + 1);
+  if (B > 0 && (hipMemcpy(jt.data(), M.joint_type, sizeof(int) * B, hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(jp.data(), M.joint_parent, sizeof(int) * B, hipMemcpyDeviceToHost) != hipSuccess))
+    return "na_model_create: could not read the joint arrays (device pointers?)";
+  if (hipMemcpy(qs.data(), M.joint_q_start, sizeof(int) * (B + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(ds.data(), M.joint_qd_start, sizeof(int) * (B + 1), hipMemcpyDeviceToHost) != hipSuccess)
+    return "na_model_create: could not read joint_q_start / joint_qd_start";
+  for (int w = 0; w < M.W; ++w) {
+    const int b0 = wbs[w], b1 = wbe[w];
+    if (b1 - b0 > NA_ART_NB) return "na_model_create: an articulated world has more than 16 bodies";
+    if (qs[b1] - qs[b0] > NA_ART_NQ) return "na_model_create: an articulated world has more than 32 coordinates";
+    if (ds[b1] - ds[b0] > NA_ART_ND) return "na_model_create: an articulated world has more than 32 dofs";
+    for (int b = b0; b < b1; ++b) {
+      const int t = jt[b], p = jp[b];
+      const int nq = qs[b + 1] - qs[b], nd = ds[b + 1] - ds[b];
+      if (t == NA_JT_FREE) {
+        if (p != -1) return "na_model_create: a FREE joint must have the world as parent";
+        if (nq != 7 || nd != 6) return "na_model_create: a FREE joint has 7 coordinates and 6 dofs";
+      } else if (t == NA_JT_REVOLUTE) {
+        if (nq != 1 || nd != 1) return "na_model_create: a revolute joint has 1 coordinate and 1 dof";
+        if (p != -1 && (p < b0 || p >= b)) return "na_model_create: a joint parent must precede its child in its world";
+      } else {
+        return "na_model_create: joint types must be 1 (revolute) or 4 (FREE)";
+      }
+    }
+  }
+  return std::string();
 }
 
 extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
@@ -1106,6 +1313,15 @@ extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
     m->max_nl = wse[w] - wss[w] > m->max_nl ? wse[w] - wss[w] : m->max_nl;
   }
   m->max_seg = m->max_nl + M.G;
+  m->articulated = d->joint_type != nullptr;
+  M.joint_type = d->joint_type; M.joint_parent = d->joint_parent; M.joint_q_start = d->joint_q_start;
+  M.joint_qd_start = d->joint_qd_start; M.joint_X_p = d->joint_X_p; M.joint_X_c = d->joint_X_c;
+  M.joint_axis = d->joint_axis; M.joint_target_pos = d->joint_target_pos; M.joint_target_ke = d->joint_target_ke;
+  M.joint_target_kd = d->joint_target_kd; M.joint_armature = d->joint_armature;
+  if (m->articulated) {
+    err = validate_articulation(M, wbs, wbe);
+    if (!err.empty()) { delete m; return fail(err.c_str()); }
+  }
   if (m->max_nb > NA_MAX_BODIES) { delete m; return fail("na_model_create: a world has more than 64 bodies"); }
   if (m->max_seg > NA_MAX_SEG) {
     delete m;
@@ -1238,13 +1454,13 @@ extern "C" int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_
   return 0;
 }
 
-static int default_grid(const na_solver* s) {
+static int default_grid(const na_solver* s, const void* kernel = (const void*)k_frame) {
   int W = s->model->M.W;
   if (s->grid_waves > 0) return s->grid_waves < W ? s->grid_waves : W;
   int dev = 0, cus = 256, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_frame, 64, 0) != hipSuccess || per_cu < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64, 0) != hipSuccess || per_cu < 1)
     per_cu = 1;
   int g = cus * per_cu;
   return g < W ? g : W;
@@ -1418,6 +1634,44 @@ extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd
   const NaModel& M = s->model->M;
   k_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, dt, q_full, qd_full, q_dbl, qd_dbl, err, solve_ok,
                                         n_contacts);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- articulated entry points
+extern "C" int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd, float* body_q, float* body_qd,
+                                   double dt_outer, void* stream) {
+  if (!s || !joint_q || !joint_qd) return fail("na_step_articulated: null argument");
+  if (!s->model->articulated) return fail("na_step_articulated: the model has no joints");
+  if (!(dt_outer > 0)) return fail("na_step_articulated: dt (dt_outer) must be > 0");
+  const NaModel& M = s->model->M;
+  const double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
+  k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
+  const int grid = default_grid(s, (const void*)k_art_frame);
+  k_art_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, joint_q, joint_qd, body_q, body_qd, dt_outer, eff,
+                                           s->queue, s->order);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_art_substep(const na_solver* s, const double* joint_q, const double* joint_qd, const double* h,
+                              double* q_out, double* qd_out, int32_t* stats, void* stream) {
+  if (!s || !joint_q || !joint_qd || !h || !q_out || !qd_out || !stats) return fail("na_art_substep: null argument");
+  if (!s->model->articulated) return fail("na_art_substep: the model has no joints");
+  const NaModel& M = s->model->M;
+  k_art_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, h, q_out, qd_out, stats);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
+                              double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
+                              int32_t* solve_ok, int32_t* n_contacts, void* stream) {
+  if (!s || !joint_q || !joint_qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok ||
+      !n_contacts)
+    return fail("na_art_attempt: null argument");
+  if (!s->model->articulated) return fail("na_art_attempt: the model has no joints");
+  const NaModel& M = s->model->M;
+  k_art_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, dt, q_full, qd_full, q_dbl, qd_dbl, err,
+                                            solve_ok, n_contacts);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -71,6 +71,9 @@ class State:
         self.body_q.copy_(other.body_q)
         self.body_qd.copy_(other.body_qd)
         self.body_f.copy_(other.body_f)
+        if self.joint_q.data_ptr() != self.body_q.data_ptr():  # articulated: coordinates of their own
+            self.joint_q.copy_(other.joint_q)
+            self.joint_qd.copy_(other.joint_qd)
 
     def clear_forces(self) -> None:
         self.body_f.zero_()

@@ -68,6 +68,11 @@ class ModelHandle:
                 d.explicit_pairs = p(torch.as_tensor(pairs, device=model.device))
                 d.world_pair_start = p(torch.as_tensor(starts, device=model.device))
                 self.with_pairs = True
+        # articulation (one joint per body); free-body models leave joint_type NULL
+        if getattr(model, "is_articulated", False):
+            for name in ("joint_type", "joint_parent", "joint_q_start", "joint_qd_start", "joint_X_p", "joint_X_c",
+                         "joint_axis", "joint_target_pos", "joint_target_ke", "joint_target_kd", "joint_armature"):
+                setattr(d, name, p(getattr(model, name)) or dummy_i.data_ptr())
         self.desc = d
         h = C.c_void_p()
         _lib.check(L.na_model_create(C.byref(d), C.byref(h)), "na_model_create")

@@ -174,6 +174,16 @@ class _SolverBase:
     def integrate(self, state, control, dt_outer: float, stream=None):
         s = _lib.stream_handle(stream)
         bf = state.body_f if getattr(state, "body_f", None) is not None else None
+        if self.model.is_articulated:
+            # generalized coordinates: joint_q / joint_qd advance, body_q / body_qd follow (eval_fk)
+            _lib.check(_lib.lib().na_step_articulated(self._h, state.joint_q.data_ptr(), state.joint_qd.data_ptr(),
+                                                      state.body_q.data_ptr(), state.body_qd.data_ptr(),
+                                                      float(dt_outer), s), "na_step_articulated")
+            if self._world_order == "cost":
+                self._order_by_cost()
+            if int(self._solve_fail.max().item()) != 0:
+                self._raise_solve_failure()
+            return state
         if self._schedule == "served":
             st = _lib.check(_lib.lib().na_step(self._h, state.body_q.data_ptr(), state.body_qd.data_ptr(),
                                                bf.data_ptr() if bf is not None else None, float(dt_outer), s),

@@ -1393,6 +1393,70 @@ void or_attempt_all(const or_model* M, const or_config* C, const double* q, cons
  * Per-world marching is exactly the lock-step loop of the reference (done worlds are no-ops).
  * q/qd are updated in place.  Returns 0 ok, 2 if any inner solve failed (RuntimeError in
  * the reference, solver_sap_adaptive.py:900-905). */
+/* Controller after one attempt of world w (size `step`, error `err`, ok = every solve converged):
+   _apply_solve_convergence_to_error + _adapt_dt (solver_sap_adaptive.py:780-808) or
+   _calc_adjusted_step + the MuJoCo commit rules (solver_mujoco_adaptive.py:835-905).  Returns
+   whether to commit; adds accepted steps to *cum. */
+static int ctrl_after_attempt(const or_config* C, or_ctrl* K, int w, double err, int ok, double step,
+                              double eff_dt_max, int64_t* cum) {
+  if (!ok) { err = C->divergence_threshold; K->solve_fail[w] = 1; }
+  K->last_error[w] = err;
+  int commit;
+  if (C->controller == 0) {
+    int32_t cum1 = 0;
+    or_adapt_dt_f64(1, &K->last_error[w], &K->sim_time[w], &K->next_time[w], &K->dt[w], &K->dt_half[w],
+                    &K->ideal_dt[w], &K->diverged[w], &K->accepted[w], &K->accepted_error[w], &K->substeps[w],
+                    &cum1, C->mode, C->tol, C->dt_min, eff_dt_max, C->divergence_threshold);
+    *cum += cum1;
+    commit = K->accepted[w];
+  } else {
+    or_calc_adjusted_step_f64(1, &K->last_error[w], &K->dt[w], &K->ideal_dt[w], &K->accepted[w], &K->commit[w],
+                              &K->diverged[w], C->tol, C->dt_min, C->divergence_threshold);
+    commit = K->commit[w];
+    if (K->accepted[w]) {
+      K->sim_time[w] += K->dt[w];
+      K->accepted_error[w] = err;
+      if (step > 0) { K->substeps[w] += 1; *cum += 1; }
+    }
+    if (K->diverged[w]) K->sim_time[w] = K->next_time[w];
+    double d = K->ideal_dt[w];
+    d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+    K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+  }
+  return commit;
+}
+
+/* Frame opening of world w: _open_frame + _seed_dt (solver_sap_adaptive.py:89-132) or _apply_dt_cap,
+   _rebase_time, _boundary_advance (solver_mujoco_adaptive.py:805-833). */
+static void ctrl_open_frame(const or_config* C, or_ctrl* K, int w, double dt_outer, double eff_dt_max) {
+  K->iterations[w] = 0;
+  if (C->controller == 0) {
+    double base = K->next_time[w];
+    K->sim_time[w] = K->sim_time[w] - base;
+    K->next_time[w] = dt_outer;
+    double d = C->mode == 0 ? C->dt_init : K->ideal_dt[w];
+    d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+    K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+    K->substeps[w] = 0; K->diverged[w] = 0;
+  } else {
+    double d = K->ideal_dt[w];
+    d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
+    K->dt[w] = d; K->dt_half[w] = 0.5 * d;
+    double base = K->next_time[w];
+    K->sim_time[w] -= base; K->next_time[w] -= base;
+    K->next_time[w] += dt_outer;
+    K->diverged[w] = 0;
+  }
+}
+
+/* _clamp_dt_to_boundary for world w; returns the attempt size */
+static double ctrl_clamp(or_ctrl* K, int w) {
+  double rem = K->next_time[w] - K->sim_time[w];
+  if (rem <= 0) { K->dt[w] = 0; K->dt_half[w] = 0; }
+  else if (K->dt[w] > rem) { K->dt[w] = rem; K->dt_half[w] = 0.5 * rem; }
+  return K->dt[w];
+}
+
 int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, const double* f, or_ctrl* K,
                   double dt_outer) {
   int status = 0;
@@ -1408,63 +1472,18 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
     double* qdf = (double*)malloc(sizeof(double) * 6 * (nb ? nb : 1));
     double* qdb = (double*)malloc(sizeof(double) * 7 * (nb ? nb : 1));
     double* qddb = (double*)malloc(sizeof(double) * 6 * (nb ? nb : 1));
-    K->iterations[w] = 0;
-    if (C->controller == 0) {
-      /* _open_frame + _seed_dt (solver_sap_adaptive.py:89-132) */
-      double base = K->next_time[w];
-      K->sim_time[w] = K->sim_time[w] - base;
-      K->next_time[w] = dt_outer;
-      double d = C->mode == 0 ? C->dt_init : K->ideal_dt[w];
-      d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
-      K->dt[w] = d; K->dt_half[w] = 0.5 * d;
-      K->substeps[w] = 0; K->diverged[w] = 0;
-    } else {
-      /* _apply_dt_cap, _rebase_time, _boundary_advance (solver_mujoco_adaptive.py:805-833) */
-      double d = K->ideal_dt[w];
-      d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
-      K->dt[w] = d; K->dt_half[w] = 0.5 * d;
-      double base = K->next_time[w];
-      K->sim_time[w] -= base; K->next_time[w] -= base;
-      K->next_time[w] += dt_outer;
-      K->diverged[w] = 0;
-    }
+    ctrl_open_frame(C, K, w, dt_outer, eff_dt_max);
     for (int it = 0; it < C->max_substeps; ++it) {
       if (!(K->sim_time[w] < K->next_time[w])) break;
       K->iterations[w]++;
-      /* _clamp_dt_to_boundary */
-      double rem = K->next_time[w] - K->sim_time[w];
-      if (rem <= 0) { K->dt[w] = 0; K->dt_half[w] = 0; }
-      else if (K->dt[w] > rem) { K->dt[w] = rem; K->dt_half[w] = 0.5 * rem; }
-      double step = K->dt[w];
+      double step = ctrl_clamp(K, w);
       int ok = 1, nc;
       double err;
       if (step > 0) err = attempt(&W, C, qw, qdw, fw, step, qf, qdf, qdb, qddb, &ok, &nc);
       else { err = 0; memcpy(qdb, qw, sizeof(double) * 7 * nb); memcpy(qddb, qdw, sizeof(double) * 6 * nb);
              memcpy(qf, qw, sizeof(double) * 7 * nb); memcpy(qdf, qdw, sizeof(double) * 6 * nb); }
-      if (!ok) { err = C->divergence_threshold; K->solve_fail[w] = 1; status = 2; }
-      K->last_error[w] = err;
-      int commit;
-      if (C->controller == 0) {
-        int32_t cum1 = 0;
-        or_adapt_dt_f64(1, &K->last_error[w], &K->sim_time[w], &K->next_time[w], &K->dt[w], &K->dt_half[w],
-                        &K->ideal_dt[w], &K->diverged[w], &K->accepted[w], &K->accepted_error[w], &K->substeps[w],
-                        &cum1, C->mode, C->tol, C->dt_min, eff_dt_max, C->divergence_threshold);
-        cum += cum1;
-        commit = K->accepted[w];
-      } else {
-        or_calc_adjusted_step_f64(1, &K->last_error[w], &K->dt[w], &K->ideal_dt[w], &K->accepted[w], &K->commit[w],
-                                  &K->diverged[w], C->tol, C->dt_min, C->divergence_threshold);
-        commit = K->commit[w];
-        if (K->accepted[w]) {
-          K->sim_time[w] += K->dt[w];
-          K->accepted_error[w] = err;
-          if (step > 0) { K->substeps[w] += 1; cum += 1; }
-        }
-        if (K->diverged[w]) K->sim_time[w] = K->next_time[w];
-        double d = K->ideal_dt[w];
-        d = d < C->dt_min ? C->dt_min : (d > eff_dt_max ? eff_dt_max : d);
-        K->dt[w] = d; K->dt_half[w] = 0.5 * d;
-      }
+      if (!ok) status = 2;
+      int commit = ctrl_after_attempt(C, K, w, err, ok, step, eff_dt_max, &cum);
       if (commit) {
         const double* sq = C->mode == 0 ? qf : qdb;
         const double* sqd = C->mode == 0 ? qdf : qddb;
@@ -1936,6 +1955,46 @@ int or_art_substep(const or_model* M, const or_art* A, const or_config* C, int w
   free(cs);
   return ok;
 }
+/* Step-doubling attempt of articulated world w from its coordinates q / velocities v (the world's
+   slices) with step h: full / doubled results into the world's slices, FP64 max-norm error
+   (1e10 sentinel), *ok = every solve converged, *n_contacts at q.  Fixed mode (C->mode 0) takes
+   the full step only (the doubled result is the full one, error 0 unless non-finite). */
+static double art_attempt_world(const or_model* M, const or_art* A, const or_config* C, int w, const double* q,
+                                const double* v, double h, double* q_full, double* v_full, double* q_dbl,
+                                double* v_dbl, int* ok_out, int* n_contacts) {
+  int j0 = A->world_joint_start[w], j1 = A->world_joint_start[w + 1];
+  int nq = A->joint_q_start[j1] - A->joint_q_start[j0], nv = A->joint_qd_start[j1] - A->joint_qd_start[j0];
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (C->max_contacts > 0 ? C->max_contacts : 1));
+  double* qm = (double*)calloc(nq + 1, sizeof(double));
+  double* vm = (double*)calloc(nv + 1, sizeof(double));
+  double* g = (double*)calloc(nv + 1, sizeof(double));
+  solve_stats st;
+  int ok = 1;
+  int nc = art_collide(M, A, C, w, q, cs);
+  if (n_contacts) *n_contacts = nc;
+  ok &= art_inner_step(M, A, C, w, q, v, h, cs, nc, NULL, q_full, v_full, &st);
+  double e = 0;
+  if (C->mode == 1) {
+    for (int i = 0; i < nv; ++i) g[i] = 0.5 * (v[i] + v_full[i]);
+    ok &= art_inner_step(M, A, C, w, q, v, 0.5 * h, cs, nc, g, qm, vm, &st);
+    nc = art_collide(M, A, C, w, qm, cs);
+    ok &= art_inner_step(M, A, C, w, qm, vm, 0.5 * h, cs, nc, v_full, q_dbl, v_dbl, &st);
+    for (int i = 0; i < nq; ++i) {
+      double d = fabs(q_dbl[i] - q_full[i]);
+      if (isnan(d)) { e = NAN; break; }
+      if (d > e) e = d;
+    }
+  } else {
+    memcpy(q_dbl, q_full, sizeof(double) * nq);
+    memcpy(v_dbl, v_full, sizeof(double) * nv);
+    for (int i = 0; i < nq; ++i) if (isnan(q_full[i])) e = NAN;
+  }
+  if (isnan(e) || isinf(e)) e = 1.0e10;
+  *ok_out = ok;
+  free(cs); free(qm); free(vm); free(g);
+  return e;
+}
+
 /* Exported step-doubling attempt of every articulated world from (q, v) with per-world dt:
    full / doubled coordinates and velocities, FP64 error norms (1e10 sentinel), solve status. */
 void or_art_attempt_all(const or_model* M, const or_art* A, const or_config* C, const double* q, const double* v,
@@ -1944,34 +2003,56 @@ void or_art_attempt_all(const or_model* M, const or_art* A, const or_config* C, 
   int W = M->world_count;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int w = 0; w < W; ++w) {
+    int j0 = A->world_joint_start[w];
+    int q0 = A->joint_q_start[j0], d0 = A->joint_qd_start[j0];
+    int ok = 1, nc = 0;
+    err[w] = art_attempt_world(M, A, C, w, q + q0, v + d0, dt[w], q_full + q0, v_full + d0, q_dbl + q0, v_dbl + d0,
+                               &ok, &nc);
+    solve_ok[w] = ok;
+    n_contacts[w] = nc;
+  }
+}
+
+/* Exported boundary call of articulated worlds: every world advances by dt_outer in place over its
+   coordinates q / velocities v with the controller state in K (the frame loop of or_step_frame
+   with the articulated attempt).  Returns 2 when a convex solve failed. */
+int or_art_step_frame(const or_model* M, const or_art* A, const or_config* C, double* q, double* v, or_ctrl* K,
+                      double dt_outer) {
+  int status = 0;
+  double eff_dt_max = C->dt_max < dt_outer ? C->dt_max : dt_outer;
+  int64_t cum = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : status) reduction(+ : cum)
+  for (int w = 0; w < M->world_count; ++w) {
     int j0 = A->world_joint_start[w], j1 = A->world_joint_start[w + 1];
     int q0 = A->joint_q_start[j0], nq = A->joint_q_start[j1] - q0;
     int d0 = A->joint_qd_start[j0], nv = A->joint_qd_start[j1] - d0;
-    or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (C->max_contacts > 0 ? C->max_contacts : 1));
-    double* qm = (double*)calloc(nq + 1, sizeof(double));
-    double* vm = (double*)calloc(nv + 1, sizeof(double));
-    double* g = (double*)calloc(nv + 1, sizeof(double));
-    solve_stats st;
-    int ok = 1;
-    double h = dt[w];
-    int nc = art_collide(M, A, C, w, q + q0, cs);
-    n_contacts[w] = nc;
-    ok &= art_inner_step(M, A, C, w, q + q0, v + d0, h, cs, nc, NULL, q_full + q0, v_full + d0, &st);
-    for (int i = 0; i < nv; ++i) g[i] = 0.5 * (v[d0 + i] + v_full[d0 + i]);
-    ok &= art_inner_step(M, A, C, w, q + q0, v + d0, 0.5 * h, cs, nc, g, qm, vm, &st);
-    nc = art_collide(M, A, C, w, qm, cs);
-    ok &= art_inner_step(M, A, C, w, qm, vm, 0.5 * h, cs, nc, v_full + d0, q_dbl + q0, v_dbl + d0, &st);
-    double e = 0;
-    for (int i = 0; i < nq; ++i) {
-      double d = fabs(q_dbl[q0 + i] - q_full[q0 + i]);
-      if (isnan(d)) { e = NAN; break; }
-      if (d > e) e = d;
+    double *qw = q + q0, *vw = v + d0;
+    double* qf = (double*)calloc(nq + 1, sizeof(double));
+    double* vf = (double*)calloc(nv + 1, sizeof(double));
+    double* qb = (double*)calloc(nq + 1, sizeof(double));
+    double* vb = (double*)calloc(nv + 1, sizeof(double));
+    ctrl_open_frame(C, K, w, dt_outer, eff_dt_max);
+    for (int it = 0; it < C->max_substeps; ++it) {
+      if (!(K->sim_time[w] < K->next_time[w])) break;
+      K->iterations[w]++;
+      double step = ctrl_clamp(K, w);
+      int ok = 1, nc = 0;
+      double err = 0;
+      if (step > 0 && j1 > j0) err = art_attempt_world(M, A, C, w, qw, vw, step, qf, vf, qb, vb, &ok, &nc);
+      else { memcpy(qf, qw, sizeof(double) * nq); memcpy(qb, qw, sizeof(double) * nq);
+             memcpy(vf, vw, sizeof(double) * nv); memcpy(vb, vw, sizeof(double) * nv); }
+      if (!ok) status = 2;
+      int commit = ctrl_after_attempt(C, K, w, err, ok, step, eff_dt_max, &cum);
+      if (commit) {
+        memcpy(qw, C->mode == 0 ? qf : qb, sizeof(double) * nq);
+        memcpy(vw, C->mode == 0 ? vf : vb, sizeof(double) * nv);
+      }
+      if (!ok) break;
     }
-    if (isnan(e) || isinf(e)) e = 1.0e10;
-    err[w] = e;
-    solve_ok[w] = ok;
-    free(cs); free(qm); free(vm); free(g);
+    free(qf); free(vf); free(qb); free(vb);
   }
+  K->cum_accepted[0] += cum;
+  return status;
 }
 
 /* ---------------------------------------------------------------- MuJoCo coordinate adapters */

@@ -82,6 +82,7 @@ def lib():
         L.or_collide_shapes.restype = C.c_int
         L.or_substep_world.restype = C.c_int
         L.or_step_frame.restype = C.c_int
+        L.or_art_step_frame.restype = C.c_int
         L.or_cubic_min.restype = C.c_int
         L.or_cubic_min.argtypes = [C.c_double] * 5 + [f64p]
         L.or_num_threads.restype = C.c_int
@@ -381,5 +382,11 @@ class OracleSolver:
         return lib().or_step_frame(C.byref(self.om.desc), C.byref(self.cfg), _ptr(q, f64p),
                                                  _ptr(qd, f64p), _ptr(f, f64p), C.byref(self.ctrl),
                                                  C.c_double(dt_outer))
+
+    def step_articulated(self, oa: "OracleArt", q: np.ndarray, v: np.ndarray, dt_outer: float) -> int:
+        """Boundary call of an articulated model on joint coordinates q / velocities v (in place)."""
+        assert q.dtype == np.float64 and q.flags.c_contiguous and v.dtype == np.float64 and v.flags.c_contiguous
+        return lib().or_art_step_frame(C.byref(self.om.desc), C.byref(oa.desc), C.byref(self.cfg), _ptr(q, f64p),
+                                       _ptr(v, f64p), C.byref(self.ctrl), C.c_double(dt_outer))
 
 
