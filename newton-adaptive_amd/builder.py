@@ -102,6 +102,7 @@ class ModelBuilder:
         self.shape_world: list = []
         # explicit joints (articulations); a model with none is all free bodies
         self.joints: list = []
+        self._joint_children: set = set()
 
     # ------------------------------------------------------------------ counts
     @property
@@ -196,8 +197,9 @@ class ModelBuilder:
     def _add_joint(self, jtype, parent, child, xp, xc, axis, tpos, tke, tkd, arm, q0) -> int:
         if parent >= child:
             raise ValueError("a joint's parent body must be added before its child")
-        if any(j["child"] == child for j in self.joints):
+        if child in self._joint_children:
             raise ValueError(f"body {child} already has a joint")
+        self._joint_children.add(int(child))
         self.joints.append(dict(type=int(jtype), parent=int(parent), child=int(child),
                                 X_p=tuple(float(v) for v in xp), X_c=tuple(float(v) for v in xc),
                                 axis=tuple(float(v) for v in axis), target_pos=float(tpos), target_ke=float(tke),
