@@ -87,6 +87,81 @@ def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, sample_worlds, frames, thread
                       f"({acc} accepted steps in {el:.1f} s)"}
 
 
+def articulated_form(na, args, log):
+    """Articulated worlds (half 7-dof arms, half floating 12-dof quadrupeds on box terrain): served
+    boundary calls (na_step_articulated, one persistent launch per frame) at tol `args.tol`, timed
+    over `args.art_frames` frames after as many warm-up frames, with the MFMA work of the contact
+    Hessians (2 (3 nc) nv^2 flops per Newton iteration, nc the contacts of the solve) and the FP64
+    CPU restatement timed on a sample of the same worlds from the same state."""
+    import numpy as np
+    import torch
+
+    W = args.art_worlds
+    model = na.scenes.articulated_mix(W, seed=args.seed, device="cuda:0")
+    kw = dict(KW, tol=args.tol, dt_inner_max=args.dt_outer)
+    sol = na.SolverSAPAdaptive(model, **kw)
+    st = model.state()
+    for _ in range(args.art_frames):
+        sol.step(st, st, model.control(), None, args.dt_outer)
+    torch.cuda.synchronize()
+    q0, v0 = st.joint_q.clone(), st.joint_qd.clone()
+    sol.reset_work_counters()
+    sol.reset_compute_counter()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.art_frames + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for f in range(args.art_frames):
+        sol.step(st, st, model.control(), None, args.dt_outer)
+        ev[f + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    acc = sol.accepted_steps()
+    launch_ms = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.art_frames)]
+    work = sol._work.cpu().numpy().astype(np.float64)
+    nv = np.diff(model.host("world_qd_start")).astype(np.float64)
+    collisions = np.maximum(2.0 * work[:, 0], 1.0)
+    cbar = work[:, 3] / collisions
+    flops = float((work[:, 1] * 2.0 * 3.0 * cbar * nv * nv).sum())
+    tflops = flops / (sum(launch_ms) / 1e3) / 1e12
+    log(f"[bench] articulated: {acc} accepted in {wall:.3f} s ({acc / wall:.0f}/s)")
+    res = {"value": acc / wall, "unit": "accepted world-steps/s", "worlds": W, "frames": args.art_frames,
+           "workload": f"articulated_mix_{W}: {W // 2} 7-dof arms + {W - W // 2} 12-dof floating quadrupeds "
+                       f"(18 dofs) on random box terrain, tol {args.tol:g}, {args.art_frames} frames of "
+                       f"{args.dt_outer * 1e3:g} ms after {args.art_frames} warm-up frames",
+           "form": "one persistent launch per boundary frame (na_step_articulated, k_art_frame)",
+           "accepted": acc, "wall_s": wall, "launch_ms": launch_ms, "attempts": int(work[:, 0].sum()),
+           "newton_iterations": int(work[:, 1].sum()), "contacts": int(work[:, 3].sum()),
+           "finite": bool(torch.isfinite(st.joint_q).all()),
+           "mfma": {"kernel": "k_art_frame (contact Hessian J^T G J, v_mfma_f32_32x32x2f32)",
+                    "algorithmic_flops": flops, "achieved": tflops, "peak": 157.3, "unit": "TFLOP/s",
+                    "frac": tflops / 157.3, "note": "fp32 dense MFMA peak; the solve is latency-bound per wave"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        n = min(2048, W)
+        m = na.scenes.articulated_mix(n, seed=args.seed, device="cpu")
+        om, oa = O.OracleModel(m), O.OracleArt(m)
+        cfg = O.make_config(tol=args.tol, dt_init=KW["dt_inner_init"], dt_max=args.dt_outer,
+                            max_substeps=KW["max_substeps"], max_iterations=100,
+                            opt_rel_tol=max(1e-3 * args.tol, 1e-8))
+        ref = O.OracleSolver(om, cfg)
+        nq, nd = int(oa.world_q_start[n]), int(oa.world_qd_start[n])
+        q = np.ascontiguousarray(q0.cpu().numpy()[:nq].astype(np.float64))
+        v = np.ascontiguousarray(v0.cpu().numpy()[:nd].astype(np.float64))
+        t0 = time.perf_counter()
+        for _ in range(args.art_frames):
+            ref.step_articulated(oa, q, v, args.dt_outer)
+        el = time.perf_counter() - t0
+        cacc = int(ref.a["cum_accepted"][0])
+        res["cpu_baseline"] = {"value": cacc / el, "unit": "accepted world-steps/s",
+                               "cores": int(O.lib().or_num_threads()), "kind": "port",
+                               "sample": f"FP64 CPU restatement, first {n} worlds, {args.art_frames} frames from the "
+                                         f"same state ({cacc} accepted steps in {el:.2f} s)"}
+    del sol
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +180,9 @@ def main():
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
     ap.add_argument("--cpu-worlds", type=int, default=64)
     ap.add_argument("--cpu-frames", type=int, default=10)
+    ap.add_argument("--no-articulated", action="store_true")
+    ap.add_argument("--art-worlds", type=int, default=8192)
+    ap.add_argument("--art-frames", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="finish with a 1 GiB read + 1 GiB write elementwise kernel (scales HBM byte counters)")
     args = ap.parse_args()
@@ -251,6 +329,10 @@ def main():
                               / res["static"]["accepted_steps_per_s"],
                               identical_results=same)
         out["p99_rollout_latency_s"] = res["continuous"]["latency_p99_s"]
+
+    # ---------------------------------------------------------------- articulated worlds
+    if not args.no_articulated:
+        out["articulated"] = articulated_form(na, args, log)
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:

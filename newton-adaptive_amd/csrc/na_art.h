@@ -43,7 +43,7 @@ struct ArtLDS {
     } mm;
     struct {
       double LA[NA_ART_ND][NA_ART_LD];  // Cholesky factor of A (lower)
-      float Y[NA_ART_NC][NA_ART_LD];    // per-contact L^-1 J_r^T of the Delassus diagonal
+      double wrow[3 * NA_ART_NC];       // |L^-1 J_r^T|^2 per contact row (Delassus diagonal)
     } fa;
   };
   ContactSet<NA_ART_NC> con;
@@ -338,6 +338,7 @@ __device__ __forceinline__ void art_body_velocity(const ArtWorld& aw, ArtLDS& A)
   if (l < aw.nb) {
     double V[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const uint32_t an = A.anc[l];
+#pragma unroll 4
     for (int k = 0; k < aw.nv; ++k) {
       if (!((an >> A.dof_body[k]) & 1u)) continue;
       const double xk = A.x[k];
@@ -355,6 +356,7 @@ __device__ __forceinline__ double art_matvec(const ArtWorld& aw, const ArtLDS& A
   const int l = lane_id();
   double y = 0.0;
   if (l < aw.nv) {
+#pragma unroll 8
     for (int j = 0; j < aw.nv; ++j) y += A.A[l][j] * A.x[j];
   }
   return y;
@@ -510,28 +512,108 @@ __device__ double art_dynamics(const NaModel& M, const ArtWorld& aw, ArtLDS& A, 
 }
 
 // ---------------------------------------------------------------- dense factorisations
-// In-place Cholesky of the leading n x n block (lower triangle; lane i updates row i).  Returns
-// false unless every pivot is positive.
-template <typename T>
-__device__ bool art_chol(T (*C)[NA_ART_LD], int n) {
+// In-place Cholesky of the leading n x n block (lower triangle), rows held in registers: lane i
+// owns row i, column j of the factor is broadcast with readlane (lane k's register j is L[k][j]).
+// NM >= n is the unrolled capacity.  Returns false unless every pivot is positive.
+__device__ __forceinline__ float bcast(float v, int k) { return lane_f(v, k); }
+__device__ __forceinline__ double bcast(double v, int k) { return lane_d(v, k); }
+template <int NM, typename T>
+__device__ bool art_chol_reg(T (*C)[NA_ART_LD], int n) {
   const int l = lane_id();
+  T r[NM];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) r[k] = (l < n && k <= l && k < n) ? C[l][k] : (T)0;
   bool ok = true;
-  for (int j = 0; j < n; ++j) {
-    wave_sync();
-    const T djj = C[j][j];
-    ok = ok && djj > (T)0;
-    const T d = sqrt(djj > (T)0 ? djj : (T)1);
-    const T id = (T)1 / d;
-    if (l > j && l < n) C[l][j] *= id;
-    wave_sync();
-    if (l == j) C[j][j] = d;
-    if (l > j && l < n) {
-      const T lij = C[l][j];
-      for (int k = j + 1; k <= l; ++k) C[l][k] -= lij * C[k][j];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    if (j < n) {
+      const T djj = bcast(r[j], j);
+      ok = ok && djj > (T)0;
+      const T d = sqrt(djj > (T)0 ? djj : (T)1);
+      const T id = (T)1 / d;
+      r[j] = l == j ? d : r[j] * id;  // lanes above j hold L[l][j]; the upper triangle is never read
+#pragma unroll
+      for (int k = j + 1; k < NM; ++k) {
+        if (k < n) r[k] -= r[j] * bcast(r[j], k);
+      }
     }
   }
   wave_sync();
+  if (l < n) {
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+      if (k <= l) C[l][k] = r[k];
+  }
+  wave_sync();
   return ok;
+}
+template <typename T>
+__device__ bool art_chol(T (*C)[NA_ART_LD], int n) {
+  if (n <= 8) return art_chol_reg<8, T>(C, n);
+  if (n <= 24) return art_chol_reg<24, T>(C, n);
+  return art_chol_reg<32, T>(C, n);
+}
+
+// Factor the leading n x n block of C in registers (as art_chol_reg, the factor is stored back to
+// C) and solve (L L^T) x = b in FP64: the forward sweep uses each lane's own row of L, the backward
+// sweep its column, loaded from LDS in one batch.  Lane i holds b_i and gets x_i; `ok` is false
+// unless every pivot is positive.
+template <int NM, typename T>
+__device__ double art_factor_solve_reg(T (*Cg)[NA_ART_LD], int n, double b, bool& ok) {
+  // the matrix lives in LDS: address it as such so that every access is a ds_* instruction
+  typedef __attribute__((address_space(3))) T lds_row[NA_ART_LD];
+  lds_row* C = (lds_row*)Cg;
+  const int l = lane_id();
+  n = __builtin_amdgcn_readfirstlane(n);
+  // rows and columns n..NM-1 are the identity, so the sweeps below run without guards
+  T r[NM];
+#pragma unroll
+  for (int k = 0; k < NM; ++k) r[k] = (l < n && k <= l) ? C[l][k] : (l == k ? (T)1 : (T)0);
+  T dg[NM];
+  T dmin = (T)1;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    const T djj = bcast(r[j], j);
+    dmin = djj < dmin ? djj : dmin;
+    const T d = sqrt(djj > (T)0 ? djj : (T)1);
+    dg[j] = (T)1 / d;
+    r[j] = l == j ? d : r[j] * dg[j];
+#pragma unroll
+    for (int k = j + 1; k < NM; ++k) r[k] -= r[j] * bcast(r[j], k);
+  }
+  ok = dmin > (T)0;
+  wave_sync();
+  if (l < n) {
+#pragma unroll
+    for (int k = 0; k < NM; ++k)
+      if (k <= l) C[l][k] = r[k];
+  }
+  // forward: L y = b (lane l subtracts L[l][i] y_i with its own row); dg holds 1 / L[i][i]
+  double x = l < n ? b : 0.0;
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    const double xi = lane_d(x, i) * (double)dg[i];
+    const double upd = x - (double)r[i] * xi;
+    x = l == i ? xi : (l > i ? upd : x);
+  }
+  wave_sync();
+  // backward: L^T x = y (lane l subtracts L[i][l] x_i with its column, read from LDS)
+  T ct[NM];
+#pragma unroll
+  for (int i = 0; i < NM; ++i) ct[i] = (i < n && i > l) ? C[i][l] : (T)0;
+#pragma unroll
+  for (int i = NM - 1; i >= 0; --i) {
+    const double xi = lane_d(x, i) * (double)dg[i];
+    const double upd = x - (double)ct[i] * xi;
+    x = l == i ? xi : (l < i ? upd : x);
+  }
+  return l < n ? x : 0.0;
+}
+template <typename T>
+__device__ double art_factor_solve(T (*C)[NA_ART_LD], int n, double b, bool& ok) {
+  if (n <= 8) return art_factor_solve_reg<8, T>(C, n, b, ok);
+  if (n <= 24) return art_factor_solve_reg<24, T>(C, n, b, ok);
+  return art_factor_solve_reg<32, T>(C, n, b, ok);
 }
 
 // x = (L L^T)^-1 b with FP64 substitution over the factor in C; lane i holds b_i, gets x_i.
@@ -585,6 +667,46 @@ __device__ __forceinline__ double art_sign(const ArtLDS& A, int k, int c) {
   return sb - sa;
 }
 
+// |L^-1 J_r^T|^2 for every contact row r of the nc contacts (one row 3c + r per lane, in passes of
+// 64), by forward substitution in registers against the factor of A in A.fa.LA; into A.fa.wrow.
+template <int NM>
+__device__ void art_delassus_rows(const ArtWorld& aw, ArtLDS& A, int nc) {
+  const int l = lane_id();
+  const int nv = aw.nv;
+  for (int base = 0; base < 3 * nc; base += NA_WAVE) {
+    const int row = base + l;
+    const int c = row / 3, r = row - 3 * c;
+    const bool valid = row < 3 * nc;
+    double F[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double p[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+      art_frame(A, c, F);
+      p[0] = A.con.p[0][c]; p[1] = A.con.p[1][c]; p[2] = A.con.p[2][c];
+    }
+    const double fr[3] = {F[3 * r], F[3 * r + 1], F[3 * r + 2]};
+    double y[NM];
+    double ysq = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+      y[i] = 0.0;
+      if (i < nv) {
+        // J_r,i = sign_i F_r . (S_i,lin + S_i,ang x p)
+        const double sg = valid ? art_sign(A, i, c) : 0.0;
+        const double wa[3] = {A.S[i][0], A.S[i][1], A.S[i][2]};
+        double u[3];
+        dcross3(wa, p, u);
+        double ji = sg * (fr[0] * (A.S[i][3] + u[0]) + fr[1] * (A.S[i][4] + u[1]) + fr[2] * (A.S[i][5] + u[2]));
+#pragma unroll
+        for (int k = 0; k < i; ++k) ji -= A.fa.LA[i][k] * y[k];
+        y[i] = ji / A.fa.LA[i][i];
+        ysq += y[i] * y[i];
+      }
+    }
+    if (valid) A.fa.wrow[row] = ysq;
+  }
+  wave_sync();
+}
+
 // Per-contact set-up of a step h (lane c < nc): frame, Delassus diagonal through the factor of
 // A in A.fa.LA, regularisation, v_hat, activity.
 __device__ void art_contact_setup(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int nc, double h) {
@@ -598,30 +720,11 @@ __device__ void art_contact_setup(const NaConfig& cfg, const ArtWorld& aw, ArtLD
     A.t1[0][c] = t1.x; A.t1[1][c] = t1.y; A.t1[2][c] = t1.z;
   }
   wave_sync();
+  if (nv <= 8) art_delassus_rows<8>(aw, A, nc);
+  else if (nv <= 24) art_delassus_rows<24>(aw, A, nc);
+  else art_delassus_rows<32>(aw, A, nc);
   if (c < nc) {
-    double F[9];
-    art_frame(A, c, F);
-    const double p[3] = {A.con.p[0][c], A.con.p[1][c], A.con.p[2][c]};
-    double wsum = 0.0;
-    float* y = A.fa.Y[c];
-    for (int r = 0; r < 3; ++r) {
-      // y = L^-1 J_r^T by forward substitution, J_r,i = sign_i F_r . (S_i,lin + S_i,ang x p)
-      for (int i = 0; i < nv; ++i) {
-        const double sg = art_sign(A, i, c);
-        double ji = 0.0;
-        if (sg != 0.0) {
-          const double wa[3] = {A.S[i][0], A.S[i][1], A.S[i][2]};
-          double u[3];
-          dcross3(wa, p, u);
-          ji = sg * (F[3 * r] * (A.S[i][3] + u[0]) + F[3 * r + 1] * (A.S[i][4] + u[1]) +
-                     F[3 * r + 2] * (A.S[i][5] + u[2]));
-        }
-        for (int k = 0; k < i; ++k) ji -= A.fa.LA[i][k] * (double)y[k];
-        const double yi = ji / A.fa.LA[i][i];
-        y[i] = (float)yi;
-        wsum += yi * yi;
-      }
-    }
+    const double wsum = A.fa.wrow[3 * c] + A.fa.wrow[3 * c + 1] + A.fa.wrow[3 * c + 2];
     const double wd = wsum / 3.0;
     const double tau_d = cfg.tau_d;
     if (wd < NA_ART_W_MIN) {
@@ -714,8 +817,10 @@ struct ArtStats {
 // in A.con: Newton from `guess`.  The velocity comes back in `vout` (lane k < nv), the integrated
 // coordinates in `qout` (lane k < nq).  Returns true when the SAP optimality criterion holds.
 __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int nc, double h, double v0,
-                         double cbias, double mdiag, double guess, double& vout, double& qout, ArtStats& st) {
+                         double cbias, double mdiag, double guess, double& vout, double& qout, ArtStats& st,
+                         long long* wk) {
   const int l = lane_id();
+  NA_TIC(t_setup);
   const int nv = aw.nv;
   const bool dof = l < nv;
   st.iterations = 0;
@@ -745,19 +850,22 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     for (int j = 0; j < nv; ++j) A.fa.LA[l][j] = j == l ? diag : A.A[l][j];
   }
   // FP64 factor of A for v* and the Delassus diagonals
-  bool ok = art_chol<double>(A.fa.LA, nv);
-  const double vstar = art_chol_solve<double>(A.fa.LA, nv, rhs);
+  bool ok;
+  const double vstar = art_factor_solve<double>(A.fa.LA, nv, rhs, ok);
+  wave_sync();
   art_contact_setup(cfg, aw, A, nc, h);
   const double Dk = dof ? 1.0 / sqrt(diag) : 0.0;
   if (dof) A.x[l] = vstar;
   wave_sync();
   const double Avs = art_matvec(aw, A);
+  NA_TOC(wk[7], t_setup);
   const int max_iterations = cfg.max_iterations;
   const double opt_abs = cfg.opt_abs_tol, opt_rel = cfg.opt_rel_tol, ls_rel = cfg.ls_rel_tol;
   double v = dof ? guess : 0.0;
   if (!ok) st.reason = NA_EXIT_CHOL;
   for (int it = 0; ok && it <= max_iterations; ++it) {
     // gradient g = A (v - v*) - J^T gamma in FP64
+    NA_TIC(t_eval);
     wave_sync();
     if (dof) A.x[l] = v;
     wave_sync();
@@ -790,12 +898,15 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     wave_sync();
     double jt = 0.0, d = 0.0;
     if (dof) {
+      double sl[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) sl[i] = A.S[l][i];
+#pragma unroll 2
       for (int c = 0; c < nc; ++c) {
         const double sg = art_sign(A, l, c);
-        if (sg == 0.0) continue;
         double t = 0.0;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) t += A.S[l][i] * A.W[i][c];
+        for (int i = 0; i < 6; ++i) t += sl[i] * A.W[i][c];
         jt += sg * t;
       }
       d = v - vstar;
@@ -817,6 +928,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     jn = sqrt(wave_sum_d(jn));
     const double mom0 = wave_sum_d(mom);
     const double l0 = mom0 + wave_sum_d(ccost);
+    NA_TOC(wk[8], t_eval);
     if (!(gn == gn)) {
       st.reason = NA_EXIT_NAN;
       ok = false;
@@ -830,14 +942,20 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     if (it == max_iterations) break;
 
     // Newton direction from H = A + J^T G J (f32 factor, FP64 substitution)
+    NA_TIC(t_hess);
     art_hessian(aw, A, nc);
-    if (!art_chol<float>(A.H, nv)) {
+    NA_TOC(wk[9], t_hess);
+    NA_TIC(t_fac);
+    bool spd;
+    const double dv = art_factor_solve<float>(A.H, nv, -g, spd);
+    NA_TOC(wk[2], t_fac);  // diagnostic builds: factor + solve cycles in the (unused) CG slot
+    if (!spd) {
       st.reason = NA_EXIT_CHOL;
       ok = false;
       break;
     }
     st.factorizations++;
-    const double dv = art_chol_solve<float>(A.H, nv, -g);
+    NA_TIC(t_ls);
 
     // exact line search along dv: l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a)
     wave_sync();
@@ -889,14 +1007,17 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
       alpha = cubic_min_d(l0, dl0, l1, dl1, 1.0);
       if (!(alpha > 0.0)) alpha = fmin(fmax(-dl0 / d2l0, 0.0), 1.0);
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
+      // this lane's contact in registers for the root search
+      const bool lact = l < nc && A.act[l];
+      const int lc = lact ? l : 0;
+      const d3 lu = d3{A.dvc[0][lc], A.dvc[1][lc], A.dvc[2][lc]};
+      const d3 lv = d3{A.vc0[0][lc], A.vc0[1][lc], A.vc0[2][lc] - A.vhat[lc]};
+      const double liRt = A.iRt[lc], liRn = A.iRn[lc], lRt = A.Rt[lc], lRn = A.Rn[lc], lmu = A.con.mu[lc];
       for (int k = 0; k < 60; ++k) {
         double dlc = 0.0, d2c = 0.0;
-        if (l < nc && A.act[l]) {
-          const d3 u = d3{A.dvc[0][l], A.dvc[1][l], A.dvc[2][l]};
-          const double iRt = A.iRt[l], iRn = A.iRn[l];
-          const d3 y = d3{-(A.vc0[0][l] + alpha * u.x) * iRt, -(A.vc0[1][l] + alpha * u.y) * iRt,
-                          -(A.vc0[2][l] + alpha * u.z - A.vhat[l]) * iRn};
-          sap_line((double)A.con.mu[l], A.Rt[l], A.Rn[l], iRt, iRn, y, u, dlc, d2c);
+        if (lact) {
+          const d3 y = d3{-(lv.x + alpha * lu.x) * liRt, -(lv.y + alpha * lu.y) * liRt, -(lv.z + alpha * lu.z) * liRn};
+          sap_line(lmu, lRt, lRn, liRt, liRn, y, lu, dlc, d2c);
         }
         const double dl = b0 + alpha * aa + wave_sum_d(dlc);
         const double ddl = aa + wave_sum_d(d2c);
@@ -911,6 +1032,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     }
     if (dof) v += alpha * dv;
     st.iterations = it + 1;
+    NA_TOC(wk[11], t_ls);
   }
   vout = dof ? v : 0.0;
   // integrate (lane j < nb owns joint j): revolute q += h v; FREE: the COM moves with v_com, the
@@ -952,6 +1074,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
 // Contacts of the world at the kinematics in LDS; returns the kept count (<= the capacity).
 __device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int& overflow,
                            long long* wk) {
+  NA_TIC(t0);
   stage_aabb(M, aw.w, aw.nseg, aw.nl, aw.b0, &A.bqf[0][0], NA_ART_NB, A.seg);
   const int np = stage_broad(M, aw.w, aw.nseg, aw.nl, cfg.broad_phase, A.seg);
   const int cap = min(cfg.max_contacts, NA_ART_NC);
@@ -959,6 +1082,7 @@ __device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld
   if (np > NA_MAX_PAIRS || nc > cap) overflow = 1;
   wk[3] += min(nc, cap);
   wk[4] += np;
+  NA_TOC(wk[6], t0);
   return min(nc, cap);
 }
 
@@ -971,23 +1095,27 @@ __device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWo
   const int l = lane_id();
   ArtStats st;
   double mdiag;
+  NA_TIC(t_kin);
   art_kinematics(M, aw, A, q);
   double cb = art_dynamics(M, aw, A, v, mdiag);
+  NA_TOC(wk[10], t_kin);
   int nc = art_collide(M, cfg, aw, A, overflow, wk);
   nc_out = nc;
-  ok = art_step(cfg, aw, A, nc, dt, v, cb, mdiag, v, vf, qf, st);
+  ok = art_step(cfg, aw, A, nc, dt, v, cb, mdiag, v, vf, qf, st, wk);
   wk[1] += st.iterations;
   wk[5] += 1;
   double err = 0.0;
   if (cfg.mode == 1) {
     const double g = 0.5 * (v + vf);
     double qm, vm;
-    ok = art_step(cfg, aw, A, nc, 0.5 * dt, v, cb, mdiag, g, vm, qm, st) && ok;
+    ok = art_step(cfg, aw, A, nc, 0.5 * dt, v, cb, mdiag, g, vm, qm, st, wk) && ok;
     wk[1] += st.iterations;
+    NA_TIC(t_kin2);
     art_kinematics(M, aw, A, qm);
     cb = art_dynamics(M, aw, A, vm, mdiag);
+    NA_TOC(wk[10], t_kin2);
     nc = art_collide(M, cfg, aw, A, overflow, wk);
-    ok = art_step(cfg, aw, A, nc, 0.5 * dt, vm, cb, mdiag, vf, vd, qd, st) && ok;
+    ok = art_step(cfg, aw, A, nc, 0.5 * dt, vm, cb, mdiag, vf, vd, qd, st, wk) && ok;
     wk[1] += st.iterations;
     wk[5] += 2;
     double e = 0.0;

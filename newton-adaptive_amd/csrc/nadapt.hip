@@ -1178,7 +1178,7 @@ __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, con
   const int nc = art_collide(M, cfg, aw, A, ovf, wk);
   ArtStats st;
   double vo, qo;
-  const bool ok = art_step(cfg, aw, A, nc, h[w], v, cb, mdiag, v, vo, qo, st);
+  const bool ok = art_step(cfg, aw, A, nc, h[w], v, cb, mdiag, v, vo, qo, st, wk);
   if (l < aw.nq) q_out[aw.q0 + l] = qo;
   if (l < aw.nv) qd_out[aw.d0 + l] = vo;
   if (l == 0) {
