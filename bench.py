@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--cpu-worlds", type=int, default=64)
     ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--no-articulated", action="store_true")
+    ap.add_argument("--articulated-only", action="store_true",
+                    help="run only the articulated workload (profiling of k_art_frame); prints its JSON object")
     ap.add_argument("--art-worlds", type=int, default=8192)
     ap.add_argument("--art-frames", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
@@ -200,6 +202,10 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.articulated_only:
+        print(json.dumps({"articulated": articulated_form(na, args, log)}))
+        return
 
     model = na.scenes.uneven_pile(args.worlds, seed=args.seed, device=dev)
     W = model.world_count
