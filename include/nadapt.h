@@ -237,6 +237,17 @@ int na_attempt(const na_solver* s, const double* q64, const float* qd, const flo
    COM / angular velocities at the new coordinates (eval_fk). */
 int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd, float* body_q, float* body_qd,
                         double dt_outer, void* stream);
+/* Serving form of na_step_articulated (no counterpart in the reference, like na_serve): the
+   articulated worlds listed in `active` run boundary frames until their horizon, a solve failure
+   or the device-time budget, with the semantics of na_serve.  joint_q64 [coordinates] and
+   joint_qd64 [dofs] are caller-owned FP64 shadows: a world resumes from them while they still
+   round to joint_q / joint_qd, so budget interruptions are exact; velocities are rounded to
+   float32 at every frame boundary, so F frames equal F calls of na_step_articulated. */
+int na_serve_articulated(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
+                         const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
+                         const double* world_tol, float* joint_q, float* joint_qd, double* joint_q64,
+                         double* joint_qd64, float* body_q, float* body_qd, double dt_outer, double budget_ms,
+                         int32_t max_iterations, int32_t* unfinished, void* stream);
 /* One inner step per articulated world (contacts at q, convex solve at h[w], FP64 integration);
    the parity form of substep (solver_sap_adaptive.py:714-729).  stats [W,4] int32: Newton
    iterations, factorizations, converged, contacts. */

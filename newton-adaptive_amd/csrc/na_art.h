@@ -181,10 +181,10 @@ __device__ __forceinline__ ArtWorld art_world(const NaModel& M, int w) {
   a.nb = M.world_body_end[w] - a.b0;
   a.nl = M.world_shape_end[w] - M.world_shape_start[w];
   a.nseg = a.nl + M.G;
-  a.q0 = M.joint_q_start[a.b0];
-  a.nq = M.joint_q_start[a.b0 + a.nb] - a.q0;
-  a.d0 = M.joint_qd_start[a.b0];
-  a.nv = M.joint_qd_start[a.b0 + a.nb] - a.d0;
+  a.q0 = a.nb > 0 ? M.joint_q_start[a.b0] : 0;
+  a.d0 = a.nb > 0 ? M.joint_qd_start[a.b0] : 0;
+  a.nq = 0;  // set by art_load from the joint types: a world's coordinates and dofs are contiguous
+  a.nv = 0;  // from q0 / d0, whatever follows them (a pool keeps worlds in arena ranges)
   a.dmax = 0;
   return a;
 }
@@ -203,13 +203,6 @@ __device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A) {
     for (int k = 0; k < 7; ++k) { A.Xp[l][k] = M.joint_X_p[7 * j + k]; A.Xc[l][k] = M.joint_X_c[7 * j + k]; }
 #pragma unroll
     for (int k = 0; k < 3; ++k) { A.axis[l][k] = M.joint_axis[3 * j + k]; A.com_b[l][k] = M.body_com[3 * j + k]; }
-  }
-  if (l < aw.nv) {
-    const int d = aw.d0 + l;
-    A.kp[l] = M.joint_target_ke[d];
-    A.kd[l] = M.joint_target_kd[d];
-    A.arm[l] = M.joint_armature[d];
-    A.qt[l] = M.joint_target_pos[d];
   }
   if (l == NA_ART_DUMMY) A.anc[l] = 0u;
   if (l < 6) A.V[NA_ART_DUMMY][l] = 0.0;
@@ -235,6 +228,21 @@ __device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A) {
     }
   }
   aw.dmax = wave_reduce_i(depth, OpMaxI());
+  int jq = 0, jd = 0;
+  if (l < aw.nb) {
+    const bool free_j = A.jtype[l] == NA_JT_FREE;
+    jq = free_j ? 7 : 1;
+    jd = free_j ? 6 : 1;
+  }
+  aw.nq = wave_sum_i(jq);
+  aw.nv = wave_sum_i(jd);
+  if (l < aw.nv) {
+    const int d = aw.d0 + l;
+    A.kp[l] = M.joint_target_ke[d];
+    A.kd[l] = M.joint_target_kd[d];
+    A.arm[l] = M.joint_armature[d];
+    A.qt[l] = M.joint_target_pos[d];
+  }
   wave_sync();
 }
 

@@ -483,7 +483,7 @@ __device__ __forceinline__ bool pair_passes(const NaModel& M, const SegLDS& S, i
   if (wa != -1 && wb != -1 && wa != wb) return false;
   if (!group_pair(M.shape_group[sa], M.shape_group[sb])) return false;
   int ba = M.shape_body[sa], bb = M.shape_body[sb];
-  if (ba >= 0 && ba == bb) return false;  // shapes of one body never collide
+  if (ba == bb) return false;  // shapes of one body never collide; static shapes (-1) are one body
   if (M.F > 0 && pair_excluded(M, sa, sb)) return false;
   return S.lo[0][i] <= S.hi[0][j] && S.hi[0][i] >= S.lo[0][j] && S.lo[1][i] <= S.hi[1][j] &&
          S.hi[1][i] >= S.lo[1][j] && S.lo[2][i] <= S.hi[2][j] && S.hi[2][i] >= S.lo[2][j];

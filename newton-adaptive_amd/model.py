@@ -175,7 +175,7 @@ class Model:
                 ga, gb = grp[a], grp[b]
                 ok = ~((sw[a] == -1) & (sw[b] == -1))
                 ok &= (ga != 0) & (gb != 0) & np.where(ga > 0, (ga == gb) | (gb < 0), ga != gb)
-                ok &= ~((body[a] >= 0) & (body[a] == body[b]))
+                ok &= body[a] != body[b]  # one body; the static shapes (-1) count as one body
                 if len(fkey):
                     ok &= ~np.isin(a * S + b, fkey)
                 out.append(np.stack([a[ok], b[ok]], axis=1))

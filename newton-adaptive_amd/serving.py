@@ -48,6 +48,7 @@ class RolloutRequest:
 
     bodies: dict
     shapes: dict
+    joints: dict | None = None  # articulated world: joint, dof and coordinate arrays (local indices)
     horizon_frames: int | None = None
     horizon_accepted: int | None = None
     tol: float | None = None
@@ -61,9 +62,25 @@ class RolloutRequest:
     frames: int = 0
     body_q: np.ndarray | None = None
     body_qd: np.ndarray | None = None
+    joint_q: np.ndarray | None = None
+    joint_qd: np.ndarray | None = None
     slot: int = -1
     b0: int = -1
     s0: int = -1
+    c0: int = -1
+    d0: int = -1
+
+    @property
+    def articulated(self) -> bool:
+        return self.joints is not None
+
+    @property
+    def coord_count(self) -> int:
+        return int(self.joints["joint_q"].shape[0]) if self.joints is not None else 0
+
+    @property
+    def dof_count(self) -> int:
+        return int(self.joints["joint_qd"].shape[0]) if self.joints is not None else 0
 
     @property
     def body_count(self) -> int:
@@ -83,9 +100,15 @@ _SHAPE_FIELDS = ("shape_type", "shape_body", "shape_collision_group", "shape_tra
                  "shape_margin", "shape_gap", "shape_material_ke", "shape_material_mu", "shape_collision_radius")
 
 
-def world_arrays(model, w: int, state=None) -> tuple[dict, dict]:
-    """Host arrays of world ``w`` of a finalized :class:`Model` (shape bodies as local indices).
-    With ``state``, the world starts from that state's body_q / body_qd instead of the model's."""
+_JOINT_BODY_FIELDS = ("joint_type", "joint_parent", "joint_q_start", "joint_qd_start", "joint_X_p", "joint_X_c",
+                      "joint_axis")
+_JOINT_DOF_FIELDS = ("joint_target_pos", "joint_target_ke", "joint_target_kd", "joint_armature")
+
+
+def world_arrays(model, w: int, state=None) -> tuple[dict, dict, dict | None]:
+    """Host arrays of world ``w`` of a finalized :class:`Model` (shape bodies as local indices):
+    ``(bodies, shapes, joints)``, ``joints`` None for a free-body model.  With ``state`` (a dict of
+    host arrays), the world starts from that state's body_q / body_qd (and joint_q / joint_qd)."""
     wb, ws = model.host("world_body_start"), model.host("world_shape_start")
     b0, b1, s0, s1 = int(wb[w]), int(wb[w + 1]), int(ws[w]), int(ws[w + 1])
     bodies = {k: np.array(model.host(k)[b0:b1]) for k in _BODY_FIELDS}
@@ -95,7 +118,20 @@ def world_arrays(model, w: int, state=None) -> tuple[dict, dict]:
     shapes = {k: np.array(model.host(k)[s0:s1]) for k in _SHAPE_FIELDS}
     sb = shapes["shape_body"].astype(np.int64)
     shapes["shape_body"] = np.where(sb >= 0, sb - b0, -1).astype(np.int32)
-    return bodies, shapes
+    joints = None
+    if getattr(model, "is_articulated", False):
+        qs, ds = model.host("joint_q_start"), model.host("joint_qd_start")
+        q0, q1, d0, d1 = int(qs[b0]), int(qs[b1]), int(ds[b0]), int(ds[b1])
+        joints = {k: np.array(model.host(k)[b0:b1]) for k in _JOINT_BODY_FIELDS}
+        par = joints["joint_parent"].astype(np.int64)
+        joints["joint_parent"] = np.where(par >= 0, par - b0, -1).astype(np.int32)
+        joints["joint_q_start"] = (joints["joint_q_start"] - q0).astype(np.int32)
+        joints["joint_qd_start"] = (joints["joint_qd_start"] - d0).astype(np.int32)
+        joints.update({k: np.array(model.host(k)[d0:d1]) for k in _JOINT_DOF_FIELDS})
+        src = state if state is not None and "joint_q" in state else None
+        joints["joint_q"] = np.array((src["joint_q"] if src else model.host("joint_q"))[q0:q1], dtype=np.float32)
+        joints["joint_qd"] = np.array((src["joint_qd"] if src else model.host("joint_qd"))[d0:d1], dtype=np.float32)
+    return bodies, shapes, joints
 
 
 def requests_from_model(model, *, horizon_frames=None, horizon_accepted=None, tol=None, arrivals=None,
@@ -106,14 +142,17 @@ def requests_from_model(model, *, horizon_frames=None, horizon_accepted=None, to
     host_state = None
     if state is not None:
         host_state = {"body_q": state.body_q.detach().cpu().numpy(), "body_qd": state.body_qd.detach().cpu().numpy()}
+        if getattr(model, "is_articulated", False):
+            host_state["joint_q"] = state.joint_q.detach().cpu().numpy()
+            host_state["joint_qd"] = state.joint_qd.detach().cpu().numpy()
     out = []
     for i, w in enumerate(worlds):
-        b, s = world_arrays(model, w, host_state)
+        b, s, j = world_arrays(model, w, host_state)
 
         def pick(x):
             return x[i] if isinstance(x, (list, tuple, np.ndarray)) else x
 
-        out.append(RolloutRequest(b, s, horizon_frames=pick(horizon_frames), horizon_accepted=pick(horizon_accepted),
+        out.append(RolloutRequest(b, s, j, horizon_frames=pick(horizon_frames), horizon_accepted=pick(horizon_accepted),
                                   tol=pick(tol), arrival=float(pick(arrivals)) if arrivals is not None else 0.0,
                                   rid=i))
     return out
@@ -165,7 +204,8 @@ class WorldPool:
     its bodies in ``[world_body_start[w], world_body_end[w])`` and its shapes likewise.
     """
 
-    def __init__(self, globals_model, max_worlds: int, body_capacity: int, shape_capacity: int, device):
+    def __init__(self, globals_model, max_worlds: int, body_capacity: int, shape_capacity: int, device,
+                 articulated: bool = False):
         self.device = torch.device(device)
         d = self.device
         g = globals_model.host("global_shapes").astype(np.int64)
@@ -214,6 +254,26 @@ class WorldPool:
         for name in ("shape_transform", "shape_scale", "shape_margin", "shape_gap", "shape_material_ke",
                      "shape_material_mu", "shape_collision_radius"):
             getattr(self, name)[gs] = _t(globals_model.host(name)[g], torch.float32, d)
+        # articulated slots: joint arrays per body slot (joint_type 0 marks the bodies of free-body
+        # worlds), and coordinate / dof arenas of 7 / 6 entries per body slot
+        self.is_articulated = bool(articulated)
+        if self.is_articulated:
+            Qc, Dc = 7 * Bc, 6 * Bc
+            self.coord_capacity, self.dof_capacity = Qc, Dc
+            self.joint_type = torch.zeros(Bc, **i32)
+            self.joint_parent = torch.full((Bc,), -1, **i32)
+            self.joint_q_start = torch.zeros(Bc + 1, **i32)
+            self.joint_qd_start = torch.zeros(Bc + 1, **i32)
+            ident = torch.tensor([0, 0, 0, 0, 0, 0, 1], **f32)
+            self.joint_X_p = ident.repeat(Bc, 1)
+            self.joint_X_c = ident.repeat(Bc, 1)
+            self.joint_axis = torch.zeros((Bc, 3), **f32)
+            for name in _JOINT_DOF_FIELDS:
+                setattr(self, name, torch.zeros(Dc, **f32))
+            self.joint_q = torch.zeros(Qc, **f32)
+            self.joint_qd = torch.zeros(Dc, **f32)
+            self.joint_q64 = torch.zeros(Qc, dtype=torch.float64, device=d)
+            self.joint_qd64 = torch.zeros(Dc, dtype=torch.float64, device=d)
 
     def state(self):
         from .model import State
@@ -228,6 +288,18 @@ def _bytes_per_body() -> int:
 
 def _bytes_per_shape() -> int:
     return 4 * 4 + 28 + 12 + 6 * 4
+
+
+def _bytes_per_coord() -> int:
+    return 4 + 8  # joint_q + FP64 shadow
+
+
+def _bytes_per_dof() -> int:
+    return 4 + 8 + 16  # joint_qd + FP64 shadow + drive targets, gains, armature
+
+
+def _bytes_per_joint() -> int:
+    return 4 * 4 + 17 * 4  # type, parent, q / qd starts; X_p, X_c, axis
 
 
 def _bytes_per_slot() -> int:
@@ -247,6 +319,8 @@ def check_request(r: RolloutRequest, global_count: int, budget_bytes: int) -> No
         raise ValueError(f"tol must be > 0, got {r.tol!r}")
     if r.body_count < 1 or r.body_count > 64 or r.shape_count + global_count > 128:
         raise ValueError("a world holds 1..64 bodies and at most 128 segment shapes (world + global)")
+    if r.articulated and (r.body_count > 16 or r.coord_count > 32 or r.dof_count > 32):
+        raise ValueError("an articulated world holds at most 16 bodies, 32 coordinates and 32 dofs")
     if ServingEngine.request_bytes(r) > budget_bytes:
         raise ValueError("request larger than the HBM budget")
 
@@ -291,7 +365,7 @@ class ServingEngine:
     def __init__(self, globals_model, *, max_worlds: int = 4096, body_capacity: int | None = None,
                  shape_capacity: int | None = None, hbm_budget_bytes: int | None = None, dt_outer: float = 0.01,
                  budget_ms: float = 20.0, controller: str = "sap", schedule: str = "continuous", device="cuda:0",
-                 **solver_kw):
+                 articulated: bool = False, **solver_kw):
         if schedule not in ("continuous", "static"):
             raise ValueError(f"schedule must be 'continuous' or 'static', got {schedule!r}")
         self.schedule = schedule
@@ -301,7 +375,7 @@ class ServingEngine:
             raise ValueError("dt_outer must be > 0")
         body_capacity = int(body_capacity if body_capacity is not None else 32 * max_worlds)
         shape_capacity = int(shape_capacity if shape_capacity is not None else 32 * max_worlds)
-        self.pool = WorldPool(globals_model, max_worlds, body_capacity, shape_capacity, device)
+        self.pool = WorldPool(globals_model, max_worlds, body_capacity, shape_capacity, device, articulated)
         self.G = int(self.pool.global_shapes.numel())
         self.dt_outer = float(dt_outer)
         self.budget_ms = float(budget_ms)
@@ -325,6 +399,8 @@ class ServingEngine:
         self.resident_bytes = 0
         self._bodies = _Arena(body_capacity)
         self._shapes = _Arena(shape_capacity)
+        self._coords = _Arena(self.pool.coord_capacity if articulated else 0)
+        self._dofs = _Arena(self.pool.dof_capacity if articulated else 0)
         self._free_slots = list(range(W - 1, -1, -1))
         self.pending: list[RolloutRequest] = []
         self.resident: dict[int, RolloutRequest] = {}
@@ -337,13 +413,19 @@ class ServingEngine:
     # ------------------------------------------------------------------ admission
     @staticmethod
     def request_bytes(req: RolloutRequest) -> int:
-        return req.body_count * _bytes_per_body() + req.shape_count * _bytes_per_shape() + _bytes_per_slot()
+        b = req.body_count * _bytes_per_body() + req.shape_count * _bytes_per_shape() + _bytes_per_slot()
+        if req.articulated:
+            b += (req.body_count * _bytes_per_joint() + req.coord_count * _bytes_per_coord()
+                  + req.dof_count * _bytes_per_dof())
+        return b
 
     def submit(self, reqs) -> None:
         """Queue requests (kept in arrival order)."""
         reqs = [reqs] if isinstance(reqs, RolloutRequest) else list(reqs)
         for r in reqs:
             check_request(r, self.G, self.hbm_budget_bytes)
+            if r.articulated and not self.pool.is_articulated:
+                raise ValueError("an articulated request needs an engine built with articulated=True")
         self.pending.extend(reqs)
         self.pending.sort(key=lambda r: r.arrival)
 
@@ -362,8 +444,18 @@ class ServingEngine:
             if s0 is None:
                 self._bodies.release(b0, r.body_count)
                 break
+            c0 = d0 = 0
+            if r.articulated:
+                c0 = self._coords.alloc(r.coord_count)
+                d0 = self._dofs.alloc(r.dof_count) if c0 is not None else None
+                if c0 is None or d0 is None:
+                    if c0 is not None:
+                        self._coords.release(c0, r.coord_count)
+                    self._bodies.release(b0, r.body_count)
+                    self._shapes.release(s0, r.shape_count)
+                    break
             self.pending.pop(0)
-            r.slot, r.b0, r.s0 = self._free_slots.pop(), b0, s0
+            r.slot, r.b0, r.s0, r.c0, r.d0 = self._free_slots.pop(), b0, s0, c0, d0
             r.admitted_at = now
             self.resident_bytes += need
             self.resident[r.slot] = r
@@ -394,6 +486,8 @@ class ServingEngine:
             dst.index_copy_(0, si, _t(vals, dt, d).reshape((-1,) + tuple(dst.shape[1:])))
         P.shape_world.index_copy_(0, si, torch.as_tensor(
             np.concatenate([np.full(r.shape_count, r.slot) for r in batch]), dtype=torch.int32, device=d))
+        if P.is_articulated:
+            self._upload_joints(batch, bi)
         slots = torch.as_tensor([r.slot for r in batch], dtype=torch.int64, device=d)
 
         def put(t, vals, dtype):
@@ -417,6 +511,40 @@ class ServingEngine:
         self._reset_mask = mask  # kept alive until the asynchronous reset has run
         _lib.check(_lib.lib().na_reset(self.solver._h, mask.data_ptr(), _lib.stream_handle()), "na_reset")
 
+    def _upload_joints(self, batch, bi) -> None:
+        """Joint arrays of the batch's body slots (type 0 for free-body worlds), the articulated
+        worlds' per-dof arrays, coordinates, velocities and their FP64 shadows."""
+        P, d = self.pool, self.pool.device
+        jt = np.concatenate([r.joints["joint_type"] if r.articulated else np.zeros(r.body_count, np.int32)
+                             for r in batch]).astype(np.int32)
+        P.joint_type.index_copy_(0, bi, torch.as_tensor(jt, device=d))
+        art = [r for r in batch if r.articulated]
+        if not art:
+            return
+        ab = torch.as_tensor(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in art]), dtype=torch.int64,
+                             device=d)
+        par = np.concatenate([np.where(r.joints["joint_parent"] >= 0, r.joints["joint_parent"] + r.b0, -1)
+                              for r in art])
+        P.joint_parent.index_copy_(0, ab, torch.as_tensor(par, dtype=torch.int32, device=d))
+        P.joint_q_start.index_copy_(0, ab, torch.as_tensor(
+            np.concatenate([r.joints["joint_q_start"] + r.c0 for r in art]), dtype=torch.int32, device=d))
+        P.joint_qd_start.index_copy_(0, ab, torch.as_tensor(
+            np.concatenate([r.joints["joint_qd_start"] + r.d0 for r in art]), dtype=torch.int32, device=d))
+        for k in ("joint_X_p", "joint_X_c", "joint_axis"):
+            dst = getattr(P, k)
+            dst.index_copy_(0, ab, _t(np.concatenate([r.joints[k] for r in art]), torch.float32, d).reshape(
+                (-1,) + tuple(dst.shape[1:])))
+        di = torch.as_tensor(np.concatenate([np.arange(r.d0, r.d0 + r.dof_count) for r in art]), dtype=torch.int64,
+                             device=d)
+        ci = torch.as_tensor(np.concatenate([np.arange(r.c0, r.c0 + r.coord_count) for r in art]), dtype=torch.int64,
+                             device=d)
+        for k in _JOINT_DOF_FIELDS:
+            getattr(P, k).index_copy_(0, di, _t(np.concatenate([r.joints[k] for r in art]), torch.float32, d))
+        P.joint_q.index_copy_(0, ci, _t(np.concatenate([r.joints["joint_q"] for r in art]), torch.float32, d))
+        P.joint_qd.index_copy_(0, di, _t(np.concatenate([r.joints["joint_qd"] for r in art]), torch.float32, d))
+        P.joint_q64.index_copy_(0, ci, P.joint_q.index_select(0, ci).double())
+        P.joint_qd64.index_copy_(0, di, P.joint_qd.index_select(0, di).double())
+
     # ------------------------------------------------------------------ rounds
     def launch(self, max_iterations: int = 0, unfinished: torch.Tensor | None = None) -> int:
         """Enqueue one persistent launch over the resident, unfinished worlds (oldest first).
@@ -424,16 +552,32 @@ class ServingEngine:
         live = sorted((r for r in self.resident.values() if r.status == 0), key=lambda r: (r.arrival, r.rid))
         if not live:
             return 0
-        self._active = torch.as_tensor([r.slot for r in live], dtype=torch.int32, device=self.pool.device)
         P = self.pool
         L = _lib.lib()
-        _lib.check(L.na_serve(self.solver._h, self._active.data_ptr(), len(live), self.horizon_frames.data_ptr(),
-                              self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
-                              self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
-                              P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
-                              self.budget_ms if max_iterations <= 0 else 0.0, int(max_iterations),
-                              unfinished.data_ptr() if unfinished is not None else None,
-                              _lib.stream_handle()), "na_serve")
+        budget = self.budget_ms if max_iterations <= 0 else 0.0
+        flag = unfinished.data_ptr() if unfinished is not None else None
+        free = [r for r in live if not r.articulated]
+        art = [r for r in live if r.articulated]
+        if free:
+            self._active = torch.as_tensor([r.slot for r in free], dtype=torch.int32, device=P.device)
+            _lib.check(L.na_serve(self.solver._h, self._active.data_ptr(), len(free), self.horizon_frames.data_ptr(),
+                                  self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
+                                  self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
+                                  P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
+                                  budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve")
+        if art:
+            # after the free-body launch on the same stream; it would otherwise clear the flag it set
+            self._active_art = torch.as_tensor([r.slot for r in art], dtype=torch.int32, device=P.device)
+            if free and unfinished is not None:
+                self._flag_free = unfinished.clone()
+            _lib.check(L.na_serve_articulated(
+                self.solver._h, self._active_art.data_ptr(), len(art), self.horizon_frames.data_ptr(),
+                self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(), self.accepted.data_ptr(),
+                self.status.data_ptr(), self.world_tol.data_ptr(), P.joint_q.data_ptr(), P.joint_qd.data_ptr(),
+                P.joint_q64.data_ptr(), P.joint_qd64.data_ptr(), P.body_q.data_ptr(), P.body_qd.data_ptr(),
+                self.dt_outer, budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve_articulated")
+            if free and unfinished is not None:
+                unfinished.bitwise_or_(self._flag_free)
         self.stats.rounds += 1
         return len(live)
 
@@ -452,11 +596,19 @@ class ServingEngine:
                                dtype=torch.int64, device=self.pool.device)
         q = self.pool.body_q.index_select(0, bidx).cpu().numpy()
         qd = self.pool.body_qd.index_select(0, bidx).cpu().numpy()
+        if any(r.articulated for r in done):
+            jq = self.pool.joint_q.cpu().numpy()
+            jqd = self.pool.joint_qd.cpu().numpy()
         o = 0
         for r in done:
             n = r.body_count
             r.body_q, r.body_qd = q[o:o + n], qd[o:o + n]
             o += n
+            if r.articulated:
+                r.joint_q = jq[r.c0:r.c0 + r.coord_count].copy()
+                r.joint_qd = jqd[r.d0:r.d0 + r.dof_count].copy()
+                self._coords.release(r.c0, r.coord_count)
+                self._dofs.release(r.d0, r.dof_count)
             r.status, r.accepted, r.frames = int(status[r.slot]), int(acc[r.slot]), int(frames[r.slot])
             r.finished_at = now
             self.resident_bytes -= self.request_bytes(r)

@@ -701,7 +701,8 @@ int or_world_pairs(const or_model* M, int w, const double* lo, const double* hi,
       int wa = M->shape_world[sa], wb = M->shape_world[sb];
       if (wa == -1 && wb == -1) continue; /* global-global pairs belong to the dedicated -1 segment */
       if (!world_group_pair(wa, wb, M->shape_group[sa], M->shape_group[sb])) continue;
-      if (M->shape_body[sa] >= 0 && M->shape_body[sa] == M->shape_body[sb]) continue; /* one body */
+      /* one body; the static shapes (body -1) count as one body too (builder.py:6204-6208) */
+      if (M->shape_body[sa] == M->shape_body[sb]) continue;
       if (M->filter_count > 0 && excluded(M, sa, sb)) continue;
       const double *la = lo + 3 * i, *ua = hi + 3 * i, *lb = lo + 3 * j, *ub = hi + 3 * j;
       if (la[0] <= ub[0] && ua[0] >= lb[0] && la[1] <= ub[1] && ua[1] >= lb[1] && la[2] <= ub[2] && ua[2] >= lb[2]) {

@@ -7,7 +7,11 @@ GPU (through ``na_serve``):
 * a request's result is bit-identical alone and in a mix where it is admitted later, into a
   recycled slot and arena range, next to other rollouts of different sizes and tolerances;
 * admission never exceeds the HBM budget, and horizons in frames and in accepted steps end the
-  rollouts where they say.
+  rollouts where they say;
+* articulated requests (arms, quadrupeds) served by ``na_serve_articulated`` next to free-body
+  piles: a rollout of F frames equals F calls of ``na_step_articulated`` on the standalone model
+  (also across budget interruptions), a request is the same alone and in a mixed stream, and the
+  lock-step baseline gives the same results.
 """
 
 import importlib
@@ -51,6 +55,26 @@ def test_requests_from_model_local_indices():
     with pytest.raises(ValueError):
         serving.check_request(reqs[0], 5, big - 1)
     serving.check_request(reqs[0], 5, big)
+
+
+def test_articulated_requests_local_indices_and_limits():
+    m = na.scenes.articulated_mix(4, seed=3, device="cpu")
+    reqs = na.requests_from_model(m, horizon_frames=2)
+    for r in reqs:
+        assert r.articulated and r.joints["joint_parent"][0] == -1
+        assert r.joints["joint_parent"].max() < r.body_count
+        assert r.joints["joint_q_start"][0] == 0 and r.joints["joint_qd_start"][0] == 0
+        assert r.coord_count == int(r.joints["joint_q_start"][-1]) + (7 if r.joints["joint_type"][-1] == 4 else 1)
+        assert len(r.joints["joint_target_ke"]) == r.dof_count
+    assert (reqs[0].coord_count, reqs[1].coord_count, reqs[1].dof_count) == (7, 19, 18)
+    big = dict(reqs[1].joints)
+    with pytest.raises(ValueError):  # more than 32 dofs
+        big["joint_qd"] = np.zeros(33, np.float32)
+        serving.check_request(serving.RolloutRequest(reqs[1].bodies, reqs[1].shapes, big, horizon_frames=1), 1,
+                              1 << 30)
+    model, mixed = na.scenes.serving_requests(200, seed=4, rate=100.0, distinct=40, articulated_share=0.25)
+    share = np.mean([r.articulated for r in mixed])
+    assert 0.15 < share < 0.35
 
 
 # ---------------------------------------------------------------- GPU
@@ -161,3 +185,74 @@ def test_static_lockstep_batches_equal_continuous():
         assert np.array_equal(x.body_q, y.body_q) and np.array_equal(x.body_qd, y.body_qd)
     for r in b:
         assert (r.frames == 3) if r.horizon_frames else (r.accepted == r.horizon_accepted)
+
+
+def _art_boundary_calls(model, frames, **kw):
+    state = model.state()
+    solver = na.SolverSAPAdaptive(model, **kw)
+    for _ in range(frames):
+        solver.step(state, state, model.control(), None, 0.01)
+    torch.cuda.synchronize()
+    return state.joint_q.cpu().numpy(), state.joint_qd.cpu().numpy(), solver.accepted_steps()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("budget_ms", [0.0, 0.05])
+def test_articulated_served_rollout_equals_boundary_calls(budget_ms):
+    model = na.scenes.articulated_mix(24, seed=13, device=DEV)
+    frames = 6
+    print(f"articulated serving vs boundary calls, budget {budget_ms} ms", flush=True)
+    q_ref, qd_ref, acc_ref = _art_boundary_calls(model, frames, **KW)
+    eng = na.ServingEngine(model, max_worlds=24, body_capacity=400, shape_capacity=400, budget_ms=budget_ms,
+                           device=DEV, articulated=True, **KW)
+    reqs = na.requests_from_model(model, horizon_frames=frames)
+    summary = eng.run(reqs, realtime=False)
+    print("serve summary", summary, flush=True)
+    assert summary["completed"] == 24 and summary["failed"] == 0
+    if budget_ms > 0:
+        assert summary["rounds"] > 1
+    qs, ds = model.host("joint_q_start"), model.host("joint_qd_start")
+    wb = model.host("world_body_start")
+    for r in reqs:
+        b0, b1 = int(wb[r.rid]), int(wb[r.rid + 1])
+        q0, q1, d0, d1 = int(qs[b0]), int(qs[b1]), int(ds[b0]), int(ds[b1])
+        assert r.frames == frames
+        assert np.array_equal(r.joint_q, q_ref[q0:q1]), f"world {r.rid}: coordinates differ"
+        assert np.array_equal(r.joint_qd, qd_ref[d0:d1]), f"world {r.rid}: velocities differ"
+    assert sum(r.accepted for r in reqs) == acc_ref
+
+
+@pytest.mark.gpu
+def test_mixed_stream_articulated_alone_equals_in_mix_and_static():
+    model, reqs = na.scenes.serving_requests(48, seed=9, distinct=24, max_prims=12, horizons=(30, 80),
+                                             articulated_share=0.25, device=DEV)
+    art = [i for i, r in enumerate(reqs) if r.articulated]
+    assert art, "the stream holds articulated requests"
+    target = art[len(art) // 2]
+    mk = lambda: na.scenes.serving_requests(48, seed=9, distinct=24, max_prims=12, horizons=(30, 80),
+                                            articulated_share=0.25, device=DEV)[1]
+    print(f"mixed stream: {len(art)} articulated of 48, target {target}", flush=True)
+    alone = mk()[target]
+    e1 = na.ServingEngine(model, max_worlds=4, body_capacity=200, shape_capacity=200, budget_ms=2.0, device=DEV,
+                          articulated=True, **KW)
+    assert e1.run([alone], realtime=False)["completed"] == 1
+    mix = mk()
+    e2 = na.ServingEngine(model, max_worlds=12, body_capacity=400, shape_capacity=400, budget_ms=3.0, device=DEV,
+                          articulated=True, **KW)
+    s2 = e2.run(mix, realtime=False)
+    print("mix summary", s2, flush=True)
+    assert s2["completed"] == 48 and s2["rounds"] > 2
+    r = mix[target]
+    assert r.accepted == alone.accepted == r.horizon_accepted
+    assert np.array_equal(r.joint_q, alone.joint_q) and np.array_equal(r.joint_qd, alone.joint_qd)
+    assert np.array_equal(r.body_q, alone.body_q)
+    static = mk()
+    e3 = na.ServingEngine(model, max_worlds=12, body_capacity=400, shape_capacity=400, schedule="static",
+                          device=DEV, articulated=True, **KW)
+    s3 = e3.run(static, realtime=False)
+    assert s3["completed"] == 48
+    for x, y in zip(mix, static):
+        assert x.accepted == y.accepted
+        assert np.array_equal(x.body_q, y.body_q)
+        if x.articulated:
+            assert np.array_equal(x.joint_q, y.joint_q) and np.array_equal(x.joint_qd, y.joint_qd)
