@@ -248,6 +248,16 @@ int na_serve_articulated(na_solver* s, const int32_t* active, int32_t n_active, 
                          const double* world_tol, float* joint_q, float* joint_qd, double* joint_q64,
                          double* joint_qd64, float* body_q, float* body_qd, double dt_outer, double budget_ms,
                          int32_t max_iterations, int32_t* unfinished, void* stream);
+/* Serving launch over a pool holding free-body and articulated worlds (no counterpart in the
+   reference): one persistent launch in which every wave takes the next listed world of either kind,
+   so both kinds share the device; per world the arithmetic of na_serve (body_q / body_qd / body_f)
+   or na_serve_articulated (joint_q / joint_qd with their FP64 shadows, body_q / body_qd by forward
+   kinematics).  A world is articulated when its first body has joint_type != 0. */
+int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
+                   const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
+                   const double* world_tol, float* body_q, float* body_qd, const float* body_f, float* joint_q,
+                   float* joint_qd, double* joint_q64, double* joint_qd64, double dt_outer, double budget_ms,
+                   int32_t max_iterations, int32_t* unfinished, void* stream);
 /* One inner step per articulated world (contacts at q, convex solve at h[w], FP64 integration);
    the parity form of substep (solver_sap_adaptive.py:714-729).  stats [W,4] int32: Newton
    iterations, factorizations, converged, contacts. */

@@ -89,6 +89,8 @@ SIGNATURES = {
     "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, vp]),
     "na_serve_articulated": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        C.c_double, i32, vp, vp]),
+    "na_serve_mixed": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
+                                 C.c_double, i32, vp, vp]),
     "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_art_attempt": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),

@@ -628,6 +628,58 @@ struct ServeSlots {
 // `unfinished` (optional) gets a bit set when any world is still running after its turn.  Every
 // wave exits: the loop ends when the list is exhausted or the budget has passed, and each world
 // loop runs at most max_substeps iterations per frame.
+// One world's turn of a serving launch: boundary frames until the world's horizon, a solve
+// failure, max_iters iterations (<= 0: no limit) or the device clock passing t_end.
+__device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64, float* body_q,
+                            float* body_qd, const float* body_f, double dt_outer, double eff_max, const ServeSlots& SV,
+                            int w, long long t_end, int max_iters, int32_t* unfinished, WorldLDS& S) {
+  WorldCtx wc = world_ctx(M, w);
+  NaConfig wcfg = cfg;
+  if (SV.tol) {
+    wcfg.tol = SV.tol[w];
+    wcfg.opt_rel_tol = (float)fmax(1.0e-3 * wcfg.tol, 1.0e-8);
+  }
+  double q[7];
+  float v[6], f[6];
+  load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
+  WCtrl c = load_ctrl(K, w);
+  long long wk[NA_WORK] = {};
+  int frames = SV.frames_done[w], accepted = SV.accepted[w], status = 0, acc = 0;
+  const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
+  // a world without any finite horizon has nothing to run (keeps every wave's loop finite)
+  if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
+  int iters = 0;
+  while (status == 0) {
+    if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
+      // at a frame boundary (or the frame's iteration cap): finish, or open the next frame
+      if (c.iterations > 0 || c.sim_time > 0.0) ++frames;
+      if (frames >= hf) { status = 1; break; }
+      open_frame(wcfg, c, dt_outer, eff_max);
+    }
+    int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, eff_max, wk);
+    acc += inc;
+    accepted += inc;
+    if (c.solve_fail) { status = 2; break; }
+    if (accepted >= ha) { status = 1; break; }
+    if (++iters == max_iters) {
+      // lock-step form: a world whose frame just ended and whose last frame it was is done now
+      if (!(c.sim_time < c.next_time) && frames + 1 >= hf) { ++frames; status = 1; }
+      break;
+    }
+    if (wall_clock64() >= t_end) break;
+  }
+  store_body(wc, q64, body_q, body_qd, q, v);
+  if (lane_id() == 0) {
+    store_ctrl(K, w, c);
+    SV.frames_done[w] = frames;
+    SV.accepted[w] = accepted;
+    SV.status[w] = status;
+    if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    if (status == 0 && unfinished) atomicOr(unfinished, 1);
+  }
+  flush_work(K, w, wk);
+}
+
 __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
                                               float* body_qd, const float* body_f, double dt_outer, double eff_max,
                                               int32_t* queue, const int32_t* active, int n_active, ServeSlots SV,
@@ -641,51 +693,7 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
     if (slot >= n_active) break;
     const int w = active[slot];
     if (SV.status[w] != 0) continue;
-    WorldCtx wc = world_ctx(M, w);
-    NaConfig wcfg = cfg;
-    if (SV.tol) {
-      wcfg.tol = SV.tol[w];
-      wcfg.opt_rel_tol = (float)fmax(1.0e-3 * wcfg.tol, 1.0e-8);
-    }
-    double q[7];
-    float v[6], f[6];
-    load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
-    WCtrl c = load_ctrl(K, w);
-    long long wk[NA_WORK] = {};
-    int frames = SV.frames_done[w], accepted = SV.accepted[w], status = 0, acc = 0;
-    const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
-    // a world without any finite horizon has nothing to run (keeps every wave's loop finite)
-    if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
-    int iters = 0;
-    while (status == 0) {
-      if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
-        // at a frame boundary (or the frame's iteration cap): finish, or open the next frame
-        if (c.iterations > 0 || c.sim_time > 0.0) ++frames;
-        if (frames >= hf) { status = 1; break; }
-        open_frame(wcfg, c, dt_outer, eff_max);
-      }
-      int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, eff_max, wk);
-      acc += inc;
-      accepted += inc;
-      if (c.solve_fail) { status = 2; break; }
-      if (accepted >= ha) { status = 1; break; }
-      if (++iters == max_iters) {
-        // lock-step form: a world whose frame just ended and whose last frame it was is done now
-        if (!(c.sim_time < c.next_time) && frames + 1 >= hf) { ++frames; status = 1; }
-        break;
-      }
-      if (wall_clock64() >= t_end) break;
-    }
-    store_body(wc, q64, body_q, body_qd, q, v);
-    if (lane_id() == 0) {
-      store_ctrl(K, w, c);
-      SV.frames_done[w] = frames;
-      SV.accepted[w] = accepted;
-      SV.status[w] = status;
-      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
-      if (status == 0 && unfinished) atomicOr(unfinished, 1);
-    }
-    flush_work(K, w, wk);
+    serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, S);
   }
 }
 
@@ -1162,86 +1170,137 @@ __global__ __launch_bounds__(64) void k_art_frame(NaModel M, NaConfig cfg, Ctrl 
 // world interrupted by the budget continues exactly; at every frame boundary the velocity is
 // rounded to float32 as a boundary call's joint_qd would be, so a rollout of F frames leaves the
 // same state as F calls of na_step_articulated.
-__global__ __launch_bounds__(64) void k_art_serve(NaModel M, NaConfig cfg, Ctrl K, double* q64, double* qd64,
-                                                  float* joint_q, float* joint_qd, float* body_q, float* body_qd,
-                                                  double dt_outer, double eff_max, int32_t* queue,
-                                                  const int32_t* active, int n_active, ServeSlots SV, long long budget,
-                                                  int max_iters, int32_t* unfinished) {
-  __shared__ ArtLDS A;
+// Caller-owned state of articulated worlds in a serving launch.
+struct ArtServeState {
+  double* q64;      // FP64 coordinate shadow
+  double* qd64;     // FP64 velocity shadow
+  float* joint_q;
+  float* joint_qd;
+  float* body_q;    // forward kinematics output (may be null)
+  float* body_qd;
+};
+
+// One articulated world's turn of a serving launch (the semantics of serve_world).
+__device__ __noinline__ void serve_world_art(const NaModel& M, const NaConfig& cfg, const Ctrl& K, const ArtServeState& X,
+                                double dt_outer, double eff_max, const ServeSlots& SV, int w, long long t_end,
+                                int max_iters, int32_t* unfinished, ArtLDS& A) {
   const int l = lane_id();
+  ArtWorld aw = art_world(M, w);
+  if (aw.nb == 0) return;
+  art_load(M, aw, A);
+  NaConfig wcfg = cfg;
+  if (SV.tol) {
+    wcfg.tol = SV.tol[w];
+    wcfg.opt_rel_tol = (float)fmax(1.0e-3 * wcfg.tol, 1.0e-8);
+  }
+  double q = 0.0, v = 0.0;
+  int same = 1;
+  if (l < aw.nq) {
+    q = X.q64[aw.q0 + l];
+    same = (float)q == X.joint_q[aw.q0 + l] ? 1 : 0;
+  }
+  if (l < aw.nv) {
+    v = X.qd64[aw.d0 + l];
+    same = same && (float)v == X.joint_qd[aw.d0 + l];
+  }
+  same = wave_min_i(same ? 1 : 0);
+  if (!same) {
+    q = l < aw.nq ? (double)X.joint_q[aw.q0 + l] : 0.0;
+    v = l < aw.nv ? (double)X.joint_qd[aw.d0 + l] : 0.0;
+  }
+  WCtrl c = load_ctrl(K, w);
+  long long wk[NA_WORK] = {};
+  int frames = SV.frames_done[w], accepted = SV.accepted[w], status = 0, acc = 0;
+  const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
+  if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
+  int iters = 0;
+  while (status == 0) {
+    if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
+      if (c.iterations > 0 || c.sim_time > 0.0) ++frames;
+      if (frames >= hf) { status = 1; break; }
+      open_frame(wcfg, c, dt_outer, eff_max);
+      v = (double)(float)v;  // the boundary state's velocity is float32
+    }
+    const int inc = art_iteration(M, wcfg, aw, A, c, q, v, eff_max, wk);
+    acc += inc;
+    accepted += inc;
+    if (c.solve_fail) { status = 2; break; }
+    if (accepted >= ha) { status = 1; break; }
+    if (++iters == max_iters) {
+      if (!(c.sim_time < c.next_time) && frames + 1 >= hf) { ++frames; status = 1; }
+      break;
+    }
+    if (wall_clock64() >= t_end) break;
+  }
+  if (l < aw.nq) {
+    X.q64[aw.q0 + l] = q;
+    X.joint_q[aw.q0 + l] = (float)q;
+  }
+  if (l < aw.nv) {
+    X.qd64[aw.d0 + l] = v;
+    X.joint_qd[aw.d0 + l] = (float)v;
+  }
+  if (X.body_q) art_store_bodies(M, aw, A, q, v, X.body_q, X.body_qd);
+  if (l == 0) {
+    store_ctrl(K, w, c);
+    SV.frames_done[w] = frames;
+    SV.accepted[w] = accepted;
+    SV.status[w] = status;
+    if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+    if (status == 0 && unfinished) atomicOr(unfinished, 1);
+  }
+  flush_work(K, w, wk);
+}
+
+// Persistent serving launch over articulated worlds (the articulated counterpart of k_serve, same
+// horizons, budget and lock-step semantics).  Coordinates and velocities resume from FP64 shadows
+// (q64 per coordinate, qd64 per dof) while they still round to the caller's joint_q / joint_qd, so a
+// world interrupted by the budget continues exactly; at every frame boundary the velocity is
+// rounded to float32 as a boundary call's joint_qd would be, so a rollout of F frames leaves the
+// same state as F calls of na_step_articulated.
+__global__ __launch_bounds__(64) void k_art_serve(NaModel M, NaConfig cfg, Ctrl K, ArtServeState X, double dt_outer,
+                                                  double eff_max, int32_t* queue, const int32_t* active, int n_active,
+                                                  ServeSlots SV, long long budget, int max_iters,
+                                                  int32_t* unfinished) {
+  __shared__ ArtLDS A;
   const long long t_end = wall_clock64() + budget;
   for (;;) {
     int slot = 0;
-    if (l == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
+    if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
     slot = __shfl(slot, 0, NA_WAVE);
     if (slot >= n_active) break;
     const int w = active[slot];
     if (SV.status[w] != 0) continue;
-    ArtWorld aw = art_world(M, w);
-    if (aw.nb == 0) continue;
-    art_load(M, aw, A);
-    NaConfig wcfg = cfg;
-    if (SV.tol) {
-      wcfg.tol = SV.tol[w];
-      wcfg.opt_rel_tol = (float)fmax(1.0e-3 * wcfg.tol, 1.0e-8);
-    }
-    double q = 0.0, v = 0.0;
-    int same = 1;
-    if (l < aw.nq) {
-      q = q64[aw.q0 + l];
-      same = (float)q == joint_q[aw.q0 + l] ? 1 : 0;
-    }
-    if (l < aw.nv) {
-      v = qd64[aw.d0 + l];
-      same = same && (float)v == joint_qd[aw.d0 + l];
-    }
-    same = wave_min_i(same ? 1 : 0);
-    if (!same) {
-      q = l < aw.nq ? (double)joint_q[aw.q0 + l] : 0.0;
-      v = l < aw.nv ? (double)joint_qd[aw.d0 + l] : 0.0;
-    }
-    WCtrl c = load_ctrl(K, w);
-    long long wk[NA_WORK] = {};
-    int frames = SV.frames_done[w], accepted = SV.accepted[w], status = 0, acc = 0;
-    const int hf = SV.horizon_frames[w], ha = SV.horizon_accepted[w];
-    if (hf >= 0x7fffffff && ha >= 0x7fffffff) status = 1;
-    int iters = 0;
-    while (status == 0) {
-      if (!(c.sim_time < c.next_time) || c.iterations >= wcfg.max_substeps) {
-        if (c.iterations > 0 || c.sim_time > 0.0) ++frames;
-        if (frames >= hf) { status = 1; break; }
-        open_frame(wcfg, c, dt_outer, eff_max);
-        v = (double)(float)v;  // the boundary state's velocity is float32
-      }
-      const int inc = art_iteration(M, wcfg, aw, A, c, q, v, eff_max, wk);
-      acc += inc;
-      accepted += inc;
-      if (c.solve_fail) { status = 2; break; }
-      if (accepted >= ha) { status = 1; break; }
-      if (++iters == max_iters) {
-        if (!(c.sim_time < c.next_time) && frames + 1 >= hf) { ++frames; status = 1; }
-        break;
-      }
-      if (wall_clock64() >= t_end) break;
-    }
-    if (l < aw.nq) {
-      q64[aw.q0 + l] = q;
-      joint_q[aw.q0 + l] = (float)q;
-    }
-    if (l < aw.nv) {
-      qd64[aw.d0 + l] = v;
-      joint_qd[aw.d0 + l] = (float)v;
-    }
-    if (body_q) art_store_bodies(M, aw, A, q, v, body_q, body_qd);
-    if (l == 0) {
-      store_ctrl(K, w, c);
-      SV.frames_done[w] = frames;
-      SV.accepted[w] = accepted;
-      SV.status[w] = status;
-      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
-      if (status == 0 && unfinished) atomicOr(unfinished, 1);
-    }
-    flush_work(K, w, wk);
+    serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, A);
+  }
+}
+
+// Serving launch over a pool that holds free-body and articulated worlds: every wave takes the
+// next listed world of either kind from one queue, so the two kinds share the device and every
+// world advances continuously.  A world is articulated when its first body carries a joint
+// (joint_type != 0).  Each world's arithmetic is that of k_serve / k_art_serve.
+__global__ __launch_bounds__(64) void k_serve_mixed(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                                    float* body_qd, const float* body_f, ArtServeState X,
+                                                    double dt_outer, double eff_max, int32_t* queue,
+                                                    const int32_t* active, int n_active, ServeSlots SV,
+                                                    long long budget, int max_iters, int32_t* unfinished) {
+  // one LDS block, viewed as the free-body or the articulated working set
+  constexpr size_t kMixBytes = sizeof(WorldLDS) > sizeof(ArtLDS) ? sizeof(WorldLDS) : sizeof(ArtLDS);
+  __shared__ __align__(16) char mix_lds[kMixBytes];
+  WorldLDS& Uf = *reinterpret_cast<WorldLDS*>(mix_lds);
+  ArtLDS& Ua = *reinterpret_cast<ArtLDS*>(mix_lds);
+  const long long t_end = wall_clock64() + budget;
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= n_active) break;
+    const int w = active[slot];
+    if (SV.status[w] != 0) continue;
+    const int b0 = M.world_body_start[w];
+    const bool art = M.world_body_end[w] > b0 && M.joint_type[b0] != 0;
+    if (art) serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, Ua);
+    else serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, Uf);
   }
 }
 
@@ -1767,8 +1826,41 @@ extern "C" int na_serve_articulated(na_solver* s, const int32_t* active, int32_t
   int grid = default_grid(s, (const void*)k_art_serve);
   grid = grid < n_active ? grid : n_active;
   if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
-  k_art_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd,
-                                   dt_outer, eff, s->queue, active, n_active, SV, budget, max_iterations, unfinished);
+  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd};
+  k_art_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, X, dt_outer, eff, s->queue, active, n_active, SV, budget,
+                                   max_iterations, unfinished);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+extern "C" int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
+                              const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted,
+                              int32_t* status, const double* world_tol, float* body_q, float* body_qd,
+                              const float* body_f, float* joint_q, float* joint_qd, double* joint_q64,
+                              double* joint_qd64, double dt_outer, double budget_ms, int32_t max_iterations,
+                              int32_t* unfinished, void* stream) {
+  if (!s || !body_q || !body_qd || !joint_q || !joint_qd || !joint_q64 || !joint_qd64)
+    return fail("na_serve_mixed: null argument");
+  if (!s->model->articulated) return fail("na_serve_mixed: the model has no joint arrays");
+  if (n_active < 0 || n_active > s->model->M.W) return fail("na_serve_mixed: n_active must be in [0, world_count]");
+  if (n_active > 0 && (!active || !horizon_frames || !horizon_accepted || !frames_done || !accepted || !status))
+    return fail("na_serve_mixed: null slot array");
+  if (!(dt_outer > 0)) return fail("na_serve_mixed: dt_outer must be > 0");
+  if (n_active == 0) return 0;
+  const NaModel& M = s->model->M;
+  hipStream_t st = S_(stream);
+  int dev = 0, rate_khz = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+  const long long budget = budget_ms > 0 ? (long long)(budget_ms * (double)rate_khz) : (1LL << 62);
+  const double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
+  ServeSlots SV{horizon_frames, horizon_accepted, frames_done, accepted, status, world_tol};
+  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd};
+  k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
+  int grid = default_grid(s, (const void*)k_serve_mixed);
+  grid = grid < n_active ? grid : n_active;
+  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
+  k_serve_mixed<<<grid, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, X, dt_outer, eff, s->queue,
+                                     active, n_active, SV, budget, max_iterations, unfinished);
   HIPCHK(hipGetLastError());
   return 0;
 }

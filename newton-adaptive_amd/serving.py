@@ -365,10 +365,11 @@ class ServingEngine:
     def __init__(self, globals_model, *, max_worlds: int = 4096, body_capacity: int | None = None,
                  shape_capacity: int | None = None, hbm_budget_bytes: int | None = None, dt_outer: float = 0.01,
                  budget_ms: float = 20.0, controller: str = "sap", schedule: str = "continuous", device="cuda:0",
-                 articulated: bool = False, **solver_kw):
+                 articulated: bool = False, split_kinds: bool = False, **solver_kw):
         if schedule not in ("continuous", "static"):
             raise ValueError(f"schedule must be 'continuous' or 'static', got {schedule!r}")
         self.schedule = schedule
+        self.split_kinds = bool(split_kinds)
         if max_worlds < 1:
             raise ValueError("max_worlds must be >= 1")
         if not dt_outer > 0:
@@ -556,6 +557,34 @@ class ServingEngine:
         L = _lib.lib()
         budget = self.budget_ms if max_iterations <= 0 else 0.0
         flag = unfinished.data_ptr() if unfinished is not None else None
+        if P.is_articulated and self.split_kinds:
+            self._launch_split(live, budget, max_iterations, unfinished)
+            self.stats.rounds += 1
+            return len(live)
+        self._active = torch.as_tensor([r.slot for r in live], dtype=torch.int32, device=P.device)
+        if P.is_articulated:
+            # one launch over both kinds: every wave takes the next world of either kind
+            _lib.check(L.na_serve_mixed(
+                self.solver._h, self._active.data_ptr(), len(live), self.horizon_frames.data_ptr(),
+                self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(), self.accepted.data_ptr(),
+                self.status.data_ptr(), self.world_tol.data_ptr(), P.body_q.data_ptr(), P.body_qd.data_ptr(),
+                P.body_f.data_ptr(), P.joint_q.data_ptr(), P.joint_qd.data_ptr(), P.joint_q64.data_ptr(),
+                P.joint_qd64.data_ptr(), self.dt_outer, budget, int(max_iterations), flag, _lib.stream_handle()),
+                "na_serve_mixed")
+        else:
+            _lib.check(L.na_serve(self.solver._h, self._active.data_ptr(), len(live), self.horizon_frames.data_ptr(),
+                                  self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
+                                  self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
+                                  P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
+                                  budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve")
+        self.stats.rounds += 1
+        return len(live)
+
+    def _launch_split(self, live, budget, max_iterations, unfinished) -> None:
+        """One launch per kind on the same stream (na_serve, then na_serve_articulated): the two
+        kinds take turns instead of sharing the device."""
+        P, L = self.pool, _lib.lib()
+        flag = unfinished.data_ptr() if unfinished is not None else None
         free = [r for r in live if not r.articulated]
         art = [r for r in live if r.articulated]
         if free:
@@ -566,9 +595,8 @@ class ServingEngine:
                                   P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
                                   budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve")
         if art:
-            # after the free-body launch on the same stream; it would otherwise clear the flag it set
             self._active_art = torch.as_tensor([r.slot for r in art], dtype=torch.int32, device=P.device)
-            if free and unfinished is not None:
+            if free and unfinished is not None:  # the second launch starts by clearing the flag
                 self._flag_free = unfinished.clone()
             _lib.check(L.na_serve_articulated(
                 self.solver._h, self._active_art.data_ptr(), len(art), self.horizon_frames.data_ptr(),
@@ -578,8 +606,6 @@ class ServingEngine:
                 self.dt_outer, budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve_articulated")
             if free and unfinished is not None:
                 unfinished.bitwise_or_(self._flag_free)
-        self.stats.rounds += 1
-        return len(live)
 
     def retire(self, now: float) -> list:
         """Read per-world status after the launch; copy out and free every finished request."""

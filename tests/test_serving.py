@@ -197,14 +197,14 @@ def _art_boundary_calls(model, frames, **kw):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("budget_ms", [0.0, 0.05])
-def test_articulated_served_rollout_equals_boundary_calls(budget_ms):
+@pytest.mark.parametrize("budget_ms,split", [(0.0, False), (0.05, False), (0.05, True)])
+def test_articulated_served_rollout_equals_boundary_calls(budget_ms, split):
     model = na.scenes.articulated_mix(24, seed=13, device=DEV)
     frames = 6
     print(f"articulated serving vs boundary calls, budget {budget_ms} ms", flush=True)
     q_ref, qd_ref, acc_ref = _art_boundary_calls(model, frames, **KW)
     eng = na.ServingEngine(model, max_worlds=24, body_capacity=400, shape_capacity=400, budget_ms=budget_ms,
-                           device=DEV, articulated=True, **KW)
+                           device=DEV, articulated=True, split_kinds=split, **KW)
     reqs = na.requests_from_model(model, horizon_frames=frames)
     summary = eng.run(reqs, realtime=False)
     print("serve summary", summary, flush=True)
@@ -248,7 +248,7 @@ def test_mixed_stream_articulated_alone_equals_in_mix_and_static():
     assert np.array_equal(r.body_q, alone.body_q)
     static = mk()
     e3 = na.ServingEngine(model, max_worlds=12, body_capacity=400, shape_capacity=400, schedule="static",
-                          device=DEV, articulated=True, **KW)
+                          device=DEV, articulated=True, split_kinds=True, **KW)
     s3 = e3.run(static, realtime=False)
     assert s3["completed"] == 48
     for x, y in zip(mix, static):
