@@ -11,7 +11,7 @@ from .contacts import CollisionPipeline, Contacts
 from .model import Control, GeoType, Model, State, eval_fk
 from .solvers import SolverMuJoCoAdaptive, SolverSAPAdaptive
 from .serving import RolloutRequest, ServingEngine, requests_from_model, world_arrays
-from . import mujoco_adapter
+from . import mujoco_adapter, rl_backend
 from . import scenes
 
 
