@@ -550,7 +550,11 @@ class ServingEngine:
     def launch(self, max_iterations: int = 0, unfinished: torch.Tensor | None = None) -> int:
         """Enqueue one persistent launch over the resident, unfinished worlds (oldest first).
         ``max_iterations`` > 0 caps the iterations per world (1: lock step)."""
-        live = sorted((r for r in self.resident.values() if r.status == 0), key=lambda r: (r.arrival, r.rid))
+        # oldest first; among requests that arrived together the largest worlds first, so the
+        # longest rollouts start early instead of forming the launch's tail (the order decides only
+        # which wave runs a world, never its arithmetic)
+        live = sorted((r for r in self.resident.values() if r.status == 0),
+                      key=lambda r: (r.arrival, -(r.body_count + r.shape_count), r.rid))
         if not live:
             return 0
         P = self.pool
