@@ -178,6 +178,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=3600)
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
+    ap.add_argument("--serve-articulated", type=float, default=0.25,
+                    help="share of requests that are articulated worlds (arm or quadruped on terrain)")
     ap.add_argument("--cpu-worlds", type=int, default=64)
     ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--no-articulated", action="store_true")
@@ -313,22 +315,27 @@ def main():
         res = {}
         for sched in ("continuous", "static"):
             smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=args.serve_rate,
-                                                       device=dev, distinct=args.serve_worlds)
+                                                       device=dev, distinct=args.serve_worlds,
+                                                       articulated_share=args.serve_articulated)
             seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
-                                    budget_ms=20.0, schedule=sched, device=dev, **kw)
+                                    budget_ms=20.0, schedule=sched, device=dev,
+                                    articulated=args.serve_articulated > 0, **kw)
             log(f"[bench] serving {len(sreqs)} requests at {args.serve_rate:g}/s, {sched} batching")
             res[sched] = seng.run(sreqs, realtime=True, max_wall_s=240.0)
-            res[sched]["final_states"] = [(r.body_q, r.body_qd) for r in sreqs]
+            res[sched]["final_states"] = [(r.body_q, r.body_qd, r.joint_q, r.joint_qd) for r in sreqs]
+            res[sched]["articulated_requests"] = int(sum(r.articulated for r in sreqs))
             log(f"[bench]   {res[sched]['accepted_steps_per_s']:.0f} accepted steps/s, latency p50 "
                 f"{res[sched]['latency_p50_s']:.2f} s p99 {res[sched]['latency_p99_s']:.2f} s")
             del seng
         same = all(a[0] is not None and b[0] is not None and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+                   and (a[2] is None or (np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])))
                    for a, b in zip(res["continuous"].pop("final_states"), res["static"].pop("final_states")))
         out["serving"] = dict(res["continuous"], requests=args.serve_requests, arrival_rate_per_s=args.serve_rate,
                               workload=f"1..64 primitives per world (uneven pile distribution, {args.serve_worlds} "
-                                       "distinct scenes), horizon U[200, 2000] accepted steps, tol in "
-                                       "{1e-2, 1e-3, 1e-4}, Poisson arrivals; continuous batching in 20 ms "
-                                       "device-time rounds",
+                                       f"distinct scenes), {args.serve_articulated:.0%} of the requests articulated "
+                                       "(7-dof arm or 18-dof quadruped on box terrain), horizon U[200, 2000] "
+                                       "accepted steps, tol in {1e-2, 1e-3, 1e-4}, Poisson arrivals; continuous "
+                                       "batching in 20 ms device-time rounds",
                               static=dict(res["static"], form="fixed batches in lock step: one launch per iteration "
                                                               "+ 4-byte readback, batch held to its longest rollout"),
                               served_over_static=res["continuous"]["accepted_steps_per_s"]

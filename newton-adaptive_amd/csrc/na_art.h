@@ -520,52 +520,16 @@ __device__ double art_dynamics(const NaModel& M, const ArtWorld& aw, ArtLDS& A, 
 }
 
 // ---------------------------------------------------------------- dense factorisations
-// In-place Cholesky of the leading n x n block (lower triangle), rows held in registers: lane i
-// owns row i, column j of the factor is broadcast with readlane (lane k's register j is L[k][j]).
-// NM >= n is the unrolled capacity.  Returns false unless every pivot is positive.
+// Column broadcasts of the register factorisation (lane k's register j is L[k][j]).
 __device__ __forceinline__ float bcast(float v, int k) { return lane_f(v, k); }
 __device__ __forceinline__ double bcast(double v, int k) { return lane_d(v, k); }
-template <int NM, typename T>
-__device__ bool art_chol_reg(T (*C)[NA_ART_LD], int n) {
-  const int l = lane_id();
-  T r[NM];
-#pragma unroll
-  for (int k = 0; k < NM; ++k) r[k] = (l < n && k <= l && k < n) ? C[l][k] : (T)0;
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < NM; ++j) {
-    if (j < n) {
-      const T djj = bcast(r[j], j);
-      ok = ok && djj > (T)0;
-      const T d = sqrt(djj > (T)0 ? djj : (T)1);
-      const T id = (T)1 / d;
-      r[j] = l == j ? d : r[j] * id;  // lanes above j hold L[l][j]; the upper triangle is never read
-#pragma unroll
-      for (int k = j + 1; k < NM; ++k) {
-        if (k < n) r[k] -= r[j] * bcast(r[j], k);
-      }
-    }
-  }
-  wave_sync();
-  if (l < n) {
-#pragma unroll
-    for (int k = 0; k < NM; ++k)
-      if (k <= l) C[l][k] = r[k];
-  }
-  wave_sync();
-  return ok;
-}
-template <typename T>
-__device__ bool art_chol(T (*C)[NA_ART_LD], int n) {
-  if (n <= 8) return art_chol_reg<8, T>(C, n);
-  if (n <= 24) return art_chol_reg<24, T>(C, n);
-  return art_chol_reg<32, T>(C, n);
-}
 
-// Factor the leading n x n block of C in registers (as art_chol_reg, the factor is stored back to
-// C) and solve (L L^T) x = b in FP64: the forward sweep uses each lane's own row of L, the backward
-// sweep its column, loaded from LDS in one batch.  Lane i holds b_i and gets x_i; `ok` is false
-// unless every pivot is positive.
+// Factor the leading n x n block of C (lower triangle) in registers -- lane i owns row i, column j
+// of the factor is broadcast with readlane, rows and columns n..NM-1 are padded with the identity
+// so that the NM-step sweeps run without guards -- store the factor back to C, and solve
+// (L L^T) x = b in FP64: the forward sweep uses each lane's own row of L, the backward sweep its
+// column, loaded from LDS in one batch.  Lane i holds b_i and gets x_i; `ok` is false unless every
+// pivot is positive.
 template <int NM, typename T>
 __device__ double art_factor_solve_reg(T (*Cg)[NA_ART_LD], int n, double b, bool& ok) {
   // the matrix lives in LDS: address it as such so that every access is a ds_* instruction
@@ -622,24 +586,6 @@ __device__ double art_factor_solve(T (*C)[NA_ART_LD], int n, double b, bool& ok)
   if (n <= 8) return art_factor_solve_reg<8, T>(C, n, b, ok);
   if (n <= 24) return art_factor_solve_reg<24, T>(C, n, b, ok);
   return art_factor_solve_reg<32, T>(C, n, b, ok);
-}
-
-// x = (L L^T)^-1 b with FP64 substitution over the factor in C; lane i holds b_i, gets x_i.
-template <typename T>
-__device__ double art_chol_solve(const T (*C)[NA_ART_LD], int n, double b) {
-  const int l = lane_id();
-  double x = l < n ? b : 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double xi = lane_d(x, i) / (double)C[i][i];
-    if (l == i) x = xi;
-    else if (l > i && l < n) x -= (double)C[l][i] * xi;
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    const double xi = lane_d(x, i) / (double)C[i][i];
-    if (l == i) x = xi;
-    else if (l < i) x -= (double)C[i][l] * xi;
-  }
-  return x;
 }
 
 // ---------------------------------------------------------------- contacts
