@@ -155,3 +155,93 @@ def test_art_served_frames_match_oracle_driver(na, O):
         sign = np.sign(np.sum(bq[b0:b1, 3:] * ref_bq[:, 3:], axis=1, keepdims=True))
         assert np.abs(bq[b0:b1, :3] - ref_bq[:, :3]).max() <= 1e-5
         assert np.abs(bq[b0:b1, 3:] - sign * ref_bq[:, 3:]).max() <= 1e-5
+
+
+def _floating_boxes(na, worlds, seed, with_pendulum):
+    """Box worlds: each box is a FREE root; with_pendulum adds a far revolute pendulum per world
+    (which makes the model articulated without touching the box)."""
+    b = na.ModelBuilder()
+    for w in range(worlds):
+        rng = np.random.default_rng(seed + w)
+        b.begin_world()
+        qq = na.scenes.random_unit_quaternion(rng)
+        body = b.add_body(xform=(0.0, 0.0, 0.051, *qq)) if not with_pendulum else b.add_link(
+            xform=(0.0, 0.0, 0.051, *qq))
+        b.add_shape_box(body, hx=0.05, hy=0.05, hz=0.05, cfg=na.ModelBuilder.ShapeConfig(ke=1e4, mu=0.5))
+        if with_pendulum:
+            b.add_joint_free(body)
+            far = b.add_link(xform=(5.0, 5.0, 5.0, 0, 0, 0, 1))
+            b.add_shape_sphere(far, radius=0.05)
+            b.add_joint_revolute(-1, far, parent_xform=(5.0, 5.0, 5.0, 0, 0, 0, 1))
+        b.end_world()
+    b.add_ground_plane(cfg=na.ModelBuilder.ShapeConfig(ke=1e4, mu=0.5))
+    return b.finalize(DEV)
+
+
+@pytest.mark.gpu
+def test_floating_base_kernel_equals_free_body_kernel(na, nalib):
+    """A box on a FREE root joint steps through the articulated kernel (dense Cholesky Newton) as
+    the same box does through the free-body kernel (PCG Newton), within the solves' tolerance."""
+    W = 16
+    free = _floating_boxes(na, W, 40, False)
+    art = _floating_boxes(na, W, 40, True)
+    print("floating base vs free body", flush=True)
+    rng = np.random.default_rng(1)
+    v6 = rng.normal(size=(W, 6)) * np.array([0.3, 0.3, 0.5, 1.0, 1.0, 1.0])
+    v6[:, 2] = -np.abs(v6[:, 2])  # moving into the ground
+    fs = na.SolverSAPAdaptive(free, tol=1e-3, optimality_rel_tol=1e-10, max_iterations=200)
+    q64 = free.body_q.double().contiguous()
+    qd = torch.as_tensor(v6, dtype=torch.float32, device=DEV).contiguous()
+    f = torch.zeros_like(qd)
+    qo = torch.zeros_like(q64)
+    vo = torch.zeros_like(qd)
+    st = torch.zeros((W, 6), dtype=torch.int32, device=DEV)
+    hh = torch.full((W,), 2e-3, dtype=torch.float64, device=DEV)
+    nalib.check(nalib.lib().na_substep(fs._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(),
+                                       qo.data_ptr(), vo.data_ptr(), st.data_ptr(), nalib.stream_handle()), "substep")
+    asol = na.SolverSAPAdaptive(art, tol=1e-3, optimality_rel_tol=1e-10, max_iterations=200)
+    qa = art.joint_q.double().cpu().numpy().copy()
+    va = np.zeros(art.joint_dof_count)
+    qs, ds = art.host("joint_q_start"), art.host("joint_qd_start")
+    wb = art.host("world_body_start")
+    for w in range(W):
+        b = int(wb[w])
+        qa[qs[b]:qs[b] + 7] = q64[w].cpu().numpy()
+        va[ds[b]:ds[b] + 6] = qd[w].cpu().numpy()
+    aq, av, ast = _art_substep(nalib, asol, qa, va, 2e-3)
+    torch.cuda.synchronize()
+    assert st[:, 2].min().item() == 1 and ast[:, 2].min() == 1
+    assert np.array_equal(st[:, 3].cpu().numpy(), ast[:, 3]), "contact counts"
+    qo, vo = qo.cpu().numpy(), vo.cpu().numpy()
+    for w in range(W):
+        b = int(wb[w])
+        assert np.abs(aq[qs[b]:qs[b] + 7] - qo[w]).max() <= 1e-6, f"world {w}: pose"
+        assert np.abs(av[ds[b]:ds[b] + 6] - vo[w]).max() <= 1e-4 * max(1.0, np.abs(vo[w]).max()), f"world {w}: velocity"
+
+
+@pytest.mark.gpu
+def test_articulated_model_layout_is_validated(na):
+    """na_model_create rejects articulated layouts the wave-per-world kernels cannot hold."""
+    b = na.ModelBuilder()
+    b.begin_world()
+    parent = -1
+    for k in range(17):  # 17 bodies in one world
+        link = b.add_link()
+        b.add_shape_sphere(link, radius=0.02)
+        b.add_joint_revolute(parent, link, parent_xform=(0, 0, 0.1, 0, 0, 0, 1))
+        parent = link
+    b.end_world()
+    with pytest.raises(RuntimeError, match="more than 16 bodies"):
+        na.SolverSAPAdaptive(b.finalize(DEV))
+    m = na.scenes.articulated_mix(2, seed=1, device=DEV)
+    m.joint_type[3] = 2  # an unsupported joint type
+    with pytest.raises(RuntimeError, match="joint types"):
+        na.SolverSAPAdaptive(m)
+    m = na.scenes.articulated_mix(2, seed=1, device=DEV)
+    m.joint_parent[7] = 8  # quadruped trunk (body 7) attached to a later body
+    with pytest.raises(RuntimeError, match="FREE joint must have the world as parent"):
+        na.SolverSAPAdaptive(m)
+    m = na.scenes.articulated_mix(2, seed=1, device=DEV)
+    m.joint_parent[8] = 9  # a quadruped hip attached to its own (later) knee link
+    with pytest.raises(RuntimeError, match="parent must precede its child"):
+        na.SolverSAPAdaptive(m)
