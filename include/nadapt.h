@@ -17,6 +17,9 @@
  *   na_reset           reset / _reset_worlds             solver_sap_adaptive.py:982-1001, solver_mujoco_adaptive.py:234,912
  *   na_collide         CollisionPipeline.collide         newton/_src/sim/collide.py:920-1250
  *   na_collide_debug   compute_shape_aabbs + broad phase collide.py:150-275, geometry/broad_phase_sap.py:510-677
+ *   na_sort_contacts   make_contact_sort_key + ContactSorter.sort_full (deterministic=True)
+ *                                                     geometry/contact_data.py:59-83, geometry/contact_sort.py:194-420,
+ *                                                     sim/collide.py:1125-1144
  *   na_substep         SolverSAPAdaptive.substep         solver_sap_adaptive.py:714-729
  *   na_attempt         _substep_body (one step-doubling attempt, no commit) solver_sap_adaptive.py:732-849
  *   na_ctrl_*          controller kernels                solver_mujoco_adaptive.py:63-378, solver_sap_adaptive.py:237-345
@@ -196,13 +199,34 @@ int na_coords_from_mujoco(const na_model* m, const float* qpos, const float* qve
    midpoints from the previous call's cur_mid, normals, per-world offsets and unclamped counts).
    Writes match_index for every current contact (>= 0 previous index, -1 new pair, -2 broken),
    the current world midpoints into cur_mid [capacity,3] and prev_matched[k] = 1 for every
-   previous contact that was matched (prev_matched must be zero on entry). */
+   previous contact that was matched (prev_matched must be zero on entry).  Sticky mode
+   (contact_matching="sticky", ContactMatcher.replay_matched, contact_match.py; all four prev_point0/1,
+   prev_offset0/1 [prev_capacity,3] set, else all NULL): a matched contact takes the previous
+   contact's body-frame points, offsets and normal, and its midpoint is recomputed from them. */
 int na_match_contacts(const na_model* m, const float* body_q, const na_contacts_desc* cur, int32_t per_world,
                       const int32_t* cur_offsets, const int32_t* prev_shape0, const int32_t* prev_shape1,
                       const float* prev_mid, const float* prev_normal, const int32_t* prev_offsets,
                       const int32_t* prev_counts, int32_t prev_capacity, float pos_threshold,
                       float normal_dot_threshold, int32_t* match_index, float* cur_mid, int32_t* prev_matched,
-                      void* stream);
+                      const float* prev_point0, const float* prev_point1, const float* prev_offset0,
+                      const float* prev_offset1, void* stream);
+
+/* Deterministic contact order (replaces make_contact_sort_key, geometry/contact_data.py:59-83, and
+   ContactSorter.sort_full, geometry/contact_sort.py:194-420, run by collide.py:1125-1144 when the
+   pipeline is deterministic).  Sorts the first min(*c->count, c->capacity) contacts of `c` in place
+   by the 63-bit key  shape0 << 43 | shape1 << 23 | sub  (20/20/23-bit fields; sub = rank of the
+   contact within its shape pair in production order), over the whole capacity with the sentinel
+   0x7FFFFFFFFFFFFFFF past the count, on `stream` without host synchronization.
+   keys_out [capacity] int64 receives the sorted keys.  Optional (NULL to skip): tids [capacity] and
+   match_index [capacity] are permuted with the contacts; a match index k >= 0 is replaced by
+   match_remap[k] (k < match_remap_len), which maps the previous frame's pre-sort positions to its
+   sorted ones; rank_out [capacity] receives, for every pre-sort position, its sorted position.
+   shape_count bounds the shape ids (<= 2^20 - 1).  workspace: device memory of at least
+   na_sort_contacts_workspace_bytes(capacity) bytes (returns -1 on error). */
+int64_t na_sort_contacts_workspace_bytes(int32_t capacity);
+int na_sort_contacts(const na_contacts_desc* c, int32_t shape_count, int32_t* tids, int32_t* match_index,
+                     const int32_t* match_remap, int32_t match_remap_len, int64_t* keys_out, int32_t* rank_out,
+                     void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
 int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);

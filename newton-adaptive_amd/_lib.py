@@ -83,7 +83,7 @@ SIGNATURES = {
     "na_reset": (C.c_int, [vp, vp, vp]),
     "na_coords_to_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_match_contacts": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, vp, vp,
-                                    vp, vp]),
+                                    vp, vp, vp, vp, vp, vp]),
     "na_coords_from_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
     "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, vp]),
@@ -94,6 +94,8 @@ SIGNATURES = {
     "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_art_attempt": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
+    "na_sort_contacts_workspace_bytes": (C.c_int64, [i32]),
+    "na_sort_contacts": (C.c_int, [C.POINTER(NaContactsDesc), i32, vp, vp, vp, i32, vp, vp, vp, C.c_int64, vp]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
     "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -108,10 +110,11 @@ SIGNATURES = {
 
 
 def build(verbose: bool = False) -> str:
-    """Compile ``csrc/nadapt.hip`` for gfx950 into ``libnadapt.so`` next to this file."""
-    src = os.path.join(_HERE, "csrc", "nadapt.hip")
+    """Compile ``csrc/nadapt.hip`` and ``csrc/na_sort.hip`` for gfx950 into ``libnadapt.so`` next
+    to this file."""
+    srcs = [os.path.join(_HERE, "csrc", f) for f in ("nadapt.hip", "na_sort.hip")]
     tmp = LIB_PATH + ".tmp"
-    cmd = ["hipcc", *HIPCC_FLAGS, "-o", tmp, src]
+    cmd = ["hipcc", *HIPCC_FLAGS, "-o", tmp, *srcs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-4000:]}")

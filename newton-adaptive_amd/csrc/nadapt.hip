@@ -33,6 +33,8 @@ static int fail(const char* msg) {
   } while (0)
 
 extern "C" const char* na_last_error_string(void) { return g_err.c_str(); }
+// error reporting for the other translation units of the library
+int na_set_error(const char* msg) { return fail(msg); }
 
 // ---------------------------------------------------------------- objects
 struct na_model {
@@ -899,7 +901,9 @@ __global__ __launch_bounds__(64) void k_match_contacts(NaModel M, const float* b
                                                        const float* prev_mid, const float* prev_n,
                                                        const int32_t* prev_off, const int32_t* prev_cnt,
                                                        int prev_cap, float pos_thr2, float dot_thr, int32_t* match,
-                                                       float* cur_mid, int32_t* prev_matched) {
+                                                       float* cur_mid, int32_t* prev_matched, const float* prev_p0,
+                                                       const float* prev_p1, const float* prev_o0,
+                                                       const float* prev_o1) {
   __shared__ int ps0[NA_MAX_CONTACTS], ps1[NA_MAX_CONTACTS];
   __shared__ float pm[3][NA_MAX_CONTACTS], pn[3][NA_MAX_CONTACTS];
   __shared__ unsigned long long claim[NA_MAX_CONTACTS];
@@ -957,7 +961,22 @@ __global__ __launch_bounds__(64) void k_match_contacts(NaModel M, const float* b
       r = po + best[g];
       prev_matched[po + best[g]] = 1;
     }
-    match[co + c] = r;
+    const int i = co + c;
+    match[i] = r;
+    if (r >= 0 && prev_p0) {
+      // sticky: a matched contact keeps the previous frame's body-frame geometry and normal
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cur.point0[3 * i + d] = prev_p0[3 * r + d];
+        cur.point1[3 * i + d] = prev_p1[3 * r + d];
+        cur.offset0[3 * i + d] = prev_o0[3 * r + d];
+        cur.offset1[3 * i + d] = prev_o1[3 * r + d];
+        cur.normal[3 * i + d] = prev_n[3 * r + d];
+      }
+      const f3 mid = (contact_world_point(M, body_q, cur.shape0[i], cur.point0 + 3 * i) +
+                      contact_world_point(M, body_q, cur.shape1[i], cur.point1 + 3 * i)) * 0.5f;
+      cur_mid[3 * i] = mid.x; cur_mid[3 * i + 1] = mid.y; cur_mid[3 * i + 2] = mid.z;
+    }
   }
 }
 
@@ -1740,17 +1759,23 @@ extern "C" int na_match_contacts(const na_model* m, const float* body_q, const n
                                  const int32_t* prev_shape1, const float* prev_mid, const float* prev_normal,
                                  const int32_t* prev_offsets, const int32_t* prev_counts, int32_t prev_capacity,
                                  float pos_threshold, float normal_dot_threshold, int32_t* match_index,
-                                 float* cur_mid, int32_t* prev_matched, void* stream) {
+                                 float* cur_mid, int32_t* prev_matched, const float* prev_point0,
+                                 const float* prev_point1, const float* prev_offset0, const float* prev_offset1,
+                                 void* stream) {
   if (!m || !body_q || !cur || !cur_offsets || !prev_shape0 || !prev_shape1 || !prev_mid || !prev_normal ||
       !prev_offsets || !prev_counts || !match_index || !cur_mid || !prev_matched)
     return fail("na_match_contacts: null argument");
   if (per_world < 1 || per_world > NA_MAX_CONTACTS) return fail("na_match_contacts: per-world capacity in [1, 128]");
   if (!(pos_threshold >= 0.f) || !(normal_dot_threshold <= 1.f)) return fail("na_match_contacts: bad thresholds");
+  const int sticky = (prev_point0 != nullptr) + (prev_point1 != nullptr) + (prev_offset0 != nullptr) +
+                     (prev_offset1 != nullptr);
+  if (sticky != 0 && sticky != 4) return fail("na_match_contacts: sticky mode needs all four previous geometry arrays");
   const NaModel& M = m->M;
   k_match_contacts<<<M.W, 64, 0, S_(stream)>>>(M, body_q, *cur, per_world, cur_offsets, prev_shape0, prev_shape1,
                                                prev_mid, prev_normal, prev_offsets, prev_counts, prev_capacity,
                                                pos_threshold * pos_threshold, normal_dot_threshold, match_index,
-                                               cur_mid, prev_matched);
+                                               cur_mid, prev_matched, prev_point0, prev_point1, prev_offset0,
+                                               prev_offset1);
   HIPCHK(hipGetLastError());
   return 0;
 }

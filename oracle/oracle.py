@@ -290,6 +290,35 @@ def coords_from_mujoco(com, qpos, qvel):
     return jq, jqd
 
 
+SORT_KEY_SENTINEL = 0x7FFFFFFFFFFFFFFF
+
+
+def contact_sort_key(shape_a, shape_b, sub):
+    """``make_contact_sort_key`` (geometry/contact_data.py:59-83): shape_a in bits 62..43,
+    shape_b in bits 42..23, the sub key in bits 22..0, each field masked to its width."""
+    a = np.asarray(shape_a, dtype=np.int64) & 0xFFFFF
+    b = np.asarray(shape_b, dtype=np.int64) & 0xFFFFF
+    c = np.asarray(sub, dtype=np.int64) & 0x7FFFFF
+    return (a << 43) | (b << 23) | c
+
+
+def contact_order(shape0: np.ndarray, shape1: np.ndarray):
+    """Deterministic contact order (ContactSorter, geometry/contact_sort.py:194): the sub key of a
+    contact is its rank among the contacts of its shape pair in generated order; contacts are
+    ordered by ascending key.  Returns (keys in generated order, permutation: sorted slot ->
+    generated slot)."""
+    s0 = np.asarray(shape0, dtype=np.int64)
+    s1 = np.asarray(shape1, dtype=np.int64)
+    sub = np.zeros(len(s0), dtype=np.int64)
+    seen = {}
+    for i in range(len(s0)):
+        k = (int(s0[i]), int(s1[i]))
+        sub[i] = seen.get(k, 0)
+        seen[k] = sub[i] + 1
+    keys = contact_sort_key(s0, s1, sub)
+    return keys, np.argsort(keys, kind="stable")
+
+
 def match_contacts(prev: dict, cur: dict, pos_threshold: float, normal_dot_threshold: float) -> np.ndarray:
     """Frame-to-frame matching restated from geometry/contact_match.py (module docstring and
     ContactMatcher.match): per world, a new contact takes the closest previous contact of the same

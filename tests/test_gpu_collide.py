@@ -153,12 +153,19 @@ def test_explicit_broad_phase_contacts_equal_all_pairs(na, scenes, name):
     assert len(out["nxn"]) > 0 and np.array_equal(out["nxn"], out["explicit"])
 
 
+def _generated_order(pipe, c):
+    """Rank of every generated (world-major) contact in the deterministic sorted buffer."""
+    n = min(int(c.rigid_contact_count.item()), pipe.rigid_contact_max)
+    return pipe._prev_rank.cpu().numpy()[:n]  # the rank of the frame just collided
+
+
 def _match_inputs(pipe, m, c, body_q):
-    """Per-world offsets / counts and world midpoints of a compacted Contacts buffer."""
+    """Per-world offsets / counts and world midpoints of a compacted Contacts buffer, in the
+    generated (world-major) order the restatement takes."""
     cnt = np.minimum(c.rigid_contact_world_count.cpu().numpy(), pipe.per_world)
     off = np.concatenate([[0], np.cumsum(cnt)[:-1]])
-    n = int(c.rigid_contact_count.item())
-    s0, s1 = c.rigid_contact_shape0.cpu().numpy()[:n], c.rigid_contact_shape1.cpu().numpy()[:n]
+    rank = _generated_order(pipe, c)
+    s0, s1 = c.rigid_contact_shape0.cpu().numpy()[rank], c.rigid_contact_shape1.cpu().numpy()[rank]
     sb = m.host("shape_body")
     q = body_q.astype(np.float64)
 
@@ -174,21 +181,39 @@ def _match_inputs(pipe, m, c, body_q):
                 out[i] = q[b[i], :3] + R @ p[i]
         return out
 
-    p0 = c.rigid_contact_point0.cpu().numpy()[:n].astype(np.float64)
-    p1 = c.rigid_contact_point1.cpu().numpy()[:n].astype(np.float64)
+    p0 = c.rigid_contact_point0.cpu().numpy()[rank].astype(np.float64)
+    p1 = c.rigid_contact_point1.cpu().numpy()[rank].astype(np.float64)
     mid = 0.5 * (world(s0, p0) + world(s1, p1))
-    return {"off": off, "cnt": cnt, "s0": s0, "s1": s1, "mid": mid,
-            "n": c.rigid_contact_normal.cpu().numpy()[:n].astype(np.float64)}
+    return {"off": off, "cnt": cnt, "s0": s0, "s1": s1, "mid": mid, "rank": rank,
+            "n": c.rigid_contact_normal.cpu().numpy()[rank].astype(np.float64)}
 
 
-def test_contact_matching_against_restatement(na, O):
-    m = na.scenes.uneven_pile(32, seed=8, device=DEV, max_prims=10)
+def _settled_pile(na, worlds=32, seed=8):
+    m = na.scenes.uneven_pile(worlds, seed=seed, device=DEV, max_prims=10)
     state = m.state()
     solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
     for _ in range(40):
         solver.step(state, state, m.control(), None, 0.01)
     torch.cuda.synchronize()
-    pipe = na.CollisionPipeline(m, contact_matching=True, contact_report=True)
+    return m, state
+
+
+def _moved(na, m, state):
+    """One body of every other world moved by 1 mm along x (beyond the 0.5 mm threshold), a second
+    body by 0.1 mm along y (within it)."""
+    q = state.body_q.clone()
+    wb = m.host("world_body_start")
+    for w in range(0, m.world_count, 2):
+        if wb[w + 1] - wb[w] >= 2:
+            q[wb[w], 0] += 1e-3
+            q[wb[w] + 1, 1] += 1e-4
+    return na.State(q, state.body_qd.clone(), state.body_f.clone())
+
+
+def test_contact_matching_against_restatement(na, O):
+    m, state = _settled_pile(na)
+    pipe = na.CollisionPipeline(m, contact_matching="latest", contact_report=True)
+    assert pipe.deterministic  # matching implies the sorted order (collide.py:618-620)
     c = pipe.contacts()
     pipe.collide(state, c)
     torch.cuda.synchronize()
@@ -201,20 +226,15 @@ def test_contact_matching_against_restatement(na, O):
     torch.cuda.synchronize()
     assert np.array_equal(c.rigid_contact_match_index.cpu().numpy()[:n0], np.arange(n0))
     assert int(c.rigid_contact_new_count.item()) == 0 and int(c.rigid_contact_broken_count.item()) == 0
-    # move one body of every other world by 1 mm along x (beyond the 0.5 mm threshold) and a
-    # second body by 0.1 mm (within it)
-    q = state.body_q.clone()
-    wb = m.host("world_body_start")
-    for w in range(0, m.world_count, 2):
-        if wb[w + 1] - wb[w] >= 2:
-            q[wb[w], 0] += 1e-3
-            q[wb[w] + 1, 1] += 1e-4
-    moved = na.State(q, state.body_qd.clone(), state.body_f.clone())
+    moved = _moved(na, m, state)
     pipe.collide(moved, c)
     torch.cuda.synchronize()
     n1 = int(c.rigid_contact_count.item())
-    cur = _match_inputs(pipe, m, c, q.cpu().numpy())
-    ref = O.match_contacts(prev, cur, pipe.pos_threshold, pipe.normal_dot_threshold)
+    cur = _match_inputs(pipe, m, c, moved.body_q.cpu().numpy())
+    ref_gen = O.match_contacts(prev, cur, pipe.pos_threshold, pipe.normal_dot_threshold)
+    # restatement indices are generated-order positions of both frames: map them to sorted rows
+    ref = np.empty(n1, dtype=np.int64)
+    ref[cur["rank"]] = np.where(ref_gen >= 0, prev["rank"][np.maximum(ref_gen, 0)], ref_gen)
     got = c.rigid_contact_match_index.cpu().numpy()[:n1]
     assert (got == -2).sum() > 0 and (got >= 0).sum() > 0
     assert np.array_equal(got, ref)
@@ -222,3 +242,83 @@ def test_contact_matching_against_restatement(na, O):
     assert np.array_equal(new, np.nonzero(got < 0)[0])
     broken = c.rigid_contact_broken_indices.cpu().numpy()[:int(c.rigid_contact_broken_count.item())]
     assert np.array_equal(broken, np.setdiff1d(np.arange(n0), got[got >= 0]))
+
+
+_ROWS = ("rigid_contact_shape0", "rigid_contact_shape1", "rigid_contact_point0", "rigid_contact_point1",
+         "rigid_contact_offset0", "rigid_contact_offset1", "rigid_contact_normal", "rigid_contact_margin0",
+         "rigid_contact_margin1")
+
+
+@pytest.mark.parametrize("name,cap", [("pile", None), ("pile", 500), ("pile64", None), ("stack", None)])
+def test_deterministic_order_is_the_reference_key_order(na, O, scenes, name, cap):
+    """deterministic=True gives the generated contacts, bit for bit, in ascending
+    make_contact_sort_key order (sub key = rank within the shape pair), the sorted keys with the
+    sentinel past the count, and the same rows on every call."""
+    m = scenes[name]
+    state = m.state()
+    plain = na.CollisionPipeline(m, rigid_contact_max=cap)
+    det = na.CollisionPipeline(m, rigid_contact_max=cap, deterministic=True)
+    a, b, b2 = plain.contacts(), det.contacts(), det.contacts()
+    plain.collide(state, a)
+    det.collide(state, b)
+    det.collide(state, b2)
+    torch.cuda.synchronize()
+    n = min(int(a.rigid_contact_count.item()), plain.rigid_contact_max)
+    assert n > 0 and int(b.rigid_contact_count.item()) == int(a.rigid_contact_count.item())
+    if cap is not None:
+        assert int(a.rigid_contact_count.item()) > cap  # the clamped case sorts the stored rows
+    keys, perm = O.contact_order(a.rigid_contact_shape0.cpu().numpy()[:n], a.rigid_contact_shape1.cpu().numpy()[:n])
+    for name_ in _ROWS:
+        ga, gb = getattr(a, name_).cpu().numpy(), getattr(b, name_).cpu().numpy()
+        assert np.array_equal(gb[:n], ga[:n][perm]), name_
+        assert np.array_equal(gb, getattr(b2, name_).cpu().numpy()), name_
+    sk = det._sort_key_array.cpu().numpy()
+    assert np.array_equal(sk[:n], keys[perm]) and np.all(np.diff(sk[:n]) > 0)
+    assert np.all(sk[n:] == O.SORT_KEY_SENTINEL)
+    assert np.array_equal(det._prev_rank.cpu().numpy()[perm], np.arange(n))
+
+
+def test_sticky_matching_replays_previous_geometry(na, O):
+    """contact_matching="sticky": matched rows carry the previous frame's body-frame points,
+    offsets and normal (ContactMatcher.replay_matched); the match indices and the unmatched rows
+    are those of "latest"."""
+    m, state = _settled_pile(na)
+    latest = na.CollisionPipeline(m, contact_matching="latest")
+    sticky = na.CollisionPipeline(m, contact_matching="sticky")
+    cl, cs = latest.contacts(), sticky.contacts()
+    latest.collide(state, cl)
+    sticky.collide(state, cs)
+    torch.cuda.synchronize()
+    prev = {k: getattr(cs, k).cpu().numpy().copy() for k in _ROWS}
+    moved = _moved(na, m, state)
+    latest.collide(moved, cl)
+    sticky.collide(moved, cs)
+    torch.cuda.synchronize()
+    n = int(cs.rigid_contact_count.item())
+    ml = cl.rigid_contact_match_index.cpu().numpy()[:n]
+    ms = cs.rigid_contact_match_index.cpu().numpy()[:n]
+    assert np.array_equal(ml, ms) and (ms >= 0).sum() > 0 and (ms < 0).sum() > 0
+    hit = ms >= 0
+    for k in _ROWS:
+        vs, vl = getattr(cs, k).cpu().numpy()[:n], getattr(cl, k).cpu().numpy()[:n]
+        if k in ("rigid_contact_point0", "rigid_contact_point1", "rigid_contact_offset0", "rigid_contact_offset1",
+                 "rigid_contact_normal"):
+            assert np.array_equal(vs[hit], prev[k][ms[hit]]), k
+            assert np.array_equal(vs[~hit], vl[~hit]), k
+        else:
+            assert np.array_equal(vs, vl), k
+    # the moved bodies' replayed points differ from the freshly generated ones
+    assert not np.array_equal(cs.rigid_contact_point0.cpu().numpy()[:n][hit], cl.rigid_contact_point0.cpu().numpy()[:n][hit])
+
+
+def test_pipeline_argument_checks(na, scenes):
+    m = scenes["drop"]
+    with pytest.raises(ValueError, match="contact_matching must be one of"):
+        na.CollisionPipeline(m, contact_matching="always")
+    with pytest.raises(ValueError, match="requires contact_matching"):
+        na.CollisionPipeline(m, contact_report=True)
+    with pytest.raises(ValueError, match="non-negative"):
+        na.CollisionPipeline(m, contact_matching="latest", contact_matching_pos_threshold=-1.0)
+    det = na.CollisionPipeline(m, deterministic=True)
+    with pytest.raises(ValueError, match="sort buffer size"):
+        det.collide(m.state(), na.Contacts(det.rigid_contact_max + 1, m.device))

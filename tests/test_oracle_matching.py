@@ -40,3 +40,28 @@ def test_matching_stays_inside_a_world(O):
     prev = _frame([(0, 1, 2, (0, 0, 0), Z)], 2)
     cur = _frame([(1, 1, 2, (0, 0, 0), Z)], 2)
     assert list(O.match_contacts(prev, cur, 5e-4, 0.995)) == [-1]
+
+
+def test_sort_key_layout(O):
+    """make_contact_sort_key's documented fields (contact_data.py:62-68): shape_a in bits 62..43,
+    shape_b in 42..23, sub key in 22..0, bit 63 zero, each field masked to its width."""
+    k = O.contact_sort_key
+    assert int(k(1, 0, 0)) == 1 << 43 and int(k(0, 1, 0)) == 1 << 23 and int(k(0, 0, 1)) == 1
+    assert int(k(0xFFFFF, 0xFFFFF, 0x7FFFFF)) == O.SORT_KEY_SENTINEL == (1 << 63) - 1
+    assert int(k(0x100000 + 3, 0x100000 + 4, 0x800000 + 5)) == int(k(3, 4, 5))  # masked, not carried
+    rng = np.random.default_rng(0)
+    a, b, c = rng.integers(0, 1 << 20, 500), rng.integers(0, 1 << 20, 500), rng.integers(0, 1 << 23, 500)
+    a[:50] = a[50:100]  # ties in the leading fields
+    b[:25] = b[50:75]
+    # the integer order of the keys is the lexicographic order of (shape_a, shape_b, sub)
+    assert np.array_equal(np.argsort(k(a, b, c), kind="stable"), np.lexsort((c, b, a)))
+
+
+def test_contact_order_ranks_within_a_pair(O):
+    s0 = np.array([5, 5, 2, 5, 2, 9])
+    s1 = np.array([9, 9, 7, 3, 7, 1])
+    keys, perm = O.contact_order(s0, s1)
+    assert list(keys & 0x7FFFFF) == [0, 1, 0, 0, 1, 0]
+    # (2,7,0) (2,7,1) (5,3,0) (5,9,0) (5,9,1) (9,1,0)
+    assert list(perm) == [2, 4, 3, 0, 1, 5]
+    assert np.all(np.diff(keys[perm]) > 0)
