@@ -405,31 +405,6 @@ __device__ __forceinline__ double eval_contacts(int nc, const ContactLDS& C, Bod
   return nc > NA_WAVE ? eval_contacts_g<2>(nc, C, B, X) : eval_contacts_g<1>(nc, C, B, X);
 }
 
-// B.acc += J^T G J p for p in B.v (B.acc zeroed by the caller, both visible), f32
-template <int NG>
-__device__ __forceinline__ void apply_JGJ_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
-  const int l = lane_id();
-  CRow r[NG];
-  float G[NG][6];
-  f3 u[NG];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    const int c = l + g * NA_WAVE;
-    r[g] = load_row(C, X, c, c < nc);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) G[g][k] = X.G[k][c];
-  }
-#pragma unroll
-  for (int g = 0; g < NG; ++g) u[g] = row_vel(B, r[g]);
-#pragma unroll
-  for (int g = 0; g < NG; ++g) row_scatter(B, r[g], sym3_mul(G[g], u[g]));
-  wave_sync();
-}
-__device__ __forceinline__ void apply_JGJ(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
-  if (nc > NA_WAVE) apply_JGJ_g<2>(nc, C, B, X);
-  else apply_JGJ_g<1>(nc, C, B, X);
-}
-
 // Block-Jacobi accumulation: B.pre[b] += J_b^T G J_b for both sides of every contact.
 template <int NG>
 __device__ __forceinline__ void precond_scatter_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
@@ -512,6 +487,74 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const ContactLDS& C, const
 // the SAP optimality criterion holds.  With cfg.cg_forcing > 0 the PCG tolerance of Newton
 // iteration k is eta_k = min(cg_forcing, sqrt(|Dg_k| / |Dg_0|)), never tighter than needed
 // to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
+// Preconditioned CG on H x = -g * gscale (f32) with the block-Jacobi factor L of this lane's
+// body; stops when the preconditioned residual r.z falls to eta2 (the squared relative
+// tolerance) times its start value, or after max_cg iterations.  The lane's
+// contact rows and G blocks stay in registers for all iterations.  Returns the iteration count.
+template <int NG>
+__device__ __forceinline__ int pcg(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X, const BodyReg& br,
+                                   bool own, const float* L, const double* g, double gscale, float eta2, int max_cg,
+                                   float* x) {
+  const int l = lane_id();
+  CRow rw[NG];
+  float G[NG][6];
+#pragma unroll
+  for (int q = 0; q < NG; ++q) {
+    const int c = l + q * NA_WAVE;
+    rw[q] = load_row(C, X, c, c < nc);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) G[q][k] = X.G[k][c];
+  }
+  float r[6], z[6], pp[6];
+  float rz = 0.f;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { x[k] = 0.f; r[k] = (float)(-g[k] * gscale); z[k] = 0.f; }
+  if (own) chol6_solve(L, r, z);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { pp[k] = z[k]; rz += r[k] * z[k]; }
+  rz = wave_sum(rz);
+  const float tol2 = eta2 * rz;
+  int it = 0;
+  for (; it < max_cg && rz > tol2 && rz > 0.f; ++it) {
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { B.v[q][l] = pp[q]; B.acc[q][l] = 0.f; }
+    }
+    wave_sync();
+    f3 u[NG];
+#pragma unroll
+    for (int q = 0; q < NG; ++q) u[q] = row_vel(B, rw[q]);
+#pragma unroll
+    for (int q = 0; q < NG; ++q) row_scatter(B, rw[q], sym3_mul(G[q], u[q]));
+    wave_sync();
+    float Hp[6];
+    float pHp = 0.f;
+    if (own) {
+      applyA<float>(br, pp, Hp);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) { Hp[q] += B.acc[q][l]; pHp += pp[q] * Hp[q]; }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) Hp[q] = 0.f;
+    }
+    pHp = wave_sum(pHp);
+    if (!(pHp > 0.f)) break;
+    float alpha = rz / pHp;
+    float rzn = 0.f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) { x[q] += alpha * pp[q]; r[q] -= alpha * Hp[q]; }
+    if (own) chol6_solve(L, r, z);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) rzn += r[q] * z[q];
+    rzn = wave_sum(rzn);
+    float beta = rzn / rz;
+    rz = rzn;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) pp[q] = z[q] + beta * pp[q];
+  }
+  return it;
+}
+
 __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X,
                           const BodyReg& br, double* v, SolveStats& st) {
   const int l = lane_id();
@@ -627,48 +670,10 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       eta = fmin(e, fmax(forcing, eta));
     }
     const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
-    float x[6], r[6], z[6], pp[6];
-    float rz = 0.f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) { x[k] = 0.f; r[k] = (float)(-g[k] * gscale); z[k] = 0.f; }
-    if (own) chol6_solve(L, r, z);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) { pp[k] = z[k]; rz += r[k] * z[k]; }
-    rz = wave_sum(rz);
-    const float tol2 = (float)(eta * eta) * rz;
-    for (int k = 0; k < max_cg && rz > tol2 && rz > 0.f; ++k) {
-      if (own) {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) { B.v[q][l] = pp[q]; B.acc[q][l] = 0.f; }
-      }
-      wave_sync();
-      apply_JGJ(nc, C, B, X);
-      float Hp[6];
-      float pHp = 0.f;
-      if (own) {
-        applyA<float>(br, pp, Hp);
-#pragma unroll
-        for (int q = 0; q < 6; ++q) { Hp[q] += B.acc[q][l]; pHp += pp[q] * Hp[q]; }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 6; ++q) Hp[q] = 0.f;
-      }
-      pHp = wave_sum(pHp);
-      if (!(pHp > 0.f)) break;
-      float alpha = rz / pHp;
-      float rzn = 0.f;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) { x[q] += alpha * pp[q]; r[q] -= alpha * Hp[q]; }
-      if (own) chol6_solve(L, r, z);
-#pragma unroll
-      for (int q = 0; q < 6; ++q) rzn += r[q] * z[q];
-      rzn = wave_sum(rzn);
-      float beta = rzn / rz;
-      rz = rzn;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) pp[q] = z[q] + beta * pp[q];
-      st.cg_iterations++;
-    }
+    float x[6];
+    const float eta2 = (float)(eta * eta);
+    if (two) st.cg_iterations += pcg<2>(nc, C, B, X, br, own, L, g, gscale, eta2, max_cg, x);
+    else st.cg_iterations += pcg<1>(nc, C, B, X, br, own, L, g, gscale, eta2, max_cg, x);
     double dx[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) dx[q] = own ? (double)x[q] / gscale : 0.0;
