@@ -186,7 +186,10 @@ int na_serve(na_solver* s, const int32_t* active, int32_t n_active, const int32_
    origin linear velocity (world) + angular velocity (body frame).  Replace
    convert_warp_coords_to_mj_kernel (solvers/mujoco/kernels.py:705-789) and
    convert_mj_coords_to_warp_kernel (kernels.py:578-666) for FREE joints.  Rows of one world are
-   contiguous, so equal-size worlds give MuJoCo's [nworld, nq] layout. */
+   contiguous, so equal-size worlds give MuJoCo's [nworld, nq] layout.
+   Articulated models: qpos / qvel mirror the joint_q / joint_qd layout; a FREE root converts as
+   above with its twist (COM velocity, angular velocity) taken in the joint's parent frame
+   (joint_X_p), revolute coordinates and velocities are copied (no reference offset). */
 int na_coords_to_mujoco(const na_model* m, const float* joint_q, const float* joint_qd, float* qpos, float* qvel,
                         void* stream);
 int na_coords_from_mujoco(const na_model* m, const float* qpos, const float* qvel, float* joint_q, float* joint_qd,

@@ -287,6 +287,61 @@ __global__ void k_coords_from_mujoco(int B, const float* com, const float* qpos,
   u[0] = vc.x; u[1] = vc.y; u[2] = vc.z; u[3] = w.x; u[4] = w.y; u[5] = w.z;
 }
 
+// Articulated models: one joint per body, joint j moves body j.  FREE roots convert as above
+// after rotating the parent-frame twist by the joint's parent frame; revolute coordinates pass
+// through (no reference offset).  qpos / qvel mirror the joint_q / joint_qd layout.
+__global__ void k_coords_to_mujoco_art(int B, const int32_t* jtype, const int32_t* qs, const int32_t* ds,
+                                       const float* Xp, const float* com, const float* jq, const float* jqd,
+                                       float* qpos, float* qvel) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  const int q0 = qs[j], d0 = ds[j];
+  if (jtype[j] != NA_JT_FREE) {
+    qpos[q0] = jq[q0];
+    qvel[d0] = jqd[d0];
+    return;
+  }
+  const float* q = jq + q0;
+  const float* u = jqd + d0;
+  m3 Rp = qmat(Xp[7 * j + 3], Xp[7 * j + 4], Xp[7 * j + 5], Xp[7 * j + 6]);
+  m3 R = qmat(q[3], q[4], q[5], q[6]);
+  f3 vcw = mv(Rp, mk(u[0], u[1], u[2]));
+  f3 w = mv(Rp, mk(u[3], u[4], u[5]));
+  f3 c = mv(R, mk(com[3 * j], com[3 * j + 1], com[3 * j + 2]));
+  f3 vo = vcw - cross3(w, c);
+  f3 wb = mtv(R, w);
+  float* p = qpos + q0;
+  p[0] = q[0]; p[1] = q[1]; p[2] = q[2];
+  p[3] = q[6]; p[4] = q[3]; p[5] = q[4]; p[6] = q[5];
+  float* v = qvel + d0;
+  v[0] = vo.x; v[1] = vo.y; v[2] = vo.z; v[3] = wb.x; v[4] = wb.y; v[5] = wb.z;
+}
+__global__ void k_coords_from_mujoco_art(int B, const int32_t* jtype, const int32_t* qs, const int32_t* ds,
+                                         const float* Xp, const float* com, const float* qpos, const float* qvel,
+                                         float* jq, float* jqd) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= B) return;
+  const int q0 = qs[j], d0 = ds[j];
+  if (jtype[j] != NA_JT_FREE) {
+    jq[q0] = qpos[q0];
+    jqd[d0] = qvel[d0];
+    return;
+  }
+  const float* p = qpos + q0;
+  const float* v = qvel + d0;
+  float* q = jq + q0;
+  q[0] = p[0]; q[1] = p[1]; q[2] = p[2];
+  q[3] = p[4]; q[4] = p[5]; q[5] = p[6]; q[6] = p[3];
+  m3 Rp = qmat(Xp[7 * j + 3], Xp[7 * j + 4], Xp[7 * j + 5], Xp[7 * j + 6]);
+  m3 R = qmat(q[3], q[4], q[5], q[6]);
+  f3 w = mv(R, mk(v[3], v[4], v[5]));
+  f3 c = mv(R, mk(com[3 * j], com[3 * j + 1], com[3 * j + 2]));
+  f3 vc = mk(v[0], v[1], v[2]) + cross3(w, c);
+  f3 vcp = mtv(Rp, vc), wp = mtv(Rp, w);
+  float* u = jqd + d0;
+  u[0] = vcp.x; u[1] = vcp.y; u[2] = vcp.z; u[3] = wp.x; u[4] = wp.y; u[5] = wp.z;
+}
+
 // ---------------------------------------------------------------- world-level device pipeline
 struct WorldCtx {
   int w, b0, nb, nl, nseg;
@@ -1706,18 +1761,30 @@ extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const 
 extern "C" int na_coords_to_mujoco(const na_model* m, const float* joint_q, const float* joint_qd, float* qpos,
                                    float* qvel, void* stream) {
   if (!m || !joint_q || !joint_qd || !qpos || !qvel) return fail("na_coords_to_mujoco: null argument");
-  int B = m->M.B;
+  const NaModel& M = m->M;
+  int B = M.B;
   if (B == 0) return 0;
-  k_coords_to_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, m->M.body_com, joint_q, joint_qd, qpos, qvel);
+  if (m->articulated)
+    k_coords_to_mujoco_art<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, M.joint_type, M.joint_q_start, M.joint_qd_start,
+                                                                   M.joint_X_p, M.body_com, joint_q, joint_qd, qpos,
+                                                                   qvel);
+  else
+    k_coords_to_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, M.body_com, joint_q, joint_qd, qpos, qvel);
   HIPCHK(hipGetLastError());
   return 0;
 }
 extern "C" int na_coords_from_mujoco(const na_model* m, const float* qpos, const float* qvel, float* joint_q,
                                      float* joint_qd, void* stream) {
   if (!m || !joint_q || !joint_qd || !qpos || !qvel) return fail("na_coords_from_mujoco: null argument");
-  int B = m->M.B;
+  const NaModel& M = m->M;
+  int B = M.B;
   if (B == 0) return 0;
-  k_coords_from_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, m->M.body_com, qpos, qvel, joint_q, joint_qd);
+  if (m->articulated)
+    k_coords_from_mujoco_art<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, M.joint_type, M.joint_q_start,
+                                                                     M.joint_qd_start, M.joint_X_p, M.body_com, qpos,
+                                                                     qvel, joint_q, joint_qd);
+  else
+    k_coords_from_mujoco<<<blocks(B, 256), 256, 0, S_(stream)>>>(B, M.body_com, qpos, qvel, joint_q, joint_qd);
   HIPCHK(hipGetLastError());
   return 0;
 }

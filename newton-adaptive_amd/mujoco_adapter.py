@@ -1,9 +1,10 @@
-"""MuJoCo-path adapters: FREE-joint coordinate conversion and contact-parameter mixing.
+"""MuJoCo-path adapters: joint coordinate conversion and contact-parameter mixing.
 
 These are what sits around the MuJoCo inner step of ``SolverMuJoCoAdaptive`` in the reference
 (``_update_mjc_data`` -> ``mujoco_warp.step`` -> ``_update_newton_state``,
-``solver_mujoco.py:4086,3465,4145``), restricted to the free rigid bodies of this build: every
-body carries one FREE joint whose parent frame is the world.
+``solver_mujoco.py:4086,3465,4145``), for the joints of this build: free rigid bodies (one FREE
+joint each, parent frame the world) and articulations of revolute joints under an optional FREE
+root.
 
 * :func:`to_mujoco` / :func:`from_mujoco` run the HIP conversion kernels
   (``na_coords_to_mujoco`` / ``na_coords_from_mujoco``), which replace
@@ -29,14 +30,25 @@ MJ_MINVAL = 2.220446049250313e-16  # constants.py:44
 DEFAULT_SOLIMP = (0.9, 0.95, 0.001, 0.5, 2.0)  # MuJoCo's default geom solimp
 
 
+def _shapes(model):
+    """(rows, qpos width, qvel width): [B, 7] / [B, 6] for free bodies; the flat joint_q / joint_qd
+    sizes for articulated models."""
+    if model.is_articulated:
+        return None, model.joint_coord_count, model.joint_dof_count
+    return model.body_count, 7, 6
+
+
 def to_mujoco(model, state, qpos: torch.Tensor | None = None, qvel: torch.Tensor | None = None):
-    """MuJoCo ``qpos`` [B, 7] (origin, quaternion wxyz) and ``qvel`` [B, 6] (origin velocity in
-    world, angular velocity in the body frame) of ``state``; rows of one world are contiguous."""
-    B = model.body_count
-    qpos = qpos if qpos is not None else torch.empty((B, 7), dtype=torch.float32, device=model.device)
-    qvel = qvel if qvel is not None else torch.empty((B, 6), dtype=torch.float32, device=model.device)
-    _check_rows(B, qpos, 7, "qpos")
-    _check_rows(B, qvel, 6, "qvel")
+    """MuJoCo ``qpos`` (origin, quaternion wxyz) and ``qvel`` (origin velocity in world, angular
+    velocity in the body frame) of ``state``: [B, 7] / [B, 6] for free bodies, rows of one world
+    contiguous; for articulated models the flat ``joint_q`` / ``joint_qd`` layout, FREE roots
+    converted (``kernels.py:705-789``), revolute joints copied."""
+    B, nq, nv = _shapes(model)
+    shp_q, shp_v = ((B, 7), (B, 6)) if B is not None else ((nq,), (nv,))
+    qpos = qpos if qpos is not None else torch.empty(shp_q, dtype=torch.float32, device=model.device)
+    qvel = qvel if qvel is not None else torch.empty(shp_v, dtype=torch.float32, device=model.device)
+    _check_size(qpos, B * nq if B is not None else nq, "qpos")
+    _check_size(qvel, B * nv if B is not None else nv, "qvel")
     h = model_handle(model)
     _lib.check(_lib.lib().na_coords_to_mujoco(h.handle, state.joint_q.data_ptr(), state.joint_qd.data_ptr(),
                                               qpos.data_ptr(), qvel.data_ptr(), _lib.stream_handle()),
@@ -45,19 +57,21 @@ def to_mujoco(model, state, qpos: torch.Tensor | None = None, qvel: torch.Tensor
 
 
 def from_mujoco(model, qpos: torch.Tensor, qvel: torch.Tensor, state) -> None:
-    """Write MuJoCo ``qpos`` / ``qvel`` back into ``state.joint_q`` / ``joint_qd`` (= body_q/qd)."""
-    B = model.body_count
-    _check_rows(B, qpos, 7, "qpos")
-    _check_rows(B, qvel, 6, "qvel")
+    """Write MuJoCo ``qpos`` / ``qvel`` back into ``state.joint_q`` / ``joint_qd``
+    (``kernels.py:578-666``).  For free bodies these are ``body_q`` / ``body_qd``; an articulated
+    state's body poses follow from the coordinates at its next step."""
+    B, nq, nv = _shapes(model)
+    _check_size(qpos, B * nq if B is not None else nq, "qpos")
+    _check_size(qvel, B * nv if B is not None else nv, "qvel")
     h = model_handle(model)
     _lib.check(_lib.lib().na_coords_from_mujoco(h.handle, qpos.data_ptr(), qvel.data_ptr(), state.joint_q.data_ptr(),
                                                 state.joint_qd.data_ptr(), _lib.stream_handle()),
                "na_coords_from_mujoco")
 
 
-def _check_rows(B, t, n, name):
-    if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != B * n:
-        raise ValueError(f"{name} must be a contiguous float32 tensor of {B} x {n}")
+def _check_size(t, n, name):
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n:
+        raise ValueError(f"{name} must be a contiguous float32 tensor of {n} elements")
 
 
 def convert_solref(ke, kd, d_width=1.0, d_r=1.0) -> np.ndarray:

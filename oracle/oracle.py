@@ -419,3 +419,31 @@ class OracleSolver:
                                        _ptr(v, f64p), C.byref(self.ctrl), C.c_double(dt_outer))
 
 
+
+
+def _qrot(q, v, inverse=False):
+    """Rotate v by the unit quaternion q (x, y, z, w); inverse rotates by its conjugate."""
+    x, y, z, w = q
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                  [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                  [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+    return (R.T if inverse else R) @ np.asarray(v, dtype=np.float64)
+
+
+def coords_to_mujoco_joints(joint_type, q_start, qd_start, X_p, com, joint_q, joint_qd):
+    """Joint-wise Newton -> MuJoCo conversion of an articulated model (convert_warp_coords_to_mj_kernel,
+    solvers/mujoco/kernels.py:705-789; one joint per body, joint j moves body j): a FREE joint
+    rotates its parent-frame twist into the world by joint_X_p's rotation and then converts as a
+    free body; a revolute joint copies (no reference offset).  Returns flat (qpos, qvel)."""
+    qpos = np.array(joint_q, dtype=np.float64)
+    qvel = np.array(joint_qd, dtype=np.float64)
+    for j in range(len(joint_type)):
+        if int(joint_type[j]) != 4:
+            continue
+        q0, d0 = int(q_start[j]), int(qd_start[j])
+        qp = np.asarray(X_p[j][3:7], dtype=np.float64)
+        u = np.concatenate([_qrot(qp, joint_qd[d0:d0 + 3]), _qrot(qp, joint_qd[d0 + 3:d0 + 6])])
+        p, v = coords_to_mujoco(np.asarray(com[j]).reshape(1, 3), np.asarray(joint_q[q0:q0 + 7]).reshape(1, 7),
+                                u.reshape(1, 6))
+        qpos[q0:q0 + 7], qvel[d0:d0 + 6] = p[0], v[0]
+    return qpos, qvel

@@ -47,6 +47,23 @@ def test_oracle_round_trip_random(O):
     assert np.allclose(back_q, jq, atol=1e-13) and np.allclose(back_qd, jqd, atol=1e-13)
 
 
+def test_oracle_articulated_joints_closed_form(O):
+    """FREE root under a parent frame turned 90 degrees about z, followed by two revolute joints:
+    the parent-frame COM velocity (1, 0, 0) is (0, 1, 0) in the world; with the COM 0.1 m along
+    world x and spin 2 rad/s about z the origin moves at (0, 1, 0) - (0, 0, 2) x (0.1, 0, 0) =
+    (0, 0.8, 0); revolute entries pass through."""
+    s = math.sqrt(0.5)
+    jt = [4, 1, 1]
+    qs, ds = [0, 7, 8, 9], [0, 6, 7, 8]
+    Xp = np.array([[0, 0, 0, 0, 0, s, s], [0, 0, 0.1, 0, 0, 0, 1], [0, 0, 0.1, 0, 0, 0, 1]], dtype=np.float64)
+    com = np.array([[0.1, 0.0, 0.0], [0.0, 0.0, 0.05], [0.0, 0.0, 0.05]])
+    jq = np.array([0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 1.0, 0.3, -0.7])
+    jqd = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 2.0, 1.5, -2.5])
+    qpos, qvel = O.coords_to_mujoco_joints(jt, qs, ds, Xp, com, jq, jqd)
+    assert np.allclose(qpos, [0.0, 0.0, 0.5, 1.0, 0.0, 0.0, 0.0, 0.3, -0.7], atol=1e-15)
+    assert np.allclose(qvel, [0.0, 0.8, 0.0, 0.0, 0.0, 2.0, 1.5, -2.5], atol=1e-15)
+
+
 def test_convert_solref_formula_and_default():
     r = mj.convert_solref([1.0e4, 0.0, 2.5e3], [100.0, 100.0, -1.0], 1.0, 1.0)
     assert np.allclose(r[0], [0.02, 0.5])  # timeconst 2/(kd w), dampratio kd/2 sqrt(d_r/ke)
@@ -114,3 +131,36 @@ def test_gpu_conversion_matches_oracle_and_round_trips(O):
         1.0, np.abs(state.body_qd.cpu().numpy()).max())
     with pytest.raises(ValueError):
         mj.to_mujoco(model, state, qpos=torch.zeros(3, device=DEV))
+
+
+@pytest.mark.gpu
+def test_gpu_articulated_conversion_matches_oracle_and_round_trips(O):
+    """Arms and floating-base quadrupeds: FREE roots against the joint-wise restatement, revolute
+    coordinates copied bit for bit, and the inverse conversion restores the state."""
+    model = na.scenes.articulated_mix(32, seed=6, device=DEV)
+    state = model.state()
+    rng = np.random.default_rng(6)
+    state.joint_qd.copy_(torch.as_tensor(rng.normal(size=model.joint_dof_count), dtype=torch.float32))
+    qpos, qvel = mj.to_mujoco(model, state)
+    torch.cuda.synchronize()
+    assert qpos.shape == (model.joint_coord_count,) and qvel.shape == (model.joint_dof_count,)
+    jq, jqd = state.joint_q.cpu().numpy(), state.joint_qd.cpu().numpy()
+    jt = model.host("joint_type")
+    rq, rv = O.coords_to_mujoco_joints(jt, model.host("joint_q_start"), model.host("joint_qd_start"),
+                                       model.host("joint_X_p"), model.host("body_com"), jq, jqd)
+    gq, gv = qpos.cpu().numpy(), qvel.cpu().numpy()
+    assert (jt == 4).sum() > 0 and (jt == 1).sum() > 0
+    assert np.abs(gq - rq).max() <= 1e-6
+    assert np.abs(gv - rv).max() <= 2e-6 * max(1.0, np.abs(rv).max())
+    rev_q = np.concatenate([np.arange(a, b) for a, b, t in zip(model.host("joint_q_start")[:-1],
+                                                                model.host("joint_q_start")[1:], jt) if t == 1])
+    rev_d = np.concatenate([np.arange(a, b) for a, b, t in zip(model.host("joint_qd_start")[:-1],
+                                                                model.host("joint_qd_start")[1:], jt) if t == 1])
+    assert np.array_equal(gq[rev_q], jq[rev_q]) and np.array_equal(gv[rev_d], jqd[rev_d])
+    s2 = model.state()
+    mj.from_mujoco(model, qpos, qvel, s2)
+    torch.cuda.synchronize()
+    assert np.array_equal(s2.joint_q.cpu().numpy(), jq)
+    assert np.abs(s2.joint_qd.cpu().numpy() - jqd).max() <= 2e-6 * max(1.0, np.abs(jqd).max())
+    with pytest.raises(ValueError):
+        mj.to_mujoco(model, state, qpos=torch.zeros((model.body_count, 7), device=DEV))
