@@ -244,11 +244,15 @@ int na_collide_debug(const na_model* m, const float* body_q, float* aabb_lo, flo
                      int32_t* pair_count, int32_t max_pairs, int32_t broad_phase, void* stream);
 
 /* One inner step per world (contacts at q, convex solve at h[w], FP64 integration).
+ * guess [B,6] FP64 (NULL: start at qd) seeds the Newton iterate, as `substep(..., guess)` does
+ * (solver_sap_adaptive.py:714-729); a seed that already meets the optimality test takes zero
+ * Newton iterations.  v_solution [B,6] FP64 (NULL to skip) receives the converged velocity.
  * stats [W,6] int32: Newton iterations, CG iterations, converged, contacts, exit reason
  * (0 converged, 1 iteration cap, 2 no descent, 3 preconditioner not SPD, 4 NaN), final
  * relative residual (float bits). */
 int na_substep(const na_solver* s, const double* q64, const float* qd, const float* f, const double* h,
-               double* q64_out, float* qd_out, int32_t* stats, void* stream);
+               const double* guess, double* q64_out, float* qd_out, double* v_solution, int32_t* stats,
+               void* stream);
 /* One step-doubling attempt per world from (q64, qd) with dt[w]: full and doubled results,
  * FP64 error norm, solve status and contact count; no commit, no controller update. */
 int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,

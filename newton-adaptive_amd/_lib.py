@@ -98,7 +98,7 @@ SIGNATURES = {
     "na_sort_contacts": (C.c_int, [C.POINTER(NaContactsDesc), i32, vp, vp, vp, i32, vp, vp, vp, C.c_int64, vp]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
-    "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_attempt": (C.c_int, [vp] + [vp] * 11 + [vp]),
     "na_ctrl_calc_adjusted_step_f32": (C.c_int, [i32, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]),
     "na_ctrl_adapt_dt_f32": (C.c_int, [i32] + [vp] * 11 + [i32, C.c_float, C.c_float, C.c_float, C.c_float, vp]),

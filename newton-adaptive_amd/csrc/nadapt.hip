@@ -378,9 +378,12 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
 }
 
 // One convex step of size h from (q, v0) with the contacts in LDS; integrates into qo/vo.
+// The Newton iterate starts at guess64 when given (FP64), else at guess; v64, when given,
+// receives the converged FP64 velocity.
 __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, int nc,
                             const double* q, const float* v0, const float* f, double h, const float* guess, double* qo,
-                            float* vo, long long* wk, SolveStats* out_st = nullptr) {
+                            float* vo, long long* wk, SolveStats* out_st = nullptr, const double* guess64 = nullptr,
+                            double* v64 = nullptr) {
   const int l = lane_id();
   const bool own = l < wc.nb;
   NA_TIC(t0);
@@ -391,12 +394,16 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
   NA_TOC(wk[7], t0);
   double v[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) v[k] = guess[k];
+  for (int k = 0; k < 6; ++k) v[k] = guess64 ? guess64[k] : (double)guess[k];
   SolveStats st;
   bool ok = sap_solve(wc.nb, nc, cfg, S.con, S.body, S.sol, br, v, st);
   if (own) integrate_body(M, wc.b0 + l, q, v, h, qo);
 #pragma unroll
   for (int k = 0; k < 6; ++k) vo[k] = (float)v[k];
+  if (v64) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v64[k] = v[k];
+  }
   wk[1] += st.iterations;
   wk[2] += st.cg_iterations;
   wk[5] += 1;
@@ -1069,8 +1076,8 @@ __global__ __launch_bounds__(64) void k_collide_debug(NaModel M, int broad_phase
 
 // ---------------------------------------------------------------- parity entry kernels
 __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const double* q64, const float* qd,
-                                                const float* f, const double* h, double* q_out, float* qd_out,
-                                                int32_t* stats) {
+                                                const float* f, const double* h, const double* guess, double* q_out,
+                                                float* qd_out, double* v_solution, int32_t* stats) {
   __shared__ WorldLDS S;
   int w = blockIdx.x;
   WorldCtx wc = world_ctx(M, w);
@@ -1087,13 +1094,23 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
   int ovf = 0;
   int nc = world_collide(M, cfg, wc, S, q, ovf, wk);
   SolveStats st;
-  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st);
-  if (l < wc.nb) {
+  double g64[6], v64[6];
+  const bool own = l < wc.nb;
+  if (guess) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g64[k] = own ? guess[6 * (wc.b0 + l) + k] : 0.0;
+  }
+  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st, guess ? g64 : nullptr, v64);
+  if (own) {
     int gb = wc.b0 + l;
 #pragma unroll
     for (int k = 0; k < 7; ++k) q_out[7 * gb + k] = qo[k];
 #pragma unroll
     for (int k = 0; k < 6; ++k) qd_out[6 * gb + k] = vo[k];
+    if (v_solution) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v_solution[6 * gb + k] = v64[k];
+    }
   }
   if (l == 0) {
     stats[6 * w + 0] = st.iterations;
@@ -1858,10 +1875,11 @@ extern "C" int na_collide_debug(const na_model* m, const float* body_q, float* l
 }
 
 extern "C" int na_substep(const na_solver* s, const double* q64, const float* qd, const float* f, const double* h,
-                          double* q_out, float* qd_out, int32_t* stats, void* stream) {
+                          const double* guess, double* q_out, float* qd_out, double* v_solution, int32_t* stats,
+                          void* stream) {
   if (!s || !q64 || !qd || !h || !q_out || !qd_out || !stats) return fail("na_substep: null argument");
   const NaModel& M = s->model->M;
-  k_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, h, q_out, qd_out, stats);
+  k_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, h, guess, q_out, qd_out, v_solution, stats);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -224,6 +224,55 @@ class _SolverBase:
             f"{type(self).__name__} inner convex solve failed to converge to "
             f"optimality_rel_tol={self.optimality_rel_tol:.3e}.")
 
+    def substep(self, state_in, state_out, control, contacts, dt, guess=None):
+        """ONE inner physics step of every world at its own ``dt`` (``solver_sap_adaptive.py:714-729``):
+        contacts at ``state_in``'s pose, one convex solve, FP64 integration into ``state_out``; no
+        step doubling, no controller.  ``dt``: per-world tensor or scalar [s].  ``guess``: an explicit
+        per-body velocity seed [B, 6] (``None``: start from ``state_in``'s velocity); a seed that
+        already meets the optimality test costs no Newton iteration.  The converged FP64 velocity is
+        kept in ``last_substep_velocity``.  Returns the per-world stats [W, 6] (Newton iterations,
+        CG iterations, converged, contacts, exit reason, residual bits) for free bodies, [W, 4]
+        (Newton iterations, factorizations, converged, contacts) for articulated models."""
+        m = self.model
+        W, dev = m.world_count, m.device
+        h = dt.to(device=dev, dtype=torch.float64).contiguous() if torch.is_tensor(dt) \
+            else torch.full((W,), float(dt), dtype=torch.float64, device=dev)
+        if h.numel() != W:
+            raise ValueError(f"dt must hold one step per world ({W}), got {h.numel()}")
+        s = _lib.stream_handle()
+        if m.is_articulated:
+            if guess is not None:
+                raise ValueError("an explicit velocity guess is not supported for articulated models")
+            q = state_in.joint_q.to(torch.float64).contiguous()
+            v = state_in.joint_qd.to(torch.float64).contiguous()
+            qo, vo = torch.empty_like(q), torch.empty_like(v)
+            stats = torch.zeros((W, 4), dtype=torch.int32, device=dev)
+            _lib.check(_lib.lib().na_art_substep(self._h, q.data_ptr(), v.data_ptr(), h.data_ptr(), qo.data_ptr(),
+                                                 vo.data_ptr(), stats.data_ptr(), s), "na_art_substep")
+            state_out.joint_q.copy_(qo)
+            state_out.joint_qd.copy_(vo)
+            self.last_substep_velocity = vo
+            return stats
+        B = m.body_count
+        q64 = state_in.body_q.to(torch.float64).contiguous()
+        qd = state_in.body_qd.contiguous()
+        bf = getattr(state_in, "body_f", None)
+        f = bf.contiguous() if bf is not None else torch.zeros_like(qd)
+        g = None
+        if guess is not None:
+            g = torch.as_tensor(guess, dtype=torch.float64, device=dev).reshape(B, 6).contiguous()
+        qo = torch.empty((B, 7), dtype=torch.float64, device=dev)
+        vo = torch.empty((B, 6), dtype=torch.float32, device=dev)
+        v64 = torch.empty((B, 6), dtype=torch.float64, device=dev)
+        stats = torch.zeros((W, 6), dtype=torch.int32, device=dev)
+        _lib.check(_lib.lib().na_substep(self._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), h.data_ptr(),
+                                         g.data_ptr() if g is not None else None, qo.data_ptr(), vo.data_ptr(),
+                                         v64.data_ptr(), stats.data_ptr(), s), "na_substep")
+        state_out.body_q.copy_(qo)
+        state_out.body_qd.copy_(vo)
+        self.last_substep_velocity = v64
+        return stats
+
     def step(self, state_in, state_out, control, contacts=None, dt=None, apply_forces=None):
         """Advance every world by ``dt`` (= ``dt_outer``) in place; returns ``(state_in, state_out)``
         (``solver_sap_adaptive.py:962-975``)."""

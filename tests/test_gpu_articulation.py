@@ -197,8 +197,9 @@ def test_floating_base_kernel_equals_free_body_kernel(na, nalib):
     vo = torch.zeros_like(qd)
     st = torch.zeros((W, 6), dtype=torch.int32, device=DEV)
     hh = torch.full((W,), 2e-3, dtype=torch.float64, device=DEV)
-    nalib.check(nalib.lib().na_substep(fs._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(),
-                                       qo.data_ptr(), vo.data_ptr(), st.data_ptr(), nalib.stream_handle()), "substep")
+    nalib.check(nalib.lib().na_substep(fs._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(), None,
+                                       qo.data_ptr(), vo.data_ptr(), None, st.data_ptr(), nalib.stream_handle()),
+                "substep")
     asol = na.SolverSAPAdaptive(art, tol=1e-3, optimality_rel_tol=1e-10, max_iterations=200)
     qa = art.joint_q.double().cpu().numpy().copy()
     va = np.zeros(art.joint_dof_count)

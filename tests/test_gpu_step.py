@@ -40,8 +40,9 @@ def _substep(nalib, solver, q64, qd, f, h):
     vo = torch.zeros((B, 6), dtype=torch.float32, device=DEV)
     st = torch.zeros((W, 6), dtype=torch.int32, device=DEV)
     hh = torch.full((W,), h, dtype=torch.float64, device=DEV)
-    nalib.check(nalib.lib().na_substep(solver._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(),
-                                       qo.data_ptr(), vo.data_ptr(), st.data_ptr(), nalib.stream_handle()), "substep")
+    nalib.check(nalib.lib().na_substep(solver._h, q64.data_ptr(), qd.data_ptr(), f.data_ptr(), hh.data_ptr(), None,
+                                       qo.data_ptr(), vo.data_ptr(), None, st.data_ptr(), nalib.stream_handle()),
+                "substep")
     torch.cuda.synchronize()
     return qo.cpu().numpy(), vo.cpu().numpy(), st.cpu().numpy()
 
@@ -215,3 +216,41 @@ def test_reset_masked_and_invalid_args(na):
         na.SolverSAPAdaptive(model, mode="bogus")
     with pytest.raises(ValueError):
         na.SolverSAPAdaptive(model, tol=0.0)
+
+
+def _resting_spheres(na, worlds=8, radius=0.1):
+    """Four spheres per world pressed into the ground (test_adaptive_warmstart_gate.py:50-62)."""
+    b = na.ModelBuilder()
+    for w in range(worlds):
+        b.begin_world()
+        for i in range(4):
+            body = b.add_body(xform=(0.3 * i, 0.0, radius - 0.01 - 0.002 * w, 0.0, 0.0, 0.0, 1.0))
+            b.add_shape_sphere(body, radius=radius)
+        b.end_world()
+    b.add_ground_plane()
+    return b.finalize(DEV)
+
+
+def test_warm_start_gate(na):
+    """substep(..., guess): a seed at the solve's own converged velocity meets the optimality test
+    before any Newton iteration and is returned unchanged; a seed 0.2 m/s off converges back
+    within 1e-6 (the reference's warm-start gate, test_adaptive_warmstart_gate.py:66-130)."""
+    model = _resting_spheres(na)
+    dt = 1.0 / 60.0
+    solver = na.SolverSAPAdaptive(model, mode="fixed", dt_inner_init=dt, max_substeps=1, max_iterations=80)
+    state, out = model.state(), model.state()
+    state.body_qd[:, 2] = -1.0
+    for _ in range(8):
+        solver.step(state, state, model.control(), None, dt)
+    st0 = solver.substep(state, out, model.control(), None, dt)
+    sol = solver.last_substep_velocity.clone()
+    torch.cuda.synchronize()
+    assert st0[:, 2].min().item() == 1 and st0[:, 3].min().item() >= 4
+    st1 = solver.substep(state, out, model.control(), None, dt, guess=sol)
+    torch.cuda.synchronize()
+    assert st1[:, 0].max().item() == 0 and st1[:, 2].min().item() == 1
+    assert torch.equal(solver.last_substep_velocity, sol)
+    st2 = solver.substep(state, out, model.control(), None, dt, guess=sol + 0.2)
+    torch.cuda.synchronize()
+    assert st2[:, 0].min().item() >= 1 and st2[:, 2].min().item() == 1
+    assert (solver.last_substep_velocity - sol).abs().max().item() <= 1e-6
