@@ -254,3 +254,97 @@ def test_warm_start_gate(na):
     torch.cuda.synchronize()
     assert st2[:, 0].min().item() >= 1 and st2[:, 2].min().item() == 1
     assert (solver.last_substep_velocity - sol).abs().max().item() <= 1e-6
+
+
+def test_box_drop_energy_bound_and_rest(na):
+    """The box drop on the GPU (test_rigid_contact.py:766 box-drop invariant): no box ever moves
+    faster than free fall from its start height allows, sqrt(2 g (z0 - h)) (1% margin for the
+    contact compliance), and after 1000 accepted steps every box rests on a face at COM height =
+    half extent."""
+    m = na.scenes.box_drop(64, seed=0, device=DEV)
+    state = m.state()
+    z0 = state.body_q[:, 2].double().clone()
+    vmax = torch.sqrt(2.0 * 9.81 * (z0 - 0.05).clamp(min=0.0)) * 1.01 + 1e-3
+    solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
+    frames = 0
+    per_world = torch.zeros(m.world_count, dtype=torch.int64, device=DEV)
+    while int(per_world.min().item()) < 1000:
+        solver.step(state, state, m.control(), None, 0.01)
+        per_world += solver.substeps
+        speed = state.body_qd[:, :3].double().norm(dim=1)
+        assert bool((speed <= vmax).all()), f"frame {frames}: speed {speed.max().item():.3f}"
+        frames += 1
+        assert frames < 5000
+    torch.cuda.synchronize()
+    q = state.body_q.cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(q[:, 2] - 0.05) < 2e-3), q[:, 2]
+    assert state.body_qd.abs().max().item() < 1e-3
+    x, y, z, w = q[:, 3], q[:, 4], q[:, 5], q[:, 6]
+    rz = np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=1)
+    assert np.all(np.abs(rz).max(axis=1) > 1 - 1e-4)
+
+
+def _ragged(na, counts, seed=11):
+    """Worlds of counts[w] boxes (0 = an empty world) dropped onto the ground."""
+    b = na.ModelBuilder()
+    cfg = na.ModelBuilder.ShapeConfig(density=1000.0, ke=1.0e4, mu=0.5)
+    for w, n in enumerate(counts):
+        rng = np.random.default_rng(seed + w)
+        b.begin_world()
+        for k in range(n):
+            body = b.add_body(xform=(0.25 * k, 0.0, rng.uniform(0.1, 0.3), *na.scenes.random_unit_quaternion(rng)))
+            b.add_shape_box(body, hx=0.05, hy=0.05, hz=0.05, cfg=cfg)
+        b.end_world()
+    b.add_ground_plane(cfg=na.ModelBuilder.ShapeConfig(ke=1.0e4, mu=0.5))
+    return b.finalize(DEV)
+
+
+def test_empty_and_ragged_worlds(na):
+    """Empty worlds reach every boundary without work, and do not change the others: a model with
+    empty worlds interleaved steps its boxes bit for bit like the model without them."""
+    counts = [1, 0, 3, 0, 0, 2]
+    ragged = _ragged(na, counts)
+    # the non-empty worlds alone, with their per-world seeds
+    dense = na.ModelBuilder()
+    cfg = na.ModelBuilder.ShapeConfig(density=1000.0, ke=1.0e4, mu=0.5)
+    for w, n in enumerate(counts):
+        if not n:
+            continue
+        rng = np.random.default_rng(11 + w)
+        dense.begin_world()
+        for k in range(n):
+            body = dense.add_body(xform=(0.25 * k, 0.0, rng.uniform(0.1, 0.3), *na.scenes.random_unit_quaternion(rng)))
+            dense.add_shape_box(body, hx=0.05, hy=0.05, hz=0.05, cfg=cfg)
+        dense.end_world()
+    dense.add_ground_plane(cfg=na.ModelBuilder.ShapeConfig(ke=1.0e4, mu=0.5))
+    dense = dense.finalize(DEV)
+    assert torch.equal(ragged.body_q, dense.body_q)
+    out = {}
+    for name, m in (("ragged", ragged), ("dense", dense)):
+        s = m.state()
+        solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=128)
+        for _ in range(15):
+            solver.step(s, s, m.control(), None, 0.01)
+            assert np.allclose(solver.sim_time.cpu().numpy(), 0.01, rtol=0, atol=1e-12)
+        torch.cuda.synchronize()
+        out[name] = (s.body_q.cpu().numpy(), s.body_qd.cpu().numpy(), solver.substeps.cpu().numpy())
+    assert np.array_equal(out["ragged"][0], out["dense"][0]) and np.array_equal(out["ragged"][1], out["dense"][1])
+    assert np.array_equal(out["ragged"][2][np.array(counts) > 0], out["dense"][2])
+
+
+def test_contact_overflow_is_flagged_and_deterministic(na):
+    """Worlds with more than 128 candidate contacts (64 primitives with the builder's 0.1 m
+    contact gap) keep their first 128 in the fixed order, flag the overflow and stay finite and
+    run-to-run identical."""
+    m = na.scenes.uneven_pile(4, seed=9, device=DEV, min_prims=64, max_prims=64, gap=0.1)
+    res = []
+    for _ in range(2):
+        s = m.state()
+        solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
+        for _ in range(30):
+            solver.step(s, s, m.control(), None, 0.01)
+        torch.cuda.synchronize()
+        res.append((s.body_q.cpu().numpy(), solver._overflow.cpu().numpy().copy()))
+    assert np.isfinite(res[0][0]).all()
+    assert (res[0][1] > 0).any(), "no world overflowed its 128-contact set"
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
