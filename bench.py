@@ -91,7 +91,8 @@ def articulated_form(na, args, log):
     """Articulated worlds (half 7-dof arms, half floating 12-dof quadrupeds on box terrain): served
     boundary calls (na_step_articulated, one persistent launch per frame) at tol `args.tol`, timed
     over `args.art_frames` frames after as many warm-up frames, with the MFMA work of the contact
-    Hessians (2 (3 nc) nv^2 flops per Newton iteration, nc the contacts of the solve) and the FP64
+    Hessians (2 (3 nc) nv^2 flops per Newton iteration, nc the contacts of the solve); the same
+    frames with a fresh controller as boundary calls and as one served rollout launch; and the FP64
     CPU restatement timed on a sample of the same worlds from the same state."""
     import numpy as np
     import torch
@@ -136,6 +137,59 @@ def articulated_form(na, args, log):
            "mfma": {"kernel": "k_art_frame (contact Hessian J^T G J, v_mfma_f32_32x32x2f32)",
                     "algorithmic_flops": flops, "achieved": tflops, "peak": 157.3, "unit": "TFLOP/s",
                     "frac": tflops / 157.3, "note": "fp32 dense MFMA peak; the solve is latency-bound per wave"}}
+    # the same frames from the same state with a fresh controller, as boundary calls and as one
+    # served rollout launch (na_serve_mixed over an articulated pool, no synchronisation between
+    # frames); the two must agree bit for bit
+    import math
+
+    fresh = na.SolverSAPAdaptive(model, **kw)
+    sf = model.state()
+    sf.joint_q.copy_(q0)
+    sf.joint_qd.copy_(v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(args.art_frames):
+        fresh.step(sf, sf, model.control(), None, args.dt_outer)
+    torch.cuda.synchronize()
+    fwall = time.perf_counter() - t0
+    facc = fresh.accepted_steps()
+    log(f"[bench] articulated boundary calls, fresh controller: {facc} accepted in {fwall:.3f} s ({facc / fwall:.0f}/s)")
+    del fresh
+    q_frames, v_frames = sf.joint_q.cpu().numpy(), sf.joint_qd.cpu().numpy()
+    eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
+                           shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
+                           device="cuda:0", articulated=True, **kw)
+    start = model.state()
+    start.joint_q.copy_(q0)
+    start.joint_qd.copy_(v0)
+    reqs = na.requests_from_model(model, horizon_frames=args.art_frames, state=start)
+    eng.submit(reqs)
+    eng.admit(math.inf)
+    if eng.pending:
+        raise RuntimeError(f"articulated served rollout: {len(eng.pending)} worlds not admitted")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    eng.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    swall = time.perf_counter() - t0
+    eng.retire(swall)
+    sacc = int(sum(r.accepted for r in reqs))
+    qs, ds = model.host("joint_q_start"), model.host("joint_qd_start")
+    wb = model.host("world_body_start")
+    same = sacc == facc and all(
+        np.array_equal(r.joint_q, q_frames[qs[wb[r.rid]]:qs[wb[r.rid + 1]]])
+        and np.array_equal(r.joint_qd, v_frames[ds[wb[r.rid]]:ds[wb[r.rid + 1]]]) for r in reqs)
+    log(f"[bench] articulated served: {sacc} accepted in {swall:.3f} s ({sacc / swall:.0f}/s), "
+        f"identical to the boundary calls: {same}")
+    res["served"] = {"value": sacc / swall, "unit": "accepted world-steps/s", "accepted": sacc, "wall_s": swall,
+                     "launch_ms": e0.elapsed_time(e1), "boundary_calls_fresh_controller": facc / fwall,
+                     "identical_to_boundary_calls": bool(same),
+                     "form": "one persistent rollout launch over the articulated pool (na_serve_mixed), "
+                             "largest worlds first"}
+    del eng
     if not args.no_cpu:
         from oracle import oracle as O
 

@@ -394,8 +394,7 @@ class ServingEngine:
         self.accepted = torch.zeros(W, **i32)
         self.status = torch.zeros(W, **i32)
         self.world_tol = torch.full((W,), self.default_tol, dtype=torch.float64, device=d)
-        self.capacity_bytes = (body_capacity * _bytes_per_body() + shape_capacity * _bytes_per_shape()
-                               + W * _bytes_per_slot())
+        self.capacity_bytes = self.pool_bytes(W, body_capacity, shape_capacity, articulated)
         self.hbm_budget_bytes = int(hbm_budget_bytes) if hbm_budget_bytes is not None else self.capacity_bytes
         self.resident_bytes = 0
         self._bodies = _Arena(body_capacity)
@@ -412,6 +411,16 @@ class ServingEngine:
         self._unfinished_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
 
     # ------------------------------------------------------------------ admission
+    @staticmethod
+    def pool_bytes(worlds: int, body_capacity: int, shape_capacity: int, articulated: bool = False) -> int:
+        """Resident bytes of a full pool, the default admission budget: body, shape and slot arenas,
+        plus the joint, coordinate (7 per body slot) and dof (6 per body slot) arenas of an
+        articulated pool."""
+        b = body_capacity * _bytes_per_body() + shape_capacity * _bytes_per_shape() + worlds * _bytes_per_slot()
+        if articulated:
+            b += body_capacity * (_bytes_per_joint() + 7 * _bytes_per_coord() + 6 * _bytes_per_dof())
+        return b
+
     @staticmethod
     def request_bytes(req: RolloutRequest) -> int:
         b = req.body_count * _bytes_per_body() + req.shape_count * _bytes_per_shape() + _bytes_per_slot()

@@ -72,6 +72,12 @@ def test_articulated_requests_local_indices_and_limits():
         big["joint_qd"] = np.zeros(33, np.float32)
         serving.check_request(serving.RolloutRequest(reqs[1].bodies, reqs[1].shapes, big, horizon_frames=1), 1,
                               1 << 30)
+    # a pool sized to the model holds every request under its default budget
+    full = na.requests_from_model(m, horizon_frames=2)
+    need = sum(serving.ServingEngine.request_bytes(r) for r in full)
+    G = len(m.host("global_shapes"))
+    assert need <= serving.ServingEngine.pool_bytes(m.world_count, m.body_count, m.shape_count - G, True)
+    assert need > serving.ServingEngine.pool_bytes(m.world_count, m.body_count, m.shape_count - G, False)
     model, mixed = na.scenes.serving_requests(200, seed=4, rate=100.0, distinct=40, articulated_share=0.25)
     share = np.mean([r.articulated for r in mixed])
     assert 0.15 < share < 0.35
