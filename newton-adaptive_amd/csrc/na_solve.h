@@ -482,11 +482,6 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const ContactLDS& C, const
   }
 }
 
-// Newton solve with FP64 residuals and FP32 PCG directions.  Lane b < nb owns body b:
-// `br` its constants, `v` its velocity (in: guess, out: solution).  Returns true when
-// the SAP optimality criterion holds.  With cfg.cg_forcing > 0 the PCG tolerance of Newton
-// iteration k is eta_k = min(cg_forcing, sqrt(|Dg_k| / |Dg_0|)), never tighter than needed
-// to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
 // Preconditioned CG on H x = -g * gscale (f32) with the block-Jacobi factor L of this lane's
 // body; stops when the preconditioned residual r.z falls to eta2 (the squared relative
 // tolerance) times its start value, or after max_cg iterations.  The lane's
@@ -555,6 +550,11 @@ __device__ __forceinline__ int pcg(int nc, const ContactLDS& C, BodyLDS& B, cons
   return it;
 }
 
+// Newton solve with FP64 residuals and FP32 PCG directions.  Lane b < nb owns body b:
+// `br` its constants, `v` its velocity (in: guess, out: solution).  Returns true when
+// the SAP optimality criterion holds.  With cfg.cg_forcing > 0 the PCG tolerance of Newton
+// iteration k is eta_k = min(cg_forcing, sqrt(|Dg_k| / |Dg_0|)), never tighter than needed
+// to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
 __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X,
                           const BodyReg& br, double* v, SolveStats& st) {
   const int l = lane_id();

@@ -322,3 +322,27 @@ def test_pipeline_argument_checks(na, scenes):
     det = na.CollisionPipeline(m, deterministic=True)
     with pytest.raises(ValueError, match="sort buffer size"):
         det.collide(m.state(), na.Contacts(det.rigid_contact_max + 1, m.device))
+
+
+def test_deterministic_pipelines_agree_over_a_rollout(na):
+    """Two deterministic pipelines (broad phases nxn and sap) report bit-identical contacts and
+    sort keys on every frame of a 100-frame pile rollout (test_collision_pipeline.py:1665-1800)."""
+    m = na.scenes.uneven_pile(32, seed=12, device=DEV, max_prims=16)
+    state = m.state()
+    solver = na.SolverSAPAdaptive(m, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256)
+    pa = na.CollisionPipeline(m, broad_phase="nxn", deterministic=True)
+    pb = na.CollisionPipeline(m, broad_phase="sap", deterministic=True)
+    ca, cb = pa.contacts(), pb.contacts()
+    seen = 0
+    for frame in range(100):
+        solver.step(state, state, m.control(), None, 0.01)
+        pa.collide(state, ca)
+        pb.collide(state, cb)
+        torch.cuda.synchronize()
+        n = int(ca.rigid_contact_count.item())
+        assert n == int(cb.rigid_contact_count.item()), f"frame {frame}"
+        seen = max(seen, n)
+        for k in _ROWS:
+            assert torch.equal(getattr(ca, k)[:n], getattr(cb, k)[:n]), f"frame {frame}: {k}"
+        assert torch.equal(pa._sort_key_array, pb._sort_key_array), f"frame {frame}: sort keys"
+    assert seen > 0
