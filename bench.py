@@ -216,6 +216,46 @@ def articulated_form(na, args, log):
     return res
 
 
+def collide_form(na, model, state, log, calls=20):
+    """Contact generation through the public API (CollisionPipeline.collide, na_collide) on the
+    settled piles: contacts per second and algorithmic HBM bytes per call (176 B per segment
+    shape, 104 B per candidate pair, 80 B per contact), timed with HIP events over `calls`
+    calls, plain and with the deterministic sort."""
+    import numpy as np
+    import torch
+
+    stream = torch.cuda.current_stream()
+    pipe = na.CollisionPipeline(model)
+    _, _, _, pair_count = pipe.debug_pairs(state, max_pairs=1024)
+    npairs = int(pair_count.sum().item())
+    ns = float(model.shapes_per_world().sum())
+    res = {}
+    for name, det in (("plain", False), ("deterministic", True)):
+        p = na.CollisionPipeline(model, deterministic=det)
+        c = p.contacts()
+        for _ in range(3):
+            p.collide(state, c)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(calls):
+            p.collide(state, c)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / calls
+        n = min(int(c.rigid_contact_count.item()), p.rigid_contact_max)
+        algo = 176.0 * ns + 104.0 * npairs + 80.0 * n
+        res[name] = {"ms_per_call": ms, "contacts": n, "contacts_per_s": n / (ms / 1e3),
+                     "algorithmic_bytes": algo, "achieved_GBps": algo / (ms / 1e3) / 1e9,
+                     "frac_of_8000_GBps": algo / (ms / 1e3) / 8.0e12}
+        log(f"[bench] collide ({name}): {ms:.3f} ms per call, {n} contacts, "
+            f"{algo / (ms / 1e3) / 1e9:.1f} GB/s algorithmic")
+    res["candidate_pairs"] = npairs
+    res["form"] = "CollisionPipeline(model).collide(state, contacts) on the settled piles (sap broad phase)"
+    del pipe
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +277,7 @@ def main():
     ap.add_argument("--cpu-worlds", type=int, default=64)
     ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--no-articulated", action="store_true")
+    ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--articulated-only", action="store_true",
                     help="run only the articulated workload (profiling of k_art_frame); prints its JSON object")
     ap.add_argument("--art-worlds", type=int, default=8192)
@@ -396,6 +437,10 @@ def main():
                               / res["static"]["accepted_steps_per_s"],
                               identical_results=same)
         out["p99_rollout_latency_s"] = res["continuous"]["latency_p99_s"]
+
+    # ---------------------------------------------------------------- contact generation API
+    if not args.no_collide:
+        out["collide"] = collide_form(na, model, na.State(q0, qd0, None), log)
 
     # ---------------------------------------------------------------- articulated worlds
     if not args.no_articulated:
