@@ -437,6 +437,21 @@ def main():
                               / res["static"]["accepted_steps_per_s"],
                               identical_results=same)
         out["p99_rollout_latency_s"] = res["continuous"]["latency_p99_s"]
+        # the same stream with every request waiting at t = 0: the engine's capacity
+        sat = {}
+        for sched in ("continuous", "static"):
+            smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=None,
+                                                       device=dev, distinct=args.serve_worlds,
+                                                       articulated_share=args.serve_articulated)
+            seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
+                                    budget_ms=20.0, schedule=sched, device=dev,
+                                    articulated=args.serve_articulated > 0, **kw)
+            r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
+            sat[sched] = {k: r[k] for k in ("accepted_steps_per_s", "latency_p50_s", "latency_p99_s", "completed")}
+            log(f"[bench] saturated serving, {sched}: {r['accepted_steps_per_s']:.0f} accepted steps/s, latency p50 "
+                f"{r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
+            del seng
+        out["serving"]["saturated"] = dict(sat, form="the same requests all waiting at t = 0 (capacity)")
 
     # ---------------------------------------------------------------- contact generation API
     if not args.no_collide:
