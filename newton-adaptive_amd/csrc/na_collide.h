@@ -121,7 +121,7 @@ __device__ __forceinline__ void sphere_capsule(f3 sp, float sr, f3 cp, f3 ax, fl
   f3 pt = closest_segment_point(cp - seg, cp + seg, sp);
   sphere_sphere(sp, sr, pt, cr, d, pos, n);
 }
-__device__ void capsule_capsule(f3 p1, f3 a1, float r1, float h1, f3 p2, f3 a2, float r2, float h2, Manifold& m) {
+__device__ __forceinline__ void capsule_capsule(f3 p1, f3 a1, float r1, float h1, f3 p2, f3 a2, float r2, float h2, Manifold& m) {
   f3 ax1 = a1 * h1, ax2 = a2 * h2, dif = p1 - p2;
   float ma = dot3(ax1, ax1), mb = -dot3(ax1, ax2), mc = dot3(ax2, ax2);
   float u = -dot3(ax1, dif), v = dot3(ax2, dif);
@@ -145,26 +145,38 @@ __device__ void capsule_capsule(f3 p1, f3 a1, float r1, float h1, f3 p2, f3 a2, 
   m.nrm[0] = n0;
   m.nrm[1] = n0;
 }
-__device__ void plane_box(f3 n, f3 pp, f3 bp, const m3& R, f3 h, float margin, Manifold& m) {
+// The deepest four corners within `margin`, in the slots of an insertion that replaces the
+// shallowest kept corner.  All manifold indices are compile-time constants after unrolling, so
+// the manifold stays in registers.
+__device__ __forceinline__ void plane_box(f3 n, f3 pp, f3 bp, const m3& R, f3 h, float margin, Manifold& m) {
   float cd0 = dot3(bp - pp, n);
   int nc = 0, worst = 0;
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     f3 c = mk((i & 1) ? h.x : -h.x, (i & 2) ? h.y : -h.y, (i & 4) ? h.z : -h.z);
     c = mv(R, c);
     float cd = cd0 + dot3(n, c);
     if (cd > margin) continue;
     f3 cpos = c + bp - n * (0.5f * cd);
+    float dw = m.dist[0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+      if (k == worst) dw = m.dist[k];
     if (nc < 4) {
-      m.dist[nc] = cd;
-      m.pos[nc] = cpos;
-      if (nc == 0 || cd > m.dist[worst]) worst = nc;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k == nc) { m.dist[k] = cd; m.pos[k] = cpos; }
+      if (nc == 0 || cd > dw) worst = nc;
       nc++;
-    } else if (cd < m.dist[worst]) {
-      m.dist[worst] = cd;
-      m.pos[worst] = cpos;
+    } else if (cd < dw) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k == worst) { m.dist[k] = cd; m.pos[k] = cpos; }
       worst = 0;
+      float wd = m.dist[0];
+#pragma unroll
       for (int k = 1; k < 4; ++k)
-        if (m.dist[k] > m.dist[worst]) worst = k;
+        if (m.dist[k] > wd) { wd = m.dist[k]; worst = k; }
     }
   }
 #pragma unroll
@@ -214,7 +226,7 @@ __device__ __forceinline__ float box_sdf(f3 p, f3 h, f3& g) {
 }
 #define NA_CAPBOX_ITERS 32
 #define NA_CAPBOX_PARALLEL 0.2f
-__device__ void capsule_box(f3 cp, f3 ax, float r, float hl, f3 bp, const m3& R, f3 h, Manifold& m) {
+__device__ __forceinline__ void capsule_box(f3 cp, f3 ax, float r, float hl, f3 bp, const m3& R, f3 h, Manifold& m) {
   f3 p0 = mtv(R, (cp - ax * hl) - bp);
   f3 p1 = mtv(R, (cp + ax * hl) - bp);
   f3 d = p1 - p0;
@@ -307,7 +319,7 @@ __device__ __forceinline__ int clip_poly(const f3* in, int n, f3 pn, float off, 
   }
   return m;
 }
-__device__ void box_box(f3 pa, const m3& Ra, f3 ha, f3 pb, const m3& Rb, f3 hb, float gap, Manifold& m) {
+__device__ __forceinline__ void box_box(f3 pa, const m3& Ra, f3 ha, f3 pb, const m3& Rb, f3 hb, float gap, Manifold& m) {
   f3 dv = pb - pa;
   f3 A[3] = {col3(Ra, 0), col3(Ra, 1), col3(Ra, 2)};
   f3 B[3] = {col3(Rb, 0), col3(Rb, 1), col3(Rb, 2)};
@@ -368,21 +380,16 @@ __device__ void box_box(f3 pa, const m3& Ra, f3 ha, f3 pb, const m3& Rb, f3 hb, 
     np = clip_poly(tmp, np, -Rr[ru], -dot3(Rr[ru], pR) + hR[ru], poly);
     np = clip_poly(poly, np, Rr[rw], dot3(Rr[rw], pR) + hR[rw], tmp);
     np = clip_poly(tmp, np, -Rr[rw], -dot3(Rr[rw], pR) + hR[rw], poly);
-    float dd[8];
-    f3 pp[8];
-    int kept = 0;
-    for (int i = 0; i < np; ++i) {
+    // the whole clipped polygon is the manifold (continuous in the poses; a reduction to 4
+    // points would switch between near-equal-depth candidates).  Vertex i goes to slot i, so
+    // the kept contacts stay in polygon order and the manifold indices are constants.
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i >= np) continue;
       float di = dot3(poly[i] - fc, nref);
       if (di > gap) continue;
-      dd[kept] = di;
-      pp[kept] = poly[i] - nref * (0.5f * di);
-      kept++;
-    }
-    // the whole clipped polygon is the manifold (continuous in the poses; a reduction to 4
-    // points would switch between near-equal-depth candidates)
-    for (int i = 0; i < kept; ++i) {
-      m.dist[i] = dd[i];
-      m.pos[i] = pp[i];
+      m.dist[i] = di;
+      m.pos[i] = poly[i] - nref * (0.5f * di);
       m.nrm[i] = n;
     }
   } else {
@@ -668,7 +675,7 @@ struct PairOut {
   float phi0[NA_MF];
   float ra, rb;
 };
-__device__ void narrow_pair(const NaModel& M, const SegLDS& S, int i, int j, PairOut& o) {
+__device__ __forceinline__ void narrow_pair(const NaModel& M, const SegLDS& S, int i, int j, PairOut& o) {
   int ga = S.gidx[i], gb = S.gidx[j];
   int ta = M.shape_type[ga], tb = M.shape_type[gb];
   if (ta > tb) {
