@@ -516,12 +516,16 @@ __device__ __forceinline__ void open_frame(const NaConfig& cfg, WCtrl& c, double
 
 // Controller after an attempt of size `step` with error `err` (ok false: the convex solve
 // failed): records the error, advances time and dt, and returns whether to commit the attempt;
-// `inc` is set to 1 for an accepted step of positive size.
+// `inc` is set to 1 for an accepted step of positive size.  A failed solve counts as a diverged
+// attempt.  The SAP controller also flags it, which ends the frame with the reference's
+// RuntimeError (solver_sap_adaptive.py:901-905); the MuJoCo controller has no such error and
+// handles it like a NaN state: shrink and retry, at the dt floor hold the state and latch
+// `diverged` (solver_mujoco_adaptive.py:120-198).
 __device__ __forceinline__ bool controller_update(const NaConfig& cfg, WCtrl& c, double err, bool ok, double step,
                                                   double eff_max, int& inc) {
   if (!ok) {
     err = cfg.divergence_threshold;
-    c.solve_fail = 1;
+    if (cfg.controller == 0) c.solve_fail = 1;
   }
   c.last_error = err;
   bool commit;

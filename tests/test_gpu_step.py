@@ -348,3 +348,32 @@ def test_contact_overflow_is_flagged_and_deterministic(na):
     assert np.isfinite(res[0][0]).all()
     assert (res[0][1] > 0).any(), "no world overflowed its 128-contact set"
     assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+
+
+def test_nan_world_sap_raises_mujoco_holds_and_latches(na):
+    """A world whose velocity is NaN: the SAP solver raises the reference's RuntimeError
+    (solver_sap_adaptive.py:901-905); the MuJoCo variant has no such error, shrinks the step to the
+    floor, holds the world's state, latches `diverged` and moves its clock to the boundary
+    (solver_mujoco_adaptive.py:120-231), while every other world steps exactly as without it."""
+    model = na.scenes.box_drop(8, seed=2, device=DEV)
+    kw = dict(tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=64)
+    bad = model.state()
+    bad.body_qd[3, 2] = float("nan")
+    with pytest.raises(RuntimeError):
+        na.SolverSAPAdaptive(model, **kw).step(bad, bad, model.control(), None, 0.01)
+    bad = model.state()
+    bad.body_qd[3, 2] = float("nan")
+    good = model.state()
+    q_start = bad.body_q.clone()
+    sb, sg = na.SolverMuJoCoAdaptive(model, **kw), na.SolverMuJoCoAdaptive(model, **kw)
+    for _ in range(3):
+        sb.step(bad, bad, model.control(), None, 0.01)
+        sg.step(good, good, model.control(), None, 0.01)
+        torch.cuda.synchronize()
+        assert bool(sb.diverged[3]) and int(sb.diverged.sum()) == 1
+        assert np.allclose(sb.sim_time.cpu().numpy(), 0.01, rtol=0, atol=1e-12)
+    torch.cuda.synchronize()
+    assert torch.equal(bad.body_q[3], q_start[3])
+    others = [w for w in range(8) if w != 3]
+    assert torch.equal(bad.body_q[others], good.body_q[others])
+    assert torch.equal(bad.body_qd[others], good.body_qd[others])
