@@ -33,7 +33,15 @@
  * test_adaptive_time_rebase.py, test_adaptive_reset_worlds.py; cubic KATs from
  * test_adaptive_cubic_linesearch.py; primitive closed-form tables from
  * test_collision_primitives.py; brute-force pair sets; solver optimality and
- * invariants (free fall closed form, momentum, energy bound, rest).
+ * invariants (free fall closed form, momentum, energy bound, rest); articulated
+ * mechanics identities (host FK, kinetic energy, Coriolis/gravity power, floating base
+ * equal to the free body, PD torque balance, dt^2 error order).
+ *
+ * Parity unpinned (no reference values can be produced here, the dependencies are
+ * absent): the exact manifold points of the reference's box-box / capsule-box MPR
+ * multicontact, the values of the reference's sap_warp convex solve, and the
+ * mujoco_warp inner step behind SolverMuJoCoAdaptive.  Those parts are checked by the
+ * invariants above only.
  */
 #include <math.h>
 #include <stdint.h>

@@ -3,7 +3,12 @@
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py``, always as the checker, never as the product path.
 ``liboracle64.so`` is the FP64 restatement; timed on the host it is also the CPU baseline
-("port").
+("port").  The Python-side restatements here (contact matching, the contact sort key and order,
+the joint-wise MuJoCo coordinate conversion) are pinned by hand-made cases and closed forms in
+``tests/test_oracle_matching.py`` and ``tests/test_mujoco_adapter.py``.  Parity unpinned, for
+want of the absent dependencies: the exact manifold points of the reference's box-box /
+capsule-box MPR multicontact, the values of its ``sap_warp`` convex solve and its
+``mujoco_warp`` inner step (checked by invariants only).
 """
 
 from __future__ import annotations
