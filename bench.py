@@ -193,7 +193,7 @@ def articulated_form(na, args, log):
     if not args.no_cpu:
         from oracle import oracle as O
 
-        n = min(2048, W)
+        n = W  # the whole workload: the restatement runs it in about a second
         m = na.scenes.articulated_mix(n, seed=args.seed, device="cpu")
         om, oa = O.OracleModel(m), O.OracleArt(m)
         cfg = O.make_config(tol=args.tol, dt_init=KW["dt_inner_init"], dt_max=args.dt_outer,
@@ -274,7 +274,7 @@ def main():
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
     ap.add_argument("--serve-articulated", type=float, default=0.25,
                     help="share of requests that are articulated worlds (arm or quadruped on terrain)")
-    ap.add_argument("--cpu-worlds", type=int, default=64)
+    ap.add_argument("--cpu-worlds", type=int, default=256)
     ap.add_argument("--cpu-frames", type=int, default=10)
     ap.add_argument("--no-articulated", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
