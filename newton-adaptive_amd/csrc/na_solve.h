@@ -54,7 +54,6 @@ struct SolveLDS {
   float Rn[NA_MAX_CONTACTS], Rt[NA_MAX_CONTACTS], vhat[NA_MAX_CONTACTS];
   double iRn[NA_MAX_CONTACTS], iRt[NA_MAX_CONTACTS];  // FP64 reciprocals of Rn, Rt
   double vc0[3][NA_MAX_CONTACTS];
-  double dvc[3][NA_MAX_CONTACTS];
   float G[6][NA_MAX_CONTACTS];  // symmetric 3x3: 00 11 22 01 02 12
 };
 
@@ -367,7 +366,7 @@ __device__ void contact_setup(int nc, float h, const NaConfig& cfg, const Contac
 
 // Contact passes.  Lane l handles contacts l and l + 64 (at most NA_MAX_CONTACTS = 128); the
 // NG = 1 or 2 groups are loaded together before any arithmetic, so one wait covers all
-// gathers.  Later passes over X.vc0 / X.dvc keep this assignment and need no barrier.
+// gathers.  Later passes over X.vc0 keep this assignment and need no barrier.
 static_assert(NA_MAX_CONTACTS <= 2 * NA_WAVE, "contact passes assume at most two contacts per lane");
 
 // FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
@@ -426,30 +425,44 @@ __device__ __forceinline__ void precond_scatter_g(int nc, const ContactLDS& C, B
   wave_sync();
 }
 
-// Line-search setup: contact-space direction X.dvc = J dx (dx in B.vd) and this lane's contact
-// terms at alpha = 0 (slope d1c0, curvature d2c0) and alpha = 1 (cost lc1, slope d1c1).
+// One contact's line-search constants, held in registers for every trial step of a Newton
+// iteration: start velocity w0 and direction dv in the contact frame, the regularisation and
+// its reciprocals, friction and the stabilisation velocity.
+struct LsRow {
+  d3 w0, dv;
+  double iRt, iRn, Rt, Rn, mu, vh;
+};
+
+// Line-search setup: contact-space direction dv = J dx (dx in B.vd) into this lane's rows `lr`,
+// and the lane's contact terms at alpha = 0 (slope d1c0, curvature d2c0) and alpha = 1 (cost
+// lc1, slope d1c1).
 template <int NG>
-__device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const BodyLDS& B, SolveLDS& X, double& d1c0,
-                                           double& d2c0, double& lc1, double& d1c1) {
+__device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const BodyLDS& B, const SolveLDS& X,
+                                           LsRow* lr, double& d1c0, double& d2c0, double& lc1, double& d1c1) {
   const int l = lane_id();
   CRow r[NG];
-  d3 u[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) r[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
 #pragma unroll
-  for (int g = 0; g < NG; ++g) u[g] = row_vel_d(B, r[g]);
+  for (int g = 0; g < NG; ++g) lr[g].dv = row_vel_d(B, r[g]);
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     const int c = l + g * NA_WAVE;
-    X.dvc[0][c] = u[g].x; X.dvc[1][c] = u[g].y; X.dvc[2][c] = u[g].z;
-    double iRt = X.iRt[c], iRn = X.iRn[c], Rt = X.Rt[c], Rn = X.Rn[c], mu = C.mu[c];
-    double vh = (double)X.vhat[c];
-    d3 w0 = d3{X.vc0[0][c], X.vc0[1][c], X.vc0[2][c]};
-    d3 y0 = d3{-w0.x * iRt, -w0.y * iRt, -(w0.z - vh) * iRn};
-    d3 y1 = d3{-(w0.x + u[g].x) * iRt, -(w0.y + u[g].y) * iRt, -(w0.z + u[g].z - vh) * iRn};
+    LsRow& q = lr[g];
+    q.iRt = X.iRt[c]; q.iRn = X.iRn[c]; q.Rt = X.Rt[c]; q.Rn = X.Rn[c]; q.mu = C.mu[c];
+    q.vh = (double)X.vhat[c];
+    q.w0 = d3{X.vc0[0][c], X.vc0[1][c], X.vc0[2][c]};
+  }
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int c = l + g * NA_WAVE;
+    const LsRow& q = lr[g];
+    const d3 w0 = q.w0, u = q.dv;
+    d3 y0 = d3{-w0.x * q.iRt, -w0.y * q.iRt, -(w0.z - q.vh) * q.iRn};
+    d3 y1 = d3{-(w0.x + u.x) * q.iRt, -(w0.y + u.y) * q.iRt, -(w0.z + u.z - q.vh) * q.iRn};
     double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-    sap_line(mu, Rt, Rn, iRt, iRn, y0, u[g], a1, a2);
-    double cb = sap_line(mu, Rt, Rn, iRt, iRn, y1, u[g], b1, b2);
+    sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
+    double cb = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y1, u, b1, b2);
     if (c < nc) {
       d1c0 += a1;
       d2c0 += a2;
@@ -459,21 +472,21 @@ __device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const Bo
   }
 }
 
-// Contact part of the line function at alpha for this lane's contacts (FP64): adds the cost,
-// its first and second derivatives.
+// Contact part of the line function at alpha for this lane's contacts (FP64), from the rows
+// that ls_setup_g left in registers: adds the cost, its first and second derivatives.
 template <int NG>
-__device__ __forceinline__ void ls_contacts_g(int nc, const ContactLDS& C, const SolveLDS& X, double alpha, double& l,
-                                              double& dl, double& d2l) {
+__device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double alpha, double& l, double& dl,
+                                              double& d2l) {
   const int ln = lane_id();
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     const int c = ln + g * NA_WAVE;
-    d3 dv = d3{X.dvc[0][c], X.dvc[1][c], X.dvc[2][c]};
-    double iRt = X.iRt[c], iRn = X.iRn[c];
-    d3 y = d3{-(X.vc0[0][c] + alpha * dv.x) * iRt, -(X.vc0[1][c] + alpha * dv.y) * iRt,
-              -(X.vc0[2][c] + alpha * dv.z - (double)X.vhat[c]) * iRn};
+    const LsRow& q = lr[g];
+    const d3 dv = q.dv;
+    d3 y = d3{-(q.w0.x + alpha * dv.x) * q.iRt, -(q.w0.y + alpha * dv.y) * q.iRt,
+              -(q.w0.z + alpha * dv.z - q.vh) * q.iRn};
     double a = 0.0, b = 0.0;
-    double cc = sap_line(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, dv, a, b);
+    double cc = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y, dv, a, b);
     if (c < nc) {
       l += cc;
       dl += a;
@@ -693,8 +706,9 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     }
     wave_sync();
     double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
-    if (two) ls_setup_g<2>(nc, C, B, X, d1c0, d2c0, lc1, d1c1);
-    else ls_setup_g<1>(nc, C, B, X, d1c0, d2c0, lc1, d1c1);
+    LsRow lr[2];
+    if (two) ls_setup_g<2>(nc, C, B, X, lr, d1c0, d2c0, lc1, d1c1);
+    else ls_setup_g<1>(nc, C, B, X, lr, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
@@ -723,8 +737,8 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
       for (int k = 0; k < 60; ++k) {
         double lc = 0.0, dlc = 0.0, d2c = 0.0;
-        if (two) ls_contacts_g<2>(nc, C, X, alpha, lc, dlc, d2c);
-        else ls_contacts_g<1>(nc, C, X, alpha, lc, dlc, d2c);
+        if (two) ls_contacts_g<2>(nc, lr, alpha, lc, dlc, d2c);
+        else ls_contacts_g<1>(nc, lr, alpha, lc, dlc, d2c);
         double dl = b0 + alpha * aa + wave_sum_d(dlc);
         double ddl = aa + wave_sum_d(d2c);
         if (fabs(dl) <= ls_rel * fabs(dl0)) break;

@@ -16,7 +16,8 @@ Size-independent properties, plus the FP64 oracle on the whole batch at the end 
   the 64-world attempt tests (error norm 1e-6, doubled pose 1e-5).
 * 4096 uneven piles after the 20 warm-up frames of the throughput workload: the persistent
   per-frame form and the reference's per-iteration host loop give bit-identical states, and an
-  attempt over all 4096 worlds matches the oracle to the same bars.
+  attempt over all 4096 worlds matches the oracle to the same bars.  The oracle runs with the
+  solver's Newton cap of 100: under the reference's 30 it stops short on one of these worlds.
 * 8192 articulated worlds (arms and floating quadrupeds on box terrain) after 10 frames: an
   attempt over all worlds matches the oracle to the articulated bars (error norm within
   1e-7 + 1e-3 e, doubled coordinates 1e-6) wherever both found the same contacts.
@@ -46,8 +47,11 @@ def _attempt_matches_oracle(O, model, solver, q64, qd, tol, dts):
     for dt in dts:
         dtt = torch.full((model.world_count,), dt, dtype=torch.float64, device=DEV)
         out = GE._attempt(solver, q64, qd, f, dtt)
-        ref = O.attempt_all(om, O.make_config(tol=tol), q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64),
-                            f.cpu().numpy(), dtt.cpu().numpy())
+        # the oracle gets the solver's Newton cap (100); the reference's 30 is too few for some
+        # settled-pile worlds at dt 4e-3
+        cfg = O.make_config(tol=tol, max_iterations=solver._cfg.max_iterations)
+        ref = O.attempt_all(om, cfg, q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64), f.cpu().numpy(),
+                            dtt.cpu().numpy())
         assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
         same_nc = out["n_contacts"] == ref["n_contacts"]
         assert same_nc.mean() > 0.95, same_nc.mean()
