@@ -53,8 +53,6 @@ struct SolveLDS {
   float rb[3][NA_MAX_CONTACTS];
   float Rn[NA_MAX_CONTACTS], Rt[NA_MAX_CONTACTS], vhat[NA_MAX_CONTACTS];
   double iRn[NA_MAX_CONTACTS], iRt[NA_MAX_CONTACTS];  // FP64 reciprocals of Rn, Rt
-  double vc0[3][NA_MAX_CONTACTS];
-  float G[6][NA_MAX_CONTACTS];  // symmetric 3x3: 00 11 22 01 02 12
 };
 
 // The segment scratch is only live during contact generation and the per-contact solver
@@ -366,19 +364,17 @@ __device__ void contact_setup(int nc, float h, const NaConfig& cfg, const Contac
 
 // Contact passes.  Lane l handles contacts l and l + 64 (at most NA_MAX_CONTACTS = 128); the
 // NG = 1 or 2 groups are loaded together before any arithmetic, so one wait covers all
-// gathers.  Later passes over X.vc0 keep this assignment and need no barrier.
+// gathers.  Values a pass keeps in registers for later passes follow the same assignment.
 static_assert(NA_MAX_CONTACTS <= 2 * NA_WAVE, "contact passes assume at most two contacts per lane");
 
 // FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
-// lanes): gamma, G (f32), J^T gamma into B.accd, contact velocities into X.vc0.  Returns this
-// lane's part of the contact cost.
+// lanes) for this lane's contact rows `r`: J^T gamma into B.accd, the f32 blocks G into `G`
+// and the contact velocities into `vc`, both in registers.  Returns this lane's part of the
+// contact cost.
 template <int NG>
-__device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X) {
+__device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X,
+                                                  const CRow* r, float (*G)[6], d3* vc) {
   const int l = lane_id();
-  CRow r[NG];
-  d3 vc[NG];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) r[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
 #pragma unroll
   for (int g = 0; g < NG; ++g) vc[g] = row_vel_d(B, r[g]);
   double cost = 0.0;
@@ -386,37 +382,20 @@ __device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, B
   for (int g = 0; g < NG; ++g) {
     const int c = l + g * NA_WAVE;
     const bool valid = c < nc;
-    X.vc0[0][c] = vc[g].x; X.vc0[1][c] = vc[g].y; X.vc0[2][c] = vc[g].z;
     double iRt = X.iRt[c], iRn = X.iRn[c];
     d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)X.vhat[c]) * iRn};
     d3 gam;
-    float G[6];
-    double cc = sap_project(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, gam, G);
+    double cc = sap_project(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, gam, G[g]);
     cost += valid ? cc : 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) X.G[k][c] = G[k];
     row_scatter_d(B, r[g], gam);
   }
   wave_sync();
   return cost;
 }
-__device__ __forceinline__ double eval_contacts(int nc, const ContactLDS& C, BodyLDS& B, SolveLDS& X) {
-  return nc > NA_WAVE ? eval_contacts_g<2>(nc, C, B, X) : eval_contacts_g<1>(nc, C, B, X);
-}
 
 // Block-Jacobi accumulation: B.pre[b] += J_b^T G J_b for both sides of every contact.
 template <int NG>
-__device__ __forceinline__ void precond_scatter_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X) {
-  const int l = lane_id();
-  CRow r[NG];
-  float G[NG][6];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    const int c = l + g * NA_WAVE;
-    r[g] = load_row(C, X, c, c < nc);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) G[g][k] = X.G[k][c];
-  }
+__device__ __forceinline__ void precond_scatter_g(BodyLDS& B, const CRow* r, const float (*G)[6]) {
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     if (r[g].ha) scatter_block(B, r[g].sa, r[g], r[g].ra, G[g]);
@@ -438,11 +417,9 @@ struct LsRow {
 // lc1, slope d1c1).
 template <int NG>
 __device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const BodyLDS& B, const SolveLDS& X,
-                                           LsRow* lr, double& d1c0, double& d2c0, double& lc1, double& d1c1) {
+                                           const CRow* r, const d3* vc0, LsRow* lr, double& d1c0, double& d2c0,
+                                           double& lc1, double& d1c1) {
   const int l = lane_id();
-  CRow r[NG];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) r[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
 #pragma unroll
   for (int g = 0; g < NG; ++g) lr[g].dv = row_vel_d(B, r[g]);
 #pragma unroll
@@ -451,7 +428,7 @@ __device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const Bo
     LsRow& q = lr[g];
     q.iRt = X.iRt[c]; q.iRn = X.iRn[c]; q.Rt = X.Rt[c]; q.Rn = X.Rn[c]; q.mu = C.mu[c];
     q.vh = (double)X.vhat[c];
-    q.w0 = d3{X.vc0[0][c], X.vc0[1][c], X.vc0[2][c]};
+    q.w0 = vc0[g];
   }
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
@@ -498,21 +475,11 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double al
 // Preconditioned CG on H x = -g * gscale (f32) with the block-Jacobi factor L of this lane's
 // body; stops when the preconditioned residual r.z falls to eta2 (the squared relative
 // tolerance) times its start value, or after max_cg iterations.  The lane's
-// contact rows and G blocks stay in registers for all iterations.  Returns the iteration count.
+// contact rows `rw` and G blocks `G` are the caller's registers.  Returns the iteration count.
 template <int NG>
-__device__ __forceinline__ int pcg(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X, const BodyReg& br,
-                                   bool own, const float* L, const double* g, double gscale, float eta2, int max_cg,
-                                   float* x) {
+__device__ __forceinline__ int pcg(BodyLDS& B, const CRow* rw, const float (*G)[6], const BodyReg& br, bool own,
+                                   const float* L, const double* g, double gscale, float eta2, int max_cg, float* x) {
   const int l = lane_id();
-  CRow rw[NG];
-  float G[NG][6];
-#pragma unroll
-  for (int q = 0; q < NG; ++q) {
-    const int c = l + q * NA_WAVE;
-    rw[q] = load_row(C, X, c, c < nc);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) G[q][k] = X.G[k][c];
-  }
   float r[6], z[6], pp[6];
   float rz = 0.f;
 #pragma unroll
@@ -568,8 +535,11 @@ __device__ __forceinline__ int pcg(int nc, const ContactLDS& C, BodyLDS& B, cons
 // the SAP optimality criterion holds.  With cfg.cg_forcing > 0 the PCG tolerance of Newton
 // iteration k is eta_k = min(cg_forcing, sqrt(|Dg_k| / |Dg_0|)), never tighter than needed
 // to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
-__device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, SolveLDS& X,
-                          const BodyReg& br, double* v, SolveStats& st) {
+// NG (1 or 2) is the number of contacts per lane; each lane's contact rows, G blocks and
+// contact velocities stay in registers for the whole solve.
+template <int NG>
+__device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B,
+                                            const SolveLDS& X, const BodyReg& br, double* v, SolveStats& st) {
   const int l = lane_id();
   const bool own = l < nb;
   st.iterations = 0;
@@ -583,11 +553,15 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
   const int max_iterations = cfg.max_iterations, max_cg = cfg.max_cg;
   const double opt_abs = cfg.opt_abs_tol, opt_rel = cfg.opt_rel_tol, ls_rel = cfg.ls_rel_tol;
   const double cg_rel = cfg.cg_rel_tol, forcing = cfg.cg_forcing;
-  const bool two = nc > NA_WAVE;
   if (l == 0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) B.v[k][NA_DUMMY] = 0.f;
   }
+  CRow rows[NG];
+  float Gr[NG][6];
+  d3 vc0[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) rows[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
   double gn_first = 0.0;
   for (int it = 0; it <= max_iterations; ++it) {
     // gradient g = A (v - v*) - J^T gamma, FP64.  The owner lane also clears its
@@ -602,7 +576,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double ccost = eval_contacts(nc, C, B, X);
+    double ccost = eval_contacts_g<NG>(nc, C, B, X, rows, Gr, vc0);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -654,8 +628,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       for (int k = 0; k < 21; ++k) B.pre[k][l] = 0.f;
     }
     wave_sync();
-    if (two) precond_scatter_g<2>(nc, C, B, X);
-    else precond_scatter_g<1>(nc, C, B, X);
+    precond_scatter_g<NG>(B, rows, Gr);
     float L[21];
     bool spd = true;
     if (own) {
@@ -685,8 +658,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
     float x[6];
     const float eta2 = (float)(eta * eta);
-    if (two) st.cg_iterations += pcg<2>(nc, C, B, X, br, own, L, g, gscale, eta2, max_cg, x);
-    else st.cg_iterations += pcg<1>(nc, C, B, X, br, own, L, g, gscale, eta2, max_cg, x);
+    st.cg_iterations += pcg<NG>(B, rows, Gr, br, own, L, g, gscale, eta2, max_cg, x);
     double dx[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) dx[q] = own ? (double)x[q] / gscale : 0.0;
@@ -706,9 +678,8 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     }
     wave_sync();
     double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
-    LsRow lr[2];
-    if (two) ls_setup_g<2>(nc, C, B, X, lr, d1c0, d2c0, lc1, d1c1);
-    else ls_setup_g<1>(nc, C, B, X, lr, d1c0, d2c0, lc1, d1c1);
+    LsRow lr[NG];
+    ls_setup_g<NG>(nc, C, B, X, rows, vc0, lr, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
@@ -737,8 +708,7 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
       for (int k = 0; k < 60; ++k) {
         double lc = 0.0, dlc = 0.0, d2c = 0.0;
-        if (two) ls_contacts_g<2>(nc, lr, alpha, lc, dlc, d2c);
-        else ls_contacts_g<1>(nc, lr, alpha, lc, dlc, d2c);
+        ls_contacts_g<NG>(nc, lr, alpha, lc, dlc, d2c);
         double dl = b0 + alpha * aa + wave_sum_d(dlc);
         double ddl = aa + wave_sum_d(d2c);
         if (fabs(dl) <= ls_rel * fabs(dl0)) break;
@@ -758,6 +728,11 @@ __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS&
     st.iterations = it + 1;
   }
   return st.converged;
+}
+
+__device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, const SolveLDS& X,
+                          const BodyReg& br, double* v, SolveStats& st) {
+  return nc > NA_WAVE ? sap_solve_g<2>(nb, nc, cfg, C, B, X, br, v, st) : sap_solve_g<1>(nb, nc, cfg, C, B, X, br, v, st);
 }
 
 // ---------------------------------------------------------------- body setup and integration
