@@ -367,13 +367,23 @@ __device__ void contact_setup(int nc, float h, const NaConfig& cfg, const Contac
 // gathers.  Values a pass keeps in registers for later passes follow the same assignment.
 static_assert(NA_MAX_CONTACTS <= 2 * NA_WAVE, "contact passes assume at most two contacts per lane");
 
+// One contact's constants for a solve (regularisation, its FP64 reciprocals, stabilisation
+// velocity, friction), loaded once into registers.
+struct CConst {
+  double iRt, iRn;
+  float Rt, Rn, vh, mu;
+};
+__device__ __forceinline__ CConst load_const(const ContactLDS& C, const SolveLDS& X, int c) {
+  return CConst{X.iRt[c], X.iRn[c], X.Rt[c], X.Rn[c], X.vhat[c], C.mu[c]};
+}
+
 // FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
 // lanes) for this lane's contact rows `r`: J^T gamma into B.accd, the f32 blocks G into `G`
 // and the contact velocities into `vc`, both in registers.  Returns this lane's part of the
 // contact cost.
 template <int NG>
-__device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, BodyLDS& B, const SolveLDS& X,
-                                                  const CRow* r, float (*G)[6], d3* vc) {
+__device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow* r, const CConst* k, float (*G)[6],
+                                                  d3* vc) {
   const int l = lane_id();
 #pragma unroll
   for (int g = 0; g < NG; ++g) vc[g] = row_vel_d(B, r[g]);
@@ -382,10 +392,10 @@ __device__ __forceinline__ double eval_contacts_g(int nc, const ContactLDS& C, B
   for (int g = 0; g < NG; ++g) {
     const int c = l + g * NA_WAVE;
     const bool valid = c < nc;
-    double iRt = X.iRt[c], iRn = X.iRn[c];
-    d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)X.vhat[c]) * iRn};
+    const double iRt = k[g].iRt, iRn = k[g].iRn;
+    d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)k[g].vh) * iRn};
     d3 gam;
-    double cc = sap_project(C.mu[c], X.Rt[c], X.Rn[c], iRt, iRn, y, gam, G[g]);
+    double cc = sap_project(k[g].mu, k[g].Rt, k[g].Rn, iRt, iRn, y, gam, G[g]);
     cost += valid ? cc : 0.0;
     row_scatter_d(B, r[g], gam);
   }
@@ -416,18 +426,16 @@ struct LsRow {
 // and the lane's contact terms at alpha = 0 (slope d1c0, curvature d2c0) and alpha = 1 (cost
 // lc1, slope d1c1).
 template <int NG>
-__device__ __forceinline__ void ls_setup_g(int nc, const ContactLDS& C, const BodyLDS& B, const SolveLDS& X,
-                                           const CRow* r, const d3* vc0, LsRow* lr, double& d1c0, double& d2c0,
-                                           double& lc1, double& d1c1) {
+__device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow* r, const CConst* k, const d3* vc0,
+                                           LsRow* lr, double& d1c0, double& d2c0, double& lc1, double& d1c1) {
   const int l = lane_id();
 #pragma unroll
   for (int g = 0; g < NG; ++g) lr[g].dv = row_vel_d(B, r[g]);
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
-    const int c = l + g * NA_WAVE;
     LsRow& q = lr[g];
-    q.iRt = X.iRt[c]; q.iRn = X.iRn[c]; q.Rt = X.Rt[c]; q.Rn = X.Rn[c]; q.mu = C.mu[c];
-    q.vh = (double)X.vhat[c];
+    q.iRt = k[g].iRt; q.iRn = k[g].iRn; q.Rt = k[g].Rt; q.Rn = k[g].Rn; q.mu = k[g].mu;
+    q.vh = (double)k[g].vh;
     q.w0 = vc0[g];
   }
 #pragma unroll
@@ -562,6 +570,9 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
   d3 vc0[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) rows[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
+  CConst kc[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) kc[g] = load_const(C, X, l + g * NA_WAVE);
   double gn_first = 0.0;
   for (int it = 0; it <= max_iterations; ++it) {
     // gradient g = A (v - v*) - J^T gamma, FP64.  The owner lane also clears its
@@ -576,7 +587,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double ccost = eval_contacts_g<NG>(nc, C, B, X, rows, Gr, vc0);
+    double ccost = eval_contacts_g<NG>(nc, B, rows, kc, Gr, vc0);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -679,7 +690,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     wave_sync();
     double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
     LsRow lr[NG];
-    ls_setup_g<NG>(nc, C, B, X, rows, vc0, lr, d1c0, d2c0, lc1, d1c1);
+    ls_setup_g<NG>(nc, B, rows, kc, vc0, lr, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
