@@ -379,14 +379,17 @@ __device__ __forceinline__ CConst load_const(const ContactLDS& C, const SolveLDS
 
 // FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
 // lanes) for this lane's contact rows `r`: J^T gamma into B.accd, the f32 blocks G into `G`
-// and the contact velocities into `vc`, both in registers.  Returns this lane's part of the
+// and, when `gather` is set, the contact velocities into `vc` (otherwise `vc` already holds
+// them), both in registers.  Returns this lane's part of the
 // contact cost.
 template <int NG>
 __device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow* r, const CConst* k, float (*G)[6],
-                                                  d3* vc) {
+                                                  d3* vc, bool gather) {
   const int l = lane_id();
+  if (gather) {
 #pragma unroll
-  for (int g = 0; g < NG; ++g) vc[g] = row_vel_d(B, r[g]);
+    for (int g = 0; g < NG; ++g) vc[g] = row_vel_d(B, r[g]);
+  }
   double cost = 0.0;
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
@@ -570,6 +573,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
   d3 vc0[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) rows[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
+  bool gather = true;  // the first evaluation gathers J v from the guess
   CConst kc[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) kc[g] = load_const(C, X, l + g * NA_WAVE);
@@ -580,14 +584,17 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     NA_TIC(t0);
     if (own) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) { B.vd[k][l] = v[k]; B.accd[k][l] = 0.0; }
+      for (int k = 0; k < 6; ++k) {
+        if (gather) B.vd[k][l] = v[k];
+        B.accd[k][l] = 0.0;
+      }
     }
-    if (l == 0) {  // the static side's zero velocity (the slot is overwritten by the blocks)
+    if (l == 0 && gather) {  // the static side's zero velocity (the slot is overwritten by the blocks)
 #pragma unroll
       for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double ccost = eval_contacts_g<NG>(nc, B, rows, kc, Gr, vc0);
+    double ccost = eval_contacts_g<NG>(nc, B, rows, kc, Gr, vc0, gather);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -735,6 +742,14 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
 #pragma unroll
       for (int q = 0; q < 6; ++q) v[q] += alpha * dx[q];
     }
+    // the contact velocities are linear in v: the next iterate's are vc0 + alpha J dx
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      vc0[g].x += alpha * lr[g].dv.x;
+      vc0[g].y += alpha * lr[g].dv.y;
+      vc0[g].z += alpha * lr[g].dv.z;
+    }
+    gather = false;
     NA_TOC(st.t_ls, t3);
     st.iterations = it + 1;
   }
