@@ -98,7 +98,25 @@ typedef struct na_config {
   float optimality_rel_tol, optimality_abs_tol, contact_tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
   int32_t max_cg;
   float cg_forcing;        /* > 0: inexact-Newton PCG forcing bound (eta_k <= cg_forcing); 0: fixed cg_rel_tol */
+  int32_t line_search;     /* 0 "exact_root" (root of l'(alpha), cubic seed), 1 "armijo_decay" (backtracking,
+                              the reference default, solver_sap_adaptive.py:459) */
 } na_config;
+
+/* Control inputs of a boundary call (newton Control, newton/_src/sim/control.py:31-110; applied by
+ * the reference's SolverMuJoCo._apply_mjc_control, solvers/mujoco/solver_mujoco.py:3965-4085, and
+ * sap_control_from_newton, solver_sap_adaptive.py:922).  Device float arrays in the model's
+ * joint_qd (dof) layout; any pointer may be NULL.  joint_f: generalized forces -- a FREE joint's six
+ * entries are a wrench (force; torque) at the child body's COM in the world frame, added to that
+ * body's State.body_f; joint_target_q: position targets of the drives (dof layout, the reference's
+ * default newton.use_coord_layout_targets = False), replacing the model's joint_target_pos;
+ * joint_target_qd: velocity targets (NULL = 0); joint_act: feedforward generalized forces.
+ * A drive applies tau = ke (q_t - q) + kd (qd_t - qd) + act, implicitly in the step. */
+typedef struct na_control_desc {
+  const float* joint_f;
+  const float* joint_target_q;
+  const float* joint_target_qd;
+  const float* joint_act;
+} na_control_desc;
 
 /* Device pointers to the per-world controller vectors (owned by the solver). */
 typedef struct na_telemetry {
@@ -159,12 +177,14 @@ int na_solver_telemetry(const na_solver* s, na_telemetry* out);
 int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order);
 
 /* Boundary call: advance every world by dt_outer in place (served form: one persistent
- * launch, on-device ragged loop). */
-int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer, void* stream);
+ * launch, on-device ragged loop).  body_f [B,6] (may be NULL): applied COM wrenches, world frame;
+ * control (may be NULL): see na_control_desc (free bodies use joint_f only). */
+int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, const na_control_desc* control,
+            double dt_outer, void* stream);
 /* Same boundary call in the lock-step form: one launch over all worlds per iteration and a
  * 4-byte flag read back per iteration.  Returns the status of the host loop. */
-int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
-                  int32_t* iterations_out, void* stream);
+int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, const na_control_desc* control,
+                  double dt_outer, int32_t* iterations_out, void* stream);
 /* Continuous-batching rollout launch over the worlds listed in `active` (no counterpart in the
    reference: the serving form of the boundary call).  Each listed world w runs boundary frames of
    dt_outer until frames_done[w] reaches horizon_frames[w] or accepted[w] reaches
@@ -265,9 +285,11 @@ int na_attempt(const na_solver* s, const double* q64, const float* qd, const flo
    154-185; attempt sequence _substep_body, :732-849); replaces SolverSAPAdaptive.step (:962) for
    models with joints.  joint_q / joint_qd are updated in place (an FP64 coordinate shadow is kept
    while it still rounds to joint_q); body_q / body_qd (may be NULL) receive the bodies' poses and
-   COM / angular velocities at the new coordinates (eval_fk). */
+   COM / angular velocities at the new coordinates (eval_fk).  body_f [B,6] (may be NULL): applied
+   COM wrenches, world frame; control (may be NULL): joint forces, drive targets and feedforward
+   (na_control_desc), held for the whole call. */
 int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd, float* body_q, float* body_qd,
-                        double dt_outer, void* stream);
+                        const float* body_f, const na_control_desc* control, double dt_outer, void* stream);
 /* Serving form of na_step_articulated (no counterpart in the reference, like na_serve): the
    articulated worlds listed in `active` run boundary frames until their horizon, a solve failure
    or the device-time budget, with the semantics of na_serve.  joint_q64 [coordinates] and
@@ -283,22 +305,26 @@ int na_serve_articulated(na_solver* s, const int32_t* active, int32_t n_active, 
    reference): one persistent launch in which every wave takes the next listed world of either kind,
    so both kinds share the device; per world the arithmetic of na_serve (body_q / body_qd / body_f)
    or na_serve_articulated (joint_q / joint_qd with their FP64 shadows, body_q / body_qd by forward
-   kinematics).  A world is articulated when its first body has joint_type != 0. */
+   kinematics).  A world is articulated when its first body has joint_type != 0.  body_f (may be
+   NULL) applies to the bodies of both kinds. */
 int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_active, const int32_t* horizon_frames,
                    const int32_t* horizon_accepted, int32_t* frames_done, int32_t* accepted, int32_t* status,
                    const double* world_tol, float* body_q, float* body_qd, const float* body_f, float* joint_q,
                    float* joint_qd, double* joint_q64, double* joint_qd64, double dt_outer, double budget_ms,
                    int32_t max_iterations, int32_t* unfinished, void* stream);
 /* One inner step per articulated world (contacts at q, convex solve at h[w], FP64 integration);
-   the parity form of substep (solver_sap_adaptive.py:714-729).  stats [W,4] int32: Newton
-   iterations, factorizations, converged, contacts. */
+   the parity form of substep (solver_sap_adaptive.py:714-729).  body_f / control as in
+   na_step_articulated (may be NULL).  stats [W,4] int32: Newton iterations, factorizations,
+   converged, contacts. */
 int na_art_substep(const na_solver* s, const double* joint_q, const double* joint_qd, const double* h,
-                   double* q_out, double* qd_out, int32_t* stats, void* stream);
+                   const float* body_f, const na_control_desc* control, double* q_out, double* qd_out,
+                   int32_t* stats, void* stream);
 /* One step-doubling attempt per articulated world from (joint_q, joint_qd) with dt[w]: full and
    doubled coordinates / velocities, FP64 error norm, solve status, contacts at q. */
 int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
-                   double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok,
-                   int32_t* n_contacts, void* stream);
+                   const float* body_f, const na_control_desc* control, double* q_full, double* qd_full,
+                   double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok, int32_t* n_contacts,
+                   void* stream);
 
 /* Controller kernels with the reference float32 arithmetic (one thread per world). */
 int na_ctrl_calc_adjusted_step_f32(int32_t n, const float* err, const float* dt, float* ideal_dt, uint8_t* accepted,

@@ -51,8 +51,12 @@ class NaConfig(C.Structure):
         ("max_substeps", i32), ("max_iterations", i32), ("max_rigid_contact", i32), ("broad_phase", i32),
         ("optimality_rel_tol", C.c_float), ("optimality_abs_tol", C.c_float), ("contact_tau_d", C.c_float),
         ("sigma", C.c_float), ("beta", C.c_float), ("ls_rel_tol", C.c_float), ("cg_rel_tol", C.c_float),
-        ("max_cg", i32), ("cg_forcing", C.c_float),
+        ("max_cg", i32), ("cg_forcing", C.c_float), ("line_search", i32),
     ]
+
+
+class NaControlDesc(C.Structure):
+    _fields_ = [(n, vp) for n in ("joint_f", "joint_target_q", "joint_target_qd", "joint_act")]
 
 
 class NaTelemetry(C.Structure):
@@ -78,21 +82,21 @@ SIGNATURES = {
     "na_solver_destroy": (C.c_int, [vp]),
     "na_solver_telemetry": (C.c_int, [vp, C.POINTER(NaTelemetry)]),
     "na_solver_set_schedule": (C.c_int, [vp, C.c_int, vp]),
-    "na_step": (C.c_int, [vp, vp, vp, vp, C.c_double, vp]),
-    "na_step_plain": (C.c_int, [vp, vp, vp, vp, C.c_double, C.POINTER(i32), vp]),
+    "na_step": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, vp]),
+    "na_step_plain": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, C.POINTER(i32), vp]),
     "na_reset": (C.c_int, [vp, vp, vp]),
     "na_coords_to_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_match_contacts": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, C.c_float, C.c_float, vp, vp,
                                     vp, vp, vp, vp, vp, vp]),
     "na_coords_from_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
-    "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, C.c_double, vp]),
+    "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]),
     "na_serve_articulated": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        C.c_double, i32, vp, vp]),
     "na_serve_mixed": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                  C.c_double, i32, vp, vp]),
-    "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
-    "na_art_attempt": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "na_art_attempt": (C.c_int, [vp] * 14),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
     "na_sort_contacts_workspace_bytes": (C.c_int64, [i32]),
     "na_sort_contacts": (C.c_int, [C.POINTER(NaContactsDesc), i32, vp, vp, vp, i32, vp, vp, vp, C.c_int64, vp]),

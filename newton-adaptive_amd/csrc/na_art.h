@@ -8,7 +8,10 @@
 // For a step of size h from (q, v):
 //   M(q)     composite-rigid-body mass matrix, FP64, one lane per dof column
 //   c(q, v)  Newton-Euler velocity products and gravity, FP64
-//   A = M + diag(armature + h (kd + h ke)),   A v* = (M + diag(armature)) v + h (ke (q_t - q) - c)
+//   A = M + diag(armature + h (kd + h ke)),
+//   A v* = (M + diag(armature)) v + h (ke (q_t - q) + kd qd_t + tau - c)
+// (drive targets q_t, qd_t and applied generalized forces tau from the control; applied body
+// wrenches enter the bias c with gravity)
 //   H = A + sum_c J_c^T G_c J_c   with the contact part on FP32 MFMA (v_mfma_f32_32x32x2f32)
 // and the convex contact problem of the free-body solve with this dense A: Newton steps on the
 // FP64 gradient, Cholesky factors in LDS, FP64 substitutions, the exact line search of
@@ -54,6 +57,9 @@ struct ArtLDS {
   int dof_q[NA_ART_ND];         // coordinate of a revolute dof, -1 for FREE dofs
   float Xp[NA_ART_NB][7], Xc[NA_ART_NB][7], axis[NA_ART_NB][3], com_b[NA_ART_NB][3];
   float kp[NA_ART_ND], kd[NA_ART_ND], arm[NA_ART_ND], qt[NA_ART_ND];
+  float vt[NA_ART_ND];          // velocity target of the drive (0 without control)
+  float tau[NA_ART_ND];         // applied generalized force: revolute joint_f + joint_act
+  float fext[NA_ART_NB][6];     // applied wrench at the body's COM, world frame (f; t): body_f + FREE joint_f
   // kinematics at the evaluation coordinates qe
   double qe[NA_ART_NQ];
   double Rb[NA_ART_NB][9], pb[NA_ART_NB][3];  // body pose (rotation row-major, origin)
@@ -75,6 +81,15 @@ struct ArtLDS {
   double W[6][NA_ART_NC];  // impulse (p x f; f) of the contact force at the world origin
   float G[6][NA_ART_NC];
   int act[NA_ART_NC];
+};
+
+// Control inputs of a step (newton Control, control.py:31-110), indexed by global dof; null
+// pointers stand for the model's targets (tq) or zero.
+struct NaControl {
+  const float* jf;   // joint_f: generalized forces; a FREE joint's six are a COM wrench, world frame
+  const float* tq;   // joint_target_q (dof layout), replaces the model's joint_target_pos
+  const float* tqd;  // joint_target_qd
+  const float* act;  // joint_act: feedforward generalized forces
 };
 
 struct ArtWorld {
@@ -189,8 +204,9 @@ __device__ __forceinline__ ArtWorld art_world(const NaModel& M, int w) {
   return a;
 }
 
-// Joint data of the world into LDS; sets aw.dmax (depth of the deepest body, root = 0).
-__device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A) {
+// Joint data and control inputs of the world into LDS; sets aw.dmax (depth of the deepest body,
+// root = 0).  body_f [B,6] (may be null): applied COM wrenches, world frame (State.body_f).
+__device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A, const NaControl& U, const float* body_f) {
   const int l = lane_id();
   if (l < aw.nb) {
     const int j = aw.b0 + l;
@@ -236,12 +252,27 @@ __device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A) {
   }
   aw.nq = wave_sum_i(jq);
   aw.nv = wave_sum_i(jd);
+  wave_sync();  // dof_body of every lane is written
   if (l < aw.nv) {
     const int d = aw.d0 + l;
     A.kp[l] = M.joint_target_ke[d];
     A.kd[l] = M.joint_target_kd[d];
     A.arm[l] = M.joint_armature[d];
-    A.qt[l] = M.joint_target_pos[d];
+    A.qt[l] = U.tq ? U.tq[d] : M.joint_target_pos[d];
+    A.vt[l] = U.tqd ? U.tqd[d] : 0.f;
+    // a FREE joint's joint_f is a COM wrench (below); a revolute dof's is a torque
+    const bool rev = A.jtype[A.dof_body[l]] != NA_JT_FREE;
+    A.tau[l] = (U.act ? U.act[d] : 0.f) + (U.jf && rev ? U.jf[d] : 0.f);
+  }
+  if (l < aw.nb) {
+    const int j = aw.b0 + l;
+    const bool free_j = A.jtype[l] == NA_JT_FREE;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      float fe = body_f ? body_f[6 * j + k] : 0.f;
+      if (free_j && U.jf) fe += U.jf[aw.d0 + A.jd[l] + k];
+      A.fext[l][k] = fe;
+    }
   }
   wave_sync();
 }
@@ -428,13 +459,15 @@ __device__ double art_dynamics(const NaModel& M, const ArtWorld& aw, ArtLDS& A, 
     dcross3(Vb, IV, t0);
     dcross3(Vb + 3, IV + 3, t1);
     dcross3(Vb, IV + 3, t2);
-    const double mg[3] = {m * (double)M.gx, m * (double)M.gy, m * (double)M.gz};
-    double cmg[3];
-    dcross3(c, mg, cmg);
+    // gravity and the applied COM wrench (fe; te) act at the COM: (c x f + t; f) at the origin
+    const double fe[3] = {m * (double)M.gx + (double)A.fext[l][0], m * (double)M.gy + (double)A.fext[l][1],
+                          m * (double)M.gz + (double)A.fext[l][2]};
+    double cfe[3];
+    dcross3(c, fe, cfe);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      f[i] += t0[i] + t1[i] - cmg[i];
-      f[3 + i] += t2[i] - mg[i];
+      f[i] += t0[i] + t1[i] - cfe[i] - (double)A.fext[l][3 + i];
+      f[3 + i] += t2[i] - fe[i];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) A.V[l][i] = f[i];
@@ -796,6 +829,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     rhs += arm * v0;
     const int qi = A.dof_q[l];
     if (qi >= 0) rhs += h * ke * ((double)A.qt[l] - A.qe[qi]);
+    rhs += h * (kd * (double)A.vt[l] + (double)A.tau[l]);
     rhs -= h * cbias;
   }
   wave_sync();
@@ -916,7 +950,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     if (dof) A.x[l] = dv;
     wave_sync();
     art_body_velocity(aw, A);
-    double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
+    double lc0 = 0.0, d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
     if (l < nc) {
       double u[3] = {0.0, 0.0, 0.0};
       if (A.act[l]) {
@@ -930,7 +964,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
         const d3 y1 = d3{-(A.vc0[0][l] + u[0]) * iRt, -(A.vc0[1][l] + u[1]) * iRt,
                          -(A.vc0[2][l] + u[2] - vh) * iRn};
         double d2c1 = 0.0;
-        sap_line(mu, Rt, Rn, iRt, iRn, y0, uu, d1c0, d2c0);
+        lc0 = sap_line(mu, Rt, Rn, iRt, iRn, y0, uu, d1c0, d2c0);
         lc1 = sap_line(mu, Rt, Rn, iRt, iRn, y1, uu, d1c1, d2c1);
       }
 #pragma unroll
@@ -954,19 +988,32 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
       break;
     }
     double alpha;
-    if (dl1 <= 0.0) {
+    // this lane's contact in registers for the search
+    const bool lact = l < nc && A.act[l];
+    const int lc = lact ? l : 0;
+    const d3 lu = d3{A.dvc[0][lc], A.dvc[1][lc], A.dvc[2][lc]};
+    const d3 lv = d3{A.vc0[0][lc], A.vc0[1][lc], A.vc0[2][lc] - A.vhat[lc]};
+    const double liRt = A.iRt[lc], liRn = A.iRn[lc], lRt = A.Rt[lc], lRn = A.Rn[lc], lmu = A.con.mu[lc];
+    alpha = -1.0;
+    if (cfg.line_search == 1) {
+      alpha = armijo_decay(dl0, [&](double a) {
+        double cc = 0.0, d1 = 0.0, d2 = 0.0;
+        if (lact) {
+          const d3 y = d3{-(lv.x + a * lu.x) * liRt, -(lv.y + a * lu.y) * liRt, -(lv.z + a * lu.z) * liRn};
+          cc = sap_line(lmu, lRt, lRn, liRt, liRn, y, lu, d1, d2) - lc0;
+        }
+        return a * b0 + 0.5 * a * a * aa + wave_sum_d(cc);
+      });
+    }
+    if (alpha > 0.0) {
+      // backtracking step taken
+    } else if (dl1 <= 0.0) {
       alpha = 1.0;
     } else {
       double lo = 0.0, hi = 1.0;
       alpha = cubic_min_d(l0, dl0, l1, dl1, 1.0);
       if (!(alpha > 0.0)) alpha = fmin(fmax(-dl0 / d2l0, 0.0), 1.0);
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
-      // this lane's contact in registers for the root search
-      const bool lact = l < nc && A.act[l];
-      const int lc = lact ? l : 0;
-      const d3 lu = d3{A.dvc[0][lc], A.dvc[1][lc], A.dvc[2][lc]};
-      const d3 lv = d3{A.vc0[0][lc], A.vc0[1][lc], A.vc0[2][lc] - A.vhat[lc]};
-      const double liRt = A.iRt[lc], liRn = A.iRn[lc], lRt = A.Rt[lc], lRn = A.Rn[lc], lmu = A.con.mu[lc];
       for (int k = 0; k < 60; ++k) {
         double dlc = 0.0, d2c = 0.0;
         if (lact) {

@@ -8,7 +8,8 @@
 // contacts (LDS); J^T gamma and J^T G J are scattered through LDS float atomics.
 // The Newton direction comes from block-Jacobi preconditioned conjugate gradients on
 // H = A + J^T G J (matrix free), the step length from an exact line search seeded by the
-// cubic Hermite minimiser (N&W eq. 3.59, test_adaptive_cubic_linesearch.py:36-60).
+// cubic Hermite minimiser (N&W eq. 3.59, test_adaptive_cubic_linesearch.py:36-60) or, as the
+// "armijo_decay" variant, from backtracking with Armijo's condition.
 // Optimality follows the SAP criterion |D g| <= eps_a + eps_r max(|D p|, |D J^T gamma|).
 #pragma once
 #include "na_collide.h"
@@ -21,6 +22,7 @@ struct NaConfig {
   float opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol, cg_rel_tol;
   int max_cg;
   float cg_forcing;  // > 0: upper bound of the inexact-Newton PCG forcing term; 0: fixed cg_rel_tol
+  int line_search;   // 0 exact (root of l'(alpha), cubic seed), 1 armijo_decay (backtracking)
 };
 
 // Body slot NA_DUMMY stands for the static side of a contact: its velocity entries stay zero
@@ -305,6 +307,40 @@ __device__ __forceinline__ double cubic_min_d(double c0, double d0, double c1, d
   return a;
 }
 
+// Backtracking line search with Armijo's sufficient decrease and geometric decay: the
+// "armijo_decay" variant of the reference's SolverSAPAdaptive (solver_sap_adaptive.py:459, the
+// default there; its solver sap_warp is absent), restated from Drake's SAP backtracking search
+// (SapSolverParameters: alpha_max 1.5, rho 0.8, sufficient decrease c = 1e-4, 40 evaluations).
+// Steps alpha_max rho^k are tried in turn; the search stops at the first step whose cost rises
+// above the previous one's once the previous one met Armijo's condition
+// l(a) - l(0) < c a l'(0) (on a convex line function that is the best step of the sequence),
+// else ends with the last step if it meets the condition.  `delta(alpha)` returns the
+// wave-uniform decrease l(alpha) - l(0), summed term by term (the momentum part in closed form,
+// the contact costs as differences per lane) so that it is not lost in the rounding of l itself.
+// Returns the step, or -1 when no evaluated step met the condition: near the optimum the
+// decrease can fall below the rounding of the contact costs, and the caller then takes the exact
+// search's step.
+#define NA_LS_ALPHA_MAX 1.5
+#define NA_LS_RHO 0.8
+#define NA_LS_ARMIJO_C 1.0e-4
+#define NA_LS_MAX_EVAL 40
+template <class DeltaFn>
+__device__ __forceinline__ double armijo_decay(double dl0, DeltaFn delta) {
+  double alpha = NA_LS_ALPHA_MAX;
+  double d = delta(alpha);
+  bool armijo = d < NA_LS_ARMIJO_C * alpha * dl0;
+  double alpha_prev = alpha, d_prev = d;
+  for (int k = 1; k < NA_LS_MAX_EVAL; ++k) {
+    alpha *= NA_LS_RHO;
+    d = delta(alpha);
+    if (d > d_prev && armijo) return alpha_prev;
+    armijo = d < NA_LS_ARMIJO_C * alpha * dl0;
+    alpha_prev = alpha;
+    d_prev = d;
+  }
+  return armijo ? alpha : -1.0;
+}
+
 // exit reasons of the Newton loop
 enum { NA_EXIT_CONVERGED = 0, NA_EXIT_MAX_ITER = 1, NA_EXIT_NOT_DESCENT = 2, NA_EXIT_CHOL = 3, NA_EXIT_NAN = 4 };
 struct SolveStats {
@@ -426,11 +462,12 @@ struct LsRow {
 };
 
 // Line-search setup: contact-space direction dv = J dx (dx in B.vd) into this lane's rows `lr`,
-// and the lane's contact terms at alpha = 0 (slope d1c0, curvature d2c0) and alpha = 1 (cost
-// lc1, slope d1c1).
+// and the lane's contact terms at alpha = 0 (cost lc0, slope d1c0, curvature d2c0) and alpha = 1
+// (cost lc1, slope d1c1).
 template <int NG>
 __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow* r, const CConst* k, const d3* vc0,
-                                           LsRow* lr, double& d1c0, double& d2c0, double& lc1, double& d1c1) {
+                                           LsRow* lr, double& lc0, double& d1c0, double& d2c0, double& lc1,
+                                           double& d1c1) {
   const int l = lane_id();
 #pragma unroll
   for (int g = 0; g < NG; ++g) lr[g].dv = row_vel_d(B, r[g]);
@@ -449,9 +486,10 @@ __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow*
     d3 y0 = d3{-w0.x * q.iRt, -w0.y * q.iRt, -(w0.z - q.vh) * q.iRn};
     d3 y1 = d3{-(w0.x + u.x) * q.iRt, -(w0.y + u.y) * q.iRt, -(w0.z + u.z - q.vh) * q.iRn};
     double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-    sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
+    const double ca = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
     double cb = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y1, u, b1, b2);
     if (c < nc) {
+      lc0 += ca;
       d1c0 += a1;
       d2c0 += a2;
       lc1 += cb;
@@ -564,6 +602,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
   const int max_iterations = cfg.max_iterations, max_cg = cfg.max_cg;
   const double opt_abs = cfg.opt_abs_tol, opt_rel = cfg.opt_rel_tol, ls_rel = cfg.ls_rel_tol;
   const double cg_rel = cfg.cg_rel_tol, forcing = cfg.cg_forcing;
+  const int line_search = cfg.line_search;
   if (l == 0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) B.v[k][NA_DUMMY] = 0.f;
@@ -695,9 +734,9 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       for (int q = 0; q < 6; ++q) B.vd[q][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
+    double lc0 = 0.0, d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
     LsRow lr[NG];
-    ls_setup_g<NG>(nc, B, rows, kc, vc0, lr, d1c0, d2c0, lc1, d1c1);
+    ls_setup_g<NG>(nc, B, rows, kc, vc0, lr, lc0, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
@@ -716,8 +755,17 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       st.reason = NA_EXIT_NOT_DESCENT;
       break;
     }
-    double alpha;
-    if (dl1 <= 0.0) {
+    double alpha = -1.0;
+    if (line_search == 1) {
+      alpha = armijo_decay(dl0, [&](double a) {
+        double lc = 0.0, dlc = 0.0, d2c = 0.0;
+        ls_contacts_g<NG>(nc, lr, a, lc, dlc, d2c);
+        return a * b0 + 0.5 * a * a * aa + wave_sum_d(lc - lc0);
+      });
+    }
+    if (alpha > 0.0) {
+      // backtracking step taken
+    } else if (dl1 <= 0.0) {
       alpha = 1.0;
     } else {
       double lo = 0.0, hi = 1.0;

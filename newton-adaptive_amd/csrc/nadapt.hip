@@ -591,7 +591,8 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
 
 // Load this lane's body (FP64 shadow kept when it still rounds to the caller's f32 pose).
 __device__ __forceinline__ void load_body(const WorldCtx& wc, const double* q64, const float* body_q,
-                                          const float* body_qd, const float* body_f, double* q, float* v, float* f) {
+                                          const float* body_qd, const float* body_f, double* q, float* v, float* f,
+                                          const float* joint_f = nullptr) {
   const int l = lane_id();
   if (l < wc.nb) {
     int gb = wc.b0 + l;
@@ -608,7 +609,8 @@ __device__ __forceinline__ void load_body(const WorldCtx& wc, const double* q64,
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       v[k] = body_qd[6 * gb + k];
-      f[k] = body_f ? body_f[6 * gb + k] : 0.f;
+      // the body's FREE joint force is a COM wrench in the world frame, like body_f
+      f[k] = (body_f ? body_f[6 * gb + k] : 0.f) + (joint_f ? joint_f[6 * gb + k] : 0.f);
     }
   } else {
 #pragma unroll
@@ -641,8 +643,8 @@ __device__ __forceinline__ void flush_work(const Ctrl& K, int w, const long long
 
 // ---------------------------------------------------------------- served boundary call
 __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
-                                              float* body_qd, const float* body_f, double dt_outer, double eff_max,
-                                              int32_t* queue, const int32_t* order) {
+                                              float* body_qd, const float* body_f, NaControl U, double dt_outer,
+                                              double eff_max, int32_t* queue, const int32_t* order) {
   __shared__ WorldLDS S;
   for (;;) {
     int slot = 0;
@@ -653,7 +655,7 @@ __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, d
     WorldCtx wc = world_ctx(M, w);
     double q[7];
     float v[6], f[6];
-    load_body(wc, q64, body_q, body_qd, body_f, q, v, f);
+    load_body(wc, q64, body_q, body_qd, body_f, q, v, f, U.jf);
     WCtrl c = load_ctrl(K, w);
     open_frame(cfg, c, dt_outer, eff_max);
     long long wk[NA_WORK] = {};
@@ -791,7 +793,7 @@ __global__ void k_zero_i32(int32_t* p, int n) {
   if (i < n) p[i] = 0;
 }
 __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_qd,
-                                                   const float* body_f, double eff_max, int32_t* flag) {
+                                                   const float* body_f, NaControl U, double eff_max, int32_t* flag) {
   __shared__ WorldLDS S;
   int w = blockIdx.x;
   WCtrl c = load_ctrl(K, w);
@@ -807,7 +809,7 @@ __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       v[k] = body_qd[6 * gb + k];
-      f[k] = body_f ? body_f[6 * gb + k] : 0.f;
+      f[k] = (body_f ? body_f[6 * gb + k] : 0.f) + (U.jf ? U.jf[6 * gb + k] : 0.f);
     }
   } else {
 #pragma unroll
@@ -1214,8 +1216,9 @@ __device__ void art_store_bodies(const NaModel& M, const ArtWorld& aw, ArtLDS& A
 // marched to its boundary on the device with FP64 coordinates (shadow q64 kept while it rounds to
 // the caller's joint_q) and velocities.
 __global__ __launch_bounds__(64) void k_art_frame(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* joint_q,
-                                                  float* joint_qd, float* body_q, float* body_qd, double dt_outer,
-                                                  double eff_max, int32_t* queue, const int32_t* order) {
+                                                  float* joint_qd, float* body_q, float* body_qd, const float* body_f,
+                                                  NaControl U, double dt_outer, double eff_max, int32_t* queue,
+                                                  const int32_t* order) {
   __shared__ ArtLDS A;
   const int l = lane_id();
   for (;;) {
@@ -1226,7 +1229,7 @@ __global__ __launch_bounds__(64) void k_art_frame(NaModel M, NaConfig cfg, Ctrl 
     const int w = order ? order[slot] : slot;
     ArtWorld aw = art_world(M, w);
     if (aw.nb == 0) continue;
-    art_load(M, aw, A);
+    art_load(M, aw, A, U, body_f);
     double q = 0.0, v = 0.0;
     int same = 1;
     if (l < aw.nq) {
@@ -1273,6 +1276,7 @@ struct ArtServeState {
   float* joint_qd;
   float* body_q;    // forward kinematics output (may be null)
   float* body_qd;
+  const float* body_f;  // applied COM wrenches (may be null)
 };
 
 // One articulated world's turn of a serving launch (the semantics of serve_world).
@@ -1282,7 +1286,7 @@ __device__ __noinline__ void serve_world_art(const NaModel& M, const NaConfig& c
   const int l = lane_id();
   ArtWorld aw = art_world(M, w);
   if (aw.nb == 0) return;
-  art_load(M, aw, A);
+  art_load(M, aw, A, NaControl{}, X.body_f);
   NaConfig wcfg = cfg;
   if (SV.tol) {
     wcfg.tol = SV.tol[w];
@@ -1401,7 +1405,8 @@ __global__ __launch_bounds__(64) void k_serve_mixed(NaModel M, NaConfig cfg, Ctr
 
 // Parity entry kernels: one inner step / one step-doubling attempt per articulated world.
 __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
-                                                    const double* h, double* q_out, double* qd_out, int32_t* stats) {
+                                                    const double* h, const float* body_f, NaControl U, double* q_out,
+                                                    double* qd_out, int32_t* stats) {
   __shared__ ArtLDS A;
   const int w = blockIdx.x;
   const int l = lane_id();
@@ -1410,7 +1415,7 @@ __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, con
     if (l < 4) stats[4 * w + l] = l == 2 ? 1 : 0;
     return;
   }
-  art_load(M, aw, A);
+  art_load(M, aw, A, U, body_f);
   const double q = l < aw.nq ? q64[aw.q0 + l] : 0.0;
   const double v = l < aw.nv ? qd64[aw.d0 + l] : 0.0;
   long long wk[NA_WORK] = {};
@@ -1432,9 +1437,9 @@ __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, con
   }
 }
 __global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
-                                                    const double* dt, double* q_full, double* qd_full, double* q_dbl,
-                                                    double* qd_dbl, double* err, int32_t* solve_ok,
-                                                    int32_t* n_contacts) {
+                                                    const double* dt, const float* body_f, NaControl U,
+                                                    double* q_full, double* qd_full, double* q_dbl, double* qd_dbl,
+                                                    double* err, int32_t* solve_ok, int32_t* n_contacts) {
   __shared__ ArtLDS A;
   const int w = blockIdx.x;
   const int l = lane_id();
@@ -1443,7 +1448,7 @@ __global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, con
     if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
     return;
   }
-  art_load(M, aw, A);
+  art_load(M, aw, A, U, body_f);
   const double q = l < aw.nq ? q64[aw.q0 + l] : 0.0;
   const double v = l < aw.nv ? qd64[aw.d0 + l] : 0.0;
   long long wk[NA_WORK] = {};
@@ -1583,6 +1588,17 @@ static bool broad_phase_ok(const na_model* m, int bp) {
   return bp == 0 || bp == 1 || (bp == 2 && m->M.P > 0);
 }
 
+static NaControl to_control(const na_control_desc* c) {
+  NaControl u{};
+  if (c) {
+    u.jf = c->joint_f;
+    u.tq = c->joint_target_q;
+    u.tqd = c->joint_target_qd;
+    u.act = c->joint_act;
+  }
+  return u;
+}
+
 static NaConfig to_cfg(const na_config* c) {
   NaConfig k;
   k.mode = c->mode; k.controller = c->controller; k.tol = c->tol; k.dt_init = c->dt_inner_init;
@@ -1592,6 +1608,7 @@ static NaConfig to_cfg(const na_config* c) {
   k.broad_phase = c->broad_phase; k.opt_rel_tol = c->optimality_rel_tol; k.opt_abs_tol = c->optimality_abs_tol;
   k.tau_d = c->contact_tau_d; k.sigma = c->sigma; k.beta = c->beta; k.ls_rel_tol = c->ls_rel_tol;
   k.cg_rel_tol = c->cg_rel_tol; k.max_cg = c->max_cg; k.cg_forcing = c->cg_forcing;
+  k.line_search = c->line_search;
   return k;
 }
 
@@ -1609,6 +1626,8 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     return fail("na_solver_create: max_rigid_contact must be in [1, 128] per world");
   if (!broad_phase_ok(m, c->broad_phase))
     return fail("na_solver_create: broad_phase must be 0 (sap), 1 (nxn) or 2 (explicit, needs the model's pairs)");
+  if (c->line_search != 0 && c->line_search != 1)
+    return fail("na_solver_create: line_search must be 0 (exact_root) or 1 (armijo_decay)");
   na_solver* s = new na_solver();
   s->model = m;
   s->cfg = to_cfg(c);
@@ -1709,16 +1728,16 @@ static int default_grid(const na_solver* s, const void* kernel = (const void*)k_
   return g < W ? g : W;
 }
 
-extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
-                       void* stream) {
+extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f,
+                       const na_control_desc* control, double dt_outer, void* stream) {
   if (!s || !body_q || !body_qd) return fail("na_step: null argument");
   if (!(dt_outer > 0)) return fail("na_step: dt (dt_outer) must be > 0");
   const NaModel& M = s->model->M;
   double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
   int grid = default_grid(s);
-  k_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
-                                       s->order);
+  k_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, to_control(control),
+                                       dt_outer, eff, s->queue, s->order);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1752,8 +1771,8 @@ extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, c
   return 0;
 }
 
-extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, double dt_outer,
-                             int32_t* iterations_out, void* stream) {
+extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f,
+                             const na_control_desc* control, double dt_outer, int32_t* iterations_out, void* stream) {
   if (!s || !body_q || !body_qd) return fail("na_step_plain: null argument");
   if (!(dt_outer > 0)) return fail("na_step_plain: dt (dt_outer) must be > 0");
   const NaModel& M = s->model->M;
@@ -1764,7 +1783,7 @@ extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const 
   int status = 0, it = 0;
   for (; it < s->cfg.max_substeps; ++it) {
     k_zero_i32<<<1, 64, 0, st>>>(s->flag, 1);
-    k_plain_iter<<<M.W, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_qd, body_f, eff, s->flag);
+    k_plain_iter<<<M.W, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_qd, body_f, to_control(control), eff, s->flag);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1902,7 +1921,8 @@ extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd
 
 // ---------------------------------------------------------------- articulated entry points
 extern "C" int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd, float* body_q, float* body_qd,
-                                   double dt_outer, void* stream) {
+                                   const float* body_f, const na_control_desc* control, double dt_outer,
+                                   void* stream) {
   if (!s || !joint_q || !joint_qd) return fail("na_step_articulated: null argument");
   if (!s->model->articulated) return fail("na_step_articulated: the model has no joints");
   if (!(dt_outer > 0)) return fail("na_step_articulated: dt (dt_outer) must be > 0");
@@ -1910,8 +1930,8 @@ extern "C" int na_step_articulated(na_solver* s, float* joint_q, float* joint_qd
   const double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
   const int grid = default_grid(s, (const void*)k_art_frame);
-  k_art_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, joint_q, joint_qd, body_q, body_qd, dt_outer, eff,
-                                           s->queue, s->order);
+  k_art_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, joint_q, joint_qd, body_q, body_qd, body_f,
+                                           to_control(control), dt_outer, eff, s->queue, s->order);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -1940,7 +1960,7 @@ extern "C" int na_serve_articulated(na_solver* s, const int32_t* active, int32_t
   int grid = default_grid(s, (const void*)k_art_serve);
   grid = grid < n_active ? grid : n_active;
   if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
-  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd};
+  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd, nullptr};
   k_art_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, X, dt_outer, eff, s->queue, active, n_active, SV, budget,
                                    max_iterations, unfinished);
   HIPCHK(hipGetLastError());
@@ -1968,7 +1988,7 @@ extern "C" int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_act
   const long long budget = budget_ms > 0 ? (long long)(budget_ms * (double)rate_khz) : (1LL << 62);
   const double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   ServeSlots SV{horizon_frames, horizon_accepted, frames_done, accepted, status, world_tol};
-  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd};
+  ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd, body_f};
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
   int grid = default_grid(s, (const void*)k_serve_mixed);
   grid = grid < n_active ? grid : n_active;
@@ -1979,24 +1999,27 @@ extern "C" int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_act
   return 0;
 }
 extern "C" int na_art_substep(const na_solver* s, const double* joint_q, const double* joint_qd, const double* h,
-                              double* q_out, double* qd_out, int32_t* stats, void* stream) {
+                              const float* body_f, const na_control_desc* control, double* q_out, double* qd_out,
+                              int32_t* stats, void* stream) {
   if (!s || !joint_q || !joint_qd || !h || !q_out || !qd_out || !stats) return fail("na_art_substep: null argument");
   if (!s->model->articulated) return fail("na_art_substep: the model has no joints");
   const NaModel& M = s->model->M;
-  k_art_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, h, q_out, qd_out, stats);
+  k_art_substep<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, h, body_f, to_control(control), q_out,
+                                             qd_out, stats);
   HIPCHK(hipGetLastError());
   return 0;
 }
 extern "C" int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
-                              double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
-                              int32_t* solve_ok, int32_t* n_contacts, void* stream) {
+                              const float* body_f, const na_control_desc* control, double* q_full, double* qd_full,
+                              double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok, int32_t* n_contacts,
+                              void* stream) {
   if (!s || !joint_q || !joint_qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok ||
       !n_contacts)
     return fail("na_art_attempt: null argument");
   if (!s->model->articulated) return fail("na_art_attempt: the model has no joints");
   const NaModel& M = s->model->M;
-  k_art_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, dt, q_full, qd_full, q_dbl, qd_dbl, err,
-                                            solve_ok, n_contacts);
+  k_art_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, dt, body_f, to_control(control), q_full,
+                                             qd_full, q_dbl, qd_dbl, err, solve_ok, n_contacts);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -80,13 +80,55 @@ class State:
 
 
 class Control:
-    """Control inputs (``newton/_src/sim/control.py:31``); free bodies take no joint targets."""
+    """Control inputs (``newton/_src/sim/control.py:31-180``), float32 per dof of the ``joint_qd``
+    layout (the reference's default ``newton.use_coord_layout_targets = False``):
+
+    ``joint_f``          generalized forces; a FREE joint's six entries are a wrench (force; torque)
+                         at the child body's COM in the world frame, added to ``State.body_f``;
+    ``joint_target_q``   position targets of the joint drives (from the model's ``joint_target_pos``);
+    ``joint_target_qd``  velocity targets;
+    ``joint_act``        feedforward generalized forces added to the drive.
+
+    A drive applies ``ke (q_t - q) + kd (qd_t - qd) + act`` (gains on the model), implicitly in the
+    step.  Every array is read at each boundary call, so a policy acts by writing them between calls.
+    """
 
     def __init__(self, model: "Model"):
-        self.joint_f = torch.zeros(model.joint_dof_count, dtype=torch.float32, device=model.device)
+        D, dev = model.joint_dof_count, model.device
+        self.joint_f = torch.zeros(D, dtype=torch.float32, device=dev)
+        tp = getattr(model, "joint_target_pos", None)
+        self.joint_target_q = tp.clone() if tp is not None else torch.zeros(D, dtype=torch.float32, device=dev)
+        self.joint_target_qd = torch.zeros(D, dtype=torch.float32, device=dev)
+        self.joint_act = torch.zeros(D, dtype=torch.float32, device=dev)
 
-    def clear(self) -> None:
+    # deprecated aliases of the reference (dof layout)
+    @property
+    def joint_target_pos(self):
+        return self.joint_target_q
+
+    @joint_target_pos.setter
+    def joint_target_pos(self, value):
+        self.joint_target_q = value
+
+    @property
+    def joint_target_vel(self):
+        return self.joint_target_qd
+
+    @joint_target_vel.setter
+    def joint_target_vel(self, value):
+        self.joint_target_qd = value
+
+    def clear(self, model: "Model | None" = None) -> None:
+        """Zero every input; ``joint_target_q`` is restored from ``model`` when one is given
+        (``control.py:136-166``)."""
         self.joint_f.zero_()
+        self.joint_target_qd.zero_()
+        self.joint_act.zero_()
+        tp = getattr(model, "joint_target_pos", None) if model is not None else None
+        if tp is not None:
+            self.joint_target_q.copy_(tp)
+        else:
+            self.joint_target_q.zero_()
 
 
 class Model:

@@ -308,8 +308,76 @@ def _bytes_per_slot() -> int:
     return 7 * 8 + 3 + 4 * 4 + 12 * 8 + 5 * 4 + 8 + 4 * 4
 
 
+_BODY_SHAPES = {"body_q": (7,), "body_qd": (6,), "body_com": (3,), "body_mass": (), "body_inv_mass": (),
+                "body_inertia": (3, 3), "body_inv_inertia": (3, 3)}
+_SHAPE_SHAPES = {"shape_type": (), "shape_body": (), "shape_collision_group": (), "shape_transform": (7,),
+                 "shape_scale": (3,), "shape_margin": (), "shape_gap": (), "shape_material_ke": (),
+                 "shape_material_mu": (), "shape_collision_radius": ()}
+_WORLD_SHAPE_TYPES = (1, 3, 4, 7)  # plane, sphere, capsule, box (GeoType codes)
+_JT_REVOLUTE, _JT_FREE = 1, 4
+
+
+def _check_rows(arrays: dict, spec: dict, n: int, what: str) -> None:
+    for k, tail in spec.items():
+        if k not in arrays:
+            raise ValueError(f"{what} array {k!r} is missing")
+        a = np.asarray(arrays[k])
+        if a.size != n * int(np.prod(tail, dtype=np.int64)):
+            raise ValueError(f"{what} array {k!r} must hold {n} rows of shape {tail}, got shape {a.shape}")
+        if a.dtype.kind == "f" and not np.all(np.isfinite(a)):
+            raise ValueError(f"{what} array {k!r} holds non-finite values")
+
+
+def _check_joints(r: RolloutRequest) -> None:
+    """The articulation rules the device kernels index by (the host twin of the library's
+    validate_articulation): one joint per body, revolute (1 coordinate / 1 dof, parent -1 or an
+    earlier body of the world) or FREE (root only: parent -1, 7 coordinates / 6 dofs), contiguous
+    coordinate and dof starts from 0, at most 16 bodies and 32 coordinates / dofs."""
+    j, nb = r.joints, r.body_count
+    for k in ("joint_type", "joint_parent", "joint_q_start", "joint_qd_start", "joint_X_p", "joint_X_c",
+              "joint_axis", "joint_q", "joint_qd", "joint_target_pos", "joint_target_ke", "joint_target_kd",
+              "joint_armature"):
+        if k not in j:
+            raise ValueError(f"joint array {k!r} is missing")
+    jt = np.asarray(j["joint_type"]).reshape(-1)
+    par = np.asarray(j["joint_parent"]).reshape(-1)
+    qs = np.asarray(j["joint_q_start"]).reshape(-1).astype(np.int64)
+    ds = np.asarray(j["joint_qd_start"]).reshape(-1).astype(np.int64)
+    if jt.size != nb or par.size != nb:
+        raise ValueError("joint_type and joint_parent need one entry per body")
+    if qs.size != nb or ds.size != nb or qs[0] != 0 or ds[0] != 0:
+        raise ValueError("joint_q_start / joint_qd_start need one entry per body, starting at 0")
+    # the last joint ends at the world's coordinate / dof count
+    qs = np.append(qs, r.coord_count)
+    ds = np.append(ds, r.dof_count)
+    for b in range(nb):
+        t, p = int(jt[b]), int(par[b])
+        nq, nd = int(qs[b + 1] - qs[b]), int(ds[b + 1] - ds[b])
+        if t == _JT_FREE:
+            if p != -1:
+                raise ValueError(f"joint {b}: a FREE joint must have the world (-1) as parent")
+            if (nq, nd) != (7, 6):
+                raise ValueError(f"joint {b}: a FREE joint has 7 coordinates and 6 dofs")
+        elif t == _JT_REVOLUTE:
+            if (nq, nd) != (1, 1):
+                raise ValueError(f"joint {b}: a revolute joint has 1 coordinate and 1 dof")
+            if p != -1 and not 0 <= p < b:
+                raise ValueError(f"joint {b}: the parent must be -1 or an earlier body of the world, got {p}")
+        else:
+            raise ValueError(f"joint {b}: joint types must be 1 (revolute) or 4 (FREE), got {t}")
+    if nb > 16 or r.coord_count > 32 or r.dof_count > 32:
+        raise ValueError("an articulated world holds at most 16 bodies, 32 coordinates and 32 dofs")
+    for k, n, tail in (("joint_X_p", nb, 7), ("joint_X_c", nb, 7), ("joint_axis", nb, 3),
+                       ("joint_target_pos", r.dof_count, 1), ("joint_target_ke", r.dof_count, 1),
+                       ("joint_target_kd", r.dof_count, 1), ("joint_armature", r.dof_count, 1)):
+        a = np.asarray(j[k])
+        if a.size != n * tail or not np.all(np.isfinite(a)):
+            raise ValueError(f"joint array {k!r} must hold {n} finite rows of {tail}")
+
+
 def check_request(r: RolloutRequest, global_count: int, budget_bytes: int) -> None:
-    """Raise ``ValueError`` for a request the pool can never hold."""
+    """Raise ``ValueError`` for a request the pool can never hold or whose indices the device
+    kernels could not use: array sizes, shape types and body references, and the joint rules."""
     if r.horizon_frames is None and r.horizon_accepted is None:
         raise ValueError("a request needs horizon_frames or horizon_accepted")
     for h in (r.horizon_frames, r.horizon_accepted):
@@ -317,10 +385,20 @@ def check_request(r: RolloutRequest, global_count: int, budget_bytes: int) -> No
             raise ValueError(f"horizons must be in [1, 2^31 - 2], got {h!r}")
     if r.tol is not None and not float(r.tol) > 0.0:
         raise ValueError(f"tol must be > 0, got {r.tol!r}")
+    if "body_q" not in r.bodies or "shape_type" not in r.shapes:
+        raise ValueError("a request needs body_q and shape_type arrays")
     if r.body_count < 1 or r.body_count > 64 or r.shape_count + global_count > 128:
         raise ValueError("a world holds 1..64 bodies and at most 128 segment shapes (world + global)")
-    if r.articulated and (r.body_count > 16 or r.coord_count > 32 or r.dof_count > 32):
-        raise ValueError("an articulated world holds at most 16 bodies, 32 coordinates and 32 dofs")
+    _check_rows(r.bodies, _BODY_SHAPES, r.body_count, "body")
+    _check_rows(r.shapes, _SHAPE_SHAPES, r.shape_count, "shape")
+    st = np.asarray(r.shapes["shape_type"]).reshape(-1)
+    if not np.all(np.isin(st, _WORLD_SHAPE_TYPES)):
+        raise ValueError(f"shape types must be among {_WORLD_SHAPE_TYPES} (plane, sphere, capsule, box)")
+    sb = np.asarray(r.shapes["shape_body"]).reshape(-1).astype(np.int64)
+    if np.any((sb < -1) | (sb >= r.body_count)):
+        raise ValueError("shape_body must be -1 (static) or a local body index in [0, body_count)")
+    if r.articulated:
+        _check_joints(r)
     if ServingEngine.request_bytes(r) > budget_bytes:
         raise ValueError("request larger than the HBM budget")
 

@@ -26,7 +26,8 @@ from .contacts import BROAD_PHASES, CollisionPipeline
 from .native import model_handle
 
 _MODE_CODES = {"fixed": 0, "adaptive": 1}
-_LINE_SEARCHES = ("exact_root", "armijo_decay")
+_LINE_SEARCHES = {"exact_root": 0, "armijo_decay": 1}
+_CONTROL_FIELDS = ("joint_f", "joint_target_q", "joint_target_qd", "joint_act")
 _WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves",
                 "cycles_collide", "cycles_setup", "cycles_residual", "cycles_preconditioner", "cycles_pcg",
                 "cycles_line_search")
@@ -57,7 +58,8 @@ class _SolverBase:
         if contact_preset_variant != "drake":
             raise ValueError(f"contact_preset_variant must be 'drake', got {contact_preset_variant!r}.")
         if line_search_variant not in _LINE_SEARCHES:
-            raise ValueError(f"line_search_variant must be one of {_LINE_SEARCHES}, got {line_search_variant!r}.")
+            raise ValueError(f"line_search_variant must be one of {tuple(_LINE_SEARCHES)}, "
+                             f"got {line_search_variant!r}.")
         if broad_phase not in BROAD_PHASES:
             raise ValueError(f"broad_phase must be one of {tuple(BROAD_PHASES)}, got {broad_phase!r}.")
         if not 0.0 < float(cg_rel_tol) < 1.0:
@@ -107,6 +109,8 @@ class _SolverBase:
         cfg.cg_rel_tol = float(cg_rel_tol)
         cfg.max_cg = 200
         cfg.cg_forcing = float(cg_forcing)
+        cfg.line_search = _LINE_SEARCHES[line_search_variant]
+        self.line_search_variant = line_search_variant
         self._cfg = cfg
 
         f64 = dict(dtype=torch.float64, device=dev)
@@ -159,25 +163,63 @@ class _SolverBase:
             pass
 
     # ------------------------------------------------------------------ schedule
-    def set_schedule(self, grid_waves: int = 0, order: torch.Tensor | None = None) -> None:
-        """Persistent grid size (0: one wave per resident slot) and world visiting order."""
+    def set_schedule(self, grid_waves: int = 0, order: torch.Tensor | None = None, _checked: bool = False) -> None:
+        """Persistent grid size (0: one wave per resident slot) and world visiting order.  ``order``
+        must be a permutation of ``range(world_count)``: a repeated world would run on two waves at
+        once and another world would never step."""
         if order is not None:
             order = order.to(device=self.model.device, dtype=torch.int32).contiguous()
             if order.numel() != self._world_count:
                 raise ValueError("order must be a permutation of the worlds")
+            if not _checked:
+                ref = torch.arange(self._world_count, dtype=torch.int32, device=order.device)
+                if not torch.equal(torch.sort(order).values, ref):
+                    raise ValueError("order must be a permutation of the worlds (each index once)")
         self._order = order
         _lib.check(_lib.lib().na_solver_set_schedule(self._h, int(grid_waves),
                                                      order.data_ptr() if order is not None else None),
                    "na_solver_set_schedule")
 
     # ------------------------------------------------------------------ boundary call
+    def _control_desc(self, control):
+        """``na_control_desc`` of a :class:`Control` (None: no control).  Every array must hold
+        one float per dof; the tensors are kept alive until the next call."""
+        if control is None:
+            self._control_keep = None
+            return None
+        m = self.model
+        D = int(m.joint_dof_count)
+        desc = _lib.NaControlDesc()
+        keep = []
+        for name in _CONTROL_FIELDS:
+            t = getattr(control, name, None)
+            if t is None:
+                continue
+            t = torch.as_tensor(t).to(device=m.device, dtype=torch.float32).contiguous().view(-1)
+            if t.numel() != D:
+                raise ValueError(f"control.{name} must hold one value per dof ({D}), got {t.numel()}")
+            keep.append(t)
+            setattr(desc, name, t.data_ptr())
+        self._control_keep = (desc, keep)
+        return C.byref(desc)
+
+    def _body_f(self, state):
+        bf = getattr(state, "body_f", None)
+        if bf is None:
+            return None
+        if bf.dtype != torch.float32 or not bf.is_contiguous() or bf.numel() != 6 * self.model.body_count:
+            raise ValueError("state.body_f must be a contiguous float32 [body_count, 6] tensor")
+        return bf
+
     def integrate(self, state, control, dt_outer: float, stream=None):
         s = _lib.stream_handle(stream)
-        bf = state.body_f if getattr(state, "body_f", None) is not None else None
+        bf = self._body_f(state)
+        ctl = self._control_desc(control)
         if self.model.is_articulated:
             # generalized coordinates: joint_q / joint_qd advance, body_q / body_qd follow (eval_fk)
             _lib.check(_lib.lib().na_step_articulated(self._h, state.joint_q.data_ptr(), state.joint_qd.data_ptr(),
                                                       state.body_q.data_ptr(), state.body_qd.data_ptr(),
+                                                      bf.data_ptr() if bf is not None else None, ctl,
                                                       float(dt_outer), s), "na_step_articulated")
             if self._world_order == "cost":
                 self._order_by_cost()
@@ -186,7 +228,7 @@ class _SolverBase:
             return state
         if self._schedule == "served":
             st = _lib.check(_lib.lib().na_step(self._h, state.body_q.data_ptr(), state.body_qd.data_ptr(),
-                                               bf.data_ptr() if bf is not None else None, float(dt_outer), s),
+                                               bf.data_ptr() if bf is not None else None, ctl, float(dt_outer), s),
                             "na_step")
             if self._world_order == "cost":
                 self._order_by_cost()
@@ -196,8 +238,8 @@ class _SolverBase:
         else:
             it = C.c_int32(0)
             st = _lib.check(_lib.lib().na_step_plain(self._h, state.body_q.data_ptr(), state.body_qd.data_ptr(),
-                                                     bf.data_ptr() if bf is not None else None, float(dt_outer),
-                                                     C.byref(it), s), "na_step_plain")
+                                                     bf.data_ptr() if bf is not None else None, ctl,
+                                                     float(dt_outer), C.byref(it), s), "na_step_plain")
             self.last_plain_iterations = it.value
             if st == 2:
                 self._raise_solve_failure()
@@ -217,7 +259,7 @@ class _SolverBase:
         self._work_prev.copy_(w)
         cost = d[:, 2] + 5 * d[:, 1] + 10 * d[:, 0]
         order = torch.argsort(cost, descending=True, stable=True).to(torch.int32)
-        self.set_schedule(0, order)
+        self.set_schedule(0, order, _checked=True)  # an argsort is a permutation
 
     def _raise_solve_failure(self):
         raise RuntimeError(
@@ -247,8 +289,11 @@ class _SolverBase:
             v = state_in.joint_qd.to(torch.float64).contiguous()
             qo, vo = torch.empty_like(q), torch.empty_like(v)
             stats = torch.zeros((W, 4), dtype=torch.int32, device=dev)
-            _lib.check(_lib.lib().na_art_substep(self._h, q.data_ptr(), v.data_ptr(), h.data_ptr(), qo.data_ptr(),
-                                                 vo.data_ptr(), stats.data_ptr(), s), "na_art_substep")
+            bf = self._body_f(state_in)
+            _lib.check(_lib.lib().na_art_substep(self._h, q.data_ptr(), v.data_ptr(), h.data_ptr(),
+                                                 bf.data_ptr() if bf is not None else None,
+                                                 self._control_desc(control), qo.data_ptr(), vo.data_ptr(),
+                                                 stats.data_ptr(), s), "na_art_substep")
             state_out.joint_q.copy_(qo)
             state_out.joint_qd.copy_(vo)
             self.last_substep_velocity = vo

@@ -129,6 +129,7 @@ typedef struct or_config {
   double tol, dt_init, dt_min, dt_max, divergence_threshold;
   int max_substeps, max_iterations, max_contacts;
   double opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol;
+  int line_search;     /* 0 exact_root, 1 armijo_decay (solver_sap_adaptive.py:459) */
 } or_config;
 
 enum { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
@@ -1058,6 +1059,60 @@ static void chol_solve(const real* L, int N, real* x) {
 
 typedef struct { int iterations, factorizations, converged; real residual; } solve_stats;
 
+/* Decrease l(v + alpha dv) - l(v) of the line function, summed term by term: the momentum part in
+   closed form (alpha dv^T A (v - v*) + alpha^2 / 2 dv^T A dv), the contact costs as per-contact
+   differences, so that it is not lost in the rounding of l itself. */
+static real ls_delta(sap_ctx* S, const real* v, const real* dv, real alpha) {
+  int N = S->N;
+  real* d = S->tmp;
+  real* Ad = S->tmp + N;
+  real* Adv = S->tmp + 2 * N;
+  real* va = S->tmp + 3 * N;
+  for (int i = 0; i < N; ++i) d[i] = v[i] - S->vstar[i];
+  apply_A(S, d, Ad);
+  apply_A(S, dv, Adv);
+  real b0 = 0, aa = 0;
+  for (int i = 0; i < N; ++i) { b0 += dv[i] * Ad[i]; aa += dv[i] * Adv[i]; }
+  real delta = alpha * b0 + (real)0.5 * alpha * alpha * aa;
+  for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
+  for (int c = 0; c < S->nc; ++c) {
+    real vc0[3], vca[3], y[3], gl[3], Gl[9];
+    contact_velocity(S, c, v, vc0);
+    contact_velocity(S, c, va, vca);
+    y[0] = -vca[0] / S->Rt[c]; y[1] = -vca[1] / S->Rt[c]; y[2] = -(vca[2] - S->vhat[c]) / S->Rn[c];
+    real ca = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    y[0] = -vc0[0] / S->Rt[c]; y[1] = -vc0[1] / S->Rt[c]; y[2] = -(vc0[2] - S->vhat[c]) / S->Rn[c];
+    real c0 = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    delta += ca - c0;
+  }
+  return delta;
+}
+
+/* Backtracking line search with Armijo's sufficient decrease and geometric decay along S->dv from
+   v: the "armijo_decay" variant of the reference's SolverSAPAdaptive (solver_sap_adaptive.py:459;
+   its solver sap_warp is absent), restated from Drake's SAP backtracking search (alpha_max 1.5,
+   rho 0.8, c = 1e-4, 40 evaluations).  Steps alpha_max rho^k are tried in turn; the search stops at
+   the first step whose cost rises above the previous one's once the previous one met
+   l(a) - l(0) < c a l'(0), else ends with the last step if it meets that condition.  Returns the
+   step, or -1 when no evaluated step met it (near the optimum the decrease can fall below the
+   rounding of the contact costs; the caller then takes the exact search's step). */
+static real armijo_decay(sap_ctx* S, const real* v, real dl0) {
+  const real alpha_max = (real)1.5, rho = (real)0.8, c = (real)1.0e-4;
+  real alpha = alpha_max;
+  real d = ls_delta(S, v, S->dv, alpha);
+  int armijo = d < c * alpha * dl0;
+  real alpha_prev = alpha, d_prev = d;
+  for (int k = 1; k < 40; ++k) {
+    alpha *= rho;
+    d = ls_delta(S, v, S->dv, alpha);
+    if (d > d_prev && armijo) return alpha_prev;
+    armijo = d < c * alpha * dl0;
+    alpha_prev = alpha;
+    d_prev = d;
+  }
+  return armijo ? alpha : (real)-1;
+}
+
 static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) {
   int N = S->N;
   real rel = (real)C->opt_rel_tol, abs_tol = (real)C->opt_abs_tol;
@@ -1112,13 +1167,17 @@ static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) 
     st->factorizations++;
     for (int i = 0; i < N; ++i) S->dv[i] = -S->g[i];
     chol_solve(S->H, N, S->dv);
-    /* exact line search on [0, 1]: root of l'(alpha), cubic-seeded safeguarded Newton */
+    /* line search: exact on [0, 1] (root of l'(alpha), cubic-seeded safeguarded Newton) or
+       backtracking with Armijo's condition ("armijo_decay") */
     real l0, d0, dd0, l1, d1, dd1;
     ls_eval(S, v, S->dv, 0, &l0, &d0, &dd0);
     if (!(d0 < 0)) { st->iterations = it + 1; break; }
+    real alpha = -1;
+    if (C->line_search == 1) alpha = armijo_decay(S, v, d0);
     ls_eval(S, v, S->dv, 1, &l1, &d1, &dd1);
-    real alpha;
-    if (d1 <= 0) alpha = 1;
+    if (alpha > 0) {
+      /* backtracking step taken */
+    } else if (d1 <= 0) alpha = 1;
     else {
       real lo = 0, hi = 1;
       double ac;
@@ -1538,6 +1597,13 @@ typedef struct or_art {
   const double* target_ke;
   const double* target_kd;
   const double* armature;
+  /* control inputs (control.py:31-110), NULL = none: velocity targets and generalized forces per
+     dof (a FREE joint's joint_f is a COM wrench, world frame), feedforward per dof, applied COM
+     wrenches per body [B,6] (world frame, State.body_f) */
+  const double* target_vel;
+  const double* joint_f;
+  const double* joint_act;
+  const double* body_f;
 } or_art;
 
 typedef struct { double p[3]; double R[9]; } rigid;  /* world pose, R row-major */
@@ -1745,9 +1811,19 @@ static void art_dynamics(const or_model* M, const or_art* A, const art_kin* K, c
       for (int k = 0; k < 6; ++k) { f[i] += I6[6 * i + k] * ab[6 * b + k]; IV[i] += I6[6 * i + k] * Vb[6 * b + k]; }
     }
     fcross(Vb + 6 * b, IV, t);
+    /* gravity plus the applied COM wrench (fe; te): body_f of the body, and joint_f of a FREE
+       joint that moves it, both world frame; (c x fe + te; fe) at the origin */
     const double* g = M->gravity;
-    double fg[6] = {m * (cb[1] * g[2] - cb[2] * g[1]), m * (cb[2] * g[0] - cb[0] * g[2]),
-                    m * (cb[0] * g[1] - cb[1] * g[0]), m * g[0], m * g[1], m * g[2]};
+    double fe[3] = {m * g[0], m * g[1], m * g[2]}, te[3] = {0, 0, 0};
+    int gb = K->b0 + b, jj = K->j0 + b;
+    if (A->body_f)
+      for (int i = 0; i < 3; ++i) { fe[i] += A->body_f[6 * gb + i]; te[i] += A->body_f[6 * gb + 3 + i]; }
+    if (A->joint_f && A->joint_type[jj] == OR_JT_FREE) {
+      int d = A->joint_qd_start[jj];
+      for (int i = 0; i < 3; ++i) { fe[i] += A->joint_f[d + i]; te[i] += A->joint_f[d + 3 + i]; }
+    }
+    double fg[6] = {cb[1] * fe[2] - cb[2] * fe[1] + te[0], cb[2] * fe[0] - cb[0] * fe[2] + te[1],
+                    cb[0] * fe[1] - cb[1] * fe[0] + te[2], fe[0], fe[1], fe[2]};
     for (int i = 0; i < 6; ++i) f[i] += t[i] - fg[i];
     /* J_b^T f and J_b^T I J_b */
     for (int k = 0; k < nv; ++k) {
@@ -1848,6 +1924,10 @@ static int art_inner_step(const or_model* M, const or_art* A, const or_config* C
       S.Adense[(size_t)i * nv + i] += (real)(arm + h * (kd + h * ke));
       rhs[i] += arm * v0[i];
       if (A->joint_type[j] == OR_JT_REVOLUTE) rhs[i] += h * ke * (A->target_pos[gd] - q[qj]);
+      /* velocity target, feedforward, and the joint force of a revolute dof */
+      if (A->target_vel) rhs[i] += h * kd * A->target_vel[gd];
+      if (A->joint_act) rhs[i] += h * A->joint_act[gd];
+      if (A->joint_f && A->joint_type[j] == OR_JT_REVOLUTE) rhs[i] += h * A->joint_f[gd];
     }
   }
   for (int i = 0; i < nv; ++i) rhs[i] -= h * cb[i];

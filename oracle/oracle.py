@@ -50,7 +50,7 @@ class OrConfig(C.Structure):
         ("divergence_threshold", C.c_double),
         ("max_substeps", C.c_int), ("max_iterations", C.c_int), ("max_contacts", C.c_int),
         ("opt_rel_tol", C.c_double), ("opt_abs_tol", C.c_double), ("tau_d", C.c_double), ("sigma", C.c_double),
-        ("beta", C.c_double), ("ls_rel_tol", C.c_double),
+        ("beta", C.c_double), ("ls_rel_tol", C.c_double), ("line_search", C.c_int),
     ]
 
 
@@ -142,7 +142,7 @@ class OrArt(C.Structure):
         ("joint_count", C.c_int), ("world_joint_start", i32p), ("joint_type", i32p), ("joint_parent", i32p),
         ("joint_X_p", f64p), ("joint_X_c", f64p), ("joint_axis", f64p), ("joint_q_start", i32p),
         ("joint_qd_start", i32p), ("target_pos", f64p), ("target_ke", f64p), ("target_kd", f64p),
-        ("armature", f64p),
+        ("armature", f64p), ("target_vel", f64p), ("joint_f", f64p), ("joint_act", f64p), ("body_f", f64p),
     ]
 
 
@@ -169,6 +169,21 @@ class OracleArt:
         self.qd_start = self.keep["joint_qd_start"]
         self.world_q_start = np.asarray(h["world_q_start"]).tolist() + [int(h["joint_q_start"][-1])]
         self.world_qd_start = np.asarray(h["world_qd_start"]).tolist() + [int(h["joint_qd_start"][-1])]
+        self._model_target = self.keep["target_pos"]
+
+    def set_control(self, joint_f=None, joint_target_q=None, joint_target_qd=None, joint_act=None, body_f=None):
+        """Control inputs of the following calls (float32-rounded, as the device reads them):
+        per-dof joint forces, position / velocity targets, feedforward, and per-body COM wrenches
+        [B, 6]; None = none (position targets: the model's)."""
+        r = lambda x: None if x is None else np.ascontiguousarray(
+            np.asarray(x, dtype=np.float32).astype(np.float64).reshape(-1))
+        self.keep["target_pos"] = r(joint_target_q) if joint_target_q is not None else self._model_target
+        self.desc.target_pos = _ptr(self.keep["target_pos"], f64p)
+        for field, val in (("target_vel", joint_target_qd), ("joint_f", joint_f), ("joint_act", joint_act),
+                           ("body_f", body_f)):
+            arr = r(val)
+            self.keep["ctl_" + field] = arr
+            setattr(self.desc, field, _ptr(arr, f64p) if arr is not None else None)
 
 
 def art_fk(om, oa, w, q):
@@ -218,7 +233,7 @@ def art_attempt_all(om, oa, cfg, q, v, dt):
 
 def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
                 max_substeps=16, max_iterations=30, max_contacts=128, opt_rel_tol=None, opt_abs_tol=1e-14,
-                tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10) -> OrConfig:
+                tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10, line_search="exact_root") -> OrConfig:
     c = OrConfig()
     c.mode = 1 if mode == "adaptive" else 0
     c.controller = 0 if controller == "sap" else 1
@@ -227,6 +242,7 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     c.max_substeps, c.max_iterations, c.max_contacts = max_substeps, max_iterations, max_contacts
     c.opt_rel_tol = opt_rel_tol if opt_rel_tol is not None else 1e-10
     c.opt_abs_tol, c.tau_d, c.sigma, c.beta, c.ls_rel_tol = opt_abs_tol, tau_d, sigma, beta, ls_rel_tol
+    c.line_search = {"exact_root": 0, "armijo_decay": 1}[line_search]
     return c
 
 

@@ -27,19 +27,33 @@ TESTS = {
 }
 
 
-class _Wp:
-    @staticmethod
-    def mat33(*a):
-        return tuple(float(x) for x in a)
-
-    @staticmethod
-    def vec3(*a):
-        return tuple(float(x) for x in a)
+_WP_CALLS = ("mat33", "vec3")  # wp.mat33(...) / wp.vec3(...) become row-major tuples
 
 
 def _eval(node, env):
-    code = compile(ast.Expression(node), "<kat>", "eval")
-    return eval(code, {"__builtins__": {}, "wp": _Wp}, env)
+    """Value of a literal expression: numbers, tuples / lists, unary minus / plus, names bound
+    earlier in the same test, and wp.mat33 / wp.vec3 calls on such values.  Anything else raises
+    ValueError; nothing from the file is executed."""
+    if isinstance(node, ast.Constant) and isinstance(node.value, (int, float, bool)):
+        return node.value
+    if isinstance(node, (ast.Tuple, ast.List)):
+        vals = [_eval(e, env) for e in node.elts]
+        return tuple(vals) if isinstance(node, ast.Tuple) else vals
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _eval(node.operand, env)
+        if not isinstance(v, (int, float)):
+            raise ValueError("unary sign on a non-number")
+        return -v if isinstance(node.op, ast.USub) else v
+    if isinstance(node, ast.Name) and node.id in env:
+        return env[node.id]
+    if (isinstance(node, ast.Call) and not node.keywords and isinstance(node.func, ast.Attribute)
+            and isinstance(node.func.value, ast.Name) and node.func.value.id == "wp"
+            and node.func.attr in _WP_CALLS):
+        args = [_eval(a, env) for a in node.args]
+        if not all(isinstance(a, (int, float)) for a in args):
+            raise ValueError("wp constructor with non-numeric arguments")
+        return tuple(float(a) for a in args)
+    raise ValueError(f"unsupported expression {type(node).__name__}")
 
 
 def extract(ref_root: str) -> dict:
@@ -58,7 +72,7 @@ def extract(ref_root: str) -> dict:
                 continue
             try:
                 val = _eval(stmt.value, env)
-            except Exception:
+            except ValueError:
                 continue
             env[tgt.id] = val
             if tgt.id == "test_cases":

@@ -44,19 +44,22 @@ def _t(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
 
 
-def _art_substep(nalib, solver, q, v, h):
+def _art_substep(nalib, solver, q, v, h, body_f=None, control=None):
     W = solver.model.world_count
     qt, vt = _t(q), _t(v)
     qo, vo = torch.zeros_like(qt), torch.zeros_like(vt)
     st = torch.zeros((W, 4), dtype=torch.int32, device=DEV)
     hh = torch.full((W,), h, dtype=torch.float64, device=DEV)
-    nalib.check(nalib.lib().na_art_substep(solver._h, qt.data_ptr(), vt.data_ptr(), hh.data_ptr(), qo.data_ptr(),
+    bf = _t(body_f, torch.float32) if body_f is not None else None
+    ctl = solver._control_desc(control)
+    nalib.check(nalib.lib().na_art_substep(solver._h, qt.data_ptr(), vt.data_ptr(), hh.data_ptr(),
+                                           bf.data_ptr() if bf is not None else None, ctl, qo.data_ptr(),
                                            vo.data_ptr(), st.data_ptr(), nalib.stream_handle()), "na_art_substep")
     torch.cuda.synchronize()
     return qo.cpu().numpy(), vo.cpu().numpy(), st.cpu().numpy()
 
 
-def _art_attempt(nalib, solver, q, v, dt):
+def _art_attempt(nalib, solver, q, v, dt, body_f=None, control=None):
     W = solver.model.world_count
     qt, vt = _t(q), _t(v)
     out = {k: torch.zeros_like(qt) for k in ("q_full", "q_double")}
@@ -65,7 +68,10 @@ def _art_attempt(nalib, solver, q, v, dt):
     ok = torch.zeros(W, dtype=torch.int32, device=DEV)
     nc = torch.zeros(W, dtype=torch.int32, device=DEV)
     dtt = torch.full((W,), dt, dtype=torch.float64, device=DEV)
+    bf = _t(body_f, torch.float32) if body_f is not None else None
+    ctl = solver._control_desc(control)
     nalib.check(nalib.lib().na_art_attempt(solver._h, qt.data_ptr(), vt.data_ptr(), dtt.data_ptr(),
+                                           bf.data_ptr() if bf is not None else None, ctl,
                                            out["q_full"].data_ptr(), out["v_full"].data_ptr(),
                                            out["q_double"].data_ptr(), out["v_double"].data_ptr(), err.data_ptr(),
                                            ok.data_ptr(), nc.data_ptr(), nalib.stream_handle()), "na_art_attempt")

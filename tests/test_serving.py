@@ -83,6 +83,44 @@ def test_articulated_requests_local_indices_and_limits():
     assert 0.15 < share < 0.35
 
 
+def _mutated(r, part, key, fn):
+    """Copy of request r with one array of bodies / shapes / joints replaced by fn(array)."""
+    parts = {"bodies": dict(r.bodies), "shapes": dict(r.shapes), "joints": dict(r.joints) if r.joints else None}
+    parts[part][key] = fn(np.array(parts[part][key]))
+    return serving.RolloutRequest(parts["bodies"], parts["shapes"], parts["joints"], horizon_frames=2)
+
+
+def _set(i, v):
+    def f(a):
+        a.reshape(-1)[i] = v
+        return a
+    return f
+
+
+MALFORMED = [
+    ("shapes", "shape_body", _set(0, 99)),       # body index past the world's bodies
+    ("shapes", "shape_body", _set(0, -2)),       # below -1
+    ("shapes", "shape_type", _set(0, 8)),        # mesh: not a supported primitive
+    ("shapes", "shape_scale", lambda a: a[:-1]),  # one row short
+    ("bodies", "body_q", _set(0, np.nan)),       # non-finite pose
+    ("joints", "joint_type", _set(1, 2)),        # unknown joint type
+    ("joints", "joint_parent", _set(1, 3)),      # parent after its child
+    ("joints", "joint_parent", _set(0, 0)),      # FREE root with a body parent
+    ("joints", "joint_q_start", _set(2, 9)),     # coordinate starts inconsistent with the types
+    ("joints", "joint_qd_start", _set(0, 1)),    # dof starts not from 0
+    ("joints", "joint_target_ke", lambda a: a[:-1]),
+]
+
+
+@pytest.mark.parametrize("part,key,fn", MALFORMED)
+def test_malformed_requests_raise(part, key, fn):
+    m = na.scenes.articulated_mix(4, seed=3, device="cpu")
+    r = na.requests_from_model(m, horizon_frames=2)[1]  # quadruped: FREE root + revolute legs
+    serving.check_request(r, 1, 1 << 30)
+    with pytest.raises(ValueError):
+        serving.check_request(_mutated(r, part, key, fn), 1, 1 << 30)
+
+
 # ---------------------------------------------------------------- GPU
 def _boundary_calls(model, frames, **kw):
     state = model.state()
