@@ -258,6 +258,17 @@ int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);
 int64_t na_collide_scratch_words(const na_model* m, int32_t per_world);
 int na_collide(const na_model* m, const float* body_q, const na_contacts_desc* out, int32_t per_world,
                int32_t broad_phase, void* stream);
+/* Allocation-free variant of na_collide.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and outputs as na_collide (out->scratch is ignored); the staging
+ * area and per-world offsets live in a static device array of NA_COLLIDE_NOALLOC_MAX_WORDS int32
+ * words (48 MiB, held by every GPU the library runs on until the process ends).  Requires
+ * world_count * per_world * 20 + 2 * world_count <= NA_COLLIDE_NOALLOC_MAX_WORDS, else returns -1.
+ * Calls on one GPU share that array, so calls that overlap on one GPU (other streams, other host
+ * threads) race; the caller orders them. */
+#define NA_COLLIDE_NOALLOC_MAX_WORDS (12 * 1024 * 1024)
+int na_collide_noalloc(const na_model* m, const float* body_q, const na_contacts_desc* out, int32_t per_world,
+                       int32_t broad_phase, void* stream);
 /* AABBs [S,3] lo/hi and per-world candidate pairs [W, max_pairs, 2] (global shape ids,
  * canonical) with counts [W]. */
 int na_collide_debug(const na_model* m, const float* body_q, float* aabb_lo, float* aabb_hi, int32_t* pairs,

@@ -101,6 +101,7 @@ SIGNATURES = {
     "na_sort_contacts_workspace_bytes": (C.c_int64, [i32]),
     "na_sort_contacts": (C.c_int, [C.POINTER(NaContactsDesc), i32, vp, vp, vp, i32, vp, vp, vp, C.c_int64, vp]),
     "na_collide": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
+    "na_collide_noalloc": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
     "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "na_attempt": (C.c_int, [vp] + [vp] * 11 + [vp]),

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1854,6 +1855,58 @@ extern "C" int na_collide(const na_model* m, const float* body_q, const na_conta
   cfg.broad_phase = broad_phase;
   int32_t* staging = out->scratch;
   int32_t* offsets = out->scratch + (long long)M.W * per_world * NA_STAGE_WORDS;
+  k_collide_api<<<M.W, 64, 0, st>>>(M, cfg, body_q, per_world, staging, out->world_count_out);
+  k_scan_counts<<<1, 1024, 0, st>>>(M.W, per_world, out->world_count_out, offsets, out->count);
+  k_compact_contacts<<<M.W, 64, 0, st>>>(M.W, per_world, staging, out->world_count_out, offsets, *out);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free contact generation
+// na_collide_noalloc keeps na_collide's staging area and per-world offsets in this file-scope
+// array instead of the caller's scratch.  Each GPU holds its own copy for the life of the process;
+// host code reaches it only through the per-device address from hipGetSymbolAddress (the array's
+// name is a host-side stand-in, not a device address).
+static __device__ __align__(256) int32_t scratch[NA_COLLIDE_NOALLOC_MAX_WORDS];
+#define NA_MAX_DEVICES 64
+static int32_t* g_scratch_dev[NA_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int32_t* scratch_on_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_dev[dev] = (int32_t*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+extern "C" int na_collide_noalloc(const na_model* m, const float* body_q, const na_contacts_desc* out,
+                                  int32_t per_world, int32_t broad_phase, void* stream) {
+  if (!m || !body_q || !out || !out->count || !out->world_count_out)
+    return fail("na_collide_noalloc: null argument");
+  if (per_world < 1 || per_world > NA_MAX_CONTACTS)
+    return fail("na_collide_noalloc: per-world capacity must be in [1, 128]");
+  if (!broad_phase_ok(m, broad_phase)) return fail("na_collide_noalloc: invalid broad_phase for this model");
+  const NaModel& M = m->M;
+  const long long words = (long long)M.W * per_world * NA_STAGE_WORDS + 2LL * M.W;
+  if (words > (long long)NA_COLLIDE_NOALLOC_MAX_WORDS)
+    return fail("na_collide_noalloc: world_count * per_world * 20 + 2 * world_count exceeds the static "
+                "scratch (NA_COLLIDE_NOALLOC_MAX_WORDS words); use na_collide with a caller scratch");
+  int32_t* base = scratch_on_current_device();
+  if (!base) return fail("na_collide_noalloc: hipGetSymbolAddress of the static scratch failed");
+  hipStream_t st = S_(stream);
+  NaConfig cfg{};
+  cfg.max_contacts = per_world;
+  cfg.broad_phase = broad_phase;
+  // every word the kernels read is written first in this call (staging slots below each world's
+  // count, all W offsets), so nothing is cleared
+  int32_t* staging = base;
+  int32_t* offsets = base + (long long)M.W * per_world * NA_STAGE_WORDS;
   k_collide_api<<<M.W, 64, 0, st>>>(M, cfg, body_q, per_world, staging, out->world_count_out);
   k_scan_counts<<<1, 1024, 0, st>>>(M.W, per_world, out->world_count_out, offsets, out->count);
   k_compact_contacts<<<M.W, 64, 0, st>>>(M.W, per_world, staging, out->world_count_out, offsets, *out);
