@@ -285,10 +285,15 @@ int na_substep(const na_solver* s, const double* q64, const float* qd, const flo
                const double* guess, double* q64_out, float* qd_out, double* v_solution, int32_t* stats,
                void* stream);
 /* One step-doubling attempt per world from (q64, qd) with dt[w]: full and doubled results,
- * FP64 error norm, solve status and contact count; no commit, no controller update. */
+ * FP64 error norm, solve status and contact count; no commit, no controller update.
+ * contacts_out [W, 2, 128, 16] float and contact_counts_out [W, 2] int32 (both NULL to skip): the
+ * contact sets the attempt solved with, at q and at the midpoint state, per contact (shape a,
+ * shape b, body a, body b (global, -1 static), manifold slot, unit normal A->B, world midpoint,
+ * distance, phi0 = distance - margins, stiffness, friction, 0) -- for parity checks that give a
+ * reference solve the same contacts. */
 int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
                double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
-               int32_t* n_contacts, void* stream);
+               int32_t* n_contacts, float* contacts_out, int32_t* contact_counts_out, void* stream);
 
 /* Articulated models (model created with joint arrays).  The boundary call on joint coordinates:
    every world advances by dt_outer with step doubling over its joint_q / joint_qd (error =
@@ -331,11 +336,12 @@ int na_art_substep(const na_solver* s, const double* joint_q, const double* join
                    const float* body_f, const na_control_desc* control, double* q_out, double* qd_out,
                    int32_t* stats, void* stream);
 /* One step-doubling attempt per articulated world from (joint_q, joint_qd) with dt[w]: full and
-   doubled coordinates / velocities, FP64 error norm, solve status, contacts at q. */
+   doubled coordinates / velocities, FP64 error norm, solve status, contacts at q; contacts_out /
+   contact_counts_out as in na_attempt (may be NULL). */
 int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
                    const float* body_f, const na_control_desc* control, double* q_full, double* qd_full,
                    double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok, int32_t* n_contacts,
-                   void* stream);
+                   float* contacts_out, int32_t* contact_counts_out, void* stream);
 
 /* Controller kernels with the reference float32 arithmetic (one thread per world). */
 int na_ctrl_calc_adjusted_step_f32(int32_t n, const float* err, const float* dt, float* ideal_dt, uint8_t* accepted,

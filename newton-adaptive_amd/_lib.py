@@ -96,7 +96,7 @@ SIGNATURES = {
     "na_serve_mixed": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                  C.c_double, i32, vp, vp]),
     "na_art_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "na_art_attempt": (C.c_int, [vp] * 14),
+    "na_art_attempt": (C.c_int, [vp] * 16),
     "na_collide_scratch_words": (C.c_int64, [vp, i32]),
     "na_sort_contacts_workspace_bytes": (C.c_int64, [i32]),
     "na_sort_contacts": (C.c_int, [C.POINTER(NaContactsDesc), i32, vp, vp, vp, i32, vp, vp, vp, C.c_int64, vp]),
@@ -104,7 +104,7 @@ SIGNATURES = {
     "na_collide_noalloc": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
     "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "na_attempt": (C.c_int, [vp] + [vp] * 11 + [vp]),
+    "na_attempt": (C.c_int, [vp] * 15),
     "na_ctrl_calc_adjusted_step_f32": (C.c_int, [i32, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]),
     "na_ctrl_adapt_dt_f32": (C.c_int, [i32] + [vp] * 11 + [i32, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
     "na_ctrl_rebase_time_f32": (C.c_int, [i32, vp, vp, vp]),

@@ -1089,10 +1089,14 @@ __device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld
 
 // Step-doubling attempt of an articulated world from (q, v) (lanes k < nq / nv): one step of dt
 // and two of dt / 2, the second half-step with contacts at the midpoint.  Returns the FP64 max
-// norm of the coordinate difference (1e10 for NaN / inf); `ok` false when a solve failed.
+// norm of the coordinate difference (1e10 for NaN / inf); `ok` false when a solve failed.  DUMP
+// builds (parity entry point only) also write both passes' contact sets to
+// dump[2][NA_MAX_CONTACTS][NA_DUMP_FLOATS] and their counts to dump_n[2].
+template <bool DUMP = false>
 __device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, double q,
                               double v, double dt, double& qf, double& vf, double& qd, double& vd, bool& ok,
-                              int& overflow, int& nc_out, long long* wk) {
+                              int& overflow, int& nc_out, long long* wk, float* dump = nullptr,
+                              int32_t* dump_n = nullptr) {
   const int l = lane_id();
   ArtStats st;
   double mdiag;
@@ -1102,6 +1106,10 @@ __device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWo
   NA_TOC(wk[10], t_kin);
   int nc = art_collide(M, cfg, aw, A, overflow, wk);
   nc_out = nc;
+  if (DUMP) {
+    dump_contacts(A.seg, A.con, aw.b0, nc, dump);
+    if (l == 0) dump_n[0] = nc;
+  }
   ok = art_step(cfg, aw, A, nc, dt, v, cb, mdiag, v, vf, qf, st, wk);
   wk[1] += st.iterations;
   wk[5] += 1;
@@ -1116,6 +1124,10 @@ __device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWo
     cb = art_dynamics(M, aw, A, vm, mdiag);
     NA_TOC(wk[10], t_kin2);
     nc = art_collide(M, cfg, aw, A, overflow, wk);
+    if (DUMP) {
+      dump_contacts(A.seg, A.con, aw.b0, nc, dump + NA_MAX_CONTACTS * NA_DUMP_FLOATS);
+      if (l == 0) dump_n[1] = nc;
+    }
     ok = art_step(cfg, aw, A, nc, 0.5 * dt, vm, cb, mdiag, vf, vd, qd, st, wk) && ok;
     wk[1] += st.iterations;
     wk[5] += 2;

@@ -798,3 +798,27 @@ __device__ __forceinline__ int stage_broad(const NaModel& M, int w, int nseg, in
   if (broad_phase == 2) return stage_broad_explicit(M, w, nl, S);
   return broad_phase == 1 ? stage_broad_nxn(M, nseg, S) : stage_broad_sap(M, nseg, S);
 }
+
+// Contact set in LDS -> HBM for the parity entry points: per contact NA_DUMP_FLOATS floats
+// (shape a, shape b, body a, body b (global, -1 static), manifold slot, normal A->B, midpoint,
+// distance, phi0, stiffness, friction, 0), exactly the values the convex solve reads.  Must run
+// before the solve set-up, which reuses the segment's LDS.
+#define NA_DUMP_FLOATS 16
+template <class CS>
+__device__ void dump_contacts(const SegLDS& seg, const CS& con, int b0, int nc, float* out) {
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    float* o = out + (long long)c * NA_DUMP_FLOATS;
+    o[0] = (float)seg.gidx[con.sa[c]];
+    o[1] = (float)seg.gidx[con.sb[c]];
+    o[2] = (float)(con.ba[c] >= 0 ? b0 + con.ba[c] : -1);
+    o[3] = (float)(con.bb[c] >= 0 ? b0 + con.bb[c] : -1);
+    o[4] = (float)con.sub[c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { o[5 + k] = con.n[k][c]; o[8 + k] = con.p[k][c]; }
+    o[11] = con.dist[c];
+    o[12] = con.phi0[c];
+    o[13] = con.k[c];
+    o[14] = con.mu[c];
+    o[15] = 0.f;
+  }
+}

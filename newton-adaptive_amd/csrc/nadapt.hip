@@ -417,14 +417,22 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
 }
 
 // Step-doubling attempt (solver_sap_adaptive.py:732-849): returns the FP64 error norm
-// (1e10 sentinel for NaN/inf), `ok` false when a convex solve did not converge.
+// (1e10 sentinel for NaN/inf), `ok` false when a convex solve did not converge.  DUMP builds
+// (parity entry point only) also write the contact sets of both collision passes to
+// dump[2][NA_MAX_CONTACTS][NA_DUMP_FLOATS] and their counts to dump_n[2].
+template <bool DUMP = false>
 __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S,
                                 const double* q, const float* v, const float* f, double dt, double* qf, float* vf,
-                                double* qd, float* vd, bool& ok, int& overflow, int& nc_out, long long* wk) {
+                                double* qd, float* vd, bool& ok, int& overflow, int& nc_out, long long* wk,
+                                float* dump = nullptr, int32_t* dump_n = nullptr) {
   const int l = lane_id();
   const bool own = l < wc.nb;
   int nc = world_collide(M, cfg, wc, S, q, overflow, wk);
   nc_out = nc;
+  if (DUMP) {
+    dump_contacts(S.seg, S.con, wc.b0, nc, dump);
+    if (l == 0) dump_n[0] = nc;
+  }
   ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, v, qf, vf, wk);
   double err = 0.0;
   if (cfg.mode == 1) {
@@ -435,6 +443,10 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
     for (int k = 0; k < 6; ++k) g1[k] = 0.5f * (v[k] + vf[k]);
     ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
     int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
+    if (DUMP) {
+      dump_contacts(S.seg, S.con, wc.b0, nc2, dump + NA_MAX_CONTACTS * NA_DUMP_FLOATS);
+      if (l == 0) dump_n[1] = nc2;
+    }
     ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, vf, qd, vd, wk) && ok;
     double e = 0.0;
     if (own) {
@@ -1131,7 +1143,7 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
 __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const double* q64, const float* qd,
                                                 const float* f, const double* dt, double* q_full, float* qd_full,
                                                 double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
-                                                int32_t* n_contacts) {
+                                                int32_t* n_contacts, float* cdump, int32_t* cdump_n) {
   __shared__ WorldLDS S;
   int w = blockIdx.x;
   WorldCtx wc = world_ctx(M, w);
@@ -1144,13 +1156,23 @@ __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const d
     for (int k = 0; k < 7; ++k) q[k] = q64[7 * (wc.b0 + l) + k];
   }
   if (wc.nb == 0) {
-    if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
+    if (l == 0) {
+      err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0;
+      if (cdump_n) { cdump_n[2 * w] = 0; cdump_n[2 * w + 1] = 0; }
+    }
     return;
   }
   long long wk[NA_WORK] = {};
   bool ok = true;
   int ovf = 0, nc = 0;
-  double e = world_attempt(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);
+  double e;
+  if (cdump) {
+    if (l == 0) cdump_n[2 * w + 1] = 0;  // fixed mode has no midpoint pass
+    e = world_attempt<true>(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk,
+                            cdump + (long long)w * 2 * NA_MAX_CONTACTS * NA_DUMP_FLOATS, cdump_n + 2 * w);
+  } else {
+    e = world_attempt(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);
+  }
   if (l < wc.nb) {
     int gb = wc.b0 + l;
 #pragma unroll
@@ -1440,13 +1462,17 @@ __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, con
 __global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
                                                     const double* dt, const float* body_f, NaControl U,
                                                     double* q_full, double* qd_full, double* q_dbl, double* qd_dbl,
-                                                    double* err, int32_t* solve_ok, int32_t* n_contacts) {
+                                                    double* err, int32_t* solve_ok, int32_t* n_contacts,
+                                                    float* cdump, int32_t* cdump_n) {
   __shared__ ArtLDS A;
   const int w = blockIdx.x;
   const int l = lane_id();
   ArtWorld aw = art_world(M, w);
   if (aw.nb == 0) {
-    if (l == 0) { err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0; }
+    if (l == 0) {
+      err[w] = 0.0; solve_ok[w] = 1; n_contacts[w] = 0;
+      if (cdump_n) { cdump_n[2 * w] = 0; cdump_n[2 * w + 1] = 0; }
+    }
     return;
   }
   art_load(M, aw, A, U, body_f);
@@ -1456,7 +1482,14 @@ __global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, con
   bool ok = true;
   int ovf = 0, nc = 0;
   double qf, vf, qd, vd;
-  const double e = art_attempt(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk);
+  double e;
+  if (cdump) {
+    if (l == 0) cdump_n[2 * w + 1] = 0;  // fixed mode has no midpoint pass
+    e = art_attempt<true>(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk,
+                          cdump + (long long)w * 2 * NA_MAX_CONTACTS * NA_DUMP_FLOATS, cdump_n + 2 * w);
+  } else {
+    e = art_attempt(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk);
+  }
   if (l < aw.nq) { q_full[aw.q0 + l] = qf; q_dbl[aw.q0 + l] = qd; }
   if (l < aw.nv) { qd_full[aw.d0 + l] = vf; qd_dbl[aw.d0 + l] = vd; }
   if (l == 0) {
@@ -1962,12 +1995,15 @@ extern "C" int na_substep(const na_solver* s, const double* q64, const float* qd
 
 extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
                           double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err,
-                          int32_t* solve_ok, int32_t* n_contacts, void* stream) {
+                          int32_t* solve_ok, int32_t* n_contacts, float* contacts_out, int32_t* contact_counts_out,
+                          void* stream) {
   if (!s || !q64 || !qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok || !n_contacts)
     return fail("na_attempt: null argument");
+  if ((contacts_out == nullptr) != (contact_counts_out == nullptr))
+    return fail("na_attempt: contacts_out and contact_counts_out go together");
   const NaModel& M = s->model->M;
   k_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, dt, q_full, qd_full, q_dbl, qd_dbl, err, solve_ok,
-                                        n_contacts);
+                                        n_contacts, contacts_out, contact_counts_out);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2065,14 +2101,17 @@ extern "C" int na_art_substep(const na_solver* s, const double* joint_q, const d
 extern "C" int na_art_attempt(const na_solver* s, const double* joint_q, const double* joint_qd, const double* dt,
                               const float* body_f, const na_control_desc* control, double* q_full, double* qd_full,
                               double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok, int32_t* n_contacts,
-                              void* stream) {
+                              float* contacts_out, int32_t* contact_counts_out, void* stream) {
   if (!s || !joint_q || !joint_qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok ||
       !n_contacts)
     return fail("na_art_attempt: null argument");
+  if ((contacts_out == nullptr) != (contact_counts_out == nullptr))
+    return fail("na_art_attempt: contacts_out and contact_counts_out go together");
   if (!s->model->articulated) return fail("na_art_attempt: the model has no joints");
   const NaModel& M = s->model->M;
   k_art_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, joint_q, joint_qd, dt, body_f, to_control(control), q_full,
-                                             qd_full, q_dbl, qd_dbl, err, solve_ok, n_contacts);
+                                             qd_full, q_dbl, qd_dbl, err, solve_ok, n_contacts, contacts_out,
+                                             contact_counts_out);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -145,7 +145,7 @@ def uneven_pile(world_count: int = 4096, seed: int = 0, device="cuda:0", max_pri
 
 def serving_requests(count: int, seed: int = 0, rate: float | None = None, max_prims: int = 64,
                      horizons=(200, 2000), tols=(1.0e-2, 1.0e-3, 1.0e-4), device="cpu", distinct: int | None = None,
-                     articulated_share: float = 0.0):
+                     articulated_share: float = 0.0, return_articulated_model: bool = False):
     """Rollout requests of the serving workload: request ``i`` starts from world ``i % distinct``
     of ``uneven_pile(distinct, seed, max_prims=max_prims)`` (1..max_prims primitives), with its
     own horizon of U{horizons} accepted steps, a tolerance drawn from ``tols`` and Poisson
@@ -154,7 +154,10 @@ def serving_requests(count: int, seed: int = 0, rate: float | None = None, max_p
     (an arm or a quadruped on box terrain, from ``articulated_mix``), with the same horizon,
     tolerance and arrival draws.  Returns ``(model, requests)``; the pile model's global shapes
     (the ground, with the same material as the articulated scenes' ground) are the shared
-    environment."""
+    environment.  Request ``i`` records its source in ``rid`` (= i) and, for articulated requests,
+    ``source_world`` (the world of the articulated model it was taken from);
+    ``return_articulated_model`` adds that model (on ``device``, None without articulated
+    requests) as a third return value."""
     from .serving import RolloutRequest, requests_from_model, world_arrays
 
     distinct = count if distinct is None else min(int(distinct), count)
@@ -165,15 +168,19 @@ def serving_requests(count: int, seed: int = 0, rate: float | None = None, max_p
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count)) if rate else np.zeros(count)
     reqs = requests_from_model(model, horizon_accepted=[int(h) for h in H], tol=[float(t) for t in T],
                                arrivals=[float(a) for a in arrivals], worlds=[i % distinct for i in range(count)])
+    amodel = None
     if articulated_share > 0.0:
         art = np.flatnonzero(np.random.default_rng(seed + 2_000_003).random(count) < float(articulated_share))
         if len(art):
             n_art = max(1, min(len(art), int(math.ceil(distinct * float(articulated_share)))))
-            amodel = articulated_mix(n_art, seed=seed + 7, device="cpu")
+            amodel = articulated_mix(n_art, seed=seed + 7, device=device)
             for k, i in enumerate(art):
                 b, s, j = world_arrays(amodel, k % n_art)
                 reqs[i] = RolloutRequest(b, s, j, horizon_accepted=int(H[i]), tol=float(T[i]),
                                          arrival=float(arrivals[i]), rid=int(i))
+                reqs[i].source_world = k % n_art
+    if return_articulated_model:
+        return model, reqs, amodel
     return model, reqs
 
 

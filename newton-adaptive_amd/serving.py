@@ -54,6 +54,7 @@ class RolloutRequest:
     tol: float | None = None
     arrival: float = 0.0
     rid: int = -1
+    source_world: int = -1  # world of the source model an articulated request was taken from (scenes)
     # filled in by the engine
     admitted_at: float = math.nan
     finished_at: float = math.nan

@@ -1401,10 +1401,28 @@ typedef struct {
   int64_t* cum_accepted;
 } or_ctrl;
 
-/* Attempt of one world from (q, qd): writes full/double results, returns error (sentinel on NaN). */
+/* Contact records of the device's parity entry points (16 values per contact: shape a, shape b,
+   body a, body b (global, -1 static), manifold slot, unit normal A->B, world midpoint, distance,
+   phi0, stiffness, friction, 0) as solver contacts; the writer radii and margins are not used by
+   the solve and are left zero. */
+static void contacts_from_records(const double* rec, int n, or_contact* out) {
+  for (int i = 0; i < n; ++i) {
+    const double* r = rec + 16 * i;
+    or_contact* c = out + i;
+    memset(c, 0, sizeof(*c));
+    c->shape_a = (int)r[0]; c->shape_b = (int)r[1]; c->body_a = (int)r[2]; c->body_b = (int)r[3];
+    c->sub = (int)r[4];
+    for (int k = 0; k < 3; ++k) { c->n[k] = (real)r[5 + k]; c->c[k] = (real)r[8 + k]; }
+    c->dist = (real)r[11]; c->phi0 = (real)r[12]; c->k = (real)r[13]; c->mu = (real)r[14];
+  }
+}
+
+/* Attempt of one world from (q, qd): writes full/double results, returns error (sentinel on NaN).
+   given (NULL: collide) holds the contact sets to solve with at q and at the midpoint state,
+   given[0..given_n[0]) and given[cap..cap + given_n[1]) with cap = C->max_contacts. */
 static double attempt(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
                       double dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, int* solve_ok,
-                      int* n_contacts) {
+                      int* n_contacts, const or_contact* given, const int32_t* given_n, int* n_contacts_mid) {
   int nb = W->nb, N = 6 * nb;
   int cap = C->max_contacts;
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
@@ -1413,8 +1431,13 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   double* q_mid = (double*)malloc(sizeof(double) * 7 * (nb > 0 ? nb : 1));
   double* qd_mid = (double*)malloc(sizeof(double) * 6 * (nb > 0 ? nb : 1));
   solve_stats st;
-  int ok = 1, np;
-  int nc = or_collide_world(W->M, W->w, q, cs, cap, &np);
+  int ok = 1, np, nc;
+  if (given) {
+    nc = given_n[0] < cap ? given_n[0] : cap;
+    memcpy(cs, given, sizeof(or_contact) * nc);
+  } else {
+    nc = or_collide_world(W->M, W->w, q, cs, cap, &np);
+  }
   if (nc > cap) nc = cap;
   if (n_contacts) *n_contacts = nc;
   ok &= inner_step(W, C, q, qd, f, dt, cs, nc, NULL, q_full, qd_full, vfull, &st);
@@ -1422,8 +1445,14 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   if (C->mode == 1) {
     for (int i = 0; i < N; ++i) guess[i] = (real)0.5 * ((real)qd[i] + vfull[i]);
     ok &= inner_step(W, C, q, qd, f, 0.5 * dt, cs, nc, guess, q_mid, qd_mid, NULL, &st);
-    nc = or_collide_world(W->M, W->w, q_mid, cs, cap, &np);
+    if (given) {
+      nc = given_n[1] < cap ? given_n[1] : cap;
+      memcpy(cs, given + cap, sizeof(or_contact) * nc);
+    } else {
+      nc = or_collide_world(W->M, W->w, q_mid, cs, cap, &np);
+    }
     if (nc > cap) nc = cap;
+    if (n_contacts_mid) *n_contacts_mid = nc;
     ok &= inner_step(W, C, q_mid, qd_mid, f, 0.5 * dt, cs, nc, vfull, q_dbl, qd_dbl, NULL, &st);
     for (int i = 0; i < 7 * nb; ++i) {
       double d = fabs(q_dbl[i] - q_full[i]);
@@ -1444,16 +1473,43 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
 /* Exported single attempt for parity tests: out_err[w], and full/double states of every world. */
 void or_attempt_all(const or_model* M, const or_config* C, const double* q, const double* qd, const double* f,
                     const double* dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
-                    int32_t* solve_ok, int32_t* n_contacts) {
+                    int32_t* solve_ok, int32_t* n_contacts, int32_t* n_contacts_mid) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int w = 0; w < M->world_count; ++w) {
     int b0 = M->world_body_start[w];
     world_view W = {M, w, b0, M->world_body_start[w + 1] - b0};
-    int ok, nc;
+    int ok, nc, ncm = 0;
     err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, dt[w], q_full + 7 * b0, qd_full + 6 * b0,
-                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc);
+                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, NULL, NULL, &ncm);
     solve_ok[w] = ok;
     n_contacts[w] = nc;
+    if (n_contacts_mid) n_contacts_mid[w] = ncm;
+  }
+}
+
+/* Exported attempt of every world with given contact sets (records [W][2][C->max_contacts][16],
+   counts [W][2]: the contact sets a device attempt solved with at q and at its midpoint): the
+   solves, integration and error norm of the attempt on the same contacts, in the layout of
+   or_attempt_all. */
+void or_attempt_all_given(const or_model* M, const or_config* C, const double* q, const double* qd, const double* f,
+                          const double* dt, const double* records, const int32_t* counts, double* q_full,
+                          double* qd_full, double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok) {
+  int cap = C->max_contacts > 0 ? C->max_contacts : 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int w = 0; w < M->world_count; ++w) {
+    int b0 = M->world_body_start[w];
+    world_view W = {M, w, b0, M->world_body_start[w + 1] - b0};
+    or_contact* given = (or_contact*)malloc(sizeof(or_contact) * 2 * cap);
+    const int32_t* cw = counts + 2 * w;
+    int32_t n[2] = {cw[0] < cap ? cw[0] : cap, cw[1] < cap ? cw[1] : cap};
+    const double* rw = records + (size_t)w * 2 * cap * 16;
+    contacts_from_records(rw, n[0], given);
+    contacts_from_records(rw + 16 * cap, n[1], given + cap);
+    int ok, nc;
+    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, dt[w], q_full + 7 * b0, qd_full + 6 * b0,
+                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, given, n, NULL);
+    solve_ok[w] = ok;
+    free(given);
   }
 }
 
@@ -1547,7 +1603,7 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
       double step = ctrl_clamp(K, w);
       int ok = 1, nc;
       double err;
-      if (step > 0) err = attempt(&W, C, qw, qdw, fw, step, qf, qdf, qdb, qddb, &ok, &nc);
+      if (step > 0) err = attempt(&W, C, qw, qdw, fw, step, qf, qdf, qdb, qddb, &ok, &nc, NULL, NULL, NULL);
       else { err = 0; memcpy(qdb, qw, sizeof(double) * 7 * nb); memcpy(qddb, qdw, sizeof(double) * 6 * nb);
              memcpy(qf, qw, sizeof(double) * 7 * nb); memcpy(qdf, qdw, sizeof(double) * 6 * nb); }
       if (!ok) status = 2;
@@ -2050,7 +2106,8 @@ int or_art_substep(const or_model* M, const or_art* A, const or_config* C, int w
    the full step only (the doubled result is the full one, error 0 unless non-finite). */
 static double art_attempt_world(const or_model* M, const or_art* A, const or_config* C, int w, const double* q,
                                 const double* v, double h, double* q_full, double* v_full, double* q_dbl,
-                                double* v_dbl, int* ok_out, int* n_contacts) {
+                                double* v_dbl, int* ok_out, int* n_contacts, const or_contact* given,
+                                const int32_t* given_n, int* n_contacts_mid) {
   int j0 = A->world_joint_start[w], j1 = A->world_joint_start[w + 1];
   int nq = A->joint_q_start[j1] - A->joint_q_start[j0], nv = A->joint_qd_start[j1] - A->joint_qd_start[j0];
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (C->max_contacts > 0 ? C->max_contacts : 1));
@@ -2059,14 +2116,19 @@ static double art_attempt_world(const or_model* M, const or_art* A, const or_con
   double* g = (double*)calloc(nv + 1, sizeof(double));
   solve_stats st;
   int ok = 1;
-  int nc = art_collide(M, A, C, w, q, cs);
+  int cap = C->max_contacts > 0 ? C->max_contacts : 1;
+  int nc;
+  if (given) { nc = given_n[0] < cap ? given_n[0] : cap; memcpy(cs, given, sizeof(or_contact) * nc); }
+  else nc = art_collide(M, A, C, w, q, cs);
   if (n_contacts) *n_contacts = nc;
   ok &= art_inner_step(M, A, C, w, q, v, h, cs, nc, NULL, q_full, v_full, &st);
   double e = 0;
   if (C->mode == 1) {
     for (int i = 0; i < nv; ++i) g[i] = 0.5 * (v[i] + v_full[i]);
     ok &= art_inner_step(M, A, C, w, q, v, 0.5 * h, cs, nc, g, qm, vm, &st);
-    nc = art_collide(M, A, C, w, qm, cs);
+    if (given) { nc = given_n[1] < cap ? given_n[1] : cap; memcpy(cs, given + cap, sizeof(or_contact) * nc); }
+    else nc = art_collide(M, A, C, w, qm, cs);
+    if (n_contacts_mid) *n_contacts_mid = nc;
     ok &= art_inner_step(M, A, C, w, qm, vm, 0.5 * h, cs, nc, v_full, q_dbl, v_dbl, &st);
     for (int i = 0; i < nq; ++i) {
       double d = fabs(q_dbl[i] - q_full[i]);
@@ -2088,17 +2150,43 @@ static double art_attempt_world(const or_model* M, const or_art* A, const or_con
    full / doubled coordinates and velocities, FP64 error norms (1e10 sentinel), solve status. */
 void or_art_attempt_all(const or_model* M, const or_art* A, const or_config* C, const double* q, const double* v,
                         const double* dt, double* q_full, double* v_full, double* q_dbl, double* v_dbl, double* err,
-                        int32_t* solve_ok, int32_t* n_contacts) {
+                        int32_t* solve_ok, int32_t* n_contacts, int32_t* n_contacts_mid) {
   int W = M->world_count;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int w = 0; w < W; ++w) {
     int j0 = A->world_joint_start[w];
     int q0 = A->joint_q_start[j0], d0 = A->joint_qd_start[j0];
-    int ok = 1, nc = 0;
+    int ok = 1, nc = 0, ncm = 0;
     err[w] = art_attempt_world(M, A, C, w, q + q0, v + d0, dt[w], q_full + q0, v_full + d0, q_dbl + q0, v_dbl + d0,
-                               &ok, &nc);
+                               &ok, &nc, NULL, NULL, &ncm);
     solve_ok[w] = ok;
     n_contacts[w] = nc;
+    if (n_contacts_mid) n_contacts_mid[w] = ncm;
+  }
+}
+
+/* Exported attempt of every articulated world with given contact sets (records and counts as in
+   or_attempt_all_given), in the layout of or_art_attempt_all. */
+void or_art_attempt_all_given(const or_model* M, const or_art* A, const or_config* C, const double* q,
+                              const double* v, const double* dt, const double* records, const int32_t* counts,
+                              double* q_full, double* v_full, double* q_dbl, double* v_dbl, double* err,
+                              int32_t* solve_ok) {
+  int cap = C->max_contacts > 0 ? C->max_contacts : 1;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int w = 0; w < M->world_count; ++w) {
+    int j0 = A->world_joint_start[w];
+    int q0 = A->joint_q_start[j0], d0 = A->joint_qd_start[j0];
+    or_contact* given = (or_contact*)malloc(sizeof(or_contact) * 2 * cap);
+    const int32_t* cw = counts + 2 * w;
+    int32_t n[2] = {cw[0] < cap ? cw[0] : cap, cw[1] < cap ? cw[1] : cap};
+    const double* rw = records + (size_t)w * 2 * cap * 16;
+    contacts_from_records(rw, n[0], given);
+    contacts_from_records(rw + 16 * cap, n[1], given + cap);
+    int ok, nc;
+    err[w] = art_attempt_world(M, A, C, w, q + q0, v + d0, dt[w], q_full + q0, v_full + d0, q_dbl + q0, v_dbl + d0,
+                               &ok, &nc, given, n, NULL);
+    solve_ok[w] = ok;
+    free(given);
   }
 }
 
@@ -2127,7 +2215,8 @@ int or_art_step_frame(const or_model* M, const or_art* A, const or_config* C, do
       double step = ctrl_clamp(K, w);
       int ok = 1, nc = 0;
       double err = 0;
-      if (step > 0 && j1 > j0) err = art_attempt_world(M, A, C, w, qw, vw, step, qf, vf, qb, vb, &ok, &nc);
+      if (step > 0 && j1 > j0)
+        err = art_attempt_world(M, A, C, w, qw, vw, step, qf, vf, qb, vb, &ok, &nc, NULL, NULL, NULL);
       else { memcpy(qf, qw, sizeof(double) * nq); memcpy(qb, qw, sizeof(double) * nq);
              memcpy(vf, vw, sizeof(double) * nv); memcpy(vb, vw, sizeof(double) * nv); }
       if (!ok) status = 2;

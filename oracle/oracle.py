@@ -224,11 +224,12 @@ def art_attempt_all(om, oa, cfg, q, v, dt):
     W = om.world_count
     qf, qb = np.zeros_like(q), np.zeros_like(q)
     vf, vb = np.zeros_like(v), np.zeros_like(v)
-    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32)
+    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32); ncm = np.zeros(W, np.int32)
     lib().or_art_attempt_all(C.byref(om.desc), C.byref(oa.desc), C.byref(cfg), _ptr(q, f64p), _ptr(v, f64p),
                              _ptr(dt, f64p), _ptr(qf, f64p), _ptr(vf, f64p), _ptr(qb, f64p), _ptr(vb, f64p),
-                             _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p))
-    return dict(q_full=qf, v_full=vf, q_double=qb, v_double=vb, err=err, solve_ok=ok, n_contacts=nc)
+                             _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p), _ptr(ncm, i32p))
+    return dict(q_full=qf, v_full=vf, q_double=qb, v_double=vb, err=err, solve_ok=ok, n_contacts=nc,
+                n_contacts_mid=ncm)
 
 
 def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
@@ -399,11 +400,55 @@ def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt):
     q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
     qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
-    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32)
+    err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32); ncm = np.zeros(W, np.int32)
     lib().or_attempt_all(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
-                                  _ptr(dt, f64p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
-                                  _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p))
-    return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok, n_contacts=nc)
+                         _ptr(dt, f64p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
+                         _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p), _ptr(ncm, i32p))
+    return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok, n_contacts=nc,
+                n_contacts_mid=ncm)
+
+
+def _records(cfg, contacts, counts):
+    """Device contact records [W, 2, 128, 16] and counts [W, 2] laid out for cfg.max_contacts."""
+    cap = max(int(cfg.max_contacts), 1)
+    contacts = np.asarray(contacts)
+    W = contacts.shape[0]
+    n = np.minimum(np.asarray(counts, dtype=np.int32).reshape(W, 2), cap)
+    rec = np.zeros((W, 2, cap, 16))
+    m = min(cap, contacts.shape[2])
+    rec[:, :, :m] = contacts[:, :, :m]
+    return np.ascontiguousarray(rec), np.ascontiguousarray(n)
+
+
+def attempt_all_given(om: OracleModel, cfg: OrConfig, q, qd, f, dt, contacts, counts):
+    """Step-doubling attempt of every world that solves with the given contact sets (a device
+    attempt's records [W, 2, 128, 16] at q and at its midpoint, counts [W, 2]) instead of colliding;
+    the layout of :func:`attempt_all`."""
+    W, B = om.world_count, om.body_count
+    q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
+    rec, n = _records(cfg, contacts, counts)
+    qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
+    err = np.zeros(W); ok = np.zeros(W, np.int32)
+    lib().or_attempt_all_given(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
+                               _ptr(dt, f64p), _ptr(rec, f64p), _ptr(n, i32p), _ptr(qf, f64p), _ptr(qdf, f64p),
+                               _ptr(qb, f64p), _ptr(qdb, f64p), _ptr(err, f64p), _ptr(ok, i32p))
+    return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok)
+
+
+def art_attempt_all_given(om: OracleModel, oa: "OracleArt", cfg: OrConfig, q, v, dt, contacts, counts):
+    """Articulated counterpart of :func:`attempt_all_given` (the layout of :func:`art_attempt_all`)."""
+    q = np.ascontiguousarray(q, dtype=np.float64); v = np.ascontiguousarray(v, dtype=np.float64)
+    dt = np.ascontiguousarray(dt, dtype=np.float64)
+    W = om.world_count
+    rec, n = _records(cfg, contacts, counts)
+    qf, qb = np.zeros_like(q), np.zeros_like(q)
+    vf, vb = np.zeros_like(v), np.zeros_like(v)
+    err = np.zeros(W); ok = np.zeros(W, np.int32)
+    lib().or_art_attempt_all_given(C.byref(om.desc), C.byref(oa.desc), C.byref(cfg), _ptr(q, f64p), _ptr(v, f64p),
+                                   _ptr(dt, f64p), _ptr(rec, f64p), _ptr(n, i32p), _ptr(qf, f64p), _ptr(vf, f64p),
+                                   _ptr(qb, f64p), _ptr(vb, f64p), _ptr(err, f64p), _ptr(ok, i32p))
+    return dict(q_full=qf, v_full=vf, q_double=qb, v_double=vb, err=err, solve_ok=ok)
 
 
 class OracleSolver:

@@ -3,7 +3,8 @@ the FP64 generalized-coordinate oracle.
 
 * one inner step per world (``na_art_substep``): coordinates within 1e-6 (metres / radians /
   quaternion components), velocities within 100x the oracle's own response to a float32-size
-  perturbation of the coordinates (or 1e-5), converged solves and equal contact counts;
+  perturbation of the coordinates, at least 1e-5 and at most 1e-4 relative (the largest response
+  and every world above 1e-5 are printed), converged solves and equal contact counts;
 * one step-doubling attempt (``na_art_attempt``): error norms within 1e-7 absolute plus 1e-3
   relative, doubled coordinates within 1e-6;
 * the served boundary call (``na_step_articulated``) against the oracle's frame driver over several
@@ -59,7 +60,7 @@ def _art_substep(nalib, solver, q, v, h, body_f=None, control=None):
     return qo.cpu().numpy(), vo.cpu().numpy(), st.cpu().numpy()
 
 
-def _art_attempt(nalib, solver, q, v, dt, body_f=None, control=None):
+def _art_attempt(nalib, solver, q, v, dt, body_f=None, control=None, with_contacts=False):
     W = solver.model.world_count
     qt, vt = _t(q), _t(v)
     out = {k: torch.zeros_like(qt) for k in ("q_full", "q_double")}
@@ -70,14 +71,20 @@ def _art_attempt(nalib, solver, q, v, dt, body_f=None, control=None):
     dtt = torch.full((W,), dt, dtype=torch.float64, device=DEV)
     bf = _t(body_f, torch.float32) if body_f is not None else None
     ctl = solver._control_desc(control)
+    cs = torch.zeros((W, 2, 128, 16), dtype=torch.float32, device=DEV) if with_contacts else None
+    cn = torch.zeros((W, 2), dtype=torch.int32, device=DEV) if with_contacts else None
     nalib.check(nalib.lib().na_art_attempt(solver._h, qt.data_ptr(), vt.data_ptr(), dtt.data_ptr(),
                                            bf.data_ptr() if bf is not None else None, ctl,
                                            out["q_full"].data_ptr(), out["v_full"].data_ptr(),
                                            out["q_double"].data_ptr(), out["v_double"].data_ptr(), err.data_ptr(),
-                                           ok.data_ptr(), nc.data_ptr(), nalib.stream_handle()), "na_art_attempt")
+                                           ok.data_ptr(), nc.data_ptr(), cs.data_ptr() if with_contacts else None,
+                                           cn.data_ptr() if with_contacts else None, nalib.stream_handle()),
+                "na_art_attempt")
     torch.cuda.synchronize()
     r = {k: t.cpu().numpy() for k, t in out.items()}
     r.update(err=err.cpu().numpy(), solve_ok=ok.cpu().numpy(), n_contacts=nc.cpu().numpy())
+    if with_contacts:
+        r.update(contacts=cs.cpu().numpy(), contact_counts=cn.cpu().numpy())
     return r
 
 
@@ -93,6 +100,7 @@ def test_art_substep_matches_oracle(na, O, nalib, attempts):
     om, oa = O.OracleModel(model), O.OracleArt(model)
     cfg = O.make_config(tol=1e-3, opt_rel_tol=1e-12, max_iterations=200)
     rng = np.random.default_rng(0)
+    worst, loose = 0.0, []
     for w in range(model.world_count):
         q0, q1 = oa.world_q_start[w], oa.world_q_start[w + 1]
         d0, d1 = oa.world_qd_start[w], oa.world_qd_start[w + 1]
@@ -108,7 +116,12 @@ def test_art_substep_matches_oracle(na, O, nalib, attempts):
             _, pv, pst, _ = O.art_substep(om, oa, cfg, w, qp, v[d0:d1], 2e-3)
             if pst[3] == rst[3]:
                 sens = max(sens, np.abs(pv - rv).max() / vscale)
-        assert dv <= max(1e-5, 100.0 * sens), f"world {w}: dv={dv:.2e} oracle sensitivity={sens:.2e}"
+        worst = max(worst, sens)
+        if dv > 1e-5:
+            loose.append((w, float(dv), float(sens)))
+        assert dv <= min(max(1e-5, 100.0 * sens), 1e-4), f"world {w}: dv={dv:.2e} oracle sensitivity={sens:.2e}"
+    print(f"articulated substep ({attempts} rollout attempts): max oracle sensitivity s_w = {worst:.2e}; worlds "
+          f"with dv > 1e-5 (world, dv, s_w): {[(w, f'{a:.1e}', f'{b:.1e}') for w, a, b in loose]}", flush=True)
 
 
 @pytest.mark.gpu

@@ -1,13 +1,15 @@
 """GPU step-doubling path against the FP64 oracle (tolerances are stated per check).
 
 * one inner step per world (``na_substep``, both solves converged to a relative optimality of
-  1e-9): pose within 1e-5 relative to max(|q|, 1); velocity within max(1e-5, 100 s_w) relative to
-  max(|v|, 1 m/s), where s_w is how far the FP64 oracle's own velocity moves when its input pose
-  is perturbed at float32 rounding level (6e-8 relative) -- the contact problem of a settled
-  pile amplifies input rounding by up to ~1e3, so a fixed 1e-5 velocity bar would test the
-  conditioning of the problem rather than the kernel;
+  1e-9): pose within 1e-5 relative to max(|q|, 1); velocity within min(max(1e-5, 100 s_w), 1e-4)
+  relative to max(|v|, 1 m/s), where s_w is how far the FP64 oracle's own velocity moves when its
+  input pose is perturbed at float32 rounding level (6e-8 relative) -- the contact problem of a
+  settled pile amplifies input rounding by up to ~1e3, so a fixed 1e-5 velocity bar would test the
+  conditioning of the problem rather than the kernel; the relaxation is capped at 1e-4, and the
+  largest s_w and every world above 1e-5 are printed;
 * one step-doubling attempt (``na_attempt``): accepted-step error norms within 1e-6 absolute,
-  doubled pose within 1e-5;
+  doubled pose within 1e-5, for every world: on the contact sets the device solved with, and end
+  to end wherever the oracle's own collision found the same contact counts (``tests/parity.py``);
 * boundary calls: every world lands on dt_outer; repeated runs are bit-identical; the lock-step
   form (``na_step_plain``) and the persistent served form give bit-identical states (same
   per-world arithmetic); fixed mode does constant work; tighter tolerance does more work.
@@ -23,6 +25,7 @@ import torch
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 GE = importlib.import_module("__graft_entry__")
+P = importlib.import_module("parity")
 
 
 def _settle(na, model, frames, **kw):
@@ -73,6 +76,7 @@ def test_substep_matches_oracle(na, O, nalib, name, frames):
     qn, qdn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64)
     rng = np.random.default_rng(0)
     zero = np.zeros_like(qdn)
+    worst, loose = 0.0, []
     for w in range(model.world_count):
         b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
         rq, rv, rst, ok = O.substep_world(om, cfg, w, qn, qdn, zero, 2e-3)
@@ -87,7 +91,12 @@ def test_substep_matches_oracle(na, O, nalib, name, frames):
             _, pv, pst, _ = O.substep_world(om, cfg, w, qp, qdn, zero, 2e-3)
             if pst[3] == rst[3]:
                 sens = max(sens, np.abs(pv - rv).max() / vscale)
-        assert dv <= max(1e-5, 100.0 * sens), f"world {w}: dv={dv:.2e} oracle sensitivity={sens:.2e}"
+        worst = max(worst, sens)
+        if dv > 1e-5:
+            loose.append((w, float(dv), float(sens)))
+        assert dv <= min(max(1e-5, 100.0 * sens), 1e-4), f"world {w}: dv={dv:.2e} oracle sensitivity={sens:.2e}"
+    print(f"{name} after {frames} frames: max oracle sensitivity s_w = {worst:.2e}; worlds with dv > 1e-5 "
+          f"(world, dv, s_w): {[(w, f'{a:.1e}', f'{b:.1e}') for w, a, b in loose]}", flush=True)
 
 
 def _settle_q64(na, model, state):
@@ -102,18 +111,16 @@ def test_attempt_error_norm_matches_oracle(na, O, name, frames):
     q64 = solver.body_q64.clone()
     qd = state.body_qd.clone()
     f = torch.zeros_like(qd)
+    om = O.OracleModel(model)
     for dt in (1e-3, 4e-3):
         dtt = torch.full((model.world_count,), dt, dtype=torch.float64, device=DEV)
-        out = GE._attempt(solver, q64, qd, f, dtt)
-        ref = O.attempt_all(O.OracleModel(model), O.make_config(tol=1e-3), q64.cpu().numpy(),
-                            qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy())
+        out = GE._attempt(solver, q64, qd, f, dtt, with_contacts=True)
+        cfg = O.make_config(tol=1e-3)
+        qn, qdn, fn, dtn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy()
+        ref = O.attempt_all(om, cfg, qn, qdn, fn, dtn)
         assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
-        same_nc = out["n_contacts"] == ref["n_contacts"]
-        assert same_nc.mean() > 0.95
-        de = np.abs(out["err"] - ref["err"])[same_nc]
-        assert de.max() <= 1e-6, f"max |d err| = {de.max():.2e}"
-        dq = np.abs(out["q_double"] - ref["q_double"]).max(axis=1)
-        assert dq.max() <= 1e-5
+        print(f"{name} after {frames} frames, dt {dt}", flush=True)
+        P.check_free_attempts(O, om, cfg, out, ref, qn, qdn, fn, dtn, model.host)
 
 
 @pytest.mark.parametrize("frames", [3, 30])
@@ -129,17 +136,15 @@ def test_stiff_stack_attempt_at_tol_1e4_matches_oracle(na, O, frames):
     q64 = solver.body_q64.clone()
     qd = state.body_qd.clone()
     f = torch.zeros_like(qd)
+    om = O.OracleModel(model)
     for dt in (2.5e-4, 1e-3):
         dtt = torch.full((model.world_count,), dt, dtype=torch.float64, device=DEV)
-        out = GE._attempt(solver, q64, qd, f, dtt)
-        ref = O.attempt_all(O.OracleModel(model), O.make_config(tol=1e-4), q64.cpu().numpy(),
-                            qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy())
+        out = GE._attempt(solver, q64, qd, f, dtt, with_contacts=True)
+        cfg = O.make_config(tol=1e-4)
+        qn, qdn, fn, dtn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy()
+        ref = O.attempt_all(om, cfg, qn, qdn, fn, dtn)
         assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
-        same_nc = out["n_contacts"] == ref["n_contacts"]
-        assert same_nc.mean() > 0.95
-        de = np.abs(out["err"] - ref["err"])[same_nc]
-        assert de.max() <= 1e-6, f"max |d err| = {de.max():.2e}"
-        assert np.abs(out["q_double"] - ref["q_double"]).max() <= 1e-5
+        P.check_free_attempts(O, om, cfg, out, ref, qn, qdn, fn, dtn, model.host)
 
 
 def test_boundary_determinism_and_plain_equals_served(na):

@@ -180,3 +180,50 @@ def test_box_drop_1000_accepted_steps_comes_to_rest(na, O):
     # third row of the rotation matrix: world z of the box axes
     rz = np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=1)
     assert np.all(np.abs(rz).max(axis=1) > 1 - 1e-4)
+
+
+def _records(O, om, model, w, q_all):
+    """The oracle's own contacts of world w at q_all as device-style records [128, 16]."""
+    n, _, rows = O.collide_world(om, w, q_all, cap=128)
+    sb = model.host("shape_body")
+    rec = np.zeros((128, 16))
+    for i, r in enumerate(rows):
+        a, b = int(r[0]), int(r[1])
+        rec[i, :5] = [a, b, sb[a], sb[b], r[2]]
+        rec[i, 5:8], rec[i, 8:11] = r[3:6], r[6:9]
+        rec[i, 11:15] = [r[9], r[10], r[11], r[12]]
+    return rec, min(n, 128)
+
+
+@pytest.mark.parametrize("mode", ["fixed", "adaptive"])
+def test_attempt_on_given_contacts_equals_own_collision(na, O, mode):
+    """Given the contact sets its own collision produces at q and at the midpoint state, the
+    given-contact attempt (the entry the GPU parity tests feed with the device's contacts)
+    reproduces the oracle's own attempt bit for bit."""
+    m = na.scenes.uneven_pile(6, seed=12, device="cpu", max_prims=8)
+    om = O.OracleModel(m)
+    q, qd, f = _state(m)
+    cfg = O.make_config(mode=mode, tol=1e-3, dt_init=1e-3, dt_max=0.01, max_substeps=128, max_iterations=100)
+    drv = O.OracleSolver(om, cfg)
+    for _ in range(25):  # land the piles so that there are contacts
+        drv.step(q, qd, f, 0.01)
+    dt = np.full(m.world_count, 2e-3)
+    own = O.attempt_all(om, cfg, q, qd, f, dt)
+    W = m.world_count
+    contacts = np.zeros((W, 2, 128, 16))
+    counts = np.zeros((W, 2), np.int32)
+    wb = m.host("world_body_start")
+    for w in range(W):
+        contacts[w, 0], counts[w, 0] = _records(O, om, m, w, q)
+        if mode == "adaptive":  # the midpoint state of the attempt: full step, then the first half step
+            qf, vf, _, _ = O.substep_world(om, cfg, w, q, qd, f, dt[w])
+            g = 0.5 * (qd[wb[w]:wb[w + 1]] + vf)
+            qm, _, _, _ = O.substep_world(om, cfg, w, q, qd, f, 0.5 * dt[w], guess=g)
+            qmid = q.copy()
+            qmid[wb[w]:wb[w + 1]] = qm
+            contacts[w, 1], counts[w, 1] = _records(O, om, m, w, qmid)
+    assert counts[:, 0].sum() > 0
+    given = O.attempt_all_given(om, cfg, q, qd, f, dt, contacts, counts)
+    assert np.array_equal(given["err"], own["err"])
+    assert np.array_equal(given["q_double"], own["q_double"]) and np.array_equal(given["q_full"], own["q_full"])
+    assert np.array_equal(given["solve_ok"], own["solve_ok"])
