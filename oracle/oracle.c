@@ -916,9 +916,24 @@ static real project(real mu, real Rt, real Rn, const real* y, real* gam, real* G
     gam[0] = gam[1] = gam[2] = 0;
   } else {
     real t0 = y[0] / yr, t1 = y[1] / yr;
+    /* OR_PLANT (tests only, never in the default build): plausible mistakes in the sliding branch
+       that the friction pins must catch -- 1 drops mu mu_hat from the denominator, 2 drops the
+       mu_hat |y_t| term of the normal impulse, 3 scales the tangential impulse by mu^2 */
+#if defined(OR_PLANT) && OR_PLANT == 1
+    real den = 1;
+#else
     real den = 1 + mu * muh;
+#endif
+#if defined(OR_PLANT) && OR_PLANT == 2
+    real gn = yn / den;
+#else
     real gn = (yn + muh * yr) / den;
+#endif
+#if defined(OR_PLANT) && OR_PLANT == 3
+    gam[0] = mu * mu * gn * t0; gam[1] = mu * mu * gn * t1; gam[2] = gn;
+#else
     gam[0] = mu * gn * t0; gam[1] = mu * gn * t1; gam[2] = gn;
+#endif
     real a = mu * muh / den, b = mu * gn / yr;
     real P[9];
     P[0] = a * t0 * t0 + b * (1 - t0 * t0); P[1] = a * t0 * t1 - b * t0 * t1; P[2] = mu / den * t0;
@@ -1388,6 +1403,55 @@ int or_substep_world(const or_model* M, const or_config* C, int w, const double*
   if (stats) { stats[0] = st.iterations; stats[1] = st.factorizations; stats[2] = st.converged; stats[3] = nc; }
   free(cs);
   return ok;
+}
+
+/* Exported convex problem of one inner step of world w (contacts at q) and its solution, for
+   checks that re-evaluate the SAP optimality from the problem data alone: N = 6 nb dofs, A [N,N]
+   (dense), v* [N], contact rows J [3 nc, N] (t1, t2, n per contact), Rt, Rn, v_hat, mu [nc], and
+   the converged velocity v [N].  Capacities: N <= n_cap, nc <= c_cap.  Returns nc (or -1 when a
+   capacity is too small); *converged = the solve met its optimality test. */
+int or_substep_problem(const or_model* M, const or_config* C, int w, const double* q, const double* qd,
+                       const double* f, double h, int n_cap, int c_cap, double* A_out, double* vstar_out,
+                       double* J_out, double* Rt_out, double* Rn_out, double* vhat_out, double* mu_out,
+                       double* v_out, int* converged) {
+  world_view W = {M, w, M->world_body_start[w], M->world_body_start[w + 1] - M->world_body_start[w]};
+  int cap = C->max_contacts;
+  or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
+  int np;
+  int nc = or_collide_world(M, w, q, cs, cap, &np);
+  if (nc > cap) nc = cap;
+  int N = 6 * W.nb;
+  if (N > n_cap || nc > c_cap) { free(cs); return -1; }
+  sap_ctx S;
+  sap_alloc(&S, W.nb, nc);
+  sap_setup(&S, &W, C, q, qd, f, (real)h, cs, nc);
+  real* v = S.v;
+  for (int i = 0; i < N; ++i) v[i] = (real)qd[i];
+  solve_stats st;
+  *converged = sap_newton(&S, C, v, &st);
+  memset(A_out, 0, sizeof(double) * N * N);
+  real* e = S.tmp + 3 * N;  /* unit vectors through apply_A: the dense A */
+  for (int j = 0; j < N; ++j) {
+    real col[6 * 64];
+    for (int i = 0; i < N; ++i) e[i] = (real)(i == j);
+    apply_A(&S, e, col);
+    for (int i = 0; i < N; ++i) A_out[i * N + j] = col[i];
+  }
+  memset(J_out, 0, sizeof(double) * 3 * nc * N);
+  for (int c = 0; c < nc; ++c) {
+    for (int s2 = 0; s2 < 2; ++s2) {
+      int b = S.cb[2 * c + s2];
+      if (b < 0) continue;
+      const real* Jb = S.J + (2 * c + s2) * 18;
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 6; ++k) J_out[(3 * c + r) * N + 6 * b + k] += Jb[6 * r + k];
+    }
+    Rt_out[c] = S.Rt[c]; Rn_out[c] = S.Rn[c]; vhat_out[c] = S.vhat[c]; mu_out[c] = S.mu[c];
+  }
+  for (int i = 0; i < N; ++i) { vstar_out[i] = S.vstar[i]; v_out[i] = v[i]; }
+  sap_free(&S);
+  free(cs);
+  return nc;
 }
 
 /* ====================================================================

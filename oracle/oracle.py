@@ -75,24 +75,51 @@ def _ptr(a, t):
 _LIBS: dict = {}
 
 
+def build_planted(plant: int) -> str:
+    """Compile the oracle with planted mistake ``plant`` in the friction projection (-DOR_PLANT,
+    see ``project`` in oracle.c) into ``_build/liboracle64_plant<k>.so``; tests use it to show that
+    their checks catch such a mistake."""
+    out = os.path.join(_BUILD, f"liboracle64_plant{int(plant)}.so")
+    os.makedirs(_BUILD, exist_ok=True)
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-ffp-contract=off", "-fopenmp",
+                    f"-DOR_PLANT={int(plant)}", "-o", out, os.path.join(_HERE, "oracle.c"), "-lm"], check=True)
+    return out
+
+
+class use_library:
+    """Context manager: route every oracle call to the library at ``path`` (a planted build)."""
+
+    def __init__(self, path: str):
+        self.path = path
+
+    def __enter__(self):
+        lib()
+        self.saved = _LIBS["liboracle64.so"]
+        _LIBS["liboracle64.so"] = _load(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        _LIBS["liboracle64.so"] = self.saved
+        return False
+
+
 def lib():
     name = "liboracle64.so"
     if name not in _LIBS:
         path = os.path.join(_BUILD, name)
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        L.or_collide_world_flat.restype = C.c_int
-        L.or_world_pairs_global.restype = C.c_int
-        L.or_collide_shapes.restype = C.c_int
-        L.or_substep_world.restype = C.c_int
-        L.or_step_frame.restype = C.c_int
-        L.or_art_step_frame.restype = C.c_int
-        L.or_cubic_min.restype = C.c_int
-        L.or_cubic_min.argtypes = [C.c_double] * 5 + [f64p]
-        L.or_num_threads.restype = C.c_int
-        _LIBS[name] = L
+        _LIBS[name] = _load(path)
     return _LIBS[name]
+
+
+def _load(path):
+    L = C.CDLL(path)
+    for name in ("or_collide_world_flat", "or_world_pairs_global", "or_collide_shapes", "or_substep_world",
+                 "or_step_frame", "or_art_step_frame", "or_cubic_min", "or_num_threads"):
+        getattr(L, name).restype = C.c_int
+    L.or_cubic_min.argtypes = [C.c_double] * 5 + [f64p]
+    return L
 
 
 class OracleModel:
@@ -393,6 +420,28 @@ def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, guess=Non
                                 C.c_double(h), _ptr(g, f64p) if g is not None else None, _ptr(qo, f64p),
                                 _ptr(qdo, f64p), _ptr(stats, i32p))
     return qo, qdo, stats, ok
+
+
+def substep_problem(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h):
+    """The convex problem of one inner step of world ``w`` (contacts at q) and the oracle's solution:
+    dict(A [N,N], vstar [N], J [3 nc, N] rows t1, t2, n per contact, Rt, Rn, vhat, mu [nc], v [N],
+    converged), N = 6 bodies."""
+    b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
+    q = np.ascontiguousarray(q[b0:b1], dtype=np.float64)
+    qd = np.ascontiguousarray(qd[b0:b1], dtype=np.float64)
+    f = np.ascontiguousarray(f[b0:b1], dtype=np.float64)
+    N, ncap = 6 * (b1 - b0), max(int(cfg.max_contacts), 1)
+    A = np.zeros((N, N)); vs = np.zeros(N); J = np.zeros((3 * ncap, N)); v = np.zeros(N)
+    Rt, Rn, vh, mu = (np.zeros(ncap) for _ in range(4))
+    conv = C.c_int(0)
+    L = lib()
+    L.or_substep_problem.restype = C.c_int
+    nc = L.or_substep_problem(C.byref(om.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
+                              C.c_double(h), N, ncap, _ptr(A, f64p), _ptr(vs, f64p), _ptr(J, f64p), _ptr(Rt, f64p),
+                              _ptr(Rn, f64p), _ptr(vh, f64p), _ptr(mu, f64p), _ptr(v, f64p), C.byref(conv))
+    assert nc >= 0
+    return dict(A=A, vstar=vs, J=J[:3 * nc], Rt=Rt[:nc], Rn=Rn[:nc], vhat=vh[:nc], mu=mu[:nc], v=v,
+                converged=bool(conv.value))
 
 
 def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt):
