@@ -2,9 +2,11 @@
 
 Workload (default): 4096 uneven worlds, each N ~ U{1..32} primitives from {sphere, box, capsule}
 (size lognormal(ln 0.08 m, 0.4)) dropped into a 1 m^2 walled bin, step-doubling error control at
-tol 1e-3, dt_outer = 10 ms.  A "step" is one 10 ms boundary frame of every world.  `--warmup`
-untimed boundary calls let the piles land; the timed region then runs exactly `--steps` frames
-of every world from that settled state in three forms that leave bit-identical states:
+tol 1e-3, dt_outer = 10 ms, after 20 settling frames (part of the workload: the piles land).  A
+"step" is one 10 ms boundary frame of every world; the timed region runs exactly `--steps` frames
+of every world from the settled state.  `--warmup` is the number of untimed repeats of the timed
+launch from the same settled state before the timed one, so the timed window never depends on
+it.  The frames run in three forms that leave bit-identical states:
 
 * served (`value`): the rollout form of the serving engine -- one persistent launch (`na_serve`)
   in which waves take worlds from an atomic list and march each one through all its frames at
@@ -15,8 +17,11 @@ of every world from that settled state in three forms that leave bit-identical s
 
 The JSON line also carries the roofline of the dominant kernel (algorithmic bytes of the work
 recorded by the per-world device counters over the launch time measured with HIP events on the
-launch stream), a serving run with Poisson arrivals (p50/p99 rollout latency) and a CPU
-baseline: the FP64 CPU restatement timed on a bounded sample of the same workload.
+launch stream), a serving run with Poisson arrivals (p50/p99 rollout latency), a CPU
+baseline -- the FP32 CPU restatement of the same algorithm (FP64 residuals, as on the device) timed
+on all host cores and on one, on bounded samples of the same window -- and, from the same CPU leg,
+the precision of the device's attempts against the FP64 restatement on a sample of the timed
+worlds.
 """
 
 from __future__ import annotations
@@ -34,6 +39,7 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 KW = dict(dt_inner_init=1e-3, max_substeps=256)
+SETTLE_FRAMES = 20  # the piles land before the timed window (part of the named workload)
 
 
 def algorithmic_bytes(nb, ns, work, adaptive=True):
@@ -58,33 +64,98 @@ def algorithmic_bytes(nb, ns, work, adaptive=True):
     return float(b.sum())
 
 
-def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, sample_worlds, frames, threads):
-    """Time the FP64 CPU restatement (OpenMP over worlds) on the first `sample_worlds` worlds of
-    the workload, starting from the same settled state as the timed GPU region."""
+def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, samples, frames, threads, log):
+    """Time the FP32 CPU restatement of the path (FP32 geometry, contact rows, Hessian and dense
+    factor; FP64 Newton residual, cost, projection and line search, as on the device; OpenMP over
+    worlds) from the same settled state as the timed GPU window, on `threads` threads over the
+    first samples[0] worlds and on one thread over the first samples[1] worlds."""
     import numpy as np
 
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     from oracle import oracle as O
 
-    m = model_fn(sample_worlds)
-    om = O.OracleModel(m)
-    cfg = O.make_config(tol=tol, dt_init=KW["dt_inner_init"], dt_max=dt_outer, max_substeps=KW["max_substeps"],
-                        max_iterations=100, opt_rel_tol=max(1e-3 * tol, 1e-8))
-    s = O.OracleSolver(om, cfg)
-    B = m.body_count
-    q = np.ascontiguousarray(q0[:B].astype(np.float64))
-    qd = np.ascontiguousarray(qd0[:B].astype(np.float64))
-    f = np.zeros((B, 6))
-    t0 = time.perf_counter()
-    for _ in range(frames):
-        s.step(q, qd, f, dt_outer)
-    el = time.perf_counter() - t0
-    acc = int(s.a["cum_accepted"][0])
-    return {"value": acc / el, "unit": "accepted world-steps/s", "cores": int(O.lib().or_num_threads()),
-            "kind": "port",
-            "sample": f"FP64 CPU restatement (OpenMP over worlds), first {sample_worlds} worlds of the workload, "
-                      f"{frames} frames of {dt_outer * 1e3:.0f} ms from the same settled state "
-                      f"({acc} accepted steps in {el:.1f} s)"}
+    out = {}
+    with O.use_library(O.library_path("fp32")):
+        for n, nthreads in ((samples[0], threads), (samples[1], 1)):
+            O.set_num_threads(nthreads)
+            m = model_fn(n)
+            om = O.OracleModel(m)
+            cfg = O.make_config(tol=tol, dt_init=KW["dt_inner_init"], dt_max=dt_outer,
+                                max_substeps=KW["max_substeps"], max_iterations=100,
+                                opt_rel_tol=max(1e-3 * tol, 1e-8))
+            s = O.OracleSolver(om, cfg)
+            B = m.body_count
+            q = np.ascontiguousarray(q0[:B].astype(np.float64))
+            qd = np.ascontiguousarray(qd0[:B].astype(np.float64))
+            f = np.zeros((B, 6))
+            t0 = time.perf_counter()
+            fail = 0
+            for _ in range(frames):
+                fail |= s.step(q, qd, f, dt_outer)
+            el = time.perf_counter() - t0
+            acc = int(s.a["cum_accepted"][0])
+            out[nthreads] = (acc / el, acc, el, n, bool(fail == 0))
+            log(f"[bench] cpu baseline ({nthreads} threads): {acc} accepted in {el:.1f} s ({acc / el:.0f}/s)")
+    O.set_num_threads(threads)
+    v, acc, el, n, ok = out[threads]
+    v1, acc1, el1, n1, ok1 = out[1]
+    sample = (f"CPU restatement (FP32 with FP64 residuals, dense Cholesky Newton, OpenMP over worlds), first {n} "
+              f"worlds of the workload, {frames} frames of {dt_outer * 1e3:.0f} ms from the same settled state "
+              f"({acc} accepted steps in {el:.1f} s)")
+    return {"value": v, "unit": "accepted world-steps/s", "cores": int(threads), "kind": "port", "sample": sample,
+            "solves_converged": ok,
+            "single_thread": {"value": v1, "cores": 1, "solves_converged": ok1,
+                              "sample": f"first {n1} worlds, {frames} frames ({acc1} accepted steps in {el1:.1f} s)"}}
+
+
+def precision(na, model, q0, qd0, dt, tol, sample, log):
+    """Per-result error of the device's step-doubling attempts against the FP64 restatement, on the
+    first `sample` worlds from the timed window's start state at each world's working dt: the
+    doubled pose, the error norm and the accept decision; on the contact sets the device solved with
+    (every world), and end to end with the restatement's own collision where both passes found the
+    same contact counts below the per-world capacity."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+
+    GE = importlib.import_module("__graft_entry__")
+    sub = na.scenes.uneven_pile(sample, seed=0, device="cuda:0") if model is None else model
+    B = sub.body_count
+    solver = na.SolverSAPAdaptive(sub, tol=tol, dt_inner_init=KW["dt_inner_init"])
+    q64 = torch.as_tensor(q0[:B], dtype=torch.float64, device="cuda:0")
+    qd = torch.as_tensor(qd0[:B], dtype=torch.float32, device="cuda:0")
+    f = torch.zeros_like(qd)
+    dtt = torch.as_tensor(dt[:sub.world_count], dtype=torch.float64, device="cuda:0")
+    out = GE._attempt(solver, q64, qd, f, dtt, with_contacts=True)
+    om = O.OracleModel(sub)
+    cfg = O.make_config(tol=tol, max_iterations=100)
+    qn, qdn, fn, dtn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy()
+    given = O.attempt_all_given(om, cfg, qn, qdn, fn, dtn, out["contacts"], out["contact_counts"])
+    own = O.attempt_all(om, cfg, qn, qdn, fn, dtn)
+    n = out["contact_counts"]
+    cap = int(cfg.max_contacts)
+    comparable = (n[:, 0] == own["n_contacts"]) & (n[:, 1] == own["n_contacts_mid"]) & (n.max(axis=1) < cap) \
+        & (np.maximum(own["n_contacts"], own["n_contacts_mid"]) < cap)
+
+    def stats(ref, mask):
+        dq = np.abs(out["q_double"] - ref["q_double"]).max(axis=1)
+        wb = sub.host("world_body_start")
+        dqw = np.array([dq[wb[w]:wb[w + 1]].max(initial=0.0) for w in range(sub.world_count)])[mask]
+        de = np.abs(out["err"] - ref["err"])[mask]
+        agree = ((out["err"] <= tol) == (ref["err"] <= tol))[mask]
+        return {"worlds": int(mask.sum()), "q_double_abs_max": float(dqw.max(initial=0.0)),
+                "q_double_abs_median": float(np.median(dqw)) if len(dqw) else 0.0,
+                "err_abs_max": float(de.max(initial=0.0)), "err_abs_median": float(np.median(de)) if len(de) else 0.0,
+                "accept_decisions_agree": float(agree.mean()) if len(agree) else 1.0}
+
+    res = {"on_device_contacts": stats(given, np.ones(sub.world_count, bool)),
+           "end_to_end": stats(own, comparable),
+           "sample": f"first {sub.world_count} worlds at the timed window's start, each at its working dt",
+           "fp64_on_device": "error norm, accept/reject decision, Newton residual / cost / line search, pose shadow",
+           "reference": "FP64 CPU restatement (dense Cholesky Newton)"}
+    log(f"[bench] precision vs FP64: |dq| max {res['on_device_contacts']['q_double_abs_max']:.2e}, "
+        f"|d err| max {res['on_device_contacts']['err_abs_max']:.2e} over {sub.world_count} worlds")
+    return res
 
 
 def articulated_form(na, args, log):
@@ -260,7 +331,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2, help="untimed repeats of the timed launch")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--dt-outer", type=float, default=0.01)
@@ -274,8 +345,10 @@ def main():
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
     ap.add_argument("--serve-articulated", type=float, default=0.25,
                     help="share of requests that are articulated worlds (arm or quadruped on terrain)")
-    ap.add_argument("--cpu-worlds", type=int, default=256)
+    ap.add_argument("--cpu-worlds", type=int, default=256, help="CPU baseline sample on all host cores")
+    ap.add_argument("--cpu-worlds-1t", type=int, default=24, help="CPU baseline sample on one core")
     ap.add_argument("--cpu-frames", type=int, default=10)
+    ap.add_argument("--precision-worlds", type=int, default=512)
     ap.add_argument("--no-articulated", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--articulated-only", action="store_true",
@@ -306,35 +379,45 @@ def main():
 
     model = na.scenes.uneven_pile(args.worlds, seed=args.seed, device=dev)
     W = model.world_count
-    log(f"[bench] {W} worlds, {model.body_count} bodies; {args.warmup} warm-up frames")
+    log(f"[bench] {W} worlds, {model.body_count} bodies; {SETTLE_FRAMES} settling frames, "
+        f"{args.warmup} untimed repeats of the timed launch")
     st = model.state()
     warm = na.SolverSAPAdaptive(model, **kw)
-    for _ in range(args.warmup):
+    for _ in range(SETTLE_FRAMES):
         warm.step(st, st, model.control(), None, args.dt_outer)
     torch.cuda.synchronize()
     q0, qd0 = st.body_q.clone(), st.body_qd.clone()
+    dt0 = warm._ideal_dt.clamp(max=args.dt_outer).cpu().numpy().copy()  # each world's working step
     del warm
 
     # ---------------------------------------------------------------- served rollout (timed)
-    eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
-                           shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
-                           device=dev, **kw)
-    reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
-    eng.submit(reqs)
-    eng.admit(math.inf)
-    eng.solver.reset_work_counters()
-    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    eng.launch()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    eng.retire(wall)
-    assert not eng.resident, "every rollout finishes in the one launch without a time budget"
-    launch_ms = e0.elapsed_time(e1)
+
+    def served_run():
+        """One served rollout launch of args.steps frames of every world from the settled state."""
+        eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
+                               shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
+                               device=dev, **kw)
+        reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
+        eng.submit(reqs)
+        eng.admit(math.inf)
+        eng.solver.reset_work_counters()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        eng.launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        eng.retire(wall)
+        assert not eng.resident, "every rollout finishes in the one launch without a time budget"
+        return eng, reqs, wall, e0.elapsed_time(e1)
+
+    for k in range(args.warmup):
+        _, _, w_wall, _ = served_run()
+        log(f"[bench] untimed repeat {k + 1}/{args.warmup} of the timed launch: {w_wall:.3f} s")
+    eng, reqs, wall, launch_ms = served_run()
     accepted = sum(r.accepted for r in reqs)
     value = accepted / wall
     slots = np.array([r.slot for r in reqs])
@@ -371,7 +454,8 @@ def main():
         "data": "synthetic (seeded scenes)",
         "config": {"workload": f"uneven_pile_{W}: {W} worlds x U{{1..32}} sphere/box/capsule in a 1 m^2 bin, "
                                f"tol {args.tol:g}, {args.steps} frames of {args.dt_outer * 1e3:g} ms after "
-                               f"{args.warmup} warm-up frames",
+                               f"{SETTLE_FRAMES} settling frames",
+                   "settle_frames": SETTLE_FRAMES,
                    "worlds": W, "tol": args.tol, "dt_outer": args.dt_outer, "seed": args.seed},
         "served": {"form": "persistent rollout launch (na_serve), no barrier between worlds or frames",
                    "accepted": accepted, "wall_s": wall, "finite": finite, "attempts": int(wk[0]),
@@ -469,9 +553,12 @@ def main():
             return na.scenes.uneven_pile(n, seed=args.seed, device="cpu")
 
         out["cpu_baseline"] = cpu_baseline(cpu_model, q0.cpu().numpy(), qd0.cpu().numpy(), args.tol, args.dt_outer,
-                                           args.cpu_worlds, args.cpu_frames, threads)
+                                           (args.cpu_worlds, args.cpu_worlds_1t), args.cpu_frames, threads, log)
+        # the same CPU leg checks the device's attempts against the FP64 restatement
+        out["precision"] = precision(na, na.scenes.uneven_pile(args.precision_worlds, seed=args.seed, device=dev),
+                                     q0.cpu().numpy(), qd0.cpu().numpy(), dt0, args.tol, args.precision_worlds, log)
     traffic_file = os.path.join(REPO, "profiles", "k_serve_uneven_pile_4096_traffic.json")
-    if os.path.exists(traffic_file) and W == 4096 and args.steps == 20 and args.warmup == 20:
+    if os.path.exists(traffic_file) and W == 4096 and args.steps == 20 and args.seed == 0:
         with open(traffic_file) as fh:
             tr = json.load(fh)
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]

@@ -61,6 +61,12 @@ typedef double real;
 #define RA(x) fabs(x)
 #endif
 
+/* `acc` carries what the device path keeps in FP64 whatever `real` is: the Newton iterate, the
+   gradient, the cost, the friction-cone projection and the line search.  The FP32 build (OR_FLOAT,
+   the CPU baseline) thus runs the device's precision placement -- FP32 geometry, contact rows,
+   Hessian and factor, FP64 residual -- and the FP64 build is unchanged. */
+typedef double acc;
+
 #define OR_MAXVAL 1.0e10
 #define OR_PI 3.14159265358979323846
 
@@ -890,14 +896,14 @@ typedef struct {
   real* Iw;    /* [nb*9] world inertia */
   real* Iwi;   /* [nb*9] its inverse */
   real* xc;    /* [nb*3] COM position */
-  real* vstar; /* [N] */
-  real* Ad;    /* [N] diag(A) */
+  acc* vstar; /* [N] */
+  acc* Ad;    /* [N] diag(A) */
   /* contacts */
   int* cb;     /* [nc*2] local body indices (-1 static) */
   real* J;     /* [nc*2*18] 3x6 blocks per body (rows t1, t2, n) */
   real* Rt; real* Rn; real* vhat; real* mu;
   /* scratch */
-  real* H; real* g; real* dv; real* v; real* vc; real* dvc; real* gam; real* G; real* tmp;
+  real* H; acc* g; acc* dv; acc* v; acc* gam; real* G; acc* tmp; real* rtmp;
   /* generalized-coordinate problems (articulated worlds): dense A [N,N] and dense contact rows
      [nc, 3, N]; NULL for free bodies (block-diagonal A, per-body 3x6 blocks) */
   real* Adense;
@@ -905,53 +911,56 @@ typedef struct {
 } sap_ctx;
 
 /* P_F projection in the R norm and its Jacobian-derived Hessian G = dP/dy R^-1 */
-static real project(real mu, real Rt, real Rn, const real* y, real* gam, real* G) {
-  real yr = RS(y[0] * y[0] + y[1] * y[1]), yn = y[2];
-  real muh = mu * Rt / Rn;
+static acc project(acc mu, acc Rt, acc Rn, const acc* y, acc* gam, real* G) {
+  acc yr = sqrt(y[0] * y[0] + y[1] * y[1]), yn = y[2];
+  acc muh = mu * Rt / Rn;
   for (int i = 0; i < 9; ++i) G[i] = 0;
   if (yr <= mu * yn) {
     gam[0] = y[0]; gam[1] = y[1]; gam[2] = y[2];
-    G[0] = 1 / Rt; G[4] = 1 / Rt; G[8] = 1 / Rn;
+    G[0] = (real)(1 / Rt); G[4] = (real)(1 / Rt); G[8] = (real)(1 / Rn);
   } else if (yn <= -muh * yr) {
     gam[0] = gam[1] = gam[2] = 0;
   } else {
-    real t0 = y[0] / yr, t1 = y[1] / yr;
+    acc t0 = y[0] / yr, t1 = y[1] / yr;
     /* OR_PLANT (tests only, never in the default build): plausible mistakes in the sliding branch
        that the friction pins must catch -- 1 drops mu mu_hat from the denominator, 2 drops the
        mu_hat |y_t| term of the normal impulse, 3 scales the tangential impulse by mu^2 */
 #if defined(OR_PLANT) && OR_PLANT == 1
-    real den = 1;
+    acc den = 1;
 #else
-    real den = 1 + mu * muh;
+    acc den = 1 + mu * muh;
 #endif
 #if defined(OR_PLANT) && OR_PLANT == 2
-    real gn = yn / den;
+    acc gn = yn / den;
 #else
-    real gn = (yn + muh * yr) / den;
+    acc gn = (yn + muh * yr) / den;
 #endif
 #if defined(OR_PLANT) && OR_PLANT == 3
     gam[0] = mu * mu * gn * t0; gam[1] = mu * mu * gn * t1; gam[2] = gn;
 #else
     gam[0] = mu * gn * t0; gam[1] = mu * gn * t1; gam[2] = gn;
 #endif
-    real a = mu * muh / den, b = mu * gn / yr;
-    real P[9];
+    acc a = mu * muh / den, b = mu * gn / yr;
+    acc P[9];
     P[0] = a * t0 * t0 + b * (1 - t0 * t0); P[1] = a * t0 * t1 - b * t0 * t1; P[2] = mu / den * t0;
     P[3] = a * t1 * t0 - b * t1 * t0;       P[4] = a * t1 * t1 + b * (1 - t1 * t1); P[5] = mu / den * t1;
     P[6] = muh / den * t0;                  P[7] = muh / den * t1;               P[8] = 1 / den;
-    for (int r = 0; r < 3; ++r) { G[3 * r + 0] = P[3 * r + 0] / Rt; G[3 * r + 1] = P[3 * r + 1] / Rt; G[3 * r + 2] = P[3 * r + 2] / Rn; }
+    for (int r = 0; r < 3; ++r) {
+      G[3 * r + 0] = (real)(P[3 * r + 0] / Rt); G[3 * r + 1] = (real)(P[3 * r + 1] / Rt);
+      G[3 * r + 2] = (real)(P[3 * r + 2] / Rn);
+    }
   }
-  return (real)0.5 * (Rt * (gam[0] * gam[0] + gam[1] * gam[1]) + Rn * gam[2] * gam[2]);
+  return 0.5 * (Rt * (gam[0] * gam[0] + gam[1] * gam[1]) + Rn * gam[2] * gam[2]);
 }
 
-static void contact_velocity(const sap_ctx* S, int c, const real* v, real* vc) {
+static void contact_velocity(const sap_ctx* S, int c, const acc* v, acc* vc) {
   vc[0] = vc[1] = vc[2] = 0;
   if (S->Jd) {
     for (int r = 0; r < 3; ++r) {
       const real* row = S->Jd + ((size_t)3 * c + r) * S->N;
-      real acc = 0;
-      for (int k = 0; k < S->N; ++k) acc += row[k] * v[k];
-      vc[r] = acc;
+      acc sum = 0;
+      for (int k = 0; k < S->N; ++k) sum += row[k] * v[k];
+      vc[r] = sum;
     }
     return;
   }
@@ -959,44 +968,45 @@ static void contact_velocity(const sap_ctx* S, int c, const real* v, real* vc) {
     int b = S->cb[2 * c + s];
     if (b < 0) continue;
     const real* Jb = S->J + (2 * c + s) * 18;
-    const real* vb = v + 6 * b;
+    const acc* vb = v + 6 * b;
     for (int r = 0; r < 3; ++r) {
-      real acc = 0;
-      for (int k = 0; k < 6; ++k) acc += Jb[6 * r + k] * vb[k];
-      vc[r] += acc;
+      acc sum = 0;
+      for (int k = 0; k < 6; ++k) sum += Jb[6 * r + k] * vb[k];
+      vc[r] += sum;
     }
   }
 }
 
 /* momentum A x for block-diagonal A = diag(m I, Iw) */
-static void apply_A(const sap_ctx* S, const real* x, real* y) {
+static void apply_A(const sap_ctx* S, const acc* x, acc* y) {
   if (S->Adense) {
     for (int i = 0; i < S->N; ++i) {
-      real acc = 0;
-      for (int k = 0; k < S->N; ++k) acc += S->Adense[(size_t)i * S->N + k] * x[k];
-      y[i] = acc;
+      acc sum = 0;
+      for (int k = 0; k < S->N; ++k) sum += S->Adense[(size_t)i * S->N + k] * x[k];
+      y[i] = sum;
     }
     return;
   }
   for (int b = 0; b < S->nb; ++b) {
-    const real* xb = x + 6 * b; real* yb = y + 6 * b; const real* I = S->Iw + 9 * b;
+    const acc* xb = x + 6 * b; acc* yb = y + 6 * b; const real* I = S->Iw + 9 * b;
     yb[0] = S->m[b] * xb[0]; yb[1] = S->m[b] * xb[1]; yb[2] = S->m[b] * xb[2];
     for (int r = 0; r < 3; ++r) yb[3 + r] = I[3 * r] * xb[3] + I[3 * r + 1] * xb[4] + I[3 * r + 2] * xb[5];
   }
 }
 
 /* cost, gradient (g = A(v-v*) - J^T gamma) and, if G != NULL, per-contact Hessian blocks */
-static real eval_cost(sap_ctx* S, const real* v, real* g, real* gam, real* G, real* pnorm, real* jnorm) {
+static acc eval_cost(sap_ctx* S, const acc* v, acc* g, acc* gam, real* G, acc* pnorm, acc* jnorm) {
   int N = S->N;
-  real* d = S->tmp;
+  acc* d = S->tmp;
   for (int i = 0; i < N; ++i) d[i] = v[i] - S->vstar[i];
   apply_A(S, d, g);
-  real cost = 0;
-  for (int i = 0; i < N; ++i) cost += (real)0.5 * d[i] * g[i];
-  real* jc = S->tmp + N;
+  acc cost = 0;
+  for (int i = 0; i < N; ++i) cost += 0.5 * d[i] * g[i];
+  acc* jc = S->tmp + N;
   for (int i = 0; i < N; ++i) jc[i] = 0;
   for (int c = 0; c < S->nc; ++c) {
-    real vc[3], y[3], gl[3], Gl[9];
+    acc vc[3], y[3], gl[3];
+    real Gl[9];
     contact_velocity(S, c, v, vc);
     y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
     cost += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
@@ -1018,30 +1028,31 @@ static real eval_cost(sap_ctx* S, const real* v, real* g, real* gam, real* G, re
   }
   for (int i = 0; i < N; ++i) g[i] -= jc[i];
   if (pnorm) {
-    real* p = S->tmp + 2 * N;
+    acc* p = S->tmp + 2 * N;
     apply_A(S, v, p);
-    real pn = 0, jn = 0;
-    for (int i = 0; i < N; ++i) { real s = 1 / RS(S->Ad[i]); pn += (s * p[i]) * (s * p[i]); jn += (s * jc[i]) * (s * jc[i]); }
-    *pnorm = RS(pn); *jnorm = RS(jn);
+    acc pn = 0, jn = 0;
+    for (int i = 0; i < N; ++i) { acc s = 1 / sqrt(S->Ad[i]); pn += (s * p[i]) * (s * p[i]); jn += (s * jc[i]) * (s * jc[i]); }
+    *pnorm = sqrt(pn); *jnorm = sqrt(jn);
   }
   return cost;
 }
 
 /* l(alpha) along dv: value, first and second derivative */
-static void ls_eval(sap_ctx* S, const real* v, const real* dv, real alpha, real* l, real* dl, real* d2l) {
+static void ls_eval(sap_ctx* S, const acc* v, const acc* dv, acc alpha, acc* l, acc* dl, acc* d2l) {
   int N = S->N;
-  real* d = S->tmp;
-  real* Adv = S->tmp + N;
+  acc* d = S->tmp;
+  acc* Adv = S->tmp + N;
   for (int i = 0; i < N; ++i) d[i] = v[i] + alpha * dv[i] - S->vstar[i];
   apply_A(S, dv, Adv);
-  real mom = 0, dm = 0, ddm = 0;
-  real* Ad = S->tmp + 2 * N;
+  acc mom = 0, dm = 0, ddm = 0;
+  acc* Ad = S->tmp + 2 * N;
   apply_A(S, d, Ad);
-  for (int i = 0; i < N; ++i) { mom += (real)0.5 * d[i] * Ad[i]; dm += dv[i] * Ad[i]; ddm += dv[i] * Adv[i]; }
-  real* va = S->tmp + 3 * N;
+  for (int i = 0; i < N; ++i) { mom += 0.5 * d[i] * Ad[i]; dm += dv[i] * Ad[i]; ddm += dv[i] * Adv[i]; }
+  acc* va = S->tmp + 3 * N;
   for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
   for (int c = 0; c < S->nc; ++c) {
-    real vc[3], dvc[3], y[3], gl[3], Gl[9];
+    acc vc[3], dvc[3], y[3], gl[3];
+    real Gl[9];
     contact_velocity(S, c, va, vc);
     contact_velocity(S, c, dv, dvc);
     y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
@@ -1072,32 +1083,33 @@ static void chol_solve(const real* L, int N, real* x) {
   for (int i = N - 1; i >= 0; --i) { real s = x[i]; for (int k = i + 1; k < N; ++k) s -= L[k * N + i] * x[k]; x[i] = s / L[i * N + i]; }
 }
 
-typedef struct { int iterations, factorizations, converged; real residual; } solve_stats;
+typedef struct { int iterations, factorizations, converged; acc residual; } solve_stats;
 
 /* Decrease l(v + alpha dv) - l(v) of the line function, summed term by term: the momentum part in
    closed form (alpha dv^T A (v - v*) + alpha^2 / 2 dv^T A dv), the contact costs as per-contact
    differences, so that it is not lost in the rounding of l itself. */
-static real ls_delta(sap_ctx* S, const real* v, const real* dv, real alpha) {
+static acc ls_delta(sap_ctx* S, const acc* v, const acc* dv, acc alpha) {
   int N = S->N;
-  real* d = S->tmp;
-  real* Ad = S->tmp + N;
-  real* Adv = S->tmp + 2 * N;
-  real* va = S->tmp + 3 * N;
+  acc* d = S->tmp;
+  acc* Ad = S->tmp + N;
+  acc* Adv = S->tmp + 2 * N;
+  acc* va = S->tmp + 3 * N;
   for (int i = 0; i < N; ++i) d[i] = v[i] - S->vstar[i];
   apply_A(S, d, Ad);
   apply_A(S, dv, Adv);
-  real b0 = 0, aa = 0;
+  acc b0 = 0, aa = 0;
   for (int i = 0; i < N; ++i) { b0 += dv[i] * Ad[i]; aa += dv[i] * Adv[i]; }
-  real delta = alpha * b0 + (real)0.5 * alpha * alpha * aa;
+  acc delta = alpha * b0 + 0.5 * alpha * alpha * aa;
   for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
   for (int c = 0; c < S->nc; ++c) {
-    real vc0[3], vca[3], y[3], gl[3], Gl[9];
+    acc vc0[3], vca[3], y[3], gl[3];
+    real Gl[9];
     contact_velocity(S, c, v, vc0);
     contact_velocity(S, c, va, vca);
     y[0] = -vca[0] / S->Rt[c]; y[1] = -vca[1] / S->Rt[c]; y[2] = -(vca[2] - S->vhat[c]) / S->Rn[c];
-    real ca = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    acc ca = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
     y[0] = -vc0[0] / S->Rt[c]; y[1] = -vc0[1] / S->Rt[c]; y[2] = -(vc0[2] - S->vhat[c]) / S->Rn[c];
-    real c0 = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    acc c0 = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
     delta += ca - c0;
   }
   return delta;
@@ -1111,12 +1123,12 @@ static real ls_delta(sap_ctx* S, const real* v, const real* dv, real alpha) {
    l(a) - l(0) < c a l'(0), else ends with the last step if it meets that condition.  Returns the
    step, or -1 when no evaluated step met it (near the optimum the decrease can fall below the
    rounding of the contact costs; the caller then takes the exact search's step). */
-static real armijo_decay(sap_ctx* S, const real* v, real dl0) {
-  const real alpha_max = (real)1.5, rho = (real)0.8, c = (real)1.0e-4;
-  real alpha = alpha_max;
-  real d = ls_delta(S, v, S->dv, alpha);
+static acc armijo_decay(sap_ctx* S, const acc* v, acc dl0) {
+  const acc alpha_max = 1.5, rho = 0.8, c = 1.0e-4;
+  acc alpha = alpha_max;
+  acc d = ls_delta(S, v, S->dv, alpha);
   int armijo = d < c * alpha * dl0;
-  real alpha_prev = alpha, d_prev = d;
+  acc alpha_prev = alpha, d_prev = d;
   for (int k = 1; k < 40; ++k) {
     alpha *= rho;
     d = ls_delta(S, v, S->dv, alpha);
@@ -1125,19 +1137,19 @@ static real armijo_decay(sap_ctx* S, const real* v, real dl0) {
     alpha_prev = alpha;
     d_prev = d;
   }
-  return armijo ? alpha : (real)-1;
+  return armijo ? alpha : -1.0;
 }
 
-static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) {
+static int sap_newton(sap_ctx* S, const or_config* C, acc* v, solve_stats* st) {
   int N = S->N;
-  real rel = (real)C->opt_rel_tol, abs_tol = (real)C->opt_abs_tol;
+  acc rel = C->opt_rel_tol, abs_tol = C->opt_abs_tol;
   st->iterations = 0; st->factorizations = 0; st->converged = 0;
   for (int it = 0; it <= C->max_iterations; ++it) {
-    real pn, jn;
+    acc pn, jn;
     eval_cost(S, v, S->g, S->gam, S->G, &pn, &jn);
-    real gn = 0;
-    for (int i = 0; i < N; ++i) { real s = S->g[i] / RS(S->Ad[i]); gn += s * s; }
-    gn = RS(gn);
+    acc gn = 0;
+    for (int i = 0; i < N; ++i) { acc s = S->g[i] / sqrt(S->Ad[i]); gn += s * s; }
+    gn = sqrt(gn);
     st->residual = gn;
     if (!(gn == gn)) return 0;
     if (gn <= abs_tol + rel * maxr(pn, jn)) { st->converged = 1; return 1; }
@@ -1180,32 +1192,34 @@ static int sap_newton(sap_ctx* S, const or_config* C, real* v, solve_stats* st) 
     }
     if (!cholesky(S->H, N)) return 0;
     st->factorizations++;
-    for (int i = 0; i < N; ++i) S->dv[i] = -S->g[i];
-    chol_solve(S->H, N, S->dv);
+    /* the direction from the (real) factor; the iterate, gradient and line search stay in acc */
+    for (int i = 0; i < N; ++i) S->rtmp[i] = (real)(-S->g[i]);
+    chol_solve(S->H, N, S->rtmp);
+    for (int i = 0; i < N; ++i) S->dv[i] = S->rtmp[i];
     /* line search: exact on [0, 1] (root of l'(alpha), cubic-seeded safeguarded Newton) or
        backtracking with Armijo's condition ("armijo_decay") */
-    real l0, d0, dd0, l1, d1, dd1;
+    acc l0, d0, dd0, l1, d1, dd1;
     ls_eval(S, v, S->dv, 0, &l0, &d0, &dd0);
     if (!(d0 < 0)) { st->iterations = it + 1; break; }
-    real alpha = -1;
+    acc alpha = -1;
     if (C->line_search == 1) alpha = armijo_decay(S, v, d0);
     ls_eval(S, v, S->dv, 1, &l1, &d1, &dd1);
     if (alpha > 0) {
       /* backtracking step taken */
     } else if (d1 <= 0) alpha = 1;
     else {
-      real lo = 0, hi = 1;
+      acc lo = 0, hi = 1;
       double ac;
-      alpha = or_cubic_min(l0, d0, l1, d1, 1.0, &ac) ? (real)ac : clampr(-d0 / dd0, 0, 1);
-      if (!(alpha > lo && alpha < hi)) alpha = (real)0.5;
+      alpha = or_cubic_min(l0, d0, l1, d1, 1.0, &ac) ? ac : fmin(fmax(-d0 / dd0, 0.0), 1.0);
+      if (!(alpha > lo && alpha < hi)) alpha = 0.5;
       for (int k = 0; k < 100; ++k) {
-        real l, d, dd;
+        acc l, d, dd;
         ls_eval(S, v, S->dv, alpha, &l, &d, &dd);
-        if (RA(d) <= (real)C->ls_rel_tol * RA(d0)) break;
+        if (fabs(d) <= C->ls_rel_tol * fabs(d0)) break;
         if (d < 0) lo = alpha; else hi = alpha;
-        real an = dd > 0 ? alpha - d / dd : -1;
-        if (!(an > lo && an < hi)) an = (real)0.5 * (lo + hi);
-        if (hi - lo <= (real)1e-14 * hi) { alpha = an; break; }
+        acc an = dd > 0 ? alpha - d / dd : -1;
+        if (!(an > lo && an < hi)) an = 0.5 * (lo + hi);
+        if (hi - lo <= 1e-14 * hi) { alpha = an; break; }
         alpha = an;
       }
     }
@@ -1311,26 +1325,25 @@ static void sap_alloc(sap_ctx* S, int nb, int nc) {
   int ncc = nc > 0 ? nc : 1;
   S->m = (real*)calloc(nb, sizeof(real)); S->Iw = (real*)calloc(9 * nb, sizeof(real));
   S->Iwi = (real*)calloc(9 * nb, sizeof(real)); S->xc = (real*)calloc(3 * nb, sizeof(real));
-  S->vstar = (real*)calloc(N, sizeof(real)); S->Ad = (real*)calloc(N, sizeof(real));
+  S->vstar = (acc*)calloc(N, sizeof(acc)); S->Ad = (acc*)calloc(N, sizeof(acc));
   S->cb = (int*)calloc(2 * ncc, sizeof(int)); S->J = (real*)calloc(36 * ncc, sizeof(real));
   S->Rt = (real*)calloc(ncc, sizeof(real)); S->Rn = (real*)calloc(ncc, sizeof(real));
   S->vhat = (real*)calloc(ncc, sizeof(real)); S->mu = (real*)calloc(ncc, sizeof(real));
-  S->H = (real*)calloc((size_t)N * N, sizeof(real)); S->g = (real*)calloc(N, sizeof(real));
-  S->dv = (real*)calloc(N, sizeof(real)); S->v = (real*)calloc(N, sizeof(real));
-  S->gam = (real*)calloc(3 * ncc, sizeof(real)); S->G = (real*)calloc(9 * ncc, sizeof(real));
-  S->tmp = (real*)calloc(4 * N, sizeof(real));
-  S->vc = NULL; S->dvc = NULL;
+  S->H = (real*)calloc((size_t)N * N, sizeof(real)); S->g = (acc*)calloc(N, sizeof(acc));
+  S->dv = (acc*)calloc(N, sizeof(acc)); S->v = (acc*)calloc(N, sizeof(acc));
+  S->gam = (acc*)calloc(3 * ncc, sizeof(acc)); S->G = (real*)calloc(9 * ncc, sizeof(real));
+  S->tmp = (acc*)calloc(4 * N, sizeof(acc)); S->rtmp = (real*)calloc(N > 0 ? N : 1, sizeof(real));
   S->Adense = NULL; S->Jd = NULL;
 }
 static void sap_free(sap_ctx* S) {
   free(S->m); free(S->Iw); free(S->Iwi); free(S->xc); free(S->vstar); free(S->Ad); free(S->cb); free(S->J);
   free(S->Rt); free(S->Rn); free(S->vhat); free(S->mu); free(S->H); free(S->g); free(S->dv); free(S->v);
-  free(S->gam); free(S->G); free(S->tmp);
+  free(S->gam); free(S->G); free(S->tmp); free(S->rtmp);
 }
 
 /* Semi-implicit update (solver.py:50-94 without damping): x_com += h v,
  * q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.  Always in double. */
-static void integrate_world(const world_view* W, const double* q, const real* v, double h, double* q_out, double* qd_out) {
+static void integrate_world(const world_view* W, const double* q, const acc* v, double h, double* q_out, double* qd_out) {
   const or_model* M = W->M;
   for (int b = 0; b < W->nb; ++b) {
     int gb = W->b0 + b;
@@ -1343,7 +1356,7 @@ static void integrate_world(const world_view* W, const double* q, const real* v,
                     2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)};
     double xc[3];
     for (int i = 0; i < 3; ++i) xc[i] = bq[i] + R0[3 * i] * cx + R0[3 * i + 1] * cy + R0[3 * i + 2] * cz;
-    const real* vb = v + 6 * b;
+    const acc* vb = v + 6 * b;
     for (int i = 0; i < 3; ++i) xc[i] += h * (double)vb[i];
     double wx = vb[3], wy = vb[4], wz = vb[5];
     /* dq = (w, 0) * q */
@@ -1368,16 +1381,16 @@ static void integrate_world(const world_view* W, const double* q, const real* v,
  * given contacts computed at the step's start pose, solve and integrate.
  * Returns 1 when the convex solve converged. */
 static int inner_step(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
-                      double h, const or_contact* cs, int nc, const real* guess, double* q_out, double* qd_out,
-                      real* v_out, solve_stats* st) {
+                      double h, const or_contact* cs, int nc, const acc* guess, double* q_out, double* qd_out,
+                      acc* v_out, solve_stats* st) {
   sap_ctx S;
   sap_alloc(&S, W->nb, nc);
   sap_setup(&S, W, C, q, qd, f, (real)h, cs, nc);
-  real* v = S.v;
-  for (int i = 0; i < S.N; ++i) v[i] = guess ? guess[i] : (real)qd[i];
+  acc* v = S.v;
+  for (int i = 0; i < S.N; ++i) v[i] = guess ? guess[i] : (acc)qd[i];
   int ok = sap_newton(&S, C, v, st);
   integrate_world(W, q, v, h, q_out, qd_out);
-  if (v_out) memcpy(v_out, v, sizeof(real) * S.N);
+  if (v_out) memcpy(v_out, v, sizeof(acc) * S.N);
   sap_free(&S);
   return ok;
 }
@@ -1393,10 +1406,10 @@ int or_substep_world(const or_model* M, const or_config* C, int w, const double*
   int nc = or_collide_world(M, w, q, cs, cap, &np);
   if (nc > cap) nc = cap;
   solve_stats st;
-  real* g = NULL;
+  acc* g = NULL;
   if (guess) {
-    g = (real*)malloc(sizeof(real) * 6 * (W.nb ? W.nb : 1));
-    for (int i = 0; i < 6 * W.nb; ++i) g[i] = (real)guess[i];
+    g = (acc*)malloc(sizeof(acc) * 6 * (W.nb ? W.nb : 1));
+    for (int i = 0; i < 6 * W.nb; ++i) g[i] = guess[i];
   }
   int ok = inner_step(&W, C, q, qd, f, h, cs, nc, g, q_out, qd_out, NULL, &st);
   free(g);
@@ -1425,15 +1438,15 @@ int or_substep_problem(const or_model* M, const or_config* C, int w, const doubl
   sap_ctx S;
   sap_alloc(&S, W.nb, nc);
   sap_setup(&S, &W, C, q, qd, f, (real)h, cs, nc);
-  real* v = S.v;
-  for (int i = 0; i < N; ++i) v[i] = (real)qd[i];
+  acc* v = S.v;
+  for (int i = 0; i < N; ++i) v[i] = qd[i];
   solve_stats st;
   *converged = sap_newton(&S, C, v, &st);
   memset(A_out, 0, sizeof(double) * N * N);
-  real* e = S.tmp + 3 * N;  /* unit vectors through apply_A: the dense A */
+  acc* e = S.tmp + 3 * N;  /* unit vectors through apply_A: the dense A */
   for (int j = 0; j < N; ++j) {
-    real col[6 * 64];
-    for (int i = 0; i < N; ++i) e[i] = (real)(i == j);
+    acc col[6 * 64];
+    for (int i = 0; i < N; ++i) e[i] = (acc)(i == j);
     apply_A(&S, e, col);
     for (int i = 0; i < N; ++i) A_out[i * N + j] = col[i];
   }
@@ -1490,8 +1503,8 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   int nb = W->nb, N = 6 * nb;
   int cap = C->max_contacts;
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
-  real* vfull = (real*)malloc(sizeof(real) * (N > 0 ? N : 1));
-  real* guess = (real*)malloc(sizeof(real) * (N > 0 ? N : 1));
+  acc* vfull = (acc*)malloc(sizeof(acc) * (N > 0 ? N : 1));
+  acc* guess = (acc*)malloc(sizeof(acc) * (N > 0 ? N : 1));
   double* q_mid = (double*)malloc(sizeof(double) * 7 * (nb > 0 ? nb : 1));
   double* qd_mid = (double*)malloc(sizeof(double) * 6 * (nb > 0 ? nb : 1));
   solve_stats st;
@@ -1507,7 +1520,7 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   ok &= inner_step(W, C, q, qd, f, dt, cs, nc, NULL, q_full, qd_full, vfull, &st);
   double err = 0.0;
   if (C->mode == 1) {
-    for (int i = 0; i < N; ++i) guess[i] = (real)0.5 * ((real)qd[i] + vfull[i]);
+    for (int i = 0; i < N; ++i) guess[i] = 0.5 * (qd[i] + vfull[i]);
     ok &= inner_step(W, C, q, qd, f, 0.5 * dt, cs, nc, guess, q_mid, qd_mid, NULL, &st);
     if (given) {
       nc = given_n[1] < cap ? given_n[1] : cap;
@@ -2020,12 +2033,12 @@ static int art_inner_step(const or_model* M, const or_art* A, const or_config* C
   sap_ctx S;
   sap_alloc(&S, 1, nc);
   /* sap_alloc sized its vectors for 6 dofs; rebuild them for nv */
-  free(S.vstar); free(S.Ad); free(S.H); free(S.g); free(S.dv); free(S.v); free(S.tmp);
+  free(S.vstar); free(S.Ad); free(S.H); free(S.g); free(S.dv); free(S.v); free(S.tmp); free(S.rtmp);
   S.N = nv; S.nb = 0; S.nc = nc;
-  S.vstar = (real*)calloc(nv + 1, sizeof(real)); S.Ad = (real*)calloc(nv + 1, sizeof(real));
-  S.H = (real*)calloc((size_t)nv * nv + 1, sizeof(real)); S.g = (real*)calloc(nv + 1, sizeof(real));
-  S.dv = (real*)calloc(nv + 1, sizeof(real)); S.v = (real*)calloc(nv + 1, sizeof(real));
-  S.tmp = (real*)calloc(4 * nv + 4, sizeof(real));
+  S.vstar = (acc*)calloc(nv + 1, sizeof(acc)); S.Ad = (acc*)calloc(nv + 1, sizeof(acc));
+  S.H = (real*)calloc((size_t)nv * nv + 1, sizeof(real)); S.g = (acc*)calloc(nv + 1, sizeof(acc));
+  S.dv = (acc*)calloc(nv + 1, sizeof(acc)); S.v = (acc*)calloc(nv + 1, sizeof(acc));
+  S.tmp = (acc*)calloc(4 * nv + 4, sizeof(acc)); S.rtmp = (real*)calloc(nv + 1, sizeof(real));
   S.Adense = (real*)calloc((size_t)nv * nv + 1, sizeof(real));
   S.Jd = (real*)calloc((size_t)3 * (nc > 0 ? nc : 1) * nv, sizeof(real));
   double* rhs = (double*)calloc(nv + 1, sizeof(double));
@@ -2055,8 +2068,9 @@ static int art_inner_step(const or_model* M, const or_art* A, const or_config* C
   real* L = (real*)calloc((size_t)nv * nv + 1, sizeof(real));
   memcpy(L, S.Adense, sizeof(real) * nv * nv);
   int spd = cholesky(L, nv);
-  for (int i = 0; i < nv; ++i) S.vstar[i] = (real)rhs[i];
-  if (spd) chol_solve(L, nv, S.vstar);
+  for (int i = 0; i < nv; ++i) S.rtmp[i] = (real)rhs[i];
+  if (spd) chol_solve(L, nv, S.rtmp);
+  for (int i = 0; i < nv; ++i) S.vstar[i] = S.rtmp[i];
   for (int i = 0; i < nv; ++i) S.Ad[i] = S.Adense[(size_t)i * nv + i];
   /* contact rows and regularisation */
   real tau_d = (real)C->tau_d;
@@ -2108,8 +2122,8 @@ static int art_inner_step(const or_model* M, const or_art* A, const or_config* C
     S.vhat[c] = -k->phi0 / ((real)h + tau_d);
     S.mu[c] = k->mu;
   }
-  real* v = S.v;
-  for (int i = 0; i < nv; ++i) v[i] = guess ? (real)guess[i] : (real)v0[i];
+  acc* v = S.v;
+  for (int i = 0; i < nv; ++i) v[i] = guess ? guess[i] : v0[i];
   int ok = spd && sap_newton(&S, C, v, st);
   if (!spd) { st->iterations = 0; st->factorizations = 0; st->converged = 0; }
   double* vd = (double*)calloc(nv + 1, sizeof(double));
@@ -2334,6 +2348,15 @@ void or_coords_from_mujoco(int B, const double* com, const double* qpos, const d
     double* u = jqd + 6 * b;
     u[0] = vc.x; u[1] = vc.y; u[2] = vc.z; u[3] = w.x; u[4] = w.y; u[5] = w.z;
   }
+}
+
+/* OpenMP threads of the following parallel regions (the CPU baseline times all cores and one) */
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }
 
 int or_num_threads(void) {

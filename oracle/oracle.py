@@ -103,6 +103,20 @@ class use_library:
         return False
 
 
+def library_path(precision: str = "fp64") -> str:
+    """Path of the FP64 checker ("fp64") or of the FP32 restatement with FP64 residuals ("fp32", the
+    CPU baseline); both are built by the Makefile."""
+    path = os.path.join(_BUILD, {"fp64": "liboracle64.so", "fp32": "liboracle32.so"}[precision])
+    if not os.path.exists(path):
+        build()
+    return path
+
+
+def set_num_threads(n: int) -> None:
+    """OpenMP threads of the oracle's parallel loops from now on."""
+    lib().or_set_num_threads(int(n))
+
+
 def lib():
     name = "liboracle64.so"
     if name not in _LIBS:
