@@ -87,6 +87,19 @@ typedef struct na_model_desc {
   const float* joint_target_ke;
   const float* joint_target_kd;
   const float* joint_armature;
+  /* Optional MuJoCo geom parameters per shape, needed by contact_law 1 only (the model.mujoco
+     custom attributes of SolverMuJoCo, solver_mujoco.py:700-790, with solref resolved as
+     solver_mujoco.py:5224-5233 does: an authored solref, else convert_solref(ke, kd, 1, 1)).
+     solref [S,2] (time constant, damping ratio; <= 0: direct stiffness/damping), solimp [S,5]
+     (dmin, dmax, width, midpoint, power), solmix [S], priority [S], condim [S] (1 or 3).
+     mj_zero_margins: rows use the surface distance, as SolverMuJoCo zeroes the margins of models
+     with boxes for its native collision (solver_mujoco.py:3352-3357); else dist - margins. */
+  const float* mj_solref;
+  const float* mj_solimp;
+  const float* mj_solmix;
+  const int32_t* mj_priority;
+  const int32_t* mj_condim;
+  int32_t mj_zero_margins;
 } na_model_desc;
 
 /* Solver settings (constructor kwargs of the reference solvers). */
@@ -100,6 +113,10 @@ typedef struct na_config {
   float cg_forcing;        /* > 0: inexact-Newton PCG forcing bound (eta_k <= cg_forcing); 0: fixed cg_rel_tol */
   int32_t line_search;     /* 0 "exact_root" (root of l'(alpha), cubic seed), 1 "armijo_decay" (backtracking,
                               the reference default, solver_sap_adaptive.py:459) */
+  int32_t contact_law;     /* 0 SAP compliant contact (SolverSAPAdaptive); 1 MuJoCo soft constraints with the
+                              pyramidal cone (SolverMuJoCoAdaptive's inner step, mujoco_warp.step,
+                              solver_mujoco_adaptive.py:585-605): needs the model's mj_* arrays; free bodies */
+  float impratio;          /* MuJoCo frictional-to-normal impedance ratio (> 0; contact_law 1) */
 } na_config;
 
 /* Control inputs of a boundary call (newton Control, newton/_src/sim/control.py:31-110; applied by

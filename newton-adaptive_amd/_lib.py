@@ -40,6 +40,8 @@ class NaModelDesc(C.Structure):
         ("joint_type", vp), ("joint_parent", vp), ("joint_q_start", vp), ("joint_qd_start", vp),
         ("joint_X_p", vp), ("joint_X_c", vp), ("joint_axis", vp), ("joint_target_pos", vp),
         ("joint_target_ke", vp), ("joint_target_kd", vp), ("joint_armature", vp),
+        ("mj_solref", vp), ("mj_solimp", vp), ("mj_solmix", vp), ("mj_priority", vp), ("mj_condim", vp),
+        ("mj_zero_margins", i32),
     ]
 
 
@@ -51,7 +53,8 @@ class NaConfig(C.Structure):
         ("max_substeps", i32), ("max_iterations", i32), ("max_rigid_contact", i32), ("broad_phase", i32),
         ("optimality_rel_tol", C.c_float), ("optimality_abs_tol", C.c_float), ("contact_tau_d", C.c_float),
         ("sigma", C.c_float), ("beta", C.c_float), ("ls_rel_tol", C.c_float), ("cg_rel_tol", C.c_float),
-        ("max_cg", i32), ("cg_forcing", C.c_float), ("line_search", i32),
+        ("max_cg", i32), ("cg_forcing", C.c_float), ("line_search", i32), ("contact_law", i32),
+        ("impratio", C.c_float),
     ]
 
 

@@ -55,6 +55,16 @@ struct NaModel {
   const float* joint_target_ke;
   const float* joint_target_kd;
   const float* joint_armature;
+  // MuJoCo geom parameters per shape (null unless the MuJoCo contact law is used): solref [S,2]
+  // (time constant, damping ratio; direct stiffness/damping when the first entry is <= 0),
+  // solimp [S,5], solmix [S], priority [S], condim [S] (1 or 3); mj_zero_margins: contacts use
+  // the surface distance instead of dist - margins, as SolverMuJoCo zeroes margins of box models
+  const float* mj_solref;
+  const float* mj_solimp;
+  const float* mj_solmix;
+  const int32_t* mj_priority;
+  const int32_t* mj_condim;
+  int mj_zero_margins;
 };
 
 // Per-wave LDS scratch of the segment (world shapes + global shapes).
@@ -742,11 +752,76 @@ __device__ __forceinline__ void narrow_pair(const NaModel& M, const SegLDS& S, i
   }
 }
 
+// MuJoCo row parameters of each contact of a set (contact law 1), mixed per pair at contact
+// generation: solref time constant and damping ratio, dmax, and the impedance at the contact's
+// position (ContactSet::k holds the impedance and ContactSet::mu the friction, 0 for condim 1).
+template <int N>
+struct MjContactParams {
+  float tc[N], dr[N], dmax[N];
+};
+#define NA_MJ_MINVAL 1.0e-15f
+#define NA_MJ_MINIMP 0.0001f
+#define NA_MJ_MAXIMP 0.9999f
+#define NA_MJ_MINMU 1.0e-5f
+
+// Impedance d(pos) of solimp (dmin, dmax, width, midpoint, power): the two-piece power curve from
+// dmin at |pos| = 0 to dmax at |pos| >= width (MuJoCo's solimp definition).
+__device__ __forceinline__ float mj_impedance(const float* si, float pos) {
+  float dmin = fminf(fmaxf(si[0], NA_MJ_MINIMP), NA_MJ_MAXIMP), dmax = fminf(fmaxf(si[1], NA_MJ_MINIMP), NA_MJ_MAXIMP);
+  float width = fmaxf(NA_MJ_MINVAL, si[2]), mid = fminf(fmaxf(si[3], NA_MJ_MINIMP), NA_MJ_MAXIMP);
+  float power = fmaxf(1.f, si[4]);
+  float x = fabsf(pos) / width;
+  if (x >= 1.f) return dmax;
+  float y;
+  if (power == 1.f) y = x;
+  else if (x < mid) y = powf(x, power) / powf(mid, power - 1.f);
+  else y = 1.f - powf(1.f - x, power) / powf(1.f - mid, power - 1.f);
+  return fminf(fmaxf(dmin + y * (dmax - dmin), dmin), dmax);
+}
+
+// Contact parameters of a shape pair (contact_params, solvers/mujoco/kernels.py:127-190): the
+// higher priority shape decides; equal priorities mix solref/solimp by solmix (0.5 when both are
+// ~0, one-sided when one is), take the larger condim and the larger friction; a non-positive
+// solref time constant on either side selects the elementwise min.  Friction floored at MJ_MINMU.
+struct MjPair {
+  float solref[2], solimp[5], mu;
+  int condim;
+};
+__device__ __forceinline__ MjPair mj_pair_params(const NaModel& M, int a, int b) {
+  MjPair r;
+  int pa = M.mj_priority[a], pb = M.mj_priority[b];
+  float mix;
+  if (pa != pb) {
+    const int s = pa > pb ? a : b;
+    mix = pa > pb ? 1.f : 0.f;
+    r.condim = M.mj_condim[s];
+    r.mu = M.shape_mu[s];
+  } else {
+    float s1 = M.mj_solmix[a], s2 = M.mj_solmix[b];
+    const float tiny = 2.220446049250313e-16f;
+    mix = s1 + s2 != 0.f ? s1 / (s1 + s2) : 0.f;
+    if (s1 < tiny && s2 < tiny) mix = 0.5f;
+    else if (s1 < tiny) mix = 0.f;
+    else if (s2 < tiny) mix = 1.f;
+    r.condim = max(M.mj_condim[a], M.mj_condim[b]);
+    r.mu = fmaxf(M.shape_mu[a], M.shape_mu[b]);
+  }
+  r.mu = fmaxf(NA_MJ_MINMU, r.mu);
+  const float *ra = M.mj_solref + 2 * a, *rb = M.mj_solref + 2 * b;
+  const bool both = ra[0] > 0.f && rb[0] > 0.f;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) r.solref[k] = both ? mix * ra[k] + (1.f - mix) * rb[k] : fminf(ra[k], rb[k]);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.solimp[k] = mix * M.mj_solimp[5 * a + k] + (1.f - mix) * M.mj_solimp[5 * b + k];
+  return r;
+}
+
 // Stage 3: narrow phase over the pair list, compacted into the LDS contact set in pair order.
 // Returns the number of contacts generated (may exceed `cap`; extra contacts are dropped).
-// `cap` must not exceed the capacity of the contact set.
-template <class CS>
-__device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const SegLDS& S, CS& C) {
+// `cap` must not exceed the capacity of the contact set.  With `mj` (MuJoCo contact law) each
+// contact also gets its MuJoCo row parameters, and k / mu hold its impedance and friction.
+template <class CS, class MJ = MjContactParams<1>>
+__device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const SegLDS& S, CS& C, MJ* mj = nullptr) {
   int nc = 0;
   np = min(np, NA_MAX_PAIRS);
   for (int base = 0; base < np; base += NA_WAVE) {
@@ -766,6 +841,11 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
       int ba = M.shape_body[ga], bb = M.shape_body[gb];
       float kk = 0.5f * (M.shape_ke[ga] + M.shape_ke[gb]);
       float mu = fmaxf(M.shape_mu[ga], M.shape_mu[gb]);
+      MjPair mp;
+      if (mj) {
+        mp = mj_pair_params(M, ga, gb);
+        mu = mp.condim == 1 ? 0.f : mp.mu;
+      }
       int slot = nc + excl;
 #pragma unroll
       for (int q = 0; q < NA_MF; ++q) {
@@ -784,6 +864,14 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
           C.mu[slot] = mu;
           C.ra[slot] = o.ra;
           C.rb[slot] = o.rb;
+          if (mj) {
+            // the row position: dist - margins, or the surface distance where margins are zeroed
+            const float pos = M.mj_zero_margins ? o.m.dist[q] : o.phi0[q];
+            C.k[slot] = mj_impedance(mp.solimp, pos);
+            mj->tc[slot] = mp.solref[0];
+            mj->dr[slot] = mp.solref[1];
+            mj->dmax[slot] = fminf(fmaxf(mp.solimp[1], NA_MJ_MINIMP), NA_MJ_MAXIMP);
+          }
         }
         slot++;
       }

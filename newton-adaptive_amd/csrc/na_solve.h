@@ -23,6 +23,8 @@ struct NaConfig {
   int max_cg;
   float cg_forcing;  // > 0: upper bound of the inexact-Newton PCG forcing term; 0: fixed cg_rel_tol
   int line_search;   // 0 exact (root of l'(alpha), cubic seed), 1 armijo_decay (backtracking)
+  int contact_law;   // 0 SAP compliant cone, 1 MuJoCo soft constraints (pyramidal cone)
+  float impratio;    // MuJoCo frictional-to-normal impedance ratio (contact law 1)
 };
 
 // Body slot NA_DUMMY stands for the static side of a contact: its velocity entries stay zero
@@ -66,6 +68,7 @@ struct WorldLDS {
   };
   ContactLDS con;
   BodyLDS body;
+  MjContactParams<NA_MAX_CONTACTS> mj;  // contact law 1 only
 };
 
 // Per-lane body registers for one solve.
@@ -219,6 +222,49 @@ __device__ __forceinline__ double sap_line(double mu, double Rt, double Rn, doub
   d2 += (a * ut * ut + b * (uu - ut * ut)) * iRt + 2.0 * u.z * mu * iden * ut * iRn + u.z * u.z * iden * iRn;
   return 0.5 * (Rt * (gt0 * gt0 + gt1 * gt1) + Rn * gn * gn);
 }
+// MuJoCo soft-constraint contact (contact law 1; the inner step of SolverMuJoCoAdaptive,
+// mujoco_warp.step, restated from MuJoCo's computation model).  In velocity form a row with
+// Jacobian c (contact frame) costs 1/2 D min(c.(w - what), 0)^2, D = 1/R; a pyramidal contact of
+// condim 3 has the four rows n +- mu t_k with one common D, condim 1 (mu == 0 here) the normal row
+// alone.  y = w - what.  Returns the cost, writes the impulse gamma (minus the gradient in w) and
+// the Hessian G (00 11 22 01 02 12, contact frame t1 t2 n) in f32.
+__device__ __forceinline__ double mj_project(double mu, double D, d3 y, d3& g, float* G) {
+  g = d3{0.0, 0.0, 0.0};
+  double cost = 0.0, gnn = 0.0, g00 = 0.0, g11 = 0.0, g02 = 0.0, g12 = 0.0;
+  if (mu == 0.0) {
+    if (y.z < 0.0) { cost = 0.5 * D * y.z * y.z; g.z = -D * y.z; gnn = D; }
+  } else {
+    const double e0p = y.z + mu * y.x, e0m = y.z - mu * y.x, e1p = y.z + mu * y.y, e1m = y.z - mu * y.y;
+    const double Dm = D * mu, Dm2 = Dm * mu;
+    if (e0p < 0.0) { cost += 0.5 * D * e0p * e0p; g.z -= D * e0p; g.x -= Dm * e0p; gnn += D; g00 += Dm2; g02 += Dm; }
+    if (e0m < 0.0) { cost += 0.5 * D * e0m * e0m; g.z -= D * e0m; g.x += Dm * e0m; gnn += D; g00 += Dm2; g02 -= Dm; }
+    if (e1p < 0.0) { cost += 0.5 * D * e1p * e1p; g.z -= D * e1p; g.y -= Dm * e1p; gnn += D; g11 += Dm2; g12 += Dm; }
+    if (e1m < 0.0) { cost += 0.5 * D * e1m * e1m; g.z -= D * e1m; g.y += Dm * e1m; gnn += D; g11 += Dm2; g12 -= Dm; }
+  }
+  G[0] = (float)g00; G[1] = (float)g11; G[2] = (float)gnn; G[3] = 0.f; G[4] = (float)g02; G[5] = (float)g12;
+  return cost;
+}
+// The same rows along a line: at y with direction u (contact velocity), returns the cost and
+// adds its slope to d1 and its curvature to d2.
+__device__ __forceinline__ double mj_line(double mu, double D, d3 y, d3 u, double& d1, double& d2) {
+  double cost = 0.0;
+  if (mu == 0.0) {
+    if (y.z < 0.0) { cost = 0.5 * D * y.z * y.z; d1 += D * y.z * u.z; d2 += D * u.z * u.z; }
+    return cost;
+  }
+  const double x[4] = {y.z + mu * y.x, y.z - mu * y.x, y.z + mu * y.y, y.z - mu * y.y};
+  const double ue[4] = {u.z + mu * u.x, u.z - mu * u.x, u.z + mu * u.y, u.z - mu * u.y};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (x[e] < 0.0) {
+      cost += 0.5 * D * x[e] * x[e];
+      d1 += D * x[e] * ue[e];
+      d2 += D * ue[e] * ue[e];
+    }
+  }
+  return cost;
+}
+
 __device__ __forceinline__ f3 sym3_mul(const float* G, f3 u) {
   return mk(G[0] * u.x + G[3] * u.y + G[4] * u.z, G[3] * u.x + G[1] * u.y + G[5] * u.z,
             G[4] * u.x + G[5] * u.y + G[2] * u.z);
@@ -398,13 +444,74 @@ __device__ void contact_setup(int nc, float h, const NaConfig& cfg, const Contac
   wave_sync();
 }
 
+// Per-contact data of a step h under the MuJoCo contact law (1): MuJoCo's contact frame (t1 from
+// the y axis, or the z axis when |n_y| >= 1/2, made orthogonal to n: orthogonals,
+// solvers/mujoco/kernels.py:47-58), lever arms, and for the rows of each contact
+//   K, B     from solref, the time constant floored at 2h (refsafe), or the direct form;
+//   R        = max(MJ_MINVAL, invweight (1 - imp)/imp) with invweight the two bodies'
+//              translational inverse weights w, times (1 + mu^2) 2 mu^2 / impratio for the
+//              pyramid's rows (condim 3);
+//   what     = (1 - h B) J v0 - h K imp pos n, the contact velocity the rows drive towards
+//              (v0: the step's start velocity, in B.v; pos: dist - margins, or the surface
+//              distance where margins are zeroed).
+// Rows at pos >= 0 are inactive (D = 0).  Stores D in X.Rn, what in (X.iRt, X.iRn, X.vhat).
+// Requires B.x, B.im and B.v (with the static slot zero) for the step's start.
+__device__ void contact_setup_mj(int nc, float h, const NaConfig& cfg, int zero_margins, const ContactLDS& C,
+                                 const MjContactParams<NA_MAX_CONTACTS>& P, BodyLDS& B, SolveLDS& X) {
+  for (int c = lane_id(); c < nc; c += NA_WAVE) {
+    f3 n = mk(C.n[0][c], C.n[1][c], C.n[2][c]);
+    f3 e = (-0.5f < n.y && n.y < 0.5f) ? mk(0.f, 1.f, 0.f) : mk(0.f, 0.f, 1.f);
+    f3 t1 = nrm3(e - n * dot3(n, e));
+    X.t1[0][c] = t1.x; X.t1[1][c] = t1.y; X.t1[2][c] = t1.z;
+    f3 t2 = cross3(n, t1);
+    f3 p = mk(C.p[0][c], C.p[1][c], C.p[2][c]);
+    double w = 0.0;
+    f3 rel = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      int b = s == 0 ? C.ba[c] : C.bb[c];
+      f3 r = mk(0.f, 0.f, 0.f);
+      if (b >= 0) {
+        r = p - mk(B.x[0][b], B.x[1][b], B.x[2][b]);
+        w += (double)B.im[b];
+        f3 vb = mk(B.v[0][b], B.v[1][b], B.v[2][b]) + cross3(mk(B.v[3][b], B.v[4][b], B.v[5][b]), r);
+        rel = s == 0 ? rel - vb : rel + vb;
+      }
+      if (s == 0) { X.ra[0][c] = r.x; X.ra[1][c] = r.y; X.ra[2][c] = r.z; }
+      else { X.rb[0][c] = r.x; X.rb[1][c] = r.y; X.rb[2][c] = r.z; }
+    }
+    const double pos = zero_margins ? (double)C.dist[c] : (double)C.phi0[c];
+    const double imp = (double)C.k[c], mu = (double)C.mu[c], dmax = (double)P.dmax[c];
+    const double tc0 = (double)P.tc[c], dr = (double)P.dr[c], hd = (double)h;
+    double K, Bd;
+    if (tc0 > 0.0) {
+      const double tc = fmax(tc0, 2.0 * hd);
+      K = 1.0 / (dmax * dmax * tc * tc * dr * dr);
+      Bd = 2.0 / (dmax * tc);
+    } else {
+      K = -tc0 / (dmax * dmax);
+      Bd = -dr / dmax;
+    }
+    const double wr = mu > 0.0 ? w * (1.0 + mu * mu) * 2.0 * mu * mu / (double)cfg.impratio : w;
+    const double R = fmax((double)NA_MJ_MINVAL, wr * (1.0 - imp) / imp);
+    const double sv = 1.0 - hd * Bd;
+    X.Rn[c] = pos < 0.0 ? (float)(1.0 / R) : 0.f;
+    X.Rt[c] = 0.f;
+    X.iRt[c] = sv * (double)dot3(t1, rel);
+    X.iRn[c] = sv * (double)dot3(t2, rel);
+    X.vhat[c] = (float)(sv * (double)dot3(n, rel) - hd * K * imp * pos);
+  }
+  wave_sync();
+}
+
 // Contact passes.  Lane l handles contacts l and l + 64 (at most NA_MAX_CONTACTS = 128); the
 // NG = 1 or 2 groups are loaded together before any arithmetic, so one wait covers all
 // gathers.  Values a pass keeps in registers for later passes follow the same assignment.
 static_assert(NA_MAX_CONTACTS <= 2 * NA_WAVE, "contact passes assume at most two contacts per lane");
 
 // One contact's constants for a solve (regularisation, its FP64 reciprocals, stabilisation
-// velocity, friction), loaded once into registers.
+// velocity, friction), loaded once into registers.  Under the MuJoCo law (1) the same slots hold
+// the target contact velocity (iRt, iRn, vh = t1, t2, n components) and the rows' D (Rn).
 struct CConst {
   double iRt, iRn;
   float Rt, Rn, vh, mu;
@@ -418,7 +525,7 @@ __device__ __forceinline__ CConst load_const(const ContactLDS& C, const SolveLDS
 // and, when `gather` is set, the contact velocities into `vc` (otherwise `vc` already holds
 // them), both in registers.  Returns this lane's part of the
 // contact cost.
-template <int NG>
+template <int NG, int LAW>
 __device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow* r, const CConst* k, float (*G)[6],
                                                   d3* vc, bool gather) {
   const int l = lane_id();
@@ -432,9 +539,15 @@ __device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow
     const int c = l + g * NA_WAVE;
     const bool valid = c < nc;
     const double iRt = k[g].iRt, iRn = k[g].iRn;
-    d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)k[g].vh) * iRn};
     d3 gam;
-    double cc = sap_project(k[g].mu, k[g].Rt, k[g].Rn, iRt, iRn, y, gam, G[g]);
+    double cc;
+    if (LAW == 1) {
+      d3 y = d3{vc[g].x - iRt, vc[g].y - iRn, vc[g].z - (double)k[g].vh};
+      cc = mj_project(k[g].mu, k[g].Rn, y, gam, G[g]);
+    } else {
+      d3 y = d3{-vc[g].x * iRt, -vc[g].y * iRt, -(vc[g].z - (double)k[g].vh) * iRn};
+      cc = sap_project(k[g].mu, k[g].Rt, k[g].Rn, iRt, iRn, y, gam, G[g]);
+    }
     cost += valid ? cc : 0.0;
     row_scatter_d(B, r[g], gam);
   }
@@ -464,7 +577,7 @@ struct LsRow {
 // Line-search setup: contact-space direction dv = J dx (dx in B.vd) into this lane's rows `lr`,
 // and the lane's contact terms at alpha = 0 (cost lc0, slope d1c0, curvature d2c0) and alpha = 1
 // (cost lc1, slope d1c1).
-template <int NG>
+template <int NG, int LAW>
 __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow* r, const CConst* k, const d3* vc0,
                                            LsRow* lr, double& lc0, double& d1c0, double& d2c0, double& lc1,
                                            double& d1c1) {
@@ -483,11 +596,18 @@ __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow*
     const int c = l + g * NA_WAVE;
     const LsRow& q = lr[g];
     const d3 w0 = q.w0, u = q.dv;
-    d3 y0 = d3{-w0.x * q.iRt, -w0.y * q.iRt, -(w0.z - q.vh) * q.iRn};
-    d3 y1 = d3{-(w0.x + u.x) * q.iRt, -(w0.y + u.y) * q.iRt, -(w0.z + u.z - q.vh) * q.iRn};
-    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-    const double ca = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
-    double cb = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y1, u, b1, b2);
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0, ca, cb;
+    if (LAW == 1) {
+      d3 y0 = d3{w0.x - q.iRt, w0.y - q.iRn, w0.z - q.vh};
+      d3 y1 = d3{y0.x + u.x, y0.y + u.y, y0.z + u.z};
+      ca = mj_line(q.mu, q.Rn, y0, u, a1, a2);
+      cb = mj_line(q.mu, q.Rn, y1, u, b1, b2);
+    } else {
+      d3 y0 = d3{-w0.x * q.iRt, -w0.y * q.iRt, -(w0.z - q.vh) * q.iRn};
+      d3 y1 = d3{-(w0.x + u.x) * q.iRt, -(w0.y + u.y) * q.iRt, -(w0.z + u.z - q.vh) * q.iRn};
+      ca = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
+      cb = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y1, u, b1, b2);
+    }
     if (c < nc) {
       lc0 += ca;
       d1c0 += a1;
@@ -500,7 +620,7 @@ __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow*
 
 // Contact part of the line function at alpha for this lane's contacts (FP64), from the rows
 // that ls_setup_g left in registers: adds the cost, its first and second derivatives.
-template <int NG>
+template <int NG, int LAW>
 __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double alpha, double& l, double& dl,
                                               double& d2l) {
   const int ln = lane_id();
@@ -509,10 +629,15 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double al
     const int c = ln + g * NA_WAVE;
     const LsRow& q = lr[g];
     const d3 dv = q.dv;
-    d3 y = d3{-(q.w0.x + alpha * dv.x) * q.iRt, -(q.w0.y + alpha * dv.y) * q.iRt,
-              -(q.w0.z + alpha * dv.z - q.vh) * q.iRn};
-    double a = 0.0, b = 0.0;
-    double cc = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y, dv, a, b);
+    double a = 0.0, b = 0.0, cc;
+    if (LAW == 1) {
+      d3 y = d3{q.w0.x + alpha * dv.x - q.iRt, q.w0.y + alpha * dv.y - q.iRn, q.w0.z + alpha * dv.z - q.vh};
+      cc = mj_line(q.mu, q.Rn, y, dv, a, b);
+    } else {
+      d3 y = d3{-(q.w0.x + alpha * dv.x) * q.iRt, -(q.w0.y + alpha * dv.y) * q.iRt,
+                -(q.w0.z + alpha * dv.z - q.vh) * q.iRn};
+      cc = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y, dv, a, b);
+    }
     if (c < nc) {
       l += cc;
       dl += a;
@@ -586,7 +711,7 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const CRow* rw, const float (*G)[
 // to reach the optimality target in one step (0.1 target / |Dg_k|) nor than cg_rel_tol.
 // NG (1 or 2) is the number of contacts per lane; each lane's contact rows, G blocks and
 // contact velocities stay in registers for the whole solve.
-template <int NG>
+template <int NG, int LAW>
 __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B,
                                             const SolveLDS& X, const BodyReg& br, double* v, SolveStats& st) {
   const int l = lane_id();
@@ -633,7 +758,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double ccost = eval_contacts_g<NG>(nc, B, rows, kc, Gr, vc0, gather);
+    double ccost = eval_contacts_g<NG, LAW>(nc, B, rows, kc, Gr, vc0, gather);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -736,7 +861,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     wave_sync();
     double lc0 = 0.0, d1c0 = 0.0, d2c0 = 0.0, lc1 = 0.0, d1c1 = 0.0;
     LsRow lr[NG];
-    ls_setup_g<NG>(nc, B, rows, kc, vc0, lr, lc0, d1c0, d2c0, lc1, d1c1);
+    ls_setup_g<NG, LAW>(nc, B, rows, kc, vc0, lr, lc0, d1c0, d2c0, lc1, d1c1);
     double aa = 0.0, b0 = 0.0;
     if (own) {
       double Ax[6];
@@ -759,7 +884,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     if (line_search == 1) {
       alpha = armijo_decay(dl0, [&](double a) {
         double lc = 0.0, dlc = 0.0, d2c = 0.0;
-        ls_contacts_g<NG>(nc, lr, a, lc, dlc, d2c);
+        ls_contacts_g<NG, LAW>(nc, lr, a, lc, dlc, d2c);
         return a * b0 + 0.5 * a * a * aa + wave_sum_d(lc - lc0);
       });
     }
@@ -774,7 +899,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       if (!(alpha > lo && alpha < hi)) alpha = 0.5;
       for (int k = 0; k < 60; ++k) {
         double lc = 0.0, dlc = 0.0, d2c = 0.0;
-        ls_contacts_g<NG>(nc, lr, alpha, lc, dlc, d2c);
+        ls_contacts_g<NG, LAW>(nc, lr, alpha, lc, dlc, d2c);
         double dl = b0 + alpha * aa + wave_sum_d(dlc);
         double ddl = aa + wave_sum_d(d2c);
         if (fabs(dl) <= ls_rel * fabs(dl0)) break;
@@ -806,7 +931,11 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
 
 __device__ bool sap_solve(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B, const SolveLDS& X,
                           const BodyReg& br, double* v, SolveStats& st) {
-  return nc > NA_WAVE ? sap_solve_g<2>(nb, nc, cfg, C, B, X, br, v, st) : sap_solve_g<1>(nb, nc, cfg, C, B, X, br, v, st);
+  if (cfg.contact_law == 1)
+    return nc > NA_WAVE ? sap_solve_g<2, 1>(nb, nc, cfg, C, B, X, br, v, st)
+                        : sap_solve_g<1, 1>(nb, nc, cfg, C, B, X, br, v, st);
+  return nc > NA_WAVE ? sap_solve_g<2, 0>(nb, nc, cfg, C, B, X, br, v, st)
+                      : sap_solve_g<1, 0>(nb, nc, cfg, C, B, X, br, v, st);
 }
 
 // ---------------------------------------------------------------- body setup and integration
@@ -865,8 +994,11 @@ __device__ void body_setup(const NaModel& M, int gb, const double* q, const floa
 }
 
 // Semi-implicit update in double (solver.py:50-94, no damping): x_com += h v,
-// q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.
-__device__ void integrate_body(const NaModel& M, int gb, const double* q, const double* v, double h, double* qo) {
+// q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.  mj_quat: the rotation as MuJoCo
+// integrates a free joint (mju_quatIntegrate), by the exact exponential q = exp(h w / 2) q of the
+// new angular velocity, normalised.
+__device__ void integrate_body(const NaModel& M, int gb, const double* q, const double* v, double h, double* qo,
+                               bool mj_quat = false) {
   double qx = q[3], qy = q[4], qz = q[5], qw = q[6];
   double cx = M.body_com[3 * gb], cy = M.body_com[3 * gb + 1], cz = M.body_com[3 * gb + 2];
   double xc[3];
@@ -877,11 +1009,22 @@ __device__ void integrate_body(const NaModel& M, int gb, const double* q, const 
   xc[1] += h * v[1];
   xc[2] += h * v[2];
   double wx = v[3], wy = v[4], wz = v[5];
-  double dx = wy * qz - wz * qy + wx * qw;
-  double dy = wz * qx - wx * qz + wy * qw;
-  double dz = wx * qy - wy * qx + wz * qw;
-  double dw = -(wx * qx + wy * qy + wz * qz);
-  double nx = qx + 0.5 * h * dx, ny = qy + 0.5 * h * dy, nz = qz + 0.5 * h * dz, nw = qw + 0.5 * h * dw;
+  double nx, ny, nz, nw;
+  if (mj_quat) {
+    double wn = sqrt(wx * wx + wy * wy + wz * wz), th = 0.5 * h * wn;
+    double sc = wn > 0.0 ? sin(th) / wn : 0.0, cw = cos(th);
+    double ex = wx * sc, ey = wy * sc, ez = wz * sc;  // exp(h w / 2) = (e, cw), times q
+    nx = cw * qx + ex * qw + ey * qz - ez * qy;
+    ny = cw * qy + ey * qw + ez * qx - ex * qz;
+    nz = cw * qz + ez * qw + ex * qy - ey * qx;
+    nw = cw * qw - ex * qx - ey * qy - ez * qz;
+  } else {
+    double dx = wy * qz - wz * qy + wx * qw;
+    double dy = wz * qx - wx * qz + wy * qw;
+    double dz = wx * qy - wy * qx + wz * qw;
+    double dw = -(wx * qx + wy * qy + wz * qz);
+    nx = qx + 0.5 * h * dx; ny = qy + 0.5 * h * dy; nz = qz + 0.5 * h * dz; nw = qw + 0.5 * h * dw;
+  }
   double il = 1.0 / sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
   nx *= il; ny *= il; nz *= il; nw *= il;
   qo[0] = xc[0] - ((1 - 2 * (ny * ny + nz * nz)) * cx + 2 * (nx * ny - nz * nw) * cy + 2 * (nx * nz + ny * nw) * cz);

@@ -42,6 +42,7 @@ struct na_model {
   NaModel M;
   int max_nb, max_seg, max_nl;
   bool articulated;
+  bool has_mj;  // the MuJoCo shape arrays are present (contact law 1)
 };
 
 struct Ctrl {
@@ -370,7 +371,7 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
   stage_aabb(M, wc.w, wc.nseg, wc.nl, wc.b0, &S.body.q[0][0], NA_MAX_BODIES, S.seg);
   int np = stage_broad(M, wc.w, wc.nseg, wc.nl, cfg.broad_phase, S.seg);
   int cap = min(cfg.max_contacts, NA_MAX_CONTACTS);
-  int nc = stage_narrow(M, wc.b0, np, cap, S.seg, S.con);
+  int nc = stage_narrow(M, wc.b0, np, cap, S.seg, S.con, cfg.contact_law == 1 ? &S.mj : nullptr);
   if (np > NA_MAX_PAIRS || nc > cap) overflow = 1;
   wk[3] += min(nc, cap);
   wk[4] += np;
@@ -389,16 +390,28 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
   const bool own = l < wc.nb;
   NA_TIC(t0);
   BodyReg br;
-  if (own) body_setup(M, wc.b0 + l, q, v0, f, h, br, S.body, l);
+  const bool mj = cfg.contact_law == 1;
+  if (own) {
+    body_setup(M, wc.b0 + l, q, v0, f, h, br, S.body, l);
+    if (mj) {  // the start velocity, for the MuJoCo rows' reference velocity
+#pragma unroll
+      for (int k = 0; k < 6; ++k) S.body.v[k][l] = v0[k];
+    }
+  }
+  if (mj && l == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) S.body.v[k][NA_DUMMY] = 0.f;
+  }
   wave_sync();
-  contact_setup(nc, (float)h, cfg, S.con, S.body, S.sol);
+  if (mj) contact_setup_mj(nc, (float)h, cfg, M.mj_zero_margins, S.con, S.mj, S.body, S.sol);
+  else contact_setup(nc, (float)h, cfg, S.con, S.body, S.sol);
   NA_TOC(wk[7], t0);
   double v[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) v[k] = guess64 ? guess64[k] : (double)guess[k];
   SolveStats st;
   bool ok = sap_solve(wc.nb, nc, cfg, S.con, S.body, S.sol, br, v, st);
-  if (own) integrate_body(M, wc.b0 + l, q, v, h, qo);
+  if (own) integrate_body(M, wc.b0 + l, q, v, h, qo, mj);
 #pragma unroll
   for (int k = 0; k < 6; ++k) vo[k] = (float)v[k];
   if (v64) {
@@ -1600,6 +1613,22 @@ extern "C" int na_model_create(const na_model_desc* d, na_model** out) {
   M.joint_qd_start = d->joint_qd_start; M.joint_X_p = d->joint_X_p; M.joint_X_c = d->joint_X_c;
   M.joint_axis = d->joint_axis; M.joint_target_pos = d->joint_target_pos; M.joint_target_ke = d->joint_target_ke;
   M.joint_target_kd = d->joint_target_kd; M.joint_armature = d->joint_armature;
+  M.mj_solref = d->mj_solref; M.mj_solimp = d->mj_solimp; M.mj_solmix = d->mj_solmix;
+  M.mj_priority = d->mj_priority; M.mj_condim = d->mj_condim; M.mj_zero_margins = d->mj_zero_margins ? 1 : 0;
+  m->has_mj = d->mj_solref && d->mj_solimp && d->mj_solmix && d->mj_priority && d->mj_condim;
+  if (m->has_mj && M.S > 0) {
+    const int S = M.S;
+    std::vector<int32_t> cd(S);
+    if (hipMemcpy(cd.data(), d->mj_condim, sizeof(int32_t) * S, hipMemcpyDeviceToHost) != hipSuccess) {
+      delete m;
+      return fail("na_model_create: could not read mj_condim (device pointer?)");
+    }
+    for (int i = 0; i < S; ++i)
+      if (cd[i] != 1 && cd[i] != 3) {
+        delete m;
+        return fail("na_model_create: mj_condim must be 1 or 3 (torsional and rolling friction are not modelled)");
+      }
+  }
   if (m->articulated) {
     err = validate_articulation(M, wbs, wbe);
     if (!err.empty()) { delete m; return fail(err.c_str()); }
@@ -1643,6 +1672,8 @@ static NaConfig to_cfg(const na_config* c) {
   k.tau_d = c->contact_tau_d; k.sigma = c->sigma; k.beta = c->beta; k.ls_rel_tol = c->ls_rel_tol;
   k.cg_rel_tol = c->cg_rel_tol; k.max_cg = c->max_cg; k.cg_forcing = c->cg_forcing;
   k.line_search = c->line_search;
+  k.contact_law = c->contact_law;
+  k.impratio = c->impratio;
   return k;
 }
 
@@ -1662,6 +1693,14 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     return fail("na_solver_create: broad_phase must be 0 (sap), 1 (nxn) or 2 (explicit, needs the model's pairs)");
   if (c->line_search != 0 && c->line_search != 1)
     return fail("na_solver_create: line_search must be 0 (exact_root) or 1 (armijo_decay)");
+  if (c->contact_law != 0 && c->contact_law != 1)
+    return fail("na_solver_create: contact_law must be 0 (SAP) or 1 (MuJoCo soft constraints)");
+  if (c->contact_law == 1) {
+    if (!m->has_mj) return fail("na_solver_create: contact_law 1 needs the model's mj_* shape arrays");
+    if (m->articulated)
+      return fail("na_solver_create: contact_law 1 (MuJoCo soft constraints) is built for free-body models only");
+    if (!(c->impratio > 0)) return fail("na_solver_create: impratio must be > 0");
+  }
   na_solver* s = new na_solver();
   s->model = m;
   s->cfg = to_cfg(c);

@@ -131,6 +131,25 @@ class Control:
             self.joint_target_q.zero_()
 
 
+class MujocoShapeAttributes:
+    """MuJoCo geom parameters per shape (the ``model.mujoco`` custom attributes that
+    ``SolverMuJoCo.register_custom_attributes`` adds, ``solver_mujoco.py:712-790``), with the
+    reference's defaults: ``condim`` 3, ``geom_priority`` 0, ``geom_solimp`` (0.9, 0.95, 0.001, 0.5,
+    2), ``geom_solmix`` 1 and ``solref`` (0, 0), which means "not authored": the solver then derives
+    the shape's solref from ``shape_material_ke`` / ``shape_material_kd`` (``convert_solref(ke, kd,
+    1, 1)``, ``solver_mujoco.py:5224-5233``).  Edit them before the first solver is created on the
+    model.  ``condim`` must be 1 (frictionless) or 3 (sliding friction)."""
+
+    def __init__(self, shape_count: int, device):
+        S = int(shape_count)
+        self.condim = torch.full((S,), 3, dtype=torch.int32, device=device)
+        self.geom_priority = torch.zeros(S, dtype=torch.int32, device=device)
+        self.geom_solimp = torch.tensor([[0.9, 0.95, 0.001, 0.5, 2.0]] * S, dtype=torch.float32,
+                                        device=device).reshape(S, 5)
+        self.geom_solmix = torch.ones(S, dtype=torch.float32, device=device)
+        self.solref = torch.zeros((S, 2), dtype=torch.float32, device=device)
+
+
 class Model:
     """Static scene description (``newton/_src/sim/model.py:36``).
 
@@ -179,6 +198,7 @@ class Model:
         self.world_body_start = _t(host["world_body_start"], torch.int32, d)
         self.world_shape_start = _t(host["world_shape_start"], torch.int32, d)
         self.global_shapes = _t(host["global_shapes"], torch.int32, d)
+        self.mujoco = MujocoShapeAttributes(self.shape_count, d)
 
         # articulated worlds: one joint per body (REVOLUTE / FREE root), generalized coordinates
         self.is_articulated = "joint_type" in host

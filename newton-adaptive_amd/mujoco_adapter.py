@@ -92,6 +92,14 @@ def geom_solref(model) -> np.ndarray:
     return convert_solref(model.host("shape_material_ke"), model.host("shape_material_kd"), 1.0, 1.0)
 
 
+def resolved_solref(model) -> np.ndarray:
+    """The solref SolverMuJoCo gives each geom (``solver_mujoco.py:5224-5233``): the authored
+    ``model.mujoco.solref`` where it is not (0, 0), else ``convert_solref(ke, kd, 1, 1)``."""
+    sr = model.mujoco.solref.detach().cpu().numpy().astype(np.float64).reshape(-1, 2)
+    authored = (sr[:, 0] != 0.0) | (sr[:, 1] != 0.0)
+    return np.where(authored[:, None], sr, geom_solref(model))
+
+
 def contact_params(pairs, *, priority, solmix, solref, solimp, friction, margin, gap, condim):
     """Per-pair contact parameters from per-shape MuJoCo geom parameters.
 

@@ -68,6 +68,17 @@ class ModelHandle:
                 d.explicit_pairs = p(torch.as_tensor(pairs, device=model.device))
                 d.world_pair_start = p(torch.as_tensor(starts, device=model.device))
                 self.with_pairs = True
+        # MuJoCo geom parameters (contact law 1); models without them (a serving pool) leave NULL
+        mjat = getattr(model, "mujoco", None)
+        if mjat is not None and model.shape_count > 0:
+            from .mujoco_adapter import resolved_solref
+            d.mj_solref = p(torch.as_tensor(resolved_solref(model), dtype=torch.float32, device=model.device))
+            d.mj_solimp = p(mjat.geom_solimp.to(torch.float32))
+            d.mj_solmix = p(mjat.geom_solmix.to(torch.float32))
+            d.mj_priority = p(mjat.geom_priority.to(torch.int32))
+            d.mj_condim = p(mjat.condim.to(torch.int32))
+            # SolverMuJoCo zeroes every margin of a model with boxes (solver_mujoco.py:3352-3357)
+            d.mj_zero_margins = int(bool((model.shape_type == 7).any().item()))
         # articulation (one joint per body); free-body models leave joint_type NULL
         if getattr(model, "is_articulated", False):
             for name in ("joint_type", "joint_parent", "joint_q_start", "joint_qd_start", "joint_X_p", "joint_X_c",

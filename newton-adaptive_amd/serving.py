@@ -460,6 +460,9 @@ class ServingEngine:
         self.dt_outer = float(dt_outer)
         self.budget_ms = float(budget_ms)
         cls = {"sap": SolverSAPAdaptive, "mujoco": SolverMuJoCoAdaptive}[controller]
+        if controller == "mujoco":
+            # requests carry no MuJoCo geom parameters: the pool solves contacts with SAP
+            solver_kw.setdefault("contact_model", "sap")
         solver_kw.setdefault("max_substeps", 256)
         solver_kw.setdefault("world_order", "index")
         self.solver = cls(self.pool, **solver_kw)

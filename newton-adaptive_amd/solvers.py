@@ -40,7 +40,8 @@ class _SolverBase:
                  dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=100,
                  contact_preset_variant="drake", line_search_variant="exact_root", contact_tau_d=0.01,
                  broad_phase="sap", optimality_rel_tol=None, schedule="served", cg_rel_tol=1.0e-4,
-                 cg_forcing=0.1, world_order="cost", ls_rel_tol=1.0e-4, **_kw):
+                 cg_forcing=0.1, world_order="cost", ls_rel_tol=1.0e-4, _contact_law=0, _impratio=1.0,
+                 **_kw):
         if mode not in _MODE_CODES:
             raise ValueError(f"mode must be one of {tuple(_MODE_CODES)}, got {mode!r}.")
         if float(tol) <= 0.0:
@@ -110,6 +111,8 @@ class _SolverBase:
         cfg.max_cg = 200
         cfg.cg_forcing = float(cg_forcing)
         cfg.line_search = _LINE_SEARCHES[line_search_variant]
+        cfg.contact_law = int(_contact_law)
+        cfg.impratio = float(_impratio)
         self.line_search_variant = line_search_variant
         self._cfg = cfg
 
@@ -444,22 +447,101 @@ class SolverSAPAdaptive(_SolverBase):
     _controller = 0
 
 
+_MJ_SOLVERS = {"newton": 2, "cg": 1}       # mjtSolver: both minimise the same convex problem
+_MJ_INTEGRATORS = {"euler": 0, "implicitfast": 3}  # mjtIntegrator
+_MJ_CONES = {"pyramidal": 0}              # mjtCone
+
+
+def _mj_option(value, table, name, default):
+    """A MuJoCo enum option given by name or by its MuJoCo integer constant."""
+    if value is None:
+        return default
+    if isinstance(value, str):
+        key = value.lower()
+        if key in table:
+            return key
+    else:
+        inv = {v: k for k, v in table.items()}
+        if int(value) in inv:
+            return inv[int(value)]
+    raise ValueError(f"{name} must be one of {tuple(table)} (or their MuJoCo constants), got {value!r}.")
+
+
 class SolverMuJoCoAdaptive(_SolverBase):
     """Adaptive solver with the MuJoCo-variant controller (``solver_mujoco_adaptive.py:418``):
     ``accepted`` advances time, ``commit`` writes the doubled state, a floor-diverged world holds
-    its last good state and is latched in ``diverged``."""
+    its last good state and is latched in ``diverged``.
+
+    Each inner step is MuJoCo's soft-constraint step (``contact_model="mujoco"``, the default on
+    free-body models): contacts are rows with MuJoCo's reference acceleration (solref), impedance
+    (solimp) and regulariser, mixed per pair from ``model.mujoco`` (``kernels.py:127-190``), on the
+    pyramidal cone, minimised by Newton's method, then Euler (``solver_mujoco_adaptive.py:585-605``:
+    ``mujoco_warp.step`` with the reference's defaults solver "newton", integrator "implicitfast",
+    cone "pyramidal").  For free bodies without joint damping or actuators "euler" and
+    "implicitfast" are the same update.  ``contact_model="sap"`` keeps the SAP contact solve under
+    this controller; articulated models need it (their MuJoCo inner step is not built).
+
+    The MuJoCo options of ``SolverMuJoCo`` that bear on this step are checked, not dropped:
+    ``solver`` ("newton" / "cg": the same minimiser, solved here by Newton), ``integrator``
+    ("euler" / "implicitfast"), ``cone`` ("pyramidal"), ``impratio`` (> 0), ``iterations`` (the
+    Newton cap), ``ls_iterations``, ``tolerance``, ``ls_tolerance`` (> 0; the stopping rule here is
+    this solver's optimality test), ``nconmax`` (at most 128 contacts per world), ``njmax``,
+    ``disable_contacts`` (False only), ``use_mujoco_cpu`` (False only).  Other keywords raise
+    ``TypeError``."""
 
     _controller = 1
 
     def __init__(self, model, *, tol=1e-3, dt_inner_init=0.01, dt_inner_min=1e-6, dt_inner_max=None,
-                 dt_mode="per_world", tiling="ragged", max_substeps=256, **kwargs):
+                 dt_mode="per_world", tiling="ragged", max_substeps=256, contact_model=None, solver=None,
+                 integrator=None, cone=None, impratio=None, iterations=None, ls_iterations=None, tolerance=None,
+                 ls_tolerance=None, njmax=None, nconmax=None, disable_contacts=False, use_mujoco_cpu=False,
+                 separate_worlds=None, **kwargs):
         if dt_mode != "per_world":
             raise ValueError(f"dt_mode must be 'per_world' (the only supported mode), got {dt_mode!r}")
         if tiling != "ragged":
             raise ValueError(f"tiling must be 'ragged' (the only supported mode), got {tiling!r}")
-        kwargs.pop("mode", None)
+        articulated = bool(getattr(model, "is_articulated", False))
+        if contact_model is None:
+            contact_model = "sap" if articulated else "mujoco"
+        if contact_model not in ("mujoco", "sap"):
+            raise ValueError(f"contact_model must be 'mujoco' or 'sap', got {contact_model!r}")
+        if contact_model == "mujoco" and articulated:
+            raise ValueError("contact_model='mujoco' is built for free-body models; pass contact_model='sap' "
+                             "for articulated models")
+        self.mj_solver = _mj_option(solver, _MJ_SOLVERS, "solver", "newton")
+        self.mj_integrator = _mj_option(integrator, _MJ_INTEGRATORS, "integrator", "implicitfast")
+        self.mj_cone = _mj_option(cone, _MJ_CONES, "cone", "pyramidal")
+        impratio = 1.0 if impratio is None else float(impratio)
+        if not impratio > 0.0:
+            raise ValueError(f"impratio must be > 0, got {impratio!r}")
+        for name, val in (("tolerance", tolerance), ("ls_tolerance", ls_tolerance)):
+            if val is not None and not float(val) > 0.0:
+                raise ValueError(f"{name} must be > 0, got {val!r}")
+        for name, val in (("iterations", iterations), ("ls_iterations", ls_iterations), ("njmax", njmax),
+                          ("nconmax", nconmax)):
+            if val is not None and int(val) < 1:
+                raise ValueError(f"{name} must be >= 1, got {val!r}")
+        if nconmax is not None and int(nconmax) > 128:
+            raise ValueError(f"nconmax must be at most 128 contacts per world, got {nconmax!r}")
+        if disable_contacts:
+            raise ValueError("disable_contacts=True is not supported by this solver")
+        if use_mujoco_cpu:
+            raise ValueError("use_mujoco_cpu=True is not supported: the step runs on the device")
+        allowed = {"max_rigid_contact", "max_iterations", "broad_phase", "optimality_rel_tol", "schedule",
+                   "cg_rel_tol", "cg_forcing", "world_order", "ls_rel_tol", "line_search_variant",
+                   "contact_tau_d", "contact_preset_variant"}
+        unknown = set(kwargs) - allowed
+        if unknown:
+            raise TypeError(f"SolverMuJoCoAdaptive got unexpected keyword arguments {sorted(unknown)}")
+        if iterations is not None:
+            kwargs.setdefault("max_iterations", int(iterations))
+        if nconmax is not None:
+            kwargs.setdefault("max_rigid_contact", int(nconmax))
+        self.contact_model = contact_model
+        self.impratio = impratio
         super().__init__(model, mode="adaptive", tol=tol, dt_inner_init=dt_inner_init, dt_inner_min=dt_inner_min,
-                         dt_inner_max=dt_inner_max, max_substeps=max_substeps, **kwargs)
+                         dt_inner_max=dt_inner_max, max_substeps=max_substeps,
+                         _contact_law=1 if contact_model == "mujoco" else 0, _impratio=impratio, **kwargs)
         self._tiling = tiling
 
     @property

@@ -127,6 +127,13 @@ typedef struct or_model {
   int filter_count;
   const int32_t* filter_pairs; /* sorted canonical (min, max) */
   double gravity[3];
+  /* MuJoCo geom parameters per shape (solver_mujoco.py:700-790 custom attributes, resolved):
+     solref [S,2] (time constant, damping ratio; or direct stiffness/damping when <= 0), solimp
+     [S,5], solmix [S], priority [S], condim [S]; zero_margins: the model's margins are zeroed as
+     SolverMuJoCo does for models with boxes (solver_mujoco.py:3352-3357).  NULL when unused. */
+  const double *mj_solref, *mj_solimp, *mj_solmix;
+  const int32_t *mj_priority, *mj_condim;
+  int mj_zero_margins;
 } or_model;
 
 typedef struct or_config {
@@ -136,6 +143,8 @@ typedef struct or_config {
   int max_substeps, max_iterations, max_contacts;
   double opt_rel_tol, opt_abs_tol, tau_d, sigma, beta, ls_rel_tol;
   int line_search;     /* 0 exact_root, 1 armijo_decay (solver_sap_adaptive.py:459) */
+  int contact_law;     /* 0 SAP compliant cone, 1 MuJoCo soft constraints, pyramidal cone */
+  double impratio;     /* MuJoCo frictional-to-normal impedance ratio (law 1) */
 } or_config;
 
 enum { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
@@ -908,6 +917,10 @@ typedef struct {
      [nc, 3, N]; NULL for free bodies (block-diagonal A, per-body 3x6 blocks) */
   real* Adense;
   real* Jd;
+  /* contact law 1 (MuJoCo): per contact the regulariser D = 1/R of every row (0: inactive) and
+     the contact-frame velocity the rows are driven towards, what [3 nc] (t1, t2, n) */
+  int law;
+  real* D; real* what;
 } sap_ctx;
 
 /* P_F projection in the R norm and its Jacobian-derived Hessian G = dP/dy R^-1 */
@@ -951,6 +964,107 @@ static acc project(acc mu, acc Rt, acc Rn, const acc* y, acc* gam, real* G) {
     }
   }
   return 0.5 * (Rt * (gam[0] * gam[0] + gam[1] * gam[1]) + Rn * gam[2] * gam[2]);
+}
+
+/* ---------------------------------------------------------------- MuJoCo soft constraints
+ * The soft-constraint model of MuJoCo's convex solver (the inner step of SolverMuJoCoAdaptive,
+ * mujoco_warp.step, solver_mujoco_adaptive.py:585-605; absent dependency mujoco-warp 3.8.1),
+ * restated from MuJoCo's published computation model: each constraint row i has a reference
+ * acceleration aref_i = -B (J_i v) - K imp (pos - margin), a regulariser R_i = (1 - imp)/imp
+ * invweight_i and, for contacts, the one-sided cost 1/2 R_i^-1 min(J_i a - aref_i, 0)^2; the
+ * primal problem min_a 1/2 |a - a0|_M^2 + sum_i cost_i is solved and v' = v + h a (Euler).
+ * In velocity form (a = (v' - v)/h, scaled by h^2) a row costs 1/2 D min(J_i v' - vhat_i, 0)^2
+ * with D = 1/R and vhat_i = J_i v + h aref_i.  A pyramidal contact of condim 3 has four rows
+ * J_n +- mu J_tk, all with the invweight (1 + mu^2) w 2 mu^2 / impratio (w: the two bodies'
+ * translational inverse weights); condim 1 has the normal row alone with invweight w. */
+#define OR_MJ_MINVAL 1.0e-15
+#define OR_MJ_MINIMP 0.0001
+#define OR_MJ_MAXIMP 0.9999
+#define OR_MJ_MINMU 1.0e-5
+
+/* impedance d(pos) of solimp = (dmin, dmax, width, midpoint, power): the two-piece power curve
+   from dmin at |pos| = 0 to dmax at |pos| >= width (MuJoCo's solimp definition) */
+double or_mj_impedance(const double* solimp, double pos) {
+  double dmin = clampr(solimp[0], OR_MJ_MINIMP, OR_MJ_MAXIMP), dmax = clampr(solimp[1], OR_MJ_MINIMP, OR_MJ_MAXIMP);
+  double width = maxr(OR_MJ_MINVAL, solimp[2]), mid = clampr(solimp[3], OR_MJ_MINIMP, OR_MJ_MAXIMP);
+  double power = maxr(1.0, solimp[4]);
+  double x = fabs(pos) / width;
+  if (x >= 1.0) return dmax;
+  double y;
+  if (power == 1.0) y = x;
+  else if (x < mid) y = pow(x, power) / pow(mid, power - 1.0);
+  else y = 1.0 - pow(1.0 - x, power) / pow(1.0 - mid, power - 1.0);
+  return clampr(dmin + y * (dmax - dmin), dmin, dmax);
+}
+
+/* Contact parameters of a shape pair (contact_params, solvers/mujoco/kernels.py:127-190): the
+   higher priority shape decides; equal priorities mix solref/solimp by solmix (0.5 when both are
+   ~0, one-sided when one is), take the larger condim and the larger friction; a non-positive
+   solref time constant on either side selects the elementwise min.  Friction floored at MJ_MINMU. */
+void or_mj_contact_params(const or_model* M, int a, int b, double* solref, double* solimp, double* mu, int* condim) {
+  int pa = M->mj_priority[a], pb = M->mj_priority[b];
+  double mix;
+  if (pa > pb) { mix = 1.0; *condim = M->mj_condim[a]; *mu = M->shape_mu[a]; }
+  else if (pb > pa) { mix = 0.0; *condim = M->mj_condim[b]; *mu = M->shape_mu[b]; }
+  else {
+    double s1 = M->mj_solmix[a], s2 = M->mj_solmix[b];
+    const double tiny = 2.220446049250313e-16;
+    mix = s1 + s2 != 0.0 ? s1 / (s1 + s2) : 0.0;
+    if (s1 < tiny && s2 < tiny) mix = 0.5;
+    else if (s1 < tiny) mix = 0.0;
+    else if (s2 < tiny) mix = 1.0;
+    *condim = M->mj_condim[a] > M->mj_condim[b] ? M->mj_condim[a] : M->mj_condim[b];
+    *mu = maxr(M->shape_mu[a], M->shape_mu[b]);
+  }
+  *mu = maxr(OR_MJ_MINMU, *mu);
+  const double *ra = M->mj_solref + 2 * a, *rb = M->mj_solref + 2 * b;
+  for (int k = 0; k < 2; ++k)
+    solref[k] = (ra[0] > 0 && rb[0] > 0) ? mix * ra[k] + (1 - mix) * rb[k] : minr(ra[k], rb[k]);
+  for (int k = 0; k < 5; ++k) solimp[k] = mix * M->mj_solimp[5 * a + k] + (1 - mix) * M->mj_solimp[5 * b + k];
+}
+
+/* Cost of one contact's rows at y = w - what (contact frame t1, t2, n), its impulse gamma
+   (minus the gradient in w) and Hessian G (3x3).  mu == 0: the normal row alone (condim 1). */
+static acc mj_pyramid(acc mu, acc D, const acc* y, acc* gam, real* G) {
+  acc cost = 0;
+  gam[0] = gam[1] = gam[2] = 0;
+  for (int i = 0; i < 9; ++i) G[i] = 0;
+  if (D <= 0) return 0;
+  if (mu == 0) {
+    if (y[2] < 0) { cost = 0.5 * D * y[2] * y[2]; gam[2] = -D * y[2]; G[8] = (real)D; }
+    return cost;
+  }
+  acc Gd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < 2; ++k) {
+    for (int s = -1; s <= 1; s += 2) {
+#if defined(OR_PLANT) && OR_PLANT == 4
+      acc x = y[2] + s * y[k];
+#else
+      acc x = y[2] + s * mu * y[k];
+#endif
+      if (!(x < 0)) continue;
+      cost += 0.5 * D * x * x;
+      gam[2] -= D * x;
+      gam[k] -= D * x * s * mu;
+      Gd[8] += D;
+      Gd[4 * k] += D * mu * mu;
+      Gd[3 * k + 2] += D * s * mu;
+      Gd[6 + k] += D * s * mu;
+    }
+  }
+  for (int i = 0; i < 9; ++i) G[i] = (real)Gd[i];
+  return cost;
+}
+
+/* cost, impulse and Hessian of contact c at contact velocity vc under the context's law */
+static acc law_eval(const sap_ctx* S, int c, const acc* vc, acc* gam, real* G) {
+  acc y[3];
+  if (S->law == 1) {
+    for (int k = 0; k < 3; ++k) y[k] = vc[k] - S->what[3 * c + k];
+    return mj_pyramid(S->mu[c], S->D[c], y, gam, G);
+  }
+  y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
+  return project(S->mu[c], S->Rt[c], S->Rn[c], y, gam, G);
 }
 
 static void contact_velocity(const sap_ctx* S, int c, const acc* v, acc* vc) {
@@ -1005,11 +1119,10 @@ static acc eval_cost(sap_ctx* S, const acc* v, acc* g, acc* gam, real* G, acc* p
   acc* jc = S->tmp + N;
   for (int i = 0; i < N; ++i) jc[i] = 0;
   for (int c = 0; c < S->nc; ++c) {
-    acc vc[3], y[3], gl[3];
+    acc vc[3], gl[3];
     real Gl[9];
     contact_velocity(S, c, v, vc);
-    y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
-    cost += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    cost += law_eval(S, c, vc, gl, Gl);
     if (gam) memcpy(gam + 3 * c, gl, sizeof(gl));
     if (G) memcpy(G + 9 * c, Gl, sizeof(Gl));
     if (S->Jd) {
@@ -1051,12 +1164,11 @@ static void ls_eval(sap_ctx* S, const acc* v, const acc* dv, acc alpha, acc* l, 
   acc* va = S->tmp + 3 * N;
   for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
   for (int c = 0; c < S->nc; ++c) {
-    acc vc[3], dvc[3], y[3], gl[3];
+    acc vc[3], dvc[3], gl[3];
     real Gl[9];
     contact_velocity(S, c, va, vc);
     contact_velocity(S, c, dv, dvc);
-    y[0] = -vc[0] / S->Rt[c]; y[1] = -vc[1] / S->Rt[c]; y[2] = -(vc[2] - S->vhat[c]) / S->Rn[c];
-    mom += project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    mom += law_eval(S, c, vc, gl, Gl);
     dm -= gl[0] * dvc[0] + gl[1] * dvc[1] + gl[2] * dvc[2];
     for (int r = 0; r < 3; ++r) ddm += dvc[r] * (Gl[3 * r] * dvc[0] + Gl[3 * r + 1] * dvc[1] + Gl[3 * r + 2] * dvc[2]);
   }
@@ -1102,14 +1214,12 @@ static acc ls_delta(sap_ctx* S, const acc* v, const acc* dv, acc alpha) {
   acc delta = alpha * b0 + 0.5 * alpha * alpha * aa;
   for (int i = 0; i < N; ++i) va[i] = v[i] + alpha * dv[i];
   for (int c = 0; c < S->nc; ++c) {
-    acc vc0[3], vca[3], y[3], gl[3];
+    acc vc0[3], vca[3], gl[3];
     real Gl[9];
     contact_velocity(S, c, v, vc0);
     contact_velocity(S, c, va, vca);
-    y[0] = -vca[0] / S->Rt[c]; y[1] = -vca[1] / S->Rt[c]; y[2] = -(vca[2] - S->vhat[c]) / S->Rn[c];
-    acc ca = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
-    y[0] = -vc0[0] / S->Rt[c]; y[1] = -vc0[1] / S->Rt[c]; y[2] = -(vc0[2] - S->vhat[c]) / S->Rn[c];
-    acc c0 = project(S->mu[c], S->Rt[c], S->Rn[c], y, gl, Gl);
+    acc ca = law_eval(S, c, vca, gl, Gl);
+    acc c0 = law_eval(S, c, vc0, gl, Gl);
     delta += ca - c0;
   }
   return delta;
@@ -1280,12 +1390,21 @@ static void sap_setup(sap_ctx* S, const world_view* W, const or_config* C, const
     S->Ad[6 * b + 3] = S->Iw[9 * b]; S->Ad[6 * b + 4] = S->Iw[9 * b + 4]; S->Ad[6 * b + 5] = S->Iw[9 * b + 8];
   }
   real tau_d = (real)C->tau_d;
+  S->law = C->contact_law;
   for (int c = 0; c < nc; ++c) {
     const or_contact* k = cs + c;
     v3 n = V(k->n[0], k->n[1], k->n[2]);
-    /* tangent basis: t1 orthogonal to n using the axis least aligned with n */
-    v3 e = RA(n.x) < RA(n.y) ? (RA(n.x) < RA(n.z) ? V(1, 0, 0) : V(0, 0, 1)) : (RA(n.y) < RA(n.z) ? V(0, 1, 0) : V(0, 0, 1));
-    v3 t1 = nrm(cross(n, e));
+    v3 t1;
+    if (C->contact_law == 1) {
+      /* MuJoCo's contact frame (orthogonals, solvers/mujoco/kernels.py:47-58): t1 from the y axis,
+         or the z axis when |n_y| >= 1/2, made orthogonal to n */
+      v3 b = (-0.5f < n.y && n.y < 0.5f) ? V(0, 1, 0) : V(0, 0, 1);
+      t1 = nrm(sub(b, scl(n, dot(n, b))));
+    } else {
+      /* tangent basis: t1 orthogonal to n using the axis least aligned with n */
+      v3 e = RA(n.x) < RA(n.y) ? (RA(n.x) < RA(n.z) ? V(1, 0, 0) : V(0, 0, 1)) : (RA(n.y) < RA(n.z) ? V(0, 1, 0) : V(0, 0, 1));
+      t1 = nrm(cross(n, e));
+    }
     v3 t2 = cross(n, t1);
     v3 F[3] = {t1, t2, n};
     v3 p = V(k->c[0], k->c[1], k->c[2]);
@@ -1311,6 +1430,38 @@ static void sap_setup(sap_ctx* S, const world_view* W, const or_config* C, const
       wsum += 3 / S->m[b] + dot(r, r) * tr - dot(r, Ir);
     }
     real wdiag = wsum / 3;
+    if (C->contact_law == 1) {
+      /* MuJoCo rows: pos = dist - margin (the margins as SolverMuJoCo keeps them), inactive at
+         pos >= 0; K, B from solref with the time constant floored at 2h (refsafe) */
+      double solref[2], solimp[5], mu;
+      int condim;
+      or_mj_contact_params(M, k->shape_a, k->shape_b, solref, solimp, &mu, &condim);
+      double pos = M->mj_zero_margins ? (double)k->dist : (double)k->phi0;
+      double imp = or_mj_impedance(solimp, pos);
+      double dmax = clampr(solimp[1], OR_MJ_MINIMP, OR_MJ_MAXIMP);
+      double K, B;
+      if (solref[0] > 0) {
+        double tc = maxr(solref[0], 2.0 * (double)h), dr = solref[1];
+        K = 1.0 / (dmax * dmax * tc * tc * dr * dr);
+        B = 2.0 / (dmax * tc);
+      } else {
+        K = -solref[0] / (dmax * dmax);
+        B = -solref[1] / dmax;
+      }
+      double w = 0;
+      for (int s2 = 0; s2 < 2; ++s2) if (S->cb[2 * c + s2] >= 0) w += 1.0 / S->m[S->cb[2 * c + s2]];
+      if (condim != 1 && condim != 3) condim = 3;
+      double wr = condim == 3 ? w * (1 + mu * mu) * 2 * mu * mu / C->impratio : w;
+      double R = maxr(OR_MJ_MINVAL, wr * (1 - imp) / imp);
+      acc vt[3];
+      contact_velocity(S, c, qd, vt);  /* J v at the step's start */
+      S->D[c] = pos < 0 ? (real)(1.0 / R) : 0;
+      for (int r = 0; r < 3; ++r) S->what[3 * c + r] = (real)((1.0 - (double)h * B) * vt[r]);
+      S->what[3 * c + 2] -= (real)((double)h * K * imp * pos);
+      S->mu[c] = condim == 1 ? 0 : (real)mu;
+      S->Rn[c] = S->Rt[c] = 1; S->vhat[c] = 0;
+      continue;
+    }
     real Rn_soft = 1 / (h * k->k * (h + tau_d));
     real Rn_rigid = (real)(C->beta * C->beta / (4.0 * OR_PI * OR_PI)) * wdiag;
     S->Rn[c] = maxr(Rn_rigid, Rn_soft);
@@ -1334,16 +1485,21 @@ static void sap_alloc(sap_ctx* S, int nb, int nc) {
   S->gam = (acc*)calloc(3 * ncc, sizeof(acc)); S->G = (real*)calloc(9 * ncc, sizeof(real));
   S->tmp = (acc*)calloc(4 * N, sizeof(acc)); S->rtmp = (real*)calloc(N > 0 ? N : 1, sizeof(real));
   S->Adense = NULL; S->Jd = NULL;
+  S->law = 0;
+  S->D = (real*)calloc(ncc, sizeof(real)); S->what = (real*)calloc(3 * ncc, sizeof(real));
 }
 static void sap_free(sap_ctx* S) {
   free(S->m); free(S->Iw); free(S->Iwi); free(S->xc); free(S->vstar); free(S->Ad); free(S->cb); free(S->J);
   free(S->Rt); free(S->Rn); free(S->vhat); free(S->mu); free(S->H); free(S->g); free(S->dv); free(S->v);
-  free(S->gam); free(S->G); free(S->tmp); free(S->rtmp);
+  free(S->gam); free(S->G); free(S->tmp); free(S->rtmp); free(S->D); free(S->what);
 }
 
 /* Semi-implicit update (solver.py:50-94 without damping): x_com += h v,
- * q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.  Always in double. */
-static void integrate_world(const world_view* W, const double* q, const acc* v, double h, double* q_out, double* qd_out) {
+ * q = normalize(q + 1/2 h (w, 0) q), origin = x_com - R com.  Always in double.
+ * mj_quat: the rotation as MuJoCo integrates a free joint (mju_quatIntegrate), by the exact
+ * exponential q = exp(h w / 2) q of the new angular velocity, normalised. */
+static void integrate_world_q(const world_view* W, const double* q, const acc* v, double h, double* q_out,
+                              double* qd_out, int mj_quat) {
   const or_model* M = W->M;
   for (int b = 0; b < W->nb; ++b) {
     int gb = W->b0 + b;
@@ -1359,12 +1515,23 @@ static void integrate_world(const world_view* W, const double* q, const acc* v, 
     const acc* vb = v + 6 * b;
     for (int i = 0; i < 3; ++i) xc[i] += h * (double)vb[i];
     double wx = vb[3], wy = vb[4], wz = vb[5];
-    /* dq = (w, 0) * q */
-    double dx = wy * qz - wz * qy + wx * qw;
-    double dy = wz * qx - wx * qz + wy * qw;
-    double dz = wx * qy - wy * qx + wz * qw;
-    double dw = -(wx * qx + wy * qy + wz * qz);
-    double nx = qx + 0.5 * h * dx, ny = qy + 0.5 * h * dy, nz = qz + 0.5 * h * dz, nw = qw + 0.5 * h * dw;
+    double nx, ny, nz, nw;
+    if (mj_quat) {
+      double wn = sqrt(wx * wx + wy * wy + wz * wz), th = 0.5 * h * wn;
+      double sc = wn > 0 ? sin(th) / wn : 0.0, cw = cos(th);
+      double ex = wx * sc, ey = wy * sc, ez = wz * sc;  /* exp(h w / 2) = (e, cw), times q */
+      nx = cw * qx + ex * qw + ey * qz - ez * qy;
+      ny = cw * qy + ey * qw + ez * qx - ex * qz;
+      nz = cw * qz + ez * qw + ex * qy - ey * qx;
+      nw = cw * qw - ex * qx - ey * qy - ez * qz;
+    } else {
+      /* dq = (w, 0) * q */
+      double dx = wy * qz - wz * qy + wx * qw;
+      double dy = wz * qx - wx * qz + wy * qw;
+      double dz = wx * qy - wy * qx + wz * qw;
+      double dw = -(wx * qx + wy * qy + wz * qz);
+      nx = qx + 0.5 * h * dx; ny = qy + 0.5 * h * dy; nz = qz + 0.5 * h * dz; nw = qw + 0.5 * h * dw;
+    }
     double l = sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
     nx /= l; ny /= l; nz /= l; nw /= l;
     double R1[9] = {1 - 2 * (ny * ny + nz * nz), 2 * (nx * ny - nz * nw), 2 * (nx * nz + ny * nw),
@@ -1389,7 +1556,7 @@ static int inner_step(const world_view* W, const or_config* C, const double* q, 
   acc* v = S.v;
   for (int i = 0; i < S.N; ++i) v[i] = guess ? guess[i] : (acc)qd[i];
   int ok = sap_newton(&S, C, v, st);
-  integrate_world(W, q, v, h, q_out, qd_out);
+  integrate_world_q(W, q, v, h, q_out, qd_out, C->contact_law == 1);
   if (v_out) memcpy(v_out, v, sizeof(acc) * S.N);
   sap_free(&S);
   return ok;
@@ -1426,7 +1593,7 @@ int or_substep_world(const or_model* M, const or_config* C, int w, const double*
 int or_substep_problem(const or_model* M, const or_config* C, int w, const double* q, const double* qd,
                        const double* f, double h, int n_cap, int c_cap, double* A_out, double* vstar_out,
                        double* J_out, double* Rt_out, double* Rn_out, double* vhat_out, double* mu_out,
-                       double* v_out, int* converged) {
+                       double* v_out, int* converged, double* D_out, double* what_out) {
   world_view W = {M, w, M->world_body_start[w], M->world_body_start[w + 1] - M->world_body_start[w]};
   int cap = C->max_contacts;
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
@@ -1460,6 +1627,9 @@ int or_substep_problem(const or_model* M, const or_config* C, int w, const doubl
         for (int k = 0; k < 6; ++k) J_out[(3 * c + r) * N + 6 * b + k] += Jb[6 * r + k];
     }
     Rt_out[c] = S.Rt[c]; Rn_out[c] = S.Rn[c]; vhat_out[c] = S.vhat[c]; mu_out[c] = S.mu[c];
+    /* contact law 1: the rows' D and target contact velocity */
+    D_out[c] = S.D[c];
+    for (int r = 0; r < 3; ++r) what_out[3 * c + r] = S.what[3 * c + r];
   }
   for (int i = 0; i < N; ++i) { vstar_out[i] = S.vstar[i]; v_out[i] = v[i]; }
   sap_free(&S);

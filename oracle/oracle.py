@@ -40,6 +40,8 @@ class OrModel(C.Structure):
         ("shape_ke", f64p), ("shape_mu", f64p), ("shape_radius", f64p),
         ("filter_count", C.c_int), ("filter_pairs", i32p),
         ("gravity", C.c_double * 3),
+        ("mj_solref", f64p), ("mj_solimp", f64p), ("mj_solmix", f64p), ("mj_priority", i32p), ("mj_condim", i32p),
+        ("mj_zero_margins", C.c_int),
     ]
 
 
@@ -51,6 +53,7 @@ class OrConfig(C.Structure):
         ("max_substeps", C.c_int), ("max_iterations", C.c_int), ("max_contacts", C.c_int),
         ("opt_rel_tol", C.c_double), ("opt_abs_tol", C.c_double), ("tau_d", C.c_double), ("sigma", C.c_double),
         ("beta", C.c_double), ("ls_rel_tol", C.c_double), ("line_search", C.c_int),
+        ("contact_law", C.c_int), ("impratio", C.c_double),
     ]
 
 
@@ -133,6 +136,10 @@ def _load(path):
                  "or_step_frame", "or_art_step_frame", "or_cubic_min", "or_num_threads"):
         getattr(L, name).restype = C.c_int
     L.or_cubic_min.argtypes = [C.c_double] * 5 + [f64p]
+    L.or_mj_impedance.restype = C.c_double
+    L.or_mj_impedance.argtypes = [f64p, C.c_double]
+    L.or_mj_contact_params.restype = None
+    L.or_mj_contact_params.argtypes = [C.c_void_p, C.c_int, C.c_int, f64p, f64p, f64p, i32p]
     return L
 
 
@@ -171,6 +178,24 @@ class OracleModel:
         m.filter_count = int(len(h["shape_collision_filter_pairs"]))
         g = np.asarray(h["gravity"], dtype=np.float32).astype(np.float64)
         m.gravity = (C.c_double * 3)(*g)
+        # MuJoCo geom parameters (contact law 1), resolved as SolverMuJoCo does
+        mj = getattr(model, "mujoco", None)
+        if mj is not None and m.shape_count > 0:
+            f32 = lambda t: np.ascontiguousarray(t.detach().cpu().numpy().astype(np.float32).astype(np.float64))
+            sr = f32(mj.solref).reshape(-1, 2)
+            authored = (sr[:, 0] != 0) | (sr[:, 1] != 0)
+            conv = mj_convert_solref(k["shape_ke"], np.asarray(h["shape_material_kd"], np.float32).astype(np.float64))
+            k["mj_solref"] = np.ascontiguousarray(np.where(authored[:, None], sr, conv).astype(np.float32)
+                                                  .astype(np.float64)).reshape(-1)
+            k["mj_solimp"] = f32(mj.geom_solimp).reshape(-1)
+            k["mj_solmix"] = f32(mj.geom_solmix).reshape(-1)
+            k["mj_priority"] = np.ascontiguousarray(mj.geom_priority.detach().cpu().numpy().astype(np.int32))
+            k["mj_condim"] = np.ascontiguousarray(mj.condim.detach().cpu().numpy().astype(np.int32))
+            for name in ("mj_solref", "mj_solimp", "mj_solmix"):
+                setattr(m, name, _ptr(k[name], f64p))
+            for name in ("mj_priority", "mj_condim"):
+                setattr(m, name, _ptr(k[name], i32p))
+            m.mj_zero_margins = int(np.any(k["shape_type"] == 7))
         self.desc = m
         self.world_count = m.world_count
         self.body_count = m.body_count
@@ -273,9 +298,22 @@ def art_attempt_all(om, oa, cfg, q, v, dt):
                 n_contacts_mid=ncm)
 
 
+def mj_convert_solref(ke, kd):
+    """MuJoCo (time constant, damping ratio) of contact stiffness ke and damping kd
+    (convert_solref(ke, kd, 1, 1), solvers/mujoco/kernels.py:193-207): 2/kd and kd/2 sqrt(1/ke);
+    (0.02, 1) when either is not positive."""
+    ke = np.asarray(ke, dtype=np.float64)
+    kd = np.asarray(kd, dtype=np.float64)
+    ok = (ke > 0) & (kd > 0)
+    tc = np.where(ok, 2.0 / np.where(ok, kd, 1.0), 0.02)
+    dr = np.where(ok, kd / 2.0 * np.sqrt(1.0 / np.where(ok, ke, 1.0)), 1.0)
+    return np.stack([tc, dr], axis=-1)
+
+
 def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
                 max_substeps=16, max_iterations=30, max_contacts=128, opt_rel_tol=None, opt_abs_tol=1e-14,
-                tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10, line_search="exact_root") -> OrConfig:
+                tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10, line_search="exact_root",
+                contact_law="sap", impratio=1.0) -> OrConfig:
     c = OrConfig()
     c.mode = 1 if mode == "adaptive" else 0
     c.controller = 0 if controller == "sap" else 1
@@ -285,7 +323,24 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     c.opt_rel_tol = opt_rel_tol if opt_rel_tol is not None else 1e-10
     c.opt_abs_tol, c.tau_d, c.sigma, c.beta, c.ls_rel_tol = opt_abs_tol, tau_d, sigma, beta, ls_rel_tol
     c.line_search = {"exact_root": 0, "armijo_decay": 1}[line_search]
+    c.contact_law = {"sap": 0, "mujoco": 1}[contact_law]
+    c.impratio = float(impratio)
     return c
+
+
+def mj_impedance(solimp, pos: float) -> float:
+    """MuJoCo impedance d(pos) of solimp (dmin, dmax, width, midpoint, power)."""
+    si = np.ascontiguousarray(solimp, dtype=np.float64)
+    return float(lib().or_mj_impedance(_ptr(si, f64p), float(pos)))
+
+
+def mj_contact_params(om: "OracleModel", a: int, b: int) -> dict:
+    """MuJoCo contact parameters of the shape pair (a, b): solref, solimp, friction, condim."""
+    sr, si, mu = np.zeros(2), np.zeros(5), np.zeros(1)
+    cd = np.zeros(1, np.int32)
+    lib().or_mj_contact_params(C.byref(om.desc), int(a), int(b), _ptr(sr, f64p), _ptr(si, f64p), _ptr(mu, f64p),
+                               _ptr(cd, i32p))
+    return {"solref": sr, "solimp": si, "mu": float(mu[0]), "condim": int(cd[0])}
 
 
 def collide_world(om: OracleModel, w: int, body_q: np.ndarray, cap: int = 128):
@@ -439,23 +494,26 @@ def substep_world(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h, guess=Non
 def substep_problem(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h):
     """The convex problem of one inner step of world ``w`` (contacts at q) and the oracle's solution:
     dict(A [N,N], vstar [N], J [3 nc, N] rows t1, t2, n per contact, Rt, Rn, vhat, mu [nc], v [N],
-    converged), N = 6 bodies."""
+    converged), N = 6 bodies; under the MuJoCo law also D [nc] (the rows' 1/R, 0 inactive) and
+    what [nc, 3] (the contact velocity the rows drive towards), mu 0 for condim 1."""
     b0, b1 = int(om.world_body_start[w]), int(om.world_body_start[w + 1])
     q = np.ascontiguousarray(q[b0:b1], dtype=np.float64)
     qd = np.ascontiguousarray(qd[b0:b1], dtype=np.float64)
     f = np.ascontiguousarray(f[b0:b1], dtype=np.float64)
     N, ncap = 6 * (b1 - b0), max(int(cfg.max_contacts), 1)
     A = np.zeros((N, N)); vs = np.zeros(N); J = np.zeros((3 * ncap, N)); v = np.zeros(N)
-    Rt, Rn, vh, mu = (np.zeros(ncap) for _ in range(4))
+    Rt, Rn, vh, mu, D = (np.zeros(ncap) for _ in range(5))
+    what = np.zeros((ncap, 3))
     conv = C.c_int(0)
     L = lib()
     L.or_substep_problem.restype = C.c_int
     nc = L.or_substep_problem(C.byref(om.desc), C.byref(cfg), w, _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
                               C.c_double(h), N, ncap, _ptr(A, f64p), _ptr(vs, f64p), _ptr(J, f64p), _ptr(Rt, f64p),
-                              _ptr(Rn, f64p), _ptr(vh, f64p), _ptr(mu, f64p), _ptr(v, f64p), C.byref(conv))
+                              _ptr(Rn, f64p), _ptr(vh, f64p), _ptr(mu, f64p), _ptr(v, f64p), C.byref(conv),
+                              _ptr(D, f64p), _ptr(what, f64p))
     assert nc >= 0
     return dict(A=A, vstar=vs, J=J[:3 * nc], Rt=Rt[:nc], Rn=Rn[:nc], vhat=vh[:nc], mu=mu[:nc], v=v,
-                converged=bool(conv.value))
+                converged=bool(conv.value), D=D[:nc], what=what[:nc])
 
 
 def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt):
