@@ -158,6 +158,41 @@ def precision(na, model, q0, qd0, dt, tol, sample, log):
     return res
 
 
+def mujoco_form(na, args, log):
+    """SolverMuJoCoAdaptive with MuJoCo's soft-constraint inner step (contact law 1) on the same
+    uneven piles: its own 20 settling frames, then `args.steps` boundary calls timed (one
+    persistent k_frame launch per frame, worlds ordered by their last frame's cost)."""
+    import torch
+
+    model = na.scenes.uneven_pile(args.worlds, seed=args.seed, device="cuda:0")
+    sol = na.SolverMuJoCoAdaptive(model, **dict(KW, tol=args.tol, dt_inner_max=args.dt_outer))
+    st = model.state()
+    for _ in range(SETTLE_FRAMES):
+        sol.step(st, st, model.control(), None, args.dt_outer)
+    torch.cuda.synchronize()
+    sol.reset_work_counters()
+    before = sol.accepted_steps()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        sol.step(st, st, model.control(), None, args.dt_outer)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    acc = sol.accepted_steps() - before
+    wk = sol._work.sum(dim=0).tolist()
+    log(f"[bench] mujoco inner step: {acc} accepted in {wall:.3f} s ({acc / wall:.0f}/s)")
+    return {"value": acc / wall, "unit": "accepted world-steps/s", "accepted": int(acc), "wall_s": wall,
+            "device_ms": e0.elapsed_time(e1), "attempts": int(wk[0]), "newton_iterations": int(wk[1]),
+            "contacts": int(wk[3]), "diverged": int(sol.diverged.sum().item()),
+            "workload": f"uneven_pile_{args.worlds}, tol {args.tol:g}, {args.steps} frames of "
+                        f"{args.dt_outer * 1e3:g} ms after {SETTLE_FRAMES} settling frames",
+            "form": "SolverMuJoCoAdaptive boundary calls (k_frame per frame): MuJoCo soft-constraint rows, "
+                    "pyramidal cone, MuJoCo controller"}
+
+
 def articulated_form(na, args, log):
     """Articulated worlds (half 7-dof arms, half floating 12-dof quadrupeds on box terrain): served
     boundary calls (na_step_articulated, one persistent launch per frame) at tol `args.tol`, timed
@@ -351,6 +386,8 @@ def main():
     ap.add_argument("--precision-worlds", type=int, default=512)
     ap.add_argument("--no-articulated", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
+    ap.add_argument("--no-mujoco", action="store_true", help="skip the MuJoCo inner-step line")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule (4 waves per CU)")
     ap.add_argument("--articulated-only", action="store_true",
                     help="run only the articulated workload (profiling of k_art_frame); prints its JSON object")
     ap.add_argument("--art-worlds", type=int, default=8192)
@@ -388,16 +425,24 @@ def main():
     torch.cuda.synchronize()
     q0, qd0 = st.body_q.clone(), st.body_qd.clone()
     dt0 = warm._ideal_dt.clamp(max=args.dt_outer).cpu().numpy().copy()  # each world's working step
+    tune = None
+    if not args.no_autotune:
+        # schedule autotuner of the boundary call on the settled state (restores it)
+        tune = warm.autotune(st, args.dt_outer, frames=2)
+        log(f"[bench] autotune (boundary calls): best {tune['best']}")
     del warm
 
     # ---------------------------------------------------------------- served rollout (timed)
     stream = torch.cuda.current_stream()
 
-    def served_run():
-        """One served rollout launch of args.steps frames of every world from the settled state."""
+    def served_run(grid=0):
+        """One served rollout launch of args.steps frames of every world from the settled state
+        (``grid``: persistent grid in waves, 0 = one wave per resident slot)."""
         eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
                                shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
                                device=dev, **kw)
+        if grid:
+            eng.solver.set_schedule(grid)
         reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
         eng.submit(reqs)
         eng.admit(math.inf)
@@ -414,10 +459,23 @@ def main():
         assert not eng.resident, "every rollout finishes in the one launch without a time budget"
         return eng, reqs, wall, e0.elapsed_time(e1)
 
-    for k in range(args.warmup):
-        _, _, w_wall, _ = served_run()
+    served_grid, served_tune = 0, None
+    k0 = 0
+    if not args.no_autotune and args.warmup >= 3:
+        # the first untimed repeats time the served launch on 4, 3 and 2 waves per CU (same results)
+        def tune_run(grid):
+            e, r, w_wall, ms = served_run(grid)
+            sig = torch.as_tensor(np.concatenate([x.body_q.reshape(-1) for x in r]))
+            del e
+            return ms, sig
+        served_tune = na.serving.autotune_served(tune_run)
+        served_grid = served_tune["best"]["grid_waves"]
+        k0 = len(served_tune["candidates"])
+        log(f"[bench] autotune (served launch): {served_tune['candidates']} -> {served_tune['best']}")
+    for k in range(k0, args.warmup):
+        _, _, w_wall, _ = served_run(served_grid)
         log(f"[bench] untimed repeat {k + 1}/{args.warmup} of the timed launch: {w_wall:.3f} s")
-    eng, reqs, wall, launch_ms = served_run()
+    eng, reqs, wall, launch_ms = served_run(served_grid)
     accepted = sum(r.accepted for r in reqs)
     value = accepted / wall
     slots = np.array([r.slot for r in reqs])
@@ -465,6 +523,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_serve",
                      "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": launch_ms},
     }
+    if tune is not None:
+        out["autotune"] = {"boundary_calls": dict(tune, form="2 boundary calls of the settled piles per candidate "
+                                                               "(waves per CU x world order), from the same state")}
+        if served_tune is not None:
+            out["autotune"]["served"] = dict(served_tune, form="the timed served launch on each candidate grid "
+                                                              "(the first untimed repeats), largest worlds first")
 
     # ---------------------------------------------------------------- lock-step forms
     def frames_form(schedule):
@@ -544,6 +608,8 @@ def main():
     # ---------------------------------------------------------------- articulated worlds
     if not args.no_articulated:
         out["articulated"] = articulated_form(na, args, log)
+    if not args.no_mujoco:
+        out["mujoco"] = mujoco_form(na, args, log)
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:

@@ -159,6 +159,27 @@ def requests_from_model(model, *, horizon_frames=None, horizon_accepted=None, to
     return out
 
 
+def autotune_served(run, waves_per_cu=(4, 3, 2)) -> dict:
+    """Pick the persistent grid of a served rollout launch by timing it on the device.
+
+    ``run(grid_waves)`` performs one served rollout launch of a fixed set of requests from the same
+    start (a fresh engine each time) with that persistent grid and returns ``(launch_ms,
+    signature)``, ``signature`` a tensor of the rollout's results.  A grid decides only which wave
+    runs a world, never its arithmetic, so every candidate must return the same signature bit for
+    bit; a mismatch raises.  Returns ``{"best": row, "candidates": rows}`` with rows of
+    ``waves_per_cu``, ``grid_waves`` and ``ms``."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    rows, ref = [], None
+    for wpc in waves_per_cu:
+        ms, sig = run(int(wpc) * cus)
+        if ref is None:
+            ref = sig
+        elif not torch.equal(sig, ref):
+            raise RuntimeError("autotune_served: grids gave different results (a schedule changed arithmetic)")
+        rows.append({"waves_per_cu": int(wpc), "grid_waves": int(wpc) * cus, "ms": float(ms)})
+    return {"best": min(rows, key=lambda r: r["ms"]), "candidates": rows}
+
+
 class _Arena:
     """First-fit allocator of contiguous index ranges in ``[0, capacity)``."""
 

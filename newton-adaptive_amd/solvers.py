@@ -405,6 +405,88 @@ class _SolverBase:
         w = self._work.sum(dim=0).tolist()
         return dict(zip(_WORK_FIELDS, w))
 
+    # ------------------------------------------------------------------ autotuner
+    _TUNE_STATE = ("_dt", "_dt_half", "_ideal_dt", "_sim_time", "_next_time", "_last_error", "_accepted_error",
+                   "_accepted", "_commit", "_diverged", "_substeps_frame", "_iterations", "_solve_fail", "_overflow",
+                   "_cum_accepted", "_work", "_q64")
+
+    def autotune(self, state, dt_outer: float, frames: int = 2, waves_per_cu=(2, 3, 4),
+                 orders=("cost", "size", "index"), control=None) -> dict:
+        """Pick the persistent launch schedule of the boundary call by timing it on the device.
+
+        Each candidate (waves per CU of the persistent grid, world visiting order) runs ``frames``
+        boundary calls from the same snapshot of ``state`` and of the controller; the device time
+        of those calls decides.  Orders: "cost" (the worlds' work in an untimed probe run, most
+        expensive first), "size" (bodies + shapes, largest first), "index".  A schedule decides
+        only which wave runs a world, never its arithmetic, so every candidate must leave the same
+        state bit for bit; that is checked, and a mismatch raises.  The state and controller are
+        restored afterwards, the chosen schedule is kept for the following calls (``world_order``
+        becomes "index" so that the order is not replaced after the next call) and the table is
+        returned and kept in ``schedule_choice``."""
+        if self._schedule != "served":
+            raise ValueError("autotune applies to the served (persistent launch) schedule")
+        m = self.model
+        if m.is_articulated:
+            raise ValueError("autotune is built for free-body models (their whole step state is restorable)")
+        dev = m.device
+        snap = {k: getattr(self, k).clone() for k in self._TUNE_STATE}
+        q0 = state.body_q.clone()
+        qd0 = state.body_qd.clone()
+
+        def restore():
+            for k, v in snap.items():
+                getattr(self, k).copy_(v)
+            state.body_q.copy_(q0)
+            state.body_qd.copy_(qd0)
+
+        saved_order = self._world_order
+        self._world_order = "index"
+        # untimed probe: per-world work for the cost order
+        self.set_schedule(0, None)
+        w_before = self._work.clone()
+        for _ in range(frames):
+            self.step(state, state, control, None, dt_outer)
+        cost_w = self._work - w_before
+        restore()
+        cost = (cost_w[:, 2] + 5 * cost_w[:, 1] + 10 * cost_w[:, 0]).to(torch.float64)
+        wb = m.world_body_start.to(torch.int64)
+        ws = m.world_shape_start.to(torch.int64)
+        size = (wb[1:] - wb[:-1] + ws[1:] - ws[:-1]).to(torch.float64)
+        order_of = {"cost": torch.argsort(cost, descending=True, stable=True),
+                    "size": torch.argsort(size, descending=True, stable=True), "index": None}
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        stream = torch.cuda.current_stream(dev)
+        rows, ref = [], None
+        for wpc in waves_per_cu:
+            for name in orders:
+                order = order_of[name]
+                self.set_schedule(int(wpc) * cus, order.to(torch.int32) if order is not None else None,
+                                  _checked=True)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(frames):
+                    self.step(state, state, control, None, dt_outer)
+                e1.record(stream)
+                torch.cuda.synchronize(dev)
+                out = (state.body_q.clone(), state.body_qd.clone(), self._q64.clone())
+                if ref is None:
+                    ref = out
+                elif not all(torch.equal(a, b) for a, b in zip(out, ref)):
+                    restore()
+                    raise RuntimeError("autotune: schedules gave different states (a schedule changed arithmetic)")
+                rows.append({"waves_per_cu": int(wpc), "order": name, "ms": e0.elapsed_time(e1)})
+                restore()
+        best = min(rows, key=lambda r: r["ms"])
+        order = order_of[best["order"]]
+        self.set_schedule(best["waves_per_cu"] * cus, order.to(torch.int32) if order is not None else None,
+                          _checked=True)
+        if best["order"] != "index" or saved_order == "index":
+            self._world_order = "index"
+        else:
+            self._world_order = saved_order
+        self.schedule_choice = {"best": best, "candidates": rows, "frames": int(frames)}
+        return self.schedule_choice
+
     def reset_work_counters(self) -> None:
         self._work.zero_()
         if self._work_prev is not None:
