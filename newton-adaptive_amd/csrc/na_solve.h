@@ -5,7 +5,9 @@
 // body, free-motion velocity v* and contact Jacobian J,
 //     l(v) = 1/2 |v - v*|_A^2 + sum_i 1/2 |P_F(y_i)|_R^2,  y_i = -R^-1 (J_i v - v_hat_i),
 // minimised by Newton's method.  One lane owns one body (registers) and lanes stride over
-// contacts (LDS); J^T gamma and J^T G J are scattered through LDS float atomics.
+// contacts (LDS).  The contact -> body sums (J^T gamma, J^T G J p, the blocks J^T G J) use no LDS
+// atomics: contact lanes store one world-frame value per contact, and each body lane sums the
+// entries of its own contacts, listed once per solve in contact order (build_entries).
 // The Newton direction comes from block-Jacobi preconditioned conjugate gradients on
 // H = A + J^T G J (matrix free), the step length from an exact line search seeded by the
 // cubic Hermite minimiser (N&W eq. 3.59, test_adaptive_cubic_linesearch.py:36-60) or, as the
@@ -37,21 +39,24 @@ struct BodyLDS {
   float iwi[6][NA_MAX_BODIES];   // world inverse inertia (xx, yy, zz, xy, xz, yz)
   float im[NA_MAX_BODIES];       // inverse mass
   float v[6][NA_BSLOT];          // f32 exchange vector (PCG directions)
-  float acc[6][NA_BSLOT];        // f32 scatter target (H p)
-  // The FP64 vectors are dead from the gradient (which consumes accd) until the line search
-  // (which rewrites vd), exactly the span in which the preconditioner blocks are accumulated
-  // and factorised, so the two share storage; this keeps a world's LDS under 40 KB (four
-  // worlds per CU).
-  union {
-    struct {
-      double vd[6][NA_BSLOT];    // f64 exchange vector (Newton iterate, line-search direction)
-      double accd[6][NA_BSLOT];  // f64 scatter target (J^T gamma)
-    };
-    float pre[21][NA_BSLOT];     // block-Jacobi accumulation (6x6 lower, row-major packed)
-  };
+  double vd[6][NA_BSLOT];        // f64 exchange vector (Newton iterate, line-search direction)
+  // Per-contact values that the body lanes sum, one phase at a time: the world-frame impulse of
+  // J^T gamma (fd, FP64), the world-frame Hessian block F^T G F (gw: xx yy zz xy xz yz), the
+  // world-frame force of the PCG product J^T G J p (f).
+  union __attribute__((aligned(16))) {
+    double fd[NA_MAX_CONTACTS][4];
+    float gw[NA_MAX_CONTACTS][8];
+    float f[NA_MAX_CONTACTS][4];
+  } xb;
 };
 
 struct SolveLDS {
+  // Body entries of the solve (build_entries): entry e of body b (e in [eoff[b], eoff[b] +
+  // edeg[b])) is contact ent[e], with the lever arm from the body's COM and the side's sign
+  // (-1 body A, +1 body B) in er[e].
+  float er[2 * NA_MAX_CONTACTS][4] __attribute__((aligned(16)));
+  uint8_t ent[2 * NA_MAX_CONTACTS];
+  uint16_t eoff[NA_MAX_BODIES], edeg[NA_MAX_BODIES];
   float t1[3][NA_MAX_CONTACTS];
   float ra[3][NA_MAX_CONTACTS];  // contact point - COM of body A
   float rb[3][NA_MAX_CONTACTS];
@@ -61,7 +66,7 @@ struct SolveLDS {
 
 // The segment scratch is only live during contact generation and the per-contact solver
 // data only during a solve, so they share storage.
-struct WorldLDS {
+struct __attribute__((aligned(16))) WorldLDS {
   union {
     SegLDS seg;
     SolveLDS sol;
@@ -139,35 +144,6 @@ __device__ __forceinline__ d3 row_vel_d(const BodyLDS& B, const CRow& r) {
   d3 rel = d3{vb.x + xb.x - va.x - xa.x, vb.y + xb.y - va.y - xa.y, vb.z + xb.z - va.z - xa.z};
   return d3{ddot(dd(r.t1), rel), ddot(dd(r.t2), rel), ddot(dd(r.n), rel)};
 }
-// acc += J^T lambda (f32 LDS atomics; only real bodies are written)
-__device__ __forceinline__ void row_scatter(BodyLDS& B, const CRow& r, f3 lam) {
-  f3 fw = r.t1 * lam.x + r.t2 * lam.y + r.n * lam.z;  // world impulse on B
-  f3 tb = cross3(r.rb, fw), ta = cross3(r.ra, fw);
-  if (r.hb) {
-    atomicAdd(&B.acc[0][r.sb], fw.x); atomicAdd(&B.acc[1][r.sb], fw.y); atomicAdd(&B.acc[2][r.sb], fw.z);
-    atomicAdd(&B.acc[3][r.sb], tb.x); atomicAdd(&B.acc[4][r.sb], tb.y); atomicAdd(&B.acc[5][r.sb], tb.z);
-  }
-  if (r.ha) {
-    atomicAdd(&B.acc[0][r.sa], -fw.x); atomicAdd(&B.acc[1][r.sa], -fw.y); atomicAdd(&B.acc[2][r.sa], -fw.z);
-    atomicAdd(&B.acc[3][r.sa], -ta.x); atomicAdd(&B.acc[4][r.sa], -ta.y); atomicAdd(&B.acc[5][r.sa], -ta.z);
-  }
-}
-// accd += J^T gamma (f64 LDS atomics; only real bodies are written)
-__device__ __forceinline__ void row_scatter_d(BodyLDS& B, const CRow& r, d3 lam) {
-  d3 t1 = dd(r.t1), t2 = dd(r.t2), n = dd(r.n);
-  d3 fw = d3{t1.x * lam.x + t2.x * lam.y + n.x * lam.z, t1.y * lam.x + t2.y * lam.y + n.y * lam.z,
-             t1.z * lam.x + t2.z * lam.y + n.z * lam.z};
-  d3 tb = dcross(dd(r.rb), fw), ta = dcross(dd(r.ra), fw);
-  if (r.hb) {
-    atomicAdd(&B.accd[0][r.sb], fw.x); atomicAdd(&B.accd[1][r.sb], fw.y); atomicAdd(&B.accd[2][r.sb], fw.z);
-    atomicAdd(&B.accd[3][r.sb], tb.x); atomicAdd(&B.accd[4][r.sb], tb.y); atomicAdd(&B.accd[5][r.sb], tb.z);
-  }
-  if (r.ha) {
-    atomicAdd(&B.accd[0][r.sa], -fw.x); atomicAdd(&B.accd[1][r.sa], -fw.y); atomicAdd(&B.accd[2][r.sa], -fw.z);
-    atomicAdd(&B.accd[3][r.sa], -ta.x); atomicAdd(&B.accd[4][r.sa], -ta.y); atomicAdd(&B.accd[5][r.sa], -ta.z);
-  }
-}
-
 // SAP projection onto the friction cone in the R norm (FP64); returns cost 1/2 gamma^T R gamma,
 // writes gamma and the symmetric G = dP/dy R^-1 (00 11 22 01 02 12) in f32.  iRt = 1/Rt,
 // iRn = 1/Rn.  Regions: stiction |y_t| <= mu y_n, separation y_n <= -mu_hat |y_t|, sliding.
@@ -270,31 +246,131 @@ __device__ __forceinline__ f3 sym3_mul(const float* G, f3 u) {
             G[4] * u.x + G[5] * u.y + G[2] * u.z);
 }
 
-// scatter J_b^T G J_b (6x6 lower packed) into the block-Jacobi accumulator of body slot b
-__device__ __forceinline__ void scatter_block(BodyLDS& B, int b, const CRow& F, f3 r, const float* G) {
-  float J[3][6];
-  f3 Fk[3] = {F.t1, F.t2, F.n};
+// ---------------------------------------------------------------- contact -> body sums
+// Entries of the solve's contacts per body, built once per solve: body lane b scans the contact
+// bodies (four per 32-bit LDS word, read by every lane at once) twice, counting and then listing
+// the contacts that touch it in contact order; the offsets come from a wave scan.  Static sides
+// belong to no lane.  Requires X.ra / X.rb (the solve's lever arms).
+__device__ void build_entries(int nb, int nc, const ContactLDS& C, SolveLDS& X) {
+  const int l = lane_id();
+  const int me = l < nb ? l : -2;  // -1 is the static side
+  const int nw = (nc + 3) >> 2;
+  const int* wa = reinterpret_cast<const int*>(C.ba);
+  const int* wb = reinterpret_cast<const int*>(C.bb);
+  int deg = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int a = wa[w], b = wb[w];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (4 * w + j < nc) {
+        deg += ((int)(int8_t)((a >> (8 * j)) & 0xff) == me) + ((int)(int8_t)((b >> (8 * j)) & 0xff) == me);
+      }
+    }
+  }
+  const int off = wave_incl_scan(deg) - deg;
+  int k = off;
+  for (int w = 0; w < nw; ++w) {
+    const int a = wa[w], b = wb[w];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = 4 * w + j;
+      if (c < nc) {
+        if ((int)(int8_t)((a >> (8 * j)) & 0xff) == me) {
+          X.ent[k] = (uint8_t)c;
+          X.er[k][0] = X.ra[0][c]; X.er[k][1] = X.ra[1][c]; X.er[k][2] = X.ra[2][c]; X.er[k][3] = -1.f;
+          ++k;
+        }
+        if ((int)(int8_t)((b >> (8 * j)) & 0xff) == me) {
+          X.ent[k] = (uint8_t)c;
+          X.er[k][0] = X.rb[0][c]; X.er[k][1] = X.rb[1][c]; X.er[k][2] = X.rb[2][c]; X.er[k][3] = 1.f;
+          ++k;
+        }
+      }
+    }
+  }
+  if (l < NA_MAX_BODIES) {
+    X.eoff[l] = (uint16_t)off;
+    X.edeg[l] = (uint16_t)deg;
+  }
+  wave_sync();
+}
+
+// world-frame impulse sum_k F_k lam_k of a contact-frame vector (rows t1, t2, n)
+__device__ __forceinline__ d3 world_d(const CRow& r, d3 lam) {
+  return d3{r.t1.x * lam.x + r.t2.x * lam.y + r.n.x * lam.z, r.t1.y * lam.x + r.t2.y * lam.y + r.n.y * lam.z,
+            r.t1.z * lam.x + r.t2.z * lam.y + r.n.z * lam.z};
+}
+
+// This lane's body: jt += J_b^T gamma over its entries, from the world-frame impulses in B.xb.fd
+// (sign -1 on body A, +1 on body B; torque r x f).
+__device__ __forceinline__ void sum_entries_d(const BodyLDS& B, const SolveLDS& X, int eo, int ed, double* jt) {
+  for (int e = eo; e < eo + ed; ++e) {
+    const int c = X.ent[e];
+    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
+    const double s = rr.w;
+    const d3 f = d3{s * B.xb.fd[c][0], s * B.xb.fd[c][1], s * B.xb.fd[c][2]};
+    const d3 t = dcross(d3{rr.x, rr.y, rr.z}, f);
+    jt[0] += f.x; jt[1] += f.y; jt[2] += f.z;
+    jt[3] += t.x; jt[4] += t.y; jt[5] += t.z;
+  }
+}
+
+// This lane's body: acc += J_b^T u over its entries, from the world-frame forces in B.xb.f.
+__device__ __forceinline__ void sum_entries_f(const BodyLDS& B, const SolveLDS& X, int eo, int ed, float* acc) {
+  for (int e = eo; e < eo + ed; ++e) {
+    const int c = X.ent[e];
+    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
+    const float4 fq = *reinterpret_cast<const float4*>(B.xb.f[c]);
+    const f3 f = mk(rr.w * fq.x, rr.w * fq.y, rr.w * fq.z);
+    const f3 t = cross3(mk(rr.x, rr.y, rr.z), f);
+    acc[0] += f.x; acc[1] += f.y; acc[2] += f.z;
+    acc[3] += t.x; acc[4] += t.y; acc[5] += t.z;
+  }
+}
+
+// World-frame Hessian block F^T G F of a contact (G symmetric in the contact frame, packed
+// 00 11 22 01 02 12 over t1 t2 n), packed xx yy zz xy xz yz.
+__device__ __forceinline__ void world_block(const CRow& r, const float* G, float* W) {
+  const f3 F[3] = {r.t1, r.t2, r.n};
+  const float Gf[3][3] = {{G[0], G[3], G[4]}, {G[3], G[1], G[5]}, {G[4], G[5], G[2]}};
+  float xx = 0.f, yy = 0.f, zz = 0.f, xy = 0.f, xz = 0.f, yz = 0.f;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    f3 a = cross3(r, Fk[k]);
-    J[k][0] = Fk[k].x; J[k][1] = Fk[k].y; J[k][2] = Fk[k].z;
-    J[k][3] = a.x; J[k][4] = a.y; J[k][5] = a.z;
+    const f3 h = F[0] * Gf[k][0] + F[1] * Gf[k][1] + F[2] * Gf[k][2];  // (G F)_k as a world vector
+    xx += F[k].x * h.x; yy += F[k].y * h.y; zz += F[k].z * h.z;
+    xy += F[k].x * h.y; xz += F[k].x * h.z; yz += F[k].y * h.z;
   }
-  float GJ[3][6];
+  W[0] = xx; W[1] = yy; W[2] = zz; W[3] = xy; W[4] = xz; W[5] = yz;
+}
+
+// This lane's body: L (6x6 lower, row-major packed) += sum over its entries of
+// J_b^T G J_b = [[Gw, -Gw [r]x], [[r]x Gw, -[r]x Gw [r]x]], J_b = F [I, -[r]x].
+__device__ __forceinline__ void sum_entries_block(const BodyLDS& B, const SolveLDS& X, int eo, int ed, float* L) {
+  for (int e = eo; e < eo + ed; ++e) {
+    const int c = X.ent[e];
+    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
+    const float4 g0 = *reinterpret_cast<const float4*>(&B.xb.gw[c][0]);
+    const float4 g1 = *reinterpret_cast<const float4*>(&B.xb.gw[c][4]);
+    const float G[3][3] = {{g0.x, g0.w, g1.x}, {g0.w, g0.y, g1.y}, {g1.x, g1.y, g0.z}};
+    const float rx = rr.x, ry = rr.y, rz = rr.z;
+    // M = [r]x Gw, rows ([r]x row i) . columns of Gw
+    float M[3][3];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    f3 col = sym3_mul(G, mk(J[0][i], J[1][i], J[2][i]));
-    GJ[0][i] = col.x; GJ[1][i] = col.y; GJ[2][i] = col.z;
-  }
-  int idx = 0;
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-#pragma unroll
-    for (int j = 0; j <= i; ++j) {
-      float v = J[0][i] * GJ[0][j] + J[1][i] * GJ[1][j] + J[2][i] * GJ[2][j];
-      atomicAdd(&B.pre[idx][b], v);
-      ++idx;
+    for (int j = 0; j < 3; ++j) {
+      M[0][j] = -rz * G[1][j] + ry * G[2][j];
+      M[1][j] = rz * G[0][j] - rx * G[2][j];
+      M[2][j] = -ry * G[0][j] + rx * G[1][j];
     }
+    // S = -M [r]x: S[i][j] = -(M[i][0] X[0][j] + M[i][1] X[1][j] + M[i][2] X[2][j])
+    const float S00 = -(M[0][1] * rz - M[0][2] * ry), S11 = -(-M[1][0] * rz + M[1][2] * rx);
+    const float S22 = -(M[2][0] * ry - M[2][1] * rx);
+    const float S10 = -(M[1][1] * rz - M[1][2] * ry), S20 = -(M[2][1] * rz - M[2][2] * ry);
+    const float S21 = -(-M[2][0] * rz + M[2][2] * rx);
+    L[0] += G[0][0]; L[1] += G[1][0]; L[2] += G[1][1]; L[3] += G[2][0]; L[4] += G[2][1]; L[5] += G[2][2];
+    L[6] += M[0][0]; L[7] += M[0][1]; L[8] += M[0][2];
+    L[10] += M[1][0]; L[11] += M[1][1]; L[12] += M[1][2];
+    L[15] += M[2][0]; L[16] += M[2][1]; L[17] += M[2][2];
+    L[9] += S00; L[13] += S10; L[14] += S11; L[18] += S20; L[19] += S21; L[20] += S22;
   }
 }
 
@@ -520,14 +596,14 @@ __device__ __forceinline__ CConst load_const(const ContactLDS& C, const SolveLDS
   return CConst{X.iRt[c], X.iRn[c], X.Rt[c], X.Rn[c], X.vhat[c], C.mu[c]};
 }
 
-// FP64 evaluation at the iterate in B.vd (B.accd zeroed by the caller, both visible to all
-// lanes) for this lane's contact rows `r`: J^T gamma into B.accd, the f32 blocks G into `G`
-// and, when `gather` is set, the contact velocities into `vc` (otherwise `vc` already holds
-// them), both in registers.  Returns this lane's part of the
-// contact cost.
+// FP64 evaluation at the iterate in B.vd (visible to all lanes) for this lane's contact rows
+// `r`: the f32 blocks G into `G` and, when `gather` is set, the contact velocities into `vc`
+// (otherwise `vc` already holds them), both in registers; this lane's body gets J^T gamma in
+// `jt` (entries [eo, eo + ed)).  Returns this lane's part of the contact cost.
 template <int NG, int LAW>
-__device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow* r, const CConst* k, float (*G)[6],
-                                                  d3* vc, bool gather) {
+__device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const SolveLDS& X, const CRow* r,
+                                                  const CConst* k, float (*G)[6], d3* vc, bool gather, int eo,
+                                                  int ed, double* jt) {
   const int l = lane_id();
   if (gather) {
 #pragma unroll
@@ -549,21 +625,37 @@ __device__ __forceinline__ double eval_contacts_g(int nc, BodyLDS& B, const CRow
       cc = sap_project(k[g].mu, k[g].Rt, k[g].Rn, iRt, iRn, y, gam, G[g]);
     }
     cost += valid ? cc : 0.0;
-    row_scatter_d(B, r[g], gam);
+    if (valid) {
+      const d3 fw = world_d(r[g], gam);
+      B.xb.fd[c][0] = fw.x; B.xb.fd[c][1] = fw.y; B.xb.fd[c][2] = fw.z;
+    }
   }
   wave_sync();
+#pragma unroll
+  for (int q = 0; q < 6; ++q) jt[q] = 0.0;
+  sum_entries_d(B, X, eo, ed, jt);
   return cost;
 }
 
-// Block-Jacobi accumulation: B.pre[b] += J_b^T G J_b for both sides of every contact.
+// Block-Jacobi blocks: this lane's body gets L += sum over its contacts of J_b^T G J_b (the
+// world-frame blocks F^T G F of every contact stored in B.xb.gw first).  The caller has made
+// every lane's reads of B.xb (J^T gamma) complete.
 template <int NG>
-__device__ __forceinline__ void precond_scatter_g(BodyLDS& B, const CRow* r, const float (*G)[6]) {
+__device__ __forceinline__ void precond_blocks_g(int nc, BodyLDS& B, const SolveLDS& X, const CRow* r,
+                                                 const float (*G)[6], int eo, int ed, float* L) {
+  const int l = lane_id();
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
-    if (r[g].ha) scatter_block(B, r[g].sa, r[g], r[g].ra, G[g]);
-    if (r[g].hb) scatter_block(B, r[g].sb, r[g], r[g].rb, G[g]);
+    const int c = l + g * NA_WAVE;
+    if (c < nc) {
+      float W[6];
+      world_block(r[g], G[g], W);
+      *reinterpret_cast<float4*>(&B.xb.gw[c][0]) = make_float4(W[0], W[1], W[2], W[3]);
+      *reinterpret_cast<float4*>(&B.xb.gw[c][4]) = make_float4(W[4], W[5], 0.f, 0.f);
+    }
   }
   wave_sync();
+  sum_entries_block(B, X, eo, ed, L);
 }
 
 // One contact's line-search constants, held in registers for every trial step of a Newton
@@ -649,10 +741,13 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double al
 // Preconditioned CG on H x = -g * gscale (f32) with the block-Jacobi factor L of this lane's
 // body; stops when the preconditioned residual r.z falls to eta2 (the squared relative
 // tolerance) times its start value, or after max_cg iterations.  The lane's
-// contact rows `rw` and G blocks `G` are the caller's registers.  Returns the iteration count.
+// contact rows `rw` and G blocks `G` are the caller's registers.  The product H p = A p + J^T G J p
+// stores each contact's world-frame force G J p in B.xb.f and sums this lane's body entries
+// [eo, eo + ed).  Returns the iteration count.
 template <int NG>
-__device__ __forceinline__ int pcg(BodyLDS& B, const CRow* rw, const float (*G)[6], const BodyReg& br, bool own,
-                                   const float* L, const double* g, double gscale, float eta2, int max_cg, float* x) {
+__device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const CRow* rw, const float (*G)[6],
+                                   const BodyReg& br, bool own, int eo, int ed, const float* L, const double* g,
+                                   double gscale, float eta2, int max_cg, float* x) {
   const int l = lane_id();
   float r[6], z[6], pp[6];
   float rz = 0.f;
@@ -667,21 +762,30 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const CRow* rw, const float (*G)[
   for (; it < max_cg && rz > tol2 && rz > 0.f; ++it) {
     if (own) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) { B.v[q][l] = pp[q]; B.acc[q][l] = 0.f; }
+      for (int q = 0; q < 6; ++q) B.v[q][l] = pp[q];
     }
     wave_sync();
     f3 u[NG];
 #pragma unroll
     for (int q = 0; q < NG; ++q) u[q] = row_vel(B, rw[q]);
 #pragma unroll
-    for (int q = 0; q < NG; ++q) row_scatter(B, rw[q], sym3_mul(G[q], u[q]));
+    for (int q = 0; q < NG; ++q) {
+      const int c = l + q * NA_WAVE;
+      if (c < nc) {
+        const f3 s3 = sym3_mul(G[q], u[q]);
+        const f3 fw = rw[q].t1 * s3.x + rw[q].t2 * s3.y + rw[q].n * s3.z;
+        *reinterpret_cast<float4*>(B.xb.f[c]) = make_float4(fw.x, fw.y, fw.z, 0.f);
+      }
+    }
     wave_sync();
     float Hp[6];
     float pHp = 0.f;
     if (own) {
+      float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      sum_entries_f(B, X, eo, ed, acc);
       applyA<float>(br, pp, Hp);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) { Hp[q] += B.acc[q][l]; pHp += pp[q] * Hp[q]; }
+      for (int q = 0; q < 6; ++q) { Hp[q] += acc[q]; pHp += pp[q] * Hp[q]; }
     } else {
 #pragma unroll
       for (int q = 0; q < 6; ++q) Hp[q] = 0.f;
@@ -738,27 +842,26 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
 #pragma unroll
   for (int g = 0; g < NG; ++g) rows[g] = load_row(C, X, l + g * NA_WAVE, l + g * NA_WAVE < nc);
   bool gather = true;  // the first evaluation gathers J v from the guess
+  // this lane's body entries (build_entries)
+  const int eo = own ? (int)X.eoff[l] : 0, ed = own ? (int)X.edeg[l] : 0;
   CConst kc[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) kc[g] = load_const(C, X, l + g * NA_WAVE);
   double gn_first = 0.0;
   for (int it = 0; it <= max_iterations; ++it) {
-    // gradient g = A (v - v*) - J^T gamma, FP64.  The owner lane also clears its
-    // accumulators for this iteration (J^T gamma, preconditioner blocks).
+    // gradient g = A (v - v*) - J^T gamma, FP64
     NA_TIC(t0);
-    if (own) {
+    if (own && gather) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        if (gather) B.vd[k][l] = v[k];
-        B.accd[k][l] = 0.0;
-      }
+      for (int k = 0; k < 6; ++k) B.vd[k][l] = v[k];
     }
-    if (l == 0 && gather) {  // the static side's zero velocity (the slot is overwritten by the blocks)
+    if (l == 0 && gather) {  // the static side's zero velocity
 #pragma unroll
       for (int k = 0; k < 6; ++k) B.vd[k][NA_DUMMY] = 0.0;
     }
     wave_sync();
-    double ccost = eval_contacts_g<NG, LAW>(nc, B, rows, kc, Gr, vc0, gather);
+    double jtg[6];
+    double ccost = eval_contacts_g<NG, LAW>(nc, B, X, rows, kc, Gr, vc0, gather, eo, ed, jtg);
     double g[6], d[6], Ad[6];
     double mom = 0.0, gn = 0.0, pn = 0.0, jn = 0.0;
     if (own) {
@@ -769,7 +872,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
       applyA<double>(br, v, p);
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        double jc = B.accd[k][l];
+        double jc = jtg[k];
         g[k] = Ad[k] - jc;
         mom += 0.5 * d[k] * Ad[k];
         double s = br.D[k];
@@ -801,21 +904,16 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     if (it == max_iterations) break;
     if (it == 0) gn_first = gn;
 
-    // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32); its accumulator shares
-    // storage with vd / accd, so every lane must have read its J^T gamma first
+    // block-Jacobi preconditioner P_b = A_b + sum J_b^T G J_b (f32); the blocks reuse the
+    // exchange buffer of J^T gamma, so every lane must have read its entries first
     NA_TIC(t1);
     wave_sync();
-    if (own) {
-#pragma unroll
-      for (int k = 0; k < 21; ++k) B.pre[k][l] = 0.f;
-    }
-    wave_sync();
-    precond_scatter_g<NG>(B, rows, Gr);
     float L[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) L[k] = 0.f;
+    precond_blocks_g<NG>(nc, B, X, rows, Gr, eo, ed, L);
     bool spd = true;
     if (own) {
-#pragma unroll
-      for (int k = 0; k < 21; ++k) L[k] = B.pre[k][l];
       L[0] += br.m; L[2] += br.m; L[5] += br.m;
       // packed indices of rows/cols 3..5: (3,3)=9 (4,3)=13 (4,4)=14 (5,3)=18 (5,4)=19 (5,5)=20
       L[9] += br.I[0]; L[14] += br.I[1]; L[20] += br.I[2];
@@ -840,7 +938,7 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
     const double gscale = gn > 0.0 ? 1.0 / gn : 1.0;
     float x[6];
     const float eta2 = (float)(eta * eta);
-    st.cg_iterations += pcg<NG>(B, rows, Gr, br, own, L, g, gscale, eta2, max_cg, x);
+    st.cg_iterations += pcg<NG>(B, X, nc, rows, Gr, br, own, eo, ed, L, g, gscale, eta2, max_cg, x);
     double dx[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) dx[q] = own ? (double)x[q] / gscale : 0.0;
@@ -849,7 +947,6 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
 
     // exact line search along dx in FP64:  l(a) = mom0 + a b0 + a^2 aa / 2 + contacts(a).
     // One pass computes the contact-space direction and the contact terms at a = 0 and 1.
-    wave_sync();  // vd shares storage with the preconditioner blocks read above
     if (own) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) B.vd[q][l] = dx[q];

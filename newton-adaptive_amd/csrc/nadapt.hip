@@ -405,6 +405,7 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
   wave_sync();
   if (mj) contact_setup_mj(nc, (float)h, cfg, M.mj_zero_margins, S.con, S.mj, S.body, S.sol);
   else contact_setup(nc, (float)h, cfg, S.con, S.body, S.sol);
+  build_entries(wc.nb, nc, S.con, S.sol);
   NA_TOC(wk[7], t0);
   double v[6];
 #pragma unroll

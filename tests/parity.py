@@ -15,6 +15,12 @@ in two ways:
   contact law a row is active only at pos < 0, so a world with a contact within 1e-6 m of that
   threshold (where float32 and FP64 positions can fall on different sides) is compared on the
   device's contacts only as well (reason "activity").
+
+End to end, each side's collision rounds contact positions differently (float32 against FP64),
+and a settled pile can amplify that into the error norm and pose.  A world above the end-to-end
+bar is therefore held to min(max(bar, 10 s), 1e-5) instead, where s is how far the oracle's own
+end-to-end result moves when its input pose is perturbed at float32 rounding (6e-8 relative);
+those worlds and their s are printed.
 """
 
 
@@ -87,9 +93,23 @@ def check_free_attempts(O, om, cfg, out, ref, q64, qd, f, dt, model_host, err_ba
     dq_r = np.abs(out["q_double"] - ref["q_double"]).max(axis=1)
     dq_w = np.array([dq_r[wb[w]:wb[w + 1]].max(initial=0.0) if same[w] else 0.0 for w in range(W)])
     print(f"end to end: max |d err| {de_r.max():.2e}, max |dq| {dq_w.max():.2e}", flush=True)
-    bad = np.flatnonzero(de_r > e2e_err_bar)
+    err_bar_w = np.full(W, e2e_err_bar)
+    q_bar_w = np.full(W, e2e_q_bar)
+    if (de_r > e2e_err_bar).any() or (dq_w > e2e_q_bar).any():
+        rng = np.random.default_rng(0)
+        qp = q64 * (1.0 + rng.normal(size=q64.shape) * 6e-8)
+        refp = O.attempt_all(om, cfg, qp, qd, f, dt)
+        s_err = np.abs(refp["err"] - ref["err"])
+        sq = np.abs(refp["q_double"] - ref["q_double"]).max(axis=1)
+        s_q = np.array([sq[wb[w]:wb[w + 1]].max(initial=0.0) for w in range(W)])
+        err_bar_w = np.minimum(np.maximum(e2e_err_bar, 10.0 * s_err), 1e-5)
+        q_bar_w = np.minimum(np.maximum(e2e_q_bar, 10.0 * s_q), max(1e-5, e2e_q_bar))
+        for w in np.flatnonzero((de_r > e2e_err_bar) | (dq_w > e2e_q_bar))[:20]:
+            print(f"world {int(w)} end to end: |d err| {de_r[w]:.2e} (oracle's own s {s_err[w]:.2e}), "
+                  f"|dq| {dq_w[w]:.2e} (s {s_q[w]:.2e})", flush=True)
+    bad = np.flatnonzero(de_r > err_bar_w)
     assert len(bad) == 0, f"worlds {bad[:10].tolist()}: |d err| up to {de_r.max():.2e} against the oracle's own attempt"
-    bad = np.flatnonzero(dq_w > e2e_q_bar)
+    bad = np.flatnonzero(dq_w > q_bar_w)
     assert len(bad) == 0, f"worlds {bad[:10].tolist()}: |dq| up to {dq_w.max():.2e} against the oracle's own attempt"
     return int((~same).sum())
 
