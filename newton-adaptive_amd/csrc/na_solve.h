@@ -257,33 +257,51 @@ __device__ void build_entries(int nb, int nc, const ContactLDS& C, SolveLDS& X) 
   const int nw = (nc + 3) >> 2;
   const int* wa = reinterpret_cast<const int*>(C.ba);
   const int* wb = reinterpret_cast<const int*>(C.bb);
+  // pass 1: this body's entry count, four words (16 contacts) per step with the loads in flight
+  // together (past nc the bytes are masked out)
   int deg = 0;
-  for (int w = 0; w < nw; ++w) {
-    const int a = wa[w], b = wb[w];
+  for (int w0 = 0; w0 < nw; w0 += 4) {
+    int a[4], b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (4 * w + j < nc) {
-        deg += ((int)(int8_t)((a >> (8 * j)) & 0xff) == me) + ((int)(int8_t)((b >> (8 * j)) & 0xff) == me);
+    for (int i = 0; i < 4; ++i) {
+      a[i] = w0 + i < nw ? wa[w0 + i] : 0x7f7f7f7f;
+      b[i] = w0 + i < nw ? wb[w0 + i] : 0x7f7f7f7f;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool in = 4 * (w0 + i) + j < nc;
+        deg += (in && (int)(int8_t)((a[i] >> (8 * j)) & 0xff) == me) + (in && (int)(int8_t)((b[i] >> (8 * j)) & 0xff) == me);
       }
     }
   }
   const int off = wave_incl_scan(deg) - deg;
+  // pass 2: the entries in contact order, with the side's lever arm and sign
   int k = off;
-  for (int w = 0; w < nw; ++w) {
-    const int a = wa[w], b = wb[w];
+  for (int w0 = 0; w0 < nw && k < off + deg; w0 += 4) {
+    int a[4], b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = 4 * w + j;
-      if (c < nc) {
-        if ((int)(int8_t)((a >> (8 * j)) & 0xff) == me) {
-          X.ent[k] = (uint8_t)c;
-          X.er[k][0] = X.ra[0][c]; X.er[k][1] = X.ra[1][c]; X.er[k][2] = X.ra[2][c]; X.er[k][3] = -1.f;
-          ++k;
-        }
-        if ((int)(int8_t)((b >> (8 * j)) & 0xff) == me) {
-          X.ent[k] = (uint8_t)c;
-          X.er[k][0] = X.rb[0][c]; X.er[k][1] = X.rb[1][c]; X.er[k][2] = X.rb[2][c]; X.er[k][3] = 1.f;
-          ++k;
+    for (int i = 0; i < 4; ++i) {
+      a[i] = w0 + i < nw ? wa[w0 + i] : 0x7f7f7f7f;
+      b[i] = w0 + i < nw ? wb[w0 + i] : 0x7f7f7f7f;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 4 * (w0 + i) + j;
+        if (c < nc) {
+          if ((int)(int8_t)((a[i] >> (8 * j)) & 0xff) == me) {
+            X.ent[k] = (uint8_t)c;
+            *reinterpret_cast<float4*>(X.er[k]) = make_float4(X.ra[0][c], X.ra[1][c], X.ra[2][c], -1.f);
+            ++k;
+          }
+          if ((int)(int8_t)((b[i] >> (8 * j)) & 0xff) == me) {
+            X.ent[k] = (uint8_t)c;
+            *reinterpret_cast<float4*>(X.er[k]) = make_float4(X.rb[0][c], X.rb[1][c], X.rb[2][c], 1.f);
+            ++k;
+          }
         }
       }
     }
