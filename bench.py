@@ -64,7 +64,7 @@ def algorithmic_bytes(nb, ns, work, adaptive=True):
     return float(b.sum())
 
 
-def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, samples, frames, threads, log):
+def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, samples, frames, threads, log, warm_start="extrapolate"):
     """Time the FP32 CPU restatement of the path (FP32 geometry, contact rows, Hessian and dense
     factor; FP64 Newton residual, cost, projection and line search, as on the device; OpenMP over
     worlds) from the same settled state as the timed GPU window, on `threads` threads over the
@@ -81,7 +81,7 @@ def cpu_baseline(model_fn, q0, qd0, tol, dt_outer, samples, frames, threads, log
             om = O.OracleModel(m)
             cfg = O.make_config(tol=tol, dt_init=KW["dt_inner_init"], dt_max=dt_outer,
                                 max_substeps=KW["max_substeps"], max_iterations=100,
-                                opt_rel_tol=max(1e-3 * tol, 1e-8))
+                                opt_rel_tol=max(1e-3 * tol, 1e-8), warm_start=warm_start)
             s = O.OracleSolver(om, cfg)
             B = m.body_count
             q = np.ascontiguousarray(q0[:B].astype(np.float64))
@@ -388,6 +388,8 @@ def main():
     ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--no-mujoco", action="store_true", help="skip the MuJoCo inner-step line")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule (4 waves per CU)")
+    ap.add_argument("--warm-start", choices=("extrapolate", "reference"), default="extrapolate",
+                    help="Newton seeds of the attempts' solves (the reference's, or extrapolated)")
     ap.add_argument("--articulated-only", action="store_true",
                     help="run only the articulated workload (profiling of k_art_frame); prints its JSON object")
     ap.add_argument("--art-worlds", type=int, default=8192)
@@ -405,7 +407,7 @@ def main():
 
     na = importlib.import_module("newton-adaptive_amd")
     dev = "cuda:0"
-    kw = dict(KW, tol=args.tol, dt_inner_max=args.dt_outer)
+    kw = dict(KW, tol=args.tol, dt_inner_max=args.dt_outer, warm_start=args.warm_start)
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
@@ -513,7 +515,7 @@ def main():
         "config": {"workload": f"uneven_pile_{W}: {W} worlds x U{{1..32}} sphere/box/capsule in a 1 m^2 bin, "
                                f"tol {args.tol:g}, {args.steps} frames of {args.dt_outer * 1e3:g} ms after "
                                f"{SETTLE_FRAMES} settling frames",
-                   "settle_frames": SETTLE_FRAMES,
+                   "settle_frames": SETTLE_FRAMES, "warm_start": args.warm_start,
                    "worlds": W, "tol": args.tol, "dt_outer": args.dt_outer, "seed": args.seed},
         "served": {"form": "persistent rollout launch (na_serve), no barrier between worlds or frames",
                    "accepted": accepted, "wall_s": wall, "finite": finite, "attempts": int(wk[0]),
@@ -619,7 +621,8 @@ def main():
             return na.scenes.uneven_pile(n, seed=args.seed, device="cpu")
 
         out["cpu_baseline"] = cpu_baseline(cpu_model, q0.cpu().numpy(), qd0.cpu().numpy(), args.tol, args.dt_outer,
-                                           (args.cpu_worlds, args.cpu_worlds_1t), args.cpu_frames, threads, log)
+                                           (args.cpu_worlds, args.cpu_worlds_1t), args.cpu_frames, threads, log,
+                                           warm_start=args.warm_start)
         # the same CPU leg checks the device's attempts against the FP64 restatement
         out["precision"] = precision(na, na.scenes.uneven_pile(args.precision_worlds, seed=args.seed, device=dev),
                                      q0.cpu().numpy(), qd0.cpu().numpy(), dt0, args.tol, args.precision_worlds, log)

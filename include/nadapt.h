@@ -117,6 +117,13 @@ typedef struct na_config {
                               pyramidal cone (SolverMuJoCoAdaptive's inner step, mujoco_warp.step,
                               solver_mujoco_adaptive.py:585-605): needs the model's mj_* arrays; free bodies */
   float impratio;          /* MuJoCo frictional-to-normal impedance ratio (> 0; contact_law 1) */
+  int32_t warm_start;      /* Newton seeds of an attempt's three solves.  0: the reference's
+                              (solver_sap_adaptive.py:756-771): full step from v_t, first half step
+                              from (v_t + v_full) / 2, second from v_full.  1: extrapolated: full step
+                              from v_t + dt/2 a (a: the body's mean acceleration over its last
+                              committed step, kept in body_trend), first half step as 0, second from
+                              2 v_mid - v_t.  Seeds move where Newton starts, not what it converges to.
+                              Free-body worlds; articulated worlds use 0's seeds. */
 } na_config;
 
 /* Control inputs of a boundary call (newton Control, newton/_src/sim/control.py:31-110; applied by
@@ -156,6 +163,8 @@ typedef struct na_telemetry {
                                   then cycles of collision, setup, residual, preconditioner, PCG,
                                   line search (zero unless built with -DNA_PHASE_TIMERS) */
   double* body_q64;            /* [B,7] FP64 pose accumulator */
+  float* body_trend;           /* [B,6] mean acceleration of each body over its last committed step
+                                  (warm_start 1; may be NULL otherwise); zero = no history */
 } na_telemetry;
 
 /* Contacts SoA output (newton/_src/sim/contacts.py:137-345). */
@@ -268,7 +277,8 @@ int na_sort_contacts(const na_contacts_desc* c, int32_t shape_count, int32_t* ti
                      const int32_t* match_remap, int32_t match_remap_len, int64_t* keys_out, int32_t* rank_out,
                      void* workspace, int64_t workspace_bytes, void* stream);
 
-/* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds). */
+/* Masked controller reset (world_mask: device uint8 [W], NULL = all worlds); also clears the
+   reset worlds' body_trend. */
 int na_reset(na_solver* s, const uint8_t* world_mask, void* stream);
 
 /* Contact generation into the Contacts arrays, world-major deterministic order. */
@@ -307,10 +317,12 @@ int na_substep(const na_solver* s, const double* q64, const float* qd, const flo
  * contact sets the attempt solved with, at q and at the midpoint state, per contact (shape a,
  * shape b, body a, body b (global, -1 static), manifold slot, unit normal A->B, world midpoint,
  * distance, phi0 = distance - margins, stiffness, friction, 0) -- for parity checks that give a
- * reference solve the same contacts. */
+ * reference solve the same contacts.  trend [B,6] (may be NULL = zero): the bodies' accelerations
+ * for the full step's seed under warm_start 1 (ignored under 0). */
 int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
                double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
-               int32_t* n_contacts, float* contacts_out, int32_t* contact_counts_out, void* stream);
+               int32_t* n_contacts, float* contacts_out, int32_t* contact_counts_out, const float* trend,
+               void* stream);
 
 /* Articulated models (model created with joint arrays).  The boundary call on joint coordinates:
    every world advances by dt_outer with step doubling over its joint_q / joint_qd (error =

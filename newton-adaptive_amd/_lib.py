@@ -54,7 +54,7 @@ class NaConfig(C.Structure):
         ("optimality_rel_tol", C.c_float), ("optimality_abs_tol", C.c_float), ("contact_tau_d", C.c_float),
         ("sigma", C.c_float), ("beta", C.c_float), ("ls_rel_tol", C.c_float), ("cg_rel_tol", C.c_float),
         ("max_cg", i32), ("cg_forcing", C.c_float), ("line_search", i32), ("contact_law", i32),
-        ("impratio", C.c_float),
+        ("impratio", C.c_float), ("warm_start", i32),
     ]
 
 
@@ -66,7 +66,7 @@ class NaTelemetry(C.Structure):
     _fields_ = [(n, vp) for n in (
         "dt", "dt_half", "ideal_dt", "sim_time", "next_time", "last_raw_error", "last_error", "accepted",
         "commit", "diverged", "substeps", "iterations", "solve_fail", "overflow", "cum_accepted", "work",
-        "body_q64")]
+        "body_q64", "body_trend")]
 
 
 class NaContactsDesc(C.Structure):
@@ -107,7 +107,7 @@ SIGNATURES = {
     "na_collide_noalloc": (C.c_int, [vp, vp, C.POINTER(NaContactsDesc), i32, i32, vp]),
     "na_collide_debug": (C.c_int, [vp, vp, vp, vp, vp, vp, i32, i32, vp]),
     "na_substep": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
-    "na_attempt": (C.c_int, [vp] * 15),
+    "na_attempt": (C.c_int, [vp] * 16),
     "na_ctrl_calc_adjusted_step_f32": (C.c_int, [i32, vp, vp, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, vp]),
     "na_ctrl_adapt_dt_f32": (C.c_int, [i32] + [vp] * 11 + [i32, C.c_float, C.c_float, C.c_float, C.c_float, vp]),
     "na_ctrl_rebase_time_f32": (C.c_int, [i32, vp, vp, vp]),

@@ -27,6 +27,7 @@ struct NaConfig {
   int line_search;   // 0 exact (root of l'(alpha), cubic seed), 1 armijo_decay (backtracking)
   int contact_law;   // 0 SAP compliant cone, 1 MuJoCo soft constraints (pyramidal cone)
   float impratio;    // MuJoCo frictional-to-normal impedance ratio (contact law 1)
+  int warm_start;    // 0 the reference's Newton seeds; 1 extrapolated seeds (see world_attempt)
 };
 
 // Body slot NA_DUMMY stands for the static side of a contact: its velocity entries stay zero

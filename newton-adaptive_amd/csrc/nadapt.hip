@@ -51,6 +51,7 @@ struct Ctrl {
   int32_t *substeps, *iterations, *solve_fail, *overflow;
   unsigned long long* cum;
   long long* work;  // [W, NA_WORK]
+  float* trend;     // [B, 6] each body's mean acceleration over its last committed step (warm_start 1)
 };
 // per-world work counters: attempts, Newton iterations, CG iterations, contacts, pairs, solves,
 // then shader-clock cycles of collision, solve setup, residual evaluation, preconditioner, PCG,
@@ -434,10 +435,16 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
 // (1e10 sentinel for NaN/inf), `ok` false when a convex solve did not converge.  DUMP builds
 // (parity entry point only) also write the contact sets of both collision passes to
 // dump[2][NA_MAX_CONTACTS][NA_DUMP_FLOATS] and their counts to dump_n[2].
+// Newton seeds of the three solves.  warm_start 0 is the reference's (solver_sap_adaptive.py:
+// 756-771): the full step from v_t, the first half step from (v_t + v_full) / 2, the second from
+// v_full.  warm_start 1 extrapolates: the full step from v_t + dt/2 a, with a = tr (this lane's
+// mean acceleration over the last committed step; v_t when tr is null), the first half step from
+// (v_t + v_full) / 2, the second from 2 v_mid - v_t (v_mid: the first half step's velocity).  The
+// seeds change where Newton starts, not the problem each solve converges on.
 template <bool DUMP = false>
 __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S,
-                                const double* q, const float* v, const float* f, double dt, double* qf, float* vf,
-                                double* qd, float* vd, bool& ok, int& overflow, int& nc_out, long long* wk,
+                                const double* q, const float* v, const float* f, const float* tr, double dt, double* qf,
+                                float* vf, double* qd, float* vd, bool& ok, int& overflow, int& nc_out, long long* wk,
                                 float* dump = nullptr, int32_t* dump_n = nullptr) {
   const int l = lane_id();
   const bool own = l < wc.nb;
@@ -447,7 +454,11 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
     dump_contacts(S.seg, S.con, wc.b0, nc, dump);
     if (l == 0) dump_n[0] = nc;
   }
-  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, v, qf, vf, wk);
+  float g0[6];
+  const float hh = (float)(0.5 * dt);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) g0[k] = tr ? v[k] + hh * tr[k] : v[k];
+  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
   double err = 0.0;
   if (cfg.mode == 1) {
     float g1[6];
@@ -461,7 +472,10 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
       dump_contacts(S.seg, S.con, wc.b0, nc2, dump + NA_MAX_CONTACTS * NA_DUMP_FLOATS);
       if (l == 0) dump_n[1] = nc2;
     }
-    ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, vf, qd, vd, wk) && ok;
+    float g2[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) g2[k] = cfg.warm_start ? 2.f * vm[k] - v[k] : vf[k];
+    ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
     double e = 0.0;
     if (own) {
 #pragma unroll
@@ -580,10 +594,14 @@ __device__ __forceinline__ bool controller_update(const NaConfig& cfg, WCtrl& c,
 }
 
 // One iteration of the boundary loop for one world (clamp, attempt, controller, commit).
-// q/v hold this lane's body state and are updated in place.  Returns accepted-step count (0/1).
+// q/v hold this lane's body state and are updated in place.  trend (warm_start 1, else null) is
+// the per-body acceleration array: read for the full step's Newton seed and, after an accepted
+// step, set to (v_new - v_old) / step.  Returns accepted-step count (0/1).
 __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, WCtrl& c,
-                               double* q, float* v, const float* f, double eff_max, long long* wk) {
+                               double* q, float* v, const float* f, float* trend, double eff_max, long long* wk) {
   c.iterations += 1;
+  const int l = lane_id();
+  const bool own = l < wc.nb;
   double rem = c.next_time - c.sim_time;
   if (rem <= 0.0) { c.dt = 0.0; c.dt_half = 0.0; }
   else if (c.dt > rem) { c.dt = rem; c.dt_half = 0.5 * rem; }
@@ -594,7 +612,10 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
   int ovf = 0, nc = 0;
   double err = 0.0;
   if (step > 0.0 && wc.nb > 0) {
-    err = world_attempt(M, cfg, wc, S, q, v, f, step, qf, vf, qd, vd, ok, ovf, nc, wk);
+    float tr[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) tr[k] = trend && own ? trend[6 * (wc.b0 + l) + k] : 0.f;
+    err = world_attempt(M, cfg, wc, S, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf, nc, wk);
   } else {
 #pragma unroll
     for (int k = 0; k < 7; ++k) { qf[k] = q[k]; qd[k] = q[k]; }
@@ -607,6 +628,14 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
   if (commit) {
     const double* sq = cfg.mode == 0 ? qf : qd;
     const float* sv = cfg.mode == 0 ? vf : vd;
+    if (trend && inc && own) {
+      const float inv = (float)(1.0 / step);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const float a = (sv[k] - v[k]) * inv;
+        trend[6 * (wc.b0 + l) + k] = isfinite(a) ? a : 0.f;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < 7; ++k) q[k] = sq[k];
 #pragma unroll
@@ -689,7 +718,7 @@ __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, d
     int acc = 0;
     for (int it = 0; it < cfg.max_substeps; ++it) {
       if (!(c.sim_time < c.next_time)) break;
-      acc += world_iteration(M, cfg, wc, S, c, q, v, f, eff_max, wk);
+      acc += world_iteration(M, cfg, wc, S, c, q, v, f, K.trend, eff_max, wk);
       if (c.solve_fail) break;
     }
     store_body(wc, q64, body_q, body_qd, q, v);
@@ -753,7 +782,7 @@ __device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, 
       if (frames >= hf) { status = 1; break; }
       open_frame(wcfg, c, dt_outer, eff_max);
     }
-    int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, eff_max, wk);
+    int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, K.trend, eff_max, wk);
     acc += inc;
     accepted += inc;
     if (c.solve_fail) { status = 2; break; }
@@ -846,7 +875,7 @@ __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl
     for (int k = 0; k < 6; ++k) { v[k] = 0.f; f[k] = 0.f; }
   }
   long long wk[NA_WORK] = {};
-  int acc = world_iteration(M, cfg, wc, S, c, q, v, f, eff_max, wk);
+  int acc = world_iteration(M, cfg, wc, S, c, q, v, f, K.trend, eff_max, wk);
   if (l < wc.nb) {
     int gb = wc.b0 + l;
 #pragma unroll
@@ -864,9 +893,11 @@ __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl
 }
 
 // ---------------------------------------------------------------- reset
-__global__ void k_reset(int W, const uint8_t* mask, double dt_init, Ctrl K) {
+__global__ void k_reset(NaModel M, const uint8_t* mask, double dt_init, Ctrl K) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W || (mask && !mask[i])) return;
+  if (i >= M.W || (mask && !mask[i])) return;
+  if (K.trend)  // a reset world has no step history
+    for (int j = 6 * M.world_body_start[i]; j < 6 * M.world_body_end[i]; ++j) K.trend[j] = 0.f;
   K.ideal_dt[i] = dt_init;
   K.dt[i] = dt_init;
   K.dt_half[i] = dt_init * 0.5;
@@ -1157,7 +1188,8 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
 __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const double* q64, const float* qd,
                                                 const float* f, const double* dt, double* q_full, float* qd_full,
                                                 double* q_dbl, float* qd_dbl, double* err, int32_t* solve_ok,
-                                                int32_t* n_contacts, float* cdump, int32_t* cdump_n) {
+                                                int32_t* n_contacts, float* cdump, int32_t* cdump_n,
+                                                const float* trend) {
   __shared__ WorldLDS S;
   int w = blockIdx.x;
   WorldCtx wc = world_ctx(M, w);
@@ -1180,12 +1212,16 @@ __global__ __launch_bounds__(64) void k_attempt(NaModel M, NaConfig cfg, const d
   bool ok = true;
   int ovf = 0, nc = 0;
   double e;
+  float tr[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) tr[k] = trend && l < wc.nb ? trend[6 * (wc.b0 + l) + k] : 0.f;
+  const float* trp = trend ? tr : nullptr;
   if (cdump) {
     if (l == 0) cdump_n[2 * w + 1] = 0;  // fixed mode has no midpoint pass
-    e = world_attempt<true>(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk,
+    e = world_attempt<true>(M, cfg, wc, S, q, v, ff, trp, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk,
                             cdump + (long long)w * 2 * NA_MAX_CONTACTS * NA_DUMP_FLOATS, cdump_n + 2 * w);
   } else {
-    e = world_attempt(M, cfg, wc, S, q, v, ff, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);
+    e = world_attempt(M, cfg, wc, S, q, v, ff, trp, dt[w], qf, vf, qd2, vd, ok, ovf, nc, wk);
   }
   if (l < wc.nb) {
     int gb = wc.b0 + l;
@@ -1675,6 +1711,7 @@ static NaConfig to_cfg(const na_config* c) {
   k.line_search = c->line_search;
   k.contact_law = c->contact_law;
   k.impratio = c->impratio;
+  k.warm_start = c->warm_start;
   return k;
 }
 
@@ -1702,6 +1739,10 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
       return fail("na_solver_create: contact_law 1 (MuJoCo soft constraints) is built for free-body models only");
     if (!(c->impratio > 0)) return fail("na_solver_create: impratio must be > 0");
   }
+  if (c->warm_start != 0 && c->warm_start != 1)
+    return fail("na_solver_create: warm_start must be 0 (reference seeds) or 1 (extrapolated seeds)");
+  if (c->warm_start == 1 && buffers && !buffers->body_trend)
+    return fail("na_solver_create: warm_start 1 with supplied buffers needs body_trend");
   na_solver* s = new na_solver();
   s->model = m;
   s->cfg = to_cfg(c);
@@ -1717,7 +1758,8 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   size_t need = 0;
   size_t sizes_own[] = {8u * W, 8u * W, 8u * W, 8u * W, 8u * W, 8u * W, 8u * W, (size_t)W, (size_t)W, (size_t)W,
                         4u * W, 4u * W, 4u * W, 4u * W, 8u, sizeof(long long) * NA_WORK * W,
-                        sizeof(double) * 7 * (size_t)(B > 0 ? B : 1), 4u, 4u};
+                        sizeof(double) * 7 * (size_t)(B > 0 ? B : 1), 4u, 4u,
+                        sizeof(float) * 6 * (size_t)(B > 0 ? B : 1)};
   size_t sizes_ext[] = {4u, 4u};
   if (buffers) for (size_t z : sizes_ext) need += (z + 255) & ~(size_t)255;
   else for (size_t z : sizes_own) need += (z + 255) & ~(size_t)255;
@@ -1739,6 +1781,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     K.accepted = b->accepted; K.commit = b->commit; K.diverged = b->diverged; K.substeps = b->substeps;
     K.iterations = b->iterations; K.solve_fail = b->solve_fail; K.overflow = b->overflow;
     K.cum = b->cum_accepted; K.work = b->work; s->q64 = b->body_q64;
+    K.trend = c->warm_start == 1 ? b->body_trend : nullptr;
   } else {
     K.dt = (double*)take(8u * W); K.dt_half = (double*)take(8u * W); K.ideal_dt = (double*)take(8u * W);
     K.sim_time = (double*)take(8u * W); K.next_time = (double*)take(8u * W); K.last_error = (double*)take(8u * W);
@@ -1752,6 +1795,10 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   }
   s->queue = (int32_t*)take(4);
   s->flag = (int32_t*)take(4);
+  if (!buffers) {
+    float* tr = (float*)take(sizeof(float) * 6 * (size_t)(B > 0 ? B : 1));
+    K.trend = c->warm_start == 1 ? tr : nullptr;
+  }
   if (hipHostMalloc(&s->flag_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
     (void)hipFree(pool);
     delete s;
@@ -1760,7 +1807,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   s->order = nullptr;
   s->grid_waves = 0;
   // controller defaults (reset of all worlds)
-  k_reset<<<blocks(W, 256), 256>>>(W, nullptr, s->cfg.dt_init, K);
+  k_reset<<<blocks(W, 256), 256>>>(m->M, nullptr, s->cfg.dt_init, K);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   *out = s;
@@ -1780,7 +1827,7 @@ extern "C" int na_solver_telemetry(const na_solver* s, na_telemetry* t) {
   t->next_time = K.next_time; t->last_raw_error = K.last_error; t->last_error = K.acc_error;
   t->accepted = K.accepted; t->commit = K.commit; t->diverged = K.diverged; t->substeps = K.substeps;
   t->iterations = K.iterations; t->solve_fail = K.solve_fail; t->overflow = K.overflow;
-  t->cum_accepted = K.cum; t->work = K.work; t->body_q64 = s->q64;
+  t->cum_accepted = K.cum; t->work = K.work; t->body_q64 = s->q64; t->body_trend = K.trend;
   return 0;
 }
 extern "C" int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order) {
@@ -1906,7 +1953,7 @@ extern "C" int na_coords_from_mujoco(const na_model* m, const float* qpos, const
 extern "C" int na_reset(na_solver* s, const uint8_t* mask, void* stream) {
   if (!s) return fail("na_reset: null solver");
   int W = s->model->M.W;
-  k_reset<<<blocks(W, 256), 256, 0, S_(stream)>>>(W, mask, s->cfg.dt_init, s->K);
+  k_reset<<<blocks(W, 256), 256, 0, S_(stream)>>>(s->model->M, mask, s->cfg.dt_init, s->K);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2036,14 +2083,15 @@ extern "C" int na_substep(const na_solver* s, const double* q64, const float* qd
 extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd, const float* f, const double* dt,
                           double* q_full, float* qd_full, double* q_dbl, float* qd_dbl, double* err,
                           int32_t* solve_ok, int32_t* n_contacts, float* contacts_out, int32_t* contact_counts_out,
-                          void* stream) {
+                          const float* trend, void* stream) {
   if (!s || !q64 || !qd || !dt || !q_full || !qd_full || !q_dbl || !qd_dbl || !err || !solve_ok || !n_contacts)
     return fail("na_attempt: null argument");
   if ((contacts_out == nullptr) != (contact_counts_out == nullptr))
     return fail("na_attempt: contacts_out and contact_counts_out go together");
   const NaModel& M = s->model->M;
   k_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, dt, q_full, qd_full, q_dbl, qd_dbl, err, solve_ok,
-                                        n_contacts, contacts_out, contact_counts_out);
+                                        n_contacts, contacts_out, contact_counts_out,
+                                        s->cfg.warm_start == 1 ? trend : nullptr);
   HIPCHK(hipGetLastError());
   return 0;
 }

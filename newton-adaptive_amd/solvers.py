@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import weakref
 
 import torch
 
@@ -27,6 +28,11 @@ from .native import model_handle
 
 _MODE_CODES = {"fixed": 0, "adaptive": 1}
 _LINE_SEARCHES = {"exact_root": 0, "armijo_decay": 1}
+# Newton seeds of an attempt's three solves: "reference" as solver_sap_adaptive.py:756-771 (v_t,
+# (v_t + v_full) / 2, v_full); "extrapolate": v_t + dt/2 a (a: the body's mean acceleration over
+# its last committed step), (v_t + v_full) / 2, 2 v_mid - v_t.  Same converged solves, fewer
+# Newton iterations on settled piles.
+WARM_STARTS = {"reference": 0, "extrapolate": 1}
 _CONTROL_FIELDS = ("joint_f", "joint_target_q", "joint_target_qd", "joint_act")
 _WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves",
                 "cycles_collide", "cycles_setup", "cycles_residual", "cycles_preconditioner", "cycles_pcg",
@@ -40,8 +46,8 @@ class _SolverBase:
                  dt_inner_max=None, max_substeps=16, max_rigid_contact=128, max_iterations=100,
                  contact_preset_variant="drake", line_search_variant="exact_root", contact_tau_d=0.01,
                  broad_phase="sap", optimality_rel_tol=None, schedule="served", cg_rel_tol=1.0e-4,
-                 cg_forcing=0.1, world_order="cost", ls_rel_tol=1.0e-4, _contact_law=0, _impratio=1.0,
-                 **_kw):
+                 cg_forcing=0.1, world_order="cost", ls_rel_tol=1.0e-4, warm_start="extrapolate", _contact_law=0,
+                 _impratio=1.0, **_kw):
         if mode not in _MODE_CODES:
             raise ValueError(f"mode must be one of {tuple(_MODE_CODES)}, got {mode!r}.")
         if float(tol) <= 0.0:
@@ -71,6 +77,8 @@ class _SolverBase:
             raise ValueError(f"ls_rel_tol must be in (0, 1), got {ls_rel_tol!r}.")
         if world_order not in ("cost", "index"):
             raise ValueError(f"world_order must be 'cost' or 'index', got {world_order!r}.")
+        if warm_start not in WARM_STARTS:
+            raise ValueError(f"warm_start must be one of {tuple(WARM_STARTS)}, got {warm_start!r}.")
         if schedule not in ("served", "plain"):
             raise ValueError(f"schedule must be 'served' or 'plain', got {schedule!r}.")
         self.model = model
@@ -113,6 +121,8 @@ class _SolverBase:
         cfg.line_search = _LINE_SEARCHES[line_search_variant]
         cfg.contact_law = int(_contact_law)
         cfg.impratio = float(_impratio)
+        cfg.warm_start = WARM_STARTS[warm_start]
+        self.warm_start = warm_start
         self.line_search_variant = line_search_variant
         self._cfg = cfg
 
@@ -134,6 +144,10 @@ class _SolverBase:
         self._cum_accepted = torch.zeros(1, dtype=torch.int64, device=dev)
         self._work = torch.zeros((W, len(_WORK_FIELDS)), dtype=torch.int64, device=dev)
         self._q64 = torch.zeros((max(B, 1), 7), **f64)
+        # each body's mean acceleration over its last committed step: the full step's Newton seed
+        # under warm_start "extrapolate" (zero: no history yet)
+        self._trend = torch.zeros((max(B, 1), 6), dtype=torch.float32, device=dev)
+        self._last_state = None
         self._full_world_mask = torch.ones(W, dtype=torch.bool, device=dev)
 
         t = _lib.NaTelemetry()
@@ -146,6 +160,7 @@ class _SolverBase:
         t.solve_fail, t.overflow = self._solve_fail.data_ptr(), self._overflow.data_ptr()
         t.cum_accepted, t.work, t.body_q64 = (self._cum_accepted.data_ptr(), self._work.data_ptr(),
                                               self._q64.data_ptr())
+        t.body_trend = self._trend.data_ptr()
         self._handle = model_handle(model, with_pairs=broad_phase == "explicit")
         h = C.c_void_p()
         _lib.check(_lib.lib().na_solver_create(self._handle.handle, C.byref(cfg), C.byref(t), C.byref(h)),
@@ -328,7 +343,12 @@ class _SolverBase:
             raise ValueError(f"{type(self).__name__}.step requires dt (the outer boundary period).")
         if apply_forces is not None:
             apply_forces(state_in)
+        last = self._last_state() if self._last_state is not None else None
+        if last is not state_in:
+            # a state this solver did not advance last: its step history does not apply
+            self._trend.zero_()
         self.integrate(state_in, control, float(dt))
+        self._last_state = weakref.ref(state_in)
         return state_in, state_out
 
     def step_dt(self, dt_outer, state_0, state_1, control, apply_forces=None):
@@ -343,7 +363,7 @@ class _SolverBase:
             if mask.numel() != self._world_count:
                 raise ValueError("world_mask must have one entry per world")
         _lib.check(_lib.lib().na_reset(self._h, mask.data_ptr() if mask is not None else None,
-                                       _lib.stream_handle()), "na_reset")
+                                       _lib.stream_handle()), "na_reset")  # also clears their trend
 
     # ------------------------------------------------------------------ telemetry
     @property
@@ -408,7 +428,7 @@ class _SolverBase:
     # ------------------------------------------------------------------ autotuner
     _TUNE_STATE = ("_dt", "_dt_half", "_ideal_dt", "_sim_time", "_next_time", "_last_error", "_accepted_error",
                    "_accepted", "_commit", "_diverged", "_substeps_frame", "_iterations", "_solve_fail", "_overflow",
-                   "_cum_accepted", "_work", "_q64")
+                   "_cum_accepted", "_work", "_q64", "_trend")
 
     def autotune(self, state, dt_outer: float, frames: int = 2, waves_per_cu=(2, 3, 4),
                  orders=("cost", "size", "index"), control=None) -> dict:
@@ -611,7 +631,7 @@ class SolverMuJoCoAdaptive(_SolverBase):
             raise ValueError("use_mujoco_cpu=True is not supported: the step runs on the device")
         allowed = {"max_rigid_contact", "max_iterations", "broad_phase", "optimality_rel_tol", "schedule",
                    "cg_rel_tol", "cg_forcing", "world_order", "ls_rel_tol", "line_search_variant",
-                   "contact_tau_d", "contact_preset_variant"}
+                   "contact_tau_d", "contact_preset_variant", "warm_start"}
         unknown = set(kwargs) - allowed
         if unknown:
             raise TypeError(f"SolverMuJoCoAdaptive got unexpected keyword arguments {sorted(unknown)}")

@@ -145,6 +145,7 @@ typedef struct or_config {
   int line_search;     /* 0 exact_root, 1 armijo_decay (solver_sap_adaptive.py:459) */
   int contact_law;     /* 0 SAP compliant cone, 1 MuJoCo soft constraints, pyramidal cone */
   double impratio;     /* MuJoCo frictional-to-normal impedance ratio (law 1) */
+  int warm_start;      /* Newton seeds of an attempt: 0 the reference's, 1 extrapolated (see attempt) */
 } or_config;
 
 enum { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
@@ -1646,6 +1647,8 @@ typedef struct {
   uint8_t *accepted, *commit, *diverged;
   int32_t *substeps, *iterations, *solve_fail;
   int64_t* cum_accepted;
+  double* trend;       /* [B*6] mean acceleration of each body over its last accepted step (warm_start 1;
+                          NULL: zero) */
 } or_ctrl;
 
 /* Contact records of the device's parity entry points (16 values per contact: shape a, shape b,
@@ -1666,10 +1669,16 @@ static void contacts_from_records(const double* rec, int n, or_contact* out) {
 
 /* Attempt of one world from (q, qd): writes full/double results, returns error (sentinel on NaN).
    given (NULL: collide) holds the contact sets to solve with at q and at the midpoint state,
-   given[0..given_n[0]) and given[cap..cap + given_n[1]) with cap = C->max_contacts. */
+   given[0..given_n[0]) and given[cap..cap + given_n[1]) with cap = C->max_contacts.
+   Newton seeds (solver_sap_adaptive.py:756-771 for warm_start 0): the full step from qd, the
+   first half step from (qd + v_full) / 2, the second half step from v_full.  warm_start 1 seeds
+   the full step at qd + dt/2 trend (trend: the world's [nb*6] mean accelerations over its last
+   accepted step, NULL = zero) and the second half step at 2 v_mid - qd; the problems solved are
+   the same. */
 static double attempt(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
-                      double dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, int* solve_ok,
-                      int* n_contacts, const or_contact* given, const int32_t* given_n, int* n_contacts_mid) {
+                      const double* trend, double dt, double* q_full, double* qd_full, double* q_dbl,
+                      double* qd_dbl, int* solve_ok, int* n_contacts, const or_contact* given,
+                      const int32_t* given_n, int* n_contacts_mid) {
   int nb = W->nb, N = 6 * nb;
   int cap = C->max_contacts;
   or_contact* cs = (or_contact*)malloc(sizeof(or_contact) * (cap > 0 ? cap : 1));
@@ -1687,7 +1696,8 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   }
   if (nc > cap) nc = cap;
   if (n_contacts) *n_contacts = nc;
-  ok &= inner_step(W, C, q, qd, f, dt, cs, nc, NULL, q_full, qd_full, vfull, &st);
+  for (int i = 0; i < N; ++i) guess[i] = C->warm_start == 1 && trend ? qd[i] + 0.5 * dt * trend[i] : qd[i];
+  ok &= inner_step(W, C, q, qd, f, dt, cs, nc, guess, q_full, qd_full, vfull, &st);
   double err = 0.0;
   if (C->mode == 1) {
     for (int i = 0; i < N; ++i) guess[i] = 0.5 * (qd[i] + vfull[i]);
@@ -1700,6 +1710,8 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
     }
     if (nc > cap) nc = cap;
     if (n_contacts_mid) *n_contacts_mid = nc;
+    if (C->warm_start == 1)
+      for (int i = 0; i < N; ++i) vfull[i] = 2.0 * qd_mid[i] - qd[i];  /* the second half step's seed */
     ok &= inner_step(W, C, q_mid, qd_mid, f, 0.5 * dt, cs, nc, vfull, q_dbl, qd_dbl, NULL, &st);
     for (int i = 0; i < 7 * nb; ++i) {
       double d = fabs(q_dbl[i] - q_full[i]);
@@ -1719,15 +1731,15 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
 
 /* Exported single attempt for parity tests: out_err[w], and full/double states of every world. */
 void or_attempt_all(const or_model* M, const or_config* C, const double* q, const double* qd, const double* f,
-                    const double* dt, double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
-                    int32_t* solve_ok, int32_t* n_contacts, int32_t* n_contacts_mid) {
+                    const double* trend, const double* dt, double* q_full, double* qd_full, double* q_dbl,
+                    double* qd_dbl, double* err, int32_t* solve_ok, int32_t* n_contacts, int32_t* n_contacts_mid) {
 #pragma omp parallel for schedule(dynamic, 1)
   for (int w = 0; w < M->world_count; ++w) {
     int b0 = M->world_body_start[w];
     world_view W = {M, w, b0, M->world_body_start[w + 1] - b0};
     int ok, nc, ncm = 0;
-    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, dt[w], q_full + 7 * b0, qd_full + 6 * b0,
-                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, NULL, NULL, &ncm);
+    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, trend ? trend + 6 * b0 : NULL, dt[w],
+                     q_full + 7 * b0, qd_full + 6 * b0, q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, NULL, NULL, &ncm);
     solve_ok[w] = ok;
     n_contacts[w] = nc;
     if (n_contacts_mid) n_contacts_mid[w] = ncm;
@@ -1739,8 +1751,9 @@ void or_attempt_all(const or_model* M, const or_config* C, const double* q, cons
    solves, integration and error norm of the attempt on the same contacts, in the layout of
    or_attempt_all. */
 void or_attempt_all_given(const or_model* M, const or_config* C, const double* q, const double* qd, const double* f,
-                          const double* dt, const double* records, const int32_t* counts, double* q_full,
-                          double* qd_full, double* q_dbl, double* qd_dbl, double* err, int32_t* solve_ok) {
+                          const double* trend, const double* dt, const double* records, const int32_t* counts,
+                          double* q_full, double* qd_full, double* q_dbl, double* qd_dbl, double* err,
+                          int32_t* solve_ok) {
   int cap = C->max_contacts > 0 ? C->max_contacts : 1;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int w = 0; w < M->world_count; ++w) {
@@ -1753,8 +1766,8 @@ void or_attempt_all_given(const or_model* M, const or_config* C, const double* q
     contacts_from_records(rw, n[0], given);
     contacts_from_records(rw + 16 * cap, n[1], given + cap);
     int ok, nc;
-    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, dt[w], q_full + 7 * b0, qd_full + 6 * b0,
-                     q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, given, n, NULL);
+    err[w] = attempt(&W, C, q + 7 * b0, qd + 6 * b0, f + 6 * b0, trend ? trend + 6 * b0 : NULL, dt[w],
+                     q_full + 7 * b0, qd_full + 6 * b0, q_dbl + 7 * b0, qd_dbl + 6 * b0, &ok, &nc, given, n, NULL);
     solve_ok[w] = ok;
     free(given);
   }
@@ -1850,7 +1863,8 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
       double step = ctrl_clamp(K, w);
       int ok = 1, nc;
       double err;
-      if (step > 0) err = attempt(&W, C, qw, qdw, fw, step, qf, qdf, qdb, qddb, &ok, &nc, NULL, NULL, NULL);
+      double* tw = K->trend && C->warm_start == 1 ? K->trend + 6 * b0 : NULL;
+      if (step > 0) err = attempt(&W, C, qw, qdw, fw, tw, step, qf, qdf, qdb, qddb, &ok, &nc, NULL, NULL, NULL);
       else { err = 0; memcpy(qdb, qw, sizeof(double) * 7 * nb); memcpy(qddb, qdw, sizeof(double) * 6 * nb);
              memcpy(qf, qw, sizeof(double) * 7 * nb); memcpy(qdf, qdw, sizeof(double) * 6 * nb); }
       if (!ok) status = 2;
@@ -1858,6 +1872,11 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
       if (commit) {
         const double* sq = C->mode == 0 ? qf : qdb;
         const double* sqd = C->mode == 0 ? qdf : qddb;
+        if (tw && K->accepted[w] && step > 0)
+          for (int i = 0; i < 6 * nb; ++i) {
+            double a = (sqd[i] - qdw[i]) / step;
+            tw[i] = isfinite(a) ? a : 0.0;
+          }
         memcpy(qw, sq, sizeof(double) * 7 * nb);
         memcpy(qdw, sqd, sizeof(double) * 6 * nb);
       }

@@ -53,7 +53,7 @@ class OrConfig(C.Structure):
         ("max_substeps", C.c_int), ("max_iterations", C.c_int), ("max_contacts", C.c_int),
         ("opt_rel_tol", C.c_double), ("opt_abs_tol", C.c_double), ("tau_d", C.c_double), ("sigma", C.c_double),
         ("beta", C.c_double), ("ls_rel_tol", C.c_double), ("line_search", C.c_int),
-        ("contact_law", C.c_int), ("impratio", C.c_double),
+        ("contact_law", C.c_int), ("impratio", C.c_double), ("warm_start", C.c_int),
     ]
 
 
@@ -63,6 +63,7 @@ class OrCtrl(C.Structure):
         ("last_error", f64p), ("accepted_error", f64p),
         ("accepted", u8p), ("commit", u8p), ("diverged", u8p),
         ("substeps", i32p), ("iterations", i32p), ("solve_fail", i32p), ("cum_accepted", i64p),
+        ("trend", f64p),
     ]
 
 
@@ -313,7 +314,11 @@ def mj_convert_solref(ke, kd):
 def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_min=1e-6, dt_max=float("inf"),
                 max_substeps=16, max_iterations=30, max_contacts=128, opt_rel_tol=None, opt_abs_tol=1e-14,
                 tau_d=0.01, sigma=1e-3, beta=1.0, ls_rel_tol=1e-10, line_search="exact_root",
-                contact_law="sap", impratio=1.0) -> OrConfig:
+                contact_law="sap", impratio=1.0, warm_start="extrapolate") -> OrConfig:
+    """Oracle settings.  warm_start: the Newton seeds of an attempt's three solves, "reference"
+    (v_t, (v_t + v_full) / 2, v_full: solver_sap_adaptive.py:756-771) or "extrapolate" (v_t + dt/2 a
+    with a the body's mean acceleration over its last accepted step, (v_t + v_full) / 2,
+    2 v_mid - v_t)."""
     c = OrConfig()
     c.mode = 1 if mode == "adaptive" else 0
     c.controller = 0 if controller == "sap" else 1
@@ -325,6 +330,7 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     c.line_search = {"exact_root": 0, "armijo_decay": 1}[line_search]
     c.contact_law = {"sap": 0, "mujoco": 1}[contact_law]
     c.impratio = float(impratio)
+    c.warm_start = {"reference": 0, "extrapolate": 1}[warm_start]
     return c
 
 
@@ -516,15 +522,23 @@ def substep_problem(om: OracleModel, cfg: OrConfig, w: int, q, qd, f, h):
                 converged=bool(conv.value), D=D[:nc], what=what[:nc])
 
 
-def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt):
+def _trend(trend, B):
+    return None if trend is None else np.ascontiguousarray(np.asarray(trend, dtype=np.float64).reshape(B, 6))
+
+
+def attempt_all(om: OracleModel, cfg: OrConfig, q, qd, f, dt, trend=None):
+    """One step-doubling attempt of every world (no commit).  trend [B, 6] (None = zero): the
+    bodies' accelerations that seed the full step under warm_start "extrapolate"."""
     W, B = om.world_count, om.body_count
     q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
+    tr = _trend(trend, B)
     qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
     err = np.zeros(W); ok = np.zeros(W, np.int32); nc = np.zeros(W, np.int32); ncm = np.zeros(W, np.int32)
     lib().or_attempt_all(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
-                         _ptr(dt, f64p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
-                         _ptr(err, f64p), _ptr(ok, i32p), _ptr(nc, i32p), _ptr(ncm, i32p))
+                         _ptr(tr, f64p) if tr is not None else None, _ptr(dt, f64p), _ptr(qf, f64p),
+                         _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p), _ptr(err, f64p), _ptr(ok, i32p),
+                         _ptr(nc, i32p), _ptr(ncm, i32p))
     return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok, n_contacts=nc,
                 n_contacts_mid=ncm)
 
@@ -541,19 +555,21 @@ def _records(cfg, contacts, counts):
     return np.ascontiguousarray(rec), np.ascontiguousarray(n)
 
 
-def attempt_all_given(om: OracleModel, cfg: OrConfig, q, qd, f, dt, contacts, counts):
+def attempt_all_given(om: OracleModel, cfg: OrConfig, q, qd, f, dt, contacts, counts, trend=None):
     """Step-doubling attempt of every world that solves with the given contact sets (a device
     attempt's records [W, 2, 128, 16] at q and at its midpoint, counts [W, 2]) instead of colliding;
-    the layout of :func:`attempt_all`."""
+    the layout and the trend of :func:`attempt_all`."""
     W, B = om.world_count, om.body_count
     q = np.ascontiguousarray(q, dtype=np.float64); qd = np.ascontiguousarray(qd, dtype=np.float64)
     f = np.ascontiguousarray(f, dtype=np.float64); dt = np.ascontiguousarray(dt, dtype=np.float64)
+    tr = _trend(trend, B)
     rec, n = _records(cfg, contacts, counts)
     qf = np.zeros((B, 7)); qdf = np.zeros((B, 6)); qb = np.zeros((B, 7)); qdb = np.zeros((B, 6))
     err = np.zeros(W); ok = np.zeros(W, np.int32)
     lib().or_attempt_all_given(C.byref(om.desc), C.byref(cfg), _ptr(q, f64p), _ptr(qd, f64p), _ptr(f, f64p),
-                               _ptr(dt, f64p), _ptr(rec, f64p), _ptr(n, i32p), _ptr(qf, f64p), _ptr(qdf, f64p),
-                               _ptr(qb, f64p), _ptr(qdb, f64p), _ptr(err, f64p), _ptr(ok, i32p))
+                               _ptr(tr, f64p) if tr is not None else None, _ptr(dt, f64p), _ptr(rec, f64p),
+                               _ptr(n, i32p), _ptr(qf, f64p), _ptr(qdf, f64p), _ptr(qb, f64p), _ptr(qdb, f64p),
+                               _ptr(err, f64p), _ptr(ok, i32p))
     return dict(q_full=qf, qd_full=qdf, q_double=qb, qd_double=qdb, err=err, solve_ok=ok)
 
 
@@ -585,6 +601,7 @@ class OracleSolver:
             "accepted": np.zeros(W, np.uint8), "commit": np.zeros(W, np.uint8), "diverged": np.zeros(W, np.uint8),
             "substeps": np.zeros(W, np.int32), "iterations": np.zeros(W, np.int32),
             "solve_fail": np.zeros(W, np.int32), "cum_accepted": np.zeros(1, np.int64),
+            "trend": np.zeros((max(om.body_count, 1), 6)),
         }
         k = OrCtrl()
         for name, arr in self.a.items():

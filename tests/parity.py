@@ -60,16 +60,17 @@ def explain_mismatch(O, om, w, q_all, contacts, counts, model_host):
 
 
 def check_free_attempts(O, om, cfg, out, ref, q64, qd, f, dt, model_host, err_bar=1e-6, q_bar=1e-5,
-                        e2e_err_bar=None, e2e_q_bar=None):
+                        e2e_err_bar=None, e2e_q_bar=None, trend=None):
     """Free-body attempts: ``out`` from ``__graft_entry__._attempt(..., with_contacts=True)``,
-    ``ref`` from ``O.attempt_all`` on the same inputs.  Returns the number of worlds whose contact
+    ``ref`` from ``O.attempt_all`` on the same inputs (``trend``: the full step's seed
+    accelerations both were given, None = zero).  Returns the number of worlds whose contact
     counts differed (every world is checked on the device's contacts).  ``e2e_err_bar``,
     ``e2e_q_bar``: the bars against the oracle's own collision (default ``err_bar``, ``q_bar``)."""
     e2e_err_bar = err_bar if e2e_err_bar is None else e2e_err_bar
     e2e_q_bar = q_bar if e2e_q_bar is None else e2e_q_bar
     W = len(out["err"])
     wb = om.world_body_start
-    g = O.attempt_all_given(om, cfg, q64, qd, f, dt, out["contacts"], out["contact_counts"])
+    g = O.attempt_all_given(om, cfg, q64, qd, f, dt, out["contacts"], out["contact_counts"], trend=trend)
     assert g["solve_ok"].min() == 1, "an oracle solve on the device's contacts did not converge"
     de_g = np.abs(out["err"] - g["err"])
     dq_g = np.abs(out["q_double"] - g["q_double"]).max(axis=1)
@@ -98,7 +99,7 @@ def check_free_attempts(O, om, cfg, out, ref, q64, qd, f, dt, model_host, err_ba
     if (de_r > e2e_err_bar).any() or (dq_w > e2e_q_bar).any():
         rng = np.random.default_rng(0)
         qp = q64 * (1.0 + rng.normal(size=q64.shape) * 6e-8)
-        refp = O.attempt_all(om, cfg, qp, qd, f, dt)
+        refp = O.attempt_all(om, cfg, qp, qd, f, dt, trend=trend)
         s_err = np.abs(refp["err"] - ref["err"])
         sq = np.abs(refp["q_double"] - ref["q_double"]).max(axis=1)
         s_q = np.array([sq[wb[w]:wb[w + 1]].max(initial=0.0) for w in range(W)])

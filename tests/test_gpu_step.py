@@ -382,3 +382,95 @@ def test_nan_world_sap_raises_mujoco_holds_and_latches(na):
     others = [w for w in range(8) if w != 3]
     assert torch.equal(bad.body_q[others], good.body_q[others])
     assert torch.equal(bad.body_qd[others], good.body_qd[others])
+
+
+def test_attempt_with_trend_matches_oracle(na, O):
+    """na_attempt with the solver's own trend (the seed accelerations its boundary calls left)
+    against the oracle's attempt given the same trend: the error-norm and pose bars of the attempt
+    parity above, on every world."""
+    model = _model(na, "pile")
+    state, solver = _settle(na, model, 20)
+    assert solver.warm_start == "extrapolate"
+    trend = solver._trend.clone()
+    assert trend.abs().max().item() > 0.0
+    q64 = solver.body_q64.clone()
+    qd = state.body_qd.clone()
+    f = torch.zeros_like(qd)
+    om = O.OracleModel(model)
+    dtt = torch.full((model.world_count,), 2e-3, dtype=torch.float64, device=DEV)
+    out = GE._attempt(solver, q64, qd, f, dtt, with_contacts=True, trend=trend)
+    cfg = O.make_config(tol=1e-3, max_iterations=100)
+    qn, qdn, fn, dtn = q64.cpu().numpy(), qd.cpu().numpy().astype(np.float64), f.cpu().numpy(), dtt.cpu().numpy()
+    tn = trend.cpu().numpy().astype(np.float64)
+    ref = O.attempt_all(om, cfg, qn, qdn, fn, dtn, trend=tn)
+    assert out["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
+    P.check_free_attempts(O, om, cfg, out, ref, qn, qdn, fn, dtn, model.host, trend=tn)
+
+
+def test_trend_of_a_free_fall_is_gravity_and_reset_clears_it(na):
+    """After a frame of free fall the per-body trend is gravity (float32 velocities over
+    millisecond steps: within 1e-3 relative); reference seeds keep none; reset() clears it."""
+    for seeds in ("extrapolate", "reference"):
+        model = na.scenes.box_drop(16, seed=2, device=DEV)
+        state = model.state()
+        solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=64,
+                                      warm_start=seeds)
+        solver.step(state, state, model.control(), None, 0.01)
+        torch.cuda.synchronize()
+        assert solver.substeps.min().item() > 1
+        tr = solver._trend.cpu().numpy()
+        if seeds == "reference":
+            assert not tr.any()
+            continue
+        assert np.abs(tr[:, 2] + 9.81).max() < 9.81e-3, tr
+        assert np.abs(tr[:, [0, 1, 3, 4, 5]]).max() < 1e-3, tr
+        mask = torch.zeros(model.world_count, dtype=torch.bool, device=DEV)
+        mask[::2] = True
+        solver.reset(world_mask=mask)
+        torch.cuda.synchronize()
+        tr2 = solver._trend.cpu().numpy()
+        assert not tr2[::2].any() and np.array_equal(tr2[1::2], tr[1::2])
+
+
+@pytest.mark.parametrize("seeds", ["extrapolate", "reference"])
+def test_served_and_plain_identical_for_both_seeds(na, seeds):
+    """The served and the lock-step boundary forms give bit-identical states under either Newton
+    seed rule (the trend lives in device memory between the lock-step launches)."""
+    runs = {}
+    for schedule in ("served", "plain"):
+        model = na.scenes.uneven_pile(96, seed=21, device=DEV, max_prims=16)
+        state = model.state()
+        solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256,
+                                      schedule=schedule, warm_start=seeds)
+        for _ in range(8):
+            solver.step(state, state, model.control(), None, 0.01)
+        torch.cuda.synchronize()
+        runs[schedule] = (state.body_q.cpu().numpy(), state.body_qd.cpu().numpy(), solver._trend.cpu().numpy())
+    for k in range(3):
+        assert np.array_equal(runs["served"][k], runs["plain"][k])
+
+
+def test_extrapolated_seeds_cut_newton_iterations(na):
+    """On settled piles the extrapolated seeds take fewer Newton iterations than the reference's
+    (the oracle study: 5-25% per heavy world); both runs reach every boundary with finite states."""
+    model = na.scenes.uneven_pile(256, seed=31, device=DEV)
+    state, warm = _settle(na, model, 20, warm_start="reference")
+    q0, qd0 = state.body_q.clone(), state.body_qd.clone()
+    newton = {}
+    for seeds in ("reference", "extrapolate"):
+        st = na.State(q0.clone(), qd0.clone(), torch.zeros_like(qd0))
+        solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=128,
+                                      warm_start=seeds)
+        solver.step(st, st, model.control(), None, 0.01)  # builds the trend
+        solver.reset_work_counters()
+        for _ in range(6):
+            solver.step(st, st, model.control(), None, 0.01)
+        torch.cuda.synchronize()
+        assert np.isfinite(st.body_q.cpu().numpy()).all()
+        assert np.allclose(solver.sim_time.cpu().numpy(), 0.01, rtol=1e-12, atol=1e-15)
+        w = solver.work_counters()
+        newton[seeds] = (w["newton_iterations"], w["attempts"])
+    ratio = newton["extrapolate"][0] / newton["reference"][0]
+    print(f"Newton iterations over 6 frames: reference {newton['reference']}, extrapolate {newton['extrapolate']} "
+          f"(ratio {ratio:.3f})", flush=True)
+    assert ratio < 0.97

@@ -227,3 +227,47 @@ def test_attempt_on_given_contacts_equals_own_collision(na, O, mode):
     assert np.array_equal(given["err"], own["err"])
     assert np.array_equal(given["q_double"], own["q_double"]) and np.array_equal(given["q_full"], own["q_full"])
     assert np.array_equal(given["solve_ok"], own["solve_ok"])
+
+
+def test_trend_of_a_free_fall_is_gravity(na, O):
+    """The full step's seed acceleration (warm_start "extrapolate") is the mean acceleration of the
+    last accepted step, (v_new - v_old) / step: for boxes falling freely (no contact within the
+    frame) it equals gravity, and the reference seeds keep no trend."""
+    m = na.scenes.box_drop(4, seed=3, device="cpu")
+    om = O.OracleModel(m)
+    for seeds in ("extrapolate", "reference"):
+        q, qd, f = _state(m)
+        drv = O.OracleSolver(om, O.make_config(tol=1e-3, dt_init=1e-3, dt_max=0.01, max_substeps=64,
+                                               warm_start=seeds))
+        drv.step(q, qd, f, 0.01)
+        assert (drv.a["substeps"] > 1).all()
+        tr = drv.a["trend"][:m.body_count]
+        if seeds == "reference":
+            assert not tr.any()
+            continue
+        g = np.array([0.0, 0.0, -9.81], dtype=np.float32).astype(np.float64)
+        assert np.abs(tr[:, :3] - g).max() < 1e-9, tr
+        assert np.abs(tr[:, 3:]).max() < 1e-9, tr
+
+
+def test_extrapolated_seeds_reach_the_reference_seeds_solution(na, O):
+    """The seeds move where Newton starts, not the convex problems: an attempt seeded by the
+    extrapolation (with the trend of the settling frames) and one seeded as the reference
+    (solver_sap_adaptive.py:756-771) agree to the solve tolerance (optimality 1e-10), on settled
+    piles where the seeds differ by up to the bodies' velocity changes."""
+    m = na.scenes.uneven_pile(6, seed=12, device="cpu", max_prims=12)
+    om = O.OracleModel(m)
+    q, qd, f = _state(m)
+    cfg = O.make_config(tol=1e-3, dt_init=1e-3, dt_max=0.01, max_substeps=128, max_iterations=100)
+    drv = O.OracleSolver(om, cfg)
+    for _ in range(25):
+        drv.step(q, qd, f, 0.01)
+    trend = drv.a["trend"][:m.body_count].copy()
+    assert np.abs(trend).max() > 1e-3  # the piles still move: the seeds differ
+    dt = np.full(m.world_count, 2e-3)
+    ext = O.attempt_all(om, cfg, q, qd, f, dt, trend=trend)
+    ref = O.attempt_all(om, O.make_config(tol=1e-3, max_iterations=100, warm_start="reference"), q, qd, f, dt)
+    assert ext["solve_ok"].min() == 1 and ref["solve_ok"].min() == 1
+    assert np.abs(ext["q_double"] - ref["q_double"]).max() < 1e-9
+    assert np.abs(ext["qd_double"] - ref["qd_double"]).max() < 1e-6
+    assert np.abs(ext["err"] - ref["err"]).max() < 1e-9
