@@ -18,6 +18,9 @@ HEADER = os.path.join(_REPO, "include", "nadapt.h")
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # per-lane arrays (the work counters) stay private instead of taking 6 KB of each world's LDS,
+    # which keeps every free-body kernel at four waves per CU
+    "-mllvm", "-disable-promote-alloca-to-lds",
 ]
 
 vp = C.c_void_p

@@ -41,13 +41,15 @@ struct BodyLDS {
   float im[NA_MAX_BODIES];       // inverse mass
   float v[6][NA_BSLOT];          // f32 exchange vector (PCG directions)
   double vd[6][NA_BSLOT];        // f64 exchange vector (Newton iterate, line-search direction)
-  // Per-contact values that the body lanes sum, one phase at a time: the world-frame impulse of
-  // J^T gamma (fd, FP64), the world-frame Hessian block F^T G F (gw: xx yy zz xy xz yz), the
-  // world-frame force of the PCG product J^T G J p (f).
+  // Values that the body lanes sum, one phase at a time: per contact, the world-frame impulse of
+  // J^T gamma (fd, FP64) and the world-frame Hessian block F^T G F (gw: xx yy zz xy xz yz); per
+  // body entry, the wrench (force; torque about the body's COM) of the PCG product J^T G J p
+  // (ew: written by the contact lane into its two entry slots, so the body lane's reads need no
+  // index).
   union __attribute__((aligned(16))) {
     double fd[NA_MAX_CONTACTS][4];
     float gw[NA_MAX_CONTACTS][8];
-    float f[NA_MAX_CONTACTS][4];
+    float ew[2 * NA_MAX_CONTACTS][8];
   } xb;
 };
 
@@ -57,6 +59,7 @@ struct SolveLDS {
   // (-1 body A, +1 body B) in er[e].
   float er[2 * NA_MAX_CONTACTS][4] __attribute__((aligned(16)));
   uint8_t ent[2 * NA_MAX_CONTACTS];
+  uint8_t eslot[NA_MAX_CONTACTS][2];  // the entries of contact c's sides A and B (dynamic sides only)
   uint16_t eoff[NA_MAX_BODIES], edeg[NA_MAX_BODIES];
   float t1[3][NA_MAX_CONTACTS];
   float ra[3][NA_MAX_CONTACTS];  // contact point - COM of body A
@@ -114,6 +117,7 @@ __device__ __forceinline__ d3 dcross(d3 a, d3 b) {
 struct CRow {
   f3 t1, t2, n, ra, rb;
   int sa, sb;
+  int ea, eb;  // entry slots of the dynamic sides (build_entries)
   bool ha, hb;
 };
 __device__ __forceinline__ CRow load_row(const ContactLDS& C, const SolveLDS& X, int c, bool valid) {
@@ -128,6 +132,8 @@ __device__ __forceinline__ CRow load_row(const ContactLDS& C, const SolveLDS& X,
   r.hb = valid && bb >= 0;
   r.sa = r.ha ? ba : NA_DUMMY;
   r.sb = r.hb ? bb : NA_DUMMY;
+  r.ea = X.eslot[c][0];
+  r.eb = X.eslot[c][1];
   return r;
 }
 // relative contact velocity J v in the contact frame (f32, from B.v)
@@ -295,11 +301,13 @@ __device__ void build_entries(int nb, int nc, const ContactLDS& C, SolveLDS& X) 
         if (c < nc) {
           if ((int)(int8_t)((a[i] >> (8 * j)) & 0xff) == me) {
             X.ent[k] = (uint8_t)c;
+            X.eslot[c][0] = (uint8_t)k;
             *reinterpret_cast<float4*>(X.er[k]) = make_float4(X.ra[0][c], X.ra[1][c], X.ra[2][c], -1.f);
             ++k;
           }
           if ((int)(int8_t)((b[i] >> (8 * j)) & 0xff) == me) {
             X.ent[k] = (uint8_t)c;
+            X.eslot[c][1] = (uint8_t)k;
             *reinterpret_cast<float4*>(X.er[k]) = make_float4(X.rb[0][c], X.rb[1][c], X.rb[2][c], 1.f);
             ++k;
           }
@@ -320,30 +328,61 @@ __device__ __forceinline__ d3 world_d(const CRow& r, d3 lam) {
             r.t1.z * lam.x + r.t2.z * lam.y + r.n.z * lam.z};
 }
 
+// Entry sums run four entries per step: the step's indices, lever arms and contact values are
+// all requested before any is used (a scheduling barrier keeps the compiler from sinking the
+// loads into the arithmetic), so a body with d entries waits for 2 ceil(d / 4) LDS round trips
+// instead of 2 d.  Slots past the body's last entry re-read that entry (in bounds) and add
+// nothing.
+#define NA_ENT_STEP 4
+#define NA_LOADS_ISSUED() __builtin_amdgcn_sched_barrier(0)
+
 // This lane's body: jt += J_b^T gamma over its entries, from the world-frame impulses in B.xb.fd
 // (sign -1 on body A, +1 on body B; torque r x f).
 __device__ __forceinline__ void sum_entries_d(const BodyLDS& B, const SolveLDS& X, int eo, int ed, double* jt) {
-  for (int e = eo; e < eo + ed; ++e) {
-    const int c = X.ent[e];
-    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
-    const double s = rr.w;
-    const d3 f = d3{s * B.xb.fd[c][0], s * B.xb.fd[c][1], s * B.xb.fd[c][2]};
-    const d3 t = dcross(d3{rr.x, rr.y, rr.z}, f);
-    jt[0] += f.x; jt[1] += f.y; jt[2] += f.z;
-    jt[3] += t.x; jt[4] += t.y; jt[5] += t.z;
+  const int end = eo + ed;
+  for (int e0 = eo; e0 < end; e0 += NA_ENT_STEP) {
+    int c[NA_ENT_STEP];
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) c[i] = X.ent[min(e0 + i, end - 1)];
+    float4 rr[NA_ENT_STEP];
+    double fx[NA_ENT_STEP], fy[NA_ENT_STEP], fz[NA_ENT_STEP];
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      rr[i] = *reinterpret_cast<const float4*>(X.er[min(e0 + i, end - 1)]);
+      fx[i] = B.xb.fd[c[i]][0]; fy[i] = B.xb.fd[c[i]][1]; fz[i] = B.xb.fd[c[i]][2];
+    }
+    NA_LOADS_ISSUED();
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      const double s = e0 + i < end ? (double)rr[i].w : 0.0;
+      const d3 f = d3{s * fx[i], s * fy[i], s * fz[i]};
+      const d3 t = dcross(d3{rr[i].x, rr[i].y, rr[i].z}, f);
+      jt[0] += f.x; jt[1] += f.y; jt[2] += f.z;
+      jt[3] += t.x; jt[4] += t.y; jt[5] += t.z;
+    }
   }
 }
 
-// This lane's body: acc += J_b^T u over its entries, from the world-frame forces in B.xb.f.
-__device__ __forceinline__ void sum_entries_f(const BodyLDS& B, const SolveLDS& X, int eo, int ed, float* acc) {
-  for (int e = eo; e < eo + ed; ++e) {
-    const int c = X.ent[e];
-    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
-    const float4 fq = *reinterpret_cast<const float4*>(B.xb.f[c]);
-    const f3 f = mk(rr.w * fq.x, rr.w * fq.y, rr.w * fq.z);
-    const f3 t = cross3(mk(rr.x, rr.y, rr.z), f);
-    acc[0] += f.x; acc[1] += f.y; acc[2] += f.z;
-    acc[3] += t.x; acc[4] += t.y; acc[5] += t.z;
+// This lane's body: acc += the wrenches in its entry slots [eo, eo + ed) of B.xb.ew.  The slots
+// are contiguous, so every load of a step is issued before the first is waited for.
+__device__ __forceinline__ void sum_entry_wrenches(const BodyLDS& B, int eo, int ed, float* acc) {
+  const int end = eo + ed;
+  for (int e0 = eo; e0 < end; e0 += NA_ENT_STEP) {
+    float4 a[NA_ENT_STEP];
+    float2 b[NA_ENT_STEP];
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      const int e = min(e0 + i, end - 1);
+      a[i] = *reinterpret_cast<const float4*>(&B.xb.ew[e][0]);
+      b[i] = *reinterpret_cast<const float2*>(&B.xb.ew[e][4]);
+    }
+    NA_LOADS_ISSUED();
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      const bool in = e0 + i < end;
+      acc[0] += in ? a[i].x : 0.f; acc[1] += in ? a[i].y : 0.f; acc[2] += in ? a[i].z : 0.f;
+      acc[3] += in ? a[i].w : 0.f; acc[4] += in ? b[i].x : 0.f; acc[5] += in ? b[i].y : 0.f;
+    }
   }
 }
 
@@ -365,31 +404,45 @@ __device__ __forceinline__ void world_block(const CRow& r, const float* G, float
 // This lane's body: L (6x6 lower, row-major packed) += sum over its entries of
 // J_b^T G J_b = [[Gw, -Gw [r]x], [[r]x Gw, -[r]x Gw [r]x]], J_b = F [I, -[r]x].
 __device__ __forceinline__ void sum_entries_block(const BodyLDS& B, const SolveLDS& X, int eo, int ed, float* L) {
-  for (int e = eo; e < eo + ed; ++e) {
-    const int c = X.ent[e];
-    const float4 rr = *reinterpret_cast<const float4*>(X.er[e]);
-    const float4 g0 = *reinterpret_cast<const float4*>(&B.xb.gw[c][0]);
-    const float4 g1 = *reinterpret_cast<const float4*>(&B.xb.gw[c][4]);
-    const float G[3][3] = {{g0.x, g0.w, g1.x}, {g0.w, g0.y, g1.y}, {g1.x, g1.y, g0.z}};
-    const float rx = rr.x, ry = rr.y, rz = rr.z;
-    // M = [r]x Gw, rows ([r]x row i) . columns of Gw
-    float M[3][3];
+  const int end = eo + ed;
+  for (int e0 = eo; e0 < end; e0 += NA_ENT_STEP) {
+    int c[NA_ENT_STEP];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      M[0][j] = -rz * G[1][j] + ry * G[2][j];
-      M[1][j] = rz * G[0][j] - rx * G[2][j];
-      M[2][j] = -ry * G[0][j] + rx * G[1][j];
+    for (int i = 0; i < NA_ENT_STEP; ++i) c[i] = X.ent[min(e0 + i, end - 1)];
+    float4 rq[NA_ENT_STEP], g0q[NA_ENT_STEP], g1q[NA_ENT_STEP];
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      rq[i] = *reinterpret_cast<const float4*>(X.er[min(e0 + i, end - 1)]);
+      g0q[i] = *reinterpret_cast<const float4*>(&B.xb.gw[c[i]][0]);
+      g1q[i] = *reinterpret_cast<const float4*>(&B.xb.gw[c[i]][4]);
     }
-    // S = -M [r]x: S[i][j] = -(M[i][0] X[0][j] + M[i][1] X[1][j] + M[i][2] X[2][j])
-    const float S00 = -(M[0][1] * rz - M[0][2] * ry), S11 = -(-M[1][0] * rz + M[1][2] * rx);
-    const float S22 = -(M[2][0] * ry - M[2][1] * rx);
-    const float S10 = -(M[1][1] * rz - M[1][2] * ry), S20 = -(M[2][1] * rz - M[2][2] * ry);
-    const float S21 = -(-M[2][0] * rz + M[2][2] * rx);
-    L[0] += G[0][0]; L[1] += G[1][0]; L[2] += G[1][1]; L[3] += G[2][0]; L[4] += G[2][1]; L[5] += G[2][2];
-    L[6] += M[0][0]; L[7] += M[0][1]; L[8] += M[0][2];
-    L[10] += M[1][0]; L[11] += M[1][1]; L[12] += M[1][2];
-    L[15] += M[2][0]; L[16] += M[2][1]; L[17] += M[2][2];
-    L[9] += S00; L[13] += S10; L[14] += S11; L[18] += S20; L[19] += S21; L[20] += S22;
+    NA_LOADS_ISSUED();
+#pragma unroll
+    for (int i = 0; i < NA_ENT_STEP; ++i) {
+      const float m = e0 + i < end ? 1.f : 0.f;
+      const float4 g0 = make_float4(m * g0q[i].x, m * g0q[i].y, m * g0q[i].z, m * g0q[i].w);
+      const float4 g1 = make_float4(m * g1q[i].x, m * g1q[i].y, 0.f, 0.f);
+      const float G[3][3] = {{g0.x, g0.w, g1.x}, {g0.w, g0.y, g1.y}, {g1.x, g1.y, g0.z}};
+      const float rx = rq[i].x, ry = rq[i].y, rz = rq[i].z;
+      // M = [r]x Gw, rows ([r]x row i) . columns of Gw
+      float M[3][3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        M[0][j] = -rz * G[1][j] + ry * G[2][j];
+        M[1][j] = rz * G[0][j] - rx * G[2][j];
+        M[2][j] = -ry * G[0][j] + rx * G[1][j];
+      }
+      // S = -M [r]x: S[i][j] = -(M[i][0] X[0][j] + M[i][1] X[1][j] + M[i][2] X[2][j])
+      const float S00 = -(M[0][1] * rz - M[0][2] * ry), S11 = -(-M[1][0] * rz + M[1][2] * rx);
+      const float S22 = -(M[2][0] * ry - M[2][1] * rx);
+      const float S10 = -(M[1][1] * rz - M[1][2] * ry), S20 = -(M[2][1] * rz - M[2][2] * ry);
+      const float S21 = -(-M[2][0] * rz + M[2][2] * rx);
+      L[0] += G[0][0]; L[1] += G[1][0]; L[2] += G[1][1]; L[3] += G[2][0]; L[4] += G[2][1]; L[5] += G[2][2];
+      L[6] += M[0][0]; L[7] += M[0][1]; L[8] += M[0][2];
+      L[10] += M[1][0]; L[11] += M[1][1]; L[12] += M[1][2];
+      L[15] += M[2][0]; L[16] += M[2][1]; L[17] += M[2][2];
+      L[9] += S00; L[13] += S10; L[14] += S11; L[18] += S20; L[19] += S21; L[20] += S22;
+    }
   }
 }
 
@@ -760,9 +813,10 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double al
 // Preconditioned CG on H x = -g * gscale (f32) with the block-Jacobi factor L of this lane's
 // body; stops when the preconditioned residual r.z falls to eta2 (the squared relative
 // tolerance) times its start value, or after max_cg iterations.  The lane's
-// contact rows `rw` and G blocks `G` are the caller's registers.  The product H p = A p + J^T G J p
-// stores each contact's world-frame force G J p in B.xb.f and sums this lane's body entries
-// [eo, eo + ed).  Returns the iteration count.
+// contact rows `rw` and G blocks `G` are the caller's registers.  The product H p = A p + J^T G J p:
+// each contact lane writes the wrench of its world-frame force f = F^T G J p on each dynamic side
+// into that side's entry slot, (-f, -(r_A x f)) and (f, r_B x f), and each body lane sums its
+// slots [eo, eo + ed).  Returns the iteration count.
 template <int NG>
 __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const CRow* rw, const float (*G)[6],
                                    const BodyReg& br, bool own, int eo, int ed, const float* L, const double* g,
@@ -793,7 +847,17 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
       if (c < nc) {
         const f3 s3 = sym3_mul(G[q], u[q]);
         const f3 fw = rw[q].t1 * s3.x + rw[q].t2 * s3.y + rw[q].n * s3.z;
-        *reinterpret_cast<float4*>(B.xb.f[c]) = make_float4(fw.x, fw.y, fw.z, 0.f);
+        if (rw[q].ha) {
+          const f3 fa = mk(-fw.x, -fw.y, -fw.z);
+          const f3 ta = cross3(rw[q].ra, fa);
+          *reinterpret_cast<float4*>(&B.xb.ew[rw[q].ea][0]) = make_float4(fa.x, fa.y, fa.z, ta.x);
+          *reinterpret_cast<float2*>(&B.xb.ew[rw[q].ea][4]) = make_float2(ta.y, ta.z);
+        }
+        if (rw[q].hb) {
+          const f3 tb = cross3(rw[q].rb, fw);
+          *reinterpret_cast<float4*>(&B.xb.ew[rw[q].eb][0]) = make_float4(fw.x, fw.y, fw.z, tb.x);
+          *reinterpret_cast<float2*>(&B.xb.ew[rw[q].eb][4]) = make_float2(tb.y, tb.z);
+        }
       }
     }
     wave_sync();
@@ -801,7 +865,7 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
     float pHp = 0.f;
     if (own) {
       float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      sum_entries_f(B, X, eo, ed, acc);
+      sum_entry_wrenches(B, eo, ed, acc);
       applyA<float>(br, pp, Hp);
 #pragma unroll
       for (int q = 0; q < 6; ++q) { Hp[q] += acc[q]; pHp += pp[q] * Hp[q]; }
