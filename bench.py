@@ -597,7 +597,8 @@ def main():
                                     budget_ms=20.0, schedule=sched, device=dev,
                                     articulated=args.serve_articulated > 0, **kw)
             r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
-            sat[sched] = {k: r[k] for k in ("accepted_steps_per_s", "latency_p50_s", "latency_p99_s", "completed")}
+            sat[sched] = {k: r[k] for k in ("accepted_steps_per_s", "latency_p50_s", "latency_p99_s", "completed",
+                                            "device_busy")}
             log(f"[bench] saturated serving, {sched}: {r['accepted_steps_per_s']:.0f} accepted steps/s, latency p50 "
                 f"{r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
             del seng

@@ -11,6 +11,10 @@ The engine runs a host loop of *rounds*:
    at its own adaptive dt until its horizon or the round's device-time budget;
 3. retire finished requests (final state copied out, latency recorded) and free their slots.
 
+The rounds are pipelined: round k+1's launch is enqueued before the host waits for round k, and
+all host<->device copies go through pinned buffers without waiting for the running launch, so
+the device does not idle while the host retires and admits.
+
 Requests that arrive while others are on the GPU join the next round's launch, so their work
 overlaps on the device with the older rollouts.  A world interrupted by the budget resumes with
 its exact state, and its arithmetic never depends on the slot, arena position or the other
@@ -35,6 +39,14 @@ from .model import _t
 from .solvers import SolverMuJoCoAdaptive, SolverSAPAdaptive
 
 _I32_MAX = 2**31 - 1
+
+
+def _dev(arr, dtype, device) -> torch.Tensor:
+    """Host array -> device tensor through a pinned staging buffer, enqueued on the current stream
+    without waiting for the work already on it (a copy from pageable memory would first wait for
+    the running launch)."""
+    t = torch.as_tensor(np.ascontiguousarray(arr)).to(dtype)
+    return t.pin_memory().to(device, non_blocking=True)
 
 
 @dataclass
@@ -426,12 +438,22 @@ def check_request(r: RolloutRequest, global_count: int, budget_bytes: int) -> No
 
 
 @dataclass
+class _Snapshot:
+    """Host copies of a launch's per-world counters, valid once ``event`` has completed."""
+
+    event: torch.cuda.Event
+    bufs: list  # pinned status, accepted, frames_done [W]
+    live: list  # the requests listed in that launch
+
+
+@dataclass
 class ServingStats:
     completed: int = 0
     failed: int = 0
     accepted_steps: int = 0
     wall_s: float = 0.0
     rounds: int = 0
+    device_s: float = 0.0  # summed device time of the launches (HIP events around each)
     latencies: list = field(default_factory=list)
 
     def summary(self) -> dict:
@@ -441,7 +463,8 @@ class ServingStats:
                 "wall_s": self.wall_s, "rounds": self.rounds,
                 "accepted_steps_per_s": self.accepted_steps / self.wall_s if self.wall_s > 0 else math.nan,
                 "latency_p50_s": pct(50), "latency_p99_s": pct(99),
-                "latency_max_s": float(lat.max()) if len(lat) else math.nan}
+                "latency_max_s": float(lat.max()) if len(lat) else math.nan,
+                "device_busy": self.device_s / self.wall_s if self.wall_s > 0 else math.nan}
 
 
 class ServingEngine:
@@ -512,6 +535,11 @@ class ServingEngine:
         self._stream = torch.cuda.current_stream(d)
         self._unfinished = torch.zeros(1, dtype=torch.int32, device=d)
         self._unfinished_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self._snap_bufs = [[torch.zeros(W, dtype=torch.int32, pin_memory=True) for _ in range(3)] for _ in range(2)]
+        self._snap_flip = 0
+        self._last_live: list = []
+        self._launch_events: list = []
+        self._copyouts: list = []
 
     # ------------------------------------------------------------------ admission
     @staticmethod
@@ -581,12 +609,12 @@ class ServingEngine:
         P, d = self.pool, self.pool.device
         bidx = np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in batch]).astype(np.int64)
         sidx = np.concatenate([np.arange(r.s0, r.s0 + r.shape_count) for r in batch]).astype(np.int64)
-        bi = torch.as_tensor(bidx, device=d)
-        si = torch.as_tensor(sidx, device=d)
+        bi = _dev(bidx, torch.int64, d)
+        si = _dev(sidx, torch.int64, d)
         for k in _BODY_FIELDS:
             vals = np.concatenate([r.bodies[k] for r in batch])
             dst = getattr(P, k)
-            dst.index_copy_(0, bi, _t(vals, torch.float32, d).reshape((-1,) + tuple(dst.shape[1:])))
+            dst.index_copy_(0, bi, _dev(vals, torch.float32, d).reshape((-1,) + tuple(dst.shape[1:])))
         P.body_f.index_fill_(0, bi, 0.0)
         # FP64 pose shadow = the float32 pose, as a first boundary call would seed it
         self.solver._q64.index_copy_(0, bi, P.body_q.index_select(0, bi).double())
@@ -596,15 +624,15 @@ class ServingEngine:
                 vals = np.concatenate([np.where(r.shapes[k] >= 0, r.shapes[k] + r.b0, -1) for r in batch])
             dt = torch.int32 if k in ("shape_type", "shape_body", "shape_collision_group") else torch.float32
             dst = getattr(P, k)
-            dst.index_copy_(0, si, _t(vals, dt, d).reshape((-1,) + tuple(dst.shape[1:])))
-        P.shape_world.index_copy_(0, si, torch.as_tensor(
-            np.concatenate([np.full(r.shape_count, r.slot) for r in batch]), dtype=torch.int32, device=d))
+            dst.index_copy_(0, si, _dev(vals, dt, d).reshape((-1,) + tuple(dst.shape[1:])))
+        P.shape_world.index_copy_(0, si, _dev(np.concatenate([np.full(r.shape_count, r.slot) for r in batch]),
+                                             torch.int32, d))
         if P.is_articulated:
             self._upload_joints(batch, bi)
-        slots = torch.as_tensor([r.slot for r in batch], dtype=torch.int64, device=d)
+        slots = _dev([r.slot for r in batch], torch.int64, d)
 
         def put(t, vals, dtype):
-            t.index_copy_(0, slots, torch.as_tensor(vals, dtype=dtype, device=d))
+            t.index_copy_(0, slots, _dev(np.asarray(vals), dtype, d))
 
         put(P.world_body_start, [r.b0 for r in batch], torch.int32)
         put(P.world_body_end, [r.b0 + r.body_count for r in batch], torch.int32)
@@ -630,31 +658,28 @@ class ServingEngine:
         P, d = self.pool, self.pool.device
         jt = np.concatenate([r.joints["joint_type"] if r.articulated else np.zeros(r.body_count, np.int32)
                              for r in batch]).astype(np.int32)
-        P.joint_type.index_copy_(0, bi, torch.as_tensor(jt, device=d))
+        P.joint_type.index_copy_(0, bi, _dev(jt, torch.int32, d))
         art = [r for r in batch if r.articulated]
         if not art:
             return
-        ab = torch.as_tensor(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in art]), dtype=torch.int64,
-                             device=d)
+        ab = _dev(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in art]), torch.int64, d)
         par = np.concatenate([np.where(r.joints["joint_parent"] >= 0, r.joints["joint_parent"] + r.b0, -1)
                               for r in art])
-        P.joint_parent.index_copy_(0, ab, torch.as_tensor(par, dtype=torch.int32, device=d))
-        P.joint_q_start.index_copy_(0, ab, torch.as_tensor(
-            np.concatenate([r.joints["joint_q_start"] + r.c0 for r in art]), dtype=torch.int32, device=d))
-        P.joint_qd_start.index_copy_(0, ab, torch.as_tensor(
-            np.concatenate([r.joints["joint_qd_start"] + r.d0 for r in art]), dtype=torch.int32, device=d))
+        P.joint_parent.index_copy_(0, ab, _dev(par, torch.int32, d))
+        P.joint_q_start.index_copy_(0, ab, _dev(np.concatenate([r.joints["joint_q_start"] + r.c0 for r in art]),
+                                                torch.int32, d))
+        P.joint_qd_start.index_copy_(0, ab, _dev(np.concatenate([r.joints["joint_qd_start"] + r.d0 for r in art]),
+                                                 torch.int32, d))
         for k in ("joint_X_p", "joint_X_c", "joint_axis"):
             dst = getattr(P, k)
-            dst.index_copy_(0, ab, _t(np.concatenate([r.joints[k] for r in art]), torch.float32, d).reshape(
+            dst.index_copy_(0, ab, _dev(np.concatenate([r.joints[k] for r in art]), torch.float32, d).reshape(
                 (-1,) + tuple(dst.shape[1:])))
-        di = torch.as_tensor(np.concatenate([np.arange(r.d0, r.d0 + r.dof_count) for r in art]), dtype=torch.int64,
-                             device=d)
-        ci = torch.as_tensor(np.concatenate([np.arange(r.c0, r.c0 + r.coord_count) for r in art]), dtype=torch.int64,
-                             device=d)
+        di = _dev(np.concatenate([np.arange(r.d0, r.d0 + r.dof_count) for r in art]), torch.int64, d)
+        ci = _dev(np.concatenate([np.arange(r.c0, r.c0 + r.coord_count) for r in art]), torch.int64, d)
         for k in _JOINT_DOF_FIELDS:
-            getattr(P, k).index_copy_(0, di, _t(np.concatenate([r.joints[k] for r in art]), torch.float32, d))
-        P.joint_q.index_copy_(0, ci, _t(np.concatenate([r.joints["joint_q"] for r in art]), torch.float32, d))
-        P.joint_qd.index_copy_(0, di, _t(np.concatenate([r.joints["joint_qd"] for r in art]), torch.float32, d))
+            getattr(P, k).index_copy_(0, di, _dev(np.concatenate([r.joints[k] for r in art]), torch.float32, d))
+        P.joint_q.index_copy_(0, ci, _dev(np.concatenate([r.joints["joint_q"] for r in art]), torch.float32, d))
+        P.joint_qd.index_copy_(0, di, _dev(np.concatenate([r.joints["joint_qd"] for r in art]), torch.float32, d))
         P.joint_q64.index_copy_(0, ci, P.joint_q.index_select(0, ci).double())
         P.joint_qd64.index_copy_(0, di, P.joint_qd.index_select(0, di).double())
 
@@ -674,10 +699,15 @@ class ServingEngine:
         budget = self.budget_ms if max_iterations <= 0 else 0.0
         flag = unfinished.data_ptr() if unfinished is not None else None
         if P.is_articulated and self.split_kinds:
+            self._last_live = live
+            t_start = self._timed_event()
             self._launch_split(live, budget, max_iterations, unfinished)
+            self._launch_events.append((t_start, self._timed_event()))
             self.stats.rounds += 1
             return len(live)
-        self._active = torch.as_tensor([r.slot for r in live], dtype=torch.int32, device=P.device)
+        self._active = _dev([r.slot for r in live], torch.int32, P.device)
+        self._last_live = live
+        t_start = self._timed_event()
         if P.is_articulated:
             # one launch over both kinds: every wave takes the next world of either kind
             _lib.check(L.na_serve_mixed(
@@ -693,6 +723,7 @@ class ServingEngine:
                                   self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
                                   P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
                                   budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve")
+        self._launch_events.append((t_start, self._timed_event()))
         self.stats.rounds += 1
         return len(live)
 
@@ -704,14 +735,14 @@ class ServingEngine:
         free = [r for r in live if not r.articulated]
         art = [r for r in live if r.articulated]
         if free:
-            self._active = torch.as_tensor([r.slot for r in free], dtype=torch.int32, device=P.device)
+            self._active = _dev([r.slot for r in free], torch.int32, P.device)
             _lib.check(L.na_serve(self.solver._h, self._active.data_ptr(), len(free), self.horizon_frames.data_ptr(),
                                   self.horizon_accepted.data_ptr(), self.frames_done.data_ptr(),
                                   self.accepted.data_ptr(), self.status.data_ptr(), self.world_tol.data_ptr(),
                                   P.body_q.data_ptr(), P.body_qd.data_ptr(), P.body_f.data_ptr(), self.dt_outer,
                                   budget, int(max_iterations), flag, _lib.stream_handle()), "na_serve")
         if art:
-            self._active_art = torch.as_tensor([r.slot for r in art], dtype=torch.int32, device=P.device)
+            self._active_art = _dev([r.slot for r in art], torch.int32, P.device)
             if free and unfinished is not None:  # the second launch starts by clearing the flag
                 self._flag_free = unfinished.clone()
             _lib.check(L.na_serve_articulated(
@@ -723,32 +754,63 @@ class ServingEngine:
             if free and unfinished is not None:
                 unfinished.bitwise_or_(self._flag_free)
 
-    def retire(self, now: float) -> list:
-        """Read per-world status after the launch; copy out and free every finished request."""
-        if not self.resident:
+    def _timed_event(self) -> torch.cuda.Event:
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record(self._stream)
+        return ev
+
+    def _account_launches(self) -> None:
+        """Add the device time of every launch whose end event has completed to ``stats``."""
+        keep = []
+        for a, b in self._launch_events:
+            if b.query():
+                self.stats.device_s += a.elapsed_time(b) * 1e-3
+            else:
+                keep.append((a, b))
+        self._launch_events = keep
+
+    def _snapshot(self) -> _Snapshot:
+        """Enqueue copies of the per-world status, accepted and frame counters of the launch just
+        enqueued into pinned host buffers (two sets, used alternately) and an event after them."""
+        bufs = self._snap_bufs[self._snap_flip]
+        self._snap_flip ^= 1
+        for host, dev in zip(bufs, (self.status, self.accepted, self.frames_done)):
+            host.copy_(dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(self._stream)
+        return _Snapshot(ev, bufs, list(self._last_live))
+
+    def _retire_snapshot(self, snap: _Snapshot, now: float) -> list:
+        """Retire the requests of ``snap``'s launch that had finished by its end.  Their final
+        states are gathered on the stream behind the launches already enqueued (which skip
+        finished worlds, so the states are final) into pinned buffers; the slots and arena ranges
+        are freed at once, since any later upload into them is enqueued behind the gathers."""
+        status, acc, frames = (b.numpy() for b in snap.bufs)
+        done = [r for r in snap.live if status[r.slot] != 0 and self.resident.get(r.slot) is r]
+        if not done:
             return []
-        status = self.status.cpu().numpy()
-        slots = [s for s, r in self.resident.items() if status[s] != 0]
-        if not slots:
-            return []
-        acc = self.accepted.cpu().numpy()
-        frames = self.frames_done.cpu().numpy()
-        done = [self.resident.pop(s) for s in slots]
-        bidx = torch.as_tensor(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in done]),
-                               dtype=torch.int64, device=self.pool.device)
-        q = self.pool.body_q.index_select(0, bidx).cpu().numpy()
-        qd = self.pool.body_qd.index_select(0, bidx).cpu().numpy()
-        if any(r.articulated for r in done):
-            jq = self.pool.joint_q.cpu().numpy()
-            jqd = self.pool.joint_qd.cpu().numpy()
-        o = 0
         for r in done:
-            n = r.body_count
-            r.body_q, r.body_qd = q[o:o + n], qd[o:o + n]
-            o += n
+            del self.resident[r.slot]
+        d = self.pool.device
+        bidx = _dev(np.concatenate([np.arange(r.b0, r.b0 + r.body_count) for r in done]), torch.int64, d)
+
+        def gather(src, idx):
+            host = torch.empty((idx.numel(),) + tuple(src.shape[1:]), dtype=src.dtype, pin_memory=True)
+            host.copy_(src.index_select(0, idx), non_blocking=True)
+            return host
+
+        q, qd = gather(self.pool.body_q, bidx), gather(self.pool.body_qd, bidx)
+        jq = jqd = None
+        art = [r for r in done if r.articulated]
+        if art:
+            ci = _dev(np.concatenate([np.arange(r.c0, r.c0 + r.coord_count) for r in art]), torch.int64, d)
+            di = _dev(np.concatenate([np.arange(r.d0, r.d0 + r.dof_count) for r in art]), torch.int64, d)
+            jq, jqd = gather(self.pool.joint_q, ci), gather(self.pool.joint_qd, di)
+        ev = torch.cuda.Event()
+        ev.record(self._stream)
+        self._copyouts.append((ev, done, q, qd, jq, jqd))
+        for r in done:
             if r.articulated:
-                r.joint_q = jq[r.c0:r.c0 + r.coord_count].copy()
-                r.joint_qd = jqd[r.d0:r.d0 + r.dof_count].copy()
                 self._coords.release(r.c0, r.coord_count)
                 self._dofs.release(r.d0, r.dof_count)
             r.status, r.accepted, r.frames = int(status[r.slot]), int(acc[r.slot]), int(frames[r.slot])
@@ -766,12 +828,55 @@ class ServingEngine:
             self.finished.append(r)
         return done
 
+    def _collect_copyouts(self, block: bool) -> None:
+        """Hand the gathered final states to their requests once the gathers have run."""
+        keep = []
+        for ev, done, q, qd, jq, jqd in self._copyouts:
+            if block:
+                ev.synchronize()
+            elif not ev.query():
+                keep.append((ev, done, q, qd, jq, jqd))
+                continue
+            q, qd = q.numpy(), qd.numpy()
+            jq = jq.numpy() if jq is not None else None
+            jqd = jqd.numpy() if jqd is not None else None
+            o = oc = od = 0
+            for r in done:
+                n = r.body_count
+                r.body_q, r.body_qd = q[o:o + n].copy(), qd[o:o + n].copy()
+                o += n
+                if r.articulated:
+                    r.joint_q = jq[oc:oc + r.coord_count].copy()
+                    r.joint_qd = jqd[od:od + r.dof_count].copy()
+                    oc += r.coord_count
+                    od += r.dof_count
+        self._copyouts = keep
+
+    def retire(self, now: float) -> list:
+        """Wait for the enqueued work, then retire every finished resident request: final state
+        copied out, latency recorded, slot freed."""
+        self._stream.synchronize()
+        if not self.resident:
+            return []
+        self._last_live = list(self.resident.values())
+        snap = self._snapshot()
+        snap.event.synchronize()
+        done = self._retire_snapshot(snap, now)
+        self._collect_copyouts(block=True)
+        self._account_launches()
+        return done
+
     def run(self, requests=None, *, realtime: bool = True, max_wall_s: float | None = None) -> dict:
         """Serve ``requests`` (plus anything already queued) to completion.
 
         With ``realtime`` the requests' ``arrival`` times are seconds after the start of this call
         and requests are admitted only once they have arrived; otherwise all are admitted at
         once.  Returns :meth:`ServingStats.summary`.
+
+        Launches are pipelined: the next round's launch is enqueued before the host waits for the
+        previous one, so the device never waits for the host's retirement and admission work.
+        A launch lists every resident world the host has not yet seen finish; a world that
+        finished in the launch before is skipped on the device.
         """
         if requests is not None:
             self.submit(requests)
@@ -784,19 +889,25 @@ class ServingEngine:
             return self._run_static(t0, now, max_wall_s)
 
         self.admit(now())
-        while self.pending or self.resident:
-            if not self.resident:
+        prev = None
+        while self.pending or self.resident or prev is not None:
+            snap = self._snapshot() if self.resident and self.launch() else None
+            if prev is not None:
+                prev.event.synchronize()
+                self._retire_snapshot(prev, time.perf_counter() - t0)
+            self._collect_copyouts(block=False)
+            self._account_launches()
+            if snap is None and prev is None and not self.resident and self.pending:
                 wait = self.pending[0].arrival - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(min(wait, 0.05))
-                self.admit(now())
-                continue
-            self.launch()
-            self.admit(now())  # copies go behind the running launch on the same stream
-            self._stream.synchronize()
-            self.retire(time.perf_counter() - t0)
+            self.admit(now())
+            prev = snap
             if max_wall_s is not None and time.perf_counter() - t0 > max_wall_s:
                 break
+        self._stream.synchronize()
+        self._collect_copyouts(block=True)
+        self._account_launches()
         self.stats.wall_s = time.perf_counter() - t0
         return self.stats.summary()
 
