@@ -322,6 +322,44 @@ def articulated_form(na, args, log):
     return res
 
 
+def serving_tiles(na, args, dev, kw, log):
+    """Autotuner of the serving schedule on the free-body part of the serving workload, every
+    request waiting at t = 0: two-wave tiles for the worlds with the most measured work left
+    (``ServingEngine(tiles=k)``, concurrent Newton seeds) against one world per wave (extrapolated
+    and concurrent seeds).  Tiled and untiled runs of the same seeds must leave bit-identical
+    results; the best candidate is reported."""
+    import numpy as np
+    import torch
+
+    rows, ref = [], {}
+    for ws, tiles in (("extrapolate", 0), ("concurrent", 0), ("concurrent", 64), ("concurrent", 256)):
+        smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=None, device=dev,
+                                                   distinct=args.serve_worlds, articulated_share=0.0)
+        seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
+                                budget_ms=20.0, device=dev, tiles=tiles, **dict(kw, warm_start=ws))
+        r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
+        sig = np.concatenate([q.body_q.reshape(-1) for q in sreqs])
+        same = None
+        if ws in ref:
+            same = bool(np.array_equal(sig, ref[ws]))
+            if not same:
+                raise RuntimeError(f"serving tiles changed results under warm_start={ws}")
+        ref.setdefault(ws, sig)
+        rows.append({"warm_start": ws, "tiles": tiles, "accepted_steps_per_s": r["accepted_steps_per_s"],
+                     "latency_p50_s": r["latency_p50_s"], "latency_p99_s": r["latency_p99_s"],
+                     "completed": r["completed"], "identical_to_untiled": same})
+        log(f"[bench] free-body serving, warm_start {ws}, tiles {tiles}: {r['accepted_steps_per_s']:.0f} accepted "
+            f"steps/s, latency p50 {r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
+        del seng
+        torch.cuda.synchronize()
+    best = max(rows, key=lambda x: x["accepted_steps_per_s"])
+    return {"candidates": rows, "best": best,
+            "workload": f"{args.serve_requests} free-body requests of the serving workload's distribution, all "
+                        "waiting at t = 0, 20 ms rounds",
+            "form": "tiles: the worlds with the most measured work left run on two wavefronts each (full step and "
+                    "first half step solved at once)"}
+
+
 def collide_form(na, model, state, log, calls=20):
     """Contact generation through the public API (CollisionPipeline.collide, na_collide) on the
     settled piles: contacts per second and algorithmic HBM bytes per call (176 B per segment
@@ -378,6 +416,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=3600)
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
+    ap.add_argument("--serve-tiles", type=int, default=0,
+                    help="continuous serving: worlds per round run as two-wave tiles (needs --warm-start concurrent)")
     ap.add_argument("--serve-articulated", type=float, default=0.25,
                     help="share of requests that are articulated worlds (arm or quadruped on terrain)")
     ap.add_argument("--cpu-worlds", type=int, default=256, help="CPU baseline sample on all host cores")
@@ -388,8 +428,9 @@ def main():
     ap.add_argument("--no-collide", action="store_true")
     ap.add_argument("--no-mujoco", action="store_true", help="skip the MuJoCo inner-step line")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule (4 waves per CU)")
-    ap.add_argument("--warm-start", choices=("extrapolate", "reference"), default="extrapolate",
-                    help="Newton seeds of the attempts' solves (the reference's, or extrapolated)")
+    ap.add_argument("--warm-start", choices=("extrapolate", "concurrent", "reference"), default="extrapolate",
+                    help="Newton seeds of the attempts' solves (extrapolated; concurrent: the first half step's seed "
+                         "independent of the full step, which two-wave tiles need; or the reference's)")
     ap.add_argument("--articulated-only", action="store_true",
                     help="run only the articulated workload (profiling of k_art_frame); prints its JSON object")
     ap.add_argument("--art-worlds", type=int, default=8192)
@@ -437,12 +478,15 @@ def main():
     # ---------------------------------------------------------------- served rollout (timed)
     stream = torch.cuda.current_stream()
 
-    def served_run(grid=0):
-        """One served rollout launch of args.steps frames of every world from the settled state
-        (``grid``: persistent grid in waves, 0 = one wave per resident slot)."""
+    def served_run(grid=0, tiles=0, round_ms=0.0):
+        """The served rollout of args.steps frames of every world from the settled state
+        (``grid``: persistent grid in waves, 0 = one wave per resident slot).  Without tiles it is
+        one launch; with ``tiles`` the rollout runs in launches of ``round_ms`` of device time, each
+        running the ``tiles`` worlds with the most measured work left as two-wave tiles.  Returns
+        the engine, the requests, the wall time and the summed device time of the launches."""
         eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
-                               shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
-                               device=dev, **kw)
+                               shape_capacity=model.shape_count - int(model.global_shapes.numel()),
+                               budget_ms=round_ms, device=dev, tiles=tiles, **kw)
         if grid:
             eng.solver.set_schedule(grid)
         reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
@@ -450,34 +494,45 @@ def main():
         eng.admit(math.inf)
         eng.solver.reset_work_counters()
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        eng.launch()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        eng.retire(wall)
-        assert not eng.resident, "every rollout finishes in the one launch without a time budget"
-        return eng, reqs, wall, e0.elapsed_time(e1)
+        if tiles or round_ms > 0:
+            t0 = time.perf_counter()
+            eng.run(realtime=False, pipelined=False)
+            wall = time.perf_counter() - t0
+            ms = eng.stats.device_s * 1e3
+        else:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(stream)
+            eng.launch()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            eng.retire(wall)
+            ms = e0.elapsed_time(e1)
+        assert not eng.resident, "every rollout finishes"
+        return eng, reqs, wall, ms
 
-    served_grid, served_tune = 0, None
+    served_sched, served_tune = (0, 0, 0.0), None
     k0 = 0
     if not args.no_autotune and args.warmup >= 3:
-        # the first untimed repeats time the served launch on 4, 3 and 2 waves per CU (same results)
-        def tune_run(grid):
-            e, r, w_wall, ms = served_run(grid)
+        # the first untimed repeats time the served rollout on each candidate schedule (same results)
+        def tune_run(grid, tiles, round_ms):
+            e, r, w_wall, ms = served_run(grid, tiles, round_ms)
             sig = torch.as_tensor(np.concatenate([x.body_q.reshape(-1) for x in r]))
             del e
-            return ms, sig
-        served_tune = na.serving.autotune_served(tune_run)
-        served_grid = served_tune["best"]["grid_waves"]
+            return w_wall * 1e3, sig
+        cands = [{"waves_per_cu": w, "tiles": 0, "round_ms": 0.0} for w in (4, 3, 2)]
+        if args.warm_start == "concurrent":
+            cands += [{"waves_per_cu": 4, "tiles": t, "round_ms": r} for t, r in ((64, 250.0), (128, 250.0))]
+        served_tune = na.serving.autotune_served(tune_run, cands)
+        b = served_tune["best"]
+        served_sched = (b["grid_waves"], b["tiles"], b["round_ms"])
         k0 = len(served_tune["candidates"])
-        log(f"[bench] autotune (served launch): {served_tune['candidates']} -> {served_tune['best']}")
+        log(f"[bench] autotune (served rollout): {served_tune['candidates']} -> {served_tune['best']}")
     for k in range(k0, args.warmup):
-        _, _, w_wall, _ = served_run(served_grid)
+        _, _, w_wall, _ = served_run(*served_sched)
         log(f"[bench] untimed repeat {k + 1}/{args.warmup} of the timed launch: {w_wall:.3f} s")
-    eng, reqs, wall, launch_ms = served_run(served_grid)
+    eng, reqs, wall, launch_ms = served_run(*served_sched)
     accepted = sum(r.accepted for r in reqs)
     value = accepted / wall
     slots = np.array([r.slot for r in reqs])
@@ -517,20 +572,27 @@ def main():
                                f"{SETTLE_FRAMES} settling frames",
                    "settle_frames": SETTLE_FRAMES, "warm_start": args.warm_start,
                    "worlds": W, "tol": args.tol, "dt_outer": args.dt_outer, "seed": args.seed},
-        "served": {"form": "persistent rollout launch (na_serve), no barrier between worlds or frames",
+        "served": {"form": ("persistent rollout launch (na_serve), no barrier between worlds or frames"
+                            if not served_sched[1] else
+                            f"persistent rollout launches (na_serve) of {served_sched[2]:g} ms of device time; in "
+                            f"each, the {served_sched[1]} worlds with the most measured work left run as two-wave "
+                            "tiles (full step and first half step solved at once), the rest one world per wave"),
+                   "schedule": {"grid_waves": served_sched[0], "tiles": served_sched[1],
+                                "round_ms": served_sched[2]},
                    "accepted": accepted, "wall_s": wall, "finite": finite, "attempts": int(wk[0]),
                    "newton_iterations": int(wk[1]), "cg_iterations": int(wk[2]), "contacts": int(wk[3]),
                    "pairs": int(wk[4])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_serve",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_serve_tile" if served_sched[1] else "k_serve",
                      "algorithmic_bytes_per_launch": bytes_launch, "launch_ms": launch_ms},
     }
     if tune is not None:
         out["autotune"] = {"boundary_calls": dict(tune, form="2 boundary calls of the settled piles per candidate "
                                                                "(waves per CU x world order), from the same state")}
         if served_tune is not None:
-            out["autotune"]["served"] = dict(served_tune, form="the timed served launch on each candidate grid "
-                                                              "(the first untimed repeats), largest worlds first")
+            out["autotune"]["served"] = dict(served_tune, form="the timed served rollout on each candidate schedule "
+                                                              "(the first untimed repeats; ms of wall time)")
 
     # ---------------------------------------------------------------- lock-step forms
     def frames_form(schedule):
@@ -564,7 +626,8 @@ def main():
                                                        articulated_share=args.serve_articulated)
             seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
                                     budget_ms=20.0, schedule=sched, device=dev,
-                                    articulated=args.serve_articulated > 0, **kw)
+                                    articulated=args.serve_articulated > 0,
+                                    tiles=args.serve_tiles if sched == "continuous" else 0, **kw)
             log(f"[bench] serving {len(sreqs)} requests at {args.serve_rate:g}/s, {sched} batching")
             res[sched] = seng.run(sreqs, realtime=True, max_wall_s=240.0)
             res[sched]["final_states"] = [(r.body_q, r.body_qd, r.joint_q, r.joint_qd) for r in sreqs]
@@ -595,7 +658,8 @@ def main():
                                                        articulated_share=args.serve_articulated)
             seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
                                     budget_ms=20.0, schedule=sched, device=dev,
-                                    articulated=args.serve_articulated > 0, **kw)
+                                    articulated=args.serve_articulated > 0,
+                                    tiles=args.serve_tiles if sched == "continuous" else 0, **kw)
             r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
             sat[sched] = {k: r[k] for k in ("accepted_steps_per_s", "latency_p50_s", "latency_p99_s", "completed",
                                             "device_busy")}
@@ -603,6 +667,8 @@ def main():
                 f"{r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
             del seng
         out["serving"]["saturated"] = dict(sat, form="the same requests all waiting at t = 0 (capacity)")
+        if not args.no_autotune:
+            out["serving"]["tiles"] = serving_tiles(na, args, dev, kw, log)
 
     # ---------------------------------------------------------------- contact generation API
     if not args.no_collide:

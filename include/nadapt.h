@@ -122,8 +122,10 @@ typedef struct na_config {
                               from (v_t + v_full) / 2, second from v_full.  1: extrapolated: full step
                               from v_t + dt/2 a (a: the body's mean acceleration over its last
                               committed step, kept in body_trend), first half step as 0, second from
-                              2 v_mid - v_t.  Seeds move where Newton starts, not what it converges to.
-                              Free-body worlds; articulated worlds use 0's seeds. */
+                              2 v_mid - v_t.  2: concurrent: as 1, but the first half step from
+                              v_t + dt/4 a, so it does not depend on the full step's result.  Seeds
+                              move where Newton starts, not what it converges to.  Free-body worlds;
+                              articulated worlds use 0's seeds. */
 } na_config;
 
 /* Control inputs of a boundary call (newton Control, newton/_src/sim/control.py:31-110; applied by
@@ -201,6 +203,13 @@ int na_solver_telemetry(const na_solver* s, na_telemetry* out);
 /* Schedule: persistent grid of `grid_waves` wavefronts (<=0: automatic) pulling worlds
  * in `order` (device [W] permutation, or NULL for natural order). */
 int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order);
+/* Two-wave tiles of the served launch (na_serve): the first `tiles` entries of its world list each
+ * run on a workgroup of two wavefronts that solve an attempt's full step and first half step at
+ * the same time; the rest of the list runs one world per wavefront.  0 (default): no tiles.
+ * Needs adaptive mode, warm_start 2 and contact_law 0; a tiled world's results equal its
+ * single-wave results bit for bit.  Replaces the reference's single captured iteration body
+ * for every world (solver_sap_adaptive.py:851-907) with a per-world schedule. */
+int na_solver_set_tiles(na_solver* s, int32_t tiles);
 
 /* Boundary call: advance every world by dt_outer in place (served form: one persistent
  * launch, on-device ragged loop).  body_f [B,6] (may be NULL): applied COM wrenches, world frame;

@@ -168,6 +168,14 @@ __device__ __forceinline__ int ballot_rank(bool pred, int* total) {
   return __popcll(below);
 }
 
-// Keep LDS traffic of one wave ordered around cross-lane exchange.  A workgroup is exactly
-// one wavefront, so the barrier is cheap; it also orders LDS writes before reads.
-__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+// Order one wavefront's LDS traffic around a cross-lane exchange: the LDS operations of a wave
+// execute in program order, so writes before the fence are seen by reads after it; the
+// wavefront-scope fences keep the compiler from moving memory operations across.  It waits for
+// no other wave, so it is also correct where one wave of a larger workgroup runs alone.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Barrier of every wave of the workgroup, with the LDS traffic before it visible after it.
+__device__ __forceinline__ void group_sync() { __syncthreads(); }

@@ -51,7 +51,7 @@ struct Ctrl {
   int32_t *substeps, *iterations, *solve_fail, *overflow;
   unsigned long long* cum;
   long long* work;  // [W, NA_WORK]
-  float* trend;     // [B, 6] each body's mean acceleration over its last committed step (warm_start 1)
+  float* trend;     // [B, 6] each body's mean acceleration over its last committed step (warm_start 1, 2)
 };
 // per-world work counters: attempts, Newton iterations, CG iterations, contacts, pairs, solves,
 // then shader-clock cycles of collision, solve setup, residual evaluation, preconditioner, PCG,
@@ -68,6 +68,7 @@ struct na_solver {
   int32_t* flag_host;
   const int32_t* order;
   int grid_waves;
+  int tiles;  // leading entries of a served launch's list run as two-wave tiles (0: none)
   void* pool;
 };
 
@@ -439,8 +440,10 @@ __device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCt
 // 756-771): the full step from v_t, the first half step from (v_t + v_full) / 2, the second from
 // v_full.  warm_start 1 extrapolates: the full step from v_t + dt/2 a, with a = tr (this lane's
 // mean acceleration over the last committed step; v_t when tr is null), the first half step from
-// (v_t + v_full) / 2, the second from 2 v_mid - v_t (v_mid: the first half step's velocity).  The
-// seeds change where Newton starts, not the problem each solve converges on.
+// (v_t + v_full) / 2, the second from 2 v_mid - v_t (v_mid: the first half step's velocity).
+// warm_start 2 ("concurrent") seeds the first half step at v_t + dt/4 a, independent of the full
+// step's result, so the two solves that start at q_t need not wait for each other; the other two
+// seeds are 1's.  The seeds change where Newton starts, not the problem each solve converges on.
 template <bool DUMP = false>
 __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S,
                                 const double* q, const float* v, const float* f, const float* tr, double dt, double* qf,
@@ -464,8 +467,9 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
     float g1[6];
     double qm[7];
     float vm[6];
+    const float qh = (float)(0.25 * dt);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) g1[k] = 0.5f * (v[k] + vf[k]);
+    for (int k = 0; k < 6; ++k) g1[k] = cfg.warm_start == 2 ? (tr ? v[k] + qh * tr[k] : v[k]) : 0.5f * (v[k] + vf[k]);
     ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
     int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
     if (DUMP) {
@@ -501,6 +505,158 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
   if (isnan(err) || isinf(err)) err = 1.0e10;
   wk[0] += 1;
   return err;
+}
+
+// ---------------------------------------------------------------- two-wave tiles
+// A tile is one world run by the two waves of a 128-lane workgroup.  The first wave runs the world
+// as a single-wave world does; in each attempt the second wave solves the first half step while
+// the first solves the full step, both from q_t with the same contacts (the first wave's
+// contacts: the second wave copies the first wave's ContactLDS into its own) and separate working
+// sets (each wave's own BodyLDS and SolveLDS).  That
+// needs the first half step's Newton seed not to depend on the full step's result: warm_start 2.
+// Each solve is the same out-of-line world_solve, so a tiled world's results equal its
+// single-wave results bit for bit.  The waves meet at two workgroup barriers per attempt; the
+// first wave's LDS syncs elsewhere are wave-local.
+struct TileHdr {
+  int32_t cmd;   // 1: solve the first half step of world w; 0: no more tiles (the second wave leaves)
+  int32_t w, nc, ok;
+  double h, tol;
+  float opt_rel_tol;
+  long long wk[NA_WORK];  // the second wave's work counters of its solve
+};
+// Inputs of the second wave's solve, then its outputs, in the second wave's own segment / solve
+// area (read into registers before its solve writes that area; written after the solve).
+struct TileIn {
+  double q[7][NA_WAVE];
+  float v[6][NA_WAVE], f[6][NA_WAVE], g[6][NA_WAVE];
+};
+struct TileOut {
+  double q[7][NA_WAVE];
+  float v[6][NA_WAVE];
+};
+static_assert(sizeof(TileIn) <= sizeof(SolveLDS) && sizeof(TileOut) <= sizeof(SolveLDS),
+              "tile handoff fits the solve area");
+static_assert(sizeof(ContactLDS) % 16 == 0 && offsetof(WorldLDS, con) % 16 == 0 &&
+                  sizeof(MjContactParams<NA_MAX_CONTACTS>) % 16 == 0 && offsetof(WorldLDS, mj) % 16 == 0,
+              "contact copies move 16-byte words");
+__device__ __forceinline__ TileIn& tile_in(WorldLDS& S) { return *reinterpret_cast<TileIn*>(&S.sol); }
+__device__ __forceinline__ TileOut& tile_out(WorldLDS& S) { return *reinterpret_cast<TileOut*>(&S.sol); }
+
+// The first wave's step-doubling attempt of a tiled world (adaptive mode, warm_start 2): the
+// semantics and arithmetic of world_attempt, with the first half step solved by the second wave.
+__device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc,
+                                                  WorldLDS& S, WorldLDS& S1, TileHdr& T, const double* q,
+                                                  const float* v, const float* f, const float* tr, double dt,
+                                                  double* qf, float* vf, double* qd, float* vd, bool& ok,
+                                                  int& overflow, int& nc_out, long long* wk) {
+  const int l = lane_id();
+  const bool own = l < wc.nb;
+  int nc = world_collide(M, cfg, wc, S, q, overflow, wk);
+  nc_out = nc;
+  float g0[6], g1[6];
+  const float hh = (float)(0.5 * dt);
+  const float qh = (float)(0.25 * dt);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    g0[k] = tr ? v[k] + hh * tr[k] : v[k];
+    g1[k] = tr ? v[k] + qh * tr[k] : v[k];
+  }
+  TileIn& in = tile_in(S1);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) in.q[k][l] = q[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) { in.v[k][l] = v[k]; in.f[k][l] = f[k]; in.g[k][l] = g1[k]; }
+  if (l == 0) {
+    T.cmd = 1; T.w = wc.w; T.nc = nc; T.h = 0.5 * dt; T.tol = cfg.tol; T.opt_rel_tol = cfg.opt_rel_tol;
+  }
+  group_sync();  // the second wave starts the first half step
+  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
+  group_sync();  // its result is in
+  double qm[7];
+  float vm[6];
+  const TileOut& out = tile_out(S1);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) qm[k] = out.q[k][l];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) vm[k] = out.v[k][l];
+  ok = T.ok && ok;
+#pragma unroll
+  for (int k = 0; k < NA_WORK; ++k) wk[k] += T.wk[k];
+  int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
+  float g2[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) g2[k] = 2.f * vm[k] - v[k];
+  ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
+  double e = 0.0;
+  if (own) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      double d = fabs(qd[k] - qf[k]);
+      e = (d > e || d != d) ? d : e;
+    }
+  }
+  double err = wave_max_d(e);
+  if (isnan(err) || isinf(err)) err = 1.0e10;
+  wk[0] += 1;
+  return err;
+}
+
+// The second wave of a tile: solves first half steps on request until told to leave.  Its
+// barriers pair with the first wave's: one before each request (or the leave message), one after
+// each result.
+__device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, WorldLDS& S0, WorldLDS& S1,
+                                         TileHdr& T) {
+  const int l = lane_id();
+  for (;;) {
+    group_sync();
+    if (T.cmd != 1) break;
+    const WorldCtx wc = world_ctx(M, T.w);
+    NaConfig wcfg = cfg;
+    wcfg.tol = T.tol;
+    wcfg.opt_rel_tol = T.opt_rel_tol;
+    const int nc = T.nc;
+    const double h = T.h;
+    double q[7];
+    float v[6], f[6], g[6];
+    {
+      const TileIn& in = tile_in(S1);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) q[k] = in.q[k][l];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) { v[k] = in.v[k][l]; f[k] = in.f[k][l]; g[k] = in.g[k][l]; }
+    }
+    // the first wave's contacts into this wave's own ContactLDS (and MuJoCo parameters), so the
+    // solve reads the same values from this wave's WorldLDS
+    {
+      constexpr int n4 = (int)(sizeof(ContactLDS) / 16);
+      const int4* src = reinterpret_cast<const int4*>(&S0.con);
+      int4* dst = reinterpret_cast<int4*>(&S1.con);
+      for (int i = l; i < n4; i += NA_WAVE) dst[i] = src[i];
+      if (cfg.contact_law == 1) {
+        constexpr int m4 = (int)(sizeof(S0.mj) / 16);
+        const int4* ms = reinterpret_cast<const int4*>(&S0.mj);
+        int4* md = reinterpret_cast<int4*>(&S1.mj);
+        for (int i = l; i < m4; i += NA_WAVE) md[i] = ms[i];
+      }
+    }
+    wave_sync();  // the inputs are in registers and the contacts copied before the solve
+    long long wk[NA_WORK] = {};
+    double qo[7];
+    float vo[6];
+    const bool ok = world_solve(M, wcfg, wc, S1, nc, q, v, f, h, g, qo, vo, wk);
+    wave_sync();
+    TileOut& out = tile_out(S1);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out.q[k][l] = qo[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.v[k][l] = vo[k];
+    if (l == 0) {
+      T.ok = ok;
+#pragma unroll
+      for (int k = 0; k < NA_WORK; ++k) T.wk[k] = wk[k];
+    }
+    group_sync();
+  }
 }
 
 // Per-world controller scalars held in registers during a frame.
@@ -594,11 +750,14 @@ __device__ __forceinline__ bool controller_update(const NaConfig& cfg, WCtrl& c,
 }
 
 // One iteration of the boundary loop for one world (clamp, attempt, controller, commit).
-// q/v hold this lane's body state and are updated in place.  trend (warm_start 1, else null) is
+// q/v hold this lane's body state and are updated in place.  trend (warm_start 1 or 2, else null) is
 // the per-body acceleration array: read for the full step's Newton seed and, after an accepted
 // step, set to (v_new - v_old) / step.  Returns accepted-step count (0/1).
+// TILE: the world is a two-wave tile (S1, T: the second wave's LDS and the tile header).
+template <bool TILE = false>
 __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, WCtrl& c,
-                               double* q, float* v, const float* f, float* trend, double eff_max, long long* wk) {
+                               double* q, float* v, const float* f, float* trend, double eff_max, long long* wk,
+                               WorldLDS* S1 = nullptr, TileHdr* T = nullptr) {
   c.iterations += 1;
   const int l = lane_id();
   const bool own = l < wc.nb;
@@ -615,7 +774,11 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
     float tr[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) tr[k] = trend && own ? trend[6 * (wc.b0 + l) + k] : 0.f;
-    err = world_attempt(M, cfg, wc, S, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf, nc, wk);
+    if constexpr (TILE)
+      err = world_attempt_tile(M, cfg, wc, S, *S1, *T, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf,
+                               nc, wk);
+    else
+      err = world_attempt(M, cfg, wc, S, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf, nc, wk);
   } else {
 #pragma unroll
     for (int k = 0; k < 7; ++k) { qf[k] = q[k]; qd[k] = q[k]; }
@@ -756,9 +919,12 @@ struct ServeSlots {
 // loop runs at most max_substeps iterations per frame.
 // One world's turn of a serving launch: boundary frames until the world's horizon, a solve
 // failure, max_iters iterations (<= 0: no limit) or the device clock passing t_end.
+// TILE: the world runs as a two-wave tile (S1, T: the second wave's LDS and the tile header).
+template <bool TILE = false>
 __device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64, float* body_q,
                             float* body_qd, const float* body_f, double dt_outer, double eff_max, const ServeSlots& SV,
-                            int w, long long t_end, int max_iters, int32_t* unfinished, WorldLDS& S) {
+                            int w, long long t_end, int max_iters, int32_t* unfinished, WorldLDS& S,
+                            WorldLDS* S1 = nullptr, TileHdr* T = nullptr) {
   WorldCtx wc = world_ctx(M, w);
   NaConfig wcfg = cfg;
   if (SV.tol) {
@@ -782,7 +948,7 @@ __device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, 
       if (frames >= hf) { status = 1; break; }
       open_frame(wcfg, c, dt_outer, eff_max);
     }
-    int inc = world_iteration(M, wcfg, wc, S, c, q, v, f, K.trend, eff_max, wk);
+    int inc = world_iteration<TILE>(M, wcfg, wc, S, c, q, v, f, K.trend, eff_max, wk, S1, T);
     acc += inc;
     accepted += inc;
     if (c.solve_fail) { status = 2; break; }
@@ -820,6 +986,62 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
     const int w = active[slot];
     if (SV.status[w] != 0) continue;
     serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, S);
+  }
+}
+
+// Served launch with two-wave tiles: workgroups of two waves.  The first n_tiled entries of the
+// list run as tiles (both waves on one world, see world_attempt_tile); once a workgroup's first
+// wave takes an entry past them, it tells the second wave to leave the tile loop and both waves
+// then take worlds on their own, each with its own WorldLDS, exactly as k_serve's waves do.  The
+// list counter only grows, so no tile entry remains once any wave has passed them.  Every wave
+// exits: the tile loop ends when the list is exhausted, the budget has passed or an entry past
+// the tiles comes up, and each path sends the second wave its leave message once.
+__global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                                    float* body_qd, const float* body_f, double dt_outer,
+                                                    double eff_max, int32_t* queue, const int32_t* active,
+                                                    int n_active, int n_tiled, ServeSlots SV, long long budget,
+                                                    int max_iters, int32_t* unfinished) {
+  __shared__ WorldLDS S[2];
+  __shared__ TileHdr T;
+  const int wave = threadIdx.x >> 6;
+  const long long t_end = wall_clock64() + budget;
+  if (wave == 1) {
+    tile_helper(M, cfg, S[0], S[1], T);
+  } else {
+    for (;;) {
+      int slot = 0;
+      if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
+      slot = __shfl(slot, 0, NA_WAVE);
+      if (slot >= n_tiled || slot >= n_active) {
+        if (lane_id() == 0) T.cmd = 0;
+        group_sync();  // the second wave leaves the tile loop
+        if (slot < n_active) {
+          const int w = active[slot];
+          if (SV.status[w] == 0)
+            serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
+                        unfinished, S[0]);
+        }
+        break;
+      }
+      const int w = active[slot];
+      if (SV.status[w] != 0) continue;
+      serve_world<true>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
+                        unfinished, S[0], &S[1], &T);
+    }
+  }
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(queue, 1) : n_active;
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= n_active) break;
+    const int w = active[slot];
+    if (SV.status[w] != 0) continue;
+    if (wave == 0)
+      serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
+                  S[0]);
+    else
+      serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
+                  S[1]);
   }
 }
 
@@ -1164,7 +1386,8 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
 #pragma unroll
     for (int k = 0; k < 6; ++k) g64[k] = own ? guess[6 * (wc.b0 + l) + k] : 0.0;
   }
-  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st, guess ? g64 : nullptr, v64);
+  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st,
+                        guess ? g64 : nullptr, v64);
   if (own) {
     int gb = wc.b0 + l;
 #pragma unroll
@@ -1739,10 +1962,11 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
       return fail("na_solver_create: contact_law 1 (MuJoCo soft constraints) is built for free-body models only");
     if (!(c->impratio > 0)) return fail("na_solver_create: impratio must be > 0");
   }
-  if (c->warm_start != 0 && c->warm_start != 1)
-    return fail("na_solver_create: warm_start must be 0 (reference seeds) or 1 (extrapolated seeds)");
-  if (c->warm_start == 1 && buffers && !buffers->body_trend)
-    return fail("na_solver_create: warm_start 1 with supplied buffers needs body_trend");
+  if (c->warm_start < 0 || c->warm_start > 2)
+    return fail("na_solver_create: warm_start must be 0 (reference seeds), 1 (extrapolated seeds) or 2 "
+                "(concurrent seeds)");
+  if (c->warm_start >= 1 && buffers && !buffers->body_trend)
+    return fail("na_solver_create: warm_start 1 or 2 with supplied buffers needs body_trend");
   na_solver* s = new na_solver();
   s->model = m;
   s->cfg = to_cfg(c);
@@ -1781,7 +2005,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     K.accepted = b->accepted; K.commit = b->commit; K.diverged = b->diverged; K.substeps = b->substeps;
     K.iterations = b->iterations; K.solve_fail = b->solve_fail; K.overflow = b->overflow;
     K.cum = b->cum_accepted; K.work = b->work; s->q64 = b->body_q64;
-    K.trend = c->warm_start == 1 ? b->body_trend : nullptr;
+    K.trend = c->warm_start >= 1 ? b->body_trend : nullptr;
   } else {
     K.dt = (double*)take(8u * W); K.dt_half = (double*)take(8u * W); K.ideal_dt = (double*)take(8u * W);
     K.sim_time = (double*)take(8u * W); K.next_time = (double*)take(8u * W); K.last_error = (double*)take(8u * W);
@@ -1797,7 +2021,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   s->flag = (int32_t*)take(4);
   if (!buffers) {
     float* tr = (float*)take(sizeof(float) * 6 * (size_t)(B > 0 ? B : 1));
-    K.trend = c->warm_start == 1 ? tr : nullptr;
+    K.trend = c->warm_start >= 1 ? tr : nullptr;
   }
   if (hipHostMalloc(&s->flag_host, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
     (void)hipFree(pool);
@@ -1806,6 +2030,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   }
   s->order = nullptr;
   s->grid_waves = 0;
+  s->tiles = 0;
   // controller defaults (reset of all worlds)
   k_reset<<<blocks(W, 256), 256>>>(m->M, nullptr, s->cfg.dt_init, K);
   HIPCHK(hipGetLastError());
@@ -1834,6 +2059,16 @@ extern "C" int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_
   if (!s) return fail("na_solver_set_schedule: null solver");
   s->grid_waves = grid_waves;
   s->order = order;
+  return 0;
+}
+
+extern "C" int na_solver_set_tiles(na_solver* s, int32_t tiles) {
+  if (!s) return fail("na_solver_set_tiles: null solver");
+  if (tiles < 0) return fail("na_solver_set_tiles: tiles must be >= 0");
+  if (tiles > 0 && (s->cfg.mode != 1 || s->cfg.warm_start != 2 || s->cfg.contact_law != 0))
+    return fail("na_solver_set_tiles: two-wave tiles need adaptive mode, warm_start 2 (concurrent seeds) and the "
+                "SAP contact law");
+  s->tiles = tiles;
   return 0;
 }
 
@@ -1883,9 +2118,26 @@ extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, c
   double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   ServeSlots SV{horizon_frames, horizon_accepted, frames_done, accepted, status, world_tol};
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
+  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
+  if (s->tiles > 0) {
+    // workgroups of two waves: as many as fit, each taking two worlds at once after the tiles
+    int dev = 0, cus = 256, per_cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_serve_tile, 128, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    int grid = cus * per_cu;
+    if (s->grid_waves > 0) grid = (s->grid_waves + 1) / 2;
+    const int need = (n_active + 1) / 2 > s->tiles ? (n_active + 1) / 2 : s->tiles;
+    grid = grid < need ? grid : need;
+    k_serve_tile<<<grid, 128, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
+                                       active, n_active, s->tiles, SV, budget, max_iterations, unfinished);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   int grid = default_grid(s);
   grid = grid < n_active ? grid : n_active;
-  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
   k_serve<<<grid, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue, active,
                                n_active, SV, budget, max_iterations, unfinished);
   HIPCHK(hipGetLastError());
@@ -2091,7 +2343,7 @@ extern "C" int na_attempt(const na_solver* s, const double* q64, const float* qd
   const NaModel& M = s->model->M;
   k_attempt<<<M.W, 64, 0, S_(stream)>>>(M, s->cfg, q64, qd, f, dt, q_full, qd_full, q_dbl, qd_dbl, err, solve_ok,
                                         n_contacts, contacts_out, contact_counts_out,
-                                        s->cfg.warm_start == 1 ? trend : nullptr);
+                                        s->cfg.warm_start >= 1 ? trend : nullptr);
   HIPCHK(hipGetLastError());
   return 0;
 }

@@ -77,6 +77,7 @@ class RolloutRequest:
     body_qd: np.ndarray | None = None
     joint_q: np.ndarray | None = None
     joint_qd: np.ndarray | None = None
+    cost_estimate: float = math.nan  # remaining work from the measured cost so far (tiles pick by it)
     slot: int = -1
     b0: int = -1
     s0: int = -1
@@ -171,24 +172,30 @@ def requests_from_model(model, *, horizon_frames=None, horizon_accepted=None, to
     return out
 
 
-def autotune_served(run, waves_per_cu=(4, 3, 2)) -> dict:
-    """Pick the persistent grid of a served rollout launch by timing it on the device.
+def autotune_served(run, candidates=None) -> dict:
+    """Pick the schedule of a served rollout by timing it on the device.
 
-    ``run(grid_waves)`` performs one served rollout launch of a fixed set of requests from the same
-    start (a fresh engine each time) with that persistent grid and returns ``(launch_ms,
-    signature)``, ``signature`` a tensor of the rollout's results.  A grid decides only which wave
-    runs a world, never its arithmetic, so every candidate must return the same signature bit for
-    bit; a mismatch raises.  Returns ``{"best": row, "candidates": rows}`` with rows of
-    ``waves_per_cu``, ``grid_waves`` and ``ms``."""
+    A candidate is a dict with ``waves_per_cu`` (the persistent grid), ``tiles`` (how many worlds
+    of each launch run as two-wave tiles, the ones with the most measured work left) and
+    ``round_ms`` (device time per launch; 0: one launch for the whole rollout).  ``run(grid_waves,
+    tiles, round_ms)`` performs the served rollout of a fixed set of requests from the same start
+    (a fresh engine each time) and returns ``(ms, signature)``, ``signature`` a tensor of the
+    rollout's results.  A schedule decides only which waves run a world and when, never its
+    arithmetic, so every candidate must return the same signature bit for bit; a mismatch raises.
+    Returns ``{"best": row, "candidates": rows}``, each row the candidate with ``grid_waves`` and
+    ``ms``."""
+    if candidates is None:
+        candidates = [{"waves_per_cu": w, "tiles": 0, "round_ms": 0.0} for w in (4, 3, 2)]
     cus = torch.cuda.get_device_properties(0).multi_processor_count
     rows, ref = [], None
-    for wpc in waves_per_cu:
-        ms, sig = run(int(wpc) * cus)
+    for c in candidates:
+        grid = int(c["waves_per_cu"]) * cus
+        ms, sig = run(grid, int(c.get("tiles", 0)), float(c.get("round_ms", 0.0)))
         if ref is None:
             ref = sig
         elif not torch.equal(sig, ref):
-            raise RuntimeError("autotune_served: grids gave different results (a schedule changed arithmetic)")
-        rows.append({"waves_per_cu": int(wpc), "grid_waves": int(wpc) * cus, "ms": float(ms)})
+            raise RuntimeError("autotune_served: schedules gave different results (a schedule changed arithmetic)")
+        rows.append(dict(c, grid_waves=grid, ms=float(ms)))
     return {"best": min(rows, key=lambda r: r["ms"]), "candidates": rows}
 
 
@@ -444,6 +451,7 @@ class _Snapshot:
     event: torch.cuda.Event
     bufs: list  # pinned status, accepted, frames_done [W]
     live: list  # the requests listed in that launch
+    work: torch.Tensor | None = None  # pinned per-slot work counters [W, NA_WORK] (engines with tiles)
 
 
 @dataclass
@@ -476,6 +484,10 @@ class ServingEngine:
     launch (<= 0: a launch runs every resident world to its horizon).  Remaining keyword
     arguments go to the solver (``tol`` is the default per-request tolerance).
 
+    ``tiles``: the first ``tiles`` worlds of each launch's list (the oldest requests, largest
+    first) run as two-wave tiles (``SolverSAPAdaptive.set_tiles``; needs
+    ``warm_start="concurrent"``).
+
     ``schedule="continuous"`` is the served form described in the module docstring.
     ``schedule="static"`` is the fixed-batch baseline with the reference's host loop: when the
     pool is empty it admits every arrived request that fits, then advances the whole batch in
@@ -488,7 +500,7 @@ class ServingEngine:
     def __init__(self, globals_model, *, max_worlds: int = 4096, body_capacity: int | None = None,
                  shape_capacity: int | None = None, hbm_budget_bytes: int | None = None, dt_outer: float = 0.01,
                  budget_ms: float = 20.0, controller: str = "sap", schedule: str = "continuous", device="cuda:0",
-                 articulated: bool = False, split_kinds: bool = False, **solver_kw):
+                 articulated: bool = False, split_kinds: bool = False, tiles: int = 0, **solver_kw):
         if schedule not in ("continuous", "static"):
             raise ValueError(f"schedule must be 'continuous' or 'static', got {schedule!r}")
         self.schedule = schedule
@@ -510,6 +522,8 @@ class ServingEngine:
         solver_kw.setdefault("max_substeps", 256)
         solver_kw.setdefault("world_order", "index")
         self.solver = cls(self.pool, **solver_kw)
+        if tiles:
+            self.solver.set_tiles(tiles)
         self.default_tol = self.solver._tol
         d = self.pool.device
         W = self.pool.world_count
@@ -536,6 +550,9 @@ class ServingEngine:
         self._unfinished = torch.zeros(1, dtype=torch.int32, device=d)
         self._unfinished_host = torch.zeros(1, dtype=torch.int32, pin_memory=True)
         self._snap_bufs = [[torch.zeros(W, dtype=torch.int32, pin_memory=True) for _ in range(3)] for _ in range(2)]
+        self._snap_work = [torch.zeros(tuple(self.solver._work.shape), dtype=torch.int64, pin_memory=True)
+                           for _ in range(2)] if tiles else None
+        self.tiles = int(tiles)
         self._snap_flip = 0
         self._last_live: list = []
         self._launch_events: list = []
@@ -647,6 +664,7 @@ class ServingEngine:
         put(self.frames_done, zeros, torch.int32)
         put(self.accepted, zeros, torch.int32)
         put(self.status, zeros, torch.int32)
+        self.solver._work.index_fill_(0, slots, 0)  # the slot's work counters count this request only
         mask = torch.zeros(P.world_count, dtype=torch.uint8, device=d)
         mask.index_fill_(0, slots, 1)
         self._reset_mask = mask  # kept alive until the asynchronous reset has run
@@ -694,6 +712,14 @@ class ServingEngine:
                       key=lambda r: (r.arrival, -(r.body_count + r.shape_count), r.rid))
         if not live:
             return 0
+        if self.tiles and not self.pool.is_articulated:
+            # the worlds with the most estimated work left go first and run as two-wave tiles (before
+            # any estimate exists: the largest); the tiles only change which waves run a world
+            known = [r for r in live if r.cost_estimate == r.cost_estimate]
+            top = sorted(known, key=lambda r: (-r.cost_estimate, r.rid))[:self.tiles] if known else live[:self.tiles]
+            ids = {id(r) for r in top}
+            live = top + [r for r in live if id(r) not in ids]
+            self.solver.set_tiles(len(top))
         P = self.pool
         L = _lib.lib()
         budget = self.budget_ms if max_iterations <= 0 else 0.0
@@ -773,12 +799,15 @@ class ServingEngine:
         """Enqueue copies of the per-world status, accepted and frame counters of the launch just
         enqueued into pinned host buffers (two sets, used alternately) and an event after them."""
         bufs = self._snap_bufs[self._snap_flip]
+        work = self._snap_work[self._snap_flip] if self._snap_work is not None else None
         self._snap_flip ^= 1
         for host, dev in zip(bufs, (self.status, self.accepted, self.frames_done)):
             host.copy_(dev, non_blocking=True)
+        if work is not None:
+            work.copy_(self.solver._work, non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(self._stream)
-        return _Snapshot(ev, bufs, list(self._last_live))
+        return _Snapshot(ev, bufs, list(self._last_live), work)
 
     def _retire_snapshot(self, snap: _Snapshot, now: float) -> list:
         """Retire the requests of ``snap``'s launch that had finished by its end.  Their final
@@ -786,6 +815,8 @@ class ServingEngine:
         finished worlds, so the states are final) into pinned buffers; the slots and arena ranges
         are freed at once, since any later upload into them is enqueued behind the gathers."""
         status, acc, frames = (b.numpy() for b in snap.bufs)
+        if snap.work is not None:
+            self._estimate(snap, status, acc, frames)
         done = [r for r in snap.live if status[r.slot] != 0 and self.resident.get(r.slot) is r]
         if not done:
             return []
@@ -828,6 +859,21 @@ class ServingEngine:
             self.finished.append(r)
         return done
 
+    def _estimate(self, snap: _Snapshot, status, acc, frames) -> None:
+        """Remaining work of each running request of ``snap``'s launch, from its measured cost so far
+        (PCG iterations + 5 per Newton iteration + 10 per attempt, the relative costs of those phases)
+        per frame (or per accepted step, for a step horizon) times what is left of its horizon."""
+        w = snap.work.numpy()
+        for r in snap.live:
+            if status[r.slot] != 0 or self.resident.get(r.slot) is not r:
+                continue
+            c = float(w[r.slot, 2] + 5 * w[r.slot, 1] + 10 * w[r.slot, 0])
+            if r.horizon_frames is not None:
+                done, left = int(frames[r.slot]), r.horizon_frames - int(frames[r.slot])
+            else:
+                done, left = int(acc[r.slot]), r.horizon_accepted - int(acc[r.slot])
+            r.cost_estimate = c / max(done, 1) * max(left, 0)
+
     def _collect_copyouts(self, block: bool) -> None:
         """Hand the gathered final states to their requests once the gathers have run."""
         keep = []
@@ -866,7 +912,8 @@ class ServingEngine:
         self._account_launches()
         return done
 
-    def run(self, requests=None, *, realtime: bool = True, max_wall_s: float | None = None) -> dict:
+    def run(self, requests=None, *, realtime: bool = True, max_wall_s: float | None = None,
+            pipelined: bool = True) -> dict:
         """Serve ``requests`` (plus anything already queued) to completion.
 
         With ``realtime`` the requests' ``arrival`` times are seconds after the start of this call
@@ -876,7 +923,8 @@ class ServingEngine:
         Launches are pipelined: the next round's launch is enqueued before the host waits for the
         previous one, so the device never waits for the host's retirement and admission work.
         A launch lists every resident world the host has not yet seen finish; a world that
-        finished in the launch before is skipped on the device.
+        finished in the launch before is skipped on the device.  With ``pipelined=False`` each
+        round waits for the previous one, so its order and tiles use the latest cost estimates.
         """
         if requests is not None:
             self.submit(requests)
@@ -892,6 +940,8 @@ class ServingEngine:
         prev = None
         while self.pending or self.resident or prev is not None:
             snap = self._snapshot() if self.resident and self.launch() else None
+            if not pipelined and snap is not None:
+                prev, snap = snap, None
             if prev is not None:
                 prev.event.synchronize()
                 self._retire_snapshot(prev, time.perf_counter() - t0)

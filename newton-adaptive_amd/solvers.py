@@ -32,7 +32,7 @@ _LINE_SEARCHES = {"exact_root": 0, "armijo_decay": 1}
 # (v_t + v_full) / 2, v_full); "extrapolate": v_t + dt/2 a (a: the body's mean acceleration over
 # its last committed step), (v_t + v_full) / 2, 2 v_mid - v_t.  Same converged solves, fewer
 # Newton iterations on settled piles.
-WARM_STARTS = {"reference": 0, "extrapolate": 1}
+WARM_STARTS = {"reference": 0, "extrapolate": 1, "concurrent": 2}
 _CONTROL_FIELDS = ("joint_f", "joint_target_q", "joint_target_qd", "joint_act")
 _WORK_FIELDS = ("attempts", "newton_iterations", "cg_iterations", "contacts", "pairs", "solves",
                 "cycles_collide", "cycles_setup", "cycles_residual", "cycles_preconditioner", "cycles_pcg",
@@ -123,6 +123,7 @@ class _SolverBase:
         cfg.impratio = float(_impratio)
         cfg.warm_start = WARM_STARTS[warm_start]
         self.warm_start = warm_start
+        self.tiles = 0
         self.line_search_variant = line_search_variant
         self._cfg = cfg
 
@@ -197,6 +198,20 @@ class _SolverBase:
         _lib.check(_lib.lib().na_solver_set_schedule(self._h, int(grid_waves),
                                                      order.data_ptr() if order is not None else None),
                    "na_solver_set_schedule")
+
+    def set_tiles(self, tiles: int) -> None:
+        """Run the first ``tiles`` worlds of each served launch's list (``na_serve``: the serving
+        engine's rollouts, largest worlds first) as two-wave tiles: two wavefronts per world, one
+        solving each attempt's full step while the other solves its first half step.  Needs
+        ``warm_start="concurrent"`` (the first half step's Newton seed must not depend on the full
+        step's result).  A world's results are the same with and without tiles, bit for bit."""
+        tiles = int(tiles)
+        if tiles < 0:
+            raise ValueError("tiles must be >= 0")
+        if tiles and self.warm_start != "concurrent":
+            raise ValueError("two-wave tiles need warm_start='concurrent'")
+        _lib.check(_lib.lib().na_solver_set_tiles(self._h, tiles), "na_solver_set_tiles")
+        self.tiles = tiles
 
     # ------------------------------------------------------------------ boundary call
     def _control_desc(self, control):

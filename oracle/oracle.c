@@ -145,7 +145,7 @@ typedef struct or_config {
   int line_search;     /* 0 exact_root, 1 armijo_decay (solver_sap_adaptive.py:459) */
   int contact_law;     /* 0 SAP compliant cone, 1 MuJoCo soft constraints, pyramidal cone */
   double impratio;     /* MuJoCo frictional-to-normal impedance ratio (law 1) */
-  int warm_start;      /* Newton seeds of an attempt: 0 the reference's, 1 extrapolated (see attempt) */
+  int warm_start;      /* Newton seeds of an attempt: 0 the reference's, 1 extrapolated, 2 concurrent (see attempt) */
 } or_config;
 
 enum { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
@@ -1647,7 +1647,7 @@ typedef struct {
   uint8_t *accepted, *commit, *diverged;
   int32_t *substeps, *iterations, *solve_fail;
   int64_t* cum_accepted;
-  double* trend;       /* [B*6] mean acceleration of each body over its last accepted step (warm_start 1;
+  double* trend;       /* [B*6] mean acceleration of each body over its last accepted step (warm_start 1, 2;
                           NULL: zero) */
 } or_ctrl;
 
@@ -1673,8 +1673,9 @@ static void contacts_from_records(const double* rec, int n, or_contact* out) {
    Newton seeds (solver_sap_adaptive.py:756-771 for warm_start 0): the full step from qd, the
    first half step from (qd + v_full) / 2, the second half step from v_full.  warm_start 1 seeds
    the full step at qd + dt/2 trend (trend: the world's [nb*6] mean accelerations over its last
-   accepted step, NULL = zero) and the second half step at 2 v_mid - qd; the problems solved are
-   the same. */
+   accepted step, NULL = zero) and the second half step at 2 v_mid - qd; warm_start 2 also seeds
+   the first half step at qd + dt/4 trend instead of from v_full; the problems solved are the
+   same. */
 static double attempt(const world_view* W, const or_config* C, const double* q, const double* qd, const double* f,
                       const double* trend, double dt, double* q_full, double* qd_full, double* q_dbl,
                       double* qd_dbl, int* solve_ok, int* n_contacts, const or_contact* given,
@@ -1696,11 +1697,12 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
   }
   if (nc > cap) nc = cap;
   if (n_contacts) *n_contacts = nc;
-  for (int i = 0; i < N; ++i) guess[i] = C->warm_start == 1 && trend ? qd[i] + 0.5 * dt * trend[i] : qd[i];
+  for (int i = 0; i < N; ++i) guess[i] = C->warm_start >= 1 && trend ? qd[i] + 0.5 * dt * trend[i] : qd[i];
   ok &= inner_step(W, C, q, qd, f, dt, cs, nc, guess, q_full, qd_full, vfull, &st);
   double err = 0.0;
   if (C->mode == 1) {
-    for (int i = 0; i < N; ++i) guess[i] = 0.5 * (qd[i] + vfull[i]);
+    for (int i = 0; i < N; ++i)
+      guess[i] = C->warm_start == 2 ? (trend ? qd[i] + 0.25 * dt * trend[i] : qd[i]) : 0.5 * (qd[i] + vfull[i]);
     ok &= inner_step(W, C, q, qd, f, 0.5 * dt, cs, nc, guess, q_mid, qd_mid, NULL, &st);
     if (given) {
       nc = given_n[1] < cap ? given_n[1] : cap;
@@ -1710,7 +1712,7 @@ static double attempt(const world_view* W, const or_config* C, const double* q, 
     }
     if (nc > cap) nc = cap;
     if (n_contacts_mid) *n_contacts_mid = nc;
-    if (C->warm_start == 1)
+    if (C->warm_start >= 1)
       for (int i = 0; i < N; ++i) vfull[i] = 2.0 * qd_mid[i] - qd[i];  /* the second half step's seed */
     ok &= inner_step(W, C, q_mid, qd_mid, f, 0.5 * dt, cs, nc, vfull, q_dbl, qd_dbl, NULL, &st);
     for (int i = 0; i < 7 * nb; ++i) {
@@ -1863,7 +1865,7 @@ int or_step_frame(const or_model* M, const or_config* C, double* q, double* qd, 
       double step = ctrl_clamp(K, w);
       int ok = 1, nc;
       double err;
-      double* tw = K->trend && C->warm_start == 1 ? K->trend + 6 * b0 : NULL;
+      double* tw = K->trend && C->warm_start >= 1 ? K->trend + 6 * b0 : NULL;
       if (step > 0) err = attempt(&W, C, qw, qdw, fw, tw, step, qf, qdf, qdb, qddb, &ok, &nc, NULL, NULL, NULL);
       else { err = 0; memcpy(qdb, qw, sizeof(double) * 7 * nb); memcpy(qddb, qdw, sizeof(double) * 6 * nb);
              memcpy(qf, qw, sizeof(double) * 7 * nb); memcpy(qdf, qdw, sizeof(double) * 6 * nb); }

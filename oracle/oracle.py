@@ -318,7 +318,7 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     """Oracle settings.  warm_start: the Newton seeds of an attempt's three solves, "reference"
     (v_t, (v_t + v_full) / 2, v_full: solver_sap_adaptive.py:756-771) or "extrapolate" (v_t + dt/2 a
     with a the body's mean acceleration over its last accepted step, (v_t + v_full) / 2,
-    2 v_mid - v_t)."""
+    2 v_mid - v_t) or "concurrent" (as "extrapolate" with the first half step from v_t + dt/4 a)."""
     c = OrConfig()
     c.mode = 1 if mode == "adaptive" else 0
     c.controller = 0 if controller == "sap" else 1
@@ -330,7 +330,7 @@ def make_config(mode="adaptive", controller="sap", tol=1e-3, dt_init=0.01, dt_mi
     c.line_search = {"exact_root": 0, "armijo_decay": 1}[line_search]
     c.contact_law = {"sap": 0, "mujoco": 1}[contact_law]
     c.impratio = float(impratio)
-    c.warm_start = {"reference": 0, "extrapolate": 1}[warm_start]
+    c.warm_start = {"reference": 0, "extrapolate": 1, "concurrent": 2}[warm_start]
     return c
 
 

@@ -158,6 +158,49 @@ def test_served_rollout_equals_boundary_calls(budget_ms):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tiles,budget_ms,pipelined", [(4, 0.0, True), (16, 0.5, False), (64, 0.3, True)])
+def test_tiled_rollout_equals_boundary_calls(tiles, budget_ms, pipelined):
+    """Two-wave tiles (the full step and the first half step solved at once by two waves) change
+    which waves run a world, never its arithmetic: with the concurrent seeds a tiled served rollout
+    equals the single-wave boundary calls bit for bit, whichever worlds the tiles take (the largest
+    first, then those with the most measured work left), across budget interruptions."""
+    model = na.scenes.uneven_pile(48, seed=11, device=DEV, min_prims=4, max_prims=24)
+    frames = 6
+    kw = dict(KW, warm_start="concurrent")
+    q_ref, qd_ref, acc_ref = _boundary_calls(model, frames, **kw)
+    eng = na.ServingEngine(model, max_worlds=48, body_capacity=1200, shape_capacity=1200, budget_ms=budget_ms,
+                           device=DEV, tiles=tiles, **kw)
+    reqs = na.requests_from_model(model, horizon_frames=frames)
+    summary = eng.run(reqs, realtime=False, pipelined=pipelined)
+    print("tiled summary", tiles, budget_ms, pipelined, summary, flush=True)
+    assert summary["completed"] == 48 and summary["failed"] == 0
+    if budget_ms > 0:
+        assert summary["rounds"] > 1
+    wb = model.host("world_body_start")
+    for r in reqs:
+        b0, b1 = int(wb[r.rid]), int(wb[r.rid + 1])
+        assert r.frames == frames
+        assert np.array_equal(r.body_q, q_ref[b0:b1]), f"world {r.rid}: pose differs"
+        assert np.array_equal(r.body_qd, qd_ref[b0:b1]), f"world {r.rid}: velocity differs"
+    assert sum(r.accepted for r in reqs) == acc_ref
+
+
+@pytest.mark.gpu
+def test_tiles_need_concurrent_seeds():
+    """Tiles need the first half step's seed to be independent of the full step: the host layer and
+    the C-ABI both refuse them under the other seeds."""
+    model = na.scenes.uneven_pile(4, seed=3, device=DEV, max_prims=6)
+    solver = na.SolverSAPAdaptive(model, **KW)  # extrapolated seeds
+    with pytest.raises(ValueError):
+        solver.set_tiles(4)
+    lib = importlib.import_module("newton-adaptive_amd._lib")
+    assert lib.lib().na_solver_set_tiles(solver._h, 4) < 0
+    ok = na.SolverSAPAdaptive(model, warm_start="concurrent", **KW)
+    ok.set_tiles(4)
+    assert ok.tiles == 4
+
+
+@pytest.mark.gpu
 def test_request_alone_equals_in_mix():
     model = na.scenes.uneven_pile(40, seed=21, device=DEV, max_prims=16)
     tols = [1e-2, 1e-3, 1e-4, 1e-3] * 10
