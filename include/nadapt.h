@@ -203,9 +203,10 @@ int na_solver_telemetry(const na_solver* s, na_telemetry* out);
 /* Schedule: persistent grid of `grid_waves` wavefronts (<=0: automatic) pulling worlds
  * in `order` (device [W] permutation, or NULL for natural order). */
 int na_solver_set_schedule(na_solver* s, int grid_waves, const int32_t* order);
-/* Two-wave tiles of the served launch (na_serve): the first `tiles` entries of its world list each
- * run on a workgroup of two wavefronts that solve an attempt's full step and first half step at
- * the same time; the rest of the list runs one world per wavefront.  0 (default): no tiles.
+/* Two-wave tiles: the first `tiles` entries of a served launch's world list (na_serve) or of a
+ * boundary call's visiting order (na_step; under the cost order, the previous call's costliest
+ * worlds) each run on a workgroup of two wavefronts that solve an attempt's full step and first
+ * half step at the same time; the rest runs one world per wavefront.  0 (default): no tiles.
  * Needs adaptive mode, warm_start 2 and contact_law 0; a tiled world's results equal its
  * single-wave results bit for bit.  Replaces the reference's single captured iteration body
  * for every world (solver_sap_adaptive.py:851-907) with a per-world schedule. */

@@ -861,6 +861,35 @@ __device__ __forceinline__ void flush_work(const Ctrl& K, int w, const long long
 }
 
 // ---------------------------------------------------------------- served boundary call
+// One world's boundary frame: iterations from open_frame until the boundary, max_substeps or a
+// solve failure.  TILE: the world runs as a two-wave tile (S1, T: the second wave's LDS and the
+// tile header).
+template <bool TILE = false>
+__device__ __forceinline__ void frame_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64,
+                                            float* body_q, float* body_qd, const float* body_f, const NaControl& U,
+                                            double dt_outer, double eff_max, int w, WorldLDS& S,
+                                            WorldLDS* S1 = nullptr, TileHdr* T = nullptr) {
+  WorldCtx wc = world_ctx(M, w);
+  double q[7];
+  float v[6], f[6];
+  load_body(wc, q64, body_q, body_qd, body_f, q, v, f, U.jf);
+  WCtrl c = load_ctrl(K, w);
+  open_frame(cfg, c, dt_outer, eff_max);
+  long long wk[NA_WORK] = {};
+  int acc = 0;
+  for (int it = 0; it < cfg.max_substeps; ++it) {
+    if (!(c.sim_time < c.next_time)) break;
+    acc += world_iteration<TILE>(M, cfg, wc, S, c, q, v, f, K.trend, eff_max, wk, S1, T);
+    if (c.solve_fail) break;
+  }
+  store_body(wc, q64, body_q, body_qd, q, v);
+  if (lane_id() == 0) {
+    store_ctrl(K, w, c);
+    if (acc) atomicAdd(K.cum, (unsigned long long)acc);
+  }
+  flush_work(K, w, wk);
+}
+
 __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
                                               float* body_qd, const float* body_f, NaControl U, double dt_outer,
                                               double eff_max, int32_t* queue, const int32_t* order) {
@@ -870,26 +899,50 @@ __global__ __launch_bounds__(64) void k_frame(NaModel M, NaConfig cfg, Ctrl K, d
     if (lane_id() == 0) slot = atomicAdd(queue, 1);
     slot = __shfl(slot, 0, NA_WAVE);
     if (slot >= M.W) break;
-    int w = order ? order[slot] : slot;
-    WorldCtx wc = world_ctx(M, w);
-    double q[7];
-    float v[6], f[6];
-    load_body(wc, q64, body_q, body_qd, body_f, q, v, f, U.jf);
-    WCtrl c = load_ctrl(K, w);
-    open_frame(cfg, c, dt_outer, eff_max);
-    long long wk[NA_WORK] = {};
-    int acc = 0;
-    for (int it = 0; it < cfg.max_substeps; ++it) {
-      if (!(c.sim_time < c.next_time)) break;
-      acc += world_iteration(M, cfg, wc, S, c, q, v, f, K.trend, eff_max, wk);
-      if (c.solve_fail) break;
+    const int w = order ? order[slot] : slot;
+    frame_world(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, w, S);
+  }
+}
+
+// The boundary call with two-wave tiles: the first n_tiled worlds of the visiting order (the
+// previous call's costliest under the cost order) run as tiles, as in k_serve_tile; after them
+// both waves of a workgroup take worlds on their own.  Same exit argument as k_serve_tile.
+__global__ __launch_bounds__(128) void k_frame_tile(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                                    float* body_qd, const float* body_f, NaControl U,
+                                                    double dt_outer, double eff_max, int32_t* queue,
+                                                    const int32_t* order, int n_tiled) {
+  __shared__ WorldLDS S[2];
+  __shared__ TileHdr T;
+  const int wave = threadIdx.x >> 6;
+  if (wave == 1) {
+    tile_helper(M, cfg, S[0], S[1], T);
+  } else {
+    for (;;) {
+      int slot = 0;
+      if (lane_id() == 0) slot = atomicAdd(queue, 1);
+      slot = __shfl(slot, 0, NA_WAVE);
+      if (slot >= n_tiled || slot >= M.W) {
+        if (lane_id() == 0) T.cmd = 0;
+        group_sync();  // the second wave leaves the tile loop
+        if (slot < M.W)
+          frame_world(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, order ? order[slot] : slot,
+                      S[0]);
+        break;
+      }
+      frame_world<true>(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, order ? order[slot] : slot,
+                        S[0], &S[1], &T);
     }
-    store_body(wc, q64, body_q, body_qd, q, v);
-    if (lane_id() == 0) {
-      store_ctrl(K, w, c);
-      if (acc) atomicAdd(K.cum, (unsigned long long)acc);
-    }
-    flush_work(K, w, wk);
+  }
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = atomicAdd(queue, 1);
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= M.W) break;
+    const int w = order ? order[slot] : slot;
+    if (wave == 0)
+      frame_world(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, w, S[0]);
+    else
+      frame_world(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, w, S[1]);
   }
 }
 
@@ -2072,6 +2125,20 @@ extern "C" int na_solver_set_tiles(na_solver* s, int32_t tiles) {
   return 0;
 }
 
+// Grid of a two-wave tile launch over n listed worlds: as many workgroups as fit (or half the
+// scheduled waves), no more than the tiles or half the list need.
+static int tile_grid(const na_solver* s, const void* kernel, int n) {
+  int dev = 0, cus = 256, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 128, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  int grid = cus * per_cu;
+  if (s->grid_waves > 0) grid = (s->grid_waves + 1) / 2;
+  const int need = (n + 1) / 2 > s->tiles ? (n + 1) / 2 : s->tiles;
+  grid = grid < need ? grid : need;
+  return grid > 0 ? grid : 1;
+}
+
 static int default_grid(const na_solver* s, const void* kernel = (const void*)k_frame) {
   int W = s->model->M.W;
   if (s->grid_waves > 0) return s->grid_waves < W ? s->grid_waves : W;
@@ -2091,6 +2158,13 @@ extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float*
   const NaModel& M = s->model->M;
   double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
+  if (s->tiles > 0) {
+    int grid = tile_grid(s, (const void*)k_frame_tile, M.W);
+    k_frame_tile<<<grid, 128, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f,
+                                               to_control(control), dt_outer, eff, s->queue, s->order, s->tiles);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   int grid = default_grid(s);
   k_frame<<<grid, 64, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, to_control(control),
                                        dt_outer, eff, s->queue, s->order);
@@ -2120,17 +2194,7 @@ extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, c
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
   if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
   if (s->tiles > 0) {
-    // workgroups of two waves: as many as fit, each taking two worlds at once after the tiles
-    int dev = 0, cus = 256, per_cu = 0;
-    HIPCHK(hipGetDevice(&dev));
-    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_serve_tile, 128, 0) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    int grid = cus * per_cu;
-    if (s->grid_waves > 0) grid = (s->grid_waves + 1) / 2;
-    const int need = (n_active + 1) / 2 > s->tiles ? (n_active + 1) / 2 : s->tiles;
-    grid = grid < need ? grid : need;
+    int grid = tile_grid(s, (const void*)k_serve_tile, n_active);
     k_serve_tile<<<grid, 128, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
                                        active, n_active, s->tiles, SV, budget, max_iterations, unfinished);
     HIPCHK(hipGetLastError());

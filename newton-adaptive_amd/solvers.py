@@ -200,14 +200,17 @@ class _SolverBase:
                    "na_solver_set_schedule")
 
     def set_tiles(self, tiles: int) -> None:
-        """Run the first ``tiles`` worlds of each served launch's list (``na_serve``: the serving
-        engine's rollouts, largest worlds first) as two-wave tiles: two wavefronts per world, one
+        """Run the first ``tiles`` worlds of each boundary call's visiting order (under
+        ``world_order="cost"``, the previous call's costliest worlds) and of each served launch's
+        list (``na_serve``, the serving engine) as two-wave tiles: two wavefronts per world, one
         solving each attempt's full step while the other solves its first half step.  Needs
         ``warm_start="concurrent"`` (the first half step's Newton seed must not depend on the full
         step's result).  A world's results are the same with and without tiles, bit for bit."""
         tiles = int(tiles)
         if tiles < 0:
             raise ValueError("tiles must be >= 0")
+        if tiles and self.model.is_articulated:
+            raise ValueError("two-wave tiles are built for free-body worlds")
         if tiles and self.warm_start != "concurrent":
             raise ValueError("two-wave tiles need warm_start='concurrent'")
         _lib.check(_lib.lib().na_solver_set_tiles(self._h, tiles), "na_solver_set_tiles")
@@ -446,7 +449,7 @@ class _SolverBase:
                    "_cum_accepted", "_work", "_q64", "_trend")
 
     def autotune(self, state, dt_outer: float, frames: int = 2, waves_per_cu=(2, 3, 4),
-                 orders=("cost", "size", "index"), control=None) -> dict:
+                 orders=("cost", "size", "index"), control=None, tiles=None) -> dict:
         """Pick the persistent launch schedule of the boundary call by timing it on the device.
 
         Each candidate (waves per CU of the persistent grid, world visiting order) runs ``frames``
@@ -454,7 +457,9 @@ class _SolverBase:
         of those calls decides.  Orders: "cost" (the worlds' work in an untimed probe run, most
         expensive first), "size" (bodies + shapes, largest first), "index".  A schedule decides
         only which wave runs a world, never its arithmetic, so every candidate must leave the same
-        state bit for bit; that is checked, and a mismatch raises.  The state and controller are
+        state bit for bit; that is checked, and a mismatch raises.  Under ``warm_start="concurrent"``
+        the candidates also include two-wave tiles for the first k worlds of the cost order, k in
+        ``tiles`` (default 64, 128, 256; see :meth:`set_tiles`).  The state and controller are
         restored afterwards, the chosen schedule is kept for the following calls (``world_order``
         becomes "index" so that the order is not replaced after the next call) and the table is
         returned and kept in ``schedule_choice``."""
@@ -491,30 +496,35 @@ class _SolverBase:
                     "size": torch.argsort(size, descending=True, stable=True), "index": None}
         cus = torch.cuda.get_device_properties(dev).multi_processor_count
         stream = torch.cuda.current_stream(dev)
+        if tiles is None:
+            tiles = (64, 128, 256) if self.warm_start == "concurrent" else ()
+        cands = [(int(w), name, 0) for w in waves_per_cu for name in orders]
+        cands += [(4, "cost", int(k)) for k in tiles if k > 0]
         rows, ref = [], None
-        for wpc in waves_per_cu:
-            for name in orders:
-                order = order_of[name]
-                self.set_schedule(int(wpc) * cus, order.to(torch.int32) if order is not None else None,
-                                  _checked=True)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                for _ in range(frames):
-                    self.step(state, state, control, None, dt_outer)
-                e1.record(stream)
-                torch.cuda.synchronize(dev)
-                out = (state.body_q.clone(), state.body_qd.clone(), self._q64.clone())
-                if ref is None:
-                    ref = out
-                elif not all(torch.equal(a, b) for a, b in zip(out, ref)):
-                    restore()
-                    raise RuntimeError("autotune: schedules gave different states (a schedule changed arithmetic)")
-                rows.append({"waves_per_cu": int(wpc), "order": name, "ms": e0.elapsed_time(e1)})
+        for wpc, name, k in cands:
+            order = order_of[name]
+            self.set_schedule(int(wpc) * cus, order.to(torch.int32) if order is not None else None,
+                              _checked=True)
+            self.set_tiles(k)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(frames):
+                self.step(state, state, control, None, dt_outer)
+            e1.record(stream)
+            torch.cuda.synchronize(dev)
+            out = (state.body_q.clone(), state.body_qd.clone(), self._q64.clone())
+            if ref is None:
+                ref = out
+            elif not all(torch.equal(a, b) for a, b in zip(out, ref)):
                 restore()
+                raise RuntimeError("autotune: schedules gave different states (a schedule changed arithmetic)")
+            rows.append({"waves_per_cu": int(wpc), "order": name, "tiles": k, "ms": e0.elapsed_time(e1)})
+            restore()
         best = min(rows, key=lambda r: r["ms"])
         order = order_of[best["order"]]
         self.set_schedule(best["waves_per_cu"] * cus, order.to(torch.int32) if order is not None else None,
                           _checked=True)
+        self.set_tiles(best["tiles"])
         if best["order"] != "index" or saved_order == "index":
             self._world_order = "index"
         else:

@@ -432,7 +432,7 @@ def test_trend_of_a_free_fall_is_gravity_and_reset_clears_it(na):
         assert not tr2[::2].any() and np.array_equal(tr2[1::2], tr[1::2])
 
 
-@pytest.mark.parametrize("seeds", ["extrapolate", "reference"])
+@pytest.mark.parametrize("seeds", ["extrapolate", "concurrent", "reference"])
 def test_served_and_plain_identical_for_both_seeds(na, seeds):
     """The served and the lock-step boundary forms give bit-identical states under either Newton
     seed rule (the trend lives in device memory between the lock-step launches)."""
@@ -448,6 +448,30 @@ def test_served_and_plain_identical_for_both_seeds(na, seeds):
         runs[schedule] = (state.body_q.cpu().numpy(), state.body_qd.cpu().numpy(), solver._trend.cpu().numpy())
     for k in range(3):
         assert np.array_equal(runs["served"][k], runs["plain"][k])
+
+
+@pytest.mark.parametrize("tiles", [1, 12, 96])
+def test_tiled_boundary_calls_equal_single_wave(na, tiles):
+    """Boundary calls with two-wave tiles (the previous call's costliest worlds first, each solved
+    by two wavefronts) leave the same states as single-wave boundary calls and as the lock-step
+    form, bit for bit, under the concurrent seeds; the first call has no cost history yet."""
+    runs = {}
+    for name in ("untiled", "tiled", "plain"):
+        model = na.scenes.uneven_pile(96, seed=23, device=DEV, min_prims=2, max_prims=24)
+        state = model.state()
+        solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256,
+                                      schedule="plain" if name == "plain" else "served", warm_start="concurrent")
+        if name == "tiled":
+            solver.set_tiles(tiles)
+        for _ in range(6):
+            solver.step(state, state, model.control(), None, 0.01)
+        torch.cuda.synchronize()
+        runs[name] = (state.body_q.cpu().numpy(), state.body_qd.cpu().numpy(), solver._trend.cpu().numpy(),
+                      solver.accepted_steps())
+    for k in range(3):
+        assert np.array_equal(runs["tiled"][k], runs["untiled"][k])
+        assert np.array_equal(runs["plain"][k], runs["untiled"][k])
+    assert runs["tiled"][3] == runs["untiled"][3] > 0
 
 
 def test_extrapolated_seeds_cut_newton_iterations(na):
