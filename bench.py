@@ -611,8 +611,14 @@ def main():
 
     if not args.no_plain:
         out["plain"] = frames_form("plain")
-        out["plain"]["form"] = "reference host loop: one launch per iteration over all worlds + 4-byte flag readback"
+        out["plain"]["form"] = ("reference host loop: one launch per iteration over all worlds + 4-byte flag "
+                                "readback; worlds at their boundary skip the iteration")
         out["served_over_plain"] = value / out["plain"]["value"]
+        out["plain_batch"] = frames_form("plain_batch")
+        out["plain_batch"]["form"] = ("the same host loop with the reference's fixed-batch work pattern: worlds at "
+                                      "their boundary still run each iteration's two collision passes (batched "
+                                      "CollisionPipeline.collide); their zero-length solves are not charged")
+        out["served_over_plain_batch"] = value / out["plain_batch"]["value"]
     if not args.no_frames:
         out["frames"] = frames_form("served")
         out["frames"]["form"] = "one persistent launch per boundary frame (na_step)"

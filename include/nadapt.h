@@ -219,6 +219,12 @@ int na_step(na_solver* s, float* body_q, float* body_qd, const float* body_f, co
             double dt_outer, void* stream);
 /* Same boundary call in the lock-step form: one launch over all worlds per iteration and a
  * 4-byte flag read back per iteration.  Returns the status of the host loop. */
+/* Lock-step form's work pattern: with batch_all set, a world that has reached its boundary still
+ * runs each iteration's two collision passes and discards them, as the reference's batched
+ * CollisionPipeline.collide does for every world of its fixed batch (solver_sap_adaptive.py:754,769;
+ * its zero-length solves are not charged).  Results are the same either way; 0 (default) skips
+ * finished worlds. */
+int na_solver_set_plain_batch(na_solver* s, int32_t batch_all);
 int na_step_plain(na_solver* s, float* body_q, float* body_qd, const float* body_f, const na_control_desc* control,
                   double dt_outer, int32_t* iterations_out, void* stream);
 /* Continuous-batching rollout launch over the worlds listed in `active` (no counterpart in the

@@ -97,6 +97,7 @@ SIGNATURES = {
     "na_coords_from_mujoco": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "na_serve": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, i32, vp, vp]),
     "na_step_articulated": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_double, vp]),
+    "na_solver_set_plain_batch": (C.c_int, [vp, i32]),
     "na_solver_set_tiles": (C.c_int, [vp, i32]),
     "na_serve_articulated": (C.c_int, [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                        C.c_double, i32, vp, vp]),

@@ -68,7 +68,8 @@ struct na_solver {
   int32_t* flag_host;
   const int32_t* order;
   int grid_waves;
-  int tiles;  // leading entries of a served launch's list run as two-wave tiles (0: none)
+  int tiles;        // leading entries of a served launch's list run as two-wave tiles (0: none)
+  int plain_batch;  // lock-step form: finished worlds still run each iteration's collision passes
   void* pool;
 };
 
@@ -1123,12 +1124,28 @@ __global__ void k_zero_i32(int32_t* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
+// batch_all: a world that has reached its boundary still runs the iteration's two collision
+// passes (at its pose, as a zero-length step leaves it) and discards them, as the reference's
+// batched CollisionPipeline.collide does for every world (solver_sap_adaptive.py:754,769); its
+// state and controller are untouched.
 __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_qd,
-                                                   const float* body_f, NaControl U, double eff_max, int32_t* flag) {
+                                                   const float* body_f, NaControl U, double eff_max, int32_t* flag,
+                                                   int batch_all) {
   __shared__ WorldLDS S;
   int w = blockIdx.x;
   WCtrl c = load_ctrl(K, w);
-  if (!(c.sim_time < c.next_time)) return;
+  if (!(c.sim_time < c.next_time)) {
+    if (batch_all) {
+      WorldCtx wc = world_ctx(M, w);
+      double q[7];
+      float v[6], f[6];
+      load_body(wc, q64, nullptr, body_qd, nullptr, q, v, f);
+      long long wk[NA_WORK] = {};
+      int ovf = 0;
+      for (int pass = 0; pass < 2; ++pass) world_collide(M, cfg, wc, S, q, ovf, wk);
+    }
+    return;
+  }
   WorldCtx wc = world_ctx(M, w);
   const int l = lane_id();
   double q[7];
@@ -2084,6 +2101,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
   s->order = nullptr;
   s->grid_waves = 0;
   s->tiles = 0;
+  s->plain_batch = 0;
   // controller defaults (reset of all worlds)
   k_reset<<<blocks(W, 256), 256>>>(m->M, nullptr, s->cfg.dt_init, K);
   HIPCHK(hipGetLastError());
@@ -2122,6 +2140,12 @@ extern "C" int na_solver_set_tiles(na_solver* s, int32_t tiles) {
     return fail("na_solver_set_tiles: two-wave tiles need adaptive mode, warm_start 2 (concurrent seeds) and the "
                 "SAP contact law");
   s->tiles = tiles;
+  return 0;
+}
+
+extern "C" int na_solver_set_plain_batch(na_solver* s, int32_t batch_all) {
+  if (!s) return fail("na_solver_set_plain_batch: null solver");
+  s->plain_batch = batch_all ? 1 : 0;
   return 0;
 }
 
@@ -2220,7 +2244,8 @@ extern "C" int na_step_plain(na_solver* s, float* body_q, float* body_qd, const 
   int status = 0, it = 0;
   for (; it < s->cfg.max_substeps; ++it) {
     k_zero_i32<<<1, 64, 0, st>>>(s->flag, 1);
-    k_plain_iter<<<M.W, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_qd, body_f, to_control(control), eff, s->flag);
+    k_plain_iter<<<M.W, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_qd, body_f, to_control(control), eff, s->flag,
+                                     s->plain_batch);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

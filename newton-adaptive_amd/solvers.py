@@ -79,8 +79,11 @@ class _SolverBase:
             raise ValueError(f"world_order must be 'cost' or 'index', got {world_order!r}.")
         if warm_start not in WARM_STARTS:
             raise ValueError(f"warm_start must be one of {tuple(WARM_STARTS)}, got {warm_start!r}.")
-        if schedule not in ("served", "plain"):
-            raise ValueError(f"schedule must be 'served' or 'plain', got {schedule!r}.")
+        if schedule not in ("served", "plain", "plain_batch"):
+            raise ValueError(f"schedule must be 'served', 'plain' or 'plain_batch', got {schedule!r}.")
+        plain_batch = schedule == "plain_batch"
+        if plain_batch:
+            schedule = "plain"
         self.model = model
         self._mode = mode
         self._tol = float(tol)
@@ -167,6 +170,9 @@ class _SolverBase:
         _lib.check(_lib.lib().na_solver_create(self._handle.handle, C.byref(cfg), C.byref(t), C.byref(h)),
                    "na_solver_create")
         self._h = h
+        if plain_batch:
+            _lib.check(_lib.lib().na_solver_set_plain_batch(self._h, 1), "na_solver_set_plain_batch")
+        self.plain_batch = plain_batch
         self._order = None
         self._work_prev = None
         self._pipeline = None

@@ -434,10 +434,11 @@ def test_trend_of_a_free_fall_is_gravity_and_reset_clears_it(na):
 
 @pytest.mark.parametrize("seeds", ["extrapolate", "concurrent", "reference"])
 def test_served_and_plain_identical_for_both_seeds(na, seeds):
-    """The served and the lock-step boundary forms give bit-identical states under either Newton
-    seed rule (the trend lives in device memory between the lock-step launches)."""
+    """The served and the lock-step boundary forms give bit-identical states under every Newton
+    seed rule (the trend lives in device memory between the lock-step launches), also when the
+    lock-step form runs the reference's fixed-batch work pattern (finished worlds still collide)."""
     runs = {}
-    for schedule in ("served", "plain"):
+    for schedule in ("served", "plain", "plain_batch"):
         model = na.scenes.uneven_pile(96, seed=21, device=DEV, max_prims=16)
         state = model.state()
         solver = na.SolverSAPAdaptive(model, tol=1e-3, dt_inner_init=1e-3, dt_inner_max=0.01, max_substeps=256,
@@ -448,6 +449,7 @@ def test_served_and_plain_identical_for_both_seeds(na, seeds):
         runs[schedule] = (state.body_q.cpu().numpy(), state.body_qd.cpu().numpy(), solver._trend.cpu().numpy())
     for k in range(3):
         assert np.array_equal(runs["served"][k], runs["plain"][k])
+        assert np.array_equal(runs["served"][k], runs["plain_batch"][k])
 
 
 @pytest.mark.parametrize("tiles", [1, 12, 96])
