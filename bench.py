@@ -323,22 +323,24 @@ def articulated_form(na, args, log):
 
 
 def serving_tiles(na, args, dev, kw, log):
-    """Autotuner of the serving schedule on the free-body part of the serving workload, every
-    request waiting at t = 0: two-wave tiles for the worlds with the most measured work left
+    """Autotuner of the serving schedule on the serving workload with every request waiting at
+    t = 0: two-wave tiles for the free-body worlds with the most measured work left
     (``ServingEngine(tiles=k)``, concurrent Newton seeds) against one world per wave (extrapolated
     and concurrent seeds).  Tiled and untiled runs of the same seeds must leave bit-identical
-    results; the best candidate is reported."""
+    results; the fastest candidate is returned as ``best``."""
     import numpy as np
     import torch
 
     rows, ref = [], {}
-    for ws, tiles in (("extrapolate", 0), ("concurrent", 0), ("concurrent", 64), ("concurrent", 256)):
+    for ws, tiles in (("extrapolate", 0), ("concurrent", 0), ("concurrent", 64), ("concurrent", 128)):
         smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=None, device=dev,
-                                                   distinct=args.serve_worlds, articulated_share=0.0)
+                                                   distinct=args.serve_worlds, articulated_share=args.serve_articulated)
         seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
-                                budget_ms=20.0, device=dev, tiles=tiles, **dict(kw, warm_start=ws))
+                                budget_ms=20.0, device=dev, articulated=args.serve_articulated > 0, tiles=tiles,
+                                **dict(kw, warm_start=ws))
         r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
-        sig = np.concatenate([q.body_q.reshape(-1) for q in sreqs])
+        sig = np.concatenate([q.body_q.reshape(-1) for q in sreqs]
+                             + [q.joint_q.reshape(-1) for q in sreqs if q.joint_q is not None])
         same = None
         if ws in ref:
             same = bool(np.array_equal(sig, ref[ws]))
@@ -348,14 +350,14 @@ def serving_tiles(na, args, dev, kw, log):
         rows.append({"warm_start": ws, "tiles": tiles, "accepted_steps_per_s": r["accepted_steps_per_s"],
                      "latency_p50_s": r["latency_p50_s"], "latency_p99_s": r["latency_p99_s"],
                      "completed": r["completed"], "identical_to_untiled": same})
-        log(f"[bench] free-body serving, warm_start {ws}, tiles {tiles}: {r['accepted_steps_per_s']:.0f} accepted "
+        log(f"[bench] serving schedule, warm_start {ws}, tiles {tiles}: {r['accepted_steps_per_s']:.0f} accepted "
             f"steps/s, latency p50 {r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
         del seng
         torch.cuda.synchronize()
     best = max(rows, key=lambda x: x["accepted_steps_per_s"])
     return {"candidates": rows, "best": best,
-            "workload": f"{args.serve_requests} free-body requests of the serving workload's distribution, all "
-                        "waiting at t = 0, 20 ms rounds",
+            "workload": f"the {args.serve_requests} requests of the serving workload, all waiting at t = 0, "
+                        "20 ms rounds",
             "form": "tiles: the worlds with the most measured work left run on two wavefronts each (full step and "
                     "first half step solved at once)"}
 
@@ -416,8 +418,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=3600)
     ap.add_argument("--serve-rate", type=float, default=300.0)
     ap.add_argument("--serve-worlds", type=int, default=1200, help="distinct scenes the requests start from")
-    ap.add_argument("--serve-tiles", type=int, default=0,
-                    help="continuous serving: worlds per round run as two-wave tiles (needs --warm-start concurrent)")
+    ap.add_argument("--serve-tiles", type=int, default=64,
+                    help="continuous serving without the autotuner: worlds per round run as two-wave tiles")
+    ap.add_argument("--serve-warm-start", choices=("extrapolate", "concurrent"), default="concurrent",
+                    help="Newton seeds of the serving legs without the autotuner (tiles need concurrent)")
     ap.add_argument("--serve-articulated", type=float, default=0.25,
                     help="share of requests that are articulated worlds (arm or quadruped on terrain)")
     ap.add_argument("--cpu-worlds", type=int, default=256, help="CPU baseline sample on all host cores")
@@ -625,6 +629,14 @@ def main():
 
     # ---------------------------------------------------------------- serving (Poisson arrivals)
     if not args.no_serving:
+        # the serving schedule (Newton seeds x two-wave tiles) is picked by the tile autotuner on the
+        # saturated stream; without it the concurrent seeds with --serve-tiles
+        if not args.no_autotune:
+            tiles_tune = serving_tiles(na, args, dev, kw, log)
+            serve_ws, serve_tiles = tiles_tune["best"]["warm_start"], tiles_tune["best"]["tiles"]
+        else:
+            tiles_tune, serve_ws, serve_tiles = None, args.serve_warm_start, args.serve_tiles
+        skw = dict(kw, warm_start=serve_ws)
         res = {}
         for sched in ("continuous", "static"):
             smodel, sreqs = na.scenes.serving_requests(args.serve_requests, seed=args.seed + 1, rate=args.serve_rate,
@@ -633,7 +645,7 @@ def main():
             seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
                                     budget_ms=20.0, schedule=sched, device=dev,
                                     articulated=args.serve_articulated > 0,
-                                    tiles=args.serve_tiles if sched == "continuous" else 0, **kw)
+                                    tiles=serve_tiles if sched == "continuous" else 0, **skw)
             log(f"[bench] serving {len(sreqs)} requests at {args.serve_rate:g}/s, {sched} batching")
             res[sched] = seng.run(sreqs, realtime=True, max_wall_s=240.0)
             res[sched]["final_states"] = [(r.body_q, r.body_qd, r.joint_q, r.joint_qd) for r in sreqs]
@@ -650,6 +662,9 @@ def main():
                                        "(7-dof arm or 18-dof quadruped on box terrain), horizon U[200, 2000] "
                                        "accepted steps, tol in {1e-2, 1e-3, 1e-4}, Poisson arrivals; continuous "
                                        "batching in 20 ms device-time rounds",
+                              schedule={"warm_start": serve_ws, "tiles": serve_tiles,
+                                        "form": "each round the free-body worlds with the most measured work left "
+                                                "run as two-wave tiles, the rest one world per wave"},
                               static=dict(res["static"], form="fixed batches in lock step: one launch per iteration "
                                                               "+ 4-byte readback, batch held to its longest rollout"),
                               served_over_static=res["continuous"]["accepted_steps_per_s"]
@@ -665,7 +680,7 @@ def main():
             seng = na.ServingEngine(smodel, max_worlds=2048, body_capacity=2048 * 40, shape_capacity=2048 * 40,
                                     budget_ms=20.0, schedule=sched, device=dev,
                                     articulated=args.serve_articulated > 0,
-                                    tiles=args.serve_tiles if sched == "continuous" else 0, **kw)
+                                    tiles=serve_tiles if sched == "continuous" else 0, **skw)
             r = seng.run(sreqs, realtime=True, max_wall_s=240.0)
             sat[sched] = {k: r[k] for k in ("accepted_steps_per_s", "latency_p50_s", "latency_p99_s", "completed",
                                             "device_busy")}
@@ -673,8 +688,8 @@ def main():
                 f"{r['latency_p50_s']:.2f} s p99 {r['latency_p99_s']:.2f} s")
             del seng
         out["serving"]["saturated"] = dict(sat, form="the same requests all waiting at t = 0 (capacity)")
-        if not args.no_autotune:
-            out["serving"]["tiles"] = serving_tiles(na, args, dev, kw, log)
+        if tiles_tune is not None:
+            out["serving"]["tiles"] = tiles_tune
 
     # ---------------------------------------------------------------- contact generation API
     if not args.no_collide:

@@ -63,7 +63,7 @@ struct na_solver {
   NaConfig cfg;
   Ctrl K;
   double* q64;
-  int32_t* queue;  // [1]
+  int32_t* queue;  // [2]
   int32_t* flag;   // [1]
   int32_t* flag_host;
   const int32_t* order;
@@ -524,7 +524,45 @@ struct TileHdr {
   double h, tol;
   float opt_rel_tol;
   long long wk[NA_WORK];  // the second wave's work counters of its solve
+  // handshake of the LDS-flag protocol (workgroups with waves outside the tile): sequence numbers
+  // of the last request and the last answer, the first wave's count, and whether the second wave
+  // stopped answering (then the first solves the half steps itself, with the same arithmetic)
+  int32_t req, done, seq, dead;
 };
+
+// The tile's two waves meet at each request and each answer.  In a two-wave workgroup that is a
+// workgroup barrier.  Where the workgroup has other waves, which never meet the tile's barriers
+// (NA_TILE_FLAGS), the sender publishes a sequence number in LDS after a release fence and the
+// receiver polls it, sleeping between reads, then fences (acquire).  Every wait is bounded by the
+// constant-rate wall clock (NA_TILE_WAIT_TICKS, 60 s at 100 MHz); a wait that runs out returns
+// false and its caller falls back to working alone.
+#define NA_TILE_BARRIER 1
+#define NA_TILE_FLAGS 2
+#define NA_TILE_WAIT_TICKS 6000000000LL
+template <int MODE>
+__device__ __forceinline__ void tile_signal(int32_t* word, int32_t v) {
+  if constexpr (MODE == NA_TILE_FLAGS) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane_id() == 0) __hip_atomic_store(word, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    group_sync();
+  }
+}
+template <int MODE>
+__device__ __forceinline__ bool tile_await(int32_t* word, int32_t v) {
+  if constexpr (MODE == NA_TILE_FLAGS) {
+    const long long t0 = wall_clock64();
+    for (int spins = 1; __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != v; ++spins) {
+      __builtin_amdgcn_s_sleep(2);
+      if ((spins & 255) == 0 && wall_clock64() - t0 > NA_TILE_WAIT_TICKS) return false;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+  } else {
+    group_sync();
+    return true;
+  }
+}
 // Inputs of the second wave's solve, then its outputs, in the second wave's own segment / solve
 // area (read into registers before its solve writes that area; written after the solve).
 struct TileIn {
@@ -545,6 +583,7 @@ __device__ __forceinline__ TileOut& tile_out(WorldLDS& S) { return *reinterpret_
 
 // The first wave's step-doubling attempt of a tiled world (adaptive mode, warm_start 2): the
 // semantics and arithmetic of world_attempt, with the first half step solved by the second wave.
+template <int MODE>
 __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc,
                                                   WorldLDS& S, WorldLDS& S1, TileHdr& T, const double* q,
                                                   const float* v, const float* f, const float* tr, double dt,
@@ -562,27 +601,39 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
     g0[k] = tr ? v[k] + hh * tr[k] : v[k];
     g1[k] = tr ? v[k] + qh * tr[k] : v[k];
   }
-  TileIn& in = tile_in(S1);
-#pragma unroll
-  for (int k = 0; k < 7; ++k) in.q[k][l] = q[k];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) { in.v[k][l] = v[k]; in.f[k][l] = f[k]; in.g[k][l] = g1[k]; }
-  if (l == 0) {
-    T.cmd = 1; T.w = wc.w; T.nc = nc; T.h = 0.5 * dt; T.tol = cfg.tol; T.opt_rel_tol = cfg.opt_rel_tol;
-  }
-  group_sync();  // the second wave starts the first half step
-  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
-  group_sync();  // its result is in
   double qm[7];
   float vm[6];
-  const TileOut& out = tile_out(S1);
+  const bool alone = MODE == NA_TILE_FLAGS && T.dead;
+  if (!alone) {
+    TileIn& in = tile_in(S1);
 #pragma unroll
-  for (int k = 0; k < 7; ++k) qm[k] = out.q[k][l];
+    for (int k = 0; k < 7; ++k) in.q[k][l] = q[k];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) vm[k] = out.v[k][l];
-  ok = T.ok && ok;
+    for (int k = 0; k < 6; ++k) { in.v[k][l] = v[k]; in.f[k][l] = f[k]; in.g[k][l] = g1[k]; }
+    if (l == 0) {
+      T.cmd = 1; T.w = wc.w; T.nc = nc; T.h = 0.5 * dt; T.tol = cfg.tol; T.opt_rel_tol = cfg.opt_rel_tol;
+    }
+  }
+  const int32_t seq = T.seq + 1;
+  wave_sync();
+  if (l == 0) T.seq = seq;
+  if (!alone) tile_signal<MODE>(&T.req, seq);  // the second wave starts the first half step
+  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
+  if (!alone && tile_await<MODE>(&T.done, seq)) {  // its result is in
+    const TileOut& out = tile_out(S1);
 #pragma unroll
-  for (int k = 0; k < NA_WORK; ++k) wk[k] += T.wk[k];
+    for (int k = 0; k < 7; ++k) qm[k] = out.q[k][l];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vm[k] = out.v[k][l];
+    ok = T.ok && ok;
+#pragma unroll
+    for (int k = 0; k < NA_WORK; ++k) wk[k] += T.wk[k];
+  } else {
+    // the second wave did not answer in time (LDS-flag protocol only): the first half step here,
+    // the same solve with the same inputs, and no more requests to that wave
+    if (l == 0) T.dead = 1;
+    ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
+  }
   int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
   float g2[6];
 #pragma unroll
@@ -605,11 +656,12 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
 // The second wave of a tile: solves first half steps on request until told to leave.  Its
 // barriers pair with the first wave's: one before each request (or the leave message), one after
 // each result.
+template <int MODE>
 __device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, WorldLDS& S0, WorldLDS& S1,
                                          TileHdr& T) {
   const int l = lane_id();
-  for (;;) {
-    group_sync();
+  for (int32_t seq = 1;; ++seq) {
+    if (!tile_await<MODE>(&T.req, seq)) break;
     if (T.cmd != 1) break;
     const WorldCtx wc = world_ctx(M, T.w);
     NaConfig wcfg = cfg;
@@ -656,7 +708,7 @@ __device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, 
 #pragma unroll
       for (int k = 0; k < NA_WORK; ++k) T.wk[k] = wk[k];
     }
-    group_sync();
+    tile_signal<MODE>(&T.done, seq);
   }
 }
 
@@ -754,8 +806,9 @@ __device__ __forceinline__ bool controller_update(const NaConfig& cfg, WCtrl& c,
 // q/v hold this lane's body state and are updated in place.  trend (warm_start 1 or 2, else null) is
 // the per-body acceleration array: read for the full step's Newton seed and, after an accepted
 // step, set to (v_new - v_old) / step.  Returns accepted-step count (0/1).
-// TILE: the world is a two-wave tile (S1, T: the second wave's LDS and the tile header).
-template <bool TILE = false>
+// TILE: 0, or the world is a two-wave tile with that handoff protocol (NA_TILE_BARRIER,
+// NA_TILE_FLAGS; S1, T: the second wave's LDS and the tile header).
+template <int TILE = 0>
 __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, WCtrl& c,
                                double* q, float* v, const float* f, float* trend, double eff_max, long long* wk,
                                WorldLDS* S1 = nullptr, TileHdr* T = nullptr) {
@@ -775,8 +828,8 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
     float tr[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) tr[k] = trend && own ? trend[6 * (wc.b0 + l) + k] : 0.f;
-    if constexpr (TILE)
-      err = world_attempt_tile(M, cfg, wc, S, *S1, *T, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf,
+    if constexpr (TILE != 0)
+      err = world_attempt_tile<TILE>(M, cfg, wc, S, *S1, *T, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf,
                                nc, wk);
     else
       err = world_attempt(M, cfg, wc, S, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf, nc, wk);
@@ -865,7 +918,7 @@ __device__ __forceinline__ void flush_work(const Ctrl& K, int w, const long long
 // One world's boundary frame: iterations from open_frame until the boundary, max_substeps or a
 // solve failure.  TILE: the world runs as a two-wave tile (S1, T: the second wave's LDS and the
 // tile header).
-template <bool TILE = false>
+template <int TILE = 0>
 __device__ __forceinline__ void frame_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64,
                                             float* body_q, float* body_qd, const float* body_f, const NaControl& U,
                                             double dt_outer, double eff_max, int w, WorldLDS& S,
@@ -916,7 +969,7 @@ __global__ __launch_bounds__(128) void k_frame_tile(NaModel M, NaConfig cfg, Ctr
   __shared__ TileHdr T;
   const int wave = threadIdx.x >> 6;
   if (wave == 1) {
-    tile_helper(M, cfg, S[0], S[1], T);
+    tile_helper<NA_TILE_BARRIER>(M, cfg, S[0], S[1], T);
   } else {
     for (;;) {
       int slot = 0;
@@ -930,7 +983,7 @@ __global__ __launch_bounds__(128) void k_frame_tile(NaModel M, NaConfig cfg, Ctr
                       S[0]);
         break;
       }
-      frame_world<true>(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, order ? order[slot] : slot,
+      frame_world<NA_TILE_BARRIER>(M, cfg, K, q64, body_q, body_qd, body_f, U, dt_outer, eff_max, order ? order[slot] : slot,
                         S[0], &S[1], &T);
     }
   }
@@ -973,8 +1026,9 @@ struct ServeSlots {
 // loop runs at most max_substeps iterations per frame.
 // One world's turn of a serving launch: boundary frames until the world's horizon, a solve
 // failure, max_iters iterations (<= 0: no limit) or the device clock passing t_end.
-// TILE: the world runs as a two-wave tile (S1, T: the second wave's LDS and the tile header).
-template <bool TILE = false>
+// TILE: 0, or the world runs as a two-wave tile with that handoff protocol (S1, T: the second
+// wave's LDS and the tile header).
+template <int TILE = 0>
 __device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64, float* body_q,
                             float* body_qd, const float* body_f, double dt_outer, double eff_max, const ServeSlots& SV,
                             int w, long long t_end, int max_iters, int32_t* unfinished, WorldLDS& S,
@@ -1060,7 +1114,7 @@ __global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctr
   const int wave = threadIdx.x >> 6;
   const long long t_end = wall_clock64() + budget;
   if (wave == 1) {
-    tile_helper(M, cfg, S[0], S[1], T);
+    tile_helper<NA_TILE_BARRIER>(M, cfg, S[0], S[1], T);
   } else {
     for (;;) {
       int slot = 0;
@@ -1079,7 +1133,7 @@ __global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctr
       }
       const int w = active[slot];
       if (SV.status[w] != 0) continue;
-      serve_world<true>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
+      serve_world<NA_TILE_BARRIER>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
                         unfinished, S[0], &S[1], &T);
     }
   }
@@ -1769,6 +1823,93 @@ __global__ __launch_bounds__(64) void k_serve_mixed(NaModel M, NaConfig cfg, Ctr
   }
 }
 
+// Mixed serving pool with two-wave tiles: workgroups of three waves, one LDS block each (as in
+// k_serve_mixed, so still three waves per CU).  The list's first n_tiled entries (free-body worlds
+// with the most work left) are taken, through their own counter queue[0], only by the first wave
+// of each workgroup, which runs each as a tile with the second wave (see world_attempt_tile); the
+// third wave takes the rest of the list through queue[1] from the start, and the first two join it
+// once the tile entries are exhausted.  An articulated entry among the tiles runs on the first wave
+// alone.  The tile pair meets through the LDS-flag handshake (NA_TILE_FLAGS), since the third wave
+// never meets their barriers.  Every wave exits: each loop ends when its counter passes its list or
+// the budget has passed, the first wave sends the second its leave message once, on leaving the tile
+// loop, and every wait of the handshake is bounded.
+__device__ __forceinline__ void serve_mixed_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64,
+                                                  float* body_q, float* body_qd, const float* body_f,
+                                                  const ArtServeState& X, double dt_outer, double eff_max,
+                                                  const ServeSlots& SV, int w, long long t_end, int max_iters,
+                                                  int32_t* unfinished, char* block) {
+  const int b0 = M.world_body_start[w];
+  const bool art = M.world_body_end[w] > b0 && M.joint_type[b0] != 0;
+  if (art)
+    serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
+                    *reinterpret_cast<ArtLDS*>(block));
+  else
+    serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
+                *reinterpret_cast<WorldLDS*>(block));
+}
+
+__global__ __launch_bounds__(192) void k_serve_mixed_tile(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
+                                                         float* body_qd, const float* body_f, ArtServeState X,
+                                                         double dt_outer, double eff_max, int32_t* queue,
+                                                         const int32_t* active, int n_active, int n_tiled,
+                                                         ServeSlots SV, long long budget, int max_iters,
+                                                         int32_t* unfinished) {
+  constexpr size_t kMixBytes = sizeof(WorldLDS) > sizeof(ArtLDS) ? sizeof(WorldLDS) : sizeof(ArtLDS);
+  __shared__ __align__(16) char mix0[kMixBytes];
+  __shared__ __align__(16) char mix1[kMixBytes];
+  __shared__ __align__(16) char mix2[kMixBytes];
+  __shared__ TileHdr T;
+  WorldLDS& W0 = *reinterpret_cast<WorldLDS*>(mix0);
+  WorldLDS& W1 = *reinterpret_cast<WorldLDS*>(mix1);
+  const int wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) { T.req = 0; T.done = 0; T.seq = 0; T.dead = 0; }
+  group_sync();  // the only workgroup barrier: every wave passes it once, before the roles split
+  const long long t_end = wall_clock64() + budget;
+  const int n_tiles = n_tiled < n_active ? n_tiled : n_active;
+  if (wave == 1) {
+    tile_helper<NA_TILE_FLAGS>(M, cfg, W0, W1, T);
+  } else if (wave == 0) {
+    for (;;) {
+      int slot = 0;
+      if (lane_id() == 0) slot = wall_clock64() < t_end ? atomicAdd(&queue[0], 1) : n_tiles;
+      slot = __shfl(slot, 0, NA_WAVE);
+      if (slot >= n_tiles) {
+        if (lane_id() == 0) T.cmd = 0;
+        const int32_t seq = T.seq + 1;
+        wave_sync();
+        tile_signal<NA_TILE_FLAGS>(&T.req, seq);  // the second wave leaves the tile loop
+        break;
+      }
+      const int w = active[slot];
+      if (SV.status[w] != 0) continue;
+      const int b0 = M.world_body_start[w];
+      if (M.world_body_end[w] > b0 && M.joint_type[b0] != 0)
+        serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
+                        *reinterpret_cast<ArtLDS*>(mix0));
+      else
+        serve_world<NA_TILE_FLAGS>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
+                          unfinished, W0, &W1, &T);
+    }
+  }
+  for (;;) {
+    int slot = 0;
+    if (lane_id() == 0) slot = wall_clock64() < t_end ? n_tiles + atomicAdd(&queue[1], 1) : n_active;
+    slot = __shfl(slot, 0, NA_WAVE);
+    if (slot >= n_active) break;
+    const int w = active[slot];
+    if (SV.status[w] != 0) continue;
+    if (wave == 0)
+      serve_mixed_world(M, cfg, K, q64, body_q, body_qd, body_f, X, dt_outer, eff_max, SV, w, t_end, max_iters,
+                        unfinished, mix0);
+    else if (wave == 1)
+      serve_mixed_world(M, cfg, K, q64, body_q, body_qd, body_f, X, dt_outer, eff_max, SV, w, t_end, max_iters,
+                        unfinished, mix1);
+    else
+      serve_mixed_world(M, cfg, K, q64, body_q, body_qd, body_f, X, dt_outer, eff_max, SV, w, t_end, max_iters,
+                        unfinished, mix2);
+  }
+}
+
 // Parity entry kernels: one inner step / one step-doubling attempt per articulated world.
 __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, const double* q64, const double* qd64,
                                                     const double* h, const float* body_f, NaControl U, double* q_out,
@@ -2087,7 +2228,7 @@ extern "C" int na_solver_create(const na_model* m, const na_config* c, const na_
     K.work = (long long*)take(sizeof(long long) * NA_WORK * W);
     s->q64 = (double*)take(sizeof(double) * 7 * (size_t)(B > 0 ? B : 1));
   }
-  s->queue = (int32_t*)take(4);
+  s->queue = (int32_t*)take(8);  // [0] the list counter; [1] the second list of k_serve_mixed_tile
   s->flag = (int32_t*)take(4);
   if (!buffers) {
     float* tr = (float*)take(sizeof(float) * 6 * (size_t)(B > 0 ? B : 1));
@@ -2149,16 +2290,21 @@ extern "C" int na_solver_set_plain_batch(na_solver* s, int32_t batch_all) {
   return 0;
 }
 
-// Grid of a two-wave tile launch over n listed worlds: as many workgroups as fit (or half the
-// scheduled waves), no more than the tiles or half the list need.
-static int tile_grid(const na_solver* s, const void* kernel, int n) {
+// Grid of a tile launch over n listed worlds, workgroups of `waves` waves: as many workgroups as
+// fit (or the scheduled waves), no more than the list needs to give every world its waves at
+// once: a tile takes two waves, any other world one, so (n + tiles) / waves workgroups, and at
+// least one per tile.
+static int tile_grid(const na_solver* s, const void* kernel, int n, int waves) {
   int dev = 0, cus = 256, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 128, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * waves, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
   int grid = cus * per_cu;
-  if (s->grid_waves > 0) grid = (s->grid_waves + 1) / 2;
-  const int need = (n + 1) / 2 > s->tiles ? (n + 1) / 2 : s->tiles;
+  if (s->grid_waves > 0) grid = (s->grid_waves + waves - 1) / waves;
+  const int tiles = s->tiles < n ? s->tiles : n;
+  int need = (n + tiles + waves - 1) / waves;
+  need = need > tiles ? need : tiles;
   grid = grid < need ? grid : need;
   return grid > 0 ? grid : 1;
 }
@@ -2183,7 +2329,7 @@ extern "C" int na_step(na_solver* s, float* body_q, float* body_qd, const float*
   double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   k_zero_i32<<<1, 64, 0, S_(stream)>>>(s->queue, 1);
   if (s->tiles > 0) {
-    int grid = tile_grid(s, (const void*)k_frame_tile, M.W);
+    int grid = tile_grid(s, (const void*)k_frame_tile, M.W, 2);
     k_frame_tile<<<grid, 128, 0, S_(stream)>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f,
                                                to_control(control), dt_outer, eff, s->queue, s->order, s->tiles);
     HIPCHK(hipGetLastError());
@@ -2218,7 +2364,7 @@ extern "C" int na_serve(na_solver* s, const int32_t* active, int32_t n_active, c
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
   if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
   if (s->tiles > 0) {
-    int grid = tile_grid(s, (const void*)k_serve_tile, n_active);
+    int grid = tile_grid(s, (const void*)k_serve_tile, n_active, 2);
     k_serve_tile<<<grid, 128, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, dt_outer, eff, s->queue,
                                        active, n_active, s->tiles, SV, budget, max_iterations, unfinished);
     HIPCHK(hipGetLastError());
@@ -2507,10 +2653,19 @@ extern "C" int na_serve_mixed(na_solver* s, const int32_t* active, int32_t n_act
   const double eff = s->cfg.dt_max < dt_outer ? s->cfg.dt_max : dt_outer;
   ServeSlots SV{horizon_frames, horizon_accepted, frames_done, accepted, status, world_tol};
   ArtServeState X{joint_q64, joint_qd64, joint_q, joint_qd, body_q, body_qd, body_f};
+  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
+  if (s->tiles > 0) {
+    k_zero_i32<<<1, 64, 0, st>>>(s->queue, 2);
+    const int grid = tile_grid(s, (const void*)k_serve_mixed_tile, n_active, 3);
+    k_serve_mixed_tile<<<grid, 192, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, X, dt_outer, eff,
+                                             s->queue, active, n_active, s->tiles, SV, budget, max_iterations,
+                                             unfinished);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   k_zero_i32<<<1, 64, 0, st>>>(s->queue, 1);
   int grid = default_grid(s, (const void*)k_serve_mixed);
   grid = grid < n_active ? grid : n_active;
-  if (unfinished) k_zero_i32<<<1, 64, 0, st>>>(unfinished, 1);
   k_serve_mixed<<<grid, 64, 0, st>>>(M, s->cfg, s->K, s->q64, body_q, body_qd, body_f, X, dt_outer, eff, s->queue,
                                      active, n_active, SV, budget, max_iterations, unfinished);
   HIPCHK(hipGetLastError());

@@ -712,11 +712,13 @@ class ServingEngine:
                       key=lambda r: (r.arrival, -(r.body_count + r.shape_count), r.rid))
         if not live:
             return 0
-        if self.tiles and not self.pool.is_articulated:
-            # the worlds with the most estimated work left go first and run as two-wave tiles (before
-            # any estimate exists: the largest); the tiles only change which waves run a world
-            known = [r for r in live if r.cost_estimate == r.cost_estimate]
-            top = sorted(known, key=lambda r: (-r.cost_estimate, r.rid))[:self.tiles] if known else live[:self.tiles]
+        if self.tiles:
+            # the free-body worlds with the most estimated work left go first and run as two-wave
+            # tiles (before any estimate exists: the largest); the tiles only change which waves run
+            # a world
+            free = [r for r in live if not r.articulated]
+            known = [r for r in free if r.cost_estimate == r.cost_estimate]
+            top = sorted(known, key=lambda r: (-r.cost_estimate, r.rid))[:self.tiles] if known else free[:self.tiles]
             ids = {id(r) for r in top}
             live = top + [r for r in live if id(r) not in ids]
             self.solver.set_tiles(len(top))

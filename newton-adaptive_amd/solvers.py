@@ -211,12 +211,12 @@ class _SolverBase:
         list (``na_serve``, the serving engine) as two-wave tiles: two wavefronts per world, one
         solving each attempt's full step while the other solves its first half step.  Needs
         ``warm_start="concurrent"`` (the first half step's Newton seed must not depend on the full
-        step's result).  A world's results are the same with and without tiles, bit for bit."""
+        step's result).  Tiles apply to free-body worlds (a serving pool's articulated worlds and
+        articulated boundary calls run one world per wave).  A world's results are the same with
+        and without tiles, bit for bit."""
         tiles = int(tiles)
         if tiles < 0:
             raise ValueError("tiles must be >= 0")
-        if tiles and self.model.is_articulated:
-            raise ValueError("two-wave tiles are built for free-body worlds")
         if tiles and self.warm_start != "concurrent":
             raise ValueError("two-wave tiles need warm_start='concurrent'")
         _lib.check(_lib.lib().na_solver_set_tiles(self._h, tiles), "na_solver_set_tiles")

@@ -345,6 +345,32 @@ def test_mixed_stream_articulated_alone_equals_in_mix_and_static():
             assert np.array_equal(x.joint_q, y.joint_q) and np.array_equal(x.joint_qd, y.joint_qd)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("tiles,budget_ms,pipelined", [(6, 2.0, True), (24, 3.0, False)])
+def test_mixed_pool_tiles_equal_untiled(tiles, budget_ms, pipelined):
+    """A mixed pool (free-body and articulated requests) with two-wave tiles for its costliest
+    free-body worlds (three waves per workgroup: a tile pair and a third wave on the rest of the
+    list) gives every request the same result as the untiled pool, bit for bit."""
+    mk = lambda: na.scenes.serving_requests(48, seed=9, distinct=24, max_prims=24, horizons=(30, 80),
+                                            articulated_share=0.25, device=DEV)
+    model, ref = mk()
+    kw = dict(KW, warm_start="concurrent")
+    e1 = na.ServingEngine(model, max_worlds=12, body_capacity=600, shape_capacity=600, budget_ms=budget_ms,
+                          device=DEV, articulated=True, **kw)
+    s1 = e1.run(ref, realtime=False)
+    _, got = mk()
+    e2 = na.ServingEngine(model, max_worlds=12, body_capacity=600, shape_capacity=600, budget_ms=budget_ms,
+                          device=DEV, articulated=True, tiles=tiles, **kw)
+    s2 = e2.run(got, realtime=False, pipelined=pipelined)
+    print("untiled", s1, "tiled", s2, flush=True)
+    assert s1["completed"] == s2["completed"] == 48 and s2["rounds"] > 2
+    for x, y in zip(ref, got):
+        assert x.accepted == y.accepted
+        assert np.array_equal(x.body_q, y.body_q) and np.array_equal(x.body_qd, y.body_qd)
+        if x.articulated:
+            assert np.array_equal(x.joint_q, y.joint_q) and np.array_equal(x.joint_qd, y.joint_qd)
+
+
 def _by_tol_boundary_calls(model, frames, tols, articulated):
     """Boundary-call states of every world of ``model`` after ``frames`` frames, one run per
     tolerance (worlds are independent, so a world's result is its own boundary-call result)."""
