@@ -362,6 +362,56 @@ def serving_tiles(na, args, dev, kw, log):
                     "first half step solved at once)"}
 
 
+def gap_form(na, args, kw, log):
+    """The served rollout of the uneven-pile workload built with the reference's default contact
+    gap (10 cm, ModelBuilder.ShapeConfig.gap) instead of the workload's 2 cm: far more speculative
+    contacts, and worlds whose contacts exceed their 128-contact set keep the 128 closest.  Same
+    window as the headline (20 settling frames, then args.steps frames in one launch)."""
+    import numpy as np
+    import torch
+
+    dev = "cuda:0"
+    model = na.scenes.uneven_pile(args.worlds, seed=args.seed, device=dev, gap=0.1)
+    W = model.world_count
+    st = model.state()
+    warm = na.SolverSAPAdaptive(model, **kw)
+    over = torch.zeros(W, dtype=torch.int32, device=dev)
+    for _ in range(SETTLE_FRAMES):
+        warm.step(st, st, model.control(), None, args.dt_outer)
+        over |= warm._overflow
+    torch.cuda.synchronize()
+    q0, qd0 = st.body_q.clone(), st.body_qd.clone()
+    del warm
+    eng = na.ServingEngine(model, max_worlds=W, body_capacity=model.body_count,
+                           shape_capacity=model.shape_count - int(model.global_shapes.numel()), budget_ms=0.0,
+                           device=dev, **kw)
+    reqs = na.requests_from_model(model, horizon_frames=args.steps, state=na.State(q0, qd0, None))
+    eng.submit(reqs)
+    eng.admit(math.inf)
+    eng.solver.reset_work_counters()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    eng.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    wk = eng.solver._work.sum(dim=0).tolist()
+    ovf_window = int(eng.solver._overflow.sum().item())
+    eng.retire(wall)
+    acc = sum(r.accepted for r in reqs)
+    log(f"[bench] gap 0.1 m: {acc} accepted in {wall:.3f} s ({acc / wall:.0f}/s), "
+        f"{int((over > 0).sum())} worlds over 128 contacts while settling")
+    return {"value": acc / wall, "unit": "accepted world-steps/s", "accepted": acc, "wall_s": wall,
+            "launch_ms": e0.elapsed_time(e1), "attempts": int(wk[0]), "newton_iterations": int(wk[1]),
+            "contacts_per_collision": wk[3] / max(2 * wk[0], 1),
+            "worlds_over_128_contacts_settling": int((over > 0).sum()),
+            "worlds_over_128_contacts_last_frame": ovf_window,
+            "workload": f"uneven_pile_{W} with the reference's default 10 cm contact gap, tol {args.tol:g}, "
+                        f"{args.steps} frames after {SETTLE_FRAMES} settling frames, one served launch"}
+
+
 def collide_form(na, model, state, log, calls=20):
     """Contact generation through the public API (CollisionPipeline.collide, na_collide) on the
     settled piles: contacts per second and algorithmic HBM bytes per call (176 B per segment
@@ -430,6 +480,7 @@ def main():
     ap.add_argument("--precision-worlds", type=int, default=512)
     ap.add_argument("--no-articulated", action="store_true")
     ap.add_argument("--no-collide", action="store_true")
+    ap.add_argument("--no-gap", action="store_true", help="skip the 10 cm contact-gap variant of the workload")
     ap.add_argument("--no-mujoco", action="store_true", help="skip the MuJoCo inner-step line")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default schedule (4 waves per CU)")
     ap.add_argument("--warm-start", choices=("extrapolate", "concurrent", "reference"), default="extrapolate",
@@ -690,6 +741,9 @@ def main():
         out["serving"]["saturated"] = dict(sat, form="the same requests all waiting at t = 0 (capacity)")
         if tiles_tune is not None:
             out["serving"]["tiles"] = tiles_tune
+
+    if not args.no_gap:
+        out["gap_0.1"] = gap_form(na, args, kw, log)
 
     # ---------------------------------------------------------------- contact generation API
     if not args.no_collide:

@@ -816,10 +816,26 @@ __device__ __forceinline__ MjPair mj_pair_params(const NaModel& M, int a, int b)
   return r;
 }
 
+// Selection key of a contact when a world generates more contacts than its set holds: the gap
+// distance phi0 (as an order-preserving integer), then the pair's segment positions (lower,
+// higher) and the manifold slot.  Keys are unique within a world, so "the cap smallest keys" is
+// one set whatever order the contacts come in.
+__device__ __forceinline__ unsigned long long contact_key(float phi0, int sa, int sb, int sub) {
+  unsigned int b = __float_as_uint(phi0);
+  b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  const unsigned int lo = (unsigned int)min(sa, sb), hi = (unsigned int)max(sa, sb);
+  return ((unsigned long long)b << 32) | (lo << 16) | (hi << 8) | (unsigned int)sub;
+}
+
 // Stage 3: narrow phase over the pair list, compacted into the LDS contact set in pair order.
-// Returns the number of contacts generated (may exceed `cap`; extra contacts are dropped).
-// `cap` must not exceed the capacity of the contact set.  With `mj` (MuJoCo contact law) each
-// contact also gets its MuJoCo row parameters, and k / mu hold its impedance and friction.
+// Returns the number of contacts generated.  Up to `cap` (which must not exceed the capacity of
+// the contact set) they are all kept in pair order; beyond it the set keeps the `cap` contacts
+// with the smallest gap distance (contact_key): each further contact replaces the set's largest
+// key when its own is smaller.  The reference appends every contact to one pool of 128 per world
+// (rigid_contact_max, solver_sap_adaptive.py:573-577) and drops what overflows the pool; a world
+// here never holds more than its LDS set, and what it drops is the farthest contacts.  With `mj`
+// (MuJoCo contact law) each contact also gets its MuJoCo row parameters, and k / mu hold its
+// impedance and friction.
 template <class CS, class MJ = MjContactParams<1>>
 __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const SegLDS& S, CS& C, MJ* mj = nullptr) {
   int nc = 0;
@@ -832,6 +848,30 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
       uint32_t pr = S.pairs[k];
       narrow_pair(M, S, (int)(pr >> 8), (int)(pr & 0xff), o);
     }
+    // this lane's contact of manifold slot q into set slot `slot`
+    auto put = [&](int slot, int q, int ba, int bb, float kk, float mu, const MjPair& mp) {
+      C.ba[slot] = (int8_t)(ba < 0 ? -1 : ba - b0);
+      C.bb[slot] = (int8_t)(bb < 0 ? -1 : bb - b0);
+      C.sa[slot] = (uint8_t)o.sa;
+      C.sb[slot] = (uint8_t)o.sb;
+      C.sub[slot] = (uint8_t)q;
+      C.n[0][slot] = o.m.nrm[q].x; C.n[1][slot] = o.m.nrm[q].y; C.n[2][slot] = o.m.nrm[q].z;
+      C.p[0][slot] = o.m.pos[q].x; C.p[1][slot] = o.m.pos[q].y; C.p[2][slot] = o.m.pos[q].z;
+      C.dist[slot] = o.m.dist[q];
+      C.phi0[slot] = o.phi0[q];
+      C.k[slot] = kk;
+      C.mu[slot] = mu;
+      C.ra[slot] = o.ra;
+      C.rb[slot] = o.rb;
+      if (mj) {
+        // the row position: dist - margins, or the surface distance where margins are zeroed
+        const float pos = M.mj_zero_margins ? o.m.dist[q] : o.phi0[q];
+        C.k[slot] = mj_impedance(mp.solimp, pos);
+        mj->tc[slot] = mp.solref[0];
+        mj->dr[slot] = mp.solref[1];
+        mj->dmax[slot] = fminf(fmaxf(mp.solimp[1], NA_MJ_MINIMP), NA_MJ_MAXIMP);
+      }
+    };
     int cnt = __popc(o.mask);
     int incl = wave_incl_scan(cnt);
     int excl = incl - cnt;
@@ -850,30 +890,56 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
 #pragma unroll
       for (int q = 0; q < NA_MF; ++q) {
         if (!((o.mask >> q) & 1)) continue;
-        if (slot < cap) {
-          C.ba[slot] = (int8_t)(ba < 0 ? -1 : ba - b0);
-          C.bb[slot] = (int8_t)(bb < 0 ? -1 : bb - b0);
-          C.sa[slot] = (uint8_t)o.sa;
-          C.sb[slot] = (uint8_t)o.sb;
-          C.sub[slot] = (uint8_t)q;
-          C.n[0][slot] = o.m.nrm[q].x; C.n[1][slot] = o.m.nrm[q].y; C.n[2][slot] = o.m.nrm[q].z;
-          C.p[0][slot] = o.m.pos[q].x; C.p[1][slot] = o.m.pos[q].y; C.p[2][slot] = o.m.pos[q].z;
-          C.dist[slot] = o.m.dist[q];
-          C.phi0[slot] = o.phi0[q];
-          C.k[slot] = kk;
-          C.mu[slot] = mu;
-          C.ra[slot] = o.ra;
-          C.rb[slot] = o.rb;
-          if (mj) {
-            // the row position: dist - margins, or the surface distance where margins are zeroed
-            const float pos = M.mj_zero_margins ? o.m.dist[q] : o.phi0[q];
-            C.k[slot] = mj_impedance(mp.solimp, pos);
-            mj->tc[slot] = mp.solref[0];
-            mj->dr[slot] = mp.solref[1];
-            mj->dmax[slot] = fminf(fmaxf(mp.solimp[1], NA_MJ_MINIMP), NA_MJ_MAXIMP);
+        if (slot < cap) put(slot, q, ba, bb, kk, mu, mp);
+        slot++;
+      }
+    }
+    if (nc + tot > cap) {
+      // contacts past the set's capacity, lane by lane in mask order: each replaces the set's
+      // largest key if its own is smaller (the wave recomputes the largest after each replacement)
+      wave_sync();
+      const int first = nc + excl;  // this lane's first slot in generation order
+      unsigned long long lanes = __ballot(cnt > 0 && first + cnt > cap);
+      while (lanes) {
+        const int src = __ffsll((long long)lanes) - 1;
+        lanes &= lanes - 1;
+        const int m = __shfl(o.mask, src, NA_WAVE);
+        const int f = __shfl(first, src, NA_WAVE);
+        const int psa = __shfl(o.sa, src, NA_WAVE), psb = __shfl(o.sb, src, NA_WAVE);
+        int r = 0;
+        for (int q = 0; q < NA_MF; ++q) {
+          if (!((m >> q) & 1)) continue;
+          const int gen = f + r++;
+          if (gen < cap) continue;
+          const float cphi = __shfl(o.phi0[q], src, NA_WAVE);
+          const unsigned long long ck = contact_key(cphi, psa, psb, q);
+          // the set's largest key and its slot (keys are unique)
+          unsigned long long best = 0ull;
+          int arg = 0;
+          for (int c = lane_id(); c < cap; c += NA_WAVE) {
+            const unsigned long long k2 = contact_key(C.phi0[c], C.sa[c], C.sb[c], C.sub[c]);
+            if (k2 > best) { best = k2; arg = c; }
+          }
+          for (int off = NA_WAVE / 2; off > 0; off >>= 1) {
+            const unsigned long long ob = __shfl_xor(best, off, NA_WAVE);
+            const int oa = __shfl_xor(arg, off, NA_WAVE);
+            if (ob > best) { best = ob; arg = oa; }
+          }
+          if (ck < best) {
+            if (lane_id() == src) {
+              const int ga = S.gidx[o.sa], gb = S.gidx[o.sb];
+              const int ba = M.shape_body[ga], bb = M.shape_body[gb];
+              MjPair mp;
+              float mu = fmaxf(M.shape_mu[ga], M.shape_mu[gb]);
+              if (mj) {
+                mp = mj_pair_params(M, ga, gb);
+                mu = mp.condim == 1 ? 0.f : mp.mu;
+              }
+              put(arg, q, ba, bb, 0.5f * (M.shape_ke[ga] + M.shape_ke[gb]), mu, mp);
+            }
+            wave_sync();
           }
         }
-        slot++;
       }
     }
     nc += tot;
@@ -881,6 +947,7 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
   wave_sync();
   return nc;
 }
+
 
 __device__ __forceinline__ int stage_broad(const NaModel& M, int w, int nseg, int nl, int broad_phase, SegLDS& S) {
   if (broad_phase == 2) return stage_broad_explicit(M, w, nl, S);

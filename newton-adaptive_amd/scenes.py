@@ -123,9 +123,9 @@ def uneven_pile(world_count: int = 4096, seed: int = 0, device="cuda:0", max_pri
     """Uneven worlds: N ~ U{min_prims..max_prims} primitives from {sphere, box, capsule},
     size ~ lognormal(ln 0.08 m, 0.4), dropped as a random pile into a 1 m^2 walled bin.
     Drop heights are chosen so that no two bounding spheres start overlapping.  Every shape
-    uses a 2 cm contact gap (``ShapeConfig.gap``) so that a settled pile of 64 primitives stays
-    within the 128-contact per-world set; the builder default (0.1 m) produces up to ~300
-    speculative contacts per 32-primitive world."""
+    uses a 2 cm contact gap (``ShapeConfig.gap``) by default, which keeps a settled pile within
+    the 128-contact per-world set; with the builder default (0.1 m) a 32-primitive world
+    produces up to ~300 speculative contacts, of which it keeps the 128 closest."""
     builder = ModelBuilder()
     for w in range(world_count):
         rng = np.random.default_rng(seed + w)

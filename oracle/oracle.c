@@ -857,6 +857,30 @@ int or_collide_shapes(int ta, const double* xa, const double* sa_, int tb, const
 /* Contact generation of one world at pose body_q (that world's bodies only):
  * AABBs -> brute-force pairs -> narrow phase.
  * Contacts are ordered by (shape_a, shape_b) of the canonical pair, then slot. */
+/* Segment position of shape s in world w (local shapes in order, then the global shapes). */
+static int segment_position(const or_model* M, int w, int s) {
+  int s0 = M->world_shape_start[w], nl = M->world_shape_start[w + 1] - s0;
+  if (s >= s0 && s < s0 + nl) return s - s0;
+  for (int g = 0; g < M->global_count; ++g)
+    if (M->global_shapes[g] == s) return nl + g;
+  return 0;
+}
+/* Selection key of a contact when a world generates more than `cap` contacts (the device's
+ * contact_key): phi0 as an order-preserving integer of its float32 value, then the pair's
+ * segment positions (lower, higher) and the manifold slot.  Keys are unique within a world. */
+static uint64_t contact_key(const or_model* M, int w, const or_contact* c) {
+  float f = (float)c->phi0;
+  uint32_t b;
+  memcpy(&b, &f, 4);
+  b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  int pa = segment_position(M, w, c->shape_a), pb = segment_position(M, w, c->shape_b);
+  uint32_t lo = (uint32_t)(pa < pb ? pa : pb), hi = (uint32_t)(pa < pb ? pb : pa);
+  return ((uint64_t)b << 32) | (lo << 16) | (hi << 8) | (uint32_t)c->sub;
+}
+
+/* Contacts of world w at its body poses, in pair order.  Returns the number generated; up to
+ * `cap` are all kept, beyond it the set keeps the `cap` smallest keys (each further contact
+ * replaces the largest key of the set when its own is smaller), as the device's narrow phase. */
 int or_collide_world(const or_model* M, int w, const double* body_q, or_contact* out, int cap, int* n_pairs) {
   int n = segment_size(M, w);
   double* lo = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
@@ -872,7 +896,17 @@ int or_collide_world(const or_model* M, int w, const double* body_q, or_contact*
   for (int p = 0; p < np; ++p) {
     int k = narrow_pair(M, body_q, b0, pairs[2 * p], pairs[2 * p + 1], tmp);
     for (int i = 0; i < k; ++i) {
-      if (cnt < cap) out[cnt] = tmp[i];
+      if (cnt < cap) {
+        out[cnt] = tmp[i];
+      } else if (cap > 0) {
+        uint64_t ck = contact_key(M, w, &tmp[i]), best = 0;
+        int arg = 0;
+        for (int c = 0; c < cap; ++c) {
+          uint64_t k2 = contact_key(M, w, &out[c]);
+          if (k2 > best) { best = k2; arg = c; }
+        }
+        if (ck < best) out[arg] = tmp[i];
+      }
       cnt++;
     }
   }

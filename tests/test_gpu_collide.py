@@ -11,6 +11,8 @@
 * determinism: two collide calls give bit-identical Contacts arrays.
 """
 
+import importlib
+
 import numpy as np
 import pytest
 import torch
@@ -373,3 +375,49 @@ def test_deterministic_pipelines_agree_over_a_rollout(na):
             assert torch.equal(getattr(ca, k)[:n], getattr(cb, k)[:n]), f"frame {frame}: {k}"
         assert torch.equal(pa._sort_key_array, pb._sort_key_array), f"frame {frame}: sort keys"
     assert seen > 0
+
+
+def test_contact_overflow_keeps_the_closest_like_the_oracle(na, O):
+    """Worlds with far more contacts than their 128-contact set (27 boxes packed 4 mm apart at the
+    builder's default 10 cm gap; under the 1024 candidate pairs a world holds): the device keeps
+    the same 128 closest contacts as the oracle (sorted gap distances within 1e-5 m, so that a
+    rounding-level swap at the threshold passes)."""
+    G = importlib.import_module("test_oracle_geometry")
+    m = G._box_grid(na, worlds=4, side=3, device=DEV)
+    state = m.state()
+    pipe = na.CollisionPipeline(m)
+    c = pipe.contacts()
+    pipe.collide(state, c)
+    torch.cuda.synchronize()
+    wc = c.rigid_contact_world_count.cpu().numpy()
+    n = int(c.rigid_contact_count.item())
+    assert n == int(np.minimum(wc, 128).sum())
+    s0 = c.rigid_contact_shape0.cpu().numpy()[:n]
+    s1 = c.rigid_contact_shape1.cpu().numpy()[:n]
+    nrm = c.rigid_contact_normal.cpu().numpy()[:n].astype(np.float64)
+    p0 = c.rigid_contact_point0.cpu().numpy()[:n].astype(np.float64)
+    p1 = c.rigid_contact_point1.cpu().numpy()[:n].astype(np.float64)
+    o0 = c.rigid_contact_offset0.cpu().numpy()[:n].astype(np.float64)
+    o1 = c.rigid_contact_offset1.cpu().numpy()[:n].astype(np.float64)
+    om = O.OracleModel(m)
+    q = m.host("body_q").astype(np.float32).astype(np.float64)
+    shape_body = m.host("shape_body")
+
+    def world(b, p, off):
+        if b < 0:
+            return p + off
+        R = _qmat(q[b][3:])
+        return R @ p + q[b][:3] + R @ off
+
+    start = 0
+    for w in range(m.world_count):
+        k, _, rows = O.collide_world(om, w, q)
+        assert k > 128 and wc[w] == 128
+        g = range(start, start + 128)
+        start += 128
+        d_dev = np.sort([np.dot(world(shape_body[s1[i]], p1[i], o1[i]) - world(shape_body[s0[i]], p0[i], o0[i]),
+                                nrm[i]) for i in g])
+        d_ref = np.sort(rows[:, 10])
+        print(f"world {w}: {k} contacts, kept up to {d_ref.max():.4f} m, "
+              f"max |d| difference {np.abs(d_dev - d_ref).max():.2e}")
+        assert np.abs(d_dev - d_ref).max() < 1e-5

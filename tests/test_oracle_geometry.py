@@ -287,3 +287,49 @@ def test_aabb_closed_forms(O, na):
     assert np.allclose(lo[1], [-0.05, -0.05, 1 - 0.25], atol=1e-7)
     assert np.allclose(hi[1], [0.05, 0.05, 1 + 0.25], atol=1e-7)
     assert np.allclose(hi[2], [1e6 + 0.1] * 3, rtol=1e-7)  # infinite plane: bounding-sphere cube + gap
+
+
+def _box_grid(na, worlds=2, side=4, half=0.04, spacing=0.004, gap=0.1, device="cpu", seed=5):
+    """Dense worlds of side^3 boxes with small jitter, far more candidate contacts than a world's
+    128-contact set at a 10 cm gap (the builder's default)."""
+    B = na.ModelBuilder
+    builder = B()
+    rng = np.random.default_rng(seed)
+    cfg = B.ShapeConfig(density=1000.0, ke=1.0e4, mu=0.5, gap=gap)
+    for _ in range(worlds):
+        builder.begin_world()
+        pitch = 2 * half + spacing
+        for i in range(side):
+            for j in range(side):
+                for k in range(side):
+                    p = np.array([i, j, k]) * pitch + rng.uniform(-0.3, 0.3, 3) * spacing + [0.0, 0.0, half + 0.01]
+                    yaw = rng.uniform(-0.2, 0.2)
+                    b = builder.add_body(xform=(*p, 0.0, 0.0, math.sin(0.5 * yaw), math.cos(0.5 * yaw)))
+                    builder.add_shape_box(b, hx=half, hy=half, hz=half, cfg=cfg)
+        builder.end_world()
+    builder.add_ground_plane(cfg=B.ShapeConfig(ke=1.0e4, mu=0.8, gap=gap))
+    return builder.finalize(device)
+
+
+def test_contact_overflow_keeps_the_closest(O, na):
+    """A world with more contacts than its set keeps the 128 with the smallest gap distance: the
+    kept rows are rows of the full list, the count reports every contact, and no dropped contact
+    is closer than a kept one (float32 distances, as the device compares them)."""
+    m = _box_grid(na)
+    om = O.OracleModel(m)
+    q = m.host("body_q").astype(np.float32).astype(np.float64)
+    for w in range(m.world_count):
+        n_all, _, full = O.collide_world(om, w, q, cap=8192)
+        n_kept, _, kept = O.collide_world(om, w, q, cap=128)
+        assert n_all == n_kept > 128 and len(kept) == 128
+        key = lambda r: (int(r[0]), int(r[1]), int(r[2]))
+        index = {key(r): r for r in full}
+        assert len(index) == len(full)
+        assert all(key(r) in index and np.array_equal(r, index[key(r)]) for r in kept)
+        kept_keys = {key(r) for r in kept}
+        dropped = np.array([r[10] for r in full if key(r) not in kept_keys], dtype=np.float32)
+        assert len(dropped) == n_all - 128
+        assert np.float32(kept[:, 10]).max() <= dropped.min()
+        # the plain clamp (the first 128 in pair order) would fail that check on these worlds
+        first = np.float32(full[:128, 10])
+        assert first.max() > np.float32(full[128:, 10]).min()
