@@ -279,7 +279,8 @@ __device__ void art_load(const NaModel& M, ArtWorld& aw, ArtLDS& A, const NaCont
 
 // Forward kinematics at the coordinates held by lanes k < nq (`q`): poses, COMs, world
 // inertias, motion subspace columns and the f32 collision poses, one depth level at a time.
-__device__ void art_kinematics(const NaModel& M, const ArtWorld& aw, ArtLDS& A, double q) {
+__device__ void art_kinematics(const NaModel& M, const ArtWorld& aw, lds_ptr_t<ArtLDS> Ap, double q) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   const int l = lane_id();
   if (l < aw.nq) A.qe[l] = q;
   wave_sync();
@@ -696,7 +697,8 @@ __device__ void art_delassus_rows(const ArtWorld& aw, ArtLDS& A, int nc) {
 
 // Per-contact set-up of a step h (lane c < nc): frame, Delassus diagonal through the factor of
 // A in A.fa.LA, regularisation, v_hat, activity.
-__device__ void art_contact_setup(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int nc, double h) {
+__device__ void art_contact_setup(const NaConfig& cfg, const ArtWorld& aw, lds_ptr_t<ArtLDS> Ap, int nc, double h) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   const int c = lane_id();
   const int nv = aw.nv;
   if (c < nc) {
@@ -803,9 +805,10 @@ struct ArtStats {
 // `cbias`, mass diagonal `mdiag`, start velocity v0; one value per dof lane) with the nc contacts
 // in A.con: Newton from `guess`.  The velocity comes back in `vout` (lane k < nv), the integrated
 // coordinates in `qout` (lane k < nq).  Returns true when the SAP optimality criterion holds.
-__device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int nc, double h, double v0,
+__device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, lds_ptr_t<ArtLDS> Ap, int nc, double h, double v0,
                          double cbias, double mdiag, double guess, double& vout, double& qout, ArtStats& st,
                          long long* wk) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   const int l = lane_id();
   NA_TIC(t_setup);
   const int nv = aw.nv;
@@ -841,7 +844,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
   bool ok;
   const double vstar = art_factor_solve<double>(A.fa.LA, nv, rhs, ok);
   wave_sync();
-  art_contact_setup(cfg, aw, A, nc, h);
+  art_contact_setup(cfg, aw, lds_ptr(A), nc, h);
   const double Dk = dof ? 1.0 / sqrt(diag) : 0.0;
   if (dof) A.x[l] = vstar;
   wave_sync();
@@ -994,13 +997,14 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
     const d3 lu = d3{A.dvc[0][lc], A.dvc[1][lc], A.dvc[2][lc]};
     const d3 lv = d3{A.vc0[0][lc], A.vc0[1][lc], A.vc0[2][lc] - A.vhat[lc]};
     const double liRt = A.iRt[lc], liRn = A.iRn[lc], lRt = A.Rt[lc], lRn = A.Rn[lc], lmu = A.con.mu[lc];
+    const SapLineK lk = sap_line_k(lmu, lRt, lRn, liRt, liRn);
     alpha = -1.0;
     if (cfg.line_search == 1) {
       alpha = armijo_decay(dl0, [&](double a) {
         double cc = 0.0, d1 = 0.0, d2 = 0.0;
         if (lact) {
           const d3 y = d3{-(lv.x + a * lu.x) * liRt, -(lv.y + a * lu.y) * liRt, -(lv.z + a * lu.z) * liRn};
-          cc = sap_line(lmu, lRt, lRn, liRt, liRn, y, lu, d1, d2) - lc0;
+          cc = sap_line(lk, y, lu, d1, d2) - lc0;
         }
         return a * b0 + 0.5 * a * a * aa + wave_sum_d(cc);
       });
@@ -1018,7 +1022,7 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
         double dlc = 0.0, d2c = 0.0;
         if (lact) {
           const d3 y = d3{-(lv.x + alpha * lu.x) * liRt, -(lv.y + alpha * lu.y) * liRt, -(lv.z + alpha * lu.z) * liRn};
-          sap_line(lmu, lRt, lRn, liRt, liRn, y, lu, dlc, d2c);
+          sap_line(lk, y, lu, dlc, d2c);
         }
         const double dl = b0 + alpha * aa + wave_sum_d(dlc);
         const double ddl = aa + wave_sum_d(d2c);
@@ -1073,8 +1077,9 @@ __device__ bool art_step(const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int
 }
 
 // Contacts of the world at the kinematics in LDS; returns the kept count (<= the capacity).
-__device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, int& overflow,
+__device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, lds_ptr_t<ArtLDS> Ap, int& overflow,
                            long long* wk) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   NA_TIC(t0);
   stage_aabb(M, aw.w, aw.nseg, aw.nl, aw.b0, &A.bqf[0][0], NA_ART_NB, A.seg);
   const int np = stage_broad(M, aw.w, aw.nseg, aw.nl, cfg.broad_phase, A.seg);
@@ -1093,42 +1098,43 @@ __device__ int art_collide(const NaModel& M, const NaConfig& cfg, const ArtWorld
 // builds (parity entry point only) also write both passes' contact sets to
 // dump[2][NA_MAX_CONTACTS][NA_DUMP_FLOATS] and their counts to dump_n[2].
 template <bool DUMP = false>
-__device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, ArtLDS& A, double q,
+__device__ double art_attempt(const NaModel& M, const NaConfig& cfg, const ArtWorld& aw, lds_ptr_t<ArtLDS> Ap, double q,
                               double v, double dt, double& qf, double& vf, double& qd, double& vd, bool& ok,
                               int& overflow, int& nc_out, long long* wk, float* dump = nullptr,
                               int32_t* dump_n = nullptr) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   const int l = lane_id();
   ArtStats st;
   double mdiag;
   NA_TIC(t_kin);
-  art_kinematics(M, aw, A, q);
+  art_kinematics(M, aw, lds_ptr(A), q);
   double cb = art_dynamics(M, aw, A, v, mdiag);
   NA_TOC(wk[10], t_kin);
-  int nc = art_collide(M, cfg, aw, A, overflow, wk);
+  int nc = art_collide(M, cfg, aw, lds_ptr(A), overflow, wk);
   nc_out = nc;
   if (DUMP) {
     dump_contacts(A.seg, A.con, aw.b0, nc, dump);
     if (l == 0) dump_n[0] = nc;
   }
-  ok = art_step(cfg, aw, A, nc, dt, v, cb, mdiag, v, vf, qf, st, wk);
+  ok = art_step(cfg, aw, lds_ptr(A), nc, dt, v, cb, mdiag, v, vf, qf, st, wk);
   wk[1] += st.iterations;
   wk[5] += 1;
   double err = 0.0;
   if (cfg.mode == 1) {
     const double g = 0.5 * (v + vf);
     double qm, vm;
-    ok = art_step(cfg, aw, A, nc, 0.5 * dt, v, cb, mdiag, g, vm, qm, st, wk) && ok;
+    ok = art_step(cfg, aw, lds_ptr(A), nc, 0.5 * dt, v, cb, mdiag, g, vm, qm, st, wk) && ok;
     wk[1] += st.iterations;
     NA_TIC(t_kin2);
-    art_kinematics(M, aw, A, qm);
+    art_kinematics(M, aw, lds_ptr(A), qm);
     cb = art_dynamics(M, aw, A, vm, mdiag);
     NA_TOC(wk[10], t_kin2);
-    nc = art_collide(M, cfg, aw, A, overflow, wk);
+    nc = art_collide(M, cfg, aw, lds_ptr(A), overflow, wk);
     if (DUMP) {
       dump_contacts(A.seg, A.con, aw.b0, nc, dump + NA_MAX_CONTACTS * NA_DUMP_FLOATS);
       if (l == 0) dump_n[1] = nc;
     }
-    ok = art_step(cfg, aw, A, nc, 0.5 * dt, vm, cb, mdiag, vf, vd, qd, st, wk) && ok;
+    ok = art_step(cfg, aw, lds_ptr(A), nc, 0.5 * dt, vm, cb, mdiag, vf, vd, qd, st, wk) && ok;
     wk[1] += st.iterations;
     wk[5] += 2;
     double e = 0.0;

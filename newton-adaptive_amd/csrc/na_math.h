@@ -105,6 +105,36 @@ __device__ __forceinline__ double lane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Wave-uniform copies of values every lane holds alike (settings read through a pointer, loop
+// bounds): the compiler then keeps them in scalar registers and branches on what depends on them
+// without exec-mask bookkeeping.
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uni_d(double v) {
+  long long b = __double_as_longlong(v);
+  int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// LDS-typed pointers for out-of-line functions that work on a world's LDS: a function that takes
+// one and re-derives its references from it (lds_deref) compiles every access to a ds_*
+// instruction with the field offset folded into the instruction.  Through a generic reference
+// parameter the compiler casts each field's generic address to LDS at its use instead, with an
+// address computation per field.
+template <typename T>
+using lds_ptr_t = __attribute__((address_space(3))) T*;
+template <typename T>
+__device__ __forceinline__ lds_ptr_t<T> lds_ptr(T& r) {
+  return (lds_ptr_t<T>)(&r);
+}
+template <typename T>
+__device__ __forceinline__ T& lds_deref(lds_ptr_t<T> p) {
+  return *(T*)p;
+}
+
 struct OpAdd {
   template <typename T>
   __device__ __forceinline__ T operator()(T a, T b) const { return a + b; }

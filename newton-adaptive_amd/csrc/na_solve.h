@@ -151,12 +151,29 @@ __device__ __forceinline__ d3 row_vel_d(const BodyLDS& B, const CRow& r) {
   d3 rel = d3{vb.x + xb.x - va.x - xa.x, vb.y + xb.y - va.y - xa.y, vb.z + xb.z - va.z - xa.z};
   return d3{ddot(dd(r.t1), rel), ddot(dd(r.t2), rel), ddot(dd(r.n), rel)};
 }
+// |y_t| and 1 / |y_t| from s = |y_t|^2: the hardware reciprocal square root estimate refined by
+// two Goldschmidt steps (g -> sqrt(s), h -> 1 / (2 sqrt(s)); each step squares the relative
+// error).  s == 0 gives |y_t| = 0 (and an unused
+// reciprocal); every caller leaves the sliding branch, the only user of 1 / |y_t|, when |y_t| is 0.
+__device__ __forceinline__ void tangent_norm(double s, double& n, double& in) {
+  const double r = __builtin_amdgcn_rsq(s);
+  double g = s * r, h = 0.5 * r;
+  double e = fma(-g, h, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  e = fma(-g, h, 0.5);
+  g = fma(g, e, g);
+  h = fma(h, e, h);
+  n = s > 0.0 ? g : 0.0;
+  in = 2.0 * h;
+}
 // SAP projection onto the friction cone in the R norm (FP64); returns cost 1/2 gamma^T R gamma,
 // writes gamma and the symmetric G = dP/dy R^-1 (00 11 22 01 02 12) in f32.  iRt = 1/Rt,
 // iRn = 1/Rn.  Regions: stiction |y_t| <= mu y_n, separation y_n <= -mu_hat |y_t|, sliding.
 __device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, double iRt, double iRn, d3 y, d3& g,
                                               float* G) {
-  double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
+  double yr, iyr, yn = y.z;
+  tangent_norm(y.x * y.x + y.y * y.y, yr, iyr);
   double muh = mu * Rt * iRn;
   if (yr <= mu * yn) {
     g = y;
@@ -165,7 +182,6 @@ __device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, d
     g = d3{0.0, 0.0, 0.0};
     G[0] = G[1] = G[2] = G[3] = G[4] = G[5] = 0.f;
   } else {
-    double iyr = 1.0 / yr;
     double t0 = y.x * iyr, t1 = y.y * iyr;
     double iden = 1.0 / (1.0 + mu * muh);
     double gn = (yn + muh * yr) * iden;
@@ -180,30 +196,49 @@ __device__ __forceinline__ double sap_project(double mu, double Rt, double Rn, d
   }
   return 0.5 * (Rt * (g.x * g.x + g.y * g.y) + Rn * g.z * g.z);
 }
+// Per-contact constants of the line cost below that stay fixed along a line search:
+// muh = mu Rt / Rn, iden = 1 / (1 + mu muh) and the curvature coefficients a = mu muh iden,
+// c2 = 2 mu iden / Rn, c3 = iden / Rn.
+struct SapLineK {
+  double mu, Rt, Rn, iRt, iRn;
+  double muh, iden, a, c2, c3;
+};
+__device__ __forceinline__ SapLineK sap_line_k(double mu, double Rt, double Rn, double iRt, double iRn) {
+  SapLineK k;
+  k.mu = mu; k.Rt = Rt; k.Rn = Rn; k.iRt = iRt; k.iRn = iRn;
+  k.muh = mu * Rt * iRn;
+  k.iden = 1.0 / (1.0 + mu * k.muh);
+  k.a = mu * k.muh * k.iden;
+  k.c2 = 2.0 * mu * k.iden * iRn;
+  k.c3 = k.iden * iRn;
+  return k;
+}
 // Contact cost along a line: at y (velocity vc), with direction u = J dv in contact velocity,
 // returns cost, adds -gamma.u to `d1` and u^T G u to `d2` (G of the FP64 projection).
-__device__ __forceinline__ double sap_line(double mu, double Rt, double Rn, double iRt, double iRn, d3 y, d3 u,
-                                           double& d1, double& d2) {
-  double yr = sqrt(y.x * y.x + y.y * y.y), yn = y.z;
-  double muh = mu * Rt * iRn;
-  if (yr <= mu * yn) {
+__device__ __forceinline__ double sap_line(const SapLineK& k, d3 y, d3 u, double& d1, double& d2) {
+  double yr, iyr, yn = y.z;
+  tangent_norm(y.x * y.x + y.y * y.y, yr, iyr);
+  if (yr <= k.mu * yn) {
     d1 -= y.x * u.x + y.y * u.y + y.z * u.z;
-    d2 += (u.x * u.x + u.y * u.y) * iRt + u.z * u.z * iRn;
-    return 0.5 * (Rt * (y.x * y.x + y.y * y.y) + Rn * y.z * y.z);
+    d2 += (u.x * u.x + u.y * u.y) * k.iRt + u.z * u.z * k.iRn;
+    return 0.5 * (k.Rt * (y.x * y.x + y.y * y.y) + k.Rn * y.z * y.z);
   }
-  if (yn <= -muh * yr) return 0.0;
-  double iyr = 1.0 / yr;
+  if (yn <= -k.muh * yr) return 0.0;
   double t0 = y.x * iyr, t1 = y.y * iyr;
-  double iden = 1.0 / (1.0 + mu * muh);
-  double gn = (yn + muh * yr) * iden;
-  double gt0 = mu * gn * t0, gt1 = mu * gn * t1;
+  double gn = (yn + k.muh * yr) * k.iden;
+  double mgn = k.mu * gn;
+  double gt0 = mgn * t0, gt1 = mgn * t1;
   d1 -= gt0 * u.x + gt1 * u.y + gn * u.z;
-  double a = mu * muh * iden, b = mu * gn * iyr;
+  double b = mgn * iyr;
   double ut = u.x * t0 + u.y * t1;
   double uu = u.x * u.x + u.y * u.y;
   // u_t^T (a t t^T + b (I - t t^T)) u_t / Rt + 2 u_n (mu/den) t.u_t / Rn + u_n^2 / (den Rn)
-  d2 += (a * ut * ut + b * (uu - ut * ut)) * iRt + 2.0 * u.z * mu * iden * ut * iRn + u.z * u.z * iden * iRn;
-  return 0.5 * (Rt * (gt0 * gt0 + gt1 * gt1) + Rn * gn * gn);
+  d2 += (k.a * ut * ut + b * (uu - ut * ut)) * k.iRt + k.c2 * u.z * ut + k.c3 * u.z * u.z;
+  return 0.5 * (k.Rt * (gt0 * gt0 + gt1 * gt1) + k.Rn * gn * gn);
+}
+__device__ __forceinline__ double sap_line(double mu, double Rt, double Rn, double iRt, double iRn, d3 y, d3 u,
+                                           double& d1, double& d2) {
+  return sap_line(sap_line_k(mu, Rt, Rn, iRt, iRn), y, u, d1, d2);
 }
 // MuJoCo soft-constraint contact (contact law 1; the inner step of SolverMuJoCoAdaptive,
 // mujoco_warp.step, restated from MuJoCo's computation model).  In velocity form a row with
@@ -736,6 +771,7 @@ __device__ __forceinline__ void precond_blocks_g(int nc, BodyLDS& B, const Solve
 struct LsRow {
   d3 w0, dv;
   double iRt, iRn, Rt, Rn, mu, vh;
+  SapLineK k;  // contact law 0
 };
 
 // Line-search setup: contact-space direction dv = J dx (dx in B.vd) into this lane's rows `lr`,
@@ -754,6 +790,7 @@ __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow*
     q.iRt = k[g].iRt; q.iRn = k[g].iRn; q.Rt = k[g].Rt; q.Rn = k[g].Rn; q.mu = k[g].mu;
     q.vh = (double)k[g].vh;
     q.w0 = vc0[g];
+    if (LAW == 0) q.k = sap_line_k(q.mu, q.Rt, q.Rn, q.iRt, q.iRn);
   }
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
@@ -769,8 +806,8 @@ __device__ __forceinline__ void ls_setup_g(int nc, const BodyLDS& B, const CRow*
     } else {
       d3 y0 = d3{-w0.x * q.iRt, -w0.y * q.iRt, -(w0.z - q.vh) * q.iRn};
       d3 y1 = d3{-(w0.x + u.x) * q.iRt, -(w0.y + u.y) * q.iRt, -(w0.z + u.z - q.vh) * q.iRn};
-      ca = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y0, u, a1, a2);
-      cb = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y1, u, b1, b2);
+      ca = sap_line(q.k, y0, u, a1, a2);
+      cb = sap_line(q.k, y1, u, b1, b2);
     }
     if (c < nc) {
       lc0 += ca;
@@ -800,7 +837,7 @@ __device__ __forceinline__ void ls_contacts_g(int nc, const LsRow* lr, double al
     } else {
       d3 y = d3{-(q.w0.x + alpha * dv.x) * q.iRt, -(q.w0.y + alpha * dv.y) * q.iRt,
                 -(q.w0.z + alpha * dv.z - q.vh) * q.iRn};
-      cc = sap_line(q.mu, q.Rt, q.Rn, q.iRt, q.iRn, y, dv, a, b);
+      cc = sap_line(q.k, y, dv, a, b);
     }
     if (c < nc) {
       l += cc;
@@ -875,7 +912,7 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
     }
     pHp = wave_sum(pHp);
     if (!(pHp > 0.f)) break;
-    float alpha = rz / pHp;
+    float alpha = rz * __builtin_amdgcn_rcpf(pHp);
     float rzn = 0.f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) { x[q] += alpha * pp[q]; r[q] -= alpha * Hp[q]; }
@@ -883,7 +920,7 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
 #pragma unroll
     for (int q = 0; q < 6; ++q) rzn += r[q] * z[q];
     rzn = wave_sum(rzn);
-    float beta = rzn / rz;
+    float beta = rzn * __builtin_amdgcn_rcpf(rz);
     rz = rzn;
 #pragma unroll
     for (int q = 0; q < 6; ++q) pp[q] = z[q] + beta * pp[q];
@@ -901,6 +938,8 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
 template <int NG, int LAW>
 __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg, const ContactLDS& C, BodyLDS& B,
                                             const SolveLDS& X, const BodyReg& br, double* v, SolveStats& st) {
+  nb = uni_i(nb);
+  nc = uni_i(nc);
   const int l = lane_id();
   const bool own = l < nb;
   st.iterations = 0;
@@ -910,11 +949,11 @@ __device__ __forceinline__ bool sap_solve_g(int nb, int nc, const NaConfig& cfg,
   st.reason = NA_EXIT_MAX_ITER;
   st.residual = 0.f;
   st.t_eval = st.t_pre = st.t_cg = st.t_ls = 0;
-  // settings in registers for the loops below
-  const int max_iterations = cfg.max_iterations, max_cg = cfg.max_cg;
-  const double opt_abs = cfg.opt_abs_tol, opt_rel = cfg.opt_rel_tol, ls_rel = cfg.ls_rel_tol;
-  const double cg_rel = cfg.cg_rel_tol, forcing = cfg.cg_forcing;
-  const int line_search = cfg.line_search;
+  // settings in scalar registers for the loops below
+  const int max_iterations = uni_i(cfg.max_iterations), max_cg = uni_i(cfg.max_cg);
+  const double opt_abs = uni_f(cfg.opt_abs_tol), opt_rel = uni_f(cfg.opt_rel_tol), ls_rel = uni_f(cfg.ls_rel_tol);
+  const double cg_rel = uni_f(cfg.cg_rel_tol), forcing = uni_f(cfg.cg_forcing);
+  const int line_search = uni_i(cfg.line_search);
   if (l == 0) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) B.v[k][NA_DUMMY] = 0.f;

@@ -362,8 +362,9 @@ __device__ __forceinline__ WorldCtx world_ctx(const NaModel& M, int w) {
 }
 
 // Contacts of the world at this lane's body pose q (f64).  Returns the contact count kept.
-__device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, const double* q,
+__device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, lds_ptr_t<WorldLDS> Sp, const double* q,
                              int& overflow, long long* wk) {
+  WorldLDS& S = lds_deref<WorldLDS>(Sp);
   const int l = lane_id();
   NA_TIC(t0);
   if (l < wc.nb) {
@@ -385,10 +386,11 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
 // One convex step of size h from (q, v0) with the contacts in LDS; integrates into qo/vo.
 // The Newton iterate starts at guess64 when given (FP64), else at guess; v64, when given,
 // receives the converged FP64 velocity.
-__device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, WorldLDS& S, int nc,
+__device__ bool world_solve(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, lds_ptr_t<WorldLDS> Sp, int nc,
                             const double* q, const float* v0, const float* f, double h, const float* guess, double* qo,
                             float* vo, long long* wk, SolveStats* out_st = nullptr, const double* guess64 = nullptr,
                             double* v64 = nullptr) {
+  WorldLDS& S = lds_deref<WorldLDS>(Sp);
   const int l = lane_id();
   const bool own = l < wc.nb;
   NA_TIC(t0);
@@ -452,7 +454,7 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
                                 float* dump = nullptr, int32_t* dump_n = nullptr) {
   const int l = lane_id();
   const bool own = l < wc.nb;
-  int nc = world_collide(M, cfg, wc, S, q, overflow, wk);
+  int nc = world_collide(M, cfg, wc, lds_ptr(S), q, overflow, wk);
   nc_out = nc;
   if (DUMP) {
     dump_contacts(S.seg, S.con, wc.b0, nc, dump);
@@ -462,7 +464,7 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
   const float hh = (float)(0.5 * dt);
 #pragma unroll
   for (int k = 0; k < 6; ++k) g0[k] = tr ? v[k] + hh * tr[k] : v[k];
-  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
+  ok = world_solve(M, cfg, wc, lds_ptr(S), nc, q, v, f, dt, g0, qf, vf, wk);
   double err = 0.0;
   if (cfg.mode == 1) {
     float g1[6];
@@ -471,8 +473,8 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
     const float qh = (float)(0.25 * dt);
 #pragma unroll
     for (int k = 0; k < 6; ++k) g1[k] = cfg.warm_start == 2 ? (tr ? v[k] + qh * tr[k] : v[k]) : 0.5f * (v[k] + vf[k]);
-    ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
-    int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
+    ok = world_solve(M, cfg, wc, lds_ptr(S), nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
+    int nc2 = world_collide(M, cfg, wc, lds_ptr(S), qm, overflow, wk);
     if (DUMP) {
       dump_contacts(S.seg, S.con, wc.b0, nc2, dump + NA_MAX_CONTACTS * NA_DUMP_FLOATS);
       if (l == 0) dump_n[1] = nc2;
@@ -480,7 +482,7 @@ __device__ double world_attempt(const NaModel& M, const NaConfig& cfg, const Wor
     float g2[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) g2[k] = cfg.warm_start ? 2.f * vm[k] - v[k] : vf[k];
-    ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
+    ok = world_solve(M, cfg, wc, lds_ptr(S), nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
     double e = 0.0;
     if (own) {
 #pragma unroll
@@ -591,7 +593,7 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
                                                   int& overflow, int& nc_out, long long* wk) {
   const int l = lane_id();
   const bool own = l < wc.nb;
-  int nc = world_collide(M, cfg, wc, S, q, overflow, wk);
+  int nc = world_collide(M, cfg, wc, lds_ptr(S), q, overflow, wk);
   nc_out = nc;
   float g0[6], g1[6];
   const float hh = (float)(0.5 * dt);
@@ -618,7 +620,7 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
   wave_sync();
   if (l == 0) T.seq = seq;
   if (!alone) tile_signal<MODE>(&T.req, seq);  // the second wave starts the first half step
-  ok = world_solve(M, cfg, wc, S, nc, q, v, f, dt, g0, qf, vf, wk);
+  ok = world_solve(M, cfg, wc, lds_ptr(S), nc, q, v, f, dt, g0, qf, vf, wk);
   if (!alone && tile_await<MODE>(&T.done, seq)) {  // its result is in
     const TileOut& out = tile_out(S1);
 #pragma unroll
@@ -632,13 +634,13 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
     // the second wave did not answer in time (LDS-flag protocol only): the first half step here,
     // the same solve with the same inputs, and no more requests to that wave
     if (l == 0) T.dead = 1;
-    ok = world_solve(M, cfg, wc, S, nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
+    ok = world_solve(M, cfg, wc, lds_ptr(S), nc, q, v, f, 0.5 * dt, g1, qm, vm, wk) && ok;
   }
-  int nc2 = world_collide(M, cfg, wc, S, qm, overflow, wk);
+  int nc2 = world_collide(M, cfg, wc, lds_ptr(S), qm, overflow, wk);
   float g2[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) g2[k] = 2.f * vm[k] - v[k];
-  ok = world_solve(M, cfg, wc, S, nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
+  ok = world_solve(M, cfg, wc, lds_ptr(S), nc2, qm, vm, f, 0.5 * dt, g2, qd, vd, wk) && ok;
   double e = 0.0;
   if (own) {
 #pragma unroll
@@ -696,7 +698,7 @@ __device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, 
     long long wk[NA_WORK] = {};
     double qo[7];
     float vo[6];
-    const bool ok = world_solve(M, wcfg, wc, S1, nc, q, v, f, h, g, qo, vo, wk);
+    const bool ok = world_solve(M, wcfg, wc, lds_ptr(S1), nc, q, v, f, h, g, qo, vo, wk);
     wave_sync();
     TileOut& out = tile_out(S1);
 #pragma unroll
@@ -1196,7 +1198,7 @@ __global__ __launch_bounds__(64) void k_plain_iter(NaModel M, NaConfig cfg, Ctrl
       load_body(wc, q64, nullptr, body_qd, nullptr, q, v, f);
       long long wk[NA_WORK] = {};
       int ovf = 0;
-      for (int pass = 0; pass < 2; ++pass) world_collide(M, cfg, wc, S, q, ovf, wk);
+      for (int pass = 0; pass < 2; ++pass) world_collide(M, cfg, wc, lds_ptr(S), q, ovf, wk);
     }
     return;
   }
@@ -1271,7 +1273,7 @@ __global__ __launch_bounds__(64) void k_collide_api(NaModel M, NaConfig cfg, con
   long long wk[NA_WORK] = {};
   NaConfig c2 = cfg;
   c2.max_contacts = min(per_world, NA_MAX_CONTACTS);
-  int nc = world_collide(M, c2, wc, S, q, ovf, wk);
+  int nc = world_collide(M, c2, wc, lds_ptr(S), q, ovf, wk);
   if (l == 0) world_counts[w] = nc;
   // write_contact (collide.py:62-147): body-frame points and offsets
   for (int c = l; c < nc; c += NA_WAVE) {
@@ -1502,7 +1504,7 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
   if (wc.nb == 0) return;
   long long wk[NA_WORK] = {};
   int ovf = 0;
-  int nc = world_collide(M, cfg, wc, S, q, ovf, wk);
+  int nc = world_collide(M, cfg, wc, lds_ptr(S), q, ovf, wk);
   SolveStats st;
   double g64[6], v64[6];
   const bool own = l < wc.nb;
@@ -1510,7 +1512,7 @@ __global__ __launch_bounds__(64) void k_substep(NaModel M, NaConfig cfg, const d
 #pragma unroll
     for (int k = 0; k < 6; ++k) g64[k] = own ? guess[6 * (wc.b0 + l) + k] : 0.0;
   }
-  bool ok = world_solve(M, cfg, wc, S, nc, q, v, ff, h[w], v, qo, vo, wk, &st,
+  bool ok = world_solve(M, cfg, wc, lds_ptr(S), nc, q, v, ff, h[w], v, qo, vo, wk, &st,
                         guess ? g64 : nullptr, v64);
   if (own) {
     int gb = wc.b0 + l;
@@ -1598,7 +1600,7 @@ __device__ int art_iteration(const NaModel& M, const NaConfig& cfg, const ArtWor
   bool ok = true;
   int ovf = 0, nc = 0;
   double err = 0.0;
-  if (step > 0.0 && aw.nb > 0) err = art_attempt(M, cfg, aw, A, q, v, step, qf, vf, qd, vd, ok, ovf, nc, wk);
+  if (step > 0.0 && aw.nb > 0) err = art_attempt(M, cfg, aw, lds_ptr(A), q, v, step, qf, vf, qd, vd, ok, ovf, nc, wk);
   if (ovf) c.overflow = 1;
   int inc = 0;
   if (controller_update(cfg, c, err, ok, step, eff_max, inc)) {
@@ -1612,7 +1614,7 @@ __device__ int art_iteration(const NaModel& M, const NaConfig& cfg, const ArtWor
 __device__ void art_store_bodies(const NaModel& M, const ArtWorld& aw, ArtLDS& A, double q, double v, float* body_q,
                                  float* body_qd) {
   const int l = lane_id();
-  art_kinematics(M, aw, A, q);
+  art_kinematics(M, aw, lds_ptr(A), q);
   if (l < aw.nv) A.x[l] = v;
   wave_sync();
   art_body_velocity(aw, A);
@@ -1702,7 +1704,8 @@ struct ArtServeState {
 // One articulated world's turn of a serving launch (the semantics of serve_world).
 __device__ __noinline__ void serve_world_art(const NaModel& M, const NaConfig& cfg, const Ctrl& K, const ArtServeState& X,
                                 double dt_outer, double eff_max, const ServeSlots& SV, int w, long long t_end,
-                                int max_iters, int32_t* unfinished, ArtLDS& A) {
+                                int max_iters, int32_t* unfinished, lds_ptr_t<ArtLDS> Ap) {
+  ArtLDS& A = lds_deref<ArtLDS>(Ap);
   const int l = lane_id();
   ArtWorld aw = art_world(M, w);
   if (aw.nb == 0) return;
@@ -1790,7 +1793,7 @@ __global__ __launch_bounds__(64) void k_art_serve(NaModel M, NaConfig cfg, Ctrl 
     if (slot >= n_active) break;
     const int w = active[slot];
     if (SV.status[w] != 0) continue;
-    serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, A);
+    serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, lds_ptr(A));
   }
 }
 
@@ -1818,7 +1821,7 @@ __global__ __launch_bounds__(64) void k_serve_mixed(NaModel M, NaConfig cfg, Ctr
     if (SV.status[w] != 0) continue;
     const int b0 = M.world_body_start[w];
     const bool art = M.world_body_end[w] > b0 && M.joint_type[b0] != 0;
-    if (art) serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, Ua);
+    if (art) serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, lds_ptr(Ua));
     else serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, Uf);
   }
 }
@@ -1842,7 +1845,7 @@ __device__ __forceinline__ void serve_mixed_world(const NaModel& M, const NaConf
   const bool art = M.world_body_end[w] > b0 && M.joint_type[b0] != 0;
   if (art)
     serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
-                    *reinterpret_cast<ArtLDS*>(block));
+                    lds_ptr(*reinterpret_cast<ArtLDS*>(block)));
   else
     serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
                 *reinterpret_cast<WorldLDS*>(block));
@@ -1885,7 +1888,7 @@ __global__ __launch_bounds__(192) void k_serve_mixed_tile(NaModel M, NaConfig cf
       const int b0 = M.world_body_start[w];
       if (M.world_body_end[w] > b0 && M.joint_type[b0] != 0)
         serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
-                        *reinterpret_cast<ArtLDS*>(mix0));
+                        lds_ptr(*reinterpret_cast<ArtLDS*>(mix0)));
       else
         serve_world<NA_TILE_FLAGS>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
                           unfinished, W0, &W1, &T);
@@ -1928,12 +1931,12 @@ __global__ __launch_bounds__(64) void k_art_substep(NaModel M, NaConfig cfg, con
   long long wk[NA_WORK] = {};
   int ovf = 0;
   double mdiag;
-  art_kinematics(M, aw, A, q);
+  art_kinematics(M, aw, lds_ptr(A), q);
   const double cb = art_dynamics(M, aw, A, v, mdiag);
-  const int nc = art_collide(M, cfg, aw, A, ovf, wk);
+  const int nc = art_collide(M, cfg, aw, lds_ptr(A), ovf, wk);
   ArtStats st;
   double vo, qo;
-  const bool ok = art_step(cfg, aw, A, nc, h[w], v, cb, mdiag, v, vo, qo, st, wk);
+  const bool ok = art_step(cfg, aw, lds_ptr(A), nc, h[w], v, cb, mdiag, v, vo, qo, st, wk);
   if (l < aw.nq) q_out[aw.q0 + l] = qo;
   if (l < aw.nv) qd_out[aw.d0 + l] = vo;
   if (l == 0) {
@@ -1969,10 +1972,10 @@ __global__ __launch_bounds__(64) void k_art_attempt(NaModel M, NaConfig cfg, con
   double e;
   if (cdump) {
     if (l == 0) cdump_n[2 * w + 1] = 0;  // fixed mode has no midpoint pass
-    e = art_attempt<true>(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk,
+    e = art_attempt<true>(M, cfg, aw, lds_ptr(A), q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk,
                           cdump + (long long)w * 2 * NA_MAX_CONTACTS * NA_DUMP_FLOATS, cdump_n + 2 * w);
   } else {
-    e = art_attempt(M, cfg, aw, A, q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk);
+    e = art_attempt(M, cfg, aw, lds_ptr(A), q, v, dt[w], qf, vf, qd, vd, ok, ovf, nc, wk);
   }
   if (l < aw.nq) { q_full[aw.q0 + l] = qf; q_dbl[aw.q0 + l] = qd; }
   if (l < aw.nv) { qd_full[aw.d0 + l] = vf; qd_dbl[aw.d0 + l] = vd; }
