@@ -143,6 +143,26 @@ __device__ __forceinline__ f3 row_vel(const BodyLDS& B, const CRow& r) {
   f3 rel = (vb + cross3(wb, r.rb)) - (va + cross3(wa, r.ra));
   return mk(dot3(r.t1, rel), dot3(r.t2, rel), dot3(r.n, rel));
 }
+// relative contact velocity J p in the contact frame (f32), with p held by the body lanes (lane b:
+// body b's six components): each side's components come from its body's lane through the LDS
+// crossbar (ds_bpermute), with no LDS store, fence or load; a static side reads zero.  Every lane
+// of the wave must be active: a permute reads nothing from an inactive lane, so the permutes run
+// unconditionally and the static sides are zeroed after them.
+__device__ __forceinline__ f3 row_vel_lanes(const float* p, const CRow& r) {
+  float a[6], b[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int pk = __float_as_int(p[k]);
+    const float ta = __int_as_float(__builtin_amdgcn_ds_bpermute(r.sa << 2, pk));
+    const float tb = __int_as_float(__builtin_amdgcn_ds_bpermute(r.sb << 2, pk));
+    a[k] = r.ha ? ta : 0.f;
+    b[k] = r.hb ? tb : 0.f;
+  }
+  f3 va = mk(a[0], a[1], a[2]), wa = mk(a[3], a[4], a[5]);
+  f3 vb = mk(b[0], b[1], b[2]), wb = mk(b[3], b[4], b[5]);
+  f3 rel = (vb + cross3(wb, r.rb)) - (va + cross3(wa, r.ra));
+  return mk(dot3(r.t1, rel), dot3(r.t2, rel), dot3(r.n, rel));
+}
 // relative contact velocity J v in the contact frame (f64, from B.vd)
 __device__ __forceinline__ d3 row_vel_d(const BodyLDS& B, const CRow& r) {
   d3 va = d3{B.vd[0][r.sa], B.vd[1][r.sa], B.vd[2][r.sa]}, wa = d3{B.vd[3][r.sa], B.vd[4][r.sa], B.vd[5][r.sa]};
@@ -870,14 +890,10 @@ __device__ __forceinline__ int pcg(BodyLDS& B, const SolveLDS& X, int nc, const 
   const float tol2 = eta2 * rz;
   int it = 0;
   for (; it < max_cg && rz > tol2 && rz > 0.f; ++it) {
-    if (own) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) B.v[q][l] = pp[q];
-    }
-    wave_sync();
+    wave_sync();  // the body lanes' reads of the last product's entries precede this product's stores
     f3 u[NG];
 #pragma unroll
-    for (int q = 0; q < NG; ++q) u[q] = row_vel(B, rw[q]);
+    for (int q = 0; q < NG; ++q) u[q] = row_vel_lanes(pp, rw[q]);
 #pragma unroll
     for (int q = 0; q < NG; ++q) {
       const int c = l + q * NA_WAVE;

@@ -587,10 +587,12 @@ __device__ __forceinline__ TileOut& tile_out(WorldLDS& S) { return *reinterpret_
 // semantics and arithmetic of world_attempt, with the first half step solved by the second wave.
 template <int MODE>
 __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc,
-                                                  WorldLDS& S, WorldLDS& S1, TileHdr& T, const double* q,
-                                                  const float* v, const float* f, const float* tr, double dt,
-                                                  double* qf, float* vf, double* qd, float* vd, bool& ok,
+                                                  lds_ptr_t<WorldLDS> Sp, lds_ptr_t<WorldLDS> S1p, TileHdr& T,
+                                                  const double* q, const float* v, const float* f, const float* tr,
+                                                  double dt, double* qf, float* vf, double* qd, float* vd, bool& ok,
                                                   int& overflow, int& nc_out, long long* wk) {
+  WorldLDS& S = lds_deref<WorldLDS>(Sp);
+  WorldLDS& S1 = lds_deref<WorldLDS>(S1p);
   const int l = lane_id();
   const bool own = l < wc.nb;
   int nc = world_collide(M, cfg, wc, lds_ptr(S), q, overflow, wk);
@@ -659,8 +661,10 @@ __device__ __noinline__ double world_attempt_tile(const NaModel& M, const NaConf
 // barriers pair with the first wave's: one before each request (or the leave message), one after
 // each result.
 template <int MODE>
-__device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, WorldLDS& S0, WorldLDS& S1,
-                                         TileHdr& T) {
+__device__ __noinline__ void tile_helper(const NaModel& M, const NaConfig& cfg, lds_ptr_t<WorldLDS> S0p,
+                                         lds_ptr_t<WorldLDS> S1p, TileHdr& T) {
+  WorldLDS& S0 = lds_deref<WorldLDS>(S0p);
+  WorldLDS& S1 = lds_deref<WorldLDS>(S1p);
   const int l = lane_id();
   for (int32_t seq = 1;; ++seq) {
     if (!tile_await<MODE>(&T.req, seq)) break;
@@ -831,7 +835,7 @@ __device__ int world_iteration(const NaModel& M, const NaConfig& cfg, const Worl
 #pragma unroll
     for (int k = 0; k < 6; ++k) tr[k] = trend && own ? trend[6 * (wc.b0 + l) + k] : 0.f;
     if constexpr (TILE != 0)
-      err = world_attempt_tile<TILE>(M, cfg, wc, S, *S1, *T, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf,
+      err = world_attempt_tile<TILE>(M, cfg, wc, lds_ptr(S), lds_ptr(*S1), *T, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf,
                                nc, wk);
     else
       err = world_attempt(M, cfg, wc, S, q, v, f, trend ? tr : nullptr, step, qf, vf, qd, vd, ok, ovf, nc, wk);
@@ -971,7 +975,7 @@ __global__ __launch_bounds__(128) void k_frame_tile(NaModel M, NaConfig cfg, Ctr
   __shared__ TileHdr T;
   const int wave = threadIdx.x >> 6;
   if (wave == 1) {
-    tile_helper<NA_TILE_BARRIER>(M, cfg, S[0], S[1], T);
+    tile_helper<NA_TILE_BARRIER>(M, cfg, lds_ptr(S[0]), lds_ptr(S[1]), T);
   } else {
     for (;;) {
       int slot = 0;
@@ -1033,8 +1037,9 @@ struct ServeSlots {
 template <int TILE = 0>
 __device__ __noinline__ void serve_world(const NaModel& M, const NaConfig& cfg, const Ctrl& K, double* q64, float* body_q,
                             float* body_qd, const float* body_f, double dt_outer, double eff_max, const ServeSlots& SV,
-                            int w, long long t_end, int max_iters, int32_t* unfinished, WorldLDS& S,
+                            int w, long long t_end, int max_iters, int32_t* unfinished, lds_ptr_t<WorldLDS> Sp,
                             WorldLDS* S1 = nullptr, TileHdr* T = nullptr) {
+  WorldLDS& S = lds_deref<WorldLDS>(Sp);
   WorldCtx wc = world_ctx(M, w);
   NaConfig wcfg = cfg;
   if (SV.tol) {
@@ -1095,7 +1100,7 @@ __global__ __launch_bounds__(64) void k_serve(NaModel M, NaConfig cfg, Ctrl K, d
     if (slot >= n_active) break;
     const int w = active[slot];
     if (SV.status[w] != 0) continue;
-    serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, S);
+    serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, lds_ptr(S));
   }
 }
 
@@ -1116,7 +1121,7 @@ __global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctr
   const int wave = threadIdx.x >> 6;
   const long long t_end = wall_clock64() + budget;
   if (wave == 1) {
-    tile_helper<NA_TILE_BARRIER>(M, cfg, S[0], S[1], T);
+    tile_helper<NA_TILE_BARRIER>(M, cfg, lds_ptr(S[0]), lds_ptr(S[1]), T);
   } else {
     for (;;) {
       int slot = 0;
@@ -1129,14 +1134,14 @@ __global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctr
           const int w = active[slot];
           if (SV.status[w] == 0)
             serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
-                        unfinished, S[0]);
+                        unfinished, lds_ptr(S[0]));
         }
         break;
       }
       const int w = active[slot];
       if (SV.status[w] != 0) continue;
       serve_world<NA_TILE_BARRIER>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
-                        unfinished, S[0], &S[1], &T);
+                        unfinished, lds_ptr(S[0]), &S[1], &T);
     }
   }
   for (;;) {
@@ -1148,10 +1153,10 @@ __global__ __launch_bounds__(128) void k_serve_tile(NaModel M, NaConfig cfg, Ctr
     if (SV.status[w] != 0) continue;
     if (wave == 0)
       serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
-                  S[0]);
+                  lds_ptr(S[0]));
     else
       serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
-                  S[1]);
+                  lds_ptr(S[1]));
   }
 }
 
@@ -1822,7 +1827,7 @@ __global__ __launch_bounds__(64) void k_serve_mixed(NaModel M, NaConfig cfg, Ctr
     const int b0 = M.world_body_start[w];
     const bool art = M.world_body_end[w] > b0 && M.joint_type[b0] != 0;
     if (art) serve_world_art(M, cfg, K, X, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, lds_ptr(Ua));
-    else serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, Uf);
+    else serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished, lds_ptr(Uf));
   }
 }
 
@@ -1848,7 +1853,7 @@ __device__ __forceinline__ void serve_mixed_world(const NaModel& M, const NaConf
                     lds_ptr(*reinterpret_cast<ArtLDS*>(block)));
   else
     serve_world(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters, unfinished,
-                *reinterpret_cast<WorldLDS*>(block));
+                lds_ptr(*reinterpret_cast<WorldLDS*>(block)));
 }
 
 __global__ __launch_bounds__(192) void k_serve_mixed_tile(NaModel M, NaConfig cfg, Ctrl K, double* q64, float* body_q,
@@ -1870,7 +1875,7 @@ __global__ __launch_bounds__(192) void k_serve_mixed_tile(NaModel M, NaConfig cf
   const long long t_end = wall_clock64() + budget;
   const int n_tiles = n_tiled < n_active ? n_tiled : n_active;
   if (wave == 1) {
-    tile_helper<NA_TILE_FLAGS>(M, cfg, W0, W1, T);
+    tile_helper<NA_TILE_FLAGS>(M, cfg, lds_ptr(W0), lds_ptr(W1), T);
   } else if (wave == 0) {
     for (;;) {
       int slot = 0;
@@ -1891,7 +1896,7 @@ __global__ __launch_bounds__(192) void k_serve_mixed_tile(NaModel M, NaConfig cf
                         lds_ptr(*reinterpret_cast<ArtLDS*>(mix0)));
       else
         serve_world<NA_TILE_FLAGS>(M, cfg, K, q64, body_q, body_qd, body_f, dt_outer, eff_max, SV, w, t_end, max_iters,
-                          unfinished, W0, &W1, &T);
+                          unfinished, lds_ptr(W0), &W1, &T);
     }
   }
   for (;;) {
