@@ -80,9 +80,11 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 #define NA_DPP_HALF_MIRROR 0x141
 #define NA_DPP_MIRROR 0x140
 
+// bound_ctrl set: a lane whose source is invalid reads zero, so no zero-filled destination has to
+// be prepared first (with every lane active, the row permutes below have no invalid source).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
