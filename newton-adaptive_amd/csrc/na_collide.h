@@ -820,6 +820,20 @@ __device__ __forceinline__ MjPair mj_pair_params(const NaModel& M, int a, int b)
 // distance phi0 (as an order-preserving integer), then the pair's segment positions (lower,
 // higher) and the manifold slot.  Keys are unique within a world, so "the cap smallest keys" is
 // one set whatever order the contacts come in.
+// Position of the n-th set bit (n from 0) of m; n must be below popcount(m).
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int n) {
+  int p = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const int c = __popcll(m & ((1ull << w) - 1ull));
+    if (n >= c) {
+      n -= c;
+      m >>= w;
+      p += w;
+    }
+  }
+  return p;
+}
 __device__ __forceinline__ unsigned long long contact_key(float phi0, int sa, int sb, int sub) {
   unsigned int b = __float_as_uint(phi0);
   b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
@@ -830,8 +844,9 @@ __device__ __forceinline__ unsigned long long contact_key(float phi0, int sa, in
 // Stage 3: narrow phase over the pair list, compacted into the LDS contact set in pair order.
 // Returns the number of contacts generated.  Up to `cap` (which must not exceed the capacity of
 // the contact set) they are all kept in pair order; beyond it the set keeps the `cap` contacts
-// with the smallest gap distance (contact_key): each further contact replaces the set's largest
-// key when its own is smaller.  The reference appends every contact to one pool of 128 per world
+// with the smallest gap distance (contact_key): once the set is full, each pass of 64 pairs keeps
+// the `cap` smallest keys of the set and the pass's contacts, its kept contacts taking the freed
+// slots.  The reference appends every contact to one pool of 128 per world
 // (rigid_contact_max, solver_sap_adaptive.py:573-577) and drops what overflows the pool; a world
 // here never holds more than its LDS set, and what it drops is the farthest contacts.  With `mj`
 // (MuJoCo contact law) each contact also gets its MuJoCo row parameters, and k / mu hold its
@@ -895,52 +910,61 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
       }
     }
     if (nc + tot > cap) {
-      // contacts past the set's capacity, lane by lane in mask order: each replaces the set's
-      // largest key if its own is smaller (the wave recomputes the largest after each replacement)
+      // the set is full: it keeps the cap smallest keys of (set, this pass's contacts past it).
+      // T = the cap-th smallest of those keys (bisection on the 64-bit key, wave-wide counts);
+      // set slots with a larger key are freed, in slot order, and the pass's contacts with a key
+      // <= T fill them in generation order.  Keys are unique, so exactly cap keys are <= T.
       wave_sync();
-      const int first = nc + excl;  // this lane's first slot in generation order
-      unsigned long long lanes = __ballot(cnt > 0 && first + cnt > cap);
-      while (lanes) {
-        const int src = __ffsll((long long)lanes) - 1;
-        lanes &= lanes - 1;
-        const int m = __shfl(o.mask, src, NA_WAVE);
-        const int f = __shfl(first, src, NA_WAVE);
-        const int psa = __shfl(o.sa, src, NA_WAVE), psb = __shfl(o.sb, src, NA_WAVE);
-        int r = 0;
+      const int l = lane_id();
+      const unsigned long long none = ~0ull;  // above every contact's key
+      unsigned long long nk[NA_MF];
+      int gen = nc + excl;  // this lane's next generation index
+#pragma unroll
+      for (int q = 0; q < NA_MF; ++q) {
+        const bool has = (o.mask >> q) & 1;
+        nk[q] = (has && gen >= cap) ? contact_key(o.phi0[q], o.sa, o.sb, q) : none;
+        gen += has ? 1 : 0;
+      }
+      const unsigned long long sk0 = l < cap ? contact_key(C.phi0[l], C.sa[l], C.sb[l], C.sub[l]) : none;
+      const unsigned long long sk1 =
+          l + NA_WAVE < cap ? contact_key(C.phi0[l + NA_WAVE], C.sa[l + NA_WAVE], C.sb[l + NA_WAVE], C.sub[l + NA_WAVE])
+                            : none;
+      unsigned long long lo = 0ull, hi = none - 1ull;  // count(key <= hi) >= cap
+      while (lo < hi) {
+        const unsigned long long mid = lo + ((hi - lo) >> 1);
+        int below = (sk0 <= mid) + (sk1 <= mid);
+#pragma unroll
+        for (int q = 0; q < NA_MF; ++q) below += nk[q] <= mid;
+        if (wave_sum_i(below) >= cap) hi = mid;
+        else lo = mid + 1ull;
+      }
+      const unsigned long long T = lo;
+      const unsigned long long f0 = __ballot(sk0 != none && sk0 > T);  // freed slots 0..63
+      const unsigned long long f1 = __ballot(sk1 != none && sk1 > T);  // freed slots 64..127
+      const int n0 = __popcll(f0);
+      int mine = 0;
+#pragma unroll
+      for (int q = 0; q < NA_MF; ++q) mine += nk[q] <= T;
+      int rank = wave_incl_scan(mine) - mine;
+      if (mine > 0) {
+        const int ga = S.gidx[o.sa], gb = S.gidx[o.sb];
+        const int ba = M.shape_body[ga], bb = M.shape_body[gb];
+        float mu = fmaxf(M.shape_mu[ga], M.shape_mu[gb]);
+        MjPair mp;
+        if (mj) {
+          mp = mj_pair_params(M, ga, gb);
+          mu = mp.condim == 1 ? 0.f : mp.mu;
+        }
+        const float kk = 0.5f * (M.shape_ke[ga] + M.shape_ke[gb]);
+#pragma unroll
         for (int q = 0; q < NA_MF; ++q) {
-          if (!((m >> q) & 1)) continue;
-          const int gen = f + r++;
-          if (gen < cap) continue;
-          const float cphi = __shfl(o.phi0[q], src, NA_WAVE);
-          const unsigned long long ck = contact_key(cphi, psa, psb, q);
-          // the set's largest key and its slot (keys are unique)
-          unsigned long long best = 0ull;
-          int arg = 0;
-          for (int c = lane_id(); c < cap; c += NA_WAVE) {
-            const unsigned long long k2 = contact_key(C.phi0[c], C.sa[c], C.sb[c], C.sub[c]);
-            if (k2 > best) { best = k2; arg = c; }
-          }
-          for (int off = NA_WAVE / 2; off > 0; off >>= 1) {
-            const unsigned long long ob = __shfl_xor(best, off, NA_WAVE);
-            const int oa = __shfl_xor(arg, off, NA_WAVE);
-            if (ob > best) { best = ob; arg = oa; }
-          }
-          if (ck < best) {
-            if (lane_id() == src) {
-              const int ga = S.gidx[o.sa], gb = S.gidx[o.sb];
-              const int ba = M.shape_body[ga], bb = M.shape_body[gb];
-              MjPair mp;
-              float mu = fmaxf(M.shape_mu[ga], M.shape_mu[gb]);
-              if (mj) {
-                mp = mj_pair_params(M, ga, gb);
-                mu = mp.condim == 1 ? 0.f : mp.mu;
-              }
-              put(arg, q, ba, bb, 0.5f * (M.shape_ke[ga] + M.shape_ke[gb]), mu, mp);
-            }
-            wave_sync();
-          }
+          if (!(nk[q] <= T)) continue;
+          const int slot = rank < n0 ? nth_set_bit(f0, rank) : NA_WAVE + nth_set_bit(f1, rank - n0);
+          put(slot, q, ba, bb, kk, mu, mp);
+          ++rank;
         }
       }
+      wave_sync();
     }
     nc += tot;
   }

@@ -339,8 +339,8 @@ def test_empty_and_ragged_worlds(na):
 
 def test_contact_overflow_is_flagged_and_deterministic(na):
     """Worlds with more than 128 candidate contacts (64 primitives with the builder's 0.1 m
-    contact gap) keep their first 128 in the fixed order, flag the overflow and stay finite and
-    run-to-run identical."""
+    contact gap) keep their 128 closest, flag the overflow and stay finite and run-to-run
+    identical."""
     m = na.scenes.uneven_pile(4, seed=9, device=DEV, min_prims=64, max_prims=64, gap=0.1)
     res = []
     for _ in range(2):
