@@ -911,7 +911,7 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
     }
     if (nc + tot > cap) {
       // the set is full: it keeps the cap smallest keys of (set, this pass's contacts past it).
-      // T = the cap-th smallest of those keys (bisection on the 64-bit key, wave-wide counts);
+      // T = the cap-th smallest of those keys (bisection with wave-wide counts);
       // set slots with a larger key are freed, in slot order, and the pass's contacts with a key
       // <= T fill them in generation order.  Keys are unique, so exactly cap keys are <= T.
       wave_sync();
@@ -929,16 +929,34 @@ __device__ int stage_narrow(const NaModel& M, int b0, int np, int cap, const Seg
       const unsigned long long sk1 =
           l + NA_WAVE < cap ? contact_key(C.phi0[l + NA_WAVE], C.sa[l + NA_WAVE], C.sb[l + NA_WAVE], C.sub[l + NA_WAVE])
                             : none;
-      unsigned long long lo = 0ull, hi = none - 1ull;  // count(key <= hi) >= cap
+      // the distance bits (high word) first; the low word only where distances tie at T
+      unsigned int lo = 0u, hi = 0xffffffffu;  // count(key >> 32 <= hi) >= cap
       while (lo < hi) {
-        const unsigned long long mid = lo + ((hi - lo) >> 1);
-        int below = (sk0 <= mid) + (sk1 <= mid);
+        const unsigned int mid = lo + ((hi - lo) >> 1);
+        int below = ((unsigned int)(sk0 >> 32) <= mid) + ((unsigned int)(sk1 >> 32) <= mid);
 #pragma unroll
-        for (int q = 0; q < NA_MF; ++q) below += nk[q] <= mid;
+        for (int q = 0; q < NA_MF; ++q) below += (unsigned int)(nk[q] >> 32) <= mid;
         if (wave_sum_i(below) >= cap) hi = mid;
-        else lo = mid + 1ull;
+        else lo = mid + 1u;
       }
-      const unsigned long long T = lo;
+      const unsigned long long th = (unsigned long long)lo << 32;
+      int le = (sk0 <= (th | 0xffffffffull)) + (sk1 <= (th | 0xffffffffull));
+#pragma unroll
+      for (int q = 0; q < NA_MF; ++q) le += nk[q] <= (th | 0xffffffffull);
+      unsigned int tl = 0xffffffffu;
+      if (wave_sum_i(le) > cap) {  // several keys share the distance bits at the threshold
+        unsigned int a = 0u, b = 0xffffffffu;
+        while (a < b) {
+          const unsigned int mid = a + ((b - a) >> 1);
+          int below = (sk0 <= (th | mid)) + (sk1 <= (th | mid));
+#pragma unroll
+          for (int q = 0; q < NA_MF; ++q) below += nk[q] <= (th | mid);
+          if (wave_sum_i(below) >= cap) b = mid;
+          else a = mid + 1u;
+        }
+        tl = a;
+      }
+      const unsigned long long T = th | tl;
       const unsigned long long f0 = __ballot(sk0 != none && sk0 > T);  // freed slots 0..63
       const unsigned long long f1 = __ballot(sk1 != none && sk1 > T);  // freed slots 64..127
       const int n0 = __popcll(f0);
