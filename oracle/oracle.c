@@ -880,7 +880,8 @@ static uint64_t contact_key(const or_model* M, int w, const or_contact* c) {
 
 /* Contacts of world w at its body poses, in pair order.  Returns the number generated; up to
  * `cap` are all kept, beyond it the set keeps the `cap` smallest keys (each further contact
- * replaces the largest key of the set when its own is smaller), as the device's narrow phase. */
+ * replaces the largest key of the set when its own is smaller): the set the device's narrow phase
+ * keeps, whose bulk selection per pass of pairs orders the slots differently. */
 int or_collide_world(const or_model* M, int w, const double* body_q, or_contact* out, int cap, int* n_pairs) {
   int n = segment_size(M, w);
   double* lo = (double*)malloc(sizeof(double) * 3 * (n ? n : 1));
