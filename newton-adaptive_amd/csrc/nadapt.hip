@@ -362,9 +362,10 @@ __device__ __forceinline__ WorldCtx world_ctx(const NaModel& M, int w) {
 }
 
 // Contacts of the world at this lane's body pose q (f64).  Returns the contact count kept.
-__device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc, lds_ptr_t<WorldLDS> Sp, const double* q,
-                             int& overflow, long long* wk) {
-  WorldLDS& S = lds_deref<WorldLDS>(Sp);
+// world_collide is the one out-of-line copy the step's forms share; the collision API's kernel
+// inlines the body.
+__device__ __forceinline__ int world_collide_body(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc,
+                                                  WorldLDS& S, const double* q, int& overflow, long long* wk) {
   const int l = lane_id();
   NA_TIC(t0);
   if (l < wc.nb) {
@@ -381,6 +382,10 @@ __device__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldC
   wk[4] += np;
   NA_TOC(wk[6], t0);
   return min(nc, cap);
+}
+__device__ __noinline__ int world_collide(const NaModel& M, const NaConfig& cfg, const WorldCtx& wc,
+                                          lds_ptr_t<WorldLDS> Sp, const double* q, int& overflow, long long* wk) {
+  return world_collide_body(M, cfg, wc, lds_deref<WorldLDS>(Sp), q, overflow, wk);
 }
 
 // One convex step of size h from (q, v0) with the contacts in LDS; integrates into qo/vo.
@@ -1278,7 +1283,7 @@ __global__ __launch_bounds__(64) void k_collide_api(NaModel M, NaConfig cfg, con
   long long wk[NA_WORK] = {};
   NaConfig c2 = cfg;
   c2.max_contacts = min(per_world, NA_MAX_CONTACTS);
-  int nc = world_collide(M, c2, wc, lds_ptr(S), q, ovf, wk);
+  int nc = world_collide_body(M, c2, wc, S, q, ovf, wk);
   if (l == 0) world_counts[w] = nc;
   // write_contact (collide.py:62-147): body-frame points and offsets
   for (int c = l; c < nc; c += NA_WAVE) {
